@@ -1,0 +1,72 @@
+// Device-side data layout shared by the kernels and the batch driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "model.hpp"
+
+namespace gfn1 {
+
+// Padded AO dimension of an instance's dense matrices; padded rows/columns carry
+// an identity overlap and a diagonal Fock value of kPadEnergy, so they decouple
+// exactly and their eigenvalues sort above every physical orbital.
+constexpr double kPadEnergy = 100.0;
+constexpr int kPadMultiple = 32;
+
+// Per-instance SCC control state (one entry per instance, device resident).
+struct SccState {
+    int32_t status;        // 0 running, 1 converged, 2 failed
+    int32_t rung;          // robust-fallback ladder rung (electronic.rs:1413-1419)
+    int32_t iter;          // iterations done on the current rung
+    int32_t total_iter;
+    int32_t have_energy;   // a previous iteration energy exists on this rung
+    int32_t have_prev_p;   // level-shift projector available
+    int32_t broyden_iter;  // BroydenMixer::iter
+    int32_t kneed;         // orbitals with non-negligible occupation
+    int32_t eig_fail;
+    int32_t c_parity;      // which coefficient buffer holds the current orbitals
+    int32_t pad0, pad1;
+    double mixing, level_shift, temperature, kt;
+    double e_last, energy, rms, ts, mu, e_el, e2, e3;
+    double nelec;
+};
+
+struct DeviceTables {
+    const ShellType* shells;     // (kMaxZ+1)*kMaxShellsPerAtom
+    const ElementType* elements; // kMaxZ+1
+    const double* d3_c6;         // D3 reference C6 (layout of data/make_data.py)
+    const double* pair_scale;    // (kMaxZ+1)^2 H0 pair scaling k_AB
+    Globals glob;
+};
+
+// Flat (instance-major) index tables of a batch.
+struct DeviceBatch {
+    int32_t ninst;
+    const int32_t* atom0;    // ninst+1
+    const int32_t* sh0;      // ninst+1
+    const int32_t* ao0;      // ninst+1
+    const int32_t* ld;       // padded dimension per instance
+    const int32_t* atom_inst;
+    const int32_t* atom_z;
+    const double* atom_x;    // 3 per atom
+    const int32_t* atom_sh0; // global index of the atom's first shell
+    const int32_t* sh_atom;  // global atom index
+    const int32_t* sh_type;  // z*kMaxShellsPerAtom + s
+    const int32_t* sh_ao;    // first AO, local to the instance
+    const int32_t* ao_sh;    // global shell index
+    const int64_t* vec0;     // offset of the per-instance length-ld vectors
+    // per-instance dense matrices (column-major, ld x ld)
+    double* const* S;
+    double* const* H0;
+    double* const* F;
+    double* const* P;
+    double* const* W;
+    double* const* CB0;
+    double* const* CB1;
+    double* const* T;
+    double* const* A;
+};
+
+}  // namespace gfn1

@@ -1,0 +1,795 @@
+// Batched GFN1-xTB SCC energy + analytic gradient driver.
+//
+// Flow of Batch::run (reference: run_electronic electronic.rs:800-2031 and
+// analytic_gradient_from_result gradient.rs:122-242):
+//   setup     CN, S/H0, Cholesky-based orthonormal start basis, geometry terms
+//   SCC       per iteration and per size bucket:
+//             v(q_in) -> F -> A = Cb^T F Cb -> A = V e V^T -> C = Cb V
+//             -> Fermi -> P -> Mulliken q_out -> E, convergence, mixing
+//             The orbitals of one iteration are the start basis of the next,
+//             so A is nearly diagonal after the first iterations.
+//   gradient  W, pair/atom gradient kernels, CN chains.
+#include "engine.hpp"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace gfn1 {
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return Status::error(kErrDevice, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define RB_TRY(expr)                                                                                   \
+    do {                                                                                               \
+        rocblas_status s_ = (expr);                                                                    \
+        if (s_ != rocblas_status_success)                                                              \
+            return Status::error(kErrDevice, std::string(#expr) + ": " + rocblas_status_to_string(s_)); \
+    } while (0)
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+namespace {
+enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, kRoles };
+enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, kActRoles };
+constexpr int kWarmResetPeriod = 64;
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+}  // namespace
+
+// ------------------------------------------------------------------ DeviceModel
+Status DeviceModel::init(const Model& m) {
+    model_ = m;
+    const auto& sh = m.shell_table();
+    const auto& el = m.element_table();
+    HIP_TRY(hipMalloc(&shells_, sh.size() * sizeof(ShellType)));
+    HIP_TRY(hipMemcpy(shells_, sh.data(), sh.size() * sizeof(ShellType), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&elements_, el.size() * sizeof(ElementType)));
+    HIP_TRY(hipMemcpy(elements_, el.data(), el.size() * sizeof(ElementType), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c6_, m.d3_c6().size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(c6_, m.d3_c6().data(), m.d3_c6().size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> pair((kMaxZ + 1) * (kMaxZ + 1), 1.0);
+    for (int a = 1; a <= kMaxZ; ++a)
+        for (int b = 1; b <= kMaxZ; ++b) pair[a * (kMaxZ + 1) + b] = m.pair_scaling(a, b);
+    HIP_TRY(hipMalloc(&pair_, pair.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(pair_, pair.data(), pair.size() * sizeof(double), hipMemcpyHostToDevice));
+    return Status{};
+}
+
+DeviceModel::~DeviceModel() {
+    (void)hipFree(shells_);
+    (void)hipFree(elements_);
+    (void)hipFree(c6_);
+    (void)hipFree(pair_);
+}
+
+DeviceTables DeviceModel::tables() const {
+    DeviceTables t;
+    t.shells = shells_;
+    t.elements = elements_;
+    t.d3_c6 = c6_;
+    t.pair_scale = pair_;
+    t.glob = model_.globals();
+    return t;
+}
+
+// ------------------------------------------------------------------ Batch
+template <typename T>
+Status Batch::dalloc(T** p, size_t count) {
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    HIP_TRY(hipMalloc((void**)p, bytes));
+    HIP_TRY(hipMemset(*p, 0, bytes));
+    allocs_.push_back(*p);
+    return Status{};
+}
+
+Batch::~Batch() {
+    for (auto& bk : buckets_) {
+        if (bk.handle) rocblas_destroy_handle(bk.handle);
+        if (bk.stream) (void)hipStreamDestroy(bk.stream);
+        (void)hipFree(bk.arena);
+        (void)hipFree(bk.ptrs);
+        (void)hipFree(bk.act);
+        (void)hipFree(bk.eig);
+        (void)hipFree(bk.eig_e);
+        (void)hipFree(bk.resid);
+        (void)hipFree(bk.sweeps);
+        (void)hipFree(bk.info);
+        (void)hipFree(bk.beta);
+    }
+    for (void* p : allocs_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& systems, const EngineOptions& opt) {
+    dm_ = &dm;
+    opt_ = opt;
+    const Model& m = dm.model();
+    ninst_ = (int)systems.size();
+    h_atom0_.assign(ninst_ + 1, 0);
+    h_sh0_.assign(ninst_ + 1, 0);
+    h_ao0_.assign(ninst_ + 1, 0);
+    h_ld_.assign(ninst_, 0);
+    h_vec0_.assign(ninst_ + 1, 0);
+    h_err_.assign(ninst_, 0);
+    h_errmsg_.assign(ninst_, "");
+    h_nelec_.assign(ninst_, 0.0);
+    for (int i = 0; i < ninst_; ++i) {
+        const SystemInput& s = systems[i];
+        int nsh = 0, nao = 0;
+        double nref = 0.0;
+        for (int a = 0; a < s.nat; ++a) {
+            int z = s.z[a];
+            if (z < 1 || z > kMaxZ || !m.element(z).present)
+                return Status::error(kErrMissingElement, "missing element Z=" + std::to_string(z));
+            const ElementType& e = m.element(z);
+            h_atom_inst_.push_back(i);
+            h_atom_z_.push_back(z);
+            h_atom_sh0_.push_back(h_sh0_[i] + nsh);
+            for (int k = 0; k < 3; ++k) {
+                double v = s.x[3 * a + k];
+                if (!std::isfinite(v)) {
+                    h_err_[i] = kErrInvalidInput;
+                    h_errmsg_[i] = "non-finite coordinate";
+                    v = 0.0;
+                }
+                h_x_.push_back(v);
+            }
+            for (int sh = 0; sh < e.nshell; ++sh) {
+                const ShellType& t = m.shell(z, sh);
+                h_sh_atom_.push_back(h_atom0_[i] + a);
+                h_sh_type_.push_back(z * kMaxShellsPerAtom + sh);
+                h_sh_ao_.push_back(nao);
+                for (int k = 0; k < t.nao; ++k) h_ao_sh_.push_back(h_sh0_[i] + nsh);
+                nao += t.nao;
+                nref += t.ref_occ;
+                ++nsh;
+            }
+        }
+        h_atom0_[i + 1] = h_atom0_[i] + s.nat;
+        h_sh0_[i + 1] = h_sh0_[i] + nsh;
+        h_ao0_[i + 1] = h_ao0_[i] + nao;
+        h_ld_[i] = std::max(kPadMultiple, round_up(nao, kPadMultiple));
+        h_vec0_[i + 1] = h_vec0_[i] + h_ld_[i];
+        const double nelec = nref - s.charge;
+        h_nelec_[i] = nelec;
+        if (nao == 0) {
+            h_err_[i] = kErrInvalidInput;
+            h_errmsg_[i] = "cannot run GFN1 SCC for an empty basis";
+        } else if (!std::isfinite(nelec) || nelec < -1e-8 || nelec > 2.0 * nao + 1e-8) {
+            h_err_[i] = kErrInvalidInput;
+            h_errmsg_[i] = "electron count out of range";
+        }
+        // initial shell charges (electronic.rs:1235-1239): used when the length matches
+        for (int k = 0; k < nsh; ++k)
+            h_q0_.push_back((s.q0 && s.q0_len == nsh) ? s.q0[k] : 0.0);
+    }
+    h_err_input_ = h_err_;
+    h_errmsg_input_ = h_errmsg_;
+    natoms_ = h_atom0_[ninst_];
+    nshells_ = h_sh0_[ninst_];
+    naos_ = h_ao0_[ninst_];
+
+    // SCC ladder (electronic.rs:1413-1419)
+    const double base_mix = std::min(1.0, std::max(0.01, opt.mixing));
+    const int robust = std::max(opt.max_scc, 800);
+    prm_.energy_tolerance = opt.energy_tolerance;
+    prm_.charge_tolerance = opt.charge_tolerance;
+    const int rb[4] = {opt.scc_broyden ? 1 : 0, 0, 0, 1};
+    const double rm[4] = {base_mix, 0.20, 0.10, base_mix};
+    const double rl[4] = {opt.level_shift, std::max(opt.level_shift, 0.20), std::max(opt.level_shift, 0.10), 0.0};
+    const double rt[4] = {opt.electronic_temperature, opt.electronic_temperature, opt.electronic_temperature,
+                          std::max(opt.electronic_temperature, 3000.0)};
+    const int ri[4] = {opt.max_scc, robust, robust, std::max(opt.max_scc, 250)};
+    for (int k = 0; k < 4; ++k) {
+        prm_.rung_broyden[k] = rb[k];
+        prm_.rung_mixing[k] = rm[k];
+        prm_.rung_level_shift[k] = rl[k];
+        prm_.rung_temperature[k] = rt[k];
+        prm_.rung_max_iter[k] = ri[k];
+    }
+    broyden_memory_ = std::max(opt.scc_broyden_size, 2);
+    int max_broyden_iter = std::max(opt.scc_broyden ? opt.max_scc : 0, std::max(opt.max_scc, 250));
+    prm_.broyden_memory = broyden_memory_;
+    prm_.broyden_slots = std::min(broyden_memory_, max_broyden_iter);
+    h_broff_.assign(ninst_ + 1, 0);
+    for (int i = 0; i < ninst_; ++i)
+        h_broff_[i + 1] = h_broff_[i] + (int64_t)prm_.broyden_slots * (h_sh0_[i + 1] - h_sh0_[i]);
+
+    HIP_TRY(hipStreamCreate(&stream_));
+    // ---- device index tables
+    auto up_i = [&](int** d, const std::vector<int>& h) -> Status {
+        Status st = dalloc(d, h.size());
+        if (!st.ok()) return st;
+        HIP_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+        return Status{};
+    };
+    Status st;
+    if (!(st = up_i(&d_atom0_, h_atom0_)).ok()) return st;
+    if (!(st = up_i(&d_sh0_, h_sh0_)).ok()) return st;
+    if (!(st = up_i(&d_ao0_, h_ao0_)).ok()) return st;
+    if (!(st = up_i(&d_ld_, h_ld_)).ok()) return st;
+    if (!(st = up_i(&d_atom_inst_, h_atom_inst_)).ok()) return st;
+    if (!(st = up_i(&d_atom_z_, h_atom_z_)).ok()) return st;
+    if (!(st = up_i(&d_atom_sh0_, h_atom_sh0_)).ok()) return st;
+    if (!(st = up_i(&d_sh_atom_, h_sh_atom_)).ok()) return st;
+    if (!(st = up_i(&d_sh_type_, h_sh_type_)).ok()) return st;
+    if (!(st = up_i(&d_sh_ao_, h_sh_ao_)).ok()) return st;
+    if (!(st = up_i(&d_ao_sh_, h_ao_sh_)).ok()) return st;
+    if (!(st = dalloc(&d_fail_, ninst_)).ok()) return st;
+    if (!(st = dalloc(&d_vec0_, ninst_ + 1)).ok()) return st;
+    HIP_TRY(hipMemcpy(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (!(st = dalloc(&d_broff_, ninst_ + 1)).ok()) return st;
+    HIP_TRY(hipMemcpy(d_broff_, h_broff_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (!(st = dalloc(&d_x_, h_x_.size())).ok()) return st;
+    HIP_TRY(hipMemcpy(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!(st = dalloc(&d_q0_, nshells_)).ok()) return st;
+    HIP_TRY(hipMemcpy(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice));
+    double** dd[] = {&d_cn_, &d_cnd3_, &d_qin_, &d_qout_, &d_vin_g_, &d_vin_t_, &d_vout_g_, &d_vout_t_,
+                     &d_rowS_, &d_rowH_, &d_erep_atom_, &d_edisp_atom_, &d_erep_, &d_edisp_, &d_ehal_,
+                     &d_grep_, &d_gdisp_, &d_ghal_, &d_gel_, &d_dedcn_disp_, &d_dedcn_h_, &d_brql_, &d_brdql_};
+    size_t dn[] = {(size_t)natoms_, (size_t)natoms_, (size_t)nshells_, (size_t)nshells_, (size_t)nshells_,
+                   (size_t)nshells_, (size_t)nshells_, (size_t)nshells_, (size_t)naos_, (size_t)naos_,
+                   (size_t)natoms_, (size_t)natoms_, (size_t)ninst_, (size_t)ninst_, (size_t)ninst_,
+                   3 * (size_t)natoms_, 3 * (size_t)natoms_, 3 * (size_t)natoms_, 3 * (size_t)natoms_,
+                   (size_t)natoms_, (size_t)natoms_, (size_t)nshells_, (size_t)nshells_};
+    for (size_t k = 0; k < sizeof(dn) / sizeof(dn[0]); ++k)
+        if (!(st = dalloc(dd[k], dn[k])).ok()) return st;
+    if (!(st = dalloc(&d_gw_, 2 * (size_t)natoms_ * kMaxRef)).ok()) return st;
+    if (!(st = dalloc(&d_eps_, h_vec0_[ninst_])).ok()) return st;
+    if (!(st = dalloc(&d_occ_, h_vec0_[ninst_])).ok()) return st;
+    if (!(st = dalloc(&d_brdf_, h_broff_[ninst_])).ok()) return st;
+    if (!(st = dalloc(&d_bru_, h_broff_[ninst_])).ok()) return st;
+    if (!(st = dalloc(&d_brom_, (size_t)ninst_ * prm_.broyden_slots)).ok()) return st;
+    if (!(st = dalloc(&d_state_, ninst_)).ok()) return st;
+    if (!(st = dalloc(&d_mat_, (size_t)kRoles * ninst_)).ok()) return st;
+
+    // ---- size buckets and matrix arenas
+    std::vector<int> lds = h_ld_;
+    std::sort(lds.begin(), lds.end());
+    lds.erase(std::unique(lds.begin(), lds.end()), lds.end());
+    h_bucket_of_.assign(ninst_, 0);
+    buckets_.resize(lds.size());
+    std::vector<double*> hmat((size_t)kRoles * ninst_, nullptr);
+    for (size_t bi = 0; bi < lds.size(); ++bi) buckets_[bi].ld = lds[bi];
+    for (int i = 0; i < ninst_; ++i) {
+        int bi = (int)(std::lower_bound(lds.begin(), lds.end(), h_ld_[i]) - lds.begin());
+        h_bucket_of_[i] = bi;
+        buckets_[bi].inst.push_back(i);
+    }
+    for (auto& bk : buckets_) {
+        const size_t mat = (size_t)bk.ld * bk.ld;
+        const size_t count = bk.inst.size();
+        HIP_TRY(hipMalloc(&bk.arena, mat * count * kRoles * sizeof(double)));
+        HIP_TRY(hipMemset(bk.arena, 0, mat * count * kRoles * sizeof(double)));
+        for (size_t k = 0; k < count; ++k)
+            for (int r = 0; r < kRoles; ++r)
+                hmat[(size_t)r * ninst_ + bk.inst[k]] = bk.arena + ((size_t)r * count + k) * mat;
+        HIP_TRY(hipStreamCreate(&bk.stream));
+        RB_TRY(rocblas_create_handle(&bk.handle));
+        RB_TRY(rocblas_set_stream(bk.handle, bk.stream));
+        HIP_TRY(hipMalloc(&bk.ptrs, sizeof(double*) * kActRoles * count));
+        HIP_TRY(hipMalloc(&bk.act, sizeof(int) * count));
+        HIP_TRY(hipMalloc(&bk.eig, sizeof(double) * bk.ld * count));
+        HIP_TRY(hipMalloc(&bk.eig_e, sizeof(double) * bk.ld * count));
+        HIP_TRY(hipMalloc(&bk.resid, sizeof(double) * count));
+        HIP_TRY(hipMalloc(&bk.sweeps, sizeof(int) * count));
+        HIP_TRY(hipMalloc(&bk.info, sizeof(int) * count));
+    }
+    HIP_TRY(hipMemcpy(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
+
+    db_.ninst = ninst_;
+    db_.atom0 = d_atom0_;
+    db_.sh0 = d_sh0_;
+    db_.ao0 = d_ao0_;
+    db_.ld = d_ld_;
+    db_.atom_inst = d_atom_inst_;
+    db_.atom_z = d_atom_z_;
+    db_.atom_x = d_x_;
+    db_.atom_sh0 = d_atom_sh0_;
+    db_.sh_atom = d_sh_atom_;
+    db_.sh_type = d_sh_type_;
+    db_.sh_ao = d_sh_ao_;
+    db_.ao_sh = d_ao_sh_;
+    db_.vec0 = d_vec0_;
+    db_.S = d_mat_ + (size_t)R_S * ninst_;
+    db_.H0 = d_mat_ + (size_t)R_H0 * ninst_;
+    db_.F = d_mat_ + (size_t)R_F * ninst_;
+    db_.P = d_mat_ + (size_t)R_P * ninst_;
+    db_.W = d_mat_ + (size_t)R_W * ninst_;
+    db_.CB0 = d_mat_ + (size_t)R_CB0 * ninst_;
+    db_.CB1 = d_mat_ + (size_t)R_CB1 * ninst_;
+    db_.T = d_mat_ + (size_t)R_T * ninst_;
+    db_.A = d_mat_ + (size_t)R_A * ninst_;
+    h_final_parity_.assign(ninst_, 0);
+    return Status{};
+}
+
+Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity) {
+    // host copy of the pointer tables of the listed instances
+    std::vector<double*> hp((size_t)kActRoles * list.size());
+    std::vector<double*> all((size_t)kRoles * ninst_);
+    // reconstruct the arena pointers of this bucket
+    const size_t mat = (size_t)bk.ld * bk.ld;
+    const size_t count = bk.inst.size();
+    for (size_t j = 0; j < list.size(); ++j) {
+        const int i = list[j];
+        const size_t k = std::find(bk.inst.begin(), bk.inst.end(), i) - bk.inst.begin();
+        auto at = [&](int role) { return bk.arena + ((size_t)role * count + k) * mat; };
+        const int par = parity < 0 ? h_final_parity_[i] : parity;
+        hp[(size_t)A_F * list.size() + j] = at(R_F);
+        hp[(size_t)A_CB * list.size() + j] = at(par ? R_CB1 : R_CB0);
+        hp[(size_t)A_C * list.size() + j] = at(par ? R_CB0 : R_CB1);
+        hp[(size_t)A_T * list.size() + j] = at(R_T);
+        hp[(size_t)A_A * list.size() + j] = at(R_A);
+        hp[(size_t)A_P * list.size() + j] = at(R_P);
+        hp[(size_t)A_S * list.size() + j] = at(R_S);
+        hp[(size_t)A_X * list.size() + j] = at(R_W);
+    }
+    HIP_TRY(hipMemcpyAsync(bk.ptrs, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
+    HIP_TRY(hipMemcpyAsync(bk.act, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, bk.stream));
+    HIP_TRY(hipStreamSynchronize(bk.stream));  // host staging buffers are stack-local
+    return Status{};
+}
+
+Status Batch::eigensolve(Bucket& bk, int count) {
+    double* const* A = bk.ptrs + (size_t)A_A * count;
+    const int n = bk.ld;
+    int solver = opt_.eigensolver;
+    if (solver == 0) solver = 1;
+    if (solver == 2) {
+        RB_TRY(rocsolver_dsyevj_batched(bk.handle, rocblas_esort_ascending, rocblas_evect_original,
+                                        rocblas_fill_lower, n, (double* const*)A, n, 1e-12, bk.resid, 100,
+                                        bk.sweeps, bk.eig, n, bk.info, count));
+    } else {
+        RB_TRY(rocsolver_dsyevd_batched(bk.handle, rocblas_evect_original, rocblas_fill_lower, n,
+                                        (double* const*)A, n, bk.eig, n, bk.eig_e, n, bk.info, count));
+    }
+    return Status{};
+}
+
+Status Batch::run() {
+    // re-runnable: every run starts from the validated inputs
+    h_err_ = h_err_input_;
+    h_errmsg_ = h_errmsg_input_;
+    HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)std::max(natoms_, 1) * sizeof(double)));
+    HIP_TRY(hipMemset(d_fail_, 0, (size_t)std::max(ninst_, 1) * sizeof(int)));
+    ran_ = true;
+    auto t0 = std::chrono::steady_clock::now();
+    Status st = setup_phase();
+    if (!st.ok()) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    auto t1 = std::chrono::steady_clock::now();
+    st = scc_phase();
+    if (!st.ok()) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    auto t2 = std::chrono::steady_clock::now();
+    st = gradient_phase();
+    if (!st.ok()) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    auto t3 = std::chrono::steady_clock::now();
+    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    timings_[0] = ms(t0, t1);
+    timings_[1] = ms(t1, t2);
+    timings_[2] = ms(t2, t3);
+    timings_[3] = ms(t0, t3);
+    return Status{};
+}
+
+Status Batch::setup_phase() {
+    const DeviceTables t = dm_->tables();
+    const int na = natoms_;
+    if (na == 0) return Status{};
+    hipLaunchKernelGGL(k_coordination, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                       opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_, d_cnd3_, d_gw_,
+                       d_gw_ + (size_t)na * kMaxRef, na);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_overlap_h0, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+                       opt_.integral_cutoff);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pad_diagonal, dim3(ninst_), dim3(64), 0, stream_, db_);
+    LAUNCH_CHECK();
+    // geometry-only terms
+    hipLaunchKernelGGL(k_geometry_terms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                       (const double*)d_gw_, (const double*)d_cnd3_, opt_.enable_dispersion, d_erep_atom_,
+                       d_edisp_atom_, d_grep_, d_gdisp_, d_dedcn_disp_, na);
+    LAUNCH_CHECK();
+    if (opt_.enable_dispersion) {
+        hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                           (const double*)d_dedcn_disp_, 25.0, 1e-14, d_gdisp_, na);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_halogen, dim3((ninst_ + 63) / 64), dim3(64), 0, stream_, db_, t, d_ehal_, d_ghal_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_erep_atom_,
+                       d_erep_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_edisp_atom_,
+                       d_edisp_);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+
+    // SCC state
+    h_state_.assign(ninst_, SccState{});
+    for (int i = 0; i < ninst_; ++i) {
+        SccState& s = h_state_[i];
+        std::memset(&s, 0, sizeof(s));
+        s.status = h_err_[i] ? 2 : 0;
+        s.mixing = prm_.rung_mixing[0];
+        s.level_shift = prm_.rung_level_shift[0];
+        s.temperature = prm_.rung_temperature[0];
+        s.kt = std::max(s.temperature, 0.0) * kBoltzmannHartree;
+        s.nelec = h_nelec_[i];
+    }
+    HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+
+    // orthonormal start basis Cb = L^-T from S = L L^T (per bucket)
+    for (auto& bk : buckets_) {
+        const int count = (int)bk.inst.size();
+        Status st = upload_active(bk, bk.inst, 0);
+        if (!st.ok()) return st;
+        const size_t mat = (size_t)bk.ld * bk.ld;
+        // A <- S
+        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_A * count * mat, bk.arena + (size_t)R_S * count * mat,
+                               count * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
+        double* const* A = bk.ptrs + (size_t)A_A * count;
+        RB_TRY(rocsolver_dpotrf_batched(bk.handle, rocblas_fill_lower, bk.ld, (double* const*)A, bk.ld, bk.info,
+                                        count));
+        RB_TRY(rocsolver_dtrtri_batched(bk.handle, rocblas_fill_lower, rocblas_diagonal_non_unit, bk.ld,
+                                        (double* const*)A, bk.ld, bk.info, count));
+        std::vector<int> info(count);
+        HIP_TRY(hipMemcpyAsync(info.data(), bk.info, count * sizeof(int), hipMemcpyDeviceToHost, bk.stream));
+        HIP_TRY(hipStreamSynchronize(bk.stream));
+        for (int k = 0; k < count; ++k)
+            if (info[k] != 0 && !h_err_[bk.inst[k]]) {
+                h_err_[bk.inst[k]] = kErrInvalidInput;
+                h_errmsg_[bk.inst[k]] = "overlap matrix is not positive definite";
+            }
+        hipLaunchKernelGGL(k_make_cb, dim3(64, count), dim3(256), 0, bk.stream, db_, A, (const int*)bk.act,
+                           opt_.eigen_tolerance, d_fail_);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<int> fail(ninst_);
+    HIP_TRY(hipMemcpy(fail.data(), d_fail_, ninst_ * sizeof(int), hipMemcpyDeviceToHost));
+    bool changed = false;
+    for (int i = 0; i < ninst_; ++i)
+        if ((fail[i] || h_err_[i]) && h_state_[i].status == 0) {
+            if (!h_err_[i]) {
+                h_err_[i] = kErrInvalidInput;
+                h_errmsg_[i] = "overlap matrix is not positive definite";
+            }
+            h_state_[i].status = 2;
+            changed = true;
+        }
+    if (changed)
+        HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+    return Status{};
+}
+
+Status Batch::scc_phase() {
+    const DeviceTables t = dm_->tables();
+    for (auto& bk : buckets_) {
+        bk.active.clear();
+        for (int i : bk.inst)
+            if (h_state_[i].status == 0) bk.active.push_back(i);
+    }
+    const double one = 1.0, zero = 0.0;
+    int it = 0;
+    while (true) {
+        bool any = false;
+        for (auto& bk : buckets_) any = any || !bk.active.empty();
+        if (!any) break;
+        const int parity = it & 1;
+        // warm start: the previous orbitals; periodically restart from L^-T is not
+        // needed because C stays S-orthonormal to rounding (V orthogonal).
+        (void)kWarmResetPeriod;
+        // ---- phase 1: Fock, congruence, eigensolve, back-transform, Fermi
+        for (auto& bk : buckets_) {
+            const int cnt = (int)bk.active.size();
+            if (!cnt) continue;
+            Status st = upload_active(bk, bk.active, parity);
+            if (!st.ok()) return st;
+            int max_nsh = 0;
+            for (int i : bk.active) max_nsh = std::max(max_nsh, h_sh0_[i + 1] - h_sh0_[i]);
+            const int n = bk.ld;
+            hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
+                               t, (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               (const double*)d_vin_t_);
+            LAUNCH_CHECK();
+            // level shift F += b (S - 1/2 S P_prev S) for instances that carry a previous density
+            std::vector<int> ls;
+            for (int i : bk.active)
+                if (h_state_[i].have_prev_p && h_state_[i].level_shift != 0.0) ls.push_back(i);
+            if (!ls.empty()) {
+                // reuse the T and A buffers of these instances
+                std::vector<int> keep = bk.active;
+                st = upload_active(bk, ls, parity);
+                if (!st.ok()) return st;
+                const int c = (int)ls.size();
+                double* const* P = bk.ptrs + (size_t)A_P * c;
+                double* const* S = bk.ptrs + (size_t)A_S * c;
+                double* const* T = bk.ptrs + (size_t)A_T * c;
+                double* const* Ab = bk.ptrs + (size_t)A_A * c;
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                                             &one, (const double* const*)S, n, (const double* const*)P, n, &zero,
+                                             (double* const*)T, n, c));
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                                             &one, (const double* const*)T, n, (const double* const*)S, n, &zero,
+                                             (double* const*)Ab, n, c));
+                std::vector<double> shift(c);
+                for (int k = 0; k < c; ++k) shift[k] = h_state_[ls[k]].level_shift;
+                double* dshift = bk.eig_e;  // scratch (>= count doubles)
+                HIP_TRY(hipMemcpyAsync(dshift, shift.data(), c * sizeof(double), hipMemcpyHostToDevice, bk.stream));
+                hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                                   (const double*)dshift, (double* const*)Ab);
+                LAUNCH_CHECK();
+                HIP_TRY(hipStreamSynchronize(bk.stream));
+                st = upload_active(bk, keep, parity);
+                if (!st.ok()) return st;
+            }
+            double* const* F = bk.ptrs + (size_t)A_F * cnt;
+            double* const* Cb = bk.ptrs + (size_t)A_CB * cnt;
+            double* const* C = bk.ptrs + (size_t)A_C * cnt;
+            double* const* T = bk.ptrs + (size_t)A_T * cnt;
+            double* const* A = bk.ptrs + (size_t)A_A * cnt;
+            // T = F Cb ; A = Cb^T T
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                         (const double* const*)F, n, (const double* const*)Cb, n, &zero,
+                                         (double* const*)T, n, cnt));
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
+                                         &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
+                                         (double* const*)A, n, cnt));
+            st = eigensolve(bk, cnt);
+            if (!st.ok()) return st;
+            // C = Cb V
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                         (const double* const*)Cb, n, (const double* const*)A, n, &zero,
+                                         (double* const*)C, n, cnt));
+            hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
+            LAUNCH_CHECK();
+        }
+        HIP_TRY(hipDeviceSynchronize());
+        // eigensolver status and occupied-column count
+        HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+        for (auto& bk : buckets_) {
+            const int cnt = (int)bk.active.size();
+            if (!cnt) continue;
+            std::vector<int> info(cnt);
+            HIP_TRY(hipMemcpy(info.data(), bk.info, cnt * sizeof(int), hipMemcpyDeviceToHost));
+            for (int k = 0; k < cnt; ++k)
+                if (info[k] != 0)
+                    return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
+            int kp = 1;
+            for (int i : bk.active) kp = std::max(kp, h_state_[i].kneed);
+            bk.kP = std::min(kp, bk.ld);
+        }
+        // ---- phase 2: density, Mulliken charges, energy, mixing
+        for (auto& bk : buckets_) {
+            const int cnt = (int)bk.active.size();
+            if (!cnt) continue;
+            const int n = bk.ld;
+            int max_nsh = 0;
+            for (int i : bk.active) max_nsh = std::max(max_nsh, h_sh0_[i + 1] - h_sh0_[i]);
+            double* const* C = bk.ptrs + (size_t)A_C * cnt;
+            double* const* T = bk.ptrs + (size_t)A_T * cnt;
+            double* const* P = bk.ptrs + (size_t)A_P * cnt;
+            hipLaunchKernelGGL(k_scale_columns, dim3(bk.kP, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               C, T, (const double*)d_eps_, (const double*)d_occ_, bk.kP, 0);
+            LAUNCH_CHECK();
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                                         bk.kP, &one, (const double* const*)T, n, (const double* const*)C, n, &zero,
+                                         (double* const*)P, n, cnt));
+            const int nt = (n + 31) / 32;
+            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               P);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_,
+                               d_rowH_);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
+                               (const double*)d_rowS_, (const double*)d_rowH_, d_qout_, d_state_);
+            LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
+                               t, (const int*)bk.act, (const double*)d_qout_, d_vout_g_, d_vout_t_);
+            LAUNCH_CHECK();
+            // Broyden history longer than 32 needs global scratch
+            int mmax = 0;
+            for (int i : bk.active) mmax = std::max(mmax, std::min(h_state_[i].broyden_iter, broyden_memory_));
+            int beta_ld = std::max(mmax, 1);
+            if (mmax > 32) {
+                size_t need = (size_t)cnt * (beta_ld + 1) * beta_ld;
+                if (need > bk.beta_cap) {
+                    (void)hipFree(bk.beta);
+                    HIP_TRY(hipMalloc(&bk.beta, need * 2 * sizeof(double)));
+                    bk.beta_cap = need * 2;
+                }
+            }
+            hipLaunchKernelGGL(k_scc_update, dim3(cnt), dim3(256), 0, bk.stream, db_, t, (const int*)bk.act, prm_,
+                               (const double*)d_qout_, (const double*)d_vout_g_, d_qin_, (const double*)d_q0_,
+                               d_state_, d_brdf_, d_bru_, d_brom_, d_brql_, d_brdql_, (const int64_t*)d_broff_,
+                               bk.beta, beta_ld);
+            LAUNCH_CHECK();
+        }
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+        for (auto& bk : buckets_) {
+            std::vector<int> next;
+            for (int i : bk.active) {
+                if (h_state_[i].status == 0) next.push_back(i);
+                else h_final_parity_[i] = parity;  // its orbitals sit in the buffer after Cb[parity]
+            }
+            bk.active.swap(next);
+        }
+        ++it;
+    }
+    for (int i = 0; i < ninst_; ++i)
+        if (h_state_[i].status == 2 && !h_err_[i]) {
+            h_err_[i] = kErrSccNotConverged;
+            h_errmsg_[i] = "SCC did not converge";
+        }
+    return Status{};
+}
+
+Status Batch::gradient_phase() {
+    const DeviceTables t = dm_->tables();
+    const double one = 1.0, zero = 0.0;
+    // energy-weighted density W = C diag(eps f) C^T of every converged instance
+    for (auto& bk : buckets_) {
+        std::vector<int> done;
+        for (int i : bk.inst)
+            if (h_state_[i].status == 1) done.push_back(i);
+        const int cnt = (int)done.size();
+        if (!cnt) continue;
+        Status st = upload_active(bk, done, -1);
+        if (!st.ok()) return st;
+        const int n = bk.ld;
+        int kp = 1;
+        for (int i : done) kp = std::max(kp, h_state_[i].kneed);
+        kp = std::min(kp, n);
+        double* const* Cb = bk.ptrs + (size_t)A_C * cnt;  // final orbitals (parity -1 -> recorded)
+        double* const* T = bk.ptrs + (size_t)A_T * cnt;
+        double* const* W = bk.ptrs + (size_t)A_X * cnt;
+        hipLaunchKernelGGL(k_scale_columns, dim3(kp, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, Cb, T,
+                           (const double*)d_eps_, (const double*)d_occ_, kp, 1);
+        LAUNCH_CHECK();
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kp, &one,
+                                     (const double* const*)T, n, (const double* const*)Cb, n, &zero,
+                                     (double* const*)W, n, cnt));
+        const int nt = (n + 31) / 32;
+        hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, W);
+        LAUNCH_CHECK();
+        HIP_TRY(hipStreamSynchronize(bk.stream));
+    }
+    if (!opt_.compute_gradient || natoms_ == 0) return Status{};
+    const int na = natoms_;
+    HIP_TRY(hipMemsetAsync(d_gel_, 0, 3 * na * sizeof(double), stream_));
+    const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+    const int cn_on = opt_.enable_cn_hamiltonian;
+    if (el_on) {
+        hipLaunchKernelGGL(k_grad_pairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+                           (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
+                           d_dedcn_h_);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                           (const double*)d_qout_, cn_on, opt_.include_scc, d_dedcn_h_, d_gel_, na);
+        LAUNCH_CHECK();
+        if (cn_on) {
+            hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                               (const double*)d_dedcn_h_, opt_.coordination_cutoff, 1e-12, d_gel_, na);
+            LAUNCH_CHECK();
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return Status{};
+}
+
+Status Batch::download(std::vector<InstanceResult>& out) {
+    out.assign(ninst_, InstanceResult{});
+    const int na = natoms_;
+    std::vector<double> cn(na), gel(3 * na), grep(3 * na), gdisp(3 * na), ghal(3 * na), qout(nshells_),
+        vout(nshells_), erep(ninst_), edisp(ninst_), ehal(ninst_), eps(h_vec0_[ninst_]), occ(h_vec0_[ninst_]);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cn.data(), d_cn_, na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(qout.data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(vout.data(), d_vout_t_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(eps.data(), d_eps_, eps.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(occ.data(), d_occ_, occ.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double*> hmat((size_t)kRoles * ninst_);
+    HIP_TRY(hipMemcpy(hmat.data(), d_mat_, hmat.size() * sizeof(double*), hipMemcpyDeviceToHost));
+    for (int i = 0; i < ninst_; ++i) {
+        InstanceResult& r = out[i];
+        const SccState& s = h_state_[i];
+        const int a0 = h_atom0_[i], nat = h_atom0_[i + 1] - a0;
+        const int s0 = h_sh0_[i], nsh = h_sh0_[i + 1] - s0;
+        const int n = h_ao0_[i + 1] - h_ao0_[i], ld = h_ld_[i];
+        r.nat = nat;
+        r.nsh = nsh;
+        r.nao = n;
+        r.iterations = s.iter;
+        r.converged = s.status == 1;
+        r.rms = s.rms;
+        r.status = h_err_[i] ? h_err_[i] : (s.status == 1 ? kOk : kErrSccNotConverged);
+        r.message = h_errmsg_[i];
+        if (r.status != kOk) continue;
+        const double e_disp = opt_.enable_dispersion ? edisp[i] : 0.0;
+        const double total_internal = s.e_el + s.e2 + s.e3 + erep[i] + e_disp + ehal[i];
+        for (double& e : r.energies) e = 0.0;
+        r.energies[0] = total_internal + s.ts;
+        r.energies[1] = total_internal;
+        r.energies[2] = erep[i];
+        r.energies[3] = s.e_el;
+        r.energies[4] = s.e2;
+        r.energies[5] = s.e3;
+        r.energies[7] = e_disp;
+        r.energies[8] = ehal[i];
+        r.energies[14] = s.ts;
+        r.fermi_level = s.mu;
+        r.temperature = s.temperature;
+        r.nelec = s.nelec;
+        r.shell_charges.assign(qout.begin() + s0, qout.begin() + s0 + nsh);
+        r.shell_potential.assign(vout.begin() + s0, vout.begin() + s0 + nsh);
+        r.cn.assign(cn.begin() + a0, cn.begin() + a0 + nat);
+        r.atomic_charges.assign(nat, 0.0);
+        for (int k = 0; k < nsh; ++k) r.atomic_charges[h_sh_atom_[s0 + k] - a0] += r.shell_charges[k];
+        for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+            for (int a = 0; a < nat; ++a) v += h_x_[3 * (a0 + a) + d] * r.atomic_charges[a];
+            r.dipole[d] = v;
+        }
+        r.eps.assign(eps.begin() + h_vec0_[i], eps.begin() + h_vec0_[i] + n);
+        r.occ.assign(occ.begin() + h_vec0_[i], occ.begin() + h_vec0_[i] + n);
+        auto slice3 = [&](const std::vector<double>& g, bool on) {
+            std::vector<double> v(3 * nat, 0.0);
+            if (on)
+                for (int k = 0; k < 3 * nat; ++k) v[k] = g[3 * a0 + k];
+            return v;
+        };
+        if (opt_.compute_gradient) {
+            const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+            r.g_el = slice3(gel, el_on);
+            r.g_rep = slice3(grep, opt_.include_repulsion);
+            r.g_disp = slice3(gdisp, opt_.include_dispersion && opt_.enable_dispersion);
+            r.g_hal = slice3(ghal, opt_.include_halogen);
+            r.gradient.assign(3 * nat, 0.0);
+            for (int k = 0; k < 3 * nat; ++k) r.gradient[k] = r.g_el[k] + r.g_rep[k] + r.g_disp[k] + r.g_hal[k];
+        }
+        if (opt_.keep_matrices) {
+            auto grab = [&](int role, std::vector<double>& dst) -> Status {
+                std::vector<double> full((size_t)ld * ld);
+                HIP_TRY(hipMemcpy(full.data(), hmat[(size_t)role * ninst_ + i], full.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+                dst.assign((size_t)n * n, 0.0);
+                for (int c = 0; c < n; ++c)
+                    for (int rr = 0; rr < n; ++rr) dst[(size_t)rr * n + c] = full[(size_t)c * ld + rr];
+                return Status{};
+            };
+            Status st;
+            if (!(st = grab(R_S, r.S)).ok()) return st;
+            if (!(st = grab(R_H0, r.H0)).ok()) return st;
+            if (!(st = grab(R_F, r.F)).ok()) return st;
+            if (!(st = grab(R_P, r.P)).ok()) return st;
+            if (!(st = grab(R_W, r.W)).ok()) return st;
+        }
+    }
+    return Status{};
+}
+
+}  // namespace gfn1

@@ -1,0 +1,162 @@
+// Batched GFN1-xTB energy + gradient driver (host side of the device engine).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "model.hpp"
+
+namespace gfn1 {
+
+// Model tables resident on one device.
+class DeviceModel {
+   public:
+    Status init(const Model& m);
+    ~DeviceModel();
+    DeviceTables tables() const;
+    const Model& model() const { return model_; }
+
+   private:
+    Model model_;
+    ShellType* shells_ = nullptr;
+    ElementType* elements_ = nullptr;
+    double* c6_ = nullptr;
+    double* pair_ = nullptr;
+};
+
+struct SystemInput {
+    int nat = 0;
+    const int32_t* z = nullptr;
+    const double* x = nullptr;  // bohr
+    double charge = 0.0;
+    const double* q0 = nullptr;  // optional initial shell charges
+    int q0_len = 0;
+};
+
+struct EngineOptions {
+    // electronic
+    int max_scc = 250;
+    double energy_tolerance = 1e-6;
+    double charge_tolerance = 2e-5;
+    double mixing = 0.4;
+    int scc_broyden = 1;
+    int scc_broyden_size = 250;
+    double level_shift = 0.0;
+    double eigen_tolerance = 1e-12;
+    double electronic_temperature = 300.0;
+    int enable_dispersion = 1;
+    double integral_cutoff = 30.0;
+    double coordination_cutoff = 30.0;
+    int enable_cn_hamiltonian = 1;
+    // gradient
+    int compute_gradient = 1;
+    int include_repulsion = 1, include_dispersion = 1, include_hamiltonian = 1, include_scc = 1,
+        include_halogen = 1;
+    // execution
+    int eigensolver = 0;   // 0 auto, 1 syevd, 2 syevj
+    int keep_matrices = 0;
+};
+
+// Per-instance results on the host.
+struct InstanceResult {
+    int status = 0;
+    std::string message;
+    int iterations = 0;
+    int converged = 0;
+    double energies[15] = {};
+    double fermi_level = 0, temperature = 0, nelec = 0, dipole[3] = {};
+    double rms = 0;
+    std::vector<double> gradient, g_el, g_rep, g_disp, g_hal;
+    std::vector<double> shell_charges, atomic_charges, shell_potential, cn, eps, occ;
+    std::vector<double> S, H0, F, P, W;  // n x n, only with keep_matrices
+    int nat = 0, nsh = 0, nao = 0;
+};
+
+class Batch {
+   public:
+    Batch() = default;
+    ~Batch();
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    Status create(const DeviceModel& dm, const std::vector<SystemInput>& systems, const EngineOptions& opt);
+    Status run();
+    Status download(std::vector<InstanceResult>& out);
+    const double* timings() const { return timings_; }
+    int size() const { return ninst_; }
+
+   private:
+    struct Bucket {
+        int ld = 0;
+        std::vector<int> inst;            // instance ids
+        double* arena = nullptr;          // matrices of the bucket
+        hipStream_t stream = nullptr;
+        rocblas_handle handle = nullptr;
+        // scratch per bucket
+        double** ptrs = nullptr;          // device pointer arrays (kPtrRoles * count)
+        int* act = nullptr;               // device active list
+        double* eig = nullptr;            // count * ld eigenvalues (active order)
+        double* eig_e = nullptr;          // syevd off-diagonal workspace
+        double* resid = nullptr;
+        int* sweeps = nullptr;
+        int* info = nullptr;
+        double* beta = nullptr;           // Broyden scratch for long histories
+        size_t beta_cap = 0;
+        std::vector<int> active;
+        int kP = 0;
+    };
+
+    Status setup_phase();
+    Status scc_phase();
+    Status gradient_phase();
+    Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
+    Status eigensolve(Bucket& bk, int count);
+
+    const DeviceModel* dm_ = nullptr;
+    EngineOptions opt_;
+    int ninst_ = 0, natoms_ = 0, nshells_ = 0, naos_ = 0;
+    std::vector<int> h_atom0_, h_sh0_, h_ao0_, h_ld_, h_atom_z_, h_sh_atom_, h_sh_type_, h_sh_ao_, h_ao_sh_,
+        h_atom_sh0_, h_atom_inst_;
+    std::vector<double> h_x_, h_q0_, h_nelec_;
+    std::vector<int64_t> h_vec0_, h_broff_;
+    std::vector<int> h_bucket_of_, h_final_parity_;
+    std::vector<int> h_err_, h_err_input_;
+    std::vector<std::string> h_errmsg_, h_errmsg_input_;
+    std::vector<Bucket> buckets_;
+    std::vector<void*> allocs_;
+    SccParams prm_{};
+    int broyden_memory_ = 2;
+    double timings_[4] = {0, 0, 0, 0};
+
+    // device arrays
+    DeviceBatch db_{};
+    int *d_atom0_ = nullptr, *d_sh0_ = nullptr, *d_ao0_ = nullptr, *d_ld_ = nullptr, *d_atom_inst_ = nullptr,
+        *d_atom_z_ = nullptr, *d_atom_sh0_ = nullptr, *d_sh_atom_ = nullptr, *d_sh_type_ = nullptr,
+        *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr;
+    int64_t *d_vec0_ = nullptr, *d_broff_ = nullptr;
+    double *d_x_ = nullptr, *d_cn_ = nullptr, *d_cnd3_ = nullptr, *d_gw_ = nullptr;
+    double *d_qin_ = nullptr, *d_qout_ = nullptr, *d_q0_ = nullptr, *d_vin_g_ = nullptr, *d_vin_t_ = nullptr,
+           *d_vout_g_ = nullptr, *d_vout_t_ = nullptr;
+    double *d_rowS_ = nullptr, *d_rowH_ = nullptr, *d_eps_ = nullptr, *d_occ_ = nullptr;
+    double *d_brdf_ = nullptr, *d_bru_ = nullptr, *d_brom_ = nullptr, *d_brql_ = nullptr, *d_brdql_ = nullptr;
+    double *d_erep_atom_ = nullptr, *d_edisp_atom_ = nullptr, *d_erep_ = nullptr, *d_edisp_ = nullptr,
+           *d_ehal_ = nullptr;
+    double *d_grep_ = nullptr, *d_gdisp_ = nullptr, *d_ghal_ = nullptr, *d_gel_ = nullptr, *d_dedcn_disp_ = nullptr,
+           *d_dedcn_h_ = nullptr;
+    double** d_mat_ = nullptr;  // 9 pointer tables of ninst entries
+    SccState* d_state_ = nullptr;
+    std::vector<SccState> h_state_;
+    hipStream_t stream_ = nullptr;
+    bool ran_ = false;
+
+    template <typename T>
+    Status dalloc(T** p, size_t count);
+};
+
+}  // namespace gfn1

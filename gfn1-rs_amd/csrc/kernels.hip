@@ -1,0 +1,1170 @@
+// HIP kernels of the GFN1-xTB SCC energy + gradient path (gfx950).
+//
+// Layout: instances are stored instance-major in flat index tables (atoms,
+// shells, AOs); dense matrices are per-instance column-major ld x ld blocks
+// (ld = AO count padded to a multiple of 32).  Every kernel that walks the SCC
+// takes a list of active instance ids so that converged instances drop out.
+//
+// Reference semantics are cited per kernel (paths relative to the reference
+// repository root).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace gfn1 {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kPrimScreen = 1.0e-13;   // integrals.rs:17
+constexpr double kPairEps2 = 1.0e-24;     // pairlist.rs:16 (DIST_EPS^2)
+
+__device__ __forceinline__ double ipow(double x, int n) {
+    double r = 1.0;
+    for (int i = 0; i < n; ++i) r *= x;
+    return r;
+}
+
+__device__ __forceinline__ double binom4(int n, int k) {
+    // n <= 4
+    const double t[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+    return t[n][k];
+}
+
+// 1-D Gaussian-product overlap moment without the K_AB factor (binomial
+// expansion about the product centre, integrals.rs:952-976).
+__device__ double moment1d(int i, int j, double pa, double pb, const double* g) {
+    double sum = 0.0;
+    for (int u = 0; u <= i; ++u) {
+        double ca = binom4(i, u) * ipow(pa, i - u);
+        for (int v = 0; v <= j; ++v) {
+            if ((u + v) & 1) continue;
+            double cb = binom4(j, v) * ipow(pb, j - v);
+            sum += ca * cb * g[(u + v) >> 1];
+        }
+    }
+    return sum;
+}
+
+// Cartesian content of the real spherical AOs (basis.rs:382-441):
+// s; px py pz; d x2-y2, z2, xy, xz, yz.
+struct CartTerm {
+    int8_t px, py, pz;
+    double c;
+};
+__device__ __forceinline__ int ao_terms(int l, int ao, CartTerm* t) {
+    const double h3 = 0.86602540378443864676;  // sqrt(3)/2
+    const double s3 = 1.73205080756887729353;  // sqrt(3)
+    if (l == 0) {
+        t[0] = {0, 0, 0, 1.0};
+        return 1;
+    }
+    if (l == 1) {
+        t[0] = {(int8_t)(ao == 0), (int8_t)(ao == 1), (int8_t)(ao == 2), 1.0};
+        return 1;
+    }
+    switch (ao) {
+        case 0: t[0] = {2, 0, 0, h3}; t[1] = {0, 2, 0, -h3}; return 2;
+        case 1: t[0] = {2, 0, 0, 0.5}; t[1] = {0, 2, 0, 0.5}; t[2] = {0, 0, 2, -1.0}; return 3;
+        case 2: t[0] = {1, 1, 0, s3}; return 1;
+        case 3: t[0] = {1, 0, 1, s3}; return 1;
+        default: t[0] = {0, 1, 1, s3}; return 1;
+    }
+}
+// sum of |component coefficients| of an AO (integrals.rs:21-24)
+__device__ __forceinline__ double ao_abs_sum(int l, int ao) {
+    if (l < 2) return 1.0;
+    return ao == 1 ? 2.0 : 1.73205080756887729353;
+}
+
+// Overlap block <A mu | B nu> of two shells and, with DERIV, its derivatives
+// with respect to the bra centre (dA) and the ket centre (dB).
+// integrals.rs:290-360 (values) and :481-537, :782-868 (derivatives), with the
+// primitive screening bound of :30-45.  Block index: mu * 5 + nu.
+template <bool DERIV>
+__device__ void shell_pair(const ShellType& A, const ShellType& B, const double* ra, const double* rb, double* S,
+                           double* dA, double* dB) {
+    const int la = A.l, lb = B.l, na = 2 * la + 1, nb = 2 * lb + 1;
+    for (int k = 0; k < 25; ++k) S[k] = 0.0;
+    if (DERIV)
+        for (int k = 0; k < 75; ++k) dA[k] = dB[k] = 0.0;
+    const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    const int order = la + lb + (DERIV ? 3 : 2);
+    for (int ip = 0; ip < A.nprim; ++ip)
+        for (int jp = 0; jp < B.nprim; ++jp) {
+            const double pref = A.coeff[ip] * B.coeff[jp];
+            const double al = A.alpha[ip], be = B.alpha[jp], p = al + be;
+            const double kab = exp(-al * be / p * r2);
+            const double spread = sqrt(1.0 / p);
+            const double gpref = (kPi / p) * sqrt(kPi / p);
+            const double poly = ipow(1.0 + r + spread, order);
+            // the largest |component| product of this shell pair bounds every AO pair
+            const double csmax = ao_abs_sum(la, 1) * ao_abs_sum(lb, 1);
+            if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
+            double g[5];
+            g[0] = sqrt(kPi / p);
+            for (int m = 1; m < 5; ++m) g[m] = g[m - 1] * (m - 0.5) / p;
+            // 1-D tables m[dim][i][j], i <= la+1, j <= lb+1
+            double mt[3][4][4];
+            const int imax = la + (DERIV ? 1 : 0), jmax = lb + (DERIV ? 1 : 0);
+            for (int d = 0; d < 3; ++d) {
+                const double P = (al * ra[d] + be * rb[d]) / p;
+                const double pa = P - ra[d], pb = P - rb[d];
+                for (int i = 0; i <= imax; ++i)
+                    for (int j = 0; j <= jmax; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, g);
+            }
+            for (int mu = 0; mu < na; ++mu) {
+                CartTerm ta[3];
+                const int nta = ao_terms(la, mu, ta);
+                const double csa = ao_abs_sum(la, mu);
+                for (int nu = 0; nu < nb; ++nu) {
+                    const double cs = csa * ao_abs_sum(lb, nu);
+                    if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
+                    CartTerm tb[3];
+                    const int ntb = ao_terms(lb, nu, tb);
+                    double s = 0.0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+                    for (int a = 0; a < nta; ++a)
+                        for (int b = 0; b < ntb; ++b) {
+                            const double cf = pref * ta[a].c * tb[b].c * kab;
+                            const int ia[3] = {ta[a].px, ta[a].py, ta[a].pz};
+                            const int jb[3] = {tb[b].px, tb[b].py, tb[b].pz};
+                            const double m0 = mt[0][ia[0]][jb[0]], m1 = mt[1][ia[1]][jb[1]], m2 = mt[2][ia[2]][jb[2]];
+                            s += cf * (m0 * m1 * m2);
+                            if (DERIV) {
+                                double mm[3] = {m0, m1, m2};
+                                for (int d = 0; d < 3; ++d) {
+                                    const double o1 = mm[(d + 1) % 3] * mm[(d + 2) % 3];
+                                    const double da_ = 2.0 * al * mt[d][ia[d] + 1][jb[d]] -
+                                                       (ia[d] > 0 ? ia[d] * mt[d][ia[d] - 1][jb[d]] : 0.0);
+                                    const double db_ = 2.0 * be * mt[d][ia[d]][jb[d] + 1] -
+                                                       (jb[d] > 0 ? jb[d] * mt[d][ia[d]][jb[d] - 1] : 0.0);
+                                    ga[d] += cf * (da_ * o1);
+                                    gb[d] += cf * (db_ * o1);
+                                }
+                            }
+                        }
+                    S[mu * 5 + nu] += s;
+                    if (DERIV)
+                        for (int d = 0; d < 3; ++d) {
+                            dA[(mu * 5 + nu) * 3 + d] += ga[d];
+                            dB[(mu * 5 + nu) * 3 + d] += gb[d];
+                        }
+                }
+            }
+        }
+}
+
+__device__ __forceinline__ double kshell(const Globals& g, int l) { return l == 0 ? g.ks : l == 1 ? g.kp : g.kd; }
+__device__ __forceinline__ double kpair(const Globals& g, int la, int lb) {
+    if ((la == 0 && lb == 1) || (la == 1 && lb == 0)) return g.ksp;
+    return 0.5 * (kshell(g, la) + kshell(g, lb));
+}
+// H0 shell-pair scaling (hamiltonian.rs:140-165)
+__device__ double hscale(const DeviceTables& t, int za, const ShellType& A, int zb, const ShellType& B) {
+    const Globals& g = t.glob;
+    if (A.valence && B.valence) {
+        const double den = t.elements[za].en - t.elements[zb].en;
+        return t.pair_scale[za * (kMaxZ + 1) + zb] * kpair(g, A.l, B.l) * (1.0 + g.enscale * 0.01 * (den * den));
+    }
+    if (A.valence) return 0.5 * (kpair(g, A.l, A.l) + g.kdiff);
+    if (B.valence) return 0.5 * (kpair(g, B.l, B.l) + g.kdiff);
+    return g.kdiff;
+}
+
+// Exponential counting function and its radial derivative (coordination.rs:96-106).
+__device__ __forceinline__ void count_fn(double r, double rc, double& value, double& dvalue) {
+    const double raw = -16.0 * (rc / r - 1.0);
+    const double arg = fmin(80.0, fmax(-80.0, raw));
+    const double e = exp(arg);
+    value = 1.0 / (1.0 + e);
+    dvalue = (raw < -80.0 || raw > 80.0) ? 0.0 : -16.0 * rc * e / (r * r * ((1.0 + e) * (1.0 + e)));
+}
+
+__device__ __forceinline__ double dist2(const double* x, int a, int b) {
+    const double dx = x[3 * b] - x[3 * a], dy = x[3 * b + 1] - x[3 * a + 1], dz = x[3 * b + 2] - x[3 * a + 2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Deterministic block sum (blockDim.x a multiple of 64, <= 1024); result on every thread.
+__device__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += lds[w];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Coordination numbers for H0 (cutoff cn_cut) and for D3 (25 bohr), plus the
+// D3 Gaussian reference weights (coordination.rs:51-94; dispersion.rs:1629-1692).
+__global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
+                               double* gw, double* dgw, int natoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const double rca = t.elements[b.atom_z[a]].covalent_radius;
+    double c = 0.0, cd = 0.0;
+    const double cc2 = cn_cut * cn_cut, cd2 = 25.0 * 25.0;
+    for (int o = a0; o < a1; ++o) {
+        if (o == a) continue;
+        const double r2 = dist2(x, a, o);
+        if (r2 <= kPairEps2) continue;
+        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+        if (rc <= 1e-12) continue;
+        const double r = sqrt(r2);
+        double v, dv;
+        if (r2 <= cc2 || r2 <= cd2) count_fn(r, rc, v, dv);
+        if (r2 <= cc2) c += v;
+        if (r2 <= cd2) cd += v;
+    }
+    cn[a] = cn_on ? c : 0.0;
+    cn_d3[a] = cd;
+    // D3 reference weights
+    const ElementType& e = t.elements[b.atom_z[a]];
+    const int nref = e.d3_nref;
+    double mx = -INFINITY;
+    for (int r = 0; r < nref; ++r) {
+        const double d = cd - e.d3_refcn[r];
+        mx = fmax(mx, -4.0 * d * d);
+    }
+    double raw[kMaxRef], norm = 0.0, dnorm = 0.0;
+    for (int r = 0; r < nref; ++r) {
+        const double d = cd - e.d3_refcn[r];
+        raw[r] = exp(-4.0 * d * d - mx);
+        norm += raw[r];
+        dnorm += 8.0 * (e.d3_refcn[r] - cd) * raw[r];
+    }
+    const double ni = 1.0 / norm;
+    for (int r = 0; r < kMaxRef; ++r) {
+        double w = 0.0, dw = 0.0;
+        if (r < nref) {
+            const double expd = 8.0 * (e.d3_refcn[r] - cd) * raw[r];
+            w = raw[r] * ni;
+            dw = expd * ni - raw[r] * dnorm * ni * ni;
+            if (!isfinite(w)) w = 0.0;
+            if (!isfinite(dw)) dw = 0.0;
+        }
+        gw[a * kMaxRef + r] = w;
+        dgw[a * kMaxRef + r] = dw;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Overlap S and core Hamiltonian H0 (integrals.rs:133-242; hamiltonian.rs:91-134).
+// One work-group per atom A; lanes walk the (shell of A, shell of the instance)
+// pairs and write column block A of S and H0 (coalesced along the other index).
+__global__ void k_overlap_h0(DeviceBatch b, DeviceTables t, const double* cn, double cutoff) {
+    const int a = blockIdx.x;
+    const int inst = b.atom_inst[a];
+    const int s0 = b.sh0[inst], nshi = b.sh0[inst + 1] - s0;
+    const int za = b.atom_z[a];
+    const int nsha = t.elements[za].nshell;
+    const int ld = b.ld[inst];
+    double* S = b.S[inst];
+    double* H = b.H0[inst];
+    const double* x = b.atom_x;
+    const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+    const double c2 = cutoff * cutoff;
+    const int sa0 = b.atom_sh0[a];
+    for (int item = threadIdx.x; item < nsha * nshi; item += blockDim.x) {
+        const int sa = sa0 + item / nshi;
+        const int sb = s0 + item % nshi;
+        const int ob = b.sh_atom[sb];
+        const ShellType& A = t.shells[b.sh_type[sa]];
+        const ShellType& B = t.shells[b.sh_type[sb]];
+        const int zb = b.atom_z[ob];
+        const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
+        double blk[25];
+        bool inside = true;
+        double r2 = 0.0;
+        if (ob != a) {
+            r2 = dist2(x, a, ob);
+            inside = r2 > kPairEps2 && r2 <= c2;
+        }
+        const double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
+        if (inside) shell_pair<false>(A, B, ra, rb, blk, nullptr, nullptr);
+        else
+            for (int k = 0; k < 25; ++k) blk[k] = 0.0;
+        const double sea = A.hdiag - A.kcn * cn[a];
+        const double seb = B.hdiag - B.kcn * cn[ob];
+        double h;
+        if (ob == a) {
+            h = 0.5 * (sea + seb);
+        } else {
+            const double rr = sqrt(sqrt(r2) / (t.elements[za].atomic_radius + t.elements[zb].atomic_radius));
+            h = 0.5 * (sea + seb) * hscale(t, za, A, zb, B) * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
+        }
+        const int na = 2 * A.l + 1, nb = 2 * B.l + 1;
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j) {
+                const double s = blk[i * 5 + j];
+                // element (nu, mu) of the symmetric matrices: column mu, row nu
+                const size_t idx = (size_t)(mu0 + i) * ld + (nu0 + j);
+                S[idx] = s;
+                H[idx] = s * h;
+            }
+    }
+}
+
+// Diagonal of the padded rows/columns: S = 1, H0 = F = kPadEnergy.
+__global__ void k_pad_diagonal(DeviceBatch b) {
+    const int inst = blockIdx.x;
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    for (int i = n + threadIdx.x; i < ld; i += blockDim.x) {
+        const size_t idx = (size_t)i * ld + i;
+        b.S[inst][idx] = 1.0;
+        b.H0[inst][idx] = kPadEnergy;
+        b.F[inst][idx] = kPadEnergy;
+    }
+}
+
+// Cb = (L^-1)^T from the batched triangular inverse of the Cholesky factor
+// (stored column-major lower in Linv); also checks the factor's pivots.
+__global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, double eig_tol, int* fail) {
+    const int inst = act[blockIdx.y];
+    const int ld = b.ld[inst];
+    const int n = b.ao0[inst + 1] - b.ao0[inst];
+    double* cb = b.CB0[inst];
+    const double* li = Linv[blockIdx.y];
+    const size_t total = (size_t)ld * ld;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
+        const int row = (int)(k % ld), col = (int)(k / ld);
+        // Cb(row, col) = Linv(col, row) for col >= row
+        cb[k] = col >= row ? li[(size_t)row * ld + col] : 0.0;
+        if (row == col && row < n) {
+            // L_ii = 1/Linv_ii; an overlap eigenvalue <= tol forces a pivot^2 <= tol
+            const double lii = 1.0 / li[(size_t)row * ld + row];
+            if (!(lii * lii > eig_tol)) fail[inst] = 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shell potential v = gamma q (+ on-site third order) (coulomb.rs:82-107, :140-193).
+// vg gets gamma q only; vt the full potential.
+__global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
+                                  double* vt) {
+    const int inst = act[blockIdx.y];
+    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsh) return;
+    const int si = s0 + i;
+    const int ai = b.sh_atom[si];
+    const double hi = t.shells[b.sh_type[si]].hardness;
+    const double* x = b.atom_x;
+    double v = 0.0;
+    for (int j = 0; j < nsh; ++j) {
+        const int sj = s0 + j;
+        const int aj = b.sh_atom[sj];
+        const double hj = t.shells[b.sh_type[sj]].hardness;
+        const double h = 2.0 / (1.0 / hi + 1.0 / hj);
+        double gam;
+        if (ai == aj) gam = h;
+        else gam = 1.0 / sqrt(dist2(x, ai, aj) + 1.0 / (h * h));
+        v += gam * q[sj];
+    }
+    vg[si] = v;
+    // third order: Gamma_A q_A^2 on every shell of atom A
+    const int f0 = b.atom_sh0[ai];
+    const int za = b.atom_z[ai];
+    const int nsa = t.elements[za].nshell;
+    double qa = 0.0;
+    for (int k = 0; k < nsa; ++k) qa += q[f0 + k];
+    vt[si] = v + qa * qa * t.elements[za].gam3;
+}
+
+// F = H0 - 1/2 S (v_mu + v_nu) (electronic.rs:3133-3158); one block per column.
+__global__ void k_fock(DeviceBatch b, const int* act, const double* vt) {
+    const int inst = act[blockIdx.y];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int col = blockIdx.x;
+    if (col >= n) return;
+    const int a0 = b.ao0[inst];
+    const double vc = vt[b.ao_sh[a0 + col]];
+    const double* S = b.S[inst] + (size_t)col * ld;
+    const double* H = b.H0[inst] + (size_t)col * ld;
+    double* F = b.F[inst] + (size_t)col * ld;
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        const double vr = vt[b.ao_sh[a0 + row]];
+        F[row] = H[row] + (-S[row] * 0.5 * (vr + vc));
+    }
+}
+
+// F += b (S - 1/2 S P S) for instances on a level-shift rung (electronic.rs:2120-2133).
+__global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift, double* const* SPS) {
+    const int inst = act[blockIdx.y];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int col = blockIdx.x;
+    if (col >= n) return;
+    const double ls = shift[blockIdx.y];
+    const double* S = b.S[inst] + (size_t)col * ld;
+    const double* M = SPS[blockIdx.y] + (size_t)col * ld;
+    double* F = b.F[inst] + (size_t)col * ld;
+    for (int row = threadIdx.x; row < n; row += blockDim.x) F[row] += ls * (S[row] - 0.5 * M[row]);
+}
+
+// ---------------------------------------------------------------------------
+// Fermi occupations by bisection to the f64 fixed point (electronic.rs:3853-3921).
+// eig: eigenvalues of the active list (stride ld), copied into the per-instance
+// orbital-energy vector.  Padded orbitals get zero occupation.
+__global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int eig_stride, double* eps_all,
+                        double* occ_all, SccState* st) {
+    __shared__ double lds[16];
+    const int inst = act[blockIdx.x];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const double* e = eig + (size_t)blockIdx.x * eig_stride;
+    double* eps = eps_all + b.vec0[inst];
+    double* occ = occ_all + b.vec0[inst];
+    SccState& s = st[inst];
+    const double kt = s.kt, nelec = s.nelec;
+    for (int i = threadIdx.x; i < ld; i += blockDim.x) eps[i] = e[i];
+    double mn = INFINITY, mx = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        mn = fmin(mn, e[i]);
+        mx = fmax(mx, e[i]);
+    }
+    // block min / max
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_down(mn, off, 64));
+        mx = fmax(mx, __shfl_down(mx, off, 64));
+    }
+    __shared__ double smn[16], smx[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) { smn[wid] = mn; smx[wid] = mx; }
+    __syncthreads();
+    mn = smn[0];
+    mx = smx[0];
+    for (int w = 1; w < nw; ++w) { mn = fmin(mn, smn[w]); mx = fmax(mx, smx[w]); }
+    __syncthreads();
+    double mu = 0.0, ts = 0.0;
+    int kneed = 0;
+    if (kt <= 0.0) {
+        // aufbau (electronic.rs:3829-3851)
+        if (threadIdx.x == 0) {
+            double rem = fmax(nelec, 0.0);
+            int homo = 0;
+            for (int i = 0; i < ld; ++i) {
+                double f = 0.0;
+                if (i < n && rem > 0.0) {
+                    f = fmin(rem, 2.0);
+                    rem -= f;
+                }
+                occ[i] = f;
+                if (i < n && f > 1e-12) homo = i;
+            }
+            mu = n ? e[homo] : 0.0;
+            kneed = homo + 1;
+        }
+    } else if (nelec <= 0.0 || nelec >= 2.0 * n) {
+        const double f = nelec <= 0.0 ? 0.0 : 2.0;
+        for (int i = threadIdx.x; i < ld; i += blockDim.x) occ[i] = i < n ? f : 0.0;
+        mu = nelec <= 0.0 ? e[0] : e[n - 1];
+        kneed = nelec <= 0.0 ? 0 : n;
+    } else {
+        double lo = mn - 100.0 * kt - 10.0, hi = mx + 100.0 * kt + 10.0;
+        for (int it = 0; it < 200; ++it) {
+            const double m = 0.5 * (lo + hi);
+            double part = 0.0;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const double xx = fmin(80.0, fmax(-80.0, (e[i] - m) / kt));
+                part += 2.0 / (1.0 + exp(xx));
+            }
+            const double sum = block_sum(part, lds);
+            double nlo = lo, nhi = hi;
+            if (sum < nelec) nlo = m;
+            else nhi = m;
+            if (nlo == lo && nhi == hi) break;
+            lo = nlo;
+            hi = nhi;
+        }
+        mu = 0.5 * (lo + hi);
+        double tpart = 0.0;
+        int kpart = 0;
+        for (int i = threadIdx.x; i < ld; i += blockDim.x) {
+            double f = 0.0;
+            if (i < n) {
+                const double xx = fmin(80.0, fmax(-80.0, (e[i] - mu) / kt));
+                f = 2.0 / (1.0 + exp(xx));
+                const double h = fmin(1.0 - 1e-16, fmax(1e-16, 0.5 * f));
+                tpart += 2.0 * kt * (h * log(h) + (1.0 - h) * log(1.0 - h));
+                if (f >= 1e-28) kpart = max(kpart, i + 1);
+            }
+            occ[i] = f;
+        }
+        ts = block_sum(tpart, lds);
+        // block max of kpart
+        int km = kpart;
+        for (int off = 32; off > 0; off >>= 1) km = max(km, __shfl_down(km, off, 64));
+        __shared__ int skm[16];
+        if (lane == 0) skm[wid] = km;
+        __syncthreads();
+        km = skm[0];
+        for (int w = 1; w < nw; ++w) km = max(km, skm[w]);
+        kneed = km;
+    }
+    if (threadIdx.x == 0) {
+        s.mu = mu;
+        s.ts = ts;
+        s.kneed = kneed;
+    }
+}
+
+// T(:, j) = C(:, j) * w_j for j < k, with w = f (density) or eps*f (energy-weighted).
+__global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
+                                const double* eps_all, const double* occ_all, int k, int weighted) {
+    const int inst = act[blockIdx.y];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int col = blockIdx.x;
+    if (col >= k) return;
+    const double f = occ_all[b.vec0[inst] + col];
+    const double w = weighted ? eps_all[b.vec0[inst] + col] * f : f;
+    const double* c = C[blockIdx.y] + (size_t)col * ld;
+    double* o = T[blockIdx.y] + (size_t)col * ld;
+    for (int row = threadIdx.x; row < ld; row += blockDim.x) o[row] = row < n ? c[row] * w : 0.0;
+}
+
+// M <- (M + M^T) / 2 on the n x n block (linalg.rs:262-270); 32x32 tiles.
+__global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp) {
+    __shared__ double t1[32][33], t2[32][33];
+    const int inst = act[blockIdx.z];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    if (bj > bi) return;
+    if (bi * 32 >= n) return;
+    double* M = Mp[blockIdx.z];
+    const int tx = threadIdx.x & 31, ty0 = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
+    for (int ty = ty0; ty < 32; ty += 8) {
+        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;  // tile (bi, bj): rows bi
+        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;  // tile (bj, bi)
+        t1[ty][tx] = (r1 < n && c1 < n) ? M[(size_t)c1 * ld + r1] : 0.0;
+        t2[ty][tx] = (r2 < n && c2 < n) ? M[(size_t)c2 * ld + r2] : 0.0;
+    }
+    __syncthreads();
+    for (int ty = ty0; ty < 32; ty += 8) {
+        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;
+        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;
+        // element (r1, c1) pairs with (c1, r1) = tile2 entry at [tx'=c1-bj*32... ]
+        if (r1 < n && c1 < n) M[(size_t)c1 * ld + r1] = 0.5 * (t1[ty][tx] + t2[tx][ty]);
+        if (bi != bj && r2 < n && c2 < n) M[(size_t)c2 * ld + r2] = 0.5 * (t2[ty][tx] + t1[tx][ty]);
+    }
+}
+
+// Column sums of S o P and H0 o P (= row sums, the matrices being symmetric):
+// Mulliken populations per AO and the band energy (electronic.rs:3160-3188).
+__global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH) {
+    __shared__ double lds[16];
+    const int inst = act[blockIdx.y];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int col = blockIdx.x;
+    if (col >= n) return;
+    const double* S = b.S[inst] + (size_t)col * ld;
+    const double* H = b.H0[inst] + (size_t)col * ld;
+    const double* P = b.P[inst] + (size_t)col * ld;
+    double ps = 0.0, ph = 0.0;
+    for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        ps += S[row] * P[row];
+        ph += H[row] * P[row];
+    }
+    ps = block_sum(ps, lds);
+    ph = block_sum(ph, lds);
+    if (threadIdx.x == 0) {
+        rowS[b.ao0[inst] + col] = ps;
+        rowH[b.ao0[inst] + col] = ph;
+    }
+}
+
+// Output shell charges q = n0 - sum_{mu in shell} (S P)_mu mu and the band energy.
+__global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
+                       double* qout, SccState* st) {
+    __shared__ double lds[16];
+    const int inst = act[blockIdx.x];
+    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
+    const int a0 = b.ao0[inst], n = b.ao0[inst + 1] - a0;
+    for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+        const int si = s0 + i;
+        const ShellType& T = t.shells[b.sh_type[si]];
+        double pop = 0.0;
+        for (int k = 0; k < T.nao; ++k) pop += rowS[a0 + b.sh_ao[si] + k];
+        qout[si] = T.ref_occ - pop;
+    }
+    double e = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) e += rowH[a0 + i];
+    e = block_sum(e, lds);
+    if (threadIdx.x == 0) st[inst].e_el = e;
+}
+
+// ---------------------------------------------------------------------------
+// Convergence test and charge mixing (electronic.rs:1843-1947, 3203-3311).
+struct LadderRung {
+    int broyden;
+    double mixing, level_shift, temperature;
+    int max_iter;
+};
+
+__device__ bool solve_gj(double* a, double* bvec, int m, int lda) {
+    // Gauss-Jordan with partial pivoting (electronic.rs:3767-3806); single thread.
+    for (int col = 0; col < m; ++col) {
+        int piv = col;
+        double pa = fabs(a[col * lda + col]);
+        for (int r = col + 1; r < m; ++r)
+            if (fabs(a[r * lda + col]) > pa) {
+                piv = r;
+                pa = fabs(a[r * lda + col]);
+            }
+        if (pa < 1e-18) return false;
+        if (piv != col) {
+            for (int j = 0; j < m; ++j) {
+                double tmp = a[piv * lda + j];
+                a[piv * lda + j] = a[col * lda + j];
+                a[col * lda + j] = tmp;
+            }
+            double tmp = bvec[piv];
+            bvec[piv] = bvec[col];
+            bvec[col] = tmp;
+        }
+        const double dg = a[col * lda + col];
+        for (int j = col; j < m; ++j) a[col * lda + j] /= dg;
+        bvec[col] /= dg;
+        for (int r = 0; r < m; ++r) {
+            if (r == col) continue;
+            const double f = a[r * lda + col];
+            if (f == 0.0) continue;
+            for (int j = col; j < m; ++j) a[r * lda + j] -= f * a[col * lda + j];
+            bvec[r] -= f * bvec[col];
+        }
+    }
+    return true;
+}
+
+__global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccParams prm, const double* qout,
+                             const double* vg_out, double* qin, const double* q0, SccState* st, double* br_df,
+                             double* br_u, double* br_omega, double* br_qlast, double* br_dqlast,
+                             const int64_t* br_off, double* beta_scratch, int beta_ld) {
+    __shared__ double lds[16];
+    __shared__ double sh_beta[32 * 33];
+    __shared__ double sh_c[32];
+    __shared__ int sh_flag;
+    const int inst = act[blockIdx.x];
+    SccState& s = st[inst];
+    if (s.status != 0) return;
+    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
+    const int atom0 = b.atom0[inst], nat = b.atom0[inst + 1] - atom0;
+    // E2 = 1/2 q.gamma q, E3 = sum_A Gamma_A q_A^3 / 3
+    double e2 = 0.0, ss = 0.0;
+    for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+        e2 += qout[s0 + i] * vg_out[s0 + i];
+        const double d = qout[s0 + i] - qin[s0 + i];
+        ss += d * d;
+    }
+    double e3 = 0.0;
+    for (int a = threadIdx.x; a < nat; a += blockDim.x) {
+        const int ga = atom0 + a;
+        const int f0 = b.atom_sh0[ga];
+        const int za = b.atom_z[ga];
+        double qa = 0.0;
+        for (int k = 0; k < t.elements[za].nshell; ++k) qa += qout[f0 + k];
+        e3 += qa * qa * qa * t.elements[za].gam3 / 3.0;
+    }
+    e2 = 0.5 * block_sum(e2, lds);
+    e3 = block_sum(e3, lds);
+    ss = block_sum(ss, lds);
+    const double rms = nsh ? sqrt(ss / nsh) : 0.0;
+    const double energy = s.e_el + e2 + e3 + s.ts;
+    const double de = s.have_energy ? fabs(energy - s.e_last) : INFINITY;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s.e2 = e2;
+        s.e3 = e3;
+        s.rms = rms;
+        s.energy = energy;
+        s.iter += 1;
+        s.total_iter += 1;
+    }
+    if (de < prm.energy_tolerance && rms < prm.charge_tolerance) {
+        if (threadIdx.x == 0) s.status = 1;
+        return;
+    }
+    // ---- next input charges
+    const LadderRung rung = {prm.rung_broyden[s.rung], prm.rung_mixing[s.rung], prm.rung_level_shift[s.rung],
+                             prm.rung_temperature[s.rung], prm.rung_max_iter[s.rung]};
+    const double mix = rung.mixing;
+    double* qi = qin + s0;
+    const double* qo = qout + s0;
+    bool use_damped = true;
+    if (rung.broyden) {
+        // BroydenMixer::next; state lives in global memory
+        const int mem = prm.broyden_memory;
+        const int64_t off = br_off[inst];
+        double* df = br_df + off;
+        double* u = br_u + off;
+        double* omega = br_omega + (size_t)inst * prm.broyden_slots;
+        double* qlast = br_qlast + s0;
+        double* dqlast = br_dqlast + s0;
+        const int iter = s.broyden_iter + 1;
+        if (iter == 1) {
+            for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+                const double dq = qo[i] - qi[i];
+                dqlast[i] = dq;
+                qlast[i] = qi[i];
+            }
+            // the damped first step is the candidate; trusted-check below
+            use_damped = true;
+        } else {
+            const int itn = iter - 1;
+            const int it1 = (itn - 1) % mem;
+            double dqn = 0.0, dfn = 0.0;
+            for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+                const double dq = qo[i] - qi[i];
+                const double dfc = dq - dqlast[i];
+                dqn += dq * dq;
+                dfn += dfc * dfc;
+            }
+            dqn = sqrt(block_sum(dqn, lds));
+            dfn = block_sum(dfn, lds);
+            const double wfac = 0.01, maxw = 100000.0, minw = 1.0, omega0 = 0.01;
+            double om = dqn > wfac / maxw ? wfac / dqn : maxw;
+            if (om < minw) om = minw;
+            const double inv = 1.0 / fmax(sqrt(dfn), 2.220446049250313e-16);
+            for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+                const double dq = qo[i] - qi[i];
+                const double dfi = (dq - dqlast[i]) * inv;
+                df[(size_t)it1 * nsh + i] = dfi;
+                u[(size_t)it1 * nsh + i] = mix * dfi + inv * (qi[i] - qlast[i]);
+            }
+            if (threadIdx.x == 0) omega[it1] = om;
+            __syncthreads();
+            const int start = itn > mem ? itn - mem + 1 : 1;
+            const int m = itn - start + 1;
+            // m <= 32 lives in LDS; longer histories use a caller-provided scratch
+            // of (beta_ld + 1) * beta_ld doubles per active instance
+            double* scratch = beta_scratch + (size_t)blockIdx.x * (beta_ld + 1) * beta_ld;
+            double* beta = m <= 32 ? sh_beta : scratch;
+            const int bl = m <= 32 ? 33 : beta_ld;
+            double* cvec = m <= 32 ? sh_c : scratch + (size_t)beta_ld * beta_ld;
+            // beta and c by dot products; one (r, c) pair per thread-iteration
+            for (int pr = threadIdx.x; pr < m * (m + 1); pr += blockDim.x) {
+                const int r = pr / (m + 1), c = pr % (m + 1);
+                const int ir = (start + r - 1) % mem;
+                double dsum = 0.0;
+                if (c < m) {
+                    const int ic = (start + c - 1) % mem;
+                    for (int i = 0; i < nsh; ++i) dsum += df[(size_t)ir * nsh + i] * df[(size_t)ic * nsh + i];
+                    double v = omega[ir] * omega[ic] * dsum;
+                    if (r == c) v += omega0 * omega0;
+                    beta[r * bl + c] = v;
+                } else {
+                    for (int i = 0; i < nsh; ++i) dsum += df[(size_t)ir * nsh + i] * (qo[i] - qi[i]);
+                    cvec[r] = omega[ir] * dsum;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) sh_flag = solve_gj(beta, cvec, m, bl) ? 1 : 0;
+            __syncthreads();
+            if (sh_flag) {
+                // candidate = q + damp dq - sum_r omega_r c_r u_r ; then qlast/dqlast update
+                int bad = 0;
+                for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
+                    const double dq = qo[i] - qi[i];
+                    double v = qi[i] + mix * dq;
+                    for (int r = 0; r < m; ++r) {
+                        const int ir = (start + r - 1) % mem;
+                        v -= omega[ir] * cvec[r] * u[(size_t)ir * nsh + i];
+                    }
+                    if (!isfinite(v) || fabs(v) >= 10.0) bad = 1;
+                    dqlast[i] = dq;     // BroydenMixer::next updates dqlast/qlast on success
+                    const double qold = qi[i];
+                    qlast[i] = qold;
+                    // write candidate into qin directly; revert below if untrusted
+                    qi[i] = v;
+                }
+                const double nb = block_sum((double)bad, lds);
+                if (nb == 0.0) {
+                    use_damped = false;
+                } else {
+                    // revert: qin = qlast (the pre-step charges), then take the damped step
+                    for (int i = threadIdx.x; i < nsh; i += blockDim.x) qi[i] = qlast[i];
+                    __syncthreads();
+                    use_damped = true;
+                }
+            } else {
+                use_damped = true;  // singular: qlast/dqlast unchanged (reference returns early)
+            }
+        }
+        if (threadIdx.x == 0) s.broyden_iter = iter;
+    }
+    __syncthreads();
+    if (use_damped) {
+        for (int i = threadIdx.x; i < nsh; i += blockDim.x) qi[i] = qi[i] + mix * (qo[i] - qi[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s.have_prev_p = rung.level_shift != 0.0 ? 1 : 0;
+        s.e_last = energy;
+        s.have_energy = 1;
+    }
+    // rung exhausted -> next rung or failure (electronic.rs:1413-1446, 1950-1955)
+    if (s.iter >= rung.max_iter) {
+        __syncthreads();
+        if (s.rung >= 3) {
+            if (threadIdx.x == 0) s.status = 2;
+            return;
+        }
+        const int nr = s.rung + 1;
+        for (int i = threadIdx.x; i < nsh; i += blockDim.x) qi[i] = q0[s0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s.rung = nr;
+            s.iter = 0;
+            s.have_energy = 0;
+            s.have_prev_p = 0;
+            s.broyden_iter = 0;
+            s.mixing = prm.rung_mixing[nr];
+            s.level_shift = prm.rung_level_shift[nr];
+            s.temperature = prm.rung_temperature[nr];
+            s.kt = fmax(prm.rung_temperature[nr], 0.0) * kBoltzmannHartree;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Geometry-only terms.  One thread per atom A, looping over the instance's
+// other atoms; every pair is evaluated from both sides so the per-atom sums
+// need no atomics and are deterministic.
+//   repulsion (repulsion.rs:39-59, :139-159), 25 bohr
+//   D3(BJ) two-body energy, gradient and dE/dCN (dispersion.rs:1232-1300)
+__global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw, const double* cn_d3, int disp_on,
+                                 double* e_rep_atom, double* e_disp_atom, double* g_rep, double* g_disp,
+                                 double* dedcn_disp, int natoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const int za = b.atom_z[a];
+    const ElementType& EA = t.elements[za];
+    const Globals& g = t.glob;
+    double er = 0.0, ed_sum = 0.0, gr[3] = {0, 0, 0}, gd[3] = {0, 0, 0}, dcn = 0.0;
+    const double* gwa = gw + (size_t)a * kMaxRef;
+    const double* dga = gw + (size_t)natoms * kMaxRef + (size_t)a * kMaxRef;  // dgw follows gw
+    for (int o = a0; o < a1; ++o) {
+        if (o == a) continue;
+        const double r2 = dist2(x, a, o);
+        if (r2 <= kPairEps2) continue;
+        const double r = sqrt(r2);
+        const int zb = b.atom_z[o];
+        const ElementType& EB = t.elements[zb];
+        const double dr[3] = {x[3 * o] - x[3 * a], x[3 * o + 1] - x[3 * a + 1], x[3 * o + 2] - x[3 * a + 2]};
+        if (r2 <= 25.0 * 25.0) {
+            const double alpha = sqrt(EA.repa * EB.repa);
+            const double zeff = EA.repb * EB.repb;
+            const double k = (za <= 2 && zb <= 2) ? g.kexplight : g.kexp;
+            const double rk = pow(r, k);
+            const double e = zeff * exp(-alpha * rk) / r;
+            const double pref = (alpha * rk * k + 1.0) * e / (r * r);
+            if (o > a) er += e;
+            for (int d = 0; d < 3; ++d) gr[d] += dr[d] * pref;
+        }
+        if (disp_on && r2 <= 50.0 * 50.0 && r2 > 1e-14) {
+            const double r4r2 = 3.0 * EA.d3_r4r2 * EB.d3_r4r2;
+            const double r0 = g.a1 * sqrt(r4r2) + g.a2;
+            const double r0_2 = r0 * r0, r0_6 = r0_2 * r0_2 * r0_2, r0_8 = r0_6 * r0_2;
+            const double r4 = r2 * r2;
+            const double t6 = 1.0 / (r4 * r2 + r0_6), t8 = 1.0 / (r4 * r4 + r0_8);
+            const double d6 = -6.0 * r4 * t6 * t6, d8 = -8.0 * r4 * r2 * t8 * t8;
+            const double ed = g.s6 * t6 + g.s8 * r4r2 * t8;
+            const double gdv = g.s6 * d6 + g.s8 * r4r2 * d8;
+            // C6_AB and dC6_AB/dCN_A from the reference weights (dispersion.rs:1753-1787)
+            const double* gwb = gw + (size_t)o * kMaxRef;
+            int hi, lo, ia_is_hi;
+            if (za > zb) { hi = za; lo = zb; ia_is_hi = 1; }
+            else { hi = zb; lo = za; ia_is_hi = 0; }
+            const size_t pair = (size_t)lo + (size_t)hi * (hi - 1) / 2 - 1;
+            const double* tab = t.d3_c6 + pair * kMaxRef * kMaxRef;
+            double c6 = 0.0, dc6 = 0.0;
+            for (int ra = 0; ra < EA.d3_nref; ++ra)
+                for (int rb = 0; rb < EB.d3_nref; ++rb) {
+                    // table index [ref of lower Z][ref of higher Z]; for equal Z
+                    // the A reference is the slow index (dispersion.rs:3166-3172)
+                    double ref;
+                    if (za == zb) ref = tab[ra * kMaxRef + rb];
+                    else ref = ia_is_hi ? tab[rb * kMaxRef + ra] : tab[ra * kMaxRef + rb];
+                    c6 += gwa[ra] * gwb[rb] * ref;
+                    dc6 += dga[ra] * gwb[rb] * ref;
+                }
+            if (o > a) ed_sum -= c6 * ed;
+            for (int d = 0; d < 3; ++d) gd[d] += (-dr[d]) * (-c6 * gdv);
+            dcn -= dc6 * ed;
+        }
+    }
+    e_rep_atom[a] = er;
+    e_disp_atom[a] = ed_sum;
+    for (int d = 0; d < 3; ++d) {
+        g_rep[3 * a + d] = gr[d];
+        g_disp[3 * a + d] = gd[d];
+    }
+    dedcn_disp[a] = dcn;
+}
+
+// CN chain rule of a coordination number with cutoff cut:
+// g_A += sum_B (dE/dCN_A + dE/dCN_B) dCN/dr (R_A - R_B)/r (gradient.rs:475-489).
+__global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, double cut, double reps, double* g,
+                           int natoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const double rca = t.elements[b.atom_z[a]].covalent_radius;
+    const double c2 = cut * cut;
+    double acc[3] = {0, 0, 0};
+    for (int o = a0; o < a1; ++o) {
+        if (o == a) continue;
+        const double r2 = dist2(x, a, o);
+        if (r2 <= kPairEps2 || r2 > c2) continue;
+        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+        if (rc <= 1e-12) continue;
+        const double r = sqrt(r2);
+        if (r <= reps) continue;
+        double v, dv;
+        count_fn(r, rc, v, dv);
+        const double pref = (dedcn[a] + dedcn[o]) * dv / r;
+        for (int d = 0; d < 3; ++d) acc[d] += (x[3 * a + d] - x[3 * o + d]) * pref;
+    }
+    for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
+}
+
+// Halogen-bond correction (halogen.rs:64-160, :716-784); one thread per instance.
+__global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal) {
+    const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+    if (inst >= b.ninst) return;
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const double damp = t.glob.xbdamp, rads = t.glob.xbrad;
+    const double c2 = 20.0 * 20.0, EPS = 1e-18, ALP = 6.0, LJ2 = 6.0;
+    double e = 0.0;
+    for (int X = a0; X < a1; ++X) {
+        const int zx = b.atom_z[X];
+        if (!(zx == 17 || zx == 35 || zx == 53 || zx == 85)) continue;
+        int K = -1;
+        double best = INFINITY;
+        for (int j = a0; j < a1; ++j) {
+            if (j == X) continue;
+            const double r2 = dist2(x, X, j);
+            if (r2 <= kPairEps2 || r2 > c2) continue;
+            if (r2 < best) { best = r2; K = j; }
+        }
+        if (K < 0) continue;
+        const double cc = t.elements[zx].xb_strength;
+        if (cc == 0.0) continue;
+        for (int j = a0; j < a1; ++j) {
+            if (j == X) continue;
+            const double r2 = dist2(x, X, j);
+            if (r2 <= kPairEps2 || r2 > c2) continue;
+            const int zj = b.atom_z[j];
+            if (!(zj == 7 || zj == 8 || zj == 15 || zj == 16)) continue;
+            const double r0 = rads * (t.elements[zx].atomic_radius + t.elements[zj].atomic_radius);
+            double dxj[3], dxk[3], dkj[3];
+            for (int d = 0; d < 3; ++d) {
+                dxj[d] = x[3 * j + d] - x[3 * X + d];
+                dxk[d] = x[3 * K + d] - x[3 * X + d];
+                dkj[d] = x[3 * j + d] - x[3 * K + d];
+            }
+            const double d2jx = dxj[0] * dxj[0] + dxj[1] * dxj[1] + dxj[2] * dxj[2];
+            const double d2kx = dxk[0] * dxk[0] + dxk[1] * dxk[1] + dxk[2] * dxk[2];
+            const double d2jk = dkj[0] * dkj[0] + dkj[1] * dkj[1] + dkj[2] * dkj[2];
+            if (d2jx <= EPS || d2kx <= EPS) continue;
+            const double rjx = sqrt(d2jx) + EPS, rkx = sqrt(d2kx) + EPS;
+            const double xy = sqrt(d2kx * d2jx);
+            if (xy <= EPS) continue;
+            const double term = (d2kx + d2jx - d2jk) / xy;
+            const double ab = 0.5 - 0.25 * term;
+            const double aterm = pow(ab, ALP);
+            const double t13 = r0 / rjx;
+            const double t14lj = pow(t13, 12.0);
+            e += aterm * cc * (t14lj - damp * pow(t13, LJ2)) / (1.0 + t14lj);
+            const double t14 = pow(r0 / rjx, LJ2);
+            const double num = t14 * t14 - damp * t14, den = 1.0 + t14 * t14;
+            const double tlj = num / den;
+            double dt = 2.0 * LJ2 * num * t14 * t14 / (rjx * den * den);
+            dt += LJ2 * t14 * (damp - 2.0 * t14) / (rjx * den);
+            dt *= aterm * cc / rjx;
+            const double pf = -0.25 * ALP * pow(ab, ALP - 1.0) * cc * tlj;
+            const double djx = (2.0 / rkx - term / rjx) * pf / rjx;
+            const double dkx = (2.0 / rjx - term / rkx) * pf / rkx;
+            const double djk = 2.0 * pf / xy;
+            for (int d = 0; d < 3; ++d) {
+                g_hal[3 * j + d] += dxj[d] * dt + dxj[d] * djx - dkj[d] * djk;
+                g_hal[3 * X + d] -= dxj[d] * dt + dxj[d] * djx + dxk[d] * dkx;
+                g_hal[3 * K + d] += dxk[d] * dkx + dkj[d] * djk;
+            }
+        }
+    }
+    e_hal[inst] = e;
+}
+
+// Per-instance sums of per-atom values (fixed order).
+__global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out) {
+    __shared__ double lds[16];
+    const int inst = blockIdx.x;
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    double s = 0.0;
+    for (int a = a0 + threadIdx.x; a < a1; a += blockDim.x) s += per_atom[a];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) out[inst] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Electronic gradient, pair part (gradient.rs:333-462): overlap-derivative
+// (Pulay + shell potential) term, shell-polynomial term and the H0 dE/dCN
+// accumulators.  One 64-lane work-group per atom A; each (shell of A, shell of
+// B) item is evaluated with the reference's bra/ket assignment (bra = the
+// higher-index atom of the unique pair).
+__global__ void k_grad_pairs(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
+                             int cn_on, double cutoff, double* g_pair, double* dedcn_h) {
+    const int a = blockIdx.x;
+    const int inst = b.atom_inst[a];
+    const int s0 = b.sh0[inst], nshi = b.sh0[inst + 1] - s0;
+    const int ld = b.ld[inst];
+    const int za = b.atom_z[a];
+    const int nsha = t.elements[za].nshell;
+    const int sa0 = b.atom_sh0[a];
+    const double* x = b.atom_x;
+    const double* P = b.P[inst];
+    const double* W = b.W[inst];
+    const double c2 = cutoff * cutoff;
+    const double rA[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+    double gacc[3] = {0, 0, 0}, dacc = 0.0;
+    for (int item = threadIdx.x; item < nsha * nshi; item += blockDim.x) {
+        const int sa = sa0 + item / nshi;
+        const int sb = s0 + item % nshi;
+        const int o = b.sh_atom[sb];
+        if (o == a) continue;
+        const double r2 = dist2(x, a, o);
+        if (r2 <= kPairEps2 || r2 > c2 || r2 <= 1e-12) continue;
+        const int zb = b.atom_z[o];
+        const ShellType& A = t.shells[b.sh_type[sa]];
+        const ShellType& B = t.shells[b.sh_type[sb]];
+        const double rB[3] = {x[3 * o], x[3 * o + 1], x[3 * o + 2]};
+        const bool a_is_bra = a > o;  // unique pair (i<j): bra is j
+        double blk[25], dbra[75], dket[75];
+        if (a_is_bra) shell_pair<true>(A, B, rA, rB, blk, dbra, dket);
+        else shell_pair<true>(B, A, rB, rA, blk, dbra, dket);
+        const double radsum = t.elements[za].atomic_radius + t.elements[zb].atomic_radius;
+        const double rr = sqrt(sqrt(r2) / radsum);
+        const double fa = 1.0 + A.poly * rr, fb = 1.0 + B.poly * rr;
+        const double hs = hscale(t, za, A, zb, B) * (fa * fb);
+        const double dse_a = cn_on ? -A.kcn : 0.0, dse_b = cn_on ? -B.kcn : 0.0;
+        const double sea = A.hdiag + dse_a * cn[a], seb = B.hdiag + dse_b * cn[o];
+        const double hij = 0.5 * (sea + seb) * hs;
+        // log-derivative of the shell polynomial (gradient.rs:1157-1175), rvec = bra - ket
+        double rvec[3];
+        for (int d = 0; d < 3; ++d) rvec[d] = a_is_bra ? rA[d] - rB[d] : rB[d] - rA[d];
+        double dlp[3] = {0, 0, 0};
+        {
+            // shell order (mu = bra shell, nu = ket shell) as in the reference
+            const double pm = a_is_bra ? A.poly : B.poly, pn = a_is_bra ? B.poly : A.poly;
+            const double fi = 1.0 + pm * rr, fj = 1.0 + pn * rr;
+            const double pl = fi * fj;
+            if (fabs(pl) > 1e-18) {
+                const double dp = (fi * pn + fj * pm) * 0.5 * rr / r2;
+                for (int d = 0; d < 3; ++d) dlp[d] = rvec[d] * (dp / pl);
+            }
+        }
+        const int mua0 = b.sh_ao[sa], nub0 = b.sh_ao[sb];
+        const double va = scc_on ? vsh[sa] : 0.0, vb = scc_on ? vsh[sb] : 0.0;
+        const int na = 2 * A.l + 1, nb = 2 * B.l + 1;
+        for (int i = 0; i < na; ++i)
+            for (int j = 0; j < nb; ++j) {
+                const int mu = mua0 + i, nu = nub0 + j;  // mu on A, nu on B
+                const double p = P[(size_t)nu * ld + mu];
+                const double w = W[(size_t)nu * ld + mu];
+                const double oc = p * (2.0 * hij - (va + vb)) - 2.0 * w;
+                const int k = a_is_bra ? (i * 5 + j) : (j * 5 + i);
+                const double s = blk[k];
+                const double hp = p * hij;
+                for (int d = 0; d < 3; ++d) {
+                    const double pg = dlp[d] * (2.0 * hp * s);
+                    if (a_is_bra) gacc[d] += dbra[k * 3 + d] * oc + pg;
+                    else gacc[d] += dket[k * 3 + d] * oc - pg;
+                }
+                if (cn_on) dacc += dse_a * hs * p * s;
+            }
+    }
+    // wave reduction (blockDim == 64)
+    for (int d = 0; d < 3; ++d) gacc[d] = wave_sum(gacc[d]);
+    dacc = wave_sum(dacc);
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) g_pair[3 * a + d] = gacc[d];
+        dedcn_h[a] = dacc;
+    }
+}
+
+// Electronic gradient, per-atom part: diagonal dE/dCN (gradient.rs:465-469) and
+// the isotropic Coulomb term (gradient.rs:493-516).
+__global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
+                             double* dedcn_h, double* g_el, int natoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int inst = b.atom_inst[a];
+    const int ld = b.ld[inst];
+    const int za = b.atom_z[a];
+    const int nsa = t.elements[za].nshell;
+    const int sa0 = b.atom_sh0[a];
+    const double* P = b.P[inst];
+    if (cn_on) {
+        double d = dedcn_h[a];
+        for (int k = 0; k < nsa; ++k) {
+            const int sa = sa0 + k;
+            const ShellType& A = t.shells[b.sh_type[sa]];
+            for (int m = 0; m < A.nao; ++m) {
+                const int mu = b.sh_ao[sa] + m;
+                d += (-A.kcn) * P[(size_t)mu * ld + mu];
+            }
+        }
+        dedcn_h[a] = d;
+    }
+    if (!scc_on) return;
+    const int s0 = b.sh0[inst], s1 = b.sh0[inst + 1];
+    const double* x = b.atom_x;
+    double acc[3] = {0, 0, 0};
+    for (int k = 0; k < nsa; ++k) {
+        const int sa = sa0 + k;
+        const double ha = t.shells[b.sh_type[sa]].hardness;
+        for (int sb = s0; sb < s1; ++sb) {
+            const int o = b.sh_atom[sb];
+            if (o == a) continue;
+            double rv[3];
+            for (int d = 0; d < 3; ++d) rv[d] = x[3 * a + d] - x[3 * o + d];
+            const double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+            if (r <= 1e-12) continue;
+            const double hb = t.shells[b.sh_type[sb]].hardness;
+            const double h = 2.0 / (1.0 / ha + 1.0 / hb);
+            const double den = r * r + 1.0 / (h * h);
+            const double pref = -1.0 / (den * sqrt(den)) * (q[sa] * q[sb]);
+            for (int d = 0; d < 3; ++d) acc[d] += rv[d] * pref;
+        }
+    }
+    for (int d = 0; d < 3; ++d) g_el[3 * a + d] += acc[d];
+}
+
+
+}  // namespace gfn1

@@ -1,0 +1,57 @@
+// Kernel entry points of kernels.hip (launched by the batch driver).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace gfn1 {
+
+// SCC controls shared by every instance of a batch: the convergence thresholds
+// and the robust-fallback ladder (electronic.rs:1413-1419).
+struct SccParams {
+    double energy_tolerance;
+    double charge_tolerance;
+    int32_t broyden_memory;   // BroydenMixer memory (history modulus)
+    int32_t broyden_slots;    // allocated history slots per instance
+    int32_t rung_broyden[4];
+    double rung_mixing[4];
+    double rung_level_shift[4];
+    double rung_temperature[4];
+    int32_t rung_max_iter[4];
+};
+
+__global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
+                               double* gw, double* dgw, int natoms);
+__global__ void k_overlap_h0(DeviceBatch b, DeviceTables t, const double* cn, double cutoff);
+__global__ void k_pad_diagonal(DeviceBatch b);
+__global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, double eig_tol, int* fail);
+__global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
+                                  double* vt);
+__global__ void k_fock(DeviceBatch b, const int* act, const double* vt);
+__global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift, double* const* SPS);
+__global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int eig_stride, double* eps_all,
+                        double* occ_all, SccState* st);
+__global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
+                                const double* eps_all, const double* occ_all, int k, int weighted);
+__global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp);
+__global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH);
+__global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
+                       double* qout, SccState* st);
+__global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccParams prm, const double* qout,
+                             const double* vg_out, double* qin, const double* q0, SccState* st, double* br_df,
+                             double* br_u, double* br_omega, double* br_qlast, double* br_dqlast,
+                             const int64_t* br_off, double* beta_scratch, int beta_ld);
+__global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw, const double* cn_d3, int disp_on,
+                                 double* e_rep_atom, double* e_disp_atom, double* g_rep, double* g_disp,
+                                 double* dedcn_disp, int natoms);
+__global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, double cut, double reps, double* g,
+                           int natoms);
+__global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal);
+__global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
+__global__ void k_grad_pairs(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
+                             int cn_on, double cutoff, double* g_pair, double* dedcn_h);
+__global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
+                             double* dedcn_h, double* g_el, int natoms);
+
+}  // namespace gfn1

@@ -1,0 +1,387 @@
+"""Reference-shaped Python API over the C ABI.
+
+Mirrors the reference's run_electronic / analytic_gradient entry points
+(src/electronic.rs:800, src/gradient.rs:56), their option and result structs
+(electronic.rs:157-452, :600-712; gradient.rs:21-54) and the Python calculator
+(src/python.rs:115-381, :2845-2935).
+"""
+import ctypes
+from dataclasses import dataclass, field, fields
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _lib
+
+ANGSTROM_TO_BOHR = 1.8897261246257702        # system.rs:8
+HARTREE_TO_EV = 27.211386245988              # constants.rs:17
+FORCE_AU_TO_EV_PER_ANGSTROM = HARTREE_TO_EV * ANGSTROM_TO_BOHR
+
+
+class Gfn1Parameters:
+    """Loaded parameter set (params.rs:165-230); one device copy per GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def resolve(cls, spec: Optional[str] = None):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        err = _lib.Gfn1Error()
+        _lib.check(lib.gfn1_params_load(spec.encode() if spec else None, ctypes.byref(h), ctypes.byref(err)), err)
+        return cls(h)
+
+    @classmethod
+    def builtin(cls):
+        return cls.resolve("builtin")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.gfn1_params_free(self._h)
+            self._h = None
+
+
+@dataclass
+class ElectronicOptions:
+    """In-scope ElectronicOptions with the reference defaults (electronic.rs:395-452)."""
+    charge: Optional[float] = None
+    max_scc: int = 250
+    energy_tolerance: float = 1.0e-6
+    charge_tolerance: float = 2.0e-5
+    mixing: float = 0.4
+    scc_broyden: bool = True
+    scc_broyden_size: int = 250
+    level_shift: float = 0.0
+    nprim: int = 0
+    eigen_tolerance: float = 1.0e-12
+    electronic_temperature: float = 300.0
+    enable_dispersion: bool = True
+    integral_cutoff: float = 30.0
+    coordination_cutoff: float = 30.0
+    enable_cn_hamiltonian: bool = True
+    scc_initial_shell_charges: Optional[np.ndarray] = None
+
+    def to_c(self):
+        o = _lib.ElectronicOptions()
+        _lib.load().gfn1_electronic_options_default(ctypes.byref(o))
+        o.has_charge = 0 if self.charge is None else 1
+        o.charge = 0.0 if self.charge is None else float(self.charge)
+        for name in ("max_scc", "scc_broyden_size", "nprim"):
+            setattr(o, name, int(getattr(self, name)))
+        for name in ("energy_tolerance", "charge_tolerance", "mixing", "level_shift", "eigen_tolerance",
+                     "electronic_temperature", "integral_cutoff", "coordination_cutoff"):
+            setattr(o, name, float(getattr(self, name)))
+        o.scc_broyden = int(bool(self.scc_broyden))
+        o.enable_dispersion = int(bool(self.enable_dispersion))
+        o.enable_cn_hamiltonian = int(bool(self.enable_cn_hamiltonian))
+        keep = None
+        if self.scc_initial_shell_charges is not None:
+            keep = np.ascontiguousarray(self.scc_initial_shell_charges, dtype=np.float64)
+            o.scc_initial_shell_charges = keep.ctypes.data_as(_lib._DP)
+            o.scc_initial_shell_charges_len = len(keep)
+        return o, keep
+
+
+@dataclass
+class AnalyticGradientOptions:
+    """AnalyticGradientOptions (gradient.rs:21-41)."""
+    electronic: ElectronicOptions = field(default_factory=ElectronicOptions)
+    include_repulsion: bool = True
+    include_dispersion: bool = True
+    include_hamiltonian: bool = True
+    include_scc: bool = True
+    include_halogen: bool = True
+
+    def to_c(self):
+        e, keep = self.electronic.to_c()
+        g = _lib.GradientOptions()
+        g.electronic = e
+        for name in ("include_repulsion", "include_dispersion", "include_hamiltonian", "include_scc",
+                     "include_halogen"):
+            setattr(g, name, int(bool(getattr(self, name))))
+        return g, keep
+
+
+@dataclass
+class EnergyTerms:
+    total_free: float = 0.0
+    total_internal: float = 0.0
+    repulsion: float = 0.0
+    electronic: float = 0.0
+    isotropic_scc: float = 0.0
+    third_order: float = 0.0
+    higher_order: float = 0.0
+    dispersion: float = 0.0
+    halogen: float = 0.0
+    multipole: float = 0.0
+    exchange: float = 0.0
+    spin_polarization: float = 0.0
+    plus_u: float = 0.0
+    external_field: float = 0.0
+    electronic_entropy: float = 0.0
+
+    def named_values(self):
+        return [(f.name, getattr(self, f.name)) for f in fields(self)]
+
+
+@dataclass
+class ElectronicResult:
+    """ElectronicResult (electronic.rs:600-654); moment integrals are not produced."""
+    overlap: np.ndarray
+    h0: np.ndarray
+    fock: np.ndarray
+    density: np.ndarray
+    energy_weighted_density: np.ndarray
+    orbital_energies: np.ndarray
+    occupations: np.ndarray
+    electronic_temperature: float
+    fermi_level: float
+    charge_order: int
+    shell_charges: np.ndarray
+    atomic_charges: np.ndarray
+    shell_scc_potential: np.ndarray
+    coordination_numbers: np.ndarray
+    terms: EnergyTerms
+    dipole: np.ndarray
+    nelec: float
+    iterations: int
+    converged: bool
+
+    @property
+    def total_free(self):
+        return self.terms.total_free
+
+    @property
+    def total_internal(self):
+        return self.terms.total_internal
+
+    def energy_terms(self):
+        return self.terms
+
+
+@dataclass
+class AnalyticGradientResult:
+    """AnalyticGradientResult (gradient.rs:44-54)."""
+    electronic_result: ElectronicResult
+    total_energy: float
+    gradient: np.ndarray
+    forces: np.ndarray
+    electronic_gradient: np.ndarray
+    repulsion_gradient: np.ndarray
+    dispersion_gradient: np.ndarray
+    halogen_gradient: np.ndarray
+    max_gradient: float
+
+
+def _params(params):
+    return params if params is not None else Gfn1Parameters.resolve()
+
+
+def _read_electronic(lib, h):
+    nat, nsh, nao = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib.gfn1_electronic_result_sizes(h, ctypes.byref(nat), ctypes.byref(nsh), ctypes.byref(nao))
+    n, ns, na = nao.value, nsh.value, nat.value
+
+    def mat(k):
+        p = lib.gfn1_electronic_result_matrix(h, k)
+        return np.ctypeslib.as_array(p, shape=(n * n,)).reshape(n, n).copy() if n else np.zeros((0, 0))
+
+    def vec(k, length):
+        p = lib.gfn1_electronic_result_vector(h, k)
+        return np.ctypeslib.as_array(p, shape=(length,)).copy() if length else np.zeros(0)
+
+    e = (ctypes.c_double * 15)()
+    lib.gfn1_electronic_result_energy_terms(h, e)
+    terms = EnergyTerms(*list(e))
+    sc = lambda k: lib.gfn1_electronic_result_scalar(h, k)  # noqa: E731
+    return ElectronicResult(
+        overlap=mat(0), h0=mat(1), fock=mat(2), density=mat(3), energy_weighted_density=mat(4),
+        orbital_energies=vec(0, n), occupations=vec(1, n), electronic_temperature=sc(0), fermi_level=sc(1),
+        charge_order=int(sc(5)), shell_charges=vec(2, ns), atomic_charges=vec(3, na),
+        shell_scc_potential=vec(4, ns), coordination_numbers=vec(5, na), terms=terms, dipole=vec(6, 3),
+        nelec=sc(2), iterations=int(sc(3)), converged=bool(sc(4)))
+
+
+def run_electronic(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
+                   options: Optional[ElectronicOptions] = None, charge: float = 0.0) -> ElectronicResult:
+    """run_electronic (electronic.rs:800): positions in bohr."""
+    lib = _lib.load()
+    p = _params(params)
+    opts = options or ElectronicOptions()
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    oc, keep_q = opts.to_c()
+    h = ctypes.c_void_p()
+    err = _lib.Gfn1Error()
+    rc = lib.gfn1_run_electronic(ctypes.byref(sysc), p._h, ctypes.byref(oc), ctypes.byref(h), ctypes.byref(err))
+    _lib.check(rc, err)
+    try:
+        return _read_electronic(lib, h)
+    finally:
+        lib.gfn1_electronic_result_free(h)
+
+
+def analytic_gradient(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
+                      options: Optional[AnalyticGradientOptions] = None,
+                      charge: float = 0.0) -> AnalyticGradientResult:
+    """analytic_gradient (gradient.rs:56): gradient in Eh/bohr."""
+    lib = _lib.load()
+    p = _params(params)
+    opts = options or AnalyticGradientOptions()
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    oc, keep_q = opts.to_c()
+    h = ctypes.c_void_p()
+    err = _lib.Gfn1Error()
+    rc = lib.gfn1_analytic_gradient(ctypes.byref(sysc), p._h, ctypes.byref(oc), ctypes.byref(h), ctypes.byref(err))
+    _lib.check(rc, err)
+    try:
+        el = _read_electronic(lib, lib.gfn1_gradient_result_electronic(h))
+        nat = len(np.asarray(numbers))
+
+        def g(k):
+            ptr = lib.gfn1_gradient_result_field(h, k)
+            return np.ctypeslib.as_array(ptr, shape=(3 * nat,)).reshape(nat, 3).copy() if nat else np.zeros((0, 3))
+
+        return AnalyticGradientResult(
+            electronic_result=el, total_energy=lib.gfn1_gradient_result_total_energy(h), gradient=g(0),
+            forces=g(1), electronic_gradient=g(2), repulsion_gradient=g(3), dispersion_gradient=g(4),
+            halogen_gradient=g(5), max_gradient=lib.gfn1_gradient_result_max_gradient(h))
+    finally:
+        lib.gfn1_gradient_result_free(h)
+
+
+@dataclass
+class CalculationResult:
+    """python.rs CalculationResult (:2845-2935)."""
+    energy_hartree: float
+    gradient_hartree_per_bohr: Optional[np.ndarray]
+    charges: np.ndarray
+    dipole: np.ndarray
+    iterations: int
+    converged: bool
+    _terms: EnergyTerms
+
+    @property
+    def energy_ev(self):
+        return self.energy_hartree * HARTREE_TO_EV
+
+    @property
+    def forces_ev_per_angstrom(self):
+        if self.gradient_hartree_per_bohr is None:
+            return None
+        return -self.gradient_hartree_per_bohr * FORCE_AU_TO_EV_PER_ANGSTROM
+
+    def energy_terms_hartree(self) -> Dict[str, float]:
+        return dict(self._terms.named_values())
+
+
+class Gfn1NativeCalculator:
+    """Mirror of the PyO3 Gfn1NativeCalculator (python.rs:115-381)."""
+
+    def __init__(self, param_path=None, charge=0.0, multiplicity=None, max_scc=250, energy_tolerance=1e-6,
+                 charge_tolerance=2e-5, mixing=0.4, scc_broyden=True, scc_broyden_size=250,
+                 electronic_temperature=300.0, nprim=0, eigen_tolerance=1e-12, enable_dispersion=True,
+                 enable_cn_hamiltonian=True, level_shift=0.0):
+        if multiplicity not in (None, 1):
+            raise ValueError("only closed-shell (multiplicity 1) calculations are supported")
+        self.params = Gfn1Parameters.resolve(param_path)
+        self.charge = charge
+        self.options = ElectronicOptions(
+            max_scc=max_scc, energy_tolerance=energy_tolerance, charge_tolerance=charge_tolerance, mixing=mixing,
+            scc_broyden=scc_broyden, scc_broyden_size=scc_broyden_size,
+            electronic_temperature=electronic_temperature, nprim=nprim, eigen_tolerance=eigen_tolerance,
+            enable_dispersion=enable_dispersion, enable_cn_hamiltonian=enable_cn_hamiltonian,
+            level_shift=level_shift)
+
+    def calculate(self, numbers, positions, unit="angstrom", compute_gradient=False) -> CalculationResult:
+        unit_l = unit.lower()
+        if unit_l in ("angstrom", "ang", "a"):
+            scale = ANGSTROM_TO_BOHR
+        elif unit_l in ("bohr", "au", "a0"):
+            scale = 1.0
+        else:
+            raise ValueError("unit must be 'angstrom' or 'bohr'")
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * scale
+        if compute_gradient:
+            r = analytic_gradient(numbers, x, self.params, AnalyticGradientOptions(electronic=self.options),
+                                  charge=self.charge)
+            el = r.electronic_result
+            return CalculationResult(r.total_energy, r.gradient, el.atomic_charges, el.dipole, el.iterations,
+                                     el.converged, el.terms)
+        el = run_electronic(numbers, x, self.params, self.options, charge=self.charge)
+        return CalculationResult(el.total_free, None, el.atomic_charges, el.dipole, el.iterations, el.converged,
+                                 el.terms)
+
+
+class BatchEngine:
+    """Energy + gradient of many independent systems in one device pass."""
+
+    def __init__(self, systems, params: Optional[Gfn1Parameters] = None,
+                 options: Optional[AnalyticGradientOptions] = None, eigensolver=0, keep_matrices=False):
+        lib = _lib.load()
+        self._lib = lib
+        self.params = _params(params)
+        opts = options or AnalyticGradientOptions()
+        bo = _lib.BatchOptions()
+        lib.gfn1_batch_options_default(ctypes.byref(bo))
+        g, self._keep_q = opts.to_c()
+        bo.gradient = g
+        bo.eigensolver = int(eigensolver)
+        bo.keep_matrices = int(bool(keep_matrices))
+        arr = (_lib.System * len(systems))()
+        self._keep = []
+        self.natoms = []
+        for i, s in enumerate(systems):
+            numbers, pos = s[0], s[1]
+            charge = s[2] if len(s) > 2 else 0.0
+            sc, keep = _lib.make_system(numbers, pos, charge)
+            arr[i] = sc
+            self._keep.append(keep)
+            self.natoms.append(len(keep[0]))
+        self._h = ctypes.c_void_p()
+        err = _lib.Gfn1Error()
+        _lib.check(lib.gfn1_batch_create(len(systems), arr, self.params._h, ctypes.byref(bo), ctypes.byref(self._h),
+                                         ctypes.byref(err)), err)
+        self.n = len(systems)
+
+    def run(self):
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_run(self._h, ctypes.byref(err)), err)
+        return self
+
+    def status(self, i):
+        return self._lib.gfn1_batch_status(self._h, i)
+
+    def energy(self, i):
+        return self._lib.gfn1_batch_energy(self._h, i)
+
+    def energies(self):
+        return np.array([self.energy(i) for i in range(self.n)])
+
+    def energy_terms(self, i):
+        e = (ctypes.c_double * 15)()
+        self._lib.gfn1_batch_energy_terms(self._h, i, e)
+        return dict(zip(_lib.ENERGY_NAMES, list(e)))
+
+    def iterations(self, i):
+        return self._lib.gfn1_batch_iterations(self._h, i)
+
+    def gradient(self, i):
+        out = np.zeros((self.natoms[i], 3))
+        self._lib.gfn1_batch_gradient(self._h, i, out.ctypes.data_as(_lib._DP))
+        return out
+
+    def shell_charges(self, i, nsh):
+        out = np.zeros(nsh)
+        self._lib.gfn1_batch_shell_charges(self._h, i, out.ctypes.data_as(_lib._DP))
+        return out
+
+    def timings(self):
+        t = (ctypes.c_double * 4)()
+        self._lib.gfn1_batch_timings(self._h, t)
+        return dict(zip(("setup_ms", "scc_ms", "gradient_ms", "total_ms"), list(t)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.gfn1_batch_free(self._h)
+            self._h = None

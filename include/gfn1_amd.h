@@ -1,0 +1,214 @@
+/*
+ * gfn1_amd.h -- C ABI of the MI355X GFN1-xTB SCC energy + analytic gradient engine.
+ *
+ * This is the drop-in boundary for the reference's non-periodic SCC path.  Each
+ * entry point names the reference interface it replaces (paths relative to the
+ * reference repository root).  All coordinates are in bohr, energies in Hartree,
+ * gradients in Hartree/bohr.  Matrices are dense n x n FP64; they are symmetric,
+ * so row- and column-major layouts coincide.
+ *
+ * Ownership: results own their buffers; free them with the matching *_free.
+ * Accessor pointers stay valid until the result is freed.  Thread safety: every
+ * call is re-entrant; concurrent calls use separate device streams.
+ */
+#ifndef GFN1_AMD_H
+#define GFN1_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: one per Gfn1Error variant (src/error.rs:8-17), plus device errors. */
+enum gfn1_status {
+    GFN1_OK = 0,
+    GFN1_ERR_IO = 1,
+    GFN1_ERR_PARSE = 2,
+    GFN1_ERR_INVALID_INPUT = 3,
+    GFN1_ERR_MISSING_ELEMENT = 4,
+    GFN1_ERR_MISSING_GLOBAL = 5,
+    GFN1_ERR_LINEAR_ALGEBRA = 6,
+    GFN1_ERR_SINGULAR_CELL = 7,
+    GFN1_ERR_SCC_NOT_CONVERGED = 8,
+    GFN1_ERR_DEVICE = 100
+};
+
+typedef struct gfn1_error {
+    int code;
+    int iterations;   /* SccNotConverged{iterations, rms} payload */
+    double rms;
+    char message[256];
+} gfn1_error;
+
+typedef struct gfn1_params gfn1_params;
+
+/* Gfn1Parameters::resolve (src/params.rs:204-230).  spec: NULL or "" ->
+ * $GFN1_XTB_PARAM, else the bundled GFN1-xTB file; "builtin" -> bundled file;
+ * anything else is a parameter-file path. */
+int gfn1_params_load(const char* spec, gfn1_params** out, gfn1_error* err);
+void gfn1_params_free(gfn1_params* params);
+
+/* PeriodicSystem without a lattice (src/system.rs:17-21). */
+typedef struct gfn1_system {
+    int32_t nat;
+    const int32_t* numbers;   /* atomic numbers, nat */
+    const double* positions;  /* bohr, 3*nat, atom-major xyz */
+    double charge;
+} gfn1_system;
+
+/* In-scope subset of ElectronicOptions (src/electronic.rs:157-393) with the same
+ * defaults (:395-452), plus HamiltonianOptions (src/hamiltonian.rs:16-33). */
+typedef struct gfn1_electronic_options {
+    int32_t has_charge;           /* Option<f64> charge: 0 -> use system charge */
+    double charge;
+    int32_t max_scc;              /* 250 */
+    double energy_tolerance;      /* 1e-6 */
+    double charge_tolerance;      /* 2e-5 */
+    double mixing;                /* 0.4 */
+    int32_t scc_broyden;          /* 1 */
+    int32_t scc_broyden_size;     /* 250 */
+    double level_shift;           /* 0 */
+    int32_t nprim;                /* 0 = GFN1 primitive-count rule; only 0 is supported */
+    double eigen_tolerance;       /* 1e-12 */
+    double electronic_temperature;/* 300 K */
+    int32_t enable_dispersion;    /* 1 */
+    double integral_cutoff;       /* 30 bohr */
+    double coordination_cutoff;   /* 30 bohr */
+    int32_t enable_cn_hamiltonian;/* 1 */
+    const double* scc_initial_shell_charges; /* nullable; used when its length is nsh */
+    int32_t scc_initial_shell_charges_len;
+} gfn1_electronic_options;
+
+void gfn1_electronic_options_default(gfn1_electronic_options* options);
+
+/* AnalyticGradientOptions (src/gradient.rs:21-41). */
+typedef struct gfn1_gradient_options {
+    gfn1_electronic_options electronic;
+    int32_t include_repulsion;
+    int32_t include_dispersion;
+    int32_t include_hamiltonian;
+    int32_t include_scc;
+    int32_t include_halogen;
+} gfn1_gradient_options;
+
+void gfn1_gradient_options_default(gfn1_gradient_options* options);
+
+typedef struct gfn1_electronic_result gfn1_electronic_result;
+typedef struct gfn1_gradient_result gfn1_gradient_result;
+
+/* run_electronic (src/electronic.rs:800-847). */
+int gfn1_run_electronic(const gfn1_system* system, const gfn1_params* params,
+                        const gfn1_electronic_options* options,
+                        gfn1_electronic_result** out, gfn1_error* err);
+
+/* analytic_gradient (src/gradient.rs:56-69): run_electronic followed by
+ * analytic_gradient_from_result (src/gradient.rs:122-242). */
+int gfn1_analytic_gradient(const gfn1_system* system, const gfn1_params* params,
+                           const gfn1_gradient_options* options,
+                           gfn1_gradient_result** out, gfn1_error* err);
+
+void gfn1_electronic_result_free(gfn1_electronic_result* result);
+void gfn1_gradient_result_free(gfn1_gradient_result* result);
+
+/* --- ElectronicResult accessors (src/electronic.rs:600-654) --- */
+void gfn1_electronic_result_sizes(const gfn1_electronic_result* r, int32_t* nat, int32_t* nsh, int32_t* nao);
+
+/* EnergyTerms::named_values order (src/electronic.rs:676-694):
+ * total_free, total_internal, repulsion, electronic, isotropic_scc, third_order,
+ * higher_order, dispersion, halogen, multipole, exchange, spin_polarization,
+ * plus_u, external_field, electronic_entropy. */
+void gfn1_electronic_result_energy_terms(const gfn1_electronic_result* r, double out[15]);
+
+enum gfn1_matrix_field {
+    GFN1_MAT_OVERLAP = 0,
+    GFN1_MAT_H0 = 1,
+    GFN1_MAT_FOCK = 2,
+    GFN1_MAT_DENSITY = 3,
+    GFN1_MAT_ENERGY_WEIGHTED_DENSITY = 4
+};
+/* nao x nao; NULL for an unknown field. */
+const double* gfn1_electronic_result_matrix(const gfn1_electronic_result* r, int field);
+
+enum gfn1_vector_field {
+    GFN1_VEC_ORBITAL_ENERGIES = 0,   /* nao */
+    GFN1_VEC_OCCUPATIONS = 1,        /* nao */
+    GFN1_VEC_SHELL_CHARGES = 2,      /* nsh */
+    GFN1_VEC_ATOMIC_CHARGES = 3,     /* nat */
+    GFN1_VEC_SHELL_SCC_POTENTIAL = 4,/* nsh */
+    GFN1_VEC_COORDINATION_NUMBERS = 5, /* nat */
+    GFN1_VEC_DIPOLE = 6              /* 3 */
+};
+const double* gfn1_electronic_result_vector(const gfn1_electronic_result* r, int field);
+
+enum gfn1_scalar_field {
+    GFN1_SCALAR_ELECTRONIC_TEMPERATURE = 0,
+    GFN1_SCALAR_FERMI_LEVEL = 1,
+    GFN1_SCALAR_NELEC = 2,
+    GFN1_SCALAR_ITERATIONS = 3,
+    GFN1_SCALAR_CONVERGED = 4,
+    GFN1_SCALAR_CHARGE_ORDER = 5
+};
+double gfn1_electronic_result_scalar(const gfn1_electronic_result* r, int field);
+
+/* --- AnalyticGradientResult accessors (src/gradient.rs:44-54) --- */
+const gfn1_electronic_result* gfn1_gradient_result_electronic(const gfn1_gradient_result* r);
+double gfn1_gradient_result_total_energy(const gfn1_gradient_result* r);
+double gfn1_gradient_result_max_gradient(const gfn1_gradient_result* r);
+enum gfn1_gradient_field {
+    GFN1_GRAD_TOTAL = 0,
+    GFN1_GRAD_FORCES = 1,
+    GFN1_GRAD_ELECTRONIC = 2,
+    GFN1_GRAD_REPULSION = 3,
+    GFN1_GRAD_DISPERSION = 4,
+    GFN1_GRAD_HALOGEN = 5
+};
+/* nat x 3 (atom-major xyz). */
+const double* gfn1_gradient_result_field(const gfn1_gradient_result* r, int field);
+
+/* --- Batched energy + gradient of many independent systems ---
+ * No reference counterpart as a single call: the reference runs analytic_gradient
+ * once per system (src/gradient.rs:56).  Each instance follows exactly that
+ * path; instances run together on the device. */
+enum gfn1_precision { GFN1_PRECISION_FP64 = 0, GFN1_PRECISION_MIXED = 1 };
+enum gfn1_guess { GFN1_GUESS_ZERO = 0, GFN1_GUESS_LEARNED = 1 };
+
+typedef struct gfn1_batch_options {
+    gfn1_gradient_options gradient;
+    int32_t precision;       /* gfn1_precision */
+    int32_t guess;           /* gfn1_guess */
+    int32_t eigensolver;     /* 0 auto, 1 rocSOLVER syevd, 2 rocSOLVER syevj */
+    int32_t keep_matrices;   /* copy S/H0/F/P/W back to the host per instance */
+} gfn1_batch_options;
+
+void gfn1_batch_options_default(gfn1_batch_options* options);
+
+typedef struct gfn1_batch gfn1_batch;
+
+/* Upload the systems and build the per-instance index tables on the device. */
+int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_params* params,
+                      const gfn1_batch_options* options, gfn1_batch** out, gfn1_error* err);
+/* Run the whole energy + gradient pass on device-resident inputs.  Blocks until
+ * the results are complete on the device. */
+int gfn1_batch_run(gfn1_batch* batch, gfn1_error* err);
+/* Per-instance results after gfn1_batch_run. */
+int gfn1_batch_status(const gfn1_batch* batch, int32_t inst);           /* gfn1_status of the instance */
+double gfn1_batch_energy(const gfn1_batch* batch, int32_t inst);        /* total_free */
+void gfn1_batch_energy_terms(const gfn1_batch* batch, int32_t inst, double out[15]);
+int32_t gfn1_batch_iterations(const gfn1_batch* batch, int32_t inst);
+/* Copies nat*3 doubles into out. */
+int gfn1_batch_gradient(const gfn1_batch* batch, int32_t inst, double* out);
+/* Copies nsh doubles (converged shell charges) into out. */
+int gfn1_batch_shell_charges(const gfn1_batch* batch, int32_t inst, double* out);
+/* Device time of the last gfn1_batch_run phases, milliseconds: [setup, scc, gradient, total]. */
+void gfn1_batch_timings(const gfn1_batch* batch, double out[4]);
+void gfn1_batch_free(gfn1_batch* batch);
+
+/* Library version string. */
+const char* gfn1_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFN1_AMD_H */

@@ -1,0 +1,108 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Bar (FP64 path): |dE| <= 1e-8 Eh, max |dg| <= 1e-7 Eh/bohr, at tight SCC
+tolerances so that both sides sit on the same fixed point.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, NativeError, analytic_gradient
+from gfn1_amd import geometry
+
+from test_oracle import MOLECULES, mol
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = dict(energy_tolerance=1e-11, charge_tolerance=1e-9)
+E_TOL = 1e-8
+G_TOL = 1e-7
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", sorted(MOLECULES))
+def test_single_molecule_matches_oracle(name):
+    note("case single %s" % name)
+    z, x = mol(name)
+    ref = oracle.run(z, x, want_matrices=True, **TIGHT)
+    assert ref["status"] == 0
+    res = analytic_gradient(z, x, options=AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT)))
+    el = res.electronic_result
+    assert el.converged
+    assert abs(res.total_energy - ref["energy"]) <= E_TOL
+    for k, v in ref["energies"].items():
+        assert abs(getattr(el.terms, k) - v) <= E_TOL, k
+    assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
+    assert np.abs(res.dispersion_gradient - ref["dispersion_gradient"]).max() <= 1e-12
+    assert np.abs(res.repulsion_gradient - ref["repulsion_gradient"]).max() <= 1e-12
+    assert np.abs(res.halogen_gradient - ref["halogen_gradient"]).max() <= 1e-12
+    assert np.abs(el.overlap - ref["overlap"]).max() <= 1e-13
+    assert np.abs(el.h0 - ref["h0"]).max() <= 1e-13
+    assert np.abs(el.density - ref["density"]).max() <= 1e-7
+    assert np.abs(el.shell_charges - ref["shell_charges"]).max() <= 1e-8
+    assert np.abs(el.coordination_numbers - ref["coordination_numbers"]).max() <= 1e-13
+    assert np.abs(el.orbital_energies - ref["orbital_energies"]).max() <= 1e-8
+    assert abs(el.nelec - ref["nelec"]) < 1e-12
+
+
+def test_default_tolerances_match_iteration_count():
+    note("case default-tolerance water")
+    z, x = mol("water")
+    ref = oracle.run(z, x)
+    res = analytic_gradient(z, x)
+    assert res.electronic_result.iterations == ref["iterations"]
+    assert abs(res.total_energy - ref["energy"]) <= 1e-9
+
+
+def test_batch_of_mixed_clusters_matches_oracle():
+    note("case batch mixed clusters")
+    systems = geometry.mixed_batch(12, first_seed=1000, kmax=12)
+    eng = BatchEngine(systems, options=AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))).run()
+    for i, (z, x) in enumerate(systems):
+        ref = oracle.run(z, x, **TIGHT)
+        assert eng.status(i) == 0
+        assert abs(eng.energy(i) - ref["energy"]) <= E_TOL, i
+        assert np.abs(eng.gradient(i) - ref["gradient"]).max() <= G_TOL, i
+
+
+def test_water48_matches_oracle():
+    note("case water48")
+    z, x = geometry.water_cluster(48, 11.3, seed=0)
+    ref = oracle.run(z, x, **TIGHT)
+    res = analytic_gradient(z, x, options=AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT)))
+    assert abs(res.total_energy - ref["energy"]) <= E_TOL
+    assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
+
+
+def test_fallback_ladder_matches_oracle():
+    note("case fallback ladder")
+    z, x = mol("water")
+    ref = oracle.run(z, x, max_scc=1, **TIGHT)
+    res = analytic_gradient(z, x, options=AnalyticGradientOptions(
+        electronic=ElectronicOptions(max_scc=1, **TIGHT)))
+    assert res.electronic_result.converged
+    assert res.electronic_result.iterations == ref["iterations"]
+    assert abs(res.total_energy - ref["energy"]) <= E_TOL
+
+
+def test_error_paths():
+    note("case error paths")
+    with pytest.raises(NativeError) as e:
+        analytic_gradient([1, 1, 8], np.array([[0, 0, 0], [0, 0, 1e-7], [1.8, 0, 0]]))
+    assert e.value.kind == "InvalidInput"
+    with pytest.raises(NativeError) as e:
+        analytic_gradient([8, 1, 1], np.array([[0, 0, 0], [1.8, 0, 0], [float("nan"), 0, 0]]))
+    assert e.value.kind == "InvalidInput"
+    with pytest.raises(NativeError) as e:
+        analytic_gradient([1, 1], np.array([[0, 0, 0], [1.4, 0, 0]]), charge=40.0)
+    assert e.value.kind == "InvalidInput"
+    with pytest.raises(NativeError) as e:
+        analytic_gradient([1, 1], np.array([[0, 0, 0], [1.4, 0, 0]]), options=AnalyticGradientOptions(
+            electronic=ElectronicOptions(max_scc=2, energy_tolerance=-1.0)))
+    assert e.value.kind == "SccNotConverged"
