@@ -56,6 +56,8 @@ struct DeviceBatch {
     const int32_t* sh_type;  // z*kMaxShellsPerAtom + s
     const int32_t* sh_ao;    // first AO, local to the instance
     const int32_t* ao_sh;    // global shell index
+    const int32_t* sh_by_l;  // global shell indices, grouped by l within each instance
+    const int32_t* lofs;     // 4 per instance: start of the l = 0, 1, 2 groups in sh_by_l, end
     const int64_t* vec0;     // offset of the per-instance length-ld vectors
     // per-instance dense matrices (column-major, ld x ld)
     double* const* S;

@@ -173,6 +173,19 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     h_errmsg_input_ = h_errmsg_;
     natoms_ = h_atom0_[ninst_];
     nshells_ = h_sh0_[ninst_];
+    // shells of each instance grouped by angular momentum (stable within a group)
+    h_sh_by_l_.resize(h_sh0_[ninst_]);
+    h_lofs_.assign(4 * (size_t)ninst_, 0);
+    for (int i = 0; i < ninst_; ++i) {
+        int pos = h_sh0_[i];
+        for (int l = 0; l < 3; ++l) {
+            h_lofs_[4 * i + l] = pos;
+            for (int sh = h_sh0_[i]; sh < h_sh0_[i + 1]; ++sh)
+                if (m.shell_table()[h_sh_type_[sh]].l == l) h_sh_by_l_[pos++] = sh;
+        }
+        h_lofs_[4 * i + 3] = pos;
+        if (h_lofs_[4 * i + 3] > h_lofs_[4 * i + 2]) has_d_ = true;
+    }
     naos_ = h_ao0_[ninst_];
 
     // SCC ladder (electronic.rs:1413-1419)
@@ -221,6 +234,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = up_i(&d_sh_type_, h_sh_type_)).ok()) return st;
     if (!(st = up_i(&d_sh_ao_, h_sh_ao_)).ok()) return st;
     if (!(st = up_i(&d_ao_sh_, h_ao_sh_)).ok()) return st;
+    if (!(st = up_i(&d_sh_by_l_, h_sh_by_l_)).ok()) return st;
+    if (!(st = up_i(&d_lofs_, h_lofs_)).ok()) return st;
     if (!(st = dalloc(&d_fail_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_vec0_, ninst_ + 1)).ok()) return st;
     HIP_TRY(hipMemcpy(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -296,6 +311,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     db_.sh_type = d_sh_type_;
     db_.sh_ao = d_sh_ao_;
     db_.ao_sh = d_ao_sh_;
+    db_.sh_by_l = d_sh_by_l_;
+    db_.lofs = d_lofs_;
     db_.vec0 = d_vec0_;
     db_.S = d_mat_ + (size_t)R_S * ninst_;
     db_.H0 = d_mat_ + (size_t)R_H0 * ninst_;
@@ -389,9 +406,14 @@ Status Batch::setup_phase() {
                        opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_, d_cnd3_, d_gw_,
                        d_gw_ + (size_t)na * kMaxRef, na);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_overlap_h0, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+    hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
                        opt_.integral_cutoff);
     LAUNCH_CHECK();
+    if (has_d_) {
+        hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+                           opt_.integral_cutoff);
+        LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_pad_diagonal, dim3(ninst_), dim3(64), 0, stream_, db_);
     LAUNCH_CHECK();
     // geometry-only terms
@@ -676,10 +698,16 @@ Status Batch::gradient_phase() {
     const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
     const int cn_on = opt_.enable_cn_hamiltonian;
     if (el_on) {
-        hipLaunchKernelGGL(k_grad_pairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+        hipLaunchKernelGGL(k_grad_pairs_t<false>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
                            (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
                            d_dedcn_h_);
         LAUNCH_CHECK();
+        if (has_d_) {
+            hipLaunchKernelGGL(k_grad_pairs_t<true>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+                               (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
+                               d_dedcn_h_);
+            LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                            (const double*)d_qout_, cn_on, opt_.include_scc, d_dedcn_h_, d_gel_, na);
         LAUNCH_CHECK();

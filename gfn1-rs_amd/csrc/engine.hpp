@@ -122,7 +122,7 @@ class Batch {
     EngineOptions opt_;
     int ninst_ = 0, natoms_ = 0, nshells_ = 0, naos_ = 0;
     std::vector<int> h_atom0_, h_sh0_, h_ao0_, h_ld_, h_atom_z_, h_sh_atom_, h_sh_type_, h_sh_ao_, h_ao_sh_,
-        h_atom_sh0_, h_atom_inst_;
+        h_atom_sh0_, h_atom_inst_, h_sh_by_l_, h_lofs_;
     std::vector<double> h_x_, h_q0_, h_nelec_;
     std::vector<int64_t> h_vec0_, h_broff_;
     std::vector<int> h_bucket_of_, h_final_parity_;
@@ -138,7 +138,7 @@ class Batch {
     DeviceBatch db_{};
     int *d_atom0_ = nullptr, *d_sh0_ = nullptr, *d_ao0_ = nullptr, *d_ld_ = nullptr, *d_atom_inst_ = nullptr,
         *d_atom_z_ = nullptr, *d_atom_sh0_ = nullptr, *d_sh_atom_ = nullptr, *d_sh_type_ = nullptr,
-        *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr;
+        *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr, *d_sh_by_l_ = nullptr, *d_lofs_ = nullptr;
     int64_t *d_vec0_ = nullptr, *d_broff_ = nullptr;
     double *d_x_ = nullptr, *d_cn_ = nullptr, *d_cnd3_ = nullptr, *d_gw_ = nullptr;
     double *d_qin_ = nullptr, *d_qout_ = nullptr, *d_q0_ = nullptr, *d_vin_g_ = nullptr, *d_vin_t_ = nullptr,
@@ -154,6 +154,7 @@ class Batch {
     std::vector<SccState> h_state_;
     hipStream_t stream_ = nullptr;
     bool ran_ = false;
+    bool has_d_ = false;  // any d shell in the batch
 
     template <typename T>
     Status dalloc(T** p, size_t count);

@@ -268,62 +268,7 @@ __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int
     }
 }
 
-// ---------------------------------------------------------------------------
-// Overlap S and core Hamiltonian H0 (integrals.rs:133-242; hamiltonian.rs:91-134).
-// One work-group per atom A; lanes walk the (shell of A, shell of the instance)
-// pairs and write column block A of S and H0 (coalesced along the other index).
-__global__ void k_overlap_h0(DeviceBatch b, DeviceTables t, const double* cn, double cutoff) {
-    const int a = blockIdx.x;
-    const int inst = b.atom_inst[a];
-    const int s0 = b.sh0[inst], nshi = b.sh0[inst + 1] - s0;
-    const int za = b.atom_z[a];
-    const int nsha = t.elements[za].nshell;
-    const int ld = b.ld[inst];
-    double* S = b.S[inst];
-    double* H = b.H0[inst];
-    const double* x = b.atom_x;
-    const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
-    const double c2 = cutoff * cutoff;
-    const int sa0 = b.atom_sh0[a];
-    for (int item = threadIdx.x; item < nsha * nshi; item += blockDim.x) {
-        const int sa = sa0 + item / nshi;
-        const int sb = s0 + item % nshi;
-        const int ob = b.sh_atom[sb];
-        const ShellType& A = t.shells[b.sh_type[sa]];
-        const ShellType& B = t.shells[b.sh_type[sb]];
-        const int zb = b.atom_z[ob];
-        const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
-        double blk[25];
-        bool inside = true;
-        double r2 = 0.0;
-        if (ob != a) {
-            r2 = dist2(x, a, ob);
-            inside = r2 > kPairEps2 && r2 <= c2;
-        }
-        const double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
-        if (inside) shell_pair<false>(A, B, ra, rb, blk, nullptr, nullptr);
-        else
-            for (int k = 0; k < 25; ++k) blk[k] = 0.0;
-        const double sea = A.hdiag - A.kcn * cn[a];
-        const double seb = B.hdiag - B.kcn * cn[ob];
-        double h;
-        if (ob == a) {
-            h = 0.5 * (sea + seb);
-        } else {
-            const double rr = sqrt(sqrt(r2) / (t.elements[za].atomic_radius + t.elements[zb].atomic_radius));
-            h = 0.5 * (sea + seb) * hscale(t, za, A, zb, B) * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
-        }
-        const int na = 2 * A.l + 1, nb = 2 * B.l + 1;
-        for (int i = 0; i < na; ++i)
-            for (int j = 0; j < nb; ++j) {
-                const double s = blk[i * 5 + j];
-                // element (nu, mu) of the symmetric matrices: column mu, row nu
-                const size_t idx = (size_t)(mu0 + i) * ld + (nu0 + j);
-                S[idx] = s;
-                H[idx] = s * h;
-            }
-    }
-}
+// (S/H0 kernel: see the shell-class kernels below)
 
 // Diagonal of the padded rows/columns: S = 1, H0 = F = kPadEnergy.
 __global__ void k_pad_diagonal(DeviceBatch b) {
@@ -1032,92 +977,6 @@ __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* ou
     if (threadIdx.x == 0) out[inst] = s;
 }
 
-// ---------------------------------------------------------------------------
-// Electronic gradient, pair part (gradient.rs:333-462): overlap-derivative
-// (Pulay + shell potential) term, shell-polynomial term and the H0 dE/dCN
-// accumulators.  One 64-lane work-group per atom A; each (shell of A, shell of
-// B) item is evaluated with the reference's bra/ket assignment (bra = the
-// higher-index atom of the unique pair).
-__global__ void k_grad_pairs(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
-                             int cn_on, double cutoff, double* g_pair, double* dedcn_h) {
-    const int a = blockIdx.x;
-    const int inst = b.atom_inst[a];
-    const int s0 = b.sh0[inst], nshi = b.sh0[inst + 1] - s0;
-    const int ld = b.ld[inst];
-    const int za = b.atom_z[a];
-    const int nsha = t.elements[za].nshell;
-    const int sa0 = b.atom_sh0[a];
-    const double* x = b.atom_x;
-    const double* P = b.P[inst];
-    const double* W = b.W[inst];
-    const double c2 = cutoff * cutoff;
-    const double rA[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
-    double gacc[3] = {0, 0, 0}, dacc = 0.0;
-    for (int item = threadIdx.x; item < nsha * nshi; item += blockDim.x) {
-        const int sa = sa0 + item / nshi;
-        const int sb = s0 + item % nshi;
-        const int o = b.sh_atom[sb];
-        if (o == a) continue;
-        const double r2 = dist2(x, a, o);
-        if (r2 <= kPairEps2 || r2 > c2 || r2 <= 1e-12) continue;
-        const int zb = b.atom_z[o];
-        const ShellType& A = t.shells[b.sh_type[sa]];
-        const ShellType& B = t.shells[b.sh_type[sb]];
-        const double rB[3] = {x[3 * o], x[3 * o + 1], x[3 * o + 2]};
-        const bool a_is_bra = a > o;  // unique pair (i<j): bra is j
-        double blk[25], dbra[75], dket[75];
-        if (a_is_bra) shell_pair<true>(A, B, rA, rB, blk, dbra, dket);
-        else shell_pair<true>(B, A, rB, rA, blk, dbra, dket);
-        const double radsum = t.elements[za].atomic_radius + t.elements[zb].atomic_radius;
-        const double rr = sqrt(sqrt(r2) / radsum);
-        const double fa = 1.0 + A.poly * rr, fb = 1.0 + B.poly * rr;
-        const double hs = hscale(t, za, A, zb, B) * (fa * fb);
-        const double dse_a = cn_on ? -A.kcn : 0.0, dse_b = cn_on ? -B.kcn : 0.0;
-        const double sea = A.hdiag + dse_a * cn[a], seb = B.hdiag + dse_b * cn[o];
-        const double hij = 0.5 * (sea + seb) * hs;
-        // log-derivative of the shell polynomial (gradient.rs:1157-1175), rvec = bra - ket
-        double rvec[3];
-        for (int d = 0; d < 3; ++d) rvec[d] = a_is_bra ? rA[d] - rB[d] : rB[d] - rA[d];
-        double dlp[3] = {0, 0, 0};
-        {
-            // shell order (mu = bra shell, nu = ket shell) as in the reference
-            const double pm = a_is_bra ? A.poly : B.poly, pn = a_is_bra ? B.poly : A.poly;
-            const double fi = 1.0 + pm * rr, fj = 1.0 + pn * rr;
-            const double pl = fi * fj;
-            if (fabs(pl) > 1e-18) {
-                const double dp = (fi * pn + fj * pm) * 0.5 * rr / r2;
-                for (int d = 0; d < 3; ++d) dlp[d] = rvec[d] * (dp / pl);
-            }
-        }
-        const int mua0 = b.sh_ao[sa], nub0 = b.sh_ao[sb];
-        const double va = scc_on ? vsh[sa] : 0.0, vb = scc_on ? vsh[sb] : 0.0;
-        const int na = 2 * A.l + 1, nb = 2 * B.l + 1;
-        for (int i = 0; i < na; ++i)
-            for (int j = 0; j < nb; ++j) {
-                const int mu = mua0 + i, nu = nub0 + j;  // mu on A, nu on B
-                const double p = P[(size_t)nu * ld + mu];
-                const double w = W[(size_t)nu * ld + mu];
-                const double oc = p * (2.0 * hij - (va + vb)) - 2.0 * w;
-                const int k = a_is_bra ? (i * 5 + j) : (j * 5 + i);
-                const double s = blk[k];
-                const double hp = p * hij;
-                for (int d = 0; d < 3; ++d) {
-                    const double pg = dlp[d] * (2.0 * hp * s);
-                    if (a_is_bra) gacc[d] += dbra[k * 3 + d] * oc + pg;
-                    else gacc[d] += dket[k * 3 + d] * oc - pg;
-                }
-                if (cn_on) dacc += dse_a * hs * p * s;
-            }
-    }
-    // wave reduction (blockDim == 64)
-    for (int d = 0; d < 3; ++d) gacc[d] = wave_sum(gacc[d]);
-    dacc = wave_sum(dacc);
-    if (threadIdx.x == 0) {
-        for (int d = 0; d < 3; ++d) g_pair[3 * a + d] = gacc[d];
-        dedcn_h[a] = dacc;
-    }
-}
-
 // Electronic gradient, per-atom part: diagonal dE/dCN (gradient.rs:465-469) and
 // the isotropic Coulomb term (gradient.rs:493-516).
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
@@ -1166,5 +1025,374 @@ __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int
     for (int d = 0; d < 3; ++d) g_el[3 * a + d] += acc[d];
 }
 
+
+// ---------------------------------------------------------------------------
+// Shell-pair integrals specialised by angular-momentum class (LA, LB).
+// Overlap block <A mu|B nu> (integrals.rs:290-360) and, with DERIV, its
+// derivative with respect to the bra centre (integrals.rs:481-537, :782-868),
+// with the primitive screening bound of integrals.rs:30-45.
+// Block layout: S[mu * NB + nu], dA[(mu * NB + nu) * 3 + d].
+namespace {
+
+template <int LA, int LB, bool DERIV>
+__device__ __forceinline__ void shell_block(const ShellType& A, const ShellType& B, const double* ra,
+                                            const double* rb, double* __restrict__ S, double* __restrict__ dA) {
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+    constexpr int IM = LA + (DERIV ? 1 : 0);
+    constexpr int GM = (IM + LB) / 2 + 1;
+    constexpr int order = LA + LB + (DERIV ? 3 : 2);
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) S[k] = 0.0;
+    if constexpr (DERIV) {
+#pragma unroll
+        for (int k = 0; k < 3 * NA * NB; ++k) dA[k] = 0.0;
+    }
+    const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    const double csmax = ao_abs_sum(LA, 1) * ao_abs_sum(LB, 1);
+    for (int ip = 0; ip < A.nprim; ++ip) {
+        const double al = A.alpha[ip], cai = A.coeff[ip];
+        for (int jp = 0; jp < B.nprim; ++jp) {
+            const double be = B.alpha[jp];
+            const double pref = cai * B.coeff[jp];
+            const double p = al + be;
+            const double kab = exp(-al * be / p * r2);
+            const double spread = sqrt(1.0 / p);
+            const double gpref = (kPi / p) * sqrt(kPi / p);
+            const double poly = ipow(1.0 + r + spread, order);
+            if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
+            double g[GM];
+            g[0] = sqrt(kPi / p);
+#pragma unroll
+            for (int m = 1; m < GM; ++m) g[m] = g[m - 1] * (m - 0.5) / p;
+            double mt[3][IM + 1][LB + 1];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double P = (al * ra[d] + be * rb[d]) / p;
+                const double pa = P - ra[d], pb = P - rb[d];
+#pragma unroll
+                for (int i = 0; i <= IM; ++i)
+#pragma unroll
+                    for (int j = 0; j <= LB; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, g);
+            }
+#pragma unroll
+            for (int mu = 0; mu < NA; ++mu) {
+                CartTerm ta[3];
+                const int nta = ao_terms(LA, mu, ta);
+#pragma unroll
+                for (int nu = 0; nu < NB; ++nu) {
+                    if constexpr (LA == 2 || LB == 2) {
+                        const double cs = ao_abs_sum(LA, mu) * ao_abs_sum(LB, nu);
+                        if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
+                    }
+                    CartTerm tb[3];
+                    const int ntb = ao_terms(LB, nu, tb);
+                    double sv = 0.0, ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;
+                    for (int a = 0; a < nta; ++a)
+                        for (int bq = 0; bq < ntb; ++bq) {
+                            const double cf = pref * ta[a].c * tb[bq].c * kab;
+                            const int i0 = ta[a].px, i1 = ta[a].py, i2 = ta[a].pz;
+                            const int j0 = tb[bq].px, j1 = tb[bq].py, j2 = tb[bq].pz;
+                            const double m0 = mt[0][i0][j0], m1 = mt[1][i1][j1], m2 = mt[2][i2][j2];
+                            sv += cf * (m0 * m1 * m2);
+                            if constexpr (DERIV) {
+                                const double d0 = 2.0 * al * mt[0][i0 + 1][j0] - (i0 > 0 ? i0 * mt[0][i0 > 0 ? i0 - 1 : 0][j0] : 0.0);
+                                const double d1 = 2.0 * al * mt[1][i1 + 1][j1] - (i1 > 0 ? i1 * mt[1][i1 > 0 ? i1 - 1 : 0][j1] : 0.0);
+                                const double d2 = 2.0 * al * mt[2][i2 + 1][j2] - (i2 > 0 ? i2 * mt[2][i2 > 0 ? i2 - 1 : 0][j2] : 0.0);
+                                ga0 += cf * (d0 * (m1 * m2));
+                                ga1 += cf * (d1 * (m2 * m0));
+                                ga2 += cf * (d2 * (m0 * m1));
+                            }
+                        }
+                    S[mu * NB + nu] += sv;
+                    if constexpr (DERIV) {
+                        dA[(mu * NB + nu) * 3 + 0] += ga0;
+                        dA[(mu * NB + nu) * 3 + 1] += ga1;
+                        dA[(mu * NB + nu) * 3 + 2] += ga2;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// S and H0 for one (shell of A, shell of B) pair with B <= A; writes both triangles.
+template <int LA, int LB>
+__device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn, double c2,
+                                          int a, int sa, int sb, int ld, double* S, double* H) {
+    const int ob = b.sh_atom[sb];
+    const ShellType& A = t.shells[b.sh_type[sa]];
+    const ShellType& B = t.shells[b.sh_type[sb]];
+    const double* x = b.atom_x;
+    const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+    const double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+    double blk[NA * NB];
+    double r2 = 0.0;
+    bool inside = true;
+    if (ob != a) {
+        r2 = dist2(x, a, ob);
+        inside = r2 > kPairEps2 && r2 <= c2;
+    }
+    if (inside) shell_block<LA, LB, false>(A, B, ra, rb, blk, nullptr);
+    else
+        for (int k = 0; k < NA * NB; ++k) blk[k] = 0.0;
+    const int za = b.atom_z[a], zb = b.atom_z[ob];
+    const double sea = A.hdiag - A.kcn * cn[a];
+    const double seb = B.hdiag - B.kcn * cn[ob];
+    double h;
+    if (ob == a) {
+        h = 0.5 * (sea + seb);
+    } else {
+        const double rr = sqrt(sqrt(r2) / (t.elements[za].atomic_radius + t.elements[zb].atomic_radius));
+        h = 0.5 * (sea + seb) * hscale(t, za, A, zb, B) * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
+    }
+    const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const double sv = blk[i * NB + j];
+            const size_t e1 = (size_t)(mu0 + i) * ld + (nu0 + j);  // element (nu, mu)
+            const size_t e2 = (size_t)(nu0 + j) * ld + (mu0 + i);  // element (mu, nu)
+            S[e1] = sv;
+            S[e2] = sv;
+            H[e1] = sv * h;
+            H[e2] = sv * h;
+        }
+}
+
+// Electronic gradient contribution of one (shell of A, shell of B) pair to atom A
+// (gradient.rs:358-455): Pulay/shell-potential term, polynomial term, dE/dCN_A.
+// The AO-pair weights oc = P (2 h - v_mu - v_nu) - 2 W and P are applied inside
+// the primitive loop, so neither the overlap block nor its derivative block
+// is ever stored.
+template <int LA, int LB>
+__device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn,
+                                          const double* vsh, int scc_on, int cn_on, int a, int sa, int sb, int ld,
+                                          const double* P, const double* W, double* g, double& dc) {
+    const int ob = b.sh_atom[sb];
+    const ShellType& A = t.shells[b.sh_type[sa]];
+    const ShellType& B = t.shells[b.sh_type[sb]];
+    const double* x = b.atom_x;
+    const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+    const double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+    const int za = b.atom_z[a], zb = b.atom_z[ob];
+    const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = sqrt(r2);
+    const double rr = sqrt(r / (t.elements[za].atomic_radius + t.elements[zb].atomic_radius));
+    const double fa = 1.0 + A.poly * rr, fb = 1.0 + B.poly * rr;
+    const double hs = hscale(t, za, A, zb, B) * (fa * fb);
+    const double dse_a = cn_on ? -A.kcn : 0.0, dse_b = cn_on ? -B.kcn : 0.0;
+    const double hij = 0.5 * ((A.hdiag + dse_a * cn[a]) + (B.hdiag + dse_b * cn[ob])) * hs;
+    const double vv = scc_on ? vsh[sa] + vsh[sb] : 0.0;
+    const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
+    double oc[NA * NB], pw[NA * NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+        const double* Pc = P + (size_t)(mu0 + i) * ld + nu0;
+        const double* Wc = W + (size_t)(mu0 + i) * ld + nu0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const double p = Pc[j];
+            pw[i * NB + j] = p;
+            oc[i * NB + j] = p * (2.0 * hij - vv) - 2.0 * Wc[j];
+        }
+    }
+    constexpr int IM = LA + 1;
+    constexpr int GM = (IM + LB) / 2 + 1;
+    constexpr int order = LA + LB + 3;
+    const double csmax = ao_abs_sum(LA, 1) * ao_abs_sum(LB, 1);
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, sp = 0.0;
+    for (int ip = 0; ip < A.nprim; ++ip) {
+        const double al = A.alpha[ip], cai = A.coeff[ip];
+        for (int jp = 0; jp < B.nprim; ++jp) {
+            const double be = B.alpha[jp];
+            const double pref = cai * B.coeff[jp];
+            const double p = al + be;
+            const double kab = exp(-al * be / p * r2);
+            const double spread = sqrt(1.0 / p);
+            const double gpref = (kPi / p) * sqrt(kPi / p);
+            const double poly = ipow(1.0 + r + spread, order);
+            if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
+            double gm[GM];
+            gm[0] = sqrt(kPi / p);
+#pragma unroll
+            for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * (m - 0.5) / p;
+            double mt[3][IM + 1][LB + 1];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double Pc = (al * ra[d] + be * rb[d]) / p;
+                const double pa = Pc - ra[d], pb = Pc - rb[d];
+#pragma unroll
+                for (int i = 0; i <= IM; ++i)
+#pragma unroll
+                    for (int j = 0; j <= LB; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, gm);
+            }
+#pragma unroll
+            for (int mu = 0; mu < NA; ++mu) {
+                CartTerm ta[3];
+                const int nta = ao_terms(LA, mu, ta);
+#pragma unroll
+                for (int nu = 0; nu < NB; ++nu) {
+                    if constexpr (LA == 2 || LB == 2) {
+                        const double cs = ao_abs_sum(LA, mu) * ao_abs_sum(LB, nu);
+                        if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
+                    }
+                    CartTerm tb[3];
+                    const int ntb = ao_terms(LB, nu, tb);
+                    double sv = 0.0, ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;
+                    for (int aq = 0; aq < nta; ++aq)
+                        for (int bq = 0; bq < ntb; ++bq) {
+                            const double cf = pref * ta[aq].c * tb[bq].c * kab;
+                            const int i0 = ta[aq].px, i1 = ta[aq].py, i2 = ta[aq].pz;
+                            const int j0 = tb[bq].px, j1 = tb[bq].py, j2 = tb[bq].pz;
+                            const double m0 = mt[0][i0][j0], m1 = mt[1][i1][j1], m2 = mt[2][i2][j2];
+                            sv += cf * (m0 * m1 * m2);
+                            const double d0 = 2.0 * al * mt[0][i0 + 1][j0] - (i0 > 0 ? i0 * mt[0][i0 > 0 ? i0 - 1 : 0][j0] : 0.0);
+                            const double d1 = 2.0 * al * mt[1][i1 + 1][j1] - (i1 > 0 ? i1 * mt[1][i1 > 0 ? i1 - 1 : 0][j1] : 0.0);
+                            const double d2 = 2.0 * al * mt[2][i2 + 1][j2] - (i2 > 0 ? i2 * mt[2][i2 > 0 ? i2 - 1 : 0][j2] : 0.0);
+                            ga0 += cf * (d0 * (m1 * m2));
+                            ga1 += cf * (d1 * (m2 * m0));
+                            ga2 += cf * (d2 * (m0 * m1));
+                        }
+                    const double w = oc[mu * NB + nu];
+                    g0 += ga0 * w;
+                    g1 += ga1 * w;
+                    g2 += ga2 * w;
+                    sp += sv * pw[mu * NB + nu];
+                }
+            }
+        }
+    }
+    // polynomial term 2 P hij S dln(poly)/dR_A (gradient.rs:1157-1175) and dE/dCN_A
+    const double pl = fa * fb;
+    double dlp0 = 0.0, dlp1 = 0.0, dlp2 = 0.0;
+    if (fabs(pl) > 1e-18) {
+        const double dp = (fa * B.poly + fb * A.poly) * 0.5 * rr / r2;
+        dlp0 = dx * (dp / pl);
+        dlp1 = dy * (dp / pl);
+        dlp2 = dz * (dp / pl);
+    }
+    const double hp2 = 2.0 * hij * sp;
+    g[0] += g0 + dlp0 * hp2;
+    g[1] += g1 + dlp1 * hp2;
+    g[2] += g2 + dlp2 * hp2;
+    if (cn_on) dc += dse_a * hs * sp;
+}
+
+}  // namespace
+
+// S and H0 (integrals.rs:133-242; hamiltonian.rs:91-134).  One 64-lane
+// work-group per atom A; for every shell of A and every angular-momentum class
+// the lanes walk the instance's shells of that class on atoms B <= A, so a
+// wavefront always evaluates one (LA, LB) specialisation.
+template <bool DCLASS>
+__global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables t, const double* cn, double cutoff) {
+    const int a = blockIdx.x;
+    const int inst = b.atom_inst[a];
+    const int ld = b.ld[inst];
+    double* S = b.S[inst];
+    double* H = b.H0[inst];
+    const double c2 = cutoff * cutoff;
+    const int sa0 = b.atom_sh0[a];
+    const int nsha = t.elements[b.atom_z[a]].nshell;
+    const int* lo = b.lofs + 4 * inst;
+    for (int k = 0; k < nsha; ++k) {
+        const int sa = sa0 + k;
+        const int la = t.shells[b.sh_type[sa]].l;
+        for (int lb = 0; lb < 3; ++lb) {
+            // s/p classes and d classes go to separate instantiations (register budget)
+            if ((la == 2 || lb == 2) != DCLASS) continue;
+            for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
+                const int sb = b.sh_by_l[idx];
+                const int ob = b.sh_atom[sb];
+                if (ob > a || (ob == a && sb > sa)) continue;
+                if constexpr (!DCLASS) {
+                    switch (la * 3 + lb) {
+                        case 0: s_h0_pair<0, 0>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        case 1: s_h0_pair<0, 1>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        case 3: s_h0_pair<1, 0>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        default: s_h0_pair<1, 1>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                    }
+                } else {
+                    switch (la * 3 + lb) {
+                        case 2: s_h0_pair<0, 2>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        case 5: s_h0_pair<1, 2>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        case 6: s_h0_pair<2, 0>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        case 7: s_h0_pair<2, 1>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                        default: s_h0_pair<2, 2>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Electronic gradient, pair part (gradient.rs:333-462) for atom A: every pair
+// with B != A inside the integral cutoff, A taken as the bra.  Deterministic
+// per-atom sums (one 64-lane wavefront per atom, no atomics).
+template <bool DCLASS>
+__global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn,
+                                                     const double* vsh, int scc_on, int cn_on, double cutoff,
+                                                     double* g_pair, double* dedcn_h) {
+    const int a = blockIdx.x;
+    const int inst = b.atom_inst[a];
+    const int ld = b.ld[inst];
+    const double* P = b.P[inst];
+    const double* W = b.W[inst];
+    const double c2 = cutoff * cutoff;
+    const int sa0 = b.atom_sh0[a];
+    const int nsha = t.elements[b.atom_z[a]].nshell;
+    const int* lo = b.lofs + 4 * inst;
+    double g[3] = {0.0, 0.0, 0.0}, dc = 0.0;
+    for (int k = 0; k < nsha; ++k) {
+        const int sa = sa0 + k;
+        const int la = t.shells[b.sh_type[sa]].l;
+        for (int lb = 0; lb < 3; ++lb) {
+            if ((la == 2 || lb == 2) != DCLASS) continue;
+            for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
+                const int sb = b.sh_by_l[idx];
+                const int ob = b.sh_atom[sb];
+                if (ob == a) continue;
+                const double r2 = dist2(b.atom_x, a, ob);
+                if (r2 <= kPairEps2 || r2 > c2 || r2 <= 1e-12) continue;
+                if constexpr (!DCLASS) {
+                    switch (la * 3 + lb) {
+                        case 0: grad_pair<0, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 1: grad_pair<0, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 3: grad_pair<1, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        default: grad_pair<1, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                    }
+                } else {
+                    switch (la * 3 + lb) {
+                        case 2: grad_pair<0, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 5: grad_pair<1, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 6: grad_pair<2, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 7: grad_pair<2, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        default: grad_pair<2, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                    }
+                }
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) g[d] = wave_sum(g[d]);
+    dc = wave_sum(dc);
+    if (threadIdx.x == 0) {
+        // the s/p pass initialises, the d pass accumulates
+        for (int d = 0; d < 3; ++d) g_pair[3 * a + d] = DCLASS ? g_pair[3 * a + d] + g[d] : g[d];
+        dedcn_h[a] = DCLASS ? dedcn_h[a] + dc : dc;
+    }
+}
+
+template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const double*, double);
+template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const double*, double);
+template __global__ void k_grad_pairs_t<false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+                                               double, double*, double*);
+template __global__ void k_grad_pairs_t<true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+                                              double, double*, double*);
 
 }  // namespace gfn1

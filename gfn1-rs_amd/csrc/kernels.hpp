@@ -23,7 +23,8 @@ struct SccParams {
 
 __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
                                double* gw, double* dgw, int natoms);
-__global__ void k_overlap_h0(DeviceBatch b, DeviceTables t, const double* cn, double cutoff);
+template <bool DCLASS>
+__global__ void k_overlap_h0_t(DeviceBatch b, DeviceTables t, const double* cn, double cutoff);
 __global__ void k_pad_diagonal(DeviceBatch b);
 __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, double eig_tol, int* fail);
 __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
@@ -49,8 +50,9 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
                            int natoms);
 __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal);
 __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
-__global__ void k_grad_pairs(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
-                             int cn_on, double cutoff, double* g_pair, double* dedcn_h);
+template <bool DCLASS>
+__global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
+                               int cn_on, double cutoff, double* g_pair, double* dedcn_h);
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
                              double* dedcn_h, double* g_el, int natoms);
 
