@@ -27,7 +27,8 @@ struct SccState {
     int32_t kneed;         // orbitals with non-negligible occupation
     int32_t eig_fail;
     int32_t c_parity;      // which coefficient buffer holds the current orbitals
-    int32_t pad0, pad1;
+    int32_t kneed_full;    // kneed before clamping to the eigenvector capacity
+    int32_t pad1;
     double mixing, level_shift, temperature, kt;
     double e_last, energy, rms, ts, mu, e_el, e2, e3;
     double nelec;

@@ -1,0 +1,785 @@
+// Batched symmetric eigensolver for the SCC (gfx950).
+//
+//   k_tridiag         blocked Householder reduction A = Q T Q^T (LAPACK dsytrd
+//                     'L' sequence with dlatrd panels), one work-group per
+//                     matrix; the symmetric matrix-vector product reads the
+//                     lower triangle once per column
+//   k_tridiag_eig     all eigenvalues of T by Sturm-count bisection, Fermi
+//                     occupations, and the eigenvectors of every orbital with a
+//                     non-negligible occupation by inverse iteration (LAPACK
+//                     dlagtf/dlagts factorisation), one thread per eigenvector;
+//                     groups of nearly equal eigenvalues are re-orthogonalised and
+//                     resolved by a Rayleigh-Ritz step on the group
+//   k_backtransform   Z <- Q Z in compact-WY blocks, one lane per eigenvector
+//
+// The SCC needs the occupied subspace (density matrix), the energies of all
+// orbitals (occupations, orbital-energy output) and, at the end, the
+// energy-weighted density; eigenvectors of empty orbitals are never formed.
+// Eigenvalues are converged to machine precision, so orbital energies and
+// occupations match a full dense eigensolver (the reference uses faer's
+// self-adjoint eigensolver, linalg.rs:153-191).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "device.hpp"
+#include "eigen.hpp"
+
+namespace gfn1 {
+namespace {
+
+constexpr double kEps = 2.220446049250313e-16;
+constexpr double kSafeMin = 2.2250738585072014e-308;
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kBT = 16;  // reflectors per compact-WY block in the back-transformation
+
+__device__ __forceinline__ double wsum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return __shfl(v, 0, 64);
+}
+
+// deterministic work-group sum; every thread gets the result
+__device__ double bsum(double v, double* red) {
+    v = wsum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    __syncthreads();
+    return s;
+}
+
+// Sums eight per-lane values over the wavefront at once (10 exchanges instead
+// of 8 x 6).  Lane L receives the total of s[q] with
+// q = 4 * bit5(L) + 2 * bit4(L) + bit3(L).
+__device__ __forceinline__ double reduce8(const double (&s)[8], int lane) {
+    const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
+    double t4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double send = h32 ? s[q] : s[q + 4];
+        const double keep = h32 ? s[q + 4] : s[q];
+        t4[q] = keep + __shfl_xor(send, 32, 64);
+    }
+    double t2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double send = h16 ? t4[q] : t4[q + 2];
+        const double keep = h16 ? t4[q + 2] : t4[q];
+        t2[q] = keep + __shfl_xor(send, 16, 64);
+    }
+    const double send = h8 ? t2[0] : t2[1];
+    double t1 = (h8 ? t2[1] : t2[0]) + __shfl_xor(send, 8, 64);
+    t1 += __shfl_xor(t1, 4, 64);
+    t1 += __shfl_xor(t1, 2, 64);
+    t1 += __shfl_xor(t1, 1, 64);
+    return t1;
+}
+
+// ---------------------------------------------------------------------------
+// Householder tridiagonalisation.  Within a panel of kEigPanel columns the
+// trailing matrix is kept as A - V W^T - W V^T (V: reflectors stored in A,
+// W: scratch); the rank-2b update is applied once per panel.  Lane L of a
+// wavefront owns rows L, L + 64, ... (RPL of them) in the matrix-vector product.
+template <int RPL>
+__global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ act, const int* __restrict__ nvec, int ld,
+                                                      double* const* Aptr, double* const* Wptr, double* dvec,
+                                                      double* evec, double* tauvec) {
+    extern __shared__ double sm[];
+    const int l = blockIdx.x;
+    const int n = nvec[act[l]];
+    double* __restrict__ A = Aptr[l];
+    double* __restrict__ Wp = Wptr[l];
+    double* d = dvec + (size_t)l * ld;
+    double* e = evec + (size_t)l * ld;
+    double* tau = tauvec + (size_t)l * ld;
+    double* xs = sm;                       // column k after the pending updates
+    double* vs = sm + ld;                  // reflector
+    double* ps = sm + 2 * ld;              // tau * (current trailing matrix) * v
+    double* pd = sm + 3 * ld;              // column-dot part of the product
+    double* part = sm + 4 * ld;            // row part, one slice per wavefront
+    double* ys = sm + (4 + kWaves) * ld;   // W^T v, V^T v interleaved
+    double* red = ys + 2 * kEigPanel;      // 16
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int k0 = 0; k0 < n - 1; k0 += kEigPanel) {
+        const int b = min(kEigPanel, n - 1 - k0);
+        for (int j = 0; j < b; ++j) {
+            const int k = k0 + j;
+            const int c0 = k + 1;
+            // ---- column k of the current matrix, reflector annihilating rows k+2..
+            double nrm = 0.0;
+            for (int r = k + tid; r < n; r += kThreads) {
+                double x = A[r + (size_t)k * ld];
+                for (int i = 0; i < j; ++i) {
+                    const size_t ci = (size_t)(k0 + i) * ld, wi = (size_t)i * ld;
+                    x -= A[r + ci] * Wp[k + wi] + Wp[r + wi] * A[k + ci];
+                }
+                xs[r] = x;
+                if (r > c0) nrm += x * x;
+            }
+            const double xnorm2 = bsum(nrm, red);
+            const double alpha = xs[c0];
+            double beta = alpha, tk = 0.0, scale = 0.0;
+            if (xnorm2 > 0.0) {
+                const double nr = sqrt(alpha * alpha + xnorm2);
+                beta = alpha >= 0.0 ? -nr : nr;
+                tk = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            for (int r = c0 + tid; r < n; r += kThreads) {
+                const double v = r == c0 ? 1.0 : xs[r] * scale;
+                vs[r] = v;
+                A[r + (size_t)k * ld] = v;
+            }
+            if (tid == 0) {
+                d[k] = xs[k];
+                e[k] = beta;
+                tau[k] = tk;
+            }
+            __syncthreads();
+            // ---- A_panel v over the lower triangle of the trailing block
+            double acc[RPL], vr[RPL];
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = 64 * t + lane;
+                vr[t] = (r >= c0 && r < n) ? vs[r] : 0.0;
+                acc[t] = 0.0;
+            }
+            for (int g = c0 + 8 * wid; g < n; g += 8 * kWaves) {
+                double s[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    s[q] = 0.0;
+                    const int c = g + q;
+                    if (c < n) {
+                        const double vc = vs[c];
+                        const double* col = A + (size_t)c * ld;
+#pragma unroll
+                        for (int t = 0; t < RPL; ++t) {
+                            if (64 * t + 63 < c) continue;
+                            const int r = 64 * t + lane;
+                            const double a = (r >= c && r < n) ? col[r] : 0.0;
+                            acc[t] += a * vc;
+                            s[q] += r > c ? a * vr[t] : 0.0;
+                        }
+                    }
+                }
+                const double tot = reduce8(s, lane);
+                const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+                if ((lane & 7) == 0 && g + idx < n) pd[g + idx] = tot;
+            }
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = 64 * t + lane;
+                if (r >= c0 && r < n) part[wid * ld + r] = acc[t];
+            }
+            // ---- panel corrections: ys[2i] = W_i^T v, ys[2i+1] = V_i^T v
+            for (int q = wid; q < 2 * j; q += kWaves) {
+                const int i = q >> 1;
+                const double* src = (q & 1) ? A + (size_t)(k0 + i) * ld : Wp + (size_t)i * ld;
+                double sp = 0.0;
+                for (int r = c0 + lane; r < n; r += 64) sp += src[r] * vs[r];
+                sp = wsum(sp);
+                if (lane == 0) ys[q] = sp;
+            }
+            __syncthreads();
+            // ---- p = tau (A v - V W^T v - W V^T v); w = p - tau/2 (p.v) v
+            double pv = 0.0;
+            for (int r = c0 + tid; r < n; r += kThreads) {
+                double pr = pd[r];
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) pr += part[w * ld + r];
+                for (int i = 0; i < j; ++i)
+                    pr -= A[r + (size_t)(k0 + i) * ld] * ys[2 * i] + Wp[r + (size_t)i * ld] * ys[2 * i + 1];
+                pr *= tk;
+                ps[r] = pr;
+                pv += pr * vs[r];
+            }
+            const double a2 = -0.5 * tk * bsum(pv, red);
+            for (int r = c0 + tid; r < n; r += kThreads) Wp[r + (size_t)j * ld] = ps[r] + a2 * vs[r];
+            __syncthreads();
+        }
+        // ---- rank-2b update of the trailing lower triangle (columns >= k0 + b)
+        const int cs = k0 + b;
+        if (cs < n) {
+            for (int t = cs / 64; t < RPL; ++t) {
+                if (64 * t >= n) break;
+                const int r = 64 * t + lane;
+                const bool rv = r >= cs && r < n;
+                double vr2[kEigPanel], wr2[kEigPanel];
+#pragma unroll
+                for (int i = 0; i < kEigPanel; ++i) {
+                    vr2[i] = (rv && i < b) ? A[r + (size_t)(k0 + i) * ld] : 0.0;
+                    wr2[i] = (rv && i < b) ? Wp[r + (size_t)i * ld] : 0.0;
+                }
+                const int cend = min(64 * t + 64, n);
+                for (int c = cs + wid; c < cend; c += kWaves) {
+                    double upd = 0.0;
+#pragma unroll
+                    for (int i = 0; i < kEigPanel; ++i)
+                        if (i < b) upd += vr2[i] * Wp[c + (size_t)i * ld] + wr2[i] * A[c + (size_t)(k0 + i) * ld];
+                    if (rv && r >= c) A[r + (size_t)c * ld] -= upd;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0 && n > 0) {
+        d[n - 1] = A[(size_t)(n - 1) * ld + (n - 1)];
+        e[n - 1] = 0.0;
+        tau[n - 1] = 0.0;
+    }
+}
+
+// number of eigenvalues of T strictly below x (Sturm sequence of T - x I)
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+    int c = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0.0) ++c;
+    for (int i = 1; i < n; ++i) {
+        q = d[i] - x - e2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+
+// deterministic pseudo-random start vector entry in (-1, 1)
+__device__ __forceinline__ double start_value(int i, int j) {
+    unsigned int h = (unsigned int)(i * 2654435761u) ^ (unsigned int)((j + 1) * 40503u) ^ 0x9e3779b9u;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    h *= 0x297a2d39u;
+    h ^= h >> 15;
+    return (double)(h & 0xffffff) / 8388608.0 - 1.0;
+}
+
+// Cyclic Jacobi for a small symmetric matrix H (m x m, row stride K, single
+// thread).  On exit R holds the eigenvectors as columns, ordered by ascending
+// eigenvalue.
+__device__ void jacobi_small(double* H, double* R, int m, int K) {
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) R[i * K + j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double h2 = H[i * K + j] * H[i * K + j];
+                tot += h2;
+                if (i != j) off += h2;
+            }
+        if (off <= 1e-34 * tot) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = H[p * K + q];
+                if (apq == 0.0) continue;
+                const double theta = (H[q * K + q] - H[p * K + p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double hp = H[k * K + p], hq = H[k * K + q];
+                    H[k * K + p] = c * hp - s * hq;
+                    H[k * K + q] = s * hp + c * hq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double hp = H[p * K + k], hq = H[q * K + k];
+                    H[p * K + k] = c * hp - s * hq;
+                    H[q * K + k] = s * hp + c * hq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double rp = R[k * K + p], rq = R[k * K + q];
+                    R[k * K + p] = c * rp - s * rq;
+                    R[k * K + q] = s * rp + c * rq;
+                }
+            }
+    }
+    for (int i = 0; i < m - 1; ++i) {
+        int best = i;
+        for (int j = i + 1; j < m; ++j)
+            if (H[j * K + j] < H[best * K + best]) best = j;
+        if (best != i) {
+            const double hb = H[best * K + best];
+            H[best * K + best] = H[i * K + i];
+            H[i * K + i] = hb;
+            for (int k = 0; k < m; ++k) {
+                const double r = R[k * K + i];
+                R[k * K + i] = R[k * K + best];
+                R[k * K + best] = r;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Eigenvalues, occupations (electronic.rs:3853-3921) and occupied eigenvectors
+// of T.  LDS: 5 ld + 2 kEigMaxCluster^2 + 16 doubles + ld ints.
+__global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const int* nvec, int ld, const double* dvec,
+                                                          const double* evec, double* eps_all, const int64_t* vec0,
+                                                          double* occ_all, SccState* st, double* const* Ztptr,
+                                                          double* lu, int kcap) {
+    extern __shared__ double sm[];
+    const int l = blockIdx.x;
+    const int inst = act[l];
+    const int n = nvec[inst];
+    const double* dg = dvec + (size_t)l * ld;
+    const double* eg = evec + (size_t)l * ld;
+    double* D = sm;             // diagonal
+    double* E2 = sm + ld;       // squared off-diagonals
+    double* Eo = sm + 2 * ld;   // off-diagonals
+    double* lam = sm + 3 * ld;  // eigenvalues, ascending
+    double* sh = sm + 4 * ld;   // inverse-iteration shifts
+    constexpr int KC = kEigMaxCluster;
+    double* Hc = sm + 5 * ld;   // projected cluster matrix
+    double* Rc = Hc + KC * KC;  // its eigenvectors
+    double* red = Rc + KC * KC;
+    int* tight = (int*)(red + 16);  // 1 where lam[j] joins the group of lam[j-1]
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n; i += nt) {
+        D[i] = dg[i];
+        Eo[i] = i < n - 1 ? eg[i] : 0.0;
+        E2[i] = i < n - 1 ? eg[i] * eg[i] : 0.0;
+    }
+    __syncthreads();
+    // Gershgorin interval and pivot guard (LAPACK dstebz conventions)
+    double lo = INFINITY, hi = -INFINITY, emax = 0.0, tnorm = 0.0;
+    for (int i = tid; i < n; i += nt) {
+        const double r = (i > 0 ? fabs(Eo[i - 1]) : 0.0) + (i < n - 1 ? fabs(Eo[i]) : 0.0);
+        lo = fmin(lo, D[i] - r);
+        hi = fmax(hi, D[i] + r);
+        emax = fmax(emax, i < n - 1 ? E2[i] : 0.0);
+        tnorm = fmax(tnorm, fabs(D[i]) + r);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_down(lo, off, 64));
+        hi = fmax(hi, __shfl_down(hi, off, 64));
+        emax = fmax(emax, __shfl_down(emax, off, 64));
+        tnorm = fmax(tnorm, __shfl_down(tnorm, off, 64));
+    }
+    __shared__ double s4[4][16];
+    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+    if (lane == 0) {
+        s4[0][wid] = lo;
+        s4[1][wid] = hi;
+        s4[2][wid] = emax;
+        s4[3][wid] = tnorm;
+    }
+    __syncthreads();
+    lo = s4[0][0];
+    hi = s4[1][0];
+    emax = s4[2][0];
+    tnorm = s4[3][0];
+    for (int q = 1; q < nw; ++q) {
+        lo = fmin(lo, s4[0][q]);
+        hi = fmax(hi, s4[1][q]);
+        emax = fmax(emax, s4[2][q]);
+        tnorm = fmax(tnorm, s4[3][q]);
+    }
+    __syncthreads();
+    const double pivmin = fmax(kSafeMin * fmax(1.0, emax), 1e-300);
+    const double widen = 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+    lo -= widen;
+    hi += widen;
+    // bisection: thread j finds the j-th smallest eigenvalue
+    for (int j = tid; j < n; j += nt) {
+        double a = lo, b = hi;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;
+            if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
+            if (sturm_count(D, E2, n, mid, pivmin) > j) b = mid;
+            else a = mid;
+        }
+        lam[j] = 0.5 * (a + b);
+    }
+    __syncthreads();
+    // ---- Fermi occupations (same semantics as k_fermi)
+    SccState& s = st[inst];
+    const double kt = s.kt, nelec = s.nelec;
+    double* eps = eps_all + vec0[inst];
+    double* occ = occ_all + vec0[inst];
+    for (int i = tid; i < ld; i += nt) eps[i] = i < n ? lam[i] : kPadEnergy;
+    double mu = 0.0, ts = 0.0;
+    int kneed = 0;
+    const double mn = lam[0], mx = lam[n - 1];
+    if (kt <= 0.0) {
+        if (tid == 0) {
+            double rem = fmax(nelec, 0.0);
+            int homo = 0;
+            for (int i = 0; i < ld; ++i) {
+                double f = 0.0;
+                if (i < n && rem > 0.0) {
+                    f = fmin(rem, 2.0);
+                    rem -= f;
+                }
+                occ[i] = f;
+                if (i < n && f > 1e-12) homo = i;
+            }
+            mu = lam[homo];
+            kneed = homo + 1;
+            red[0] = mu;
+            red[1] = (double)kneed;
+        }
+        __syncthreads();
+        mu = red[0];
+        kneed = (int)red[1];
+        __syncthreads();
+    } else if (nelec <= 0.0 || nelec >= 2.0 * n) {
+        const double f = nelec <= 0.0 ? 0.0 : 2.0;
+        for (int i = tid; i < ld; i += nt) occ[i] = i < n ? f : 0.0;
+        mu = nelec <= 0.0 ? lam[0] : lam[n - 1];
+        kneed = nelec <= 0.0 ? 0 : n;
+    } else {
+        double a = mn - 100.0 * kt - 10.0, b = mx + 100.0 * kt + 10.0;
+        for (int it = 0; it < 200; ++it) {
+            const double m = 0.5 * (a + b);
+            double part = 0.0;
+            for (int i = tid; i < n; i += nt) {
+                const double xx = fmin(80.0, fmax(-80.0, (lam[i] - m) / kt));
+                part += 2.0 / (1.0 + exp(xx));
+            }
+            const double sum = bsum(part, red);
+            double na = a, nb = b;
+            if (sum < nelec) na = m;
+            else nb = m;
+            if (na == a && nb == b) break;
+            a = na;
+            b = nb;
+        }
+        mu = 0.5 * (a + b);
+        double tpart = 0.0;
+        int kpart = 0;
+        for (int i = tid; i < ld; i += nt) {
+            double f = 0.0;
+            if (i < n) {
+                const double xx = fmin(80.0, fmax(-80.0, (lam[i] - mu) / kt));
+                f = 2.0 / (1.0 + exp(xx));
+                const double h = fmin(1.0 - 1e-16, fmax(1e-16, 0.5 * f));
+                tpart += 2.0 * kt * (h * log(h) + (1.0 - h) * log(1.0 - h));
+                if (f >= 1e-28) kpart = max(kpart, i + 1);
+            }
+            occ[i] = f;
+        }
+        ts = bsum(tpart, red);
+        int km = kpart;
+        for (int off = 32; off > 0; off >>= 1) km = max(km, __shfl_down(km, off, 64));
+        __shared__ int skm[16];
+        if (lane == 0) skm[wid] = km;
+        __syncthreads();
+        km = skm[0];
+        for (int q = 1; q < nw; ++q) km = max(km, skm[q]);
+        kneed = km;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s.mu = mu;
+        s.ts = ts;
+        s.kneed_full = kneed;
+        s.kneed = min(kneed, kcap);
+    }
+    kneed = min(kneed, kcap);
+    // ---- groups of nearly equal eigenvalues and inverse-iteration shifts
+    // (coincident shifts are separated by pertol as in LAPACK dstein)
+    const double tn = fmax(tnorm, 1e-300);
+    const double ctol = 1e-5 * tn, pertol = 10.0 * kEps * tn, tiny = kEps * tn;
+    for (int j = tid; j < kneed; j += nt) tight[j] = (j > 0 && lam[j] - lam[j - 1] < ctol) ? 1 : 0;
+    __syncthreads();
+    if (tid == 0) {
+        double prev = -INFINITY;
+        for (int j = 0; j < kneed; ++j) {
+            double x = lam[j];
+            if (tight[j] && x - prev < pertol) x = prev + pertol;
+            sh[j] = x;
+            prev = x;
+        }
+    }
+    __syncthreads();
+    double* __restrict__ Z = Ztptr[l];
+    const size_t fld = (size_t)ld * kcap;
+    double* __restrict__ rU1 = lu + (size_t)l * 5 * fld;  // 1 / diagonal of U
+    double* __restrict__ U2 = rU1 + fld;                  // first superdiagonal of U
+    double* __restrict__ U3 = rU1 + 2 * fld;              // second superdiagonal (row interchanges)
+    double* __restrict__ Lm = rU1 + 3 * fld;              // multipliers
+    double* __restrict__ Pv = rU1 + 4 * fld;              // 1 where rows i, i+1 were interchanged
+    // ---- inverse iteration, one thread per eigenvector; element (i, j) of every
+    //      per-vector array sits at i * kcap + j so a wavefront's accesses coalesce
+    for (int j = tid; j < kneed; j += nt) {
+        const double x = sh[j];
+        double a = D[0] - x, bb = n > 1 ? Eo[0] : 0.0;
+        for (int i = 0; i < n - 1; ++i) {
+            const double sub = Eo[i], dn = D[i + 1] - x, en = i + 1 < n - 1 ? Eo[i + 1] : 0.0;
+            const size_t o = (size_t)i * kcap + j;
+            if (fabs(a) >= fabs(sub)) {
+                const double u = fabs(a) < tiny ? copysign(tiny, a) : a;
+                const double m = sub / u;
+                rU1[o] = 1.0 / u;
+                U2[o] = bb;
+                U3[o] = 0.0;
+                Lm[o] = m;
+                Pv[o] = 0.0;
+                a = dn - m * bb;
+                bb = en;
+            } else {
+                const double m = a / sub;
+                rU1[o] = 1.0 / sub;
+                U2[o] = dn;
+                U3[o] = en;
+                Lm[o] = m;
+                Pv[o] = 1.0;
+                a = bb - m * dn;
+                bb = -m * en;
+            }
+        }
+        {
+            const size_t o = (size_t)(n - 1) * kcap + j;
+            const double u = fabs(a) < tiny ? copysign(tiny, a) : a;
+            rU1[o] = 1.0 / u;
+            U2[o] = 0.0;
+            U3[o] = 0.0;
+        }
+        double scale = 1.0, ss = 1.0;
+        for (int it = 0; it < 3; ++it) {
+            // y = L^-1 P z  (row interchanges applied on the fly)
+            double cur = it == 0 ? start_value(0, j) : Z[j] * scale;
+#pragma unroll 4
+            for (int i = 0; i < n - 1; ++i) {
+                const size_t o = (size_t)i * kcap + j;
+                const double nx = it == 0 ? start_value(i + 1, j) : Z[o + kcap] * scale;
+                const double m = Lm[o];
+                double out;
+                if (Pv[o] != 0.0) {
+                    out = nx;
+                    cur = cur - m * nx;
+                } else {
+                    out = cur;
+                    cur = nx - m * cur;
+                }
+                Z[o] = out;
+            }
+            Z[(size_t)(n - 1) * kcap + j] = cur;
+            // z = U^-1 y
+            double z1 = 0.0, z2 = 0.0, zmax = 0.0;
+            ss = 0.0;
+#pragma unroll 4
+            for (int i = n - 1; i >= 0; --i) {
+                const size_t o = (size_t)i * kcap + j;
+                const double zi = (Z[o] - U2[o] * z1 - U3[o] * z2) * rU1[o];
+                Z[o] = zi;
+                z2 = z1;
+                z1 = zi;
+                zmax = fmax(zmax, fabs(zi));
+                ss += zi * zi;
+            }
+            scale = 1.0 / zmax;
+        }
+        const double inv = 1.0 / sqrt(ss);
+        for (int i = 0; i < n; ++i) Z[(size_t)i * kcap + j] *= inv;
+    }
+    // zero rows n..ld-1 and columns kneed..kcap-1
+    for (size_t q = (size_t)n * kcap + tid; q < fld; q += nt) Z[q] = 0.0;
+    const int zc = kcap - kneed;
+    if (zc > 0)
+        for (int q = tid; q < n * zc; q += nt) Z[(size_t)(q / zc) * kcap + kneed + q % zc] = 0.0;
+    __syncthreads();
+    // ---- groups: orthonormalise (modified Gram-Schmidt, twice) and rotate to the
+    //      eigenvectors of T projected on the group (Rayleigh-Ritz)
+    int g0 = 0;
+    while (g0 < kneed) {
+        int g1 = g0 + 1;
+        while (g1 < kneed && tight[g1]) ++g1;
+        const int cs = g1 - g0;
+        if (cs > KC) {
+            if (tid == 0) s.eig_fail = 1;
+        } else if (cs >= 2) {
+            for (int a = 0; a < cs; ++a) {
+                double* za = Z + g0 + a;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int b2 = 0; b2 < a; ++b2) {
+                        const double* zb = Z + g0 + b2;
+                        double p = 0.0;
+                        for (int r = tid; r < n; r += nt) p += zb[(size_t)r * kcap] * za[(size_t)r * kcap];
+                        p = bsum(p, red);
+                        for (int r = tid; r < n; r += nt) za[(size_t)r * kcap] -= p * zb[(size_t)r * kcap];
+                    }
+                double q = 0.0;
+                for (int r = tid; r < n; r += nt) q += za[(size_t)r * kcap] * za[(size_t)r * kcap];
+                const double inv = 1.0 / sqrt(bsum(q, red));
+                for (int r = tid; r < n; r += nt) za[(size_t)r * kcap] *= inv;
+            }
+            __syncthreads();
+            const int npair = cs * (cs + 1) / 2;
+            for (int p = tid; p < npair; p += nt) {
+                int a = 0, rem = p;
+                while (rem >= cs - a) {
+                    rem -= cs - a;
+                    ++a;
+                }
+                const int b2 = a + rem;
+                const double* za = Z + g0 + a;
+                const double* zb = Z + g0 + b2;
+                double h = 0.0;
+                for (int r = 0; r < n; ++r) {
+                    double tz = D[r] * zb[(size_t)r * kcap];
+                    if (r > 0) tz += Eo[r - 1] * zb[(size_t)(r - 1) * kcap];
+                    if (r < n - 1) tz += Eo[r] * zb[(size_t)(r + 1) * kcap];
+                    h += za[(size_t)r * kcap] * tz;
+                }
+                Hc[a * KC + b2] = h;
+                Hc[b2 * KC + a] = h;
+            }
+            __syncthreads();
+            if (tid == 0) jacobi_small(Hc, Rc, cs, KC);
+            __syncthreads();
+            for (int r = tid; r < n; r += nt) {
+                double* zr = Z + (size_t)r * kcap + g0;
+                double* tr = rU1 + (size_t)r * kcap + g0;
+                for (int b2 = 0; b2 < cs; ++b2) {
+                    double v = 0.0;
+                    for (int a = 0; a < cs; ++a) v += zr[a] * Rc[a * KC + b2];
+                    tr[b2] = v;
+                }
+                for (int b2 = 0; b2 < cs; ++b2) zr[b2] = tr[b2];
+            }
+            __syncthreads();
+        }
+        g0 = g1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Z <- Q Z, Q = H_0 H_1 ... H_{n-2}, applied as blocks I - V T V^T (LAPACK
+// dlarft forward/columnwise) from the last block to the first.  Lane j owns
+// eigenvector j (Zt row-major, so each row access coalesces); the block's
+// reflectors sit in LDS as Vb[r][i].
+__global__ void __launch_bounds__(512) k_backtransform(const int* act, const int* nvec, int ld, double* const* Aptr,
+                                                       const double* tauvec, double* const* Ztptr,
+                                                       const SccState* st, int kcap) {
+    extern __shared__ double sm[];
+    double* Vb = sm;                       // (n - lo - 1) x kBT
+    double* Tm = sm + (size_t)ld * kBT;    // kBT x kBT
+    double* G = Tm + kBT * kBT;            // kBT x kBT
+    const int l = blockIdx.x;
+    const int inst = act[l];
+    const int n = nvec[inst];
+    const int k = st[inst].kneed;
+    if (k == 0) return;
+    const double* A = Aptr[l];
+    const double* tau = tauvec + (size_t)l * ld;
+    double* __restrict__ Z = Ztptr[l];
+    const int j = threadIdx.x, nt = blockDim.x;
+    const bool on = j < k;
+    for (int hi = n - 1; hi > 0; hi -= kBT) {
+        const int lo = max(0, hi - kBT), bb = hi - lo, r0 = lo + 1, m = n - r0;
+        __syncthreads();
+        for (int q = threadIdx.x; q < m * kBT; q += nt) {
+            const int r = r0 + q / kBT, i = q % kBT, kk = lo + i;
+            double v = 0.0;
+            if (i < bb) v = r < kk + 1 ? 0.0 : (r == kk + 1 ? 1.0 : A[r + (size_t)kk * ld]);
+            Vb[q] = v;
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < kBT * kBT; p += nt) {
+            const int a = p / kBT, b2 = p % kBT;
+            if (a < b2 && b2 < bb) {
+                double g = 0.0;
+                for (int r = 0; r < m; ++r) g += Vb[r * kBT + a] * Vb[r * kBT + b2];
+                G[p] = g;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int p = 0; p < kBT * kBT; ++p) Tm[p] = 0.0;
+            for (int i = 0; i < bb; ++i) {
+                const double ti = tau[lo + i];
+                Tm[i * kBT + i] = ti;
+                for (int a = 0; a < i; ++a) {
+                    double acc = 0.0;
+                    for (int c = a; c < i; ++c) acc += Tm[a * kBT + c] * G[c * kBT + i];
+                    Tm[a * kBT + i] = -ti * acc;
+                }
+            }
+        }
+        __syncthreads();
+        if (on) {
+            double y[kBT];
+#pragma unroll
+            for (int i = 0; i < kBT; ++i) y[i] = 0.0;
+            for (int r = 0; r < m; ++r) {
+                const double z = Z[(size_t)(r0 + r) * kcap + j];
+                const double* vrow = Vb + r * kBT;
+#pragma unroll
+                for (int i = 0; i < kBT; ++i) y[i] += vrow[i] * z;
+            }
+#pragma unroll
+            for (int i = 0; i < kBT; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = i; c < kBT; ++c) acc += Tm[i * kBT + c] * y[c];
+                y[i] = acc;
+            }
+            for (int r = 0; r < m; ++r) {
+                const double* vrow = Vb + r * kBT;
+                double upd = 0.0;
+#pragma unroll
+                for (int i = 0; i < kBT; ++i) upd += vrow[i] * y[i];
+                Z[(size_t)(r0 + r) * kcap + j] -= upd;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* nvec, int ld, double* const* A,
+                          double* const* Wp, double* d, double* e, double* tau) {
+    if (ld > kEigMaxDim) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)(4 + kWaves) * ld + 2 * kEigPanel + 16) * sizeof(double);
+    const dim3 grid(count), block(kThreads);
+    switch ((ld + 63) / 64) {
+        case 1: hipLaunchKernelGGL(k_tridiag<1>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 2: hipLaunchKernelGGL(k_tridiag<2>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 3: hipLaunchKernelGGL(k_tridiag<3>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 4: hipLaunchKernelGGL(k_tridiag<4>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 5: hipLaunchKernelGGL(k_tridiag<5>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 6: hipLaunchKernelGGL(k_tridiag<6>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 7: hipLaunchKernelGGL(k_tridiag<7>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        default: hipLaunchKernelGGL(k_tridiag<8>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const int* nvec, int ld, const double* d,
+                              const double* e, double* eps_all, const int64_t* vec0, double* occ_all, SccState* st,
+                              double* const* Zt, double* lu, int kcap) {
+    if (ld > kEigMaxDim || kcap > ld) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)5 * ld + 2 * kEigMaxCluster * kEigMaxCluster + 16) * sizeof(double) +
+                       (size_t)ld * sizeof(int);
+    hipLaunchKernelGGL(k_tridiag_eig, dim3(count), dim3(kThreads), lds, s, act, nvec, ld, d, e, eps_all, vec0,
+                       occ_all, st, Zt, lu, kcap);
+    return hipGetLastError();
+}
+
+hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
+                                double* const* A, const double* tau, double* const* Zt, const SccState* st,
+                                int kcap) {
+    if (ld > kEigMaxDim || kcap > ld) return hipErrorInvalidValue;
+    const int threads = std::min(512, ((kcap + 63) / 64) * 64);
+    const size_t lds = ((size_t)ld * kBT + 2 * kBT * kBT) * sizeof(double);
+    static bool attr = false;  // LDS above 64 KiB must be requested explicitly
+    if (!attr) {
+        hipError_t err = hipFuncSetAttribute((const void*)k_backtransform,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)(((size_t)kEigMaxDim * kBT + 2 * kBT * kBT) * sizeof(double)));
+        if (err != hipSuccess) return err;
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_backtransform, dim3(count), dim3(threads), lds, s, act, nvec, ld, A, tau, Zt, st, kcap);
+    return hipGetLastError();
+}
+
+}  // namespace gfn1

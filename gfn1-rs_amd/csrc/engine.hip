@@ -11,11 +11,14 @@
 //   gradient  W, pair/atom gradient kernels, CN chains.
 #include "engine.hpp"
 
+#include "eigen.hpp"
+
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace gfn1 {
@@ -101,6 +104,8 @@ Batch::~Batch() {
         (void)hipFree(bk.sweeps);
         (void)hipFree(bk.info);
         (void)hipFree(bk.beta);
+        (void)hipFree(bk.tau);
+        (void)hipFree(bk.lu);
     }
     for (void* p : allocs_) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -109,6 +114,7 @@ Batch::~Batch() {
 Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& systems, const EngineOptions& opt) {
     dm_ = &dm;
     opt_ = opt;
+    if (const char* pm = std::getenv("GFN1_PAD_MULTIPLE")) pad_multiple_ = std::max(kPadMultiple, std::atoi(pm));
     const Model& m = dm.model();
     ninst_ = (int)systems.size();
     h_atom0_.assign(ninst_ + 1, 0);
@@ -154,7 +160,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         h_atom0_[i + 1] = h_atom0_[i] + s.nat;
         h_sh0_[i + 1] = h_sh0_[i] + nsh;
         h_ao0_[i + 1] = h_ao0_[i] + nao;
-        h_ld_[i] = std::max(kPadMultiple, round_up(nao, kPadMultiple));
+        h_ld_[i] = std::max(pad_multiple_, round_up(nao, pad_multiple_));
         h_vec0_[i + 1] = h_vec0_[i] + h_ld_[i];
         const double nelec = nref - s.charge;
         h_nelec_[i] = nelec;
@@ -236,6 +242,11 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = up_i(&d_ao_sh_, h_ao_sh_)).ok()) return st;
     if (!(st = up_i(&d_sh_by_l_, h_sh_by_l_)).ok()) return st;
     if (!(st = up_i(&d_lofs_, h_lofs_)).ok()) return st;
+    {
+        std::vector<int> nao(ninst_);
+        for (int i = 0; i < ninst_; ++i) nao[i] = h_ao0_[i + 1] - h_ao0_[i];
+        if (!(st = up_i(&d_nao_, nao)).ok()) return st;
+    }
     if (!(st = dalloc(&d_fail_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_vec0_, ninst_ + 1)).ok()) return st;
     HIP_TRY(hipMemcpy(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -295,6 +306,13 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         HIP_TRY(hipMalloc(&bk.resid, sizeof(double) * count));
         HIP_TRY(hipMalloc(&bk.sweeps, sizeof(int) * count));
         HIP_TRY(hipMalloc(&bk.info, sizeof(int) * count));
+        HIP_TRY(hipMalloc(&bk.tau, sizeof(double) * bk.ld * count));
+        // eigenvector capacity: occupied orbitals of the most electron-rich
+        // instance plus a margin for fractional occupations
+        int kmax = 1;
+        for (int i : bk.inst) kmax = std::max(kmax, (int)std::ceil(0.5 * h_nelec_[i] - 1e-9));
+        bk.kcap = std::min(bk.ld, kmax + 32);
+        HIP_TRY(hipMalloc(&bk.lu, sizeof(double) * 5 * (size_t)bk.ld * bk.kcap * count));
     }
     HIP_TRY(hipMemcpy(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
 
@@ -359,7 +377,10 @@ Status Batch::eigensolve(Bucket& bk, int count) {
     const int n = bk.ld;
     int solver = opt_.eigensolver;
     if (solver == 0) solver = 1;
-    if (solver == 2) {
+    if (solver == 3) {
+        RB_TRY(rocsolver_dsyevdj_batched(bk.handle, rocblas_evect_original, rocblas_fill_lower, n,
+                                         (double* const*)A, n, bk.eig, n, bk.info, count));
+    } else if (solver == 2) {
         RB_TRY(rocsolver_dsyevj_batched(bk.handle, rocblas_esort_ascending, rocblas_evect_original,
                                         rocblas_fill_lower, n, (double* const*)A, n, 1e-12, bk.resid, 100,
                                         bk.sweeps, bk.eig, n, bk.info, count));
@@ -503,24 +524,19 @@ Status Batch::scc_phase() {
             if (h_state_[i].status == 0) bk.active.push_back(i);
     }
     const double one = 1.0, zero = 0.0;
-    int it = 0;
     while (true) {
         bool any = false;
         for (auto& bk : buckets_) any = any || !bk.active.empty();
         if (!any) break;
-        const int parity = it & 1;
-        // warm start: the previous orbitals; periodically restart from L^-T is not
-        // needed because C stays S-orthonormal to rounding (V orthogonal).
-        (void)kWarmResetPeriod;
-        // ---- phase 1: Fock, congruence, eigensolve, back-transform, Fermi
         for (auto& bk : buckets_) {
             const int cnt = (int)bk.active.size();
             if (!cnt) continue;
-            Status st = upload_active(bk, bk.active, parity);
+            Status st = upload_active(bk, bk.active, 0);
             if (!st.ok()) return st;
             int max_nsh = 0;
             for (int i : bk.active) max_nsh = std::max(max_nsh, h_sh0_[i + 1] - h_sh0_[i]);
             const int n = bk.ld;
+            // ---- Fock matrix from the input charges
             hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
                                t, (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
             LAUNCH_CHECK();
@@ -532,9 +548,7 @@ Status Batch::scc_phase() {
             for (int i : bk.active)
                 if (h_state_[i].have_prev_p && h_state_[i].level_shift != 0.0) ls.push_back(i);
             if (!ls.empty()) {
-                // reuse the T and A buffers of these instances
-                std::vector<int> keep = bk.active;
-                st = upload_active(bk, ls, parity);
+                st = upload_active(bk, ls, 0);
                 if (!st.ok()) return st;
                 const int c = (int)ls.size();
                 double* const* P = bk.ptrs + (size_t)A_P * c;
@@ -549,72 +563,69 @@ Status Batch::scc_phase() {
                                              (double* const*)Ab, n, c));
                 std::vector<double> shift(c);
                 for (int k = 0; k < c; ++k) shift[k] = h_state_[ls[k]].level_shift;
-                double* dshift = bk.eig_e;  // scratch (>= count doubles)
-                HIP_TRY(hipMemcpyAsync(dshift, shift.data(), c * sizeof(double), hipMemcpyHostToDevice, bk.stream));
+                HIP_TRY(hipMemcpyAsync(bk.eig_e, shift.data(), c * sizeof(double), hipMemcpyHostToDevice,
+                                       bk.stream));
                 hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                                   (const double*)dshift, (double* const*)Ab);
+                                   (const double*)bk.eig_e, (double* const*)Ab);
                 LAUNCH_CHECK();
                 HIP_TRY(hipStreamSynchronize(bk.stream));
-                st = upload_active(bk, keep, parity);
+                st = upload_active(bk, bk.active, 0);
                 if (!st.ok()) return st;
             }
             double* const* F = bk.ptrs + (size_t)A_F * cnt;
-            double* const* Cb = bk.ptrs + (size_t)A_CB * cnt;
-            double* const* C = bk.ptrs + (size_t)A_C * cnt;
+            double* const* Cb = bk.ptrs + (size_t)A_CB * cnt;  // L^-T (S = L L^T)
+            double* const* Z = bk.ptrs + (size_t)A_C * cnt;    // eigenvectors of the reduced problem
             double* const* T = bk.ptrs + (size_t)A_T * cnt;
             double* const* A = bk.ptrs + (size_t)A_A * cnt;
-            // T = F Cb ; A = Cb^T T
+            // ---- A = Cb^T F Cb (orthonormal basis)
             RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                                          (const double* const*)F, n, (const double* const*)Cb, n, &zero,
                                          (double* const*)T, n, cnt));
             RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
                                          &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
                                          (double* const*)A, n, cnt));
-            st = eigensolve(bk, cnt);
-            if (!st.ok()) return st;
-            // C = Cb V
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                                         (const double* const*)Cb, n, (const double* const*)A, n, &zero,
-                                         (double* const*)C, n, cnt));
-            hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
-            LAUNCH_CHECK();
-        }
-        HIP_TRY(hipDeviceSynchronize());
-        // eigensolver status and occupied-column count
-        HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
-        for (auto& bk : buckets_) {
-            const int cnt = (int)bk.active.size();
-            if (!cnt) continue;
-            std::vector<int> info(cnt);
-            HIP_TRY(hipMemcpy(info.data(), bk.info, cnt * sizeof(int), hipMemcpyDeviceToHost));
-            for (int k = 0; k < cnt; ++k)
-                if (info[k] != 0)
-                    return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
-            int kp = 1;
-            for (int i : bk.active) kp = std::max(kp, h_state_[i].kneed);
-            bk.kP = std::min(kp, bk.ld);
-        }
-        // ---- phase 2: density, Mulliken charges, energy, mixing
-        for (auto& bk : buckets_) {
-            const int cnt = (int)bk.active.size();
-            if (!cnt) continue;
-            const int n = bk.ld;
-            int max_nsh = 0;
-            for (int i : bk.active) max_nsh = std::max(max_nsh, h_sh0_[i + 1] - h_sh0_[i]);
-            double* const* C = bk.ptrs + (size_t)A_C * cnt;
-            double* const* T = bk.ptrs + (size_t)A_T * cnt;
+            const int kc = bk.kcap;
+            // matrices above kEigMaxDim go to rocSOLVER
+            const bool own_solver = opt_.eigensolver != 1 && n <= kEigMaxDim;
+            if (own_solver) {
+                // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
+                // (returned transposed in Z, kc columns)
+                double* const* Wp = bk.ptrs + (size_t)A_X * cnt;
+                HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, Wp, bk.eig,
+                                       bk.eig_e, bk.tau));
+                HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
+                                           (const double*)bk.eig, (const double*)bk.eig_e, d_eps_,
+                                           (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc));
+                HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
+                                             (const double*)bk.tau, Z, (const SccState*)d_state_, kc));
+                // C_k = Cb Z (into T)
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, kc,
+                                             n, &one, (const double* const*)Cb, n, (const double* const*)Z, kc,
+                                             &zero, (double* const*)T, n, cnt));
+            } else {
+                Status st2 = eigensolve(bk, cnt);
+                if (!st2.ok()) return st2;
+                hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                                   (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
+                LAUNCH_CHECK();
+                // rocSOLVER returns the eigenvectors in place
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, kc, n,
+                                             &one, (const double* const*)Cb, n, (const double* const*)A, n, &zero,
+                                             (double* const*)T, n, cnt));
+            }
+            // ---- P = C_k diag(f) C_k^T
             double* const* P = bk.ptrs + (size_t)A_P * cnt;
-            hipLaunchKernelGGL(k_scale_columns, dim3(bk.kP, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               C, T, (const double*)d_eps_, (const double*)d_occ_, bk.kP, 0);
+            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T,
+                               A, (const double*)d_eps_, (const double*)d_occ_, kc, 0);
             LAUNCH_CHECK();
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                                         bk.kP, &one, (const double* const*)T, n, (const double* const*)C, n, &zero,
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc,
+                                         &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
                                          (double* const*)P, n, cnt));
             const int nt = (n + 31) / 32;
             hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                                P);
             LAUNCH_CHECK();
+            // ---- Mulliken charges, energy, convergence and mixing
             hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_,
                                d_rowH_);
             LAUNCH_CHECK();
@@ -627,7 +638,7 @@ Status Batch::scc_phase() {
             // Broyden history longer than 32 needs global scratch
             int mmax = 0;
             for (int i : bk.active) mmax = std::max(mmax, std::min(h_state_[i].broyden_iter, broyden_memory_));
-            int beta_ld = std::max(mmax, 1);
+            const int beta_ld = std::max(mmax, 1);
             if (mmax > 32) {
                 size_t need = (size_t)cnt * (beta_ld + 1) * beta_ld;
                 if (need > bk.beta_cap) {
@@ -645,14 +656,23 @@ Status Batch::scc_phase() {
         HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
         for (auto& bk : buckets_) {
+            if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
+                std::vector<int> info(bk.active.size());
+                HIP_TRY(hipMemcpy(info.data(), bk.info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
+                for (int v : info)
+                    if (v != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
+            }
             std::vector<int> next;
             for (int i : bk.active) {
+                if (h_state_[i].kneed_full > bk.kcap)
+                    return Status::error(kErrLinearAlgebra,
+                                         "more partially occupied orbitals than the eigenvector capacity");
+                if (h_state_[i].eig_fail)
+                    return Status::error(kErrLinearAlgebra, "eigenvalue cluster too large for the batched solver");
                 if (h_state_[i].status == 0) next.push_back(i);
-                else h_final_parity_[i] = parity;  // its orbitals sit in the buffer after Cb[parity]
             }
             bk.active.swap(next);
         }
-        ++it;
     }
     for (int i = 0; i < ninst_; ++i)
         if (h_state_[i].status == 2 && !h_err_[i]) {
@@ -665,27 +685,25 @@ Status Batch::scc_phase() {
 Status Batch::gradient_phase() {
     const DeviceTables t = dm_->tables();
     const double one = 1.0, zero = 0.0;
-    // energy-weighted density W = C diag(eps f) C^T of every converged instance
+    // energy-weighted density W = C_k diag(eps f) C_k^T from the orbitals of the
+    // converged iteration (kept in T)
     for (auto& bk : buckets_) {
         std::vector<int> done;
         for (int i : bk.inst)
             if (h_state_[i].status == 1) done.push_back(i);
         const int cnt = (int)done.size();
         if (!cnt) continue;
-        Status st = upload_active(bk, done, -1);
+        Status st = upload_active(bk, done, 0);
         if (!st.ok()) return st;
-        const int n = bk.ld;
-        int kp = 1;
-        for (int i : done) kp = std::max(kp, h_state_[i].kneed);
-        kp = std::min(kp, n);
-        double* const* Cb = bk.ptrs + (size_t)A_C * cnt;  // final orbitals (parity -1 -> recorded)
+        const int n = bk.ld, kc = bk.kcap;
         double* const* T = bk.ptrs + (size_t)A_T * cnt;
+        double* const* A = bk.ptrs + (size_t)A_A * cnt;
         double* const* W = bk.ptrs + (size_t)A_X * cnt;
-        hipLaunchKernelGGL(k_scale_columns, dim3(kp, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, Cb, T,
-                           (const double*)d_eps_, (const double*)d_occ_, kp, 1);
+        hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
+                           (const double*)d_eps_, (const double*)d_occ_, kc, 1);
         LAUNCH_CHECK();
-        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kp, &one,
-                                     (const double* const*)T, n, (const double* const*)Cb, n, &zero,
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc, &one,
+                                     (const double* const*)A, n, (const double* const*)T, n, &zero,
                                      (double* const*)W, n, cnt));
         const int nt = (n + 31) / 32;
         hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, W);

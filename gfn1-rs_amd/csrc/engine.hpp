@@ -108,6 +108,9 @@ class Batch {
         int* info = nullptr;
         double* beta = nullptr;           // Broyden scratch for long histories
         size_t beta_cap = 0;
+        double* tau = nullptr;            // Householder factors (count * ld)
+        double* lu = nullptr;             // inverse-iteration scratch
+        int kcap = 0;                     // eigenvector columns kept per instance
         std::vector<int> active;
         int kP = 0;
     };
@@ -138,7 +141,8 @@ class Batch {
     DeviceBatch db_{};
     int *d_atom0_ = nullptr, *d_sh0_ = nullptr, *d_ao0_ = nullptr, *d_ld_ = nullptr, *d_atom_inst_ = nullptr,
         *d_atom_z_ = nullptr, *d_atom_sh0_ = nullptr, *d_sh_atom_ = nullptr, *d_sh_type_ = nullptr,
-        *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr, *d_sh_by_l_ = nullptr, *d_lofs_ = nullptr;
+        *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr, *d_sh_by_l_ = nullptr, *d_lofs_ = nullptr,
+        *d_nao_ = nullptr;
     int64_t *d_vec0_ = nullptr, *d_broff_ = nullptr;
     double *d_x_ = nullptr, *d_cn_ = nullptr, *d_cnd3_ = nullptr, *d_gw_ = nullptr;
     double *d_qin_ = nullptr, *d_qout_ = nullptr, *d_q0_ = nullptr, *d_vin_g_ = nullptr, *d_vin_t_ = nullptr,
@@ -155,6 +159,7 @@ class Batch {
     hipStream_t stream_ = nullptr;
     bool ran_ = false;
     bool has_d_ = false;  // any d shell in the batch
+    int pad_multiple_ = kPadMultiple;
 
     template <typename T>
     Status dalloc(T** p, size_t count);

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -1478,6 +1479,14 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
             gemm(n, n, n, S.data(), false, prevP->data(), false, sp.data());
             gemm(n, n, n, sp.data(), false, S.data(), false, sps.data());
             for (size_t k = 0; k < (size_t)n * n; ++k) st.fock[k] += lshift * (S[k] - 0.5 * sps[k]);
+        }
+        if (const char* dump = std::getenv("GFN1_ORACLE_DUMP_FOCK")) {
+            // debugging aid: append each iteration's Fock matrix (n x n doubles)
+            FILE* fp = std::fopen(dump, "ab");
+            if (fp) {
+                std::fwrite(st.fock.data(), sizeof(double), st.fock.size(), fp);
+                std::fclose(fp);
+            }
         }
         std::vector<double> fx((size_t)n * n), fo((size_t)n * n), V((size_t)n * n), C((size_t)n * n);
         gemm(n, n, n, st.fock.data(), false, X.data(), false, fx.data());
