@@ -79,48 +79,90 @@ __device__ __forceinline__ double reduce8(const double (&s)[8], int lane) {
     return t1;
 }
 
+// Sums sixteen per-lane values over the wavefront at once.  Lane L receives the
+// total of s[q] with q = 8 * bit5(L) + 4 * bit4(L) + 2 * bit3(L) + bit2(L).
+__device__ __forceinline__ double reduce16(const double (&s)[16], int lane) {
+    const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8, h4 = lane & 4;
+    double t8[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double send = h32 ? s[q] : s[q + 8];
+        t8[q] = (h32 ? s[q + 8] : s[q]) + __shfl_xor(send, 32, 64);
+    }
+    double t4[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double send = h16 ? t8[q] : t8[q + 4];
+        t4[q] = (h16 ? t8[q + 4] : t8[q]) + __shfl_xor(send, 16, 64);
+    }
+    double t2[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double send = h8 ? t4[q] : t4[q + 2];
+        t2[q] = (h8 ? t4[q + 2] : t4[q]) + __shfl_xor(send, 8, 64);
+    }
+    const double send = h4 ? t2[0] : t2[1];
+    double t1 = (h4 ? t2[1] : t2[0]) + __shfl_xor(send, 4, 64);
+    t1 += __shfl_xor(t1, 2, 64);
+    t1 += __shfl_xor(t1, 1, 64);
+    return t1;
+}
+
 // ---------------------------------------------------------------------------
-// Householder tridiagonalisation.  Within a panel of kEigPanel columns the
-// trailing matrix is kept as A - V W^T - W V^T (V: reflectors stored in A,
-// W: scratch); the rank-2b update is applied once per panel.  Lane L of a
-// wavefront owns rows L, L + 64, ... (RPL of them) in the matrix-vector product.
+// Householder tridiagonalisation, one work-group of kTriThreads per matrix
+// (n <= kTriThreads).  Thread r owns row r in the row phases and keeps row r
+// of the current panel's reflectors V and of W in registers, so inside a
+// panel the trailing matrix A - V W^T - W V^T is used without extra memory
+// traffic; the rank-2b update is applied once per panel in 64 x 64 tiles with
+// V and W staged through LDS.  In the matrix-vector product lane L of a
+// wavefront owns rows L, L + 64, ... (RPL of them) and the wavefronts share
+// the columns out in groups of eight.
+constexpr int kTriThreads = 512;
+constexpr int kTriWaves = kTriThreads / 64;
+constexpr int kNB = kEigPanel;
+constexpr int kVS = kNB + 1;  // LDS row stride of the staged panel (odd: no bank conflicts)
+
 template <int RPL>
-__global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ act, const int* __restrict__ nvec, int ld,
-                                                      double* const* Aptr, double* const* Wptr, double* dvec,
-                                                      double* evec, double* tauvec) {
+__global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__ act, const int* __restrict__ nvec,
+                                                         int ld, double* const* Aptr, double* dvec, double* evec,
+                                                         double* tauvec) {
     extern __shared__ double sm[];
     const int l = blockIdx.x;
     const int n = nvec[act[l]];
     double* __restrict__ A = Aptr[l];
-    double* __restrict__ Wp = Wptr[l];
     double* d = dvec + (size_t)l * ld;
     double* e = evec + (size_t)l * ld;
     double* tau = tauvec + (size_t)l * ld;
-    double* xs = sm;                       // column k after the pending updates
-    double* vs = sm + ld;                  // reflector
-    double* ps = sm + 2 * ld;              // tau * (current trailing matrix) * v
-    double* pd = sm + 3 * ld;              // column-dot part of the product
-    double* part = sm + 4 * ld;            // row part, one slice per wavefront
-    double* ys = sm + (4 + kWaves) * ld;   // W^T v, V^T v interleaved
-    double* red = ys + 2 * kEigPanel;      // 16
+    // column steps
+    double* xs = sm;               // column k after the pending panel updates
+    double* vs = sm + ld;          // reflector
+    double* pd = sm + 2 * ld;      // column-dot part of the product
+    double* part = sm + 3 * ld;    // row part, one slice per wavefront
+    // panel end (same memory): staged V and W rows
+    double* Vst = sm;
+    double* Wst = sm + (size_t)kVS * ld;
+    double* ysp = sm + (size_t)2 * kVS * ld;  // per-wavefront W^T v, V^T v
+    double* red = ysp + kTriWaves * 16;       // 16
+    double* rowk = red + 16;                  // V[k, :], W[k, :]
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int k0 = 0; k0 < n - 1; k0 += kEigPanel) {
-        const int b = min(kEigPanel, n - 1 - k0);
+    const int r = tid;
+    double vreg[kNB], wreg[kNB];
+    for (int k0 = 0; k0 < n - 1; k0 += kNB) {
+        const int b = min(kNB, n - 1 - k0);
+#pragma unroll
+        for (int i = 0; i < kNB; ++i) vreg[i] = wreg[i] = 0.0;
         for (int j = 0; j < b; ++j) {
-            const int k = k0 + j;
-            const int c0 = k + 1;
+            const int k = k0 + j, c0 = k + 1;
             // ---- column k of the current matrix, reflector annihilating rows k+2..
-            double nrm = 0.0;
-            for (int r = k + tid; r < n; r += kThreads) {
-                double x = A[r + (size_t)k * ld];
-                for (int i = 0; i < j; ++i) {
-                    const size_t ci = (size_t)(k0 + i) * ld, wi = (size_t)i * ld;
-                    x -= A[r + ci] * Wp[k + wi] + Wp[r + wi] * A[k + ci];
-                }
+            double x = 0.0;
+            if (r >= k && r < n) {
+                x = A[r + (size_t)k * ld];
+#pragma unroll
+                for (int i = 0; i < kNB; ++i)
+                    if (i < j) x -= vreg[i] * rowk[kNB + i] + wreg[i] * rowk[i];
                 xs[r] = x;
-                if (r > c0) nrm += x * x;
             }
-            const double xnorm2 = bsum(nrm, red);
+            const double xnorm2 = bsum((r > c0 && r < n) ? x * x : 0.0, red);
             const double alpha = xs[c0];
             double beta = alpha, tk = 0.0, scale = 0.0;
             if (xnorm2 > 0.0) {
@@ -129,11 +171,15 @@ __global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ ac
                 tk = (beta - alpha) / beta;
                 scale = 1.0 / (alpha - beta);
             }
-            for (int r = c0 + tid; r < n; r += kThreads) {
-                const double v = r == c0 ? 1.0 : xs[r] * scale;
+            double v = 0.0;
+            if (r >= c0 && r < n) {
+                v = r == c0 ? 1.0 : x * scale;
                 vs[r] = v;
                 A[r + (size_t)k * ld] = v;
             }
+#pragma unroll
+            for (int i = 0; i < kNB; ++i)
+                if (i == j) vreg[i] = v;
             if (tid == 0) {
                 d[k] = xs[k];
                 e[k] = beta;
@@ -144,11 +190,11 @@ __global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ ac
             double acc[RPL], vr[RPL];
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
-                const int r = 64 * t + lane;
-                vr[t] = (r >= c0 && r < n) ? vs[r] : 0.0;
+                const int rr = 64 * t + lane;
+                vr[t] = (rr >= c0 && rr < n) ? vs[rr] : 0.0;
                 acc[t] = 0.0;
             }
-            for (int g = c0 + 8 * wid; g < n; g += 8 * kWaves) {
+            for (int g = c0 + 8 * wid; g < n; g += 8 * kTriWaves) {
                 double s[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
@@ -160,10 +206,10 @@ __global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ ac
 #pragma unroll
                         for (int t = 0; t < RPL; ++t) {
                             if (64 * t + 63 < c) continue;
-                            const int r = 64 * t + lane;
-                            const double a = (r >= c && r < n) ? col[r] : 0.0;
+                            const int rr = 64 * t + lane;
+                            const double a = (rr >= c && rr < n) ? col[rr] : 0.0;
                             acc[t] += a * vc;
-                            s[q] += r > c ? a * vr[t] : 0.0;
+                            s[q] += rr > c ? a * vr[t] : 0.0;
                         }
                     }
                 }
@@ -173,55 +219,103 @@ __global__ void __launch_bounds__(kThreads) k_tridiag(const int* __restrict__ ac
             }
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
-                const int r = 64 * t + lane;
-                if (r >= c0 && r < n) part[wid * ld + r] = acc[t];
+                const int rr = 64 * t + lane;
+                if (rr >= c0 && rr < n) part[wid * ld + rr] = acc[t];
             }
-            // ---- panel corrections: ys[2i] = W_i^T v, ys[2i+1] = V_i^T v
-            for (int q = wid; q < 2 * j; q += kWaves) {
-                const int i = q >> 1;
-                const double* src = (q & 1) ? A + (size_t)(k0 + i) * ld : Wp + (size_t)i * ld;
-                double sp = 0.0;
-                for (int r = c0 + lane; r < n; r += 64) sp += src[r] * vs[r];
-                sp = wsum(sp);
-                if (lane == 0) ys[q] = sp;
+            if (j > 0) {
+                double py[16];
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) {
+                    const bool on = i < j && r >= c0 && r < n;
+                    py[i] = on ? wreg[i] * v : 0.0;        // W_i^T v
+                    py[kNB + i] = on ? vreg[i] * v : 0.0;  // V_i^T v
+                }
+                const double tot = reduce16(py, lane);
+                const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
+                                ((lane >> 2) & 1);
+                if ((lane & 3) == 0) ysp[wid * 16 + idx] = tot;
             }
             __syncthreads();
             // ---- p = tau (A v - V W^T v - W V^T v); w = p - tau/2 (p.v) v
-            double pv = 0.0;
-            for (int r = c0 + tid; r < n; r += kThreads) {
-                double pr = pd[r];
+            double p = 0.0;
+            if (r >= c0 && r < n) {
+                p = pd[r];
 #pragma unroll
-                for (int w = 0; w < kWaves; ++w) pr += part[w * ld + r];
-                for (int i = 0; i < j; ++i)
-                    pr -= A[r + (size_t)(k0 + i) * ld] * ys[2 * i] + Wp[r + (size_t)i * ld] * ys[2 * i + 1];
-                pr *= tk;
-                ps[r] = pr;
-                pv += pr * vs[r];
+                for (int w = 0; w < kTriWaves; ++w) p += part[w * ld + r];
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) {
+                    if (i < j) {
+                        double y1 = 0.0, y2 = 0.0;
+#pragma unroll
+                        for (int w = 0; w < kTriWaves; ++w) {
+                            y1 += ysp[w * 16 + i];
+                            y2 += ysp[w * 16 + kNB + i];
+                        }
+                        p -= vreg[i] * y1 + wreg[i] * y2;
+                    }
+                }
+                p *= tk;
             }
-            const double a2 = -0.5 * tk * bsum(pv, red);
-            for (int r = c0 + tid; r < n; r += kThreads) Wp[r + (size_t)j * ld] = ps[r] + a2 * vs[r];
+            const double a2 = -0.5 * tk * bsum(p * v, red);
+            const double w = p + a2 * v;
+#pragma unroll
+            for (int i = 0; i < kNB; ++i)
+                if (i == j) wreg[i] = w;
+            if (r == k + 1 && j + 1 < b) {
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) {
+                    rowk[i] = vreg[i];
+                    rowk[kNB + i] = wreg[i];
+                }
+            }
             __syncthreads();
         }
-        // ---- rank-2b update of the trailing lower triangle (columns >= k0 + b)
+        // ---- rank-2b update of the trailing lower triangle (columns >= cs)
         const int cs = k0 + b;
         if (cs < n) {
-            for (int t = cs / 64; t < RPL; ++t) {
-                if (64 * t >= n) break;
-                const int r = 64 * t + lane;
-                const bool rv = r >= cs && r < n;
-                double vr2[kEigPanel], wr2[kEigPanel];
+            if (r < n) {
 #pragma unroll
-                for (int i = 0; i < kEigPanel; ++i) {
-                    vr2[i] = (rv && i < b) ? A[r + (size_t)(k0 + i) * ld] : 0.0;
-                    wr2[i] = (rv && i < b) ? Wp[r + (size_t)i * ld] : 0.0;
+                for (int i = 0; i < kNB; ++i) {
+                    Vst[(size_t)r * kVS + i] = vreg[i];
+                    Wst[(size_t)r * kVS + i] = wreg[i];
                 }
-                const int cend = min(64 * t + 64, n);
-                for (int c = cs + wid; c < cend; c += kWaves) {
-                    double upd = 0.0;
+            }
+            __syncthreads();
+            const int tb0 = cs / 64, nb = (n + 63) / 64 - tb0;
+            const int ntile = nb * (nb + 1) / 2;
+            for (int q = wid; q < ntile; q += kTriWaves) {
+                int t = 0, u = q;
+                while (u > t) {
+                    u -= t + 1;
+                    ++t;
+                }
+                const int rr = 64 * (tb0 + t) + lane;
+                const bool rv = rr >= cs && rr < n;
+                double vr2[kNB], wr2[kNB];
 #pragma unroll
-                    for (int i = 0; i < kEigPanel; ++i)
-                        if (i < b) upd += vr2[i] * Wp[c + (size_t)i * ld] + wr2[i] * A[c + (size_t)(k0 + i) * ld];
-                    if (rv && r >= c) A[r + (size_t)c * ld] -= upd;
+                for (int i = 0; i < kNB; ++i) {
+                    vr2[i] = rv ? Vst[(size_t)rr * kVS + i] : 0.0;
+                    wr2[i] = rv ? Wst[(size_t)rr * kVS + i] : 0.0;
+                }
+                const int cb = max(cs, 64 * (tb0 + u)), ce = min(64 * (tb0 + u) + 64, n);
+                for (int c1 = cb; c1 < ce; c1 += 16) {
+                    double a[16];
+#pragma unroll
+                    for (int q2 = 0; q2 < 16; ++q2) {
+                        const int c = c1 + q2;
+                        a[q2] = (rv && c < ce && c <= rr) ? A[rr + (size_t)c * ld] : 0.0;
+                    }
+#pragma unroll
+                    for (int q2 = 0; q2 < 16; ++q2) {
+                        const int c = c1 + q2;
+                        if (c < ce) {
+                            double upd = 0.0;
+#pragma unroll
+                            for (int i = 0; i < kNB; ++i)
+                                upd += vr2[i] * Wst[(size_t)c * kVS + i] + wr2[i] * Vst[(size_t)c * kVS + i];
+                            if (rv && c <= rr) A[rr + (size_t)c * ld] = a[q2] - upd;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -736,19 +830,31 @@ __global__ void __launch_bounds__(512) k_backtransform(const int* act, const int
 
 // ---------------------------------------------------------------------------
 hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* nvec, int ld, double* const* A,
-                          double* const* Wp, double* d, double* e, double* tau) {
-    if (ld > kEigMaxDim) return hipErrorInvalidValue;
-    const size_t lds = ((size_t)(4 + kWaves) * ld + 2 * kEigPanel + 16) * sizeof(double);
-    const dim3 grid(count), block(kThreads);
+                          double* d, double* e, double* tau) {
+    if (ld > kEigMaxDim || ld > kTriThreads) return hipErrorInvalidValue;
+    const size_t lds = ((size_t)2 * kVS * ld + kTriWaves * 16 + 16 + 2 * kNB) * sizeof(double);
+    const int lds_max = (int)(((size_t)2 * kVS * kEigMaxDim + kTriWaves * 16 + 16 + 2 * kNB) * sizeof(double));
+    static bool attr = false;  // LDS above 64 KiB must be requested explicitly
+    if (!attr) {
+        const void* fns[] = {(const void*)k_tridiag<1>, (const void*)k_tridiag<2>, (const void*)k_tridiag<3>,
+                             (const void*)k_tridiag<4>, (const void*)k_tridiag<5>, (const void*)k_tridiag<6>,
+                             (const void*)k_tridiag<7>, (const void*)k_tridiag<8>};
+        for (const void* f : fns) {
+            hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+            if (err != hipSuccess) return err;
+        }
+        attr = true;
+    }
+    const dim3 grid(count), block(kTriThreads);
     switch ((ld + 63) / 64) {
-        case 1: hipLaunchKernelGGL(k_tridiag<1>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 2: hipLaunchKernelGGL(k_tridiag<2>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 3: hipLaunchKernelGGL(k_tridiag<3>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 4: hipLaunchKernelGGL(k_tridiag<4>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 5: hipLaunchKernelGGL(k_tridiag<5>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 6: hipLaunchKernelGGL(k_tridiag<6>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        case 7: hipLaunchKernelGGL(k_tridiag<7>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
-        default: hipLaunchKernelGGL(k_tridiag<8>, grid, block, lds, s, act, nvec, ld, A, Wp, d, e, tau); break;
+        case 1: hipLaunchKernelGGL(k_tridiag<1>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 2: hipLaunchKernelGGL(k_tridiag<2>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 3: hipLaunchKernelGGL(k_tridiag<3>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 4: hipLaunchKernelGGL(k_tridiag<4>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 5: hipLaunchKernelGGL(k_tridiag<5>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 6: hipLaunchKernelGGL(k_tridiag<6>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        case 7: hipLaunchKernelGGL(k_tridiag<7>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
+        default: hipLaunchKernelGGL(k_tridiag<8>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
     }
     return hipGetLastError();
 }

@@ -15,15 +15,14 @@
 namespace gfn1 {
 
 constexpr int kEigMaxDim = 512;     // largest matrix the kernels accept
-constexpr int kEigPanel = 16;       // Householder panel width (columns of the W scratch)
+constexpr int kEigPanel = 8;        // Householder panel width
 constexpr int kEigMaxCluster = 32;  // largest tight eigenvalue cluster resolved by Rayleigh-Ritz
 
 // Householder reduction of the lower triangle of A[l] (n = nvec[act[l]]).  On
 // exit d, e hold the tridiagonal matrix, A[l] column k rows k+1.. holds the
 // reflector v_k (v_k[k+1] = 1 stored explicitly) and tau[k] its factor.
-// Wp[l] is an ld x kEigPanel scratch matrix.
 hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* nvec, int ld, double* const* A,
-                          double* const* Wp, double* d, double* e, double* tau);
+                          double* d, double* e, double* tau);
 
 // Eigenvalues of the tridiagonal matrices, occupations (writes eps, occ, mu, ts,
 // kneed into the instance state) and the first min(kneed, kcap) eigenvectors of

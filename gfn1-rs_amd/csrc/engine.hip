@@ -590,8 +590,7 @@ Status Batch::scc_phase() {
             if (own_solver) {
                 // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
                 // (returned transposed in Z, kc columns)
-                double* const* Wp = bk.ptrs + (size_t)A_X * cnt;
-                HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, Wp, bk.eig,
+                HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, bk.eig,
                                        bk.eig_e, bk.tau));
                 HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
                                            (const double*)bk.eig, (const double*)bk.eig_e, d_eps_,
