@@ -9,6 +9,13 @@
 
 namespace gfn1 {
 
+// Pointers fetched from device-side pointer arrays are generic to the compiler,
+// which then emits FLAT accesses that also count against the LDS wait counter.
+// Casting them to the global address space gives plain global loads/stores.
+#define GFN1_GLOBAL __attribute__((address_space(1)))
+template <class T>
+using gptr = GFN1_GLOBAL T*;
+
 // Padded AO dimension of an instance's dense matrices; padded rows/columns carry
 // an identity overlap and a diagonal Fock value of kPadEnergy, so they decouple
 // exactly and their eigenvalues sort above every physical orbital.

@@ -35,9 +35,28 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kBT = 16;  // reflectors per compact-WY block in the back-transformation
 
+// Value of v held by the lane that the DPP control CTRL selects (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// partners: lane ^ 1, lane ^ 2 (quad permutes), 7 - lane within each group of
+// eight (opposite bit 2), lane ^ 8 (rotation of a 16-lane row by 8)
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppRor8 = 0x128;
+
+// wavefront sum, identical in every lane (butterfly: each stage adds the same
+// two partial sums in every lane)
 __device__ __forceinline__ double wsum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return __shfl(v, 0, 64);
+    v += dpp<kDppXor1>(v);
+    v += dpp<kDppXor2>(v);
+    v += dpp<kDppHalfMirror>(v);
+    v += dpp<kDppRor8>(v);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
 }
 
 // deterministic work-group sum; every thread gets the result
@@ -52,60 +71,70 @@ __device__ double bsum(double v, double* red) {
     return s;
 }
 
-// Sums eight per-lane values over the wavefront at once (10 exchanges instead
-// of 8 x 6).  Lane L receives the total of s[q] with
-// q = 4 * bit5(L) + 2 * bit4(L) + bit3(L).
-__device__ __forceinline__ double reduce8(const double (&s)[8], int lane) {
-    const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
-    double t4[4];
+// Transposed wavefront sums: N per-lane values are halved at every exchange
+// (each lane keeps the half its partner sends away), so one pass of log2(64)
+// exchanges leaves each lane with one finished total.  The first four
+// exchanges are DPP moves inside a 16-lane row; only bits 4 and 5 of the lane
+// index go through the cross-row permute.
+//
+// reduce32: lane L receives the total of s[q], q = 16 b2 + 8 b1 + 4 b0 + 2 b3 + b4
+// (bk = bit k of L); lanes L and L + 32 hold the same total.  s is overwritten.
+__device__ __forceinline__ double reduce32(double (&s)[32], int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8, b4 = lane & 16;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const double send = h32 ? s[q] : s[q + 4];
-        const double keep = h32 ? s[q + 4] : s[q];
-        t4[q] = keep + __shfl_xor(send, 32, 64);
+    for (int q = 0; q < 16; ++q) {
+        const double send = b2 ? s[q] : s[q + 16];
+        s[q] = (b2 ? s[q + 16] : s[q]) + dpp<kDppHalfMirror>(send);
     }
-    double t2[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const double send = h16 ? t4[q] : t4[q + 2];
-        const double keep = h16 ? t4[q + 2] : t4[q];
-        t2[q] = keep + __shfl_xor(send, 16, 64);
-    }
-    const double send = h8 ? t2[0] : t2[1];
-    double t1 = (h8 ? t2[1] : t2[0]) + __shfl_xor(send, 8, 64);
-    t1 += __shfl_xor(t1, 4, 64);
-    t1 += __shfl_xor(t1, 2, 64);
-    t1 += __shfl_xor(t1, 1, 64);
-    return t1;
-}
-
-// Sums sixteen per-lane values over the wavefront at once.  Lane L receives the
-// total of s[q] with q = 8 * bit5(L) + 4 * bit4(L) + 2 * bit3(L) + bit2(L).
-__device__ __forceinline__ double reduce16(const double (&s)[16], int lane) {
-    const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8, h4 = lane & 4;
-    double t8[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const double send = h32 ? s[q] : s[q + 8];
-        t8[q] = (h32 ? s[q + 8] : s[q]) + __shfl_xor(send, 32, 64);
+        const double send = b1 ? s[q] : s[q + 8];
+        s[q] = (b1 ? s[q + 8] : s[q]) + dpp<kDppXor2>(send);
     }
-    double t4[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const double send = h16 ? t8[q] : t8[q + 4];
-        t4[q] = (h16 ? t8[q + 4] : t8[q]) + __shfl_xor(send, 16, 64);
+        const double send = b0 ? s[q] : s[q + 4];
+        s[q] = (b0 ? s[q + 4] : s[q]) + dpp<kDppXor1>(send);
     }
-    double t2[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const double send = h8 ? t4[q] : t4[q + 2];
-        t2[q] = (h8 ? t4[q + 2] : t4[q]) + __shfl_xor(send, 8, 64);
+        const double send = b3 ? s[q] : s[q + 2];
+        s[q] = (b3 ? s[q + 2] : s[q]) + dpp<kDppRor8>(send);
     }
-    const double send = h4 ? t2[0] : t2[1];
-    double t1 = (h4 ? t2[1] : t2[0]) + __shfl_xor(send, 4, 64);
-    t1 += __shfl_xor(t1, 2, 64);
-    t1 += __shfl_xor(t1, 1, 64);
-    return t1;
+    const double send = b4 ? s[0] : s[1];
+    double t = (b4 ? s[1] : s[0]) + __shfl_xor(send, 16, 64);
+    return t + __shfl_xor(t, 32, 64);
+}
+__device__ __forceinline__ int reduce32_index(int lane) {
+    return ((lane >> 2) & 1) * 16 + ((lane >> 1) & 1) * 8 + (lane & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 4) & 1);
+}
+
+// reduce16: lane L receives the total of s[q], q = 8 b2 + 4 b1 + 2 b0 + b3; every
+// lane with the same low four bits holds the same total.  s is overwritten.
+__device__ __forceinline__ double reduce16(double (&s)[16], int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double send = b2 ? s[q] : s[q + 8];
+        s[q] = (b2 ? s[q + 8] : s[q]) + dpp<kDppHalfMirror>(send);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double send = b1 ? s[q] : s[q + 4];
+        s[q] = (b1 ? s[q + 4] : s[q]) + dpp<kDppXor2>(send);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double send = b0 ? s[q] : s[q + 2];
+        s[q] = (b0 ? s[q + 2] : s[q]) + dpp<kDppXor1>(send);
+    }
+    const double send = b3 ? s[0] : s[1];
+    double t = (b3 ? s[1] : s[0]) + dpp<kDppRor8>(send);
+    t += __shfl_xor(t, 16, 64);
+    return t + __shfl_xor(t, 32, 64);
+}
+__device__ __forceinline__ int reduce16_index(int lane) {
+    return ((lane >> 2) & 1) * 8 + ((lane >> 1) & 1) * 4 + (lane & 1) * 2 + ((lane >> 3) & 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -114,30 +143,27 @@ __device__ __forceinline__ double reduce16(const double (&s)[16], int lane) {
 // of the current panel's reflectors V and of W in registers, so inside a
 // panel the trailing matrix A - V W^T - W V^T is used without extra memory
 // traffic; the rank-2b update is applied once per panel in 64 x 64 tiles with
-// V and W staged through LDS.  In the matrix-vector product lane L of a
-// wavefront owns rows L, L + 64, ... (RPL of them) and the wavefronts share
-// the columns out in groups of eight.
+// V and W staged through LDS.  The matrix-vector product reads the lower
+// triangle once, in 64 x 64 tiles shared out over the wavefronts.
 constexpr int kTriThreads = 512;
 constexpr int kTriWaves = kTriThreads / 64;
 constexpr int kNB = kEigPanel;
 constexpr int kVS = kNB + 1;  // LDS row stride of the staged panel (odd: no bank conflicts)
 
-template <int RPL>
 __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__ act, const int* __restrict__ nvec,
                                                          int ld, double* const* Aptr, double* dvec, double* evec,
                                                          double* tauvec) {
     extern __shared__ double sm[];
     const int l = blockIdx.x;
     const int n = nvec[act[l]];
-    double* __restrict__ A = Aptr[l];
+    gptr<double> A = (gptr<double>)Aptr[l];
     double* d = dvec + (size_t)l * ld;
     double* e = evec + (size_t)l * ld;
     double* tau = tauvec + (size_t)l * ld;
     // column steps
     double* xs = sm;               // column k after the pending panel updates
     double* vs = sm + ld;          // reflector
-    double* pd = sm + 2 * ld;      // column-dot part of the product
-    double* part = sm + 3 * ld;    // row part, one slice per wavefront
+    double* part = sm + 2 * ld;    // A v, one partial slice per wavefront
     // panel end (same memory): staged V and W rows
     double* Vst = sm;
     double* Wst = sm + (size_t)kVS * ld;
@@ -186,41 +212,47 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                 tau[k] = tk;
             }
             __syncthreads();
-            // ---- A_panel v over the lower triangle of the trailing block
-            double acc[RPL], vr[RPL];
-#pragma unroll
-            for (int t = 0; t < RPL; ++t) {
-                const int rr = 64 * t + lane;
-                vr[t] = (rr >= c0 && rr < n) ? vs[rr] : 0.0;
-                acc[t] = 0.0;
-            }
-            for (int g = c0 + 8 * wid; g < n; g += 8 * kTriWaves) {
-                double s[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    s[q] = 0.0;
-                    const int c = g + q;
-                    if (c < n) {
-                        const double vc = vs[c];
-                        const double* col = A + (size_t)c * ld;
-#pragma unroll
-                        for (int t = 0; t < RPL; ++t) {
-                            if (64 * t + 63 < c) continue;
-                            const int rr = 64 * t + lane;
-                            const double a = (rr >= c && rr < n) ? col[rr] : 0.0;
-                            acc[t] += a * vc;
-                            s[q] += rr > c ? a * vr[t] : 0.0;
-                        }
+            // ---- A_panel v over the lower triangle of the trailing block, in 64 x 32
+            //      tiles (rows c0 + 64t + lane, columns c0 + 32u .. +31, u <= 2t + 1);
+            //      the row and column contributions of a wavefront's tiles go to its
+            //      own slice of part, so the sum is deterministic
+            {
+                double* slice = part + wid * ld;
+                for (int rr = c0 + lane; rr < n; rr += 64) slice[rr] = 0.0;
+                const int mt = (n - c0 + 63) >> 6, mu = (n - c0 + 31) >> 5;
+                int t = 0, u = wid;  // tile q = wid, wid + kTriWaves, ... in row-block order
+                for (;;) {
+                    while (t < mt && u >= min(2 * t + 2, mu)) {
+                        u -= min(2 * t + 2, mu);
+                        ++t;
                     }
-                }
-                const double tot = reduce8(s, lane);
-                const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-                if ((lane & 7) == 0 && g + idx < n) pd[g + idx] = tot;
-            }
+                    if (t >= mt) break;
+                    const int rr = c0 + 64 * t + lane;
+                    const int cb = c0 + 32 * u;
+                    const bool rowok = rr < n;
+                    double a[32];
 #pragma unroll
-            for (int t = 0; t < RPL; ++t) {
-                const int rr = 64 * t + lane;
-                if (rr >= c0 && rr < n) part[wid * ld + rr] = acc[t];
+                    for (int qq = 0; qq < 32; ++qq) {
+                        const int c = cb + qq;
+                        const bool ok = rowok && c < n && rr >= c;
+                        const double val = A[ok ? rr + (size_t)c * ld : 0];
+                        a[qq] = ok ? val : 0.0;
+                    }
+                    const double vr = rowok ? vs[rr] : 0.0;
+                    double acc = 0.0;
+#pragma unroll
+                    for (int qq = 0; qq < 32; ++qq) {
+                        const int c = cb + qq;
+                        const double vc = vs[min(c, n - 1)];  // a[qq] = 0 for c >= n
+                        acc += a[qq] * vc;
+                        a[qq] = rr > c ? a[qq] * vr : 0.0;
+                    }
+                    const double tot = reduce32(a, lane);
+                    const int c = cb + reduce32_index(lane);
+                    if (lane < 32 && c < n) slice[c] += tot;
+                    if (rowok) slice[rr] += acc;
+                    u += kTriWaves;
+                }
             }
             if (j > 0) {
                 double py[16];
@@ -231,15 +263,12 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                     py[kNB + i] = on ? vreg[i] * v : 0.0;  // V_i^T v
                 }
                 const double tot = reduce16(py, lane);
-                const int idx = ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 +
-                                ((lane >> 2) & 1);
-                if ((lane & 3) == 0) ysp[wid * 16 + idx] = tot;
+                if (lane < 16) ysp[wid * 16 + reduce16_index(lane)] = tot;
             }
             __syncthreads();
             // ---- p = tau (A v - V W^T v - W V^T v); w = p - tau/2 (p.v) v
             double p = 0.0;
             if (r >= c0 && r < n) {
-                p = pd[r];
 #pragma unroll
                 for (int w = 0; w < kTriWaves; ++w) p += part[w * ld + r];
 #pragma unroll
@@ -594,7 +623,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         }
     }
     __syncthreads();
-    double* __restrict__ Z = Ztptr[l];
+    gptr<double> Z = (gptr<double>)Ztptr[l];
     const size_t fld = (size_t)ld * kcap;
     double* __restrict__ rU1 = lu + (size_t)l * 5 * fld;  // 1 / diagonal of U
     double* __restrict__ U2 = rU1 + fld;                  // first superdiagonal of U
@@ -692,10 +721,10 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
             if (tid == 0) s.eig_fail = 1;
         } else if (cs >= 2) {
             for (int a = 0; a < cs; ++a) {
-                double* za = Z + g0 + a;
+                gptr<double> za = Z + g0 + a;
                 for (int pass = 0; pass < 2; ++pass)
                     for (int b2 = 0; b2 < a; ++b2) {
-                        const double* zb = Z + g0 + b2;
+                        gptr<const double> zb = Z + g0 + b2;
                         double p = 0.0;
                         for (int r = tid; r < n; r += nt) p += zb[(size_t)r * kcap] * za[(size_t)r * kcap];
                         p = bsum(p, red);
@@ -715,8 +744,8 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
                     ++a;
                 }
                 const int b2 = a + rem;
-                const double* za = Z + g0 + a;
-                const double* zb = Z + g0 + b2;
+                gptr<const double> za = Z + g0 + a;
+                gptr<const double> zb = Z + g0 + b2;
                 double h = 0.0;
                 for (int r = 0; r < n; ++r) {
                     double tz = D[r] * zb[(size_t)r * kcap];
@@ -731,7 +760,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
             if (tid == 0) jacobi_small(Hc, Rc, cs, KC);
             __syncthreads();
             for (int r = tid; r < n; r += nt) {
-                double* zr = Z + (size_t)r * kcap + g0;
+                gptr<double> zr = Z + (size_t)r * kcap + g0;
                 double* tr = rU1 + (size_t)r * kcap + g0;
                 for (int b2 = 0; b2 < cs; ++b2) {
                     double v = 0.0;
@@ -763,9 +792,9 @@ __global__ void __launch_bounds__(512) k_backtransform(const int* act, const int
     const int n = nvec[inst];
     const int k = st[inst].kneed;
     if (k == 0) return;
-    const double* A = Aptr[l];
+    gptr<const double> A = (gptr<const double>)Aptr[l];
     const double* tau = tauvec + (size_t)l * ld;
-    double* __restrict__ Z = Ztptr[l];
+    gptr<double> Z = (gptr<double>)Ztptr[l];
     const int j = threadIdx.x, nt = blockDim.x;
     const bool on = j < k;
     for (int hi = n - 1; hi > 0; hi -= kBT) {
@@ -836,26 +865,12 @@ hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* n
     const int lds_max = (int)(((size_t)2 * kVS * kEigMaxDim + kTriWaves * 16 + 16 + 2 * kNB) * sizeof(double));
     static bool attr = false;  // LDS above 64 KiB must be requested explicitly
     if (!attr) {
-        const void* fns[] = {(const void*)k_tridiag<1>, (const void*)k_tridiag<2>, (const void*)k_tridiag<3>,
-                             (const void*)k_tridiag<4>, (const void*)k_tridiag<5>, (const void*)k_tridiag<6>,
-                             (const void*)k_tridiag<7>, (const void*)k_tridiag<8>};
-        for (const void* f : fns) {
-            hipError_t err = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-            if (err != hipSuccess) return err;
-        }
+        hipError_t err = hipFuncSetAttribute((const void*)k_tridiag, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             lds_max);
+        if (err != hipSuccess) return err;
         attr = true;
     }
-    const dim3 grid(count), block(kTriThreads);
-    switch ((ld + 63) / 64) {
-        case 1: hipLaunchKernelGGL(k_tridiag<1>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 2: hipLaunchKernelGGL(k_tridiag<2>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 3: hipLaunchKernelGGL(k_tridiag<3>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 4: hipLaunchKernelGGL(k_tridiag<4>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 5: hipLaunchKernelGGL(k_tridiag<5>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 6: hipLaunchKernelGGL(k_tridiag<6>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        case 7: hipLaunchKernelGGL(k_tridiag<7>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-        default: hipLaunchKernelGGL(k_tridiag<8>, grid, block, lds, s, act, nvec, ld, A, d, e, tau); break;
-    }
+    hipLaunchKernelGGL(k_tridiag, dim3(count), dim3(kTriThreads), lds, s, act, nvec, ld, A, d, e, tau);
     return hipGetLastError();
 }
 

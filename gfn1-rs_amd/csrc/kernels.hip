@@ -276,9 +276,9 @@ __global__ void k_pad_diagonal(DeviceBatch b) {
     const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
     for (int i = n + threadIdx.x; i < ld; i += blockDim.x) {
         const size_t idx = (size_t)i * ld + i;
-        b.S[inst][idx] = 1.0;
-        b.H0[inst][idx] = kPadEnergy;
-        b.F[inst][idx] = kPadEnergy;
+        ((gptr<double>)b.S[inst])[idx] = 1.0;
+        ((gptr<double>)b.H0[inst])[idx] = kPadEnergy;
+        ((gptr<double>)b.F[inst])[idx] = kPadEnergy;
     }
 }
 
@@ -288,8 +288,8 @@ __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, do
     const int inst = act[blockIdx.y];
     const int ld = b.ld[inst];
     const int n = b.ao0[inst + 1] - b.ao0[inst];
-    double* cb = b.CB0[inst];
-    const double* li = Linv[blockIdx.y];
+    gptr<double> cb = (gptr<double>)b.CB0[inst];
+    gptr<const double> li = (gptr<const double>)Linv[blockIdx.y];
     const size_t total = (size_t)ld * ld;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (size_t)gridDim.x * blockDim.x) {
         const int row = (int)(k % ld), col = (int)(k / ld);
@@ -345,9 +345,9 @@ __global__ void k_fock(DeviceBatch b, const int* act, const double* vt) {
     if (col >= n) return;
     const int a0 = b.ao0[inst];
     const double vc = vt[b.ao_sh[a0 + col]];
-    const double* S = b.S[inst] + (size_t)col * ld;
-    const double* H = b.H0[inst] + (size_t)col * ld;
-    double* F = b.F[inst] + (size_t)col * ld;
+    gptr<const double> S = (gptr<const double>)b.S[inst] + (size_t)col * ld;
+    gptr<const double> H = (gptr<const double>)b.H0[inst] + (size_t)col * ld;
+    gptr<double> F = (gptr<double>)b.F[inst] + (size_t)col * ld;
     for (int row = threadIdx.x; row < n; row += blockDim.x) {
         const double vr = vt[b.ao_sh[a0 + row]];
         F[row] = H[row] + (-S[row] * 0.5 * (vr + vc));
@@ -361,9 +361,9 @@ __global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift
     const int col = blockIdx.x;
     if (col >= n) return;
     const double ls = shift[blockIdx.y];
-    const double* S = b.S[inst] + (size_t)col * ld;
-    const double* M = SPS[blockIdx.y] + (size_t)col * ld;
-    double* F = b.F[inst] + (size_t)col * ld;
+    gptr<const double> S = (gptr<const double>)b.S[inst] + (size_t)col * ld;
+    gptr<const double> M = (gptr<const double>)SPS[blockIdx.y] + (size_t)col * ld;
+    gptr<double> F = (gptr<double>)b.F[inst] + (size_t)col * ld;
     for (int row = threadIdx.x; row < n; row += blockDim.x) F[row] += ls * (S[row] - 0.5 * M[row]);
 }
 
@@ -482,8 +482,8 @@ __global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C,
     if (col >= k) return;
     const double f = occ_all[b.vec0[inst] + col];
     const double w = weighted ? eps_all[b.vec0[inst] + col] * f : f;
-    const double* c = C[blockIdx.y] + (size_t)col * ld;
-    double* o = T[blockIdx.y] + (size_t)col * ld;
+    gptr<const double> c = (gptr<const double>)C[blockIdx.y] + (size_t)col * ld;
+    gptr<double> o = (gptr<double>)T[blockIdx.y] + (size_t)col * ld;
     for (int row = threadIdx.x; row < ld; row += blockDim.x) o[row] = row < n ? c[row] * w : 0.0;
 }
 
@@ -495,7 +495,7 @@ __global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp) {
     const int bi = blockIdx.x, bj = blockIdx.y;
     if (bj > bi) return;
     if (bi * 32 >= n) return;
-    double* M = Mp[blockIdx.z];
+    gptr<double> M = (gptr<double>)Mp[blockIdx.z];
     const int tx = threadIdx.x & 31, ty0 = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
     for (int ty = ty0; ty < 32; ty += 8) {
         const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;  // tile (bi, bj): rows bi
@@ -521,9 +521,9 @@ __global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* r
     const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= n) return;
-    const double* S = b.S[inst] + (size_t)col * ld;
-    const double* H = b.H0[inst] + (size_t)col * ld;
-    const double* P = b.P[inst] + (size_t)col * ld;
+    gptr<const double> S = (gptr<const double>)b.S[inst] + (size_t)col * ld;
+    gptr<const double> H = (gptr<const double>)b.H0[inst] + (size_t)col * ld;
+    gptr<const double> P = (gptr<const double>)b.P[inst] + (size_t)col * ld;
     double ps = 0.0, ph = 0.0;
     for (int row = threadIdx.x; row < n; row += blockDim.x) {
         ps += S[row] * P[row];
@@ -988,7 +988,7 @@ __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int
     const int za = b.atom_z[a];
     const int nsa = t.elements[za].nshell;
     const int sa0 = b.atom_sh0[a];
-    const double* P = b.P[inst];
+    gptr<const double> P = (gptr<const double>)b.P[inst];
     if (cn_on) {
         double d = dedcn_h[a];
         for (int k = 0; k < nsa; ++k) {
@@ -1120,7 +1120,7 @@ __device__ __forceinline__ void shell_block(const ShellType& A, const ShellType&
 // S and H0 for one (shell of A, shell of B) pair with B <= A; writes both triangles.
 template <int LA, int LB>
 __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn, double c2,
-                                          int a, int sa, int sb, int ld, double* S, double* H) {
+                                          int a, int sa, int sb, int ld, gptr<double> S, gptr<double> H) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
@@ -1171,7 +1171,7 @@ __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTabl
 template <int LA, int LB>
 __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn,
                                           const double* vsh, int scc_on, int cn_on, int a, int sa, int sb, int ld,
-                                          const double* P, const double* W, double* g, double& dc) {
+                                          gptr<const double> P, gptr<const double> W, double* g, double& dc) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
@@ -1193,8 +1193,8 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
     double oc[NA * NB], pw[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
-        const double* Pc = P + (size_t)(mu0 + i) * ld + nu0;
-        const double* Wc = W + (size_t)(mu0 + i) * ld + nu0;
+        gptr<const double> Pc = P + (size_t)(mu0 + i) * ld + nu0;
+        gptr<const double> Wc = W + (size_t)(mu0 + i) * ld + nu0;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const double p = Pc[j];
@@ -1295,8 +1295,8 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
     const int a = blockIdx.x;
     const int inst = b.atom_inst[a];
     const int ld = b.ld[inst];
-    double* S = b.S[inst];
-    double* H = b.H0[inst];
+    gptr<double> S = (gptr<double>)b.S[inst];
+    gptr<double> H = (gptr<double>)b.H0[inst];
     const double c2 = cutoff * cutoff;
     const int sa0 = b.atom_sh0[a];
     const int nsha = t.elements[b.atom_z[a]].nshell;
@@ -1342,8 +1342,8 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     const int a = blockIdx.x;
     const int inst = b.atom_inst[a];
     const int ld = b.ld[inst];
-    const double* P = b.P[inst];
-    const double* W = b.W[inst];
+    gptr<const double> P = (gptr<const double>)b.P[inst];
+    gptr<const double> W = (gptr<const double>)b.W[inst];
     const double c2 = cutoff * cutoff;
     const int sa0 = b.atom_sh0[a];
     const int nsha = t.elements[b.atom_z[a]].nshell;
