@@ -23,6 +23,21 @@ import numpy as np  # noqa: E402
 
 METRIC = "SCC energy+gradient molecules/sec (batched, ML guess+mixed vs plain FP64)"
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tridiag_traffic_batch4096.txt")
+
+
+def measured_traffic():
+    """HBM bytes per k_tridiag launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def dense_flops(n, iters):
@@ -32,6 +47,19 @@ def dense_flops(n, iters):
     return iters * 19.0 * n ** 3 + 11.0 * n ** 3
 
 
+def calibrate_copy():
+    """Plain streaming kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE
+    counters: y = 1.0 * x over 2^27 doubles (1 GiB read, 1 GiB written)."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "elementwise mul", "bytes_read": x.numel() * 8,
+                      "bytes_written": y.numel() * 8, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,9 +67,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--eigensolver", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=24, help="instances timed on the host oracle (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="instances timed on the host oracle (0 = skip)")
     ap.add_argument("--check", type=int, default=4, help="instances checked against the oracle")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB read + 1 GiB write elementwise kernel (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU; every run here has exactly one device (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
@@ -60,17 +93,24 @@ def main():
         eng.run()
         print("warmup %d done: %s" % (w, eng.timings()), file=sys.stderr, flush=True)
     times = []
+    tri_ms, tri_n, tri_bytes = 0.0, 0, 0.0
     for s in range(args.steps):
         t = time.perf_counter()
         eng.run()
         times.append(time.perf_counter() - t)
+        k_ms, k_n, k_b = eng.kernel_stats()
+        tri_ms, tri_n, tri_bytes = tri_ms + k_ms, tri_n + k_n, tri_bytes + k_b
         print("step %d: %.3fs %s" % (s, times[-1], eng.timings()), file=sys.stderr, flush=True)
     ms = 1000.0 * float(np.mean(times))
     value = len(systems) / (ms / 1000.0)
     status = np.array([eng.status(i) for i in range(len(systems))])
     iters = np.array([eng.iterations(i) for i in range(len(systems))])
     flops = float(sum(dense_flops(n, it) for n, it in zip(nao, iters)))
-    achieved = flops / (ms / 1000.0) / 1e12
+    dense_tflops = flops / (ms / 1000.0) / 1e12
+    # dominant kernel: tridiagonal reduction, algorithmic bytes / mean launch time
+    tri_avg_ms = tri_ms / max(tri_n, 1)
+    tri_bytes_per_launch = tri_bytes / max(tri_n, 1)
+    achieved = tri_bytes_per_launch / (tri_avg_ms / 1000.0) / 1e9 if tri_n else 0.0
 
     # parity spot check against the oracle
     check = {}
@@ -103,9 +143,16 @@ def main():
                    "global_batch": len(systems), "ao_min": int(nao.min()), "ao_max": int(nao.max()),
                    "ao_mean": float(nao.mean()), "path": "plain FP64", "parallelism": "single GPU"},
         "converged": int((status == 0).sum()), "mean_scc_iterations": float(iters.mean()),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "whole-pass dense FP64 work (19n^3 per SCC iteration + 11n^3) over wall time"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_tridiag", "launches": tri_n, "avg_launch_ms": tri_avg_ms,
+                     "algorithmic_bytes_per_launch": tri_bytes_per_launch,
+                     "note": "algorithmic bytes = 8 (n-1) n (n+1) / 6 per matrix (one pass over the trailing "
+                             "lower triangle per column step), summed over the launch; launch time from HIP "
+                             "events on the bucket stream"},
+        "dense_fp64": {"achieved_tflops": dense_tflops, "peak_tflops": FP64_PEAK_TFLOPS,
+                       "frac": dense_tflops / FP64_PEAK_TFLOPS,
+                       "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time"},
         "cpu_baseline": cpu, "parity": check, "phase_ms": eng.timings(),
     }
     print(json.dumps(out))

@@ -411,6 +411,14 @@ int gfn1_batch_shell_charges(const gfn1_batch* b, int32_t inst, double* out) {
     return b->results[inst].status;
 }
 
+int gfn1_batch_kernel_stats(const gfn1_batch* b, int kernel, double* total_ms, int64_t* launches, double* bytes) {
+    if (!b || !b->batch || kernel != GFN1_KERNEL_TRIDIAG || !total_ms || !launches || !bytes) return GFN1_ERR_INVALID_INPUT;
+    long long nl = 0;
+    b->batch->tridiag_stats(total_ms, &nl, bytes);
+    *launches = nl;
+    return GFN1_OK;
+}
+
 void gfn1_batch_timings(const gfn1_batch* b, double out[4]) {
     if (!b) return;
     std::memcpy(out, b->batch->timings(), sizeof(double) * 4);

@@ -94,6 +94,8 @@ Status Batch::dalloc(T** p, size_t count) {
 Batch::~Batch() {
     for (auto& bk : buckets_) {
         if (bk.handle) rocblas_destroy_handle(bk.handle);
+        if (bk.ev0) (void)hipEventDestroy(bk.ev0);
+        if (bk.ev1) (void)hipEventDestroy(bk.ev1);
         if (bk.stream) (void)hipStreamDestroy(bk.stream);
         (void)hipFree(bk.arena);
         (void)hipFree(bk.ptrs);
@@ -297,6 +299,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
             for (int r = 0; r < kRoles; ++r)
                 hmat[(size_t)r * ninst_ + bk.inst[k]] = bk.arena + ((size_t)r * count + k) * mat;
         HIP_TRY(hipStreamCreate(&bk.stream));
+        HIP_TRY(hipEventCreate(&bk.ev0));
+        HIP_TRY(hipEventCreate(&bk.ev1));
         RB_TRY(rocblas_create_handle(&bk.handle));
         RB_TRY(rocblas_set_stream(bk.handle, bk.stream));
         HIP_TRY(hipMalloc(&bk.ptrs, sizeof(double*) * kActRoles * count));
@@ -398,6 +402,8 @@ Status Batch::run() {
     HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)std::max(natoms_, 1) * sizeof(double)));
     HIP_TRY(hipMemset(d_fail_, 0, (size_t)std::max(ninst_, 1) * sizeof(int)));
     ran_ = true;
+    tri_ms_ = tri_bytes_ = 0.0;
+    tri_launches_ = 0;
     auto t0 = std::chrono::steady_clock::now();
     Status st = setup_phase();
     if (!st.ok()) return st;
@@ -590,8 +596,16 @@ Status Batch::scc_phase() {
             if (own_solver) {
                 // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
                 // (returned transposed in Z, kc columns)
+                HIP_TRY(hipEventRecord(bk.ev0, bk.stream));
                 HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, bk.eig,
                                        bk.eig_e, bk.tau));
+                HIP_TRY(hipEventRecord(bk.ev1, bk.stream));
+                bk.ev_pending = true;
+                bk.ev_bytes = 0.0;
+                for (int i : bk.active) {
+                    const double na = h_ao0_[i + 1] - h_ao0_[i];
+                    bk.ev_bytes += 8.0 * (na - 1.0) * na * (na + 1.0) / 6.0;
+                }
                 HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
                                            (const double*)bk.eig, (const double*)bk.eig_e, d_eps_,
                                            (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc));
@@ -655,6 +669,14 @@ Status Batch::scc_phase() {
         HIP_TRY(hipDeviceSynchronize());
         HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
         for (auto& bk : buckets_) {
+            if (bk.ev_pending) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, bk.ev0, bk.ev1));
+                tri_ms_ += ms;
+                tri_bytes_ += bk.ev_bytes;
+                ++tri_launches_;
+                bk.ev_pending = false;
+            }
             if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
                 std::vector<int> info(bk.active.size());
                 HIP_TRY(hipMemcpy(info.data(), bk.info, info.size() * sizeof(int), hipMemcpyDeviceToHost));

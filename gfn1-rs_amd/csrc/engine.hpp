@@ -90,6 +90,14 @@ class Batch {
     Status download(std::vector<InstanceResult>& out);
     const double* timings() const { return timings_; }
     int size() const { return ninst_; }
+    // Device time of the tridiagonal reduction over the last run(): summed launch
+    // durations (HIP events on each bucket's stream), launches, and algorithmic
+    // bytes (one pass over the trailing lower triangle per column step).
+    void tridiag_stats(double* ms, long long* launches, double* bytes) const {
+        *ms = tri_ms_;
+        *launches = tri_launches_;
+        *bytes = tri_bytes_;
+    }
 
    private:
     struct Bucket {
@@ -111,6 +119,9 @@ class Batch {
         double* tau = nullptr;            // Householder factors (count * ld)
         double* lu = nullptr;             // inverse-iteration scratch
         int kcap = 0;                     // eigenvector columns kept per instance
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around the tridiagonal reduction
+        double ev_bytes = 0.0;            // algorithmic bytes of the timed launch
+        bool ev_pending = false;
         std::vector<int> active;
         int kP = 0;
     };
@@ -136,6 +147,8 @@ class Batch {
     SccParams prm_{};
     int broyden_memory_ = 2;
     double timings_[4] = {0, 0, 0, 0};
+    double tri_ms_ = 0.0, tri_bytes_ = 0.0;
+    long long tri_launches_ = 0;
 
     // device arrays
     DeviceBatch db_{};

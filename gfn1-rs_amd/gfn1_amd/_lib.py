@@ -94,6 +94,8 @@ SIGNATURES = [
     ("gfn1_batch_gradient", ctypes.c_int, [_VP, ctypes.c_int32, _DP]),
     ("gfn1_batch_shell_charges", ctypes.c_int, [_VP, ctypes.c_int32, _DP]),
     ("gfn1_batch_timings", None, [_VP, _DP]),
+    ("gfn1_batch_kernel_stats", ctypes.c_int,
+     [_VP, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_int64), _DP]),
     ("gfn1_batch_free", None, [_VP]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]

@@ -381,6 +381,18 @@ class BatchEngine:
         self._lib.gfn1_batch_timings(self._h, t)
         return dict(zip(("setup_ms", "scc_ms", "gradient_ms", "total_ms"), list(t)))
 
+    KERNEL_TRIDIAG = 0
+
+    def kernel_stats(self, kernel=KERNEL_TRIDIAG):
+        """(summed launch milliseconds, launches, algorithmic bytes) of one kernel
+        over the last run, timed with HIP events on the launch streams."""
+        ms, nl, nb = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+        rc = self._lib.gfn1_batch_kernel_stats(self._h, kernel, ctypes.byref(ms), ctypes.byref(nl),
+                                               ctypes.byref(nb))
+        if rc != 0:
+            raise ValueError("unknown kernel id %r" % (kernel,))
+        return ms.value, nl.value, nb.value
+
     def __del__(self):
         if getattr(self, "_h", None):
             self._lib.gfn1_batch_free(self._h)

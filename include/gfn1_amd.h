@@ -202,6 +202,16 @@ int gfn1_batch_gradient(const gfn1_batch* batch, int32_t inst, double* out);
 int gfn1_batch_shell_charges(const gfn1_batch* batch, int32_t inst, double* out);
 /* Device time of the last gfn1_batch_run phases, milliseconds: [setup, scc, gradient, total]. */
 void gfn1_batch_timings(const gfn1_batch* batch, double out[4]);
+
+/* Device-side timing of one kernel over the last gfn1_batch_run: summed launch
+ * durations (HIP events recorded on the stream each launch runs on), number of
+ * launches, and algorithmic bytes of those launches.  For
+ * GFN1_KERNEL_TRIDIAG the algorithmic bytes of an n x n matrix are one pass
+ * over the trailing lower triangle per column step, 8 (n-1) n (n+1) / 6.
+ * No reference counterpart (measurement hook of the batch engine). */
+enum { GFN1_KERNEL_TRIDIAG = 0 };
+int gfn1_batch_kernel_stats(const gfn1_batch* batch, int kernel, double* total_ms, int64_t* launches,
+                            double* bytes);
 void gfn1_batch_free(gfn1_batch* batch);
 
 /* Library version string. */
