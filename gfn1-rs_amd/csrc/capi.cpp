@@ -48,8 +48,14 @@ struct gfn1_gradient_result {
     double max_gradient = 0.0;
 };
 
+struct gfn1_guess_model {
+    GuessModel m;
+};
+
 struct gfn1_batch {
     std::unique_ptr<Batch> batch;
+    bool learned_guess = false;   // GFN1_GUESS_LEARNED requested
+    bool guess_set = false;
     mutable std::vector<InstanceResult> results;
     mutable bool downloaded = false;
     bool ran = false;
@@ -339,8 +345,8 @@ int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_par
     }
     if (options->precision != GFN1_PRECISION_FP64)
         return fail(err, Status::error(kErrInvalidInput, "only the FP64 precision mode is implemented"));
-    if (options->guess != GFN1_GUESS_ZERO)
-        return fail(err, Status::error(kErrInvalidInput, "only the standard (zero) charge guess is implemented"));
+    if (options->guess != GFN1_GUESS_ZERO && options->guess != GFN1_GUESS_LEARNED)
+        return fail(err, Status::error(kErrInvalidInput, "unknown charge guess mode"));
     std::vector<SystemInput> sys;
     for (int i = 0; i < n_inst; ++i) {
         Status st = validate(&systems[i], &options->gradient.electronic);
@@ -363,12 +369,25 @@ int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_par
     b->batch = std::make_unique<Batch>();
     st = b->batch->create(*dm, sys, e);
     if (!st.ok()) return fail(err, st);
+    b->learned_guess = options->guess == GFN1_GUESS_LEARNED;
     *out = b.release();
+    return GFN1_OK;
+}
+
+int gfn1_batch_set_guess(gfn1_batch* b, const gfn1_guess_model* model, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch) return fail(err, Status::error(kErrInvalidInput, "invalid batch"));
+    Status st = b->batch->set_guess(model ? &model->m : nullptr);
+    if (!st.ok()) return fail(err, st);
+    b->guess_set = model != nullptr;
+    b->learned_guess = model != nullptr;
     return GFN1_OK;
 }
 
 int gfn1_batch_run(gfn1_batch* batch, gfn1_error* err) {
     clear(err);
+    if (batch && batch->learned_guess && !batch->guess_set)
+        return fail(err, Status::error(kErrInvalidInput, "learned charge guess selected but no model set"));
     if (!batch) return fail(err, Status::error(kErrInvalidInput, "null batch"));
     batch->downloaded = false;
     Status st = batch->batch->run();
@@ -427,3 +446,203 @@ void gfn1_batch_timings(const gfn1_batch* b, double out[4]) {
 void gfn1_batch_free(gfn1_batch* b) { delete b; }
 
 }  // extern "C"
+
+// --- Learned SCC start charges ---------------------------------------------
+
+namespace {
+
+// Systems -> concatenated atoms; instance usable when every atom is H, C, N or O
+// with two shells.  nsh receives the shell count of every system.
+Status to_guess_atoms(int32_t n_sys, const gfn1_system* systems, const gfn1_params* params, GuessAtoms& a,
+                      std::vector<int>* nsh) {
+    a = GuessAtoms{};
+    a.inst_atom0.push_back(0);
+    if (nsh) nsh->assign(n_sys, 0);
+    for (int i = 0; i < n_sys; ++i) {
+        const gfn1_system& s = systems[i];
+        if (s.nat < 0 || (s.nat > 0 && (!s.numbers || !s.positions)))
+            return Status::error(kErrInvalidInput, "invalid system " + std::to_string(i));
+        int ok = 1;
+        for (int k = 0; k < s.nat; ++k) {
+            const int z = s.numbers[k];
+            int ns = 0;
+            if (params) {
+                if (z < 1 || z > kMaxZ || !params->model.element(z).present)
+                    return Status::error(kErrMissingElement, "missing element Z=" + std::to_string(z));
+                ns = params->model.element(z).nshell;
+            }
+            if (guess_element_slot(z) < 0 || (params && ns != 2)) ok = 0;
+            if (nsh) (*nsh)[i] += ns;
+            a.z.push_back(z);
+            a.inst.push_back(i);
+            for (int d = 0; d < 3; ++d) a.x.push_back(s.positions[3 * k + d]);
+        }
+        a.inst_atom0.push_back((int)a.z.size());
+        a.usable.push_back(ok);
+    }
+    return Status{};
+}
+
+// Two labels per atom from concatenated shell charges (usable systems only).
+void to_labels(int32_t n_sys, const GuessAtoms& a, const std::vector<int>& nsh, const double* q,
+               std::vector<float>& labels) {
+    labels.assign(2 * a.z.size(), 0.0f);
+    size_t off = 0;
+    for (int i = 0; i < n_sys; ++i) {
+        if (a.usable[i])
+            for (int k = a.inst_atom0[i]; k < a.inst_atom0[i + 1]; ++k) {
+                const size_t sh = off + 2 * (size_t)(k - a.inst_atom0[i]);
+                labels[2 * (size_t)k] = (float)q[sh];
+                labels[2 * (size_t)k + 1] = (float)q[sh + 1];
+            }
+        off += nsh[i];
+    }
+}
+
+}  // namespace
+
+int gfn1_params_shell_count(const gfn1_params* p, int z) {
+    if (!p || z < 1 || z > kMaxZ || !p->model.element(z).present) return -1;
+    return p->model.element(z).nshell;
+}
+
+int gfn1_guess_model_create(uint64_t seed, gfn1_guess_model** out, gfn1_error* err) {
+    clear(err);
+    if (!out) return fail(err, Status::error(kErrInvalidInput, "null output"));
+    auto m = std::make_unique<gfn1_guess_model>();
+    Status st = m->m.init_random(seed);
+    if (!st.ok()) return fail(err, st);
+    *out = m.release();
+    return GFN1_OK;
+}
+
+int gfn1_guess_model_load(const char* path, gfn1_guess_model** out, gfn1_error* err) {
+    clear(err);
+    if (!out || !path) return fail(err, Status::error(kErrInvalidInput, "null argument"));
+    auto m = std::make_unique<gfn1_guess_model>();
+    Status st = m->m.load(path);
+    if (!st.ok()) return fail(err, st);
+    *out = m.release();
+    return GFN1_OK;
+}
+
+int gfn1_guess_model_save(const gfn1_guess_model* m, const char* path, gfn1_error* err) {
+    clear(err);
+    if (!m || !path) return fail(err, Status::error(kErrInvalidInput, "null argument"));
+    Status st = m->m.save(path);
+    return st.ok() ? GFN1_OK : fail(err, st);
+}
+
+int gfn1_guess_model_get_params(const gfn1_guess_model* m, float* theta, float* norm) {
+    if (!m || !theta || m->m.params().size() != (size_t)kGuessParams) return GFN1_ERR_INVALID_INPUT;
+    std::memcpy(theta, m->m.params().data(), sizeof(float) * kGuessParams);
+    if (norm) std::memcpy(norm, m->m.norm().data(), sizeof(float) * 2 * kGuessFeat);
+    return GFN1_OK;
+}
+
+int gfn1_guess_model_set_params(gfn1_guess_model* m, const float* theta) {
+    if (!m || !theta) return GFN1_ERR_INVALID_INPUT;
+    return m->m.set_params(theta).ok() ? GFN1_OK : GFN1_ERR_INVALID_INPUT;
+}
+
+void gfn1_guess_model_free(gfn1_guess_model* m) { delete m; }
+
+void gfn1_guess_train_options_default(gfn1_guess_train_options* o) {
+    if (!o) return;
+    GuessTrainOptions d;
+    o->epochs = d.epochs;
+    o->batch_atoms = d.batch_atoms;
+    o->learning_rate = d.learning_rate;
+    o->seed = d.seed;
+}
+
+int gfn1_guess_train(gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems, const gfn1_params* params,
+                     const double* shell_charges, const gfn1_guess_train_options* options, float* loss_per_epoch,
+                     gfn1_error* err) {
+    clear(err);
+    if (!m || n_sys <= 0 || !systems || !params || !shell_charges)
+        return fail(err, Status::error(kErrInvalidInput, "invalid training arguments"));
+    gfn1_guess_train_options def;
+    if (!options) {
+        gfn1_guess_train_options_default(&def);
+        options = &def;
+    }
+    GuessAtoms a;
+    std::vector<int> nsh;
+    Status st = to_guess_atoms(n_sys, systems, params, a, &nsh);
+    if (!st.ok()) return fail(err, st);
+    std::vector<float> labels;
+    to_labels(n_sys, a, nsh, shell_charges, labels);
+    GuessTrainOptions o;
+    o.epochs = options->epochs;
+    o.batch_atoms = options->batch_atoms;
+    o.learning_rate = options->learning_rate;
+    o.seed = options->seed;
+    std::vector<float> loss;
+    st = m->m.train(a, labels, o, loss);
+    if (!st.ok()) return fail(err, st);
+    if (loss_per_epoch) std::copy(loss.begin(), loss.end(), loss_per_epoch);
+    return GFN1_OK;
+}
+
+int gfn1_guess_predict(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems,
+                       const gfn1_params* params, double* shell_charges, gfn1_error* err) {
+    clear(err);
+    if (!m || n_sys < 0 || (n_sys > 0 && !systems) || !params || !shell_charges)
+        return fail(err, Status::error(kErrInvalidInput, "invalid prediction arguments"));
+    GuessAtoms a;
+    std::vector<int> nsh;
+    Status st = to_guess_atoms(n_sys, systems, params, a, &nsh);
+    if (!st.ok()) return fail(err, st);
+    std::vector<float> y;
+    st = m->m.predict_atoms(a, y);
+    if (!st.ok()) return fail(err, st);
+    size_t off = 0;
+    for (int i = 0; i < n_sys; ++i) {
+        for (int k = 0; k < nsh[i]; ++k) shell_charges[off + k] = 0.0;
+        if (a.usable[i] && nsh[i] > 0) {
+            double sum = 0.0;
+            for (int k = a.inst_atom0[i]; k < a.inst_atom0[i + 1]; ++k) sum += (double)y[2 * k] + (double)y[2 * k + 1];
+            const double shift = (sum - systems[i].charge) / nsh[i];
+            for (int k = a.inst_atom0[i]; k < a.inst_atom0[i + 1]; ++k) {
+                const size_t sh = off + 2 * (size_t)(k - a.inst_atom0[i]);
+                shell_charges[sh] = (double)y[2 * k] - shift;
+                shell_charges[sh + 1] = (double)y[2 * k + 1] - shift;
+            }
+        }
+        off += nsh[i];
+    }
+    return GFN1_OK;
+}
+
+int gfn1_guess_features(int32_t n_sys, const gfn1_system* systems, float* out, gfn1_error* err) {
+    clear(err);
+    if (n_sys < 0 || (n_sys > 0 && !systems) || !out)
+        return fail(err, Status::error(kErrInvalidInput, "invalid arguments"));
+    GuessAtoms a;
+    Status st = to_guess_atoms(n_sys, systems, nullptr, a, nullptr);
+    if (!st.ok()) return fail(err, st);
+    std::vector<float> f;
+    st = guess_features(a, f);
+    if (!st.ok()) return fail(err, st);
+    std::copy(f.begin(), f.end(), out);
+    return GFN1_OK;
+}
+
+int gfn1_guess_loss_gradient(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems,
+                             const gfn1_params* params, const double* shell_charges, double* loss, float* grad,
+                             gfn1_error* err) {
+    clear(err);
+    if (!m || n_sys <= 0 || !systems || !params || !shell_charges || !loss || !grad)
+        return fail(err, Status::error(kErrInvalidInput, "invalid arguments"));
+    GuessAtoms a;
+    std::vector<int> nsh;
+    Status st = to_guess_atoms(n_sys, systems, params, a, &nsh);
+    if (!st.ok()) return fail(err, st);
+    std::vector<float> labels, g;
+    to_labels(n_sys, a, nsh, shell_charges, labels);
+    st = m->m.loss_gradient(a, labels, loss, g);
+    if (!st.ok()) return fail(err, st);
+    std::copy(g.begin(), g.end(), grad);
+    return GFN1_OK;
+}

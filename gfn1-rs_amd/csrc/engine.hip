@@ -127,8 +127,11 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     h_err_.assign(ninst_, 0);
     h_errmsg_.assign(ninst_, "");
     h_nelec_.assign(ninst_, 0.0);
+    h_guess_usable_.assign(ninst_, 1);
+    h_charge_.assign(ninst_, 0.0);
     for (int i = 0; i < ninst_; ++i) {
         const SystemInput& s = systems[i];
+        h_charge_[i] = s.charge;
         int nsh = 0, nao = 0;
         double nref = 0.0;
         for (int a = 0; a < s.nat; ++a) {
@@ -136,6 +139,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
             if (z < 1 || z > kMaxZ || !m.element(z).present)
                 return Status::error(kErrMissingElement, "missing element Z=" + std::to_string(z));
             const ElementType& e = m.element(z);
+            if (guess_element_slot(z) < 0 || e.nshell != 2) h_guess_usable_[i] = 0;
             h_atom_inst_.push_back(i);
             h_atom_z_.push_back(z);
             h_atom_sh0_.push_back(h_sh0_[i] + nsh);
@@ -425,6 +429,23 @@ Status Batch::run() {
     return Status{};
 }
 
+Status Batch::set_guess(const GuessModel* model) {
+    if (model && !d_gfeat_ && natoms_ > 0) {
+        Status st;
+        if (!(st = dalloc(&d_charge_, ninst_)).ok()) return st;
+        HIP_TRY(hipMalloc(&d_guess_usable_, ninst_ * sizeof(int)));
+        allocs_.push_back(d_guess_usable_);
+        HIP_TRY(hipMalloc(&d_gfeat_, (size_t)natoms_ * kGuessFeat * sizeof(float)));
+        allocs_.push_back(d_gfeat_);
+        HIP_TRY(hipMalloc(&d_gout_, (size_t)natoms_ * kGuessOut * sizeof(float)));
+        allocs_.push_back(d_gout_);
+        HIP_TRY(hipMemcpy(d_charge_, h_charge_.data(), ninst_ * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_guess_usable_, h_guess_usable_.data(), ninst_ * sizeof(int), hipMemcpyHostToDevice));
+    }
+    guess_ = model;
+    return Status{};
+}
+
 Status Batch::setup_phase() {
     const DeviceTables t = dm_->tables();
     const int na = natoms_;
@@ -461,6 +482,21 @@ Status Batch::setup_phase() {
     hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_edisp_atom_,
                        d_edisp_);
     LAUNCH_CHECK();
+    if (guess_ || q0_dirty_) {
+        // the caller's start vector, then the learned one where the model applies
+        HIP_TRY(hipMemcpyAsync(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+        q0_dirty_ = false;
+    }
+    if (guess_) {
+        const float *th, *nm;
+        Status gs = guess_->device_ptrs(&th, &nm);
+        if (!gs.ok()) return gs;
+        HIP_TRY(guess_launch_features(stream_, na, d_atom_z_, d_x_, d_atom_inst_, d_atom0_, d_gfeat_));
+        HIP_TRY(guess_launch_forward(stream_, na, th, nm, d_gfeat_, d_gout_));
+        HIP_TRY(guess_launch_apply(stream_, ninst_, d_guess_usable_, d_atom0_, d_atom_sh0_, d_sh0_, d_charge_, d_gout_,
+                                   d_q0_));
+        q0_dirty_ = true;
+    }
     HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
 

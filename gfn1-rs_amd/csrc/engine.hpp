@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "guess.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 
@@ -90,6 +91,8 @@ class Batch {
     Status download(std::vector<InstanceResult>& out);
     const double* timings() const { return timings_; }
     int size() const { return ninst_; }
+    // Learned start charges for the next runs (nullptr: the caller's or zero start).
+    Status set_guess(const GuessModel* model);
     // Device time of the tridiagonal reduction over the last run(): summed launch
     // durations (HIP events on each bucket's stream), launches, and algorithmic
     // bytes (one pass over the trailing lower triangle per column step).
@@ -147,6 +150,13 @@ class Batch {
     SccParams prm_{};
     int broyden_memory_ = 2;
     double timings_[4] = {0, 0, 0, 0};
+    const GuessModel* guess_ = nullptr;
+    bool q0_dirty_ = false;                  // d_q0_ holds a learned start vector
+    std::vector<int> h_guess_usable_;
+    std::vector<double> h_charge_;
+    int* d_guess_usable_ = nullptr;
+    double* d_charge_ = nullptr;
+    float *d_gfeat_ = nullptr, *d_gout_ = nullptr;
     double tri_ms_ = 0.0, tri_bytes_ = 0.0;
     long long tri_launches_ = 0;
 

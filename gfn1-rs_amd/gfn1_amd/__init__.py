@@ -6,6 +6,7 @@ Python surface of the C ABI in include/gfn1_amd.h:
   analytic_gradient(...)        gradient.rs:56
   Gfn1NativeCalculator          python.rs:115-381
   BatchEngine                   batched energy + gradient of many systems
+  GuessModel                    learned SCC start charges (GPU-trained MLP)
 """
 from .api import (  # noqa: F401
     ANGSTROM_TO_BOHR,
@@ -17,7 +18,10 @@ from .api import (  # noqa: F401
     ElectronicResult,
     Gfn1NativeCalculator,
     Gfn1Parameters,
+    GuessModel,
     HARTREE_TO_EV,
+    basis_counts,
+    guess_features,
     analytic_gradient,
     run_electronic,
 )

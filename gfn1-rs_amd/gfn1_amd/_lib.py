@@ -62,6 +62,16 @@ class BatchOptions(ctypes.Structure):
 
 # (name, restype, argtypes) of every exported entry point
 _VP = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+class GuessTrainOptions(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int32), ("batch_atoms", ctypes.c_int32), ("learning_rate", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+GUESS_PARAMS = 8706
+GUESS_FEATURES = 68
 SIGNATURES = [
     ("gfn1_params_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),
     ("gfn1_params_free", None, [_VP]),
@@ -97,6 +107,22 @@ SIGNATURES = [
     ("gfn1_batch_kernel_stats", ctypes.c_int,
      [_VP, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_int64), _DP]),
     ("gfn1_batch_free", None, [_VP]),
+    ("gfn1_params_shell_count", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("gfn1_guess_model_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_model_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_model_save", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_model_get_params", ctypes.c_int, [_VP, _FP, _FP]),
+    ("gfn1_guess_model_set_params", ctypes.c_int, [_VP, _FP]),
+    ("gfn1_guess_model_free", None, [_VP]),
+    ("gfn1_guess_train_options_default", None, [ctypes.POINTER(GuessTrainOptions)]),
+    ("gfn1_guess_train", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP,
+                                        ctypes.POINTER(GuessTrainOptions), _FP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_predict", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP,
+                                          ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_features", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _FP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_loss_gradient", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP, _DP, _FP,
+                                                ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_batch_set_guess", ctypes.c_int, [_VP, _VP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

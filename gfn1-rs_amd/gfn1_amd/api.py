@@ -6,6 +6,7 @@ Mirrors the reference's run_electronic / analytic_gradient entry points
 (src/python.rs:115-381, :2845-2935).
 """
 import ctypes
+import os
 from dataclasses import dataclass, field, fields
 from typing import Dict, Optional
 
@@ -313,11 +314,144 @@ class Gfn1NativeCalculator:
                                  el.terms)
 
 
+def basis_counts(numbers, params: Optional[Gfn1Parameters] = None):
+    """(shells,) of a system in the loaded parameters."""
+    p = _params(params)
+    lib = _lib.load()
+    nsh = 0
+    for z in numbers:
+        k = lib.gfn1_params_shell_count(p._h, int(z))
+        if k < 0:
+            raise _lib.NativeError(4, "missing element Z=%d" % int(z))
+        nsh += k
+    return (nsh,)
+
+
+def _system_array(systems):
+    """(numbers, positions_bohr[, charge]) tuples -> ctypes System array + keep-alive list."""
+    arr = (_lib.System * len(systems))()
+    keep = []
+    for i, s in enumerate(systems):
+        charge = s[2] if len(s) > 2 else 0.0
+        sc, k = _lib.make_system(s[0], s[1], charge)
+        arr[i] = sc
+        keep.append(k)
+    return arr, keep
+
+
+class GuessModel:
+    """Learned SCC start charges: a per-atom MLP on radial-environment descriptors
+    (68 -> 64 SiLU -> 64 SiLU -> 2 shell charges), trained and evaluated on the GPU
+    by the library's own kernels.  Its output is the SCC start vector (the
+    reference's scc_initial_shell_charges, electronic.rs:386-392); the converged
+    result does not depend on it."""
+
+    PARAMS = _lib.GUESS_PARAMS
+    FEATURES = _lib.GUESS_FEATURES
+
+    def __init__(self, seed=1, _handle=None):
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            err = _lib.Gfn1Error()
+            _lib.check(self._lib.gfn1_guess_model_create(int(seed), ctypes.byref(self._h), ctypes.byref(err)), err)
+
+    @classmethod
+    def load(cls, path):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        err = _lib.Gfn1Error()
+        _lib.check(lib.gfn1_guess_model_load(os.fsencode(path), ctypes.byref(h), ctypes.byref(err)), err)
+        return cls(_handle=h)
+
+    def save(self, path):
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_guess_model_save(self._h, os.fsencode(path), ctypes.byref(err)), err)
+
+    def params(self):
+        theta = np.zeros(self.PARAMS, dtype=np.float32)
+        norm = np.zeros(2 * self.FEATURES, dtype=np.float32)
+        rc = self._lib.gfn1_guess_model_get_params(self._h, theta.ctypes.data_as(_lib._FP),
+                                                   norm.ctypes.data_as(_lib._FP))
+        if rc != 0:
+            raise ValueError("model has no parameters")
+        return theta, norm
+
+    def set_params(self, theta):
+        theta = np.ascontiguousarray(theta, dtype=np.float32)
+        if theta.size != self.PARAMS:
+            raise ValueError("expected %d parameters" % self.PARAMS)
+        if self._lib.gfn1_guess_model_set_params(self._h, theta.ctypes.data_as(_lib._FP)) != 0:
+            raise ValueError("invalid parameters")
+
+    def train(self, systems, shell_charges, params: Optional[Gfn1Parameters] = None, epochs=40, batch_atoms=1024,
+              learning_rate=1e-3, seed=1):
+        """Fit to converged shell charges (one array per system); returns the
+        per-epoch mean squared error."""
+        p = _params(params)
+        arr, keep = _system_array(systems)
+        q = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in shell_charges]))
+        o = _lib.GuessTrainOptions()
+        self._lib.gfn1_guess_train_options_default(ctypes.byref(o))
+        o.epochs, o.batch_atoms, o.learning_rate, o.seed = int(epochs), int(batch_atoms), float(learning_rate), int(seed)
+        loss = np.zeros(max(int(epochs), 1), dtype=np.float32)
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_guess_train(self._h, len(systems), arr, p._h, q.ctypes.data_as(_lib._DP),
+                                              ctypes.byref(o), loss.ctypes.data_as(_lib._FP), ctypes.byref(err)), err)
+        del keep
+        return loss[:int(epochs)]
+
+    def predict(self, systems, params: Optional[Gfn1Parameters] = None):
+        """Start vectors, one array of shell charges per system."""
+        p = _params(params)
+        arr, keep = _system_array(systems)
+        nsh = [basis_counts(s[0], p)[0] for s in systems]
+        out = np.zeros(sum(nsh))
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_guess_predict(self._h, len(systems), arr, p._h, out.ctypes.data_as(_lib._DP),
+                                                ctypes.byref(err)), err)
+        del keep
+        return np.split(out, np.cumsum(nsh)[:-1])
+
+    def loss_gradient(self, systems, shell_charges, params: Optional[Gfn1Parameters] = None):
+        p = _params(params)
+        arr, keep = _system_array(systems)
+        q = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64) for x in shell_charges]))
+        loss = ctypes.c_double()
+        grad = np.zeros(self.PARAMS, dtype=np.float32)
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_guess_loss_gradient(self._h, len(systems), arr, p._h, q.ctypes.data_as(_lib._DP),
+                                                      ctypes.byref(loss), grad.ctypes.data_as(_lib._FP),
+                                                      ctypes.byref(err)), err)
+        del keep
+        return loss.value, grad
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.gfn1_guess_model_free(self._h)
+            self._h = None
+
+
+def guess_features(systems):
+    """Descriptors of every atom (natoms x 68, FP32), computed on the GPU."""
+    lib = _lib.load()
+    arr, keep = _system_array(systems)
+    nat = sum(len(s[0]) for s in systems)
+    out = np.zeros((nat, _lib.GUESS_FEATURES), dtype=np.float32)
+    err = _lib.Gfn1Error()
+    _lib.check(lib.gfn1_guess_features(len(systems), arr, out.ctypes.data_as(_lib._FP), ctypes.byref(err)), err)
+    del keep
+    return out
+
+
 class BatchEngine:
     """Energy + gradient of many independent systems in one device pass."""
 
     def __init__(self, systems, params: Optional[Gfn1Parameters] = None,
-                 options: Optional[AnalyticGradientOptions] = None, eigensolver=0, keep_matrices=False):
+                 options: Optional[AnalyticGradientOptions] = None, eigensolver=0, keep_matrices=False,
+                 guess: Optional[GuessModel] = None):
         lib = _lib.load()
         self._lib = lib
         self.params = _params(params)
@@ -343,6 +477,14 @@ class BatchEngine:
         _lib.check(lib.gfn1_batch_create(len(systems), arr, self.params._h, ctypes.byref(bo), ctypes.byref(self._h),
                                          ctypes.byref(err)), err)
         self.n = len(systems)
+        self.set_guess(guess)
+
+    def set_guess(self, model: Optional["GuessModel"]):
+        """Start the SCC from the model's charges (None: the standard start)."""
+        self._guess = model
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_set_guess(self._h, model._h if model is not None else None,
+                                                  ctypes.byref(err)), err)
 
     def run(self):
         err = _lib.Gfn1Error()

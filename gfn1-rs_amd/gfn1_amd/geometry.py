@@ -12,6 +12,9 @@ Workloads (all positions returned in bohr):
   mixed_batch(4096)                   4096 clusters of 2-40 molecules drawn
                                       from {H2O, CH4, NH3} at 0.9 g/cm^3,
                                       instance i seeded with 1000 + i
+  noisy_snapshots(20000)              learned-guess training data: mixed
+                                      clusters (seed 10^6 + j) displaced by
+                                      N(0, 0.05 A) per coordinate
 """
 import math
 
@@ -159,6 +162,23 @@ def mixed_cluster(seed, density=0.9, kmin=2, kmax=40):
 
 def mixed_batch(count, first_seed=1000, **kw):
     return [mixed_cluster(first_seed + i, **kw) for i in range(count)]
+
+
+def noisy_snapshot(j, sigma_angstrom=0.05, density=0.9, kmin=2, kmax=40):
+    """Snapshot j of the learned-guess data: the mixed cluster of seed 10^6 + j with
+    every coordinate displaced by N(0, sigma) drawn from the same random stream
+    right after placement; bohr."""
+    rng = Xoshiro256(1_000_000 + j)
+    k = rng.integer(kmin, kmax)
+    kinds = [MONOMER_ORDER[rng.integer(0, 2)] for _ in range(k)]
+    side = cube_side(kinds, density)
+    z, x = place(kinds, side, rng)
+    noise = np.array([rng.normal() for _ in range(x.size)]).reshape(x.shape)
+    return z, (x + noise * sigma_angstrom) * ANGSTROM_TO_BOHR
+
+
+def noisy_snapshots(count, first=0, **kw):
+    return [noisy_snapshot(first + j, **kw) for j in range(count)]
 
 
 def perturbed(z, x_bohr, seed, sigma_angstrom=0.05):

@@ -48,6 +48,10 @@ typedef struct gfn1_params gfn1_params;
  * anything else is a parameter-file path. */
 int gfn1_params_load(const char* spec, gfn1_params** out, gfn1_error* err);
 void gfn1_params_free(gfn1_params* params);
+/* Number of shells of element z in the loaded parameters (-1 if absent).  The
+ * shell layout of a system is its atoms' shells in input order
+ * (src/basis.rs:121-233). */
+int gfn1_params_shell_count(const gfn1_params* params, int z);
 
 /* PeriodicSystem without a lattice (src/system.rs:17-21). */
 typedef struct gfn1_system {
@@ -213,6 +217,52 @@ enum { GFN1_KERNEL_TRIDIAG = 0 };
 int gfn1_batch_kernel_stats(const gfn1_batch* batch, int kernel, double* total_ms, int64_t* launches,
                             double* bytes);
 void gfn1_batch_free(gfn1_batch* batch);
+
+/* --- Learned SCC start charges ---
+ * A per-atom MLP (68 radial-environment inputs, two hidden layers of 64 SiLU
+ * units, two outputs: the atom's two shell charges) predicts the SCC start
+ * vector; per system the predictions are shifted to sum to the system charge.
+ * The start vector enters through the reference's scc_initial_shell_charges
+ * hook (src/electronic.rs:386-392, :1231-1239); the converged result does not
+ * depend on it.  Systems with atoms other than H, C, N, O keep the standard
+ * start.  No reference counterpart otherwise.  Training and inference run on
+ * the GPU with the library's own kernels (FP32, Adam). */
+enum { GFN1_GUESS_PARAMS = 8706, GFN1_GUESS_FEATURES = 68 };
+typedef struct gfn1_guess_model gfn1_guess_model;
+int gfn1_guess_model_create(uint64_t seed, gfn1_guess_model** out, gfn1_error* err); /* random initialisation */
+int gfn1_guess_model_load(const char* path, gfn1_guess_model** out, gfn1_error* err);
+int gfn1_guess_model_save(const gfn1_guess_model* model, const char* path, gfn1_error* err);
+/* theta: GFN1_GUESS_PARAMS floats; norm (nullable): descriptor mean[68] then inverse spread[68]. */
+int gfn1_guess_model_get_params(const gfn1_guess_model* model, float* theta, float* norm);
+int gfn1_guess_model_set_params(gfn1_guess_model* model, const float* theta);
+void gfn1_guess_model_free(gfn1_guess_model* model);
+
+typedef struct gfn1_guess_train_options {
+    int32_t epochs;        /* passes over the training atoms (default 40) */
+    int32_t batch_atoms;   /* atoms per Adam step (default 1024) */
+    float learning_rate;   /* Adam step size (default 1e-3) */
+    uint64_t seed;         /* shuffling */
+} gfn1_guess_train_options;
+void gfn1_guess_train_options_default(gfn1_guess_train_options* options);
+/* shell_charges: converged shell charges of every system, concatenated (nsh per
+ * system).  loss_per_epoch (nullable): epochs mean squared errors. */
+int gfn1_guess_train(gfn1_guess_model* model, int32_t n_sys, const gfn1_system* systems, const gfn1_params* params,
+                     const double* shell_charges, const gfn1_guess_train_options* options, float* loss_per_epoch,
+                     gfn1_error* err);
+/* Start vectors (nsh per system, concatenated). */
+int gfn1_guess_predict(const gfn1_guess_model* model, int32_t n_sys, const gfn1_system* systems,
+                       const gfn1_params* params, double* shell_charges, gfn1_error* err);
+/* Descriptors of every atom (68 floats per atom, systems concatenated). */
+int gfn1_guess_features(int32_t n_sys, const gfn1_system* systems, float* out, gfn1_error* err);
+/* Training loss (mean squared shell-charge error over covered atoms) and its
+ * gradient with respect to the parameters, full batch. */
+int gfn1_guess_loss_gradient(const gfn1_guess_model* model, int32_t n_sys, const gfn1_system* systems,
+                             const gfn1_params* params, const double* shell_charges, double* loss, float* grad,
+                             gfn1_error* err);
+/* Use the model for the start vectors of the batch's next runs (nullptr: back
+ * to the standard start).  A batch created with GFN1_GUESS_LEARNED needs a model
+ * before gfn1_batch_run. */
+int gfn1_batch_set_guess(gfn1_batch* batch, const gfn1_guess_model* model, gfn1_error* err);
 
 /* Library version string. */
 const char* gfn1_version(void);
