@@ -7,7 +7,13 @@ molecules drawn from {H2O, CH4, NH3} at 0.9 g/cm^3 (instance i seeded with
 smearing, SCC tolerances 1e-7 Eh (energy) and 2e-5 (charge rms).
 
 One step = one full energy + gradient pass over the whole batch; inputs are
-uploaded to HBM before the timed region.  Prints one JSON line.
+uploaded to HBM before the timed region.  Two paths are timed on the same batch:
+  plain    FP64 throughout, standard (zero) SCC start
+  learned  SCC start from the GPU-trained charge model (gfn1-rs_amd/data/guess_mlp.bin)
+           + mixed precision (FP32 MFMA density/energy-weighted-density products
+           and FP32 gradient contraction; eigensolve, mixing, energy sums FP64)
+`value` is the learned + mixed throughput; the plain path and the ratio are
+reported beside it.  Prints one JSON line.
 """
 import argparse
 import json
@@ -60,6 +66,36 @@ def calibrate_copy():
                       "bytes_written": y.numel() * 8, "launches": 3}))
 
 
+def run_path(eng, label, steps, warmup):
+    for w in range(warmup):
+        eng.run()
+        print("%s warmup %d: %s" % (label, w, eng.timings()), file=sys.stderr, flush=True)
+    times = []
+    tri = [0.0, 0, 0.0]
+    for s in range(steps):
+        t = time.perf_counter()
+        eng.run()
+        times.append(time.perf_counter() - t)
+        k_ms, k_n, k_b = eng.kernel_stats()
+        tri = [tri[0] + k_ms, tri[1] + k_n, tri[2] + k_b]
+        print("%s step %d: %.3fs %s" % (label, s, times[-1], eng.timings()), file=sys.stderr, flush=True)
+    n = eng.n
+    return {"ms": 1000.0 * float(np.mean(times)), "tri": tri,
+            "status": np.array([eng.status(i) for i in range(n)]),
+            "iters": np.array([eng.iterations(i) for i in range(n)])}
+
+
+def parity(eng, systems, count):
+    import oracle
+    de, dg = 0.0, 0.0
+    for i in range(count):
+        z, x = systems[i]
+        ref = oracle.run(z, x, energy_tolerance=1e-7)
+        de = max(de, abs(ref["energy"] - eng.energy(i)))
+        dg = max(dg, float(np.abs(ref["gradient"] - eng.gradient(i)).max()))
+    return {"instances": count, "max_abs_dE": de, "max_abs_dg": dg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +105,9 @@ def main():
     ap.add_argument("--eigensolver", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=256, help="instances timed on the host oracle (0 = skip)")
     ap.add_argument("--check", type=int, default=4, help="instances checked against the oracle")
+    ap.add_argument("--path", choices=("both", "plain", "learned"), default="both",
+                    help="paths to time (profiling runs use one)")
+    ap.add_argument("--model", default=os.path.join(ROOT, "gfn1-rs_amd", "data", "guess_mlp.bin"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -80,7 +119,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, geometry
+    from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, GuessModel, geometry
 
     t0 = time.time()
     systems = geometry.mixed_batch(args.batch, first_seed=1000)
@@ -88,41 +127,37 @@ def main():
     print("generated %d clusters in %.1fs (AO %d..%d, mean %.1f)" % (len(systems), time.time() - t0, nao.min(),
                                                                    nao.max(), nao.mean()), file=sys.stderr, flush=True)
     opts = AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-7))
-    eng = BatchEngine(systems, options=opts, eigensolver=args.eigensolver)
-    for w in range(args.warmup):
-        eng.run()
-        print("warmup %d done: %s" % (w, eng.timings()), file=sys.stderr, flush=True)
-    times = []
-    tri_ms, tri_n, tri_bytes = 0.0, 0, 0.0
-    for s in range(args.steps):
-        t = time.perf_counter()
-        eng.run()
-        times.append(time.perf_counter() - t)
-        k_ms, k_n, k_b = eng.kernel_stats()
-        tri_ms, tri_n, tri_bytes = tri_ms + k_ms, tri_n + k_n, tri_bytes + k_b
-        print("step %d: %.3fs %s" % (s, times[-1], eng.timings()), file=sys.stderr, flush=True)
-    ms = 1000.0 * float(np.mean(times))
+    res, engines = {}, {}
+    if args.path in ("both", "plain"):
+        engines["plain"] = BatchEngine(systems, options=opts, eigensolver=args.eigensolver)
+        res["plain"] = run_path(engines["plain"], "plain", args.steps, args.warmup)
+    if args.path in ("both", "learned"):
+        model = GuessModel.load(args.model)
+        engines["learned"] = BatchEngine(systems, options=opts, eigensolver=args.eigensolver, guess=model,
+                                         precision="mixed")
+        res["learned"] = run_path(engines["learned"], "learned+mixed", args.steps, args.warmup)
+    head = "learned" if "learned" in res else "plain"
+    r = res[head]
+    ms = r["ms"]
     value = len(systems) / (ms / 1000.0)
-    status = np.array([eng.status(i) for i in range(len(systems))])
-    iters = np.array([eng.iterations(i) for i in range(len(systems))])
-    flops = float(sum(dense_flops(n, it) for n, it in zip(nao, iters)))
+    flops = float(sum(dense_flops(n, it) for n, it in zip(nao, r["iters"])))
     dense_tflops = flops / (ms / 1000.0) / 1e12
-    # dominant kernel: tridiagonal reduction, algorithmic bytes / mean launch time
+    tri_ms, tri_n, tri_bytes = r["tri"]
     tri_avg_ms = tri_ms / max(tri_n, 1)
     tri_bytes_per_launch = tri_bytes / max(tri_n, 1)
     achieved = tri_bytes_per_launch / (tri_avg_ms / 1000.0) / 1e9 if tri_n else 0.0
 
-    # parity spot check against the oracle
-    check = {}
-    if args.check > 0:
-        import oracle
-        de, dg = 0.0, 0.0
-        for i in range(args.check):
-            z, x = systems[i]
-            ref = oracle.run(z, x, energy_tolerance=1e-7)
-            de = max(de, abs(ref["energy"] - eng.energy(i)))
-            dg = max(dg, float(np.abs(ref["gradient"] - eng.gradient(i)).max()))
-        check = {"instances": args.check, "max_abs_dE": de, "max_abs_dg": dg}
+    paths = {}
+    for name, rr in res.items():
+        paths[name] = {"molecules_per_s": len(systems) / (rr["ms"] / 1000.0), "ms_per_step": rr["ms"],
+                       "mean_scc_iterations": float(rr["iters"].mean()),
+                       "converged": int((rr["status"] == 0).sum()),
+                       "parity_vs_oracle": parity(engines[name], systems, args.check) if args.check > 0 else None}
+    if "plain" in res and "learned" in res:
+        ep, el = engines["plain"], engines["learned"]
+        paths["learned"]["max_abs_dE_vs_plain"] = float(max(abs(ep.energy(i) - el.energy(i))
+                                                            for i in range(len(systems))))
+        paths["learned"]["speedup_vs_plain"] = res["plain"]["ms"] / res["learned"]["ms"]
 
     cpu = None
     if args.cpu_sample > 0:
@@ -130,7 +165,7 @@ def main():
         nthreads = min(16, os.cpu_count() or 1)
         sample = systems[: args.cpu_sample]
         t = time.perf_counter()
-        e, g, it, st = oracle.run_batch(sample, nthreads=nthreads, energy_tolerance=1e-7)
+        oracle.run_batch(sample, nthreads=nthreads, energy_tolerance=1e-7)
         dt = time.perf_counter() - t
         cpu = {"value": len(sample) / dt, "unit": "molecules/s", "cores": nthreads, "kind": "port",
                "sample": "first %d clusters of the batch, FP64 C++ oracle, OpenMP over instances" % len(sample)}
@@ -138,22 +173,25 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "molecules/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "batched mixed H2O/CH4/NH3 clusters, 2-40 molecules, 0.9 g/cm3, seeds 1000+i",
+        "vs_baseline": None, "dtype": "f64/f32" if head == "learned" else "f64", "data": "synthetic",
+        "config": {"workload": "batched mixed H2O/CH4/NH3 clusters, 2-40 molecules, 0.9 g/cm3, seeds 1000+i, "
+                               "energy + gradient, 300 K, SCC energy tolerance 1e-7",
                    "global_batch": len(systems), "ao_min": int(nao.min()), "ao_max": int(nao.max()),
-                   "ao_mean": float(nao.mean()), "path": "plain FP64", "parallelism": "single GPU"},
-        "converged": int((status == 0).sum()), "mean_scc_iterations": float(iters.mean()),
+                   "ao_mean": float(nao.mean()),
+                   "path": "learned start + mixed precision" if head == "learned" else "plain FP64",
+                   "parallelism": "single GPU"},
+        "paths": paths,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_tridiag", "launches": tri_n, "avg_launch_ms": tri_avg_ms,
                      "algorithmic_bytes_per_launch": tri_bytes_per_launch,
                      "note": "algorithmic bytes = 8 (n-1) n (n+1) / 6 per matrix (one pass over the trailing "
                              "lower triangle per column step), summed over the launch; launch time from HIP "
-                             "events on the bucket stream"},
+                             "events on the bucket stream; headline path"},
         "dense_fp64": {"achieved_tflops": dense_tflops, "peak_tflops": FP64_PEAK_TFLOPS,
                        "frac": dense_tflops / FP64_PEAK_TFLOPS,
-                       "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time"},
-        "cpu_baseline": cpu, "parity": check, "phase_ms": eng.timings(),
+                       "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time, headline path"},
+        "cpu_baseline": cpu, "phase_ms": engines[head].timings(),
     }
     print(json.dumps(out))
 

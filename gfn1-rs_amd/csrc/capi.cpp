@@ -343,8 +343,8 @@ int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_par
         gfn1_batch_options_default(&def);
         options = &def;
     }
-    if (options->precision != GFN1_PRECISION_FP64)
-        return fail(err, Status::error(kErrInvalidInput, "only the FP64 precision mode is implemented"));
+    if (options->precision != GFN1_PRECISION_FP64 && options->precision != GFN1_PRECISION_MIXED)
+        return fail(err, Status::error(kErrInvalidInput, "unknown precision mode"));
     if (options->guess != GFN1_GUESS_ZERO && options->guess != GFN1_GUESS_LEARNED)
         return fail(err, Status::error(kErrInvalidInput, "unknown charge guess mode"));
     std::vector<SystemInput> sys;
@@ -362,6 +362,7 @@ int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_par
     e.include_halogen = options->gradient.include_halogen;
     e.eigensolver = options->eigensolver;
     e.keep_matrices = options->keep_matrices;
+    e.mixed = options->precision == GFN1_PRECISION_MIXED;
     Status st;
     const DeviceModel* dm = const_cast<gfn1_params*>(params)->device_model(st);
     if (!dm) return fail(err, st);

@@ -39,7 +39,12 @@ namespace gfn1 {
 
 namespace {
 enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, kRoles };
-enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, kActRoles };
+// A_C32: FP32 operand behind the A32 operand in the A matrix (mixed precision)
+// Above this electronic temperature (K) the occupied-plus-smeared orbitals are not
+// bounded by the occupied count, so all eigenvectors are kept.
+constexpr double kWideSmearing = 1000.0;
+
+enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, kActRoles };
 constexpr int kWarmResetPeriod = 64;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -319,7 +324,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         // instance plus a margin for fractional occupations
         int kmax = 1;
         for (int i : bk.inst) kmax = std::max(kmax, (int)std::ceil(0.5 * h_nelec_[i] - 1e-9));
-        bk.kcap = std::min(bk.ld, kmax + 32);
+        bk.kcap = opt.electronic_temperature > kWideSmearing ? bk.ld : std::min(bk.ld, kmax + 32);
         HIP_TRY(hipMalloc(&bk.lu, sizeof(double) * 5 * (size_t)bk.ld * bk.kcap * count));
     }
     HIP_TRY(hipMemcpy(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
@@ -353,6 +358,15 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     return Status{};
 }
 
+Status Batch::grow_kcap(Bucket& bk, int kcap) {
+    HIP_TRY(hipDeviceSynchronize());
+    (void)hipFree(bk.lu);
+    bk.lu = nullptr;
+    HIP_TRY(hipMalloc(&bk.lu, sizeof(double) * 5 * (size_t)bk.ld * kcap * bk.inst.size()));
+    bk.kcap = kcap;
+    return Status{};
+}
+
 Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity) {
     // host copy of the pointer tables of the listed instances
     std::vector<double*> hp((size_t)kActRoles * list.size());
@@ -373,6 +387,7 @@ Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity
         hp[(size_t)A_P * list.size() + j] = at(R_P);
         hp[(size_t)A_S * list.size() + j] = at(R_S);
         hp[(size_t)A_X * list.size() + j] = at(R_W);
+        hp[(size_t)A_C32 * list.size() + j] = (double*)((float*)at(R_A) + (size_t)bk.ld * bk.kcap);
     }
     HIP_TRY(hipMemcpyAsync(bk.ptrs, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
     HIP_TRY(hipMemcpyAsync(bk.act, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, bk.stream));
@@ -664,15 +679,16 @@ Status Batch::scc_phase() {
             }
             // ---- P = C_k diag(f) C_k^T
             double* const* P = bk.ptrs + (size_t)A_P * cnt;
-            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T,
-                               A, (const double*)d_eps_, (const double*)d_occ_, kc, 0);
-            LAUNCH_CHECK();
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc,
-                                         &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
-                                         (double* const*)P, n, cnt));
             const int nt = (n + 31) / 32;
-            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               P);
+            // (FP64 in every precision mode: the SCC energy and charges need it)
+            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               T, A, (const double*)d_eps_, (const double*)d_occ_, kc, 0);
+            LAUNCH_CHECK();
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                                         kc, &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
+                                         (double* const*)P, n, cnt));
+            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_,
+                               (const int*)bk.act, P);
             LAUNCH_CHECK();
             // ---- Mulliken charges, energy, convergence and mixing
             hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_,
@@ -719,6 +735,13 @@ Status Batch::scc_phase() {
                 for (int v : info)
                     if (v != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
             }
+            // the last ladder rung smears at >= 3000 K: keep every eigenvector from then on
+            bool hot = false;
+            for (int i : bk.active) hot = hot || h_state_[i].temperature > kWideSmearing;
+            if (hot && bk.kcap < bk.ld) {
+                Status gs = grow_kcap(bk, bk.ld);
+                if (!gs.ok()) return gs;
+            }
             std::vector<int> next;
             for (int i : bk.active) {
                 if (h_state_[i].kneed_full > bk.kcap)
@@ -756,15 +779,31 @@ Status Batch::gradient_phase() {
         double* const* T = bk.ptrs + (size_t)A_T * cnt;
         double* const* A = bk.ptrs + (size_t)A_A * cnt;
         double* const* W = bk.ptrs + (size_t)A_X * cnt;
-        hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
-                           (const double*)d_eps_, (const double*)d_occ_, kc, 1);
-        LAUNCH_CHECK();
-        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc, &one,
-                                     (const double* const*)A, n, (const double* const*)T, n, &zero,
-                                     (double* const*)W, n, cnt));
         const int nt = (n + 31) / 32;
-        hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, W);
-        LAUNCH_CHECK();
+        if (opt_.mixed) {
+            double* const* C32 = bk.ptrs + (size_t)A_C32 * cnt;
+            double* const* X32 = bk.ptrs + (size_t)A_C * cnt;  // the eigenvector buffer is free after the SCC
+            hipLaunchKernelGGL(k_operands_fp32, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T,
+                               A, C32, (const double*)d_eps_, (const double*)d_occ_, kc, 1);
+            LAUNCH_CHECK();
+            const float onef = 1.0f, zerof = 0.0f;
+            RB_TRY(rocblas_sgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc,
+                                         &onef, (const float* const*)A, n, (const float* const*)C32, n, &zerof,
+                                         (float* const*)X32, n, cnt));
+            hipLaunchKernelGGL(k_symmetrize_fp32, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_,
+                               (const int*)bk.act, X32, W);
+            LAUNCH_CHECK();
+        } else {
+            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
+                               (const double*)d_eps_, (const double*)d_occ_, kc, 1);
+            LAUNCH_CHECK();
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc,
+                                         &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
+                                         (double* const*)W, n, cnt));
+            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                               W);
+            LAUNCH_CHECK();
+        }
         HIP_TRY(hipStreamSynchronize(bk.stream));
     }
     if (!opt_.compute_gradient || natoms_ == 0) return Status{};
@@ -773,12 +812,14 @@ Status Batch::gradient_phase() {
     const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
     const int cn_on = opt_.enable_cn_hamiltonian;
     if (el_on) {
-        hipLaunchKernelGGL(k_grad_pairs_t<false>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+        auto pairs = opt_.mixed ? k_grad_pairs_t<false, true> : k_grad_pairs_t<false, false>;
+        hipLaunchKernelGGL(pairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
                            (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
                            d_dedcn_h_);
         LAUNCH_CHECK();
         if (has_d_) {
-            hipLaunchKernelGGL(k_grad_pairs_t<true>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
+            auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true> : k_grad_pairs_t<true, false>;
+            hipLaunchKernelGGL(dpairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
                                (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
                                d_dedcn_h_);
             LAUNCH_CHECK();

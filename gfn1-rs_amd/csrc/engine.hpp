@@ -62,6 +62,10 @@ struct EngineOptions {
     // execution
     int eigensolver = 0;   // 0 auto, 1 syevd, 2 syevj
     int keep_matrices = 0;
+    // mixed precision: FP32 MFMA for the energy-weighted density W and FP32
+    // gradient contraction with compensated per-atom sums; the eigensolve, the
+    // density P, Broyden mixing and energy sums stay FP64
+    int mixed = 0;
 };
 
 // Per-instance results on the host.
@@ -133,6 +137,7 @@ class Batch {
     Status scc_phase();
     Status gradient_phase();
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
+    Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
 
     const DeviceModel* dm_ = nullptr;

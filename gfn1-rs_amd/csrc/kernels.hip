@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -510,6 +511,54 @@ __global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp) {
         // element (r1, c1) pairs with (c1, r1) = tile2 entry at [tx'=c1-bj*32... ]
         if (r1 < n && c1 < n) M[(size_t)c1 * ld + r1] = 0.5 * (t1[ty][tx] + t2[tx][ty]);
         if (bi != bj && r2 < n && c2 < n) M[(size_t)c2 * ld + r2] = 0.5 * (t2[ty][tx] + t1[tx][ty]);
+    }
+}
+
+// Mixed precision operands of M = C_k diag(w) C_k^T: C32 = fp32(C_k) and
+// A32 = fp32(C_k diag(w)), w = f (density) or eps f (energy-weighted density);
+// column col of each at col * ld (rows n..ld-1 zero).
+__global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp, double* const* A32p,
+                                double* const* C32p, const double* eps_all, const double* occ_all, int k,
+                                int weighted) {
+    const int inst = act[blockIdx.y];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int col = blockIdx.x;
+    if (col >= k) return;
+    const double f = occ_all[b.vec0[inst] + col];
+    const double w = weighted ? eps_all[b.vec0[inst] + col] * f : f;
+    gptr<const double> c = (gptr<const double>)Cp[blockIdx.y] + (size_t)col * ld;
+    gptr<float> a32 = (gptr<float>)A32p[blockIdx.y] + (size_t)col * ld;
+    gptr<float> c32 = (gptr<float>)C32p[blockIdx.y] + (size_t)col * ld;
+    for (int row = threadIdx.x; row < ld; row += blockDim.x) {
+        const double v = row < n ? c[row] : 0.0;
+        c32[row] = (float)v;
+        a32[row] = (float)(v * w);
+    }
+}
+
+// M = (X + X^T) / 2 in FP64 from an FP32 product X (n x n, leading dimension ld).
+__global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* Xp, double* const* Mp) {
+    __shared__ float t1[32][33], t2[32][33];
+    const int inst = act[blockIdx.z];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    if (bj > bi) return;
+    if (bi * 32 >= n) return;
+    gptr<const float> X = (gptr<const float>)Xp[blockIdx.z];
+    gptr<double> M = (gptr<double>)Mp[blockIdx.z];
+    const int tx = threadIdx.x & 31, ty0 = threadIdx.x >> 5;
+    for (int ty = ty0; ty < 32; ty += 8) {
+        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;
+        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;
+        t1[ty][tx] = (r1 < n && c1 < n) ? X[(size_t)c1 * ld + r1] : 0.0f;
+        t2[ty][tx] = (r2 < n && c2 < n) ? X[(size_t)c2 * ld + r2] : 0.0f;
+    }
+    __syncthreads();
+    for (int ty = ty0; ty < 32; ty += 8) {
+        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;
+        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;
+        if (r1 < n && c1 < n) M[(size_t)c1 * ld + r1] = 0.5 * ((double)t1[ty][tx] + (double)t2[tx][ty]);
+        if (bi != bj && r2 < n && c2 < n) M[(size_t)c2 * ld + r2] = 0.5 * ((double)t2[ty][tx] + (double)t1[tx][ty]);
     }
 }
 
@@ -1168,10 +1217,30 @@ __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTabl
 // The AO-pair weights oc = P (2 h - v_mu - v_nu) - 2 W and P are applied inside
 // the primitive loop, so neither the overlap block nor its derivative block
 // is ever stored.
-template <int LA, int LB>
+// Per-atom gradient accumulator: plain FP64 sums, or FP32 sums with Kahan
+// compensation (mixed precision).
+template <typename R>
+struct GradAcc {
+    R g[3] = {0, 0, 0}, dc = 0;
+    R cg[3] = {0, 0, 0}, cdc = 0;
+    __device__ __forceinline__ static void add(R& s, R& c, R v) {
+        if constexpr (sizeof(R) == sizeof(float)) {
+            const R y = v - c;
+            const R t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += v;
+        }
+    }
+    __device__ __forceinline__ double value(int d) const { return (double)g[d] - (double)cg[d]; }
+    __device__ __forceinline__ double dcn() const { return (double)dc - (double)cdc; }
+};
+
+template <int LA, int LB, typename R>
 __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn,
                                           const double* vsh, int scc_on, int cn_on, int a, int sa, int sb, int ld,
-                                          gptr<const double> P, gptr<const double> W, double* g, double& dc) {
+                                          gptr<const double> P, gptr<const double> W, GradAcc<R>& acc) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
@@ -1190,7 +1259,8 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
     const double hij = 0.5 * ((A.hdiag + dse_a * cn[a]) + (B.hdiag + dse_b * cn[ob])) * hs;
     const double vv = scc_on ? vsh[sa] + vsh[sb] : 0.0;
     const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
-    double oc[NA * NB], pw[NA * NB];
+    // contraction coefficients P (2 h - v_mu - v_nu) - 2 W (gradient.rs:400-440)
+    R oc[NA * NB], pw[NA * NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         gptr<const double> Pc = P + (size_t)(mu0 + i) * ld + nu0;
@@ -1198,15 +1268,15 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const double p = Pc[j];
-            pw[i * NB + j] = p;
-            oc[i * NB + j] = p * (2.0 * hij - vv) - 2.0 * Wc[j];
+            pw[i * NB + j] = (R)p;
+            oc[i * NB + j] = (R)(p * (2.0 * hij - vv) - 2.0 * Wc[j]);
         }
     }
     constexpr int IM = LA + 1;
     constexpr int GM = (IM + LB) / 2 + 1;
     constexpr int order = LA + LB + 3;
     const double csmax = ao_abs_sum(LA, 1) * ao_abs_sum(LB, 1);
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0, sp = 0.0;
+    R g0 = 0, g1 = 0, g2 = 0, sp = 0;
     for (int ip = 0; ip < A.nprim; ++ip) {
         const double al = A.alpha[ip], cai = A.coeff[ip];
         for (int jp = 0; jp < B.nprim; ++jp) {
@@ -1259,11 +1329,11 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
                             ga1 += cf * (d1 * (m2 * m0));
                             ga2 += cf * (d2 * (m0 * m1));
                         }
-                    const double w = oc[mu * NB + nu];
-                    g0 += ga0 * w;
-                    g1 += ga1 * w;
-                    g2 += ga2 * w;
-                    sp += sv * pw[mu * NB + nu];
+                    const R w = oc[mu * NB + nu];
+                    g0 += (R)ga0 * w;
+                    g1 += (R)ga1 * w;
+                    g2 += (R)ga2 * w;
+                    sp += (R)sv * pw[mu * NB + nu];
                 }
             }
         }
@@ -1277,11 +1347,11 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
         dlp1 = dy * (dp / pl);
         dlp2 = dz * (dp / pl);
     }
-    const double hp2 = 2.0 * hij * sp;
-    g[0] += g0 + dlp0 * hp2;
-    g[1] += g1 + dlp1 * hp2;
-    g[2] += g2 + dlp2 * hp2;
-    if (cn_on) dc += dse_a * hs * sp;
+    const double hp2 = 2.0 * hij * (double)sp;
+    GradAcc<R>::add(acc.g[0], acc.cg[0], (R)((double)g0 + dlp0 * hp2));
+    GradAcc<R>::add(acc.g[1], acc.cg[1], (R)((double)g1 + dlp1 * hp2));
+    GradAcc<R>::add(acc.g[2], acc.cg[2], (R)((double)g2 + dlp2 * hp2));
+    if (cn_on) GradAcc<R>::add(acc.dc, acc.cdc, (R)(dse_a * hs * (double)sp));
 }
 
 }  // namespace
@@ -1335,7 +1405,7 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
 // Electronic gradient, pair part (gradient.rs:333-462) for atom A: every pair
 // with B != A inside the integral cutoff, A taken as the bra.  Deterministic
 // per-atom sums (one 64-lane wavefront per atom, no atomics).
-template <bool DCLASS>
+template <bool DCLASS, bool MIXED>
 __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn,
                                                      const double* vsh, int scc_on, int cn_on, double cutoff,
                                                      double* g_pair, double* dedcn_h) {
@@ -1348,7 +1418,8 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     const int sa0 = b.atom_sh0[a];
     const int nsha = t.elements[b.atom_z[a]].nshell;
     const int* lo = b.lofs + 4 * inst;
-    double g[3] = {0.0, 0.0, 0.0}, dc = 0.0;
+    using R = typename std::conditional<MIXED, float, double>::type;
+    GradAcc<R> acc;
     for (int k = 0; k < nsha; ++k) {
         const int sa = sa0 + k;
         const int la = t.shells[b.sh_type[sa]].l;
@@ -1362,25 +1433,26 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
                 if (r2 <= kPairEps2 || r2 > c2 || r2 <= 1e-12) continue;
                 if constexpr (!DCLASS) {
                     switch (la * 3 + lb) {
-                        case 0: grad_pair<0, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        case 1: grad_pair<0, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        case 3: grad_pair<1, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        default: grad_pair<1, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 0: grad_pair<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        case 1: grad_pair<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        case 3: grad_pair<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        default: grad_pair<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
                     }
                 } else {
                     switch (la * 3 + lb) {
-                        case 2: grad_pair<0, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        case 5: grad_pair<1, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        case 6: grad_pair<2, 0>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        case 7: grad_pair<2, 1>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
-                        default: grad_pair<2, 2>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, g, dc); break;
+                        case 2: grad_pair<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        case 5: grad_pair<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        case 6: grad_pair<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        case 7: grad_pair<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
+                        default: grad_pair<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc); break;
                     }
                 }
             }
         }
     }
-    for (int d = 0; d < 3; ++d) g[d] = wave_sum(g[d]);
-    dc = wave_sum(dc);
+    double g[3];
+    for (int d = 0; d < 3; ++d) g[d] = wave_sum(acc.value(d));
+    const double dc = wave_sum(acc.dcn());
     if (threadIdx.x == 0) {
         // the s/p pass initialises, the d pass accumulates
         for (int d = 0; d < 3; ++d) g_pair[3 * a + d] = DCLASS ? g_pair[3 * a + d] + g[d] : g[d];
@@ -1390,9 +1462,13 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
 
 template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const double*, double);
 template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const double*, double);
-template __global__ void k_grad_pairs_t<false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+template __global__ void k_grad_pairs_t<false, false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
                                                double, double*, double*);
-template __global__ void k_grad_pairs_t<true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+template __global__ void k_grad_pairs_t<true, false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+                                              double, double*, double*);
+template __global__ void k_grad_pairs_t<false, true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
+                                               double, double*, double*);
+template __global__ void k_grad_pairs_t<true, true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
                                               double, double*, double*);
 
 }  // namespace gfn1

@@ -36,6 +36,10 @@ __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int ei
 __global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
                                 const double* eps_all, const double* occ_all, int k, int weighted);
 __global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp);
+__global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp, double* const* A32p,
+                                double* const* C32p, const double* eps_all, const double* occ_all, int k,
+                                int weighted);
+__global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* Xp, double* const* Mp);
 __global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH);
 __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
                        double* qout, SccState* st);
@@ -50,7 +54,7 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
                            int natoms);
 __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal);
 __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
-template <bool DCLASS>
+template <bool DCLASS, bool MIXED>
 __global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
                                int cn_on, double cutoff, double* g_pair, double* dedcn_h);
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,

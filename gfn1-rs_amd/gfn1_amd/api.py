@@ -451,7 +451,7 @@ class BatchEngine:
 
     def __init__(self, systems, params: Optional[Gfn1Parameters] = None,
                  options: Optional[AnalyticGradientOptions] = None, eigensolver=0, keep_matrices=False,
-                 guess: Optional[GuessModel] = None):
+                 guess: Optional[GuessModel] = None, precision="fp64"):
         lib = _lib.load()
         self._lib = lib
         self.params = _params(params)
@@ -462,6 +462,11 @@ class BatchEngine:
         bo.gradient = g
         bo.eigensolver = int(eigensolver)
         bo.keep_matrices = int(bool(keep_matrices))
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        # mixed: FP32 MFMA density/energy-weighted-density products and FP32
+        # gradient contraction; eigensolve, mixing and energy sums stay FP64
+        bo.precision = 1 if precision == "mixed" else 0
         arr = (_lib.System * len(systems))()
         self._keep = []
         self.natoms = []

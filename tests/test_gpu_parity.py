@@ -106,3 +106,24 @@ def test_error_paths():
         analytic_gradient([1, 1], np.array([[0, 0, 0], [1.4, 0, 0]]), options=AnalyticGradientOptions(
             electronic=ElectronicOptions(max_scc=2, energy_tolerance=-1.0)))
     assert e.value.kind == "SccNotConverged"
+
+
+MIXED_E_TOL = 1e-6
+MIXED_G_TOL = 1e-5
+
+
+def test_mixed_precision_batch_matches_oracle():
+    """FP32 density/energy-weighted-density products and gradient contraction:
+    |dE| <= 1e-6 Eh, max |dg| <= 1e-5 Eh/bohr against the FP64 oracle."""
+    systems = geometry.mixed_batch(8, first_seed=4200) + [geometry.water_cluster(48, 11.3, seed=0)]
+    note("case mixed batch of %d" % len(systems))
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))
+    eng = BatchEngine(systems, options=opts, precision="mixed").run()
+    for i, (z, x) in enumerate(systems):
+        ref = oracle.run(z, x, **TIGHT)
+        assert eng.status(i) == 0
+        de = abs(eng.energy(i) - ref["energy"])
+        dg = float(np.abs(eng.gradient(i) - ref["gradient"]).max())
+        note("instance %d: n_atoms %d  dE %.2e  dg %.2e" % (i, len(z), de, dg))
+        assert de <= MIXED_E_TOL
+        assert dg <= MIXED_G_TOL
