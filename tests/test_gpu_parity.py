@@ -127,3 +127,22 @@ def test_mixed_precision_batch_matches_oracle():
         note("instance %d: n_atoms %d  dE %.2e  dg %.2e" % (i, len(z), de, dg))
         assert de <= MIXED_E_TOL
         assert dg <= MIXED_G_TOL
+
+
+def test_batch_matches_committed_fixtures():
+    """All fixture systems in one batch against tests/golden/oracle_fixtures.json."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_fixtures.json")) as f:
+        fx = json.load(f)
+    names = sorted(fx["systems"])
+    systems = [(np.array(fx["systems"][k]["numbers"]), np.array(fx["systems"][k]["positions"])) for k in names]
+    note("case fixtures batch of %d" % len(systems))
+    eng = BatchEngine(systems, options=AnalyticGradientOptions(electronic=ElectronicOptions(**fx["tolerances"]))).run()
+    for i, k in enumerate(names):
+        s = fx["systems"][k]
+        assert eng.status(i) == 0, k
+        assert abs(eng.energy(i) - s["energy"]) <= E_TOL, k
+        assert np.abs(eng.gradient(i) - np.array(s["gradient"])).max() <= G_TOL, k
+        q = eng.shell_charges(i, len(s["shell_charges"]))
+        assert np.abs(q - np.array(s["shell_charges"])).max() <= 1e-7, k

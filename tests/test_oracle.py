@@ -245,3 +245,21 @@ def test_overlap_derivative_matches_fd():
             xm = x.copy(); xm[a, k] -= h
             fd = (np.sum(W * oracle.overlap(z, xp)) - np.sum(W * oracle.overlap(z, xm))) / (2 * h)
             assert abs(fd - dS[a, k]) < 1e-8
+
+
+def _fixtures():
+    import json
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_fixtures.json")) as f:
+        return json.load(f)
+
+
+def test_oracle_reproduces_committed_fixtures():
+    """Regression pin: the oracle still produces the committed fixtures
+    (tests/golden/make_golden.py)."""
+    fx = _fixtures()
+    for name, s in fx["systems"].items():
+        r = oracle.run(np.array(s["numbers"]), np.array(s["positions"]), **fx["tolerances"])
+        assert r["status"] == 0, name
+        assert abs(r["energy"] - s["energy"]) <= 1e-10, name
+        assert np.abs(r["gradient"] - np.array(s["gradient"])).max() <= 1e-9, name
+        assert np.abs(r["shell_charges"] - np.array(s["shell_charges"])).max() <= 1e-8, name
