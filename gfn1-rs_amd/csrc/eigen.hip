@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
     double* ysp = sm + (size_t)2 * kVS * ld;  // per-wavefront W^T v, V^T v
     double* red = ysp + kTriWaves * 16;       // 16
     double* rowk = red + 16;                  // V[k, :], W[k, :]
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront-uniform
     const int r = tid;
     double vreg[kNB], wreg[kNB];
     for (int k0 = 0; k0 < n - 1; k0 += kNB) {
@@ -227,25 +228,38 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                         ++t;
                     }
                     if (t >= mt) break;
-                    const int rr = c0 + 64 * t + lane;
-                    const int cb = c0 + 32 * u;
+                    const int rb = c0 + 64 * t, cb = c0 + 32 * u;  // first row / column (uniform)
+                    const int rr = rb + lane;
                     const bool rowok = rr < n;
-                    double a[32];
-#pragma unroll
-                    for (int qq = 0; qq < 32; ++qq) {
-                        const int c = cb + qq;
-                        const bool ok = rowok && c < n && rr >= c;
-                        const double val = A[ok ? rr + (size_t)c * ld : 0];
-                        a[qq] = ok ? val : 0.0;
-                    }
                     const double vr = rowok ? vs[rr] : 0.0;
+                    double a[32];
                     double acc = 0.0;
+                    if (cb + 31 < rb && rb + 63 < n) {
+                        // interior tile: strictly below the diagonal and inside the matrix,
+                        // so every load is a uniform column base plus the lane's row
+                        const unsigned ru = (unsigned)rr;
 #pragma unroll
-                    for (int qq = 0; qq < 32; ++qq) {
-                        const int c = cb + qq;
-                        const double vc = vs[min(c, n - 1)];  // a[qq] = 0 for c >= n
-                        acc += a[qq] * vc;
-                        a[qq] = rr > c ? a[qq] * vr : 0.0;
+                        for (int qq = 0; qq < 32; ++qq) a[qq] = (A + (size_t)(cb + qq) * ld)[ru];
+#pragma unroll
+                        for (int qq = 0; qq < 32; ++qq) {
+                            acc += a[qq] * vs[cb + qq];
+                            a[qq] *= vr;
+                        }
+                    } else {
+                        // diagonal or edge tile: clamped addresses, masked values
+                        const unsigned rc = (unsigned)min(rr, n - 1);
+#pragma unroll
+                        for (int qq = 0; qq < 32; ++qq) {
+                            const int c = cb + qq;
+                            const double val = (A + (size_t)min(c, n - 1) * ld)[rc];
+                            a[qq] = (rowok && c < n && rr >= c) ? val : 0.0;
+                        }
+#pragma unroll
+                        for (int qq = 0; qq < 32; ++qq) {
+                            const int c = cb + qq;
+                            acc += a[qq] * vs[min(c, n - 1)];
+                            a[qq] = rr > c ? a[qq] * vr : 0.0;
+                        }
                     }
                     const double tot = reduce32(a, lane);
                     const int c = cb + reduce32_index(lane);
