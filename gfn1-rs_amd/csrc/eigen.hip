@@ -371,18 +371,55 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
     }
 }
 
-// number of eigenvalues of T strictly below x (Sturm sequence of T - x I)
-__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-    int c = 0;
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    if (q < 0.0) ++c;
-    for (int i = 1; i < n; ++i) {
-        q = d[i] - x - e2[i - 1] / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        if (q < 0.0) ++c;
+// Numbers of eigenvalues of T below x0 and below x1 (Sturm counts) from the
+// signs of the leading principal minors of T - x I,
+//     p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2},   p_{-1} = 1,
+// counting i where p_i and p_{i-1} differ in sign (the pivot q_i = p_i/p_{i-1}
+// of the LDL^T recurrence is negative).  No divisions: one multiply-add per step
+// on each chain, and the two shifts run as independent chains.  Both minors are
+// rescaled by a power of two every eight steps (signs and ratios unchanged).  A
+// minor that is exactly zero counts as non-negative (that only decides on which
+// side of an exact eigenvalue of a leading block the shift is counted) and is
+// replaced by a tiny negative pivot at the next rescaling.
+__device__ __forceinline__ void sturm_count2(const double* d, const double* e2, int n, double x0, double x1,
+                                             int& c0, int& c1) {
+    double p0 = d[0] - x0, q0 = 1.0;  // p_i, p_{i-1}
+    double p1 = d[0] - x1, q1 = 1.0;
+    int k0 = p0 < 0.0, k1 = p1 < 0.0;
+    auto step = [&](double di, double ei) {
+        const double n0 = fma(di - x0, p0, -ei * q0);
+        const double n1 = fma(di - x1, p1, -ei * q1);
+        k0 += (n0 < 0.0) != (p0 < 0.0);
+        k1 += (n1 < 0.0) != (p1 < 0.0);
+        q0 = p0;
+        p0 = n0;
+        q1 = p1;
+        p1 = n1;
+    };
+    int i = 1;
+    // eight steps per block: the sixteen LDS reads are issued before the arithmetic
+    for (; i + 8 <= n; i += 8) {
+        double dd[8], ee[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            dd[u] = d[i + u];
+            ee[u] = e2[i + u - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) step(dd[u], ee[u]);
+        // an exact zero minor followed by a zero off-diagonal would stall the
+        // recurrence: treat it as a tiny negative pivot (LAPACK's -pivmin)
+        if (p0 == 0.0) p0 = (q0 == 0.0 ? -1.0 : -q0) * 1e-280;
+        if (p1 == 0.0) p1 = (q1 == 0.0 ? -1.0 : -q1) * 1e-280;
+        const int s0 = -ilogb(p0), s1 = -ilogb(p1);
+        p0 = ldexp(p0, s0);
+        q0 = ldexp(q0, s0);
+        p1 = ldexp(p1, s1);
+        q1 = ldexp(q1, s1);
     }
-    return c;
+    for (; i < n; ++i) step(d[i], e2[i - 1]);
+    c0 = k0;
+    c1 = k1;
 }
 
 // deterministic pseudo-random start vector entry in (-1, 1)
@@ -523,17 +560,29 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     const double widen = 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
     lo -= widen;
     hi += widen;
-    // bisection: thread j finds the j-th smallest eigenvalue
-    for (int j = tid; j < n; j += nt) {
-        double a = lo, b = hi;
+    // bisection: thread t finds the eigenvalues j0 = t and j1 = t + nt together
+    for (int j0 = tid; j0 < n; j0 += 2 * nt) {
+        const int j1 = j0 + nt;
+        double a0 = lo, b0 = hi, a1 = lo, b1 = hi;
+        bool done0 = false, done1 = j1 >= n;
         for (int it = 0; it < 200; ++it) {
-            const double mid = 0.5 * (a + b);
-            if (mid <= a || mid >= b) break;
-            if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
-            if (sturm_count(D, E2, n, mid, pivmin) > j) b = mid;
-            else a = mid;
+            const double m0 = 0.5 * (a0 + b0), m1 = 0.5 * (a1 + b1);
+            done0 = done0 || m0 <= a0 || m0 >= b0 || b0 - a0 <= 2.0 * kEps * fmax(fabs(a0), fabs(b0)) + pivmin;
+            done1 = done1 || m1 <= a1 || m1 >= b1 || b1 - a1 <= 2.0 * kEps * fmax(fabs(a1), fabs(b1)) + pivmin;
+            if (done0 && done1) break;
+            int c0, c1;
+            sturm_count2(D, E2, n, m0, m1, c0, c1);
+            if (!done0) {
+                if (c0 > j0) b0 = m0;
+                else a0 = m0;
+            }
+            if (!done1) {
+                if (c1 > j1) b1 = m1;
+                else a1 = m1;
+            }
         }
-        lam[j] = 0.5 * (a + b);
+        lam[j0] = 0.5 * (a0 + b0);
+        if (j1 < n) lam[j1] = 0.5 * (a1 + b1);
     }
     __syncthreads();
     // ---- Fermi occupations (same semantics as k_fermi)
