@@ -863,11 +863,13 @@ __global__ void __launch_bounds__(512) k_backtransform(const int* act, const int
     for (int hi = n - 1; hi > 0; hi -= kBT) {
         const int lo = max(0, hi - kBT), bb = hi - lo, r0 = lo + 1, m = n - r0;
         __syncthreads();
+        // reflectors of the block into LDS (row-major, i fastest); the global
+        // reads walk down each reflector column
         for (int q = threadIdx.x; q < m * kBT; q += nt) {
-            const int r = r0 + q / kBT, i = q % kBT, kk = lo + i;
+            const int i = q / m, r = r0 + q % m, kk = lo + i;
             double v = 0.0;
             if (i < bb) v = r < kk + 1 ? 0.0 : (r == kk + 1 ? 1.0 : A[r + (size_t)kk * ld]);
-            Vb[q] = v;
+            Vb[(r - r0) * kBT + i] = v;
         }
         __syncthreads();
         for (int p = threadIdx.x; p < kBT * kBT; p += nt) {
@@ -879,23 +881,33 @@ __global__ void __launch_bounds__(512) k_backtransform(const int* act, const int
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int p = 0; p < kBT * kBT; ++p) Tm[p] = 0.0;
-            for (int i = 0; i < bb; ++i) {
-                const double ti = tau[lo + i];
-                Tm[i * kBT + i] = ti;
-                for (int a = 0; a < i; ++a) {
+        // T (upper triangular): T[i][i] = tau_i, T[a][i] = -tau_i sum_{c=a}^{i-1} T[a][c] G[c][i];
+        // row a depends only on its own earlier entries, so lane a builds row a
+        if (threadIdx.x < kBT) {
+            const int a = threadIdx.x;
+            double row[kBT];
+#pragma unroll
+            for (int i = 0; i < kBT; ++i) row[i] = 0.0;
+            if (a < bb) row[a] = tau[lo + a];
+#pragma unroll
+            for (int i = 1; i < kBT; ++i) {
+                if (i > a && i < bb) {
                     double acc = 0.0;
-                    for (int c = a; c < i; ++c) acc += Tm[a * kBT + c] * G[c * kBT + i];
-                    Tm[a * kBT + i] = -ti * acc;
+#pragma unroll
+                    for (int c = 0; c < kBT; ++c)
+                        if (c >= a && c < i) acc += row[c] * G[c * kBT + i];
+                    row[i] = -tau[lo + i] * acc;
                 }
             }
+#pragma unroll
+            for (int i = 0; i < kBT; ++i) Tm[a * kBT + i] = row[i];
         }
         __syncthreads();
         if (on) {
             double y[kBT];
 #pragma unroll
             for (int i = 0; i < kBT; ++i) y[i] = 0.0;
+#pragma unroll 4
             for (int r = 0; r < m; ++r) {
                 const double z = Z[(size_t)(r0 + r) * kcap + j];
                 const double* vrow = Vb + r * kBT;
@@ -909,6 +921,7 @@ __global__ void __launch_bounds__(512) k_backtransform(const int* act, const int
                 for (int c = i; c < kBT; ++c) acc += Tm[i * kBT + c] * y[c];
                 y[i] = acc;
             }
+#pragma unroll 4
             for (int r = 0; r < m; ++r) {
                 const double* vrow = Vb + r * kBT;
                 double upd = 0.0;
