@@ -38,13 +38,14 @@ namespace gfn1 {
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
 namespace {
-enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, kRoles };
+// R_L: Cholesky factor of S; R_A0: Cb^T H0 Cb (both fixed for the run)
+enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, R_L, R_A0, kRoles };
 // A_C32: FP32 operand behind the A32 operand in the A matrix (mixed precision)
 // Above this electronic temperature (K) the occupied-plus-smeared orbitals are not
 // bounded by the occupied count, so all eigenvectors are kept.
 constexpr double kWideSmearing = 1000.0;
 
-enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, kActRoles };
+enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, A_H0, A_L, A_A0, kActRoles };
 constexpr int kWarmResetPeriod = 64;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -388,6 +389,9 @@ Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity
         hp[(size_t)A_S * list.size() + j] = at(R_S);
         hp[(size_t)A_X * list.size() + j] = at(R_W);
         hp[(size_t)A_C32 * list.size() + j] = (double*)((float*)at(R_A) + (size_t)bk.ld * bk.kcap);
+        hp[(size_t)A_H0 * list.size() + j] = at(R_H0);
+        hp[(size_t)A_L * list.size() + j] = at(R_L);
+        hp[(size_t)A_A0 * list.size() + j] = at(R_A0);
     }
     HIP_TRY(hipMemcpyAsync(bk.ptrs, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
     HIP_TRY(hipMemcpyAsync(bk.act, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, bk.stream));
@@ -541,6 +545,9 @@ Status Batch::setup_phase() {
         double* const* A = bk.ptrs + (size_t)A_A * count;
         RB_TRY(rocsolver_dpotrf_batched(bk.handle, rocblas_fill_lower, bk.ld, (double* const*)A, bk.ld, bk.info,
                                         count));
+        // keep L (S = L L^T) for the reduced congruence of the SCC iterations
+        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_L * count * mat, bk.arena + (size_t)R_A * count * mat,
+                               count * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
         RB_TRY(rocsolver_dtrtri_batched(bk.handle, rocblas_fill_lower, rocblas_diagonal_non_unit, bk.ld,
                                         (double* const*)A, bk.ld, bk.info, count));
         std::vector<int> info(count);
@@ -554,6 +561,21 @@ Status Batch::setup_phase() {
         hipLaunchKernelGGL(k_make_cb, dim3(64, count), dim3(256), 0, bk.stream, db_, A, (const int*)bk.act,
                            opt_.eigen_tolerance, d_fail_);
         LAUNCH_CHECK();
+        // A0 = Cb^T H0 Cb
+        {
+            const double one = 1.0, zero = 0.0;
+            double* const* Cb = bk.ptrs + (size_t)A_CB * count;
+            double* const* H0 = bk.ptrs + (size_t)A_H0 * count;
+            double* const* T = bk.ptrs + (size_t)A_T * count;
+            double* const* A0 = bk.ptrs + (size_t)A_A0 * count;
+            const int n = bk.ld;
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                         (const double* const*)H0, n, (const double* const*)Cb, n, &zero,
+                                         (double* const*)T, n, count));
+            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
+                                         &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
+                                         (double* const*)A0, n, count));
+        }
     }
     HIP_TRY(hipDeviceSynchronize());
     std::vector<int> fail(ninst_);
@@ -597,13 +619,17 @@ Status Batch::scc_phase() {
             hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
                                t, (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
             LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               (const double*)d_vin_t_);
-            LAUNCH_CHECK();
             // level shift F += b (S - 1/2 S P_prev S) for instances that carry a previous density
             std::vector<int> ls;
             for (int i : bk.active)
                 if (h_state_[i].have_prev_p && h_state_[i].level_shift != 0.0) ls.push_back(i);
+            // F itself is formed only for the result matrices and the level-shift rungs;
+            // otherwise A comes from the reduced congruence below
+            if (opt_.keep_matrices || !ls.empty()) {
+                hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                                   (const double*)d_vin_t_);
+                LAUNCH_CHECK();
+            }
             if (!ls.empty()) {
                 st = upload_active(bk, ls, 0);
                 if (!st.ok()) return st;
@@ -634,13 +660,21 @@ Status Batch::scc_phase() {
             double* const* Z = bk.ptrs + (size_t)A_C * cnt;    // eigenvectors of the reduced problem
             double* const* T = bk.ptrs + (size_t)A_T * cnt;
             double* const* A = bk.ptrs + (size_t)A_A * cnt;
-            // ---- A = Cb^T F Cb (orthonormal basis)
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                                         (const double* const*)F, n, (const double* const*)Cb, n, &zero,
-                                         (double* const*)T, n, cnt));
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
-                                         &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
-                                         (double* const*)A, n, cnt));
+            // ---- A = Cb^T F Cb (orthonormal basis; lower triangle and diagonal are used)
+            if (!ls.empty()) {
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
+                                             &one, (const double* const*)F, n, (const double* const*)Cb, n, &zero,
+                                             (double* const*)T, n, cnt));
+                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n,
+                                             n, &one, (const double* const*)Cb, n, (const double* const*)T, n,
+                                             &zero, (double* const*)A, n, cnt));
+            } else {
+                const int nt64 = (n + 63) / 64;
+                hipLaunchKernelGGL(k_reduce_fock, dim3(nt64, nt64, cnt), dim3(256), 0, bk.stream, db_,
+                                   (const int*)bk.act, (const double*)d_vin_t_, Cb,
+                                   bk.ptrs + (size_t)A_L * cnt, bk.ptrs + (size_t)A_A0 * cnt, A);
+                LAUNCH_CHECK();
+            }
             const int kc = bk.kcap;
             // matrices above kEigMaxDim go to rocSOLVER
             const bool own_solver = opt_.eigensolver != 1 && n <= kEigMaxDim;

@@ -355,6 +355,72 @@ __global__ void k_fock(DeviceBatch b, const int* act, const double* vt) {
     }
 }
 
+// Lower triangle of A = Cb^T F Cb without forming F.  With F = H0 - (V S + S V)/2
+// (electronic.rs:3133-3158, V = diag of the AO potential), S = L L^T and
+// Cb = L^-T:
+//     A = A0 - (M + M^T) / 2,   A0 = Cb^T H0 Cb,   M = L^-1 V L  (lower triangular),
+// so A_ij = A0_ij - M_ij / 2 below the diagonal and A_ii = A0_ii - v_i.  M costs
+// n^3 / 3 flops against 4 n^3 for the two GEMMs of the explicit congruence.  One
+// 64 x 64 tile of the lower triangle per work-group (256 threads, 4 x 4 outputs
+// each); M_ij = sum_{k=j..i} Linv_ik v_k L_kj runs over 16-row chunks in LDS.
+__global__ void __launch_bounds__(256) k_reduce_fock(DeviceBatch b, const int* act, const double* vt,
+                                                     double* const* Cbp, double* const* Lp, double* const* A0p,
+                                                     double* const* Ap) {
+    const int ti = blockIdx.x, tj = blockIdx.y;
+    if (tj > ti) return;
+    const int inst = act[blockIdx.z];
+    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int I0 = ti * 64, J0 = tj * 64;
+    if (I0 >= ld) return;
+    const int a0 = b.ao0[inst];
+    gptr<const double> Cb = (gptr<const double>)Cbp[blockIdx.z];
+    gptr<const double> L = (gptr<const double>)Lp[blockIdx.z];
+    gptr<const double> A0 = (gptr<const double>)A0p[blockIdx.z];
+    gptr<double> A = (gptr<double>)Ap[blockIdx.z];
+    __shared__ double Bi[16][64], Bj[16][64];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[p][c] = 0.0;
+    const int kend = min(I0 + 64, ld);
+    for (int k0 = J0; k0 < kend; k0 += 16) {
+        for (int q = threadIdx.x; q < 16 * 64; q += 256) {
+            const int kk = q & 15, ii = q >> 4;
+            const int k = k0 + kk, i = I0 + ii, j = J0 + ii;
+            const double vk = k < n ? vt[b.ao_sh[a0 + k]] : 0.0;
+            Bi[kk][ii] = (k < ld && i < ld && k <= i) ? Cb[k + (size_t)i * ld] * vk : 0.0;
+            Bj[kk][ii] = (k < ld && j < ld && k >= j) ? L[k + (size_t)j * ld] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            double ai[4], bj[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) ai[p] = Bi[kk][ty * 4 + p];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bj[c] = Bj[kk][tx * 4 + c];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[p][c] += ai[p] * bj[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = I0 + ty * 4 + p, j = J0 + tx * 4 + c;
+            if (i < ld && j < ld && i >= j) {
+                const size_t o = i + (size_t)j * ld;
+                if (i == j) A[o] = A0[o] - (i < n ? vt[b.ao_sh[a0 + i]] : 0.0);
+                else A[o] = A0[o] - 0.5 * acc[p][c];
+            }
+        }
+}
+
 // F += b (S - 1/2 S P S) for instances on a level-shift rung (electronic.rs:2120-2133).
 __global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift, double* const* SPS) {
     const int inst = act[blockIdx.y];

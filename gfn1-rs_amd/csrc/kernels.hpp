@@ -36,6 +36,8 @@ __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int ei
 __global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
                                 const double* eps_all, const double* occ_all, int k, int weighted);
 __global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp);
+__global__ void k_reduce_fock(DeviceBatch b, const int* act, const double* vt, double* const* Cbp, double* const* Lp,
+                              double* const* A0p, double* const* Ap);
 __global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp, double* const* A32p,
                                 double* const* C32p, const double* eps_all, const double* occ_all, int k,
                                 int weighted);
