@@ -576,6 +576,13 @@ Status Batch::setup_phase() {
                                          &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
                                          (double* const*)A0, n, count));
         }
+        // The buckets are set up one after another.  With this bucket's batched
+        // GEMMs still running on its stream while the next bucket's rocSOLVER
+        // potrf/trtri ran on another stream, about 1% of the instances of the
+        // buckets of 256 AO and more converged to wrong energies (differences up
+        // to 1e-2 Eh, different instances on every run, on MI355X / ROCm 7); with
+        // this synchronisation every run is reproducible.
+        HIP_TRY(hipStreamSynchronize(bk.stream));
     }
     HIP_TRY(hipDeviceSynchronize());
     std::vector<int> fail(ninst_);
@@ -701,8 +708,11 @@ Status Batch::scc_phase() {
                                              n, &one, (const double* const*)Cb, n, (const double* const*)Z, kc,
                                              &zero, (double* const*)T, n, cnt));
             } else {
+                // rocSOLVER runs alone on the device, as in the setup phase
+                HIP_TRY(hipDeviceSynchronize());
                 Status st2 = eigensolve(bk, cnt);
                 if (!st2.ok()) return st2;
+                HIP_TRY(hipStreamSynchronize(bk.stream));
                 hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                                    (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
                 LAUNCH_CHECK();
