@@ -77,38 +77,6 @@ __device__ double bsum(double v, double* red) {
 // exchanges are DPP moves inside a 16-lane row; only bits 4 and 5 of the lane
 // index go through the cross-row permute.
 //
-// reduce32: lane L receives the total of s[q], q = 16 b2 + 8 b1 + 4 b0 + 2 b3 + b4
-// (bk = bit k of L); lanes L and L + 32 hold the same total.  s is overwritten.
-__device__ __forceinline__ double reduce32(double (&s)[32], int lane) {
-    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4, b3 = lane & 8, b4 = lane & 16;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const double send = b2 ? s[q] : s[q + 16];
-        s[q] = (b2 ? s[q + 16] : s[q]) + dpp<kDppHalfMirror>(send);
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const double send = b1 ? s[q] : s[q + 8];
-        s[q] = (b1 ? s[q + 8] : s[q]) + dpp<kDppXor2>(send);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const double send = b0 ? s[q] : s[q + 4];
-        s[q] = (b0 ? s[q + 4] : s[q]) + dpp<kDppXor1>(send);
-    }
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const double send = b3 ? s[q] : s[q + 2];
-        s[q] = (b3 ? s[q + 2] : s[q]) + dpp<kDppRor8>(send);
-    }
-    const double send = b4 ? s[0] : s[1];
-    double t = (b4 ? s[1] : s[0]) + __shfl_xor(send, 16, 64);
-    return t + __shfl_xor(t, 32, 64);
-}
-__device__ __forceinline__ int reduce32_index(int lane) {
-    return ((lane >> 2) & 1) * 16 + ((lane >> 1) & 1) * 8 + (lane & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 4) & 1);
-}
-
 // reduce16: lane L receives the total of s[q], q = 8 b2 + 4 b1 + 2 b0 + b3; every
 // lane with the same low four bits holds the same total.  s is overwritten.
 __device__ __forceinline__ double reduce16(double (&s)[16], int lane) {
@@ -144,9 +112,10 @@ __device__ __forceinline__ int reduce16_index(int lane) {
 // panel the trailing matrix A - V W^T - W V^T is used without extra memory
 // traffic; the rank-2b update is applied once per panel in 64 x 64 tiles with
 // V and W staged through LDS.  The matrix-vector product reads the lower
-// triangle once, in 64 x 64 tiles shared out over the wavefronts.
+// triangle once, in 64 x 16 tiles shared out over the wavefronts.
 constexpr int kTriThreads = 512;
 constexpr int kTriWaves = kTriThreads / 64;
+constexpr int kTW = 16;  // columns per symv tile (64 x 16): keeps the kernel within 128 VGPRs, two work-groups per CU
 constexpr int kNB = kEigPanel;
 constexpr int kVS = kNB + 1;  // LDS row stride of the staged panel (odd: no bank conflicts)
 
@@ -213,35 +182,36 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                 tau[k] = tk;
             }
             __syncthreads();
-            // ---- A_panel v over the lower triangle of the trailing block, in 64 x 32
-            //      tiles (rows c0 + 64t + lane, columns c0 + 32u .. +31, u <= 2t + 1);
-            //      the row and column contributions of a wavefront's tiles go to its
-            //      own slice of part, so the sum is deterministic
+            // ---- A_panel v over the lower triangle of the trailing block, in 64 x kTW
+            //      tiles (rows c0 + 64t + lane, columns c0 + kTW u .. + kTW - 1, with
+            //      u < (64 / kTW)(t + 1)); the row and column contributions of a
+            //      wavefront's tiles go to its own slice of part, so the sum is
+            //      deterministic
             {
                 double* slice = part + wid * ld;
                 for (int rr = c0 + lane; rr < n; rr += 64) slice[rr] = 0.0;
-                const int mt = (n - c0 + 63) >> 6, mu = (n - c0 + 31) >> 5;
+                const int mt = (n - c0 + 63) >> 6, mu = (n - c0 + kTW - 1) / kTW;
                 int t = 0, u = wid;  // tile q = wid, wid + kTriWaves, ... in row-block order
                 for (;;) {
-                    while (t < mt && u >= min(2 * t + 2, mu)) {
-                        u -= min(2 * t + 2, mu);
+                    while (t < mt && u >= min((64 / kTW) * (t + 1), mu)) {
+                        u -= min((64 / kTW) * (t + 1), mu);
                         ++t;
                     }
                     if (t >= mt) break;
-                    const int rb = c0 + 64 * t, cb = c0 + 32 * u;  // first row / column (uniform)
+                    const int rb = c0 + 64 * t, cb = c0 + kTW * u;  // first row / column (uniform)
                     const int rr = rb + lane;
                     const bool rowok = rr < n;
                     const double vr = rowok ? vs[rr] : 0.0;
-                    double a[32];
+                    double a[kTW];
                     double acc = 0.0;
-                    if (cb + 31 < rb && rb + 63 < n) {
+                    if (cb + kTW - 1 < rb && rb + 63 < n) {
                         // interior tile: strictly below the diagonal and inside the matrix,
                         // so every load is a uniform column base plus the lane's row
                         const unsigned ru = (unsigned)rr;
 #pragma unroll
-                        for (int qq = 0; qq < 32; ++qq) a[qq] = (A + (size_t)(cb + qq) * ld)[ru];
+                        for (int qq = 0; qq < kTW; ++qq) a[qq] = (A + (size_t)(cb + qq) * ld)[ru];
 #pragma unroll
-                        for (int qq = 0; qq < 32; ++qq) {
+                        for (int qq = 0; qq < kTW; ++qq) {
                             acc += a[qq] * vs[cb + qq];
                             a[qq] *= vr;
                         }
@@ -249,21 +219,21 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                         // diagonal or edge tile: clamped addresses, masked values
                         const unsigned rc = (unsigned)min(rr, n - 1);
 #pragma unroll
-                        for (int qq = 0; qq < 32; ++qq) {
+                        for (int qq = 0; qq < kTW; ++qq) {
                             const int c = cb + qq;
                             const double val = (A + (size_t)min(c, n - 1) * ld)[rc];
                             a[qq] = (rowok && c < n && rr >= c) ? val : 0.0;
                         }
 #pragma unroll
-                        for (int qq = 0; qq < 32; ++qq) {
+                        for (int qq = 0; qq < kTW; ++qq) {
                             const int c = cb + qq;
                             acc += a[qq] * vs[min(c, n - 1)];
                             a[qq] = rr > c ? a[qq] * vr : 0.0;
                         }
                     }
-                    const double tot = reduce32(a, lane);
-                    const int c = cb + reduce32_index(lane);
-                    if (lane < 32 && c < n) slice[c] += tot;
+                    const double tot = reduce16(a, lane);
+                    const int c = cb + reduce16_index(lane);
+                    if (lane < kTW && c < n) slice[c] += tot;
                     if (rowok) slice[rr] += acc;
                     u += kTriWaves;
                 }
