@@ -459,15 +459,75 @@ __device__ void jacobi_small(double* H, double* R, int m, int K) {
     }
 }
 
+// Eigenvector of T for an isolated eigenvalue x by a twisted factorisation
+// (K. V. Fernando, SIAM J. Matrix Anal. Appl. 18 (1997) 1013; Dhillon & Parlett,
+// Linear Algebra Appl. 387 (2004) 1):
+//     T - x I = L+ D+ L+^T (top down) = U- D- U-^T (bottom up),
+// gamma_r = D+_r - b_r U-_r; at the twist index r of smallest |gamma_r| the
+// vector solving N_r Delta_r N_r^T z = gamma_r e_r is z_r = 1,
+// z_i = -L+_i z_{i+1} (i < r), z_i = -U-_{i-1} z_{i-1} (i > r), L+_i = b_i / D+_i,
+// U-_i = b_i / D-_{i+1}.  Three passes over per-vector columns of Dp, Lp, Um
+// and Z (element i at i * kcap); the up-solve runs inside the bottom-up pass,
+// restarted whenever a better twist index appears.  Returns false if the
+// vector overflowed (the caller then uses inverse iteration).
+__device__ bool twisted_vector(const double* D, const double* Eo, int n, double x, double pivmin, int kcap,
+                               gptr<double> Z, double* __restrict__ Dp, double* __restrict__ Lp,
+                               double* __restrict__ Um) {
+    double dp = D[0] - x;
+    for (int i = 0; i < n - 1; ++i) {
+        if (fabs(dp) < pivmin) dp = -pivmin;
+        const size_t o = (size_t)i * kcap;
+        const double lf = Eo[i] / dp;
+        Dp[o] = dp;
+        Lp[o] = lf;
+        dp = (D[i + 1] - x) - lf * Eo[i];
+    }
+    if (fabs(dp) < pivmin) dp = -pivmin;
+    double dm = D[n - 1] - x, gbest = dp, zr = 1.0, up2 = 1.0;
+    int r = n - 1;
+    Z[(size_t)(n - 1) * kcap] = 1.0;
+    for (int i = n - 2; i >= 0; --i) {
+        if (fabs(dm) < pivmin) dm = -pivmin;
+        const size_t o = (size_t)i * kcap;
+        const double u = Eo[i] / dm;
+        Um[o] = u;
+        const double g = Dp[o] - Eo[i] * u;
+        dm = (D[i] - x) - u * Eo[i];
+        double zi;
+        if (fabs(g) < fabs(gbest)) {
+            gbest = g;
+            r = i;
+            zi = 1.0;
+            up2 = 0.0;
+        } else {
+            zi = -Lp[o] * zr;
+        }
+        Z[o] = zi;
+        up2 += zi * zi;
+        zr = zi;
+    }
+    double zd = 1.0, dn2 = 0.0;
+    for (int i = r + 1; i < n; ++i) {
+        zd = -Um[(size_t)(i - 1) * kcap] * zd;
+        Z[(size_t)i * kcap] = zd;
+        dn2 += zd * zd;
+    }
+    const double nrm2 = up2 + dn2;
+    if (!(nrm2 > 0.0 && nrm2 < INFINITY)) return false;
+    const double inv = 1.0 / sqrt(nrm2);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * kcap] *= inv;
+    return true;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Eigenvalues, occupations (electronic.rs:3853-3921) and occupied eigenvectors
-// of T.  LDS: 5 ld + 2 kEigMaxCluster^2 + 16 doubles + ld ints.
+// of T.  LDS: 5 ld + 2 kEigMaxCluster^2 + 16 doubles + 3 ld ints.
 __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const int* nvec, int ld, const double* dvec,
                                                           const double* evec, double* eps_all, const int64_t* vec0,
                                                           double* occ_all, SccState* st, double* const* Ztptr,
-                                                          double* lu, int kcap) {
+                                                          double* lu, int kcap, int full_spectrum) {
     extern __shared__ double sm[];
     const int l = blockIdx.x;
     const int inst = act[l];
@@ -484,6 +544,8 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     double* Rc = Hc + KC * KC;  // its eigenvectors
     double* red = Rc + KC * KC;
     int* tight = (int*)(red + 16);  // 1 where lam[j] joins the group of lam[j-1]
+    int* vflag = tight + ld;         // 1 where eigenvector j is formed by inverse iteration
+    int* order = vflag + ld;         // eigenvector indices of the current pass
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int i = tid; i < n; i += nt) {
         D[i] = dg[i];
@@ -530,95 +592,114 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     const double widen = 2.0 * kEps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
     lo -= widen;
     hi += widen;
-    // bisection: thread t finds the eigenvalues j0 = t and j1 = t + nt together
-    for (int j0 = tid; j0 < n; j0 += 2 * nt) {
-        const int j1 = j0 + nt;
-        double a0 = lo, b0 = hi, a1 = lo, b1 = hi;
-        bool done0 = false, done1 = j1 >= n;
-        for (int it = 0; it < 200; ++it) {
-            const double m0 = 0.5 * (a0 + b0), m1 = 0.5 * (a1 + b1);
-            done0 = done0 || m0 <= a0 || m0 >= b0 || b0 - a0 <= 2.0 * kEps * fmax(fabs(a0), fabs(b0)) + pivmin;
-            done1 = done1 || m1 <= a1 || m1 >= b1 || b1 - a1 <= 2.0 * kEps * fmax(fabs(a1), fabs(b1)) + pivmin;
-            if (done0 && done1) break;
-            int c0, c1;
-            sturm_count2(D, E2, n, m0, m1, c0, c1);
-            if (!done0) {
-                if (c0 > j0) b0 = m0;
-                else a0 = m0;
-            }
-            if (!done1) {
-                if (c1 > j1) b1 = m1;
-                else a1 = m1;
-            }
-        }
-        lam[j0] = 0.5 * (a0 + b0);
-        if (j1 < n) lam[j1] = 0.5 * (a1 + b1);
-    }
-    __syncthreads();
-    // ---- Fermi occupations (same semantics as k_fermi)
+    // ---- eigenvalues by Sturm-count trisection, one per thread (the two count
+    //      chains split the bracket into thirds).  The result arrays of a single
+    //      system need every orbital energy; otherwise only the lowest m are
+    //      formed (the occupied ones plus a margin) and the set is extended to all
+    //      n when the Fermi window reaches the last computed eigenvalue.
     SccState& s = st[inst];
     const double kt = s.kt, nelec = s.nelec;
     double* eps = eps_all + vec0[inst];
     double* occ = occ_all + vec0[inst];
-    for (int i = tid; i < ld; i += nt) eps[i] = i < n ? lam[i] : kPadEnergy;
+    int m = full_spectrum ? n : min(n, (int)ceil(0.5 * nelec) + kEigMargin);
+    int have = 0;
     double mu = 0.0, ts = 0.0;
     int kneed = 0;
-    const double mn = lam[0], mx = lam[n - 1];
-    if (kt <= 0.0) {
-        if (tid == 0) {
-            double rem = fmax(nelec, 0.0);
-            int homo = 0;
-            for (int i = 0; i < ld; ++i) {
-                double f = 0.0;
-                if (i < n && rem > 0.0) {
-                    f = fmin(rem, 2.0);
-                    rem -= f;
-                }
-                occ[i] = f;
-                if (i < n && f > 1e-12) homo = i;
+    // occupation and entropy term of an orbital 80 kT or more above the Fermi level:
+    // the exponent is clamped there, so every such orbital contributes exactly these
+    const double f_tail = 2.0 / (1.0 + exp(80.0));
+    double ts_tail = 0.0;
+    {
+        const double h = fmin(1.0 - 1e-16, fmax(1e-16, 0.5 * f_tail));
+        ts_tail = 2.0 * kt * (h * log(h) + (1.0 - h) * log(1.0 - h));
+    }
+    for (;;) {
+        for (int j = have + tid; j < m; j += nt) {
+            double a = lo, b = hi;
+            for (int it = 0; it < 200; ++it) {
+                const double w = b - a;
+                const double x0 = a + w * (1.0 / 3.0), x1 = a + w * (2.0 / 3.0);
+                if (!(x0 > a && x1 > x0 && b > x1) || w <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
+                int c0, c1;
+                sturm_count2(D, E2, n, x0, x1, c0, c1);
+                if (c0 > j) b = x0;
+                else if (c1 > j) {
+                    a = x0;
+                    b = x1;
+                } else a = x1;
             }
-            mu = lam[homo];
-            kneed = homo + 1;
-            red[0] = mu;
-            red[1] = (double)kneed;
+            lam[j] = 0.5 * (a + b);
         }
         __syncthreads();
-        mu = red[0];
-        kneed = (int)red[1];
-        __syncthreads();
-    } else if (nelec <= 0.0 || nelec >= 2.0 * n) {
-        const double f = nelec <= 0.0 ? 0.0 : 2.0;
-        for (int i = tid; i < ld; i += nt) occ[i] = i < n ? f : 0.0;
-        mu = nelec <= 0.0 ? lam[0] : lam[n - 1];
-        kneed = nelec <= 0.0 ? 0 : n;
-    } else {
+        const int tail = n - m;  // orbitals not formed, all >= 80 kT above mu
+        // ---- Fermi occupations (same semantics as k_fermi)
+        const double mn = lam[0], mx = tail ? hi : lam[n - 1];
+        if (kt <= 0.0) {
+            if (tid == 0) {
+                double rem = fmax(nelec, 0.0);
+                int homo = 0;
+                for (int i = 0; i < ld; ++i) {
+                    double f = 0.0;
+                    if (i < n && rem > 0.0) {
+                        f = fmin(rem, 2.0);
+                        rem -= f;
+                    }
+                    occ[i] = f;
+                    if (i < n && f > 1e-12) homo = i;
+                }
+                mu = lam[homo];
+                kneed = homo + 1;
+                red[0] = mu;
+                red[1] = (double)kneed;
+            }
+            __syncthreads();
+            mu = red[0];
+            kneed = (int)red[1];
+            __syncthreads();
+            break;
+        } else if (nelec <= 0.0 || nelec >= 2.0 * n) {
+            const double f = nelec <= 0.0 ? 0.0 : 2.0;
+            for (int i = tid; i < ld; i += nt) occ[i] = i < n ? f : 0.0;
+            mu = nelec <= 0.0 ? lam[0] : lam[n - 1];
+            kneed = nelec <= 0.0 ? 0 : n;
+            break;
+        }
         double a = mn - 100.0 * kt - 10.0, b = mx + 100.0 * kt + 10.0;
         for (int it = 0; it < 200; ++it) {
-            const double m = 0.5 * (a + b);
+            const double mid = 0.5 * (a + b);
             double part = 0.0;
-            for (int i = tid; i < n; i += nt) {
-                const double xx = fmin(80.0, fmax(-80.0, (lam[i] - m) / kt));
+            for (int i = tid; i < m; i += nt) {
+                const double xx = fmin(80.0, fmax(-80.0, (lam[i] - mid) / kt));
                 part += 2.0 / (1.0 + exp(xx));
             }
+            if (tid == 0) part += tail * f_tail;
             const double sum = bsum(part, red);
             double na = a, nb = b;
-            if (sum < nelec) na = m;
-            else nb = m;
+            if (sum < nelec) na = mid;
+            else nb = mid;
             if (na == a && nb == b) break;
             a = na;
             b = nb;
         }
         mu = 0.5 * (a + b);
-        double tpart = 0.0;
+        if (tail && lam[m - 1] - mu < 80.0 * kt) {
+            // the smearing reaches beyond the formed eigenvalues: form the rest
+            have = m;
+            m = n;
+            continue;
+        }
+        double tpart = tid == 0 ? tail * ts_tail : 0.0;
         int kpart = 0;
         for (int i = tid; i < ld; i += nt) {
             double f = 0.0;
-            if (i < n) {
+            if (i < m) {
                 const double xx = fmin(80.0, fmax(-80.0, (lam[i] - mu) / kt));
                 f = 2.0 / (1.0 + exp(xx));
                 const double h = fmin(1.0 - 1e-16, fmax(1e-16, 0.5 * f));
                 tpart += 2.0 * kt * (h * log(h) + (1.0 - h) * log(1.0 - h));
                 if (f >= 1e-28) kpart = max(kpart, i + 1);
+            } else if (i < n) {
+                f = f_tail;
             }
             occ[i] = f;
         }
@@ -632,7 +713,11 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         for (int q = 1; q < nw; ++q) km = max(km, skm[q]);
         kneed = km;
         __syncthreads();
+        break;
     }
+    // orbital energies that were not formed are reported as the Gershgorin bound
+    // (they only reach the batch's internal arrays, with zero eigenvector columns)
+    for (int i = tid; i < ld; i += nt) eps[i] = i < m ? lam[i] : (i < n ? hi : kPadEnergy);
     if (tid == 0) {
         s.mu = mu;
         s.ts = ts;
@@ -663,9 +748,35 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     double* __restrict__ U3 = rU1 + 2 * fld;              // second superdiagonal (row interchanges)
     double* __restrict__ Lm = rU1 + 3 * fld;              // multipliers
     double* __restrict__ Pv = rU1 + 4 * fld;              // 1 where rows i, i+1 were interchanged
-    // ---- inverse iteration, one thread per eigenvector; element (i, j) of every
-    //      per-vector array sits at i * kcap + j so a wavefront's accesses coalesce
-    for (int j = tid; j < kneed; j += nt) {
+    // ---- eigenvectors, one thread each; element (i, j) of every per-vector array
+    //      sits at i * kcap + j so a wavefront's accesses coalesce.  Isolated
+    //      eigenvalues: twisted factorisation; members of groups (and any twisted
+    //      vector that overflowed): inverse iteration from distinct start vectors,
+    //      then the group treatment below.
+    __shared__ int npass;
+    for (int j = tid; j < kneed; j += nt) vflag[j] = (tight[j] || (j + 1 < kneed && tight[j + 1])) ? 1 : 0;
+    __syncthreads();
+    if (tid == 0) {
+        int c = 0;
+        for (int j = 0; j < kneed; ++j)
+            if (!vflag[j]) order[c++] = j;
+        npass = c;
+    }
+    __syncthreads();
+    for (int t = tid; t < npass; t += nt) {
+        const int j = order[t];
+        if (!twisted_vector(D, Eo, n, lam[j], pivmin, kcap, Z + j, rU1 + j, U2 + j, U3 + j)) vflag[j] = 1;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int c = 0;
+        for (int j = 0; j < kneed; ++j)
+            if (vflag[j]) order[c++] = j;
+        npass = c;
+    }
+    __syncthreads();
+    for (int t = tid; t < npass; t += nt) {
+        const int j = order[t];
         const double x = sh[j];
         double a = D[0] - x, bb = n > 1 ? Eo[0] : 0.0;
         for (int i = 0; i < n - 1; ++i) {
@@ -922,12 +1033,12 @@ hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* n
 
 hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const int* nvec, int ld, const double* d,
                               const double* e, double* eps_all, const int64_t* vec0, double* occ_all, SccState* st,
-                              double* const* Zt, double* lu, int kcap) {
+                              double* const* Zt, double* lu, int kcap, bool full_spectrum) {
     if (ld > kEigMaxDim || kcap > ld) return hipErrorInvalidValue;
     const size_t lds = ((size_t)5 * ld + 2 * kEigMaxCluster * kEigMaxCluster + 16) * sizeof(double) +
-                       (size_t)ld * sizeof(int);
+                       (size_t)3 * ld * sizeof(int);
     hipLaunchKernelGGL(k_tridiag_eig, dim3(count), dim3(kThreads), lds, s, act, nvec, ld, d, e, eps_all, vec0,
-                       occ_all, st, Zt, lu, kcap);
+                       occ_all, st, Zt, lu, kcap, full_spectrum ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@ namespace gfn1 {
 constexpr int kEigMaxDim = 512;     // largest matrix the kernels accept
 constexpr int kEigPanel = 8;        // Householder panel width
 constexpr int kEigMaxCluster = 32;  // largest tight eigenvalue cluster resolved by Rayleigh-Ritz
+constexpr int kEigMargin = 16;      // eigenvalues formed above the occupied ones (partial spectrum)
 
 // Householder reduction of the lower triangle of A[l] (n = nvec[act[l]]).  On
 // exit d, e hold the tridiagonal matrix, A[l] column k rows k+1.. holds the
@@ -27,10 +28,14 @@ hipError_t launch_tridiag(hipStream_t s, int count, const int* act, const int* n
 // Eigenvalues of the tridiagonal matrices, occupations (writes eps, occ, mu, ts,
 // kneed into the instance state) and the first min(kneed, kcap) eigenvectors of
 // T into Zt[l] (transposed, rows n..ld-1 and columns kneed..kcap-1 zero).
+// full_spectrum: every orbital energy is formed; otherwise the lowest
+// nocc + kEigMargin (more if the Fermi window needs them), the others being
+// reported as an upper bound with the exact occupation and entropy of an orbital
+// 80 kT or more above the Fermi level.
 // lu: 5 * ld * kcap doubles per active instance.
 hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const int* nvec, int ld, const double* d,
                               const double* e, double* eps_all, const int64_t* vec0, double* occ_all, SccState* st,
-                              double* const* Zt, double* lu, int kcap);
+                              double* const* Zt, double* lu, int kcap, bool full_spectrum);
 
 // Zt[l] <- (Q Z)^T with Q the product of the reflectors left by launch_tridiag.
 hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,

@@ -700,7 +700,8 @@ Status Batch::scc_phase() {
                 }
                 HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
                                            (const double*)bk.eig, (const double*)bk.eig_e, d_eps_,
-                                           (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc));
+                                           (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc,
+                                           opt_.keep_matrices != 0));
                 HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
                                              (const double*)bk.tau, Z, (const SccState*)d_state_, kc));
                 // C_k = Cb Z (into T)
