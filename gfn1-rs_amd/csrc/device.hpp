@@ -23,7 +23,10 @@ constexpr double kPadEnergy = 100.0;
 constexpr int kPadMultiple = 32;
 
 // Per-instance SCC control state (one entry per instance, device resident).
-struct SccState {
+// One instance's SCC state, padded to 256 bytes: the size buckets run on
+// concurrent streams, and the XCDs' L2 caches are only made coherent at kernel
+// boundaries, so no two instances may share a cache line.
+struct alignas(256) SccState {
     int32_t status;        // 0 running, 1 converged, 2 failed
     int32_t rung;          // robust-fallback ladder rung (electronic.rs:1413-1419)
     int32_t iter;          // iterations done on the current rung
@@ -56,6 +59,8 @@ struct DeviceBatch {
     const int32_t* sh0;      // ninst+1
     const int32_t* ao0;      // ninst+1
     const int32_t* ld;       // padded dimension per instance
+    const int32_t* nsh;      // shells per instance (sh0 starts are padded)
+    const int32_t* nao;      // AOs per instance (ao0 starts are padded)
     const int32_t* atom_inst;
     const int32_t* atom_z;
     const double* atom_x;    // 3 per atom

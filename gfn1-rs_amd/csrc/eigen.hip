@@ -920,98 +920,205 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
 }
 
 // ---------------------------------------------------------------------------
-// Z <- Q Z, Q = H_0 H_1 ... H_{n-2}, applied as blocks I - V T V^T (LAPACK
-// dlarft forward/columnwise) from the last block to the first.  Lane j owns
-// eigenvector j (Zt row-major, so each row access coalesces); the block's
-// reflectors sit in LDS as Vb[r][i].
-__global__ void __launch_bounds__(512) k_backtransform(const int* act, const int* nvec, int ld, double* const* Aptr,
-                                                       const double* tauvec, double* const* Ztptr,
-                                                       const SccState* st, int kcap) {
+// Z <- Q Z, Q = H_0 H_1 ... H_{n-2}, applied as blocks I - V T V^T of kBT = 16
+// reflectors (LAPACK dlarft forward/columnwise) from the last block to the first,
+// on the FP64 matrix cores.  One work-group per (matrix, 32 eigenvectors); the
+// 16 x 16 tiles of its Z columns stay in MFMA accumulator layout (lane l, element
+// e: row 4e + l/16, column l%16) in registers for the whole transformation, wave
+// w owning row tiles w, w + 8, ...  With that layout, k-step e of W = V^T Z takes
+// element e of a Z tile as its B operand, and the update Z -= V (T W)
+// accumulates straight into the tiles.  The block's reflectors are staged in
+// LDS; W and the Gram matrix V^T V (for T) are summed over the wavefronts in a
+// fixed order.  The chunks of one matrix are mapped onto one XCD (dispatch order
+// is round-robin over the eight XCDs), so its reflectors are shared in that L2.
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+constexpr int kBTCols = 32;                                // eigenvectors per work-group
+constexpr int kBTWaves = 8;                                // 512 threads
+constexpr int kBTTiles = kEigMaxDim / 16 / kBTWaves;      // row tiles per wavefront
+constexpr int kXcds = 8;
+constexpr int kVbS = kBT + 1;  // LDS row stride of the staged reflectors (odd: conflict-free row writes)
+
+__device__ __forceinline__ dvec4 mfma16x16x4(double a, double b, dvec4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// T factor (upper triangular, 16 x 16) of every reflector block of one matrix:
+// T[i][i] = tau_i, T[i][j] = -tau_j sum_{c=i}^{j-1} T[i][c] G[c][j] with the Gram
+// matrix G = V^T V accumulated on the matrix cores.  One wavefront per (block,
+// matrix); the factors go to work[(l * (ld / 16) + b) * 256], row-major.
+__global__ void __launch_bounds__(64) k_block_t(const int* act, const int* nvec, int ld, double* const* Aptr,
+                                                const double* tauvec, double* work) {
+    __shared__ double Gs[kBT * kBT];
+    const int b = blockIdx.x, l = blockIdx.y;
+    const int n = nvec[act[l]];
+    const int lo = b * kBT;
+    if (n < 2 || lo > n - 2) return;
+    const int bb = min(kBT, n - 1 - lo), base = (lo + 1) & ~15;
+    gptr<const double> A = (gptr<const double>)Aptr[l];
+    const int lane = threadIdx.x, li = lane & 15, lg = lane >> 4, kk = lo + li;
+    dvec4 ag = {0.0, 0.0, 0.0, 0.0};
+    for (int row0 = base; row0 < n; row0 += 16) {
+        double va[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = row0 + 4 * e + lg;
+            va[e] = (li < bb && r < n && r > kk) ? (r == kk + 1 ? 1.0 : A[r + (size_t)kk * ld]) : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ag = mfma16x16x4(va[e], va[e], ag);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) Gs[(4 * e + lg) * kBT + li] = ag[e];
+    __syncthreads();
+    if (lane < kBT) {
+        const int i = lane;
+        const double* tau = tauvec + (size_t)l * ld + lo;
+        double tj[kBT], row[kBT];
+#pragma unroll
+        for (int j = 0; j < kBT; ++j) {
+            tj[j] = j < bb ? tau[j] : 0.0;
+            row[j] = 0.0;
+        }
+        if (i < bb) row[i] = tj[i];
+#pragma unroll
+        for (int j = 1; j < kBT; ++j) {
+            if (j > i && j < bb) {
+                double acc = 0.0;
+#pragma unroll
+                for (int c = 0; c < kBT; ++c)
+                    if (c >= i && c < j) acc += row[c] * Gs[c * kBT + j];
+                row[j] = -tj[j] * acc;
+            }
+        }
+        double* out = work + ((size_t)l * (ld / 16) + b) * (kBT * kBT) + i * kBT;
+#pragma unroll
+        for (int j = 0; j < kBT; ++j) out[j] = row[j];
+    }
+}
+
+__global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act, const int* nvec, int ld,
+                                                                 double* const* Aptr, double* const* Ztptr,
+                                                                 const SccState* st, int kcap, int count, int nch,
+                                                                 const double* tfac) {
     extern __shared__ double sm[];
-    double* Vb = sm;                       // (n - lo - 1) x kBT
-    double* Tm = sm + (size_t)ld * kBT;    // kBT x kBT
-    double* G = Tm + kBT * kBT;            // kBT x kBT
-    const int l = blockIdx.x;
+    double* Vb = sm;                                  // staged reflectors, (tile-aligned rows) x kVbS
+    double* part = Vb + (size_t)ld * kVbS;            // per wavefront: T W partials (2 tiles)
+    double* Wp = part + kBTWaves * 2 * 4 * 64;        // -(T W), 16 x 32
+    double* Tl = Wp + kBT * kBTCols;                  // T of the block, 16 x 16
+    const int id = blockIdx.x, per = gridDim.x / kXcds;
+    const int logical = (id % kXcds) * per + id / kXcds;
+    const int l = logical / nch, chunk = logical % nch;
+    if (l >= count) return;
     const int inst = act[l];
     const int n = nvec[inst];
     const int k = st[inst].kneed;
-    if (k == 0) return;
+    const int c0 = chunk * kBTCols;
+    if (c0 >= k || n < 2) return;
     gptr<const double> A = (gptr<const double>)Aptr[l];
-    const double* tau = tauvec + (size_t)l * ld;
     gptr<double> Z = (gptr<double>)Ztptr[l];
-    const int j = threadIdx.x, nt = blockDim.x;
-    const bool on = j < k;
-    for (int hi = n - 1; hi > 0; hi -= kBT) {
-        const int lo = max(0, hi - kBT), bb = hi - lo, r0 = lo + 1, m = n - r0;
-        __syncthreads();
-        // reflectors of the block into LDS (row-major, i fastest); the global
-        // reads walk down each reflector column
-        for (int q = threadIdx.x; q < m * kBT; q += nt) {
-            const int i = q / m, r = r0 + q % m, kk = lo + i;
-            double v = 0.0;
-            if (i < bb) v = r < kk + 1 ? 0.0 : (r == kk + 1 ? 1.0 : A[r + (size_t)kk * ld]);
-            Vb[(r - r0) * kBT + i] = v;
-        }
-        __syncthreads();
-        for (int p = threadIdx.x; p < kBT * kBT; p += nt) {
-            const int a = p / kBT, b2 = p % kBT;
-            if (a < b2 && b2 < bb) {
-                double g = 0.0;
-                for (int r = 0; r < m; ++r) g += Vb[r * kBT + a] * Vb[r * kBT + b2];
-                G[p] = g;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, li = lane & 15, lg = lane >> 4;
+    dvec4 z[kBTTiles][2];
+#pragma unroll
+    for (int t = 0; t < kBTTiles; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int col = c0 + 16 * ct + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * (wid + kBTWaves * t) + 4 * e + lg;
+                z[t][ct][e] = (r < n && col < kcap) ? Z[(size_t)r * kcap + col] : 0.0;
             }
         }
+    const int tile_end = (n + 15) & ~15;
+    for (int lo = ((n - 2) / kBT) * kBT; lo >= 0; lo -= kBT) {
+        const int bb = min(kBT, n - 1 - lo), base = (lo + 1) & ~15;
         __syncthreads();
-        // T (upper triangular): T[i][i] = tau_i, T[a][i] = -tau_i sum_{c=a}^{i-1} T[a][c] G[c][i];
-        // row a depends only on its own earlier entries, so lane a builds row a
-        if (threadIdx.x < kBT) {
-            const int a = threadIdx.x;
-            double row[kBT];
-#pragma unroll
-            for (int i = 0; i < kBT; ++i) row[i] = 0.0;
-            if (a < bb) row[a] = tau[lo + a];
-#pragma unroll
-            for (int i = 1; i < kBT; ++i) {
-                if (i > a && i < bb) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int c = 0; c < kBT; ++c)
-                        if (c >= a && c < i) acc += row[c] * G[c * kBT + i];
-                    row[i] = -tau[lo + i] * acc;
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < kBT; ++i) Tm[a * kBT + i] = row[i];
-        }
-        __syncthreads();
-        if (on) {
-            double y[kBT];
-#pragma unroll
-            for (int i = 0; i < kBT; ++i) y[i] = 0.0;
-#pragma unroll 4
-            for (int r = 0; r < m; ++r) {
-                const double z = Z[(size_t)(r0 + r) * kcap + j];
-                const double* vrow = Vb + r * kBT;
-#pragma unroll
-                for (int i = 0; i < kBT; ++i) y[i] += vrow[i] * z;
-            }
+        // reflectors lo .. lo + bb - 1 on rows base .. tile_end - 1 (zero above each
+        // reflector's unit entry and below row n - 1): thread t stages row base + t,
+        // its 16 loads in flight together and each wavefront reading 64 consecutive
+        // rows of a column; the block's T factor alongside
+        if (tid < tile_end - base) {
+            const int r = base + tid;
+            double v[kBT];
 #pragma unroll
             for (int i = 0; i < kBT; ++i) {
-                double acc = 0.0;
-#pragma unroll
-                for (int c = i; c < kBT; ++c) acc += Tm[i * kBT + c] * y[c];
-                y[i] = acc;
+                const int kk = lo + i;
+                v[i] = (i < bb && r < n && r > kk) ? (r == kk + 1 ? 1.0 : A[r + (size_t)kk * ld]) : 0.0;
             }
-#pragma unroll 4
-            for (int r = 0; r < m; ++r) {
-                const double* vrow = Vb + r * kBT;
-                double upd = 0.0;
 #pragma unroll
-                for (int i = 0; i < kBT; ++i) upd += vrow[i] * y[i];
-                Z[(size_t)(r0 + r) * kcap + j] -= upd;
+            for (int i = 0; i < kBT; ++i) Vb[tid * kVbS + i] = v[i];
+        }
+        if (tid < kBT * kBT) Tl[tid] = tfac[((size_t)l * (ld / 16) + lo / kBT) * (kBT * kBT) + tid];
+        __syncthreads();
+        // this wavefront's share of T W = T V^T Z: W over its row tiles (accumulator
+        // layout, element e = reflectors 4e .. 4e + 3), then T W with that W as the B
+        // operand of k-step e
+        dvec4 aw0 = {0.0, 0.0, 0.0, 0.0}, aw1 = aw0;
+#pragma unroll
+        for (int t = 0; t < kBTTiles; ++t) {
+            const int row0 = 16 * (wid + kBTWaves * t);
+            if (row0 >= base && row0 < n) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double va = Vb[(row0 + 4 * e + lg - base) * kVbS + li];
+                    aw0 = mfma16x16x4(va, z[t][0][e], aw0);
+                    aw1 = mfma16x16x4(va, z[t][1][e], aw1);
+                }
+            }
+        }
+        dvec4 tw0 = {0.0, 0.0, 0.0, 0.0}, tw1 = tw0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double ta = Tl[li * kBT + 4 * e + lg];
+            tw0 = mfma16x16x4(ta, aw0[e], tw0);
+            tw1 = mfma16x16x4(ta, aw1[e], tw1);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            part[((wid * 2 + 0) * 4 + e) * 64 + lane] = tw0[e];
+            part[((wid * 2 + 1) * 4 + e) * 64 + lane] = tw1[e];
+        }
+        __syncthreads();
+        // -(T W)[i][cc]: entry (i, j) of a tile sits in element i / 4 of lane j + 16 (i % 4)
+        {
+            const int i = tid / kBTCols, cc = tid % kBTCols, ct = cc >> 4, j = cc & 15;
+            const int off = (ct * 4 + (i >> 2)) * 64 + j + 16 * (i & 3);
+            double sum = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBTWaves; ++w) sum += part[w * 2 * 4 * 64 + off];
+            Wp[i * kBTCols + cc] = -sum;
+        }
+        __syncthreads();
+        // Z -= V (T W): k-step s covers reflectors 4s .. 4s + 3
+        double wb[4][2];
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) wb[s2][ct] = Wp[(4 * s2 + lg) * kBTCols + 16 * ct + li];
+#pragma unroll
+        for (int t = 0; t < kBTTiles; ++t) {
+            const int row0 = 16 * (wid + kBTWaves * t);
+            if (row0 >= base && row0 < n) {
+#pragma unroll
+                for (int s2 = 0; s2 < 4; ++s2) {
+                    const double va = Vb[(row0 + li - base) * kVbS + 4 * s2 + lg];
+                    z[t][0] = mfma16x16x4(va, wb[s2][0], z[t][0]);
+                    z[t][1] = mfma16x16x4(va, wb[s2][1], z[t][1]);
+                }
             }
         }
     }
+#pragma unroll
+    for (int t = 0; t < kBTTiles; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int col = c0 + 16 * ct + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * (wid + kBTWaves * t) + 4 * e + lg;
+                if (r < n && col < kcap) Z[(size_t)r * kcap + col] = z[t][ct][e];
+            }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -1044,19 +1151,23 @@ hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const in
 
 hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
                                 double* const* A, const double* tau, double* const* Zt, const SccState* st,
-                                int kcap) {
-    if (ld > kEigMaxDim || kcap > ld) return hipErrorInvalidValue;
-    const int threads = std::min(512, ((kcap + 63) / 64) * 64);
-    const size_t lds = ((size_t)ld * kBT + 2 * kBT * kBT) * sizeof(double);
+                                int kcap, double* work) {
+    if (ld > kEigMaxDim || kcap > ld || ld % 16 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_block_t, dim3(ld / 16, count), dim3(64), 0, s, act, nvec, ld, A, tau, work);
+    auto lds_of = [](int l) {
+        return ((size_t)l * kVbS + kBTWaves * 2 * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
+    };
     static bool attr = false;  // LDS above 64 KiB must be requested explicitly
     if (!attr) {
         hipError_t err = hipFuncSetAttribute((const void*)k_backtransform,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)(((size_t)kEigMaxDim * kBT + 2 * kBT * kBT) * sizeof(double)));
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_of(kEigMaxDim));
         if (err != hipSuccess) return err;
         attr = true;
     }
-    hipLaunchKernelGGL(k_backtransform, dim3(count), dim3(threads), lds, s, act, nvec, ld, A, tau, Zt, st, kcap);
+    const int nch = (kcap + kBTCols - 1) / kBTCols;
+    const int blocks = (count * nch + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_backtransform, dim3(blocks), dim3(64 * kBTWaves), lds_of(ld), s, act, nvec, ld, A, Zt, st,
+                       kcap, count, nch, (const double*)work);
     return hipGetLastError();
 }
 

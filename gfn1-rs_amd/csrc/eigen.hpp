@@ -38,8 +38,10 @@ hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const in
                               double* const* Zt, double* lu, int kcap, bool full_spectrum);
 
 // Zt[l] <- (Q Z)^T with Q the product of the reflectors left by launch_tridiag.
+// work: 16 * ld doubles per active instance (the blocks' T factors); the lu
+// workspace of launch_tridiag_eig is free at this point and large enough.
 hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
                                 double* const* A, const double* tau, double* const* Zt, const SccState* st,
-                                int kcap);
+                                int kcap, double* work);
 
 }  // namespace gfn1

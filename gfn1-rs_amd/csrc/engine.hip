@@ -49,6 +49,11 @@ enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, A_H0, A_L, A_
 constexpr int kWarmResetPeriod = 64;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
+// Each instance's ranges in the per-shell and per-AO arrays start on a 256-byte
+// boundary: the size buckets update their instances on concurrent streams, and
+// the XCDs' L2 caches are only made coherent at kernel boundaries, so no cache
+// line may hold data of two instances.
+constexpr int kRangeAlign = 32;
 }  // namespace
 
 // ------------------------------------------------------------------ DeviceModel
@@ -128,6 +133,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     h_atom0_.assign(ninst_ + 1, 0);
     h_sh0_.assign(ninst_ + 1, 0);
     h_ao0_.assign(ninst_ + 1, 0);
+    h_nsh_.assign(ninst_, 0);
+    h_nao_.assign(ninst_, 0);
     h_ld_.assign(ninst_, 0);
     h_vec0_.assign(ninst_ + 1, 0);
     h_err_.assign(ninst_, 0);
@@ -170,8 +177,10 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
             }
         }
         h_atom0_[i + 1] = h_atom0_[i] + s.nat;
-        h_sh0_[i + 1] = h_sh0_[i] + nsh;
-        h_ao0_[i + 1] = h_ao0_[i] + nao;
+        h_nsh_[i] = nsh;
+        h_nao_[i] = nao;
+        h_sh0_[i + 1] = round_up(h_sh0_[i] + nsh, kRangeAlign);
+        h_ao0_[i + 1] = round_up(h_ao0_[i] + nao, kRangeAlign);
         h_ld_[i] = std::max(pad_multiple_, round_up(nao, pad_multiple_));
         h_vec0_[i + 1] = h_vec0_[i] + h_ld_[i];
         const double nelec = nref - s.charge;
@@ -186,6 +195,14 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         // initial shell charges (electronic.rs:1235-1239): used when the length matches
         for (int k = 0; k < nsh; ++k)
             h_q0_.push_back((s.q0 && s.q0_len == nsh) ? s.q0[k] : 0.0);
+        // filler entries up to the next aligned start (never part of an instance)
+        for (int k = h_sh0_[i] + nsh; k < h_sh0_[i + 1]; ++k) {
+            h_sh_atom_.push_back(s.nat > 0 ? h_atom0_[i] : 0);
+            h_sh_type_.push_back(nsh > 0 ? h_sh_type_.back() : 1 * kMaxShellsPerAtom);
+            h_sh_ao_.push_back(0);
+            h_q0_.push_back(0.0);
+        }
+        for (int k = h_ao0_[i] + nao; k < h_ao0_[i + 1]; ++k) h_ao_sh_.push_back(h_sh0_[i]);
     }
     h_err_input_ = h_err_;
     h_errmsg_input_ = h_errmsg_;
@@ -198,7 +215,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         int pos = h_sh0_[i];
         for (int l = 0; l < 3; ++l) {
             h_lofs_[4 * i + l] = pos;
-            for (int sh = h_sh0_[i]; sh < h_sh0_[i + 1]; ++sh)
+            for (int sh = h_sh0_[i]; sh < h_sh0_[i] + h_nsh_[i]; ++sh)
                 if (m.shell_table()[h_sh_type_[sh]].l == l) h_sh_by_l_[pos++] = sh;
         }
         h_lofs_[4 * i + 3] = pos;
@@ -230,7 +247,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     prm_.broyden_slots = std::min(broyden_memory_, max_broyden_iter);
     h_broff_.assign(ninst_ + 1, 0);
     for (int i = 0; i < ninst_; ++i)
-        h_broff_[i + 1] = h_broff_[i] + (int64_t)prm_.broyden_slots * (h_sh0_[i + 1] - h_sh0_[i]);
+        h_broff_[i + 1] = h_broff_[i] + round_up(prm_.broyden_slots * h_nsh_[i], kRangeAlign);
+    prm_.omega_stride = round_up(prm_.broyden_slots, kRangeAlign);
 
     HIP_TRY(hipStreamCreate(&stream_));
     // ---- device index tables
@@ -254,11 +272,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = up_i(&d_ao_sh_, h_ao_sh_)).ok()) return st;
     if (!(st = up_i(&d_sh_by_l_, h_sh_by_l_)).ok()) return st;
     if (!(st = up_i(&d_lofs_, h_lofs_)).ok()) return st;
-    {
-        std::vector<int> nao(ninst_);
-        for (int i = 0; i < ninst_; ++i) nao[i] = h_ao0_[i + 1] - h_ao0_[i];
-        if (!(st = up_i(&d_nao_, nao)).ok()) return st;
-    }
+    if (!(st = up_i(&d_nao_, h_nao_)).ok()) return st;
+    if (!(st = up_i(&d_nsh_, h_nsh_)).ok()) return st;
     if (!(st = dalloc(&d_fail_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_vec0_, ninst_ + 1)).ok()) return st;
     HIP_TRY(hipMemcpy(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -283,7 +298,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = dalloc(&d_occ_, h_vec0_[ninst_])).ok()) return st;
     if (!(st = dalloc(&d_brdf_, h_broff_[ninst_])).ok()) return st;
     if (!(st = dalloc(&d_bru_, h_broff_[ninst_])).ok()) return st;
-    if (!(st = dalloc(&d_brom_, (size_t)ninst_ * prm_.broyden_slots)).ok()) return st;
+    if (!(st = dalloc(&d_brom_, (size_t)ninst_ * prm_.omega_stride)).ok()) return st;
     if (!(st = dalloc(&d_state_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_mat_, (size_t)kRoles * ninst_)).ok()) return st;
 
@@ -313,6 +328,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         HIP_TRY(hipEventCreate(&bk.ev1));
         RB_TRY(rocblas_create_handle(&bk.handle));
         RB_TRY(rocblas_set_stream(bk.handle, bk.stream));
+        // reproducible GEMMs: no split-K kernels that accumulate with atomics
+        RB_TRY(rocblas_set_atomics_mode(bk.handle, rocblas_atomics_not_allowed));
         HIP_TRY(hipMalloc(&bk.ptrs, sizeof(double*) * kActRoles * count));
         HIP_TRY(hipMalloc(&bk.act, sizeof(int) * count));
         HIP_TRY(hipMalloc(&bk.eig, sizeof(double) * bk.ld * count));
@@ -335,6 +352,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     db_.sh0 = d_sh0_;
     db_.ao0 = d_ao0_;
     db_.ld = d_ld_;
+    db_.nsh = d_nsh_;
+    db_.nao = d_nao_;
     db_.atom_inst = d_atom_inst_;
     db_.atom_z = d_atom_z_;
     db_.atom_x = d_x_;
@@ -512,8 +531,7 @@ Status Batch::setup_phase() {
         if (!gs.ok()) return gs;
         HIP_TRY(guess_launch_features(stream_, na, d_atom_z_, d_x_, d_atom_inst_, d_atom0_, d_gfeat_));
         HIP_TRY(guess_launch_forward(stream_, na, th, nm, d_gfeat_, d_gout_));
-        HIP_TRY(guess_launch_apply(stream_, ninst_, d_guess_usable_, d_atom0_, d_atom_sh0_, d_sh0_, d_charge_, d_gout_,
-                                   d_q0_));
+        HIP_TRY(guess_launch_apply(stream_, ninst_, d_guess_usable_, d_atom0_, d_atom_sh0_, d_charge_, d_gout_, d_q0_));
         q0_dirty_ = true;
     }
     HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
@@ -620,7 +638,7 @@ Status Batch::scc_phase() {
             Status st = upload_active(bk, bk.active, 0);
             if (!st.ok()) return st;
             int max_nsh = 0;
-            for (int i : bk.active) max_nsh = std::max(max_nsh, h_sh0_[i + 1] - h_sh0_[i]);
+            for (int i : bk.active) max_nsh = std::max(max_nsh, h_nsh_[i]);
             const int n = bk.ld;
             // ---- Fock matrix from the input charges
             hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
@@ -695,7 +713,7 @@ Status Batch::scc_phase() {
                 bk.ev_pending = true;
                 bk.ev_bytes = 0.0;
                 for (int i : bk.active) {
-                    const double na = h_ao0_[i + 1] - h_ao0_[i];
+                    const double na = h_nao_[i];
                     bk.ev_bytes += 8.0 * (na - 1.0) * na * (na + 1.0) / 6.0;
                 }
                 HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
@@ -703,7 +721,7 @@ Status Batch::scc_phase() {
                                            (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc,
                                            opt_.keep_matrices != 0));
                 HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
-                                             (const double*)bk.tau, Z, (const SccState*)d_state_, kc));
+                                             (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
                 // C_k = Cb Z (into T)
                 RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, kc,
                                              n, &one, (const double* const*)Cb, n, (const double* const*)Z, kc,
@@ -907,8 +925,8 @@ Status Batch::download(std::vector<InstanceResult>& out) {
         InstanceResult& r = out[i];
         const SccState& s = h_state_[i];
         const int a0 = h_atom0_[i], nat = h_atom0_[i + 1] - a0;
-        const int s0 = h_sh0_[i], nsh = h_sh0_[i + 1] - s0;
-        const int n = h_ao0_[i + 1] - h_ao0_[i], ld = h_ld_[i];
+        const int s0 = h_sh0_[i], nsh = h_nsh_[i];
+        const int n = h_nao_[i], ld = h_ld_[i];
         r.nat = nat;
         r.nsh = nsh;
         r.nao = n;

@@ -143,7 +143,7 @@ class Batch {
     const DeviceModel* dm_ = nullptr;
     EngineOptions opt_;
     int ninst_ = 0, natoms_ = 0, nshells_ = 0, naos_ = 0;
-    std::vector<int> h_atom0_, h_sh0_, h_ao0_, h_ld_, h_atom_z_, h_sh_atom_, h_sh_type_, h_sh_ao_, h_ao_sh_,
+    std::vector<int> h_atom0_, h_sh0_, h_ao0_, h_nsh_, h_nao_, h_ld_, h_atom_z_, h_sh_atom_, h_sh_type_, h_sh_ao_, h_ao_sh_,
         h_atom_sh0_, h_atom_inst_, h_sh_by_l_, h_lofs_;
     std::vector<double> h_x_, h_q0_, h_nelec_;
     std::vector<int64_t> h_vec0_, h_broff_;
@@ -170,7 +170,7 @@ class Batch {
     int *d_atom0_ = nullptr, *d_sh0_ = nullptr, *d_ao0_ = nullptr, *d_ld_ = nullptr, *d_atom_inst_ = nullptr,
         *d_atom_z_ = nullptr, *d_atom_sh0_ = nullptr, *d_sh_atom_ = nullptr, *d_sh_type_ = nullptr,
         *d_sh_ao_ = nullptr, *d_ao_sh_ = nullptr, *d_fail_ = nullptr, *d_sh_by_l_ = nullptr, *d_lofs_ = nullptr,
-        *d_nao_ = nullptr;
+        *d_nao_ = nullptr, *d_nsh_ = nullptr;
     int64_t *d_vec0_ = nullptr, *d_broff_ = nullptr;
     double *d_x_ = nullptr, *d_cn_ = nullptr, *d_cnd3_ = nullptr, *d_gw_ = nullptr;
     double *d_qin_ = nullptr, *d_qout_ = nullptr, *d_q0_ = nullptr, *d_vin_g_ = nullptr, *d_vin_t_ = nullptr,

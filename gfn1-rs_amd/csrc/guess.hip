@@ -244,14 +244,14 @@ __global__ void k_guess_adam(float* __restrict__ theta, float* __restrict__ m, f
 // pred[2a+1]; the instance is shifted uniformly so that its shell charges sum
 // to its charge.  Instances the model does not cover keep q.
 __global__ void k_guess_apply(int ninst, const int* __restrict__ usable, const int* __restrict__ atom0,
-                              const int* __restrict__ atom_sh0, const int* __restrict__ sh0,
-                              const double* __restrict__ charge, const float* __restrict__ pred, double* q) {
+                              const int* __restrict__ atom_sh0, const double* __restrict__ charge,
+                              const float* __restrict__ pred, double* q) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ninst || !usable[i]) return;
     const int a0 = atom0[i], a1 = atom0[i + 1];
     double sum = 0.0;
     for (int a = a0; a < a1; ++a) sum += (double)pred[2 * a] + (double)pred[2 * a + 1];
-    const int nsh = sh0[i + 1] - sh0[i];
+    const int nsh = 2 * (a1 - a0);  // the model covers instances whose atoms all have two shells
     const double shift = nsh > 0 ? (sum - charge[i]) / nsh : 0.0;
     for (int a = a0; a < a1; ++a) {
         q[atom_sh0[a]] = (double)pred[2 * a] - shift;
@@ -353,9 +353,9 @@ hipError_t guess_launch_forward(hipStream_t s, int natoms, const float* theta, c
 }
 
 hipError_t guess_launch_apply(hipStream_t s, int ninst, const int* usable, const int* atom0, const int* atom_sh0,
-                              const int* sh0, const double* charge, const float* pred, double* q) {
+                              const double* charge, const float* pred, double* q) {
     if (ninst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guess_apply, dim3((ninst + 127) / 128), dim3(128), 0, s, ninst, usable, atom0, atom_sh0, sh0,
+    hipLaunchKernelGGL(k_guess_apply, dim3((ninst + 127) / 128), dim3(128), 0, s, ninst, usable, atom0, atom_sh0,
                        charge, pred, q);
     return hipGetLastError();
 }

@@ -59,7 +59,7 @@ hipError_t guess_launch_features(hipStream_t s, int natoms, const int* z, const 
 hipError_t guess_launch_forward(hipStream_t s, int natoms, const float* theta, const float* norm, const float* feat,
                                 float* out);
 hipError_t guess_launch_apply(hipStream_t s, int ninst, const int* usable, const int* atom0, const int* atom_sh0,
-                              const int* sh0, const double* charge, const float* pred, double* q);
+                              const double* charge, const float* pred, double* q);
 
 class GuessModel {
    public:

@@ -274,7 +274,7 @@ __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int
 // Diagonal of the padded rows/columns: S = 1, H0 = F = kPadEnergy.
 __global__ void k_pad_diagonal(DeviceBatch b) {
     const int inst = blockIdx.x;
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     for (int i = n + threadIdx.x; i < ld; i += blockDim.x) {
         const size_t idx = (size_t)i * ld + i;
         ((gptr<double>)b.S[inst])[idx] = 1.0;
@@ -288,7 +288,7 @@ __global__ void k_pad_diagonal(DeviceBatch b) {
 __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, double eig_tol, int* fail) {
     const int inst = act[blockIdx.y];
     const int ld = b.ld[inst];
-    const int n = b.ao0[inst + 1] - b.ao0[inst];
+    const int n = b.nao[inst];
     gptr<double> cb = (gptr<double>)b.CB0[inst];
     gptr<const double> li = (gptr<const double>)Linv[blockIdx.y];
     const size_t total = (size_t)ld * ld;
@@ -310,7 +310,7 @@ __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, do
 __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
                                   double* vt) {
     const int inst = act[blockIdx.y];
-    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
+    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsh) return;
     const int si = s0 + i;
@@ -341,7 +341,7 @@ __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act,
 // F = H0 - 1/2 S (v_mu + v_nu) (electronic.rs:3133-3158); one block per column.
 __global__ void k_fock(DeviceBatch b, const int* act, const double* vt) {
     const int inst = act[blockIdx.y];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= n) return;
     const int a0 = b.ao0[inst];
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(256) k_reduce_fock(DeviceBatch b, const int* a
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (tj > ti) return;
     const int inst = act[blockIdx.z];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int I0 = ti * 64, J0 = tj * 64;
     if (I0 >= ld) return;
     const int a0 = b.ao0[inst];
@@ -424,7 +424,7 @@ __global__ void __launch_bounds__(256) k_reduce_fock(DeviceBatch b, const int* a
 // F += b (S - 1/2 S P S) for instances on a level-shift rung (electronic.rs:2120-2133).
 __global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift, double* const* SPS) {
     const int inst = act[blockIdx.y];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= n) return;
     const double ls = shift[blockIdx.y];
@@ -442,7 +442,7 @@ __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int ei
                         double* occ_all, SccState* st) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.x];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const double* e = eig + (size_t)blockIdx.x * eig_stride;
     double* eps = eps_all + b.vec0[inst];
     double* occ = occ_all + b.vec0[inst];
@@ -544,7 +544,7 @@ __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int ei
 __global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
                                 const double* eps_all, const double* occ_all, int k, int weighted) {
     const int inst = act[blockIdx.y];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= k) return;
     const double f = occ_all[b.vec0[inst] + col];
@@ -558,7 +558,7 @@ __global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C,
 __global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp) {
     __shared__ double t1[32][33], t2[32][33];
     const int inst = act[blockIdx.z];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int bi = blockIdx.x, bj = blockIdx.y;
     if (bj > bi) return;
     if (bi * 32 >= n) return;
@@ -587,7 +587,7 @@ __global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp
                                 double* const* C32p, const double* eps_all, const double* occ_all, int k,
                                 int weighted) {
     const int inst = act[blockIdx.y];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= k) return;
     const double f = occ_all[b.vec0[inst] + col];
@@ -606,7 +606,7 @@ __global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp
 __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* Xp, double* const* Mp) {
     __shared__ float t1[32][33], t2[32][33];
     const int inst = act[blockIdx.z];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int bi = blockIdx.x, bj = blockIdx.y;
     if (bj > bi) return;
     if (bi * 32 >= n) return;
@@ -633,7 +633,7 @@ __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* 
 __global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.y];
-    const int n = b.ao0[inst + 1] - b.ao0[inst], ld = b.ld[inst];
+    const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= n) return;
     gptr<const double> S = (gptr<const double>)b.S[inst] + (size_t)col * ld;
@@ -657,8 +657,8 @@ __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const doub
                        double* qout, SccState* st) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.x];
-    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
-    const int a0 = b.ao0[inst], n = b.ao0[inst + 1] - a0;
+    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
+    const int a0 = b.ao0[inst], n = b.nao[inst];
     for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
         const int si = s0 + i;
         const ShellType& T = t.shells[b.sh_type[si]];
@@ -726,7 +726,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     const int inst = act[blockIdx.x];
     SccState& s = st[inst];
     if (s.status != 0) return;
-    const int s0 = b.sh0[inst], nsh = b.sh0[inst + 1] - s0;
+    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int atom0 = b.atom0[inst], nat = b.atom0[inst + 1] - atom0;
     // E2 = 1/2 q.gamma q, E3 = sum_A Gamma_A q_A^3 / 3
     double e2 = 0.0, ss = 0.0;
@@ -776,7 +776,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
         const int64_t off = br_off[inst];
         double* df = br_df + off;
         double* u = br_u + off;
-        double* omega = br_omega + (size_t)inst * prm.broyden_slots;
+        double* omega = br_omega + (size_t)inst * prm.omega_stride;
         double* qlast = br_qlast + s0;
         double* dqlast = br_dqlast + s0;
         const int iter = s.broyden_iter + 1;
@@ -1117,7 +1117,7 @@ __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int
         dedcn_h[a] = d;
     }
     if (!scc_on) return;
-    const int s0 = b.sh0[inst], s1 = b.sh0[inst + 1];
+    const int s0 = b.sh0[inst], s1 = s0 + b.nsh[inst];
     const double* x = b.atom_x;
     double acc[3] = {0, 0, 0};
     for (int k = 0; k < nsa; ++k) {

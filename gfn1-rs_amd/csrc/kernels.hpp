@@ -14,6 +14,7 @@ struct SccParams {
     double charge_tolerance;
     int32_t broyden_memory;   // BroydenMixer memory (history modulus)
     int32_t broyden_slots;    // allocated history slots per instance
+    int32_t omega_stride;     // per-instance stride of the Broyden weights (broyden_slots padded)
     int32_t rung_broyden[4];
     double rung_mixing[4];
     double rung_level_shift[4];
