@@ -96,6 +96,57 @@ def parity(eng, systems, count):
     return {"instances": count, "max_abs_dE": de, "max_abs_dg": dg}
 
 
+SINGLE_WORKLOADS = {
+    # name: (molecules, cube side in Angstrom, seed)
+    "water1000": (1000, 31.0, 1),
+    "water2000": (2000, 39.1, 4),
+}
+
+
+def run_single(name, precisions):
+    """One large water cluster (energy + gradient, 300 K, SCC tolerance 1e-7 Eh)
+    through the same engine: wall time per energy+gradient, iterations and the
+    size-independent checks (translation invariance of the gradient, total
+    charge); the mixed path is compared with the FP64 path on the same system.
+    Prints one JSON line (not the batched headline metric)."""
+    from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, basis_counts, geometry
+    nmol, side, seed = SINGLE_WORKLOADS[name]
+    t0 = time.time()
+    z, x = geometry.water_cluster(nmol, side, seed)
+    nsh = basis_counts(z)[0]
+    nao = geometry.basis_size(z)[1]
+    print("%s: %d atoms, %d AO, generated in %.1fs" % (name, len(z), nao, time.time() - t0), file=sys.stderr,
+          flush=True)
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-7))
+    # kernels and libraries loaded outside the timed run
+    BatchEngine([geometry.water_cluster(48, 11.3, seed=0)], options=opts).run()
+    os.environ["GFN1_PROGRESS"] = "1"
+    out = {"workload": "%s: %d rigid waters in a %.1f A cube (seed %d), %d atoms, %d AO, energy + gradient, "
+                       "300 K, SCC energy tolerance 1e-7" % (name, nmol, side, seed, len(z), nao),
+           "nao": nao, "natoms": len(z)}
+    ref = None
+    for prec in precisions:
+        eng = BatchEngine([(z, x)], options=opts, precision=prec)
+        t = time.perf_counter()
+        eng.run()
+        dt = time.perf_counter() - t
+        g = eng.gradient(0)
+        q = eng.shell_charges(0, nsh)
+        it = eng.iterations(0)
+        r = {"seconds": dt, "status": eng.status(0), "iterations": it, "energy": eng.energy(0),
+             "phase_ms": eng.timings(), "sum_gradient_max": float(np.abs(g.sum(0)).max()),
+             "charge_sum_error": float(abs(q.sum())),
+             "dense_fp64_tflops": dense_flops(nao, it) / dt / 1e12}
+        if ref is None:
+            ref = (r["energy"], g)
+        else:
+            r["max_abs_dE_vs_fp64"] = abs(r["energy"] - ref[0])
+            r["max_abs_dg_vs_fp64"] = float(np.abs(g - ref[1]).max())
+        out[prec] = r
+        print("%s %s: %.2f s, %d iterations" % (name, prec, dt, it), file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,9 +161,19 @@ def main():
     ap.add_argument("--model", default=os.path.join(ROOT, "gfn1-rs_amd", "data", "guess_mlp.bin"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (HBM counter calibration)")
+    ap.add_argument("--workload", choices=["batch"] + sorted(SINGLE_WORKLOADS), default="batch",
+                    help="batch: the batched headline metric; waterN: one large cluster (separate report)")
+    ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
+                    help="precisions run for a single-system workload (fp64 first)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload != "batch":
+        if args.gpus != 1:
+            print("bench.py measures one GPU (--gpus %d refused)" % args.gpus, file=sys.stderr)
+            sys.exit(2)
+        run_single(args.workload, ["fp64", "mixed"] if args.precision == "both" else [args.precision])
         return
     if args.gpus != 1:
         print("bench.py measures one GPU; every run here has exactly one device (--gpus %d refused)" % args.gpus,

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -628,6 +629,10 @@ Status Batch::scc_phase() {
             if (h_state_[i].status == 0) bk.active.push_back(i);
     }
     const double one = 1.0, zero = 0.0;
+    // GFN1_PROGRESS=1: one line per SCC iteration on stderr (long single-system runs)
+    const bool progress = std::getenv("GFN1_PROGRESS") != nullptr;
+    const auto t_scc = std::chrono::steady_clock::now();
+    int sweep = 0;
     while (true) {
         bool any = false;
         for (auto& bk : buckets_) any = any || !bk.active.empty();
@@ -815,6 +820,14 @@ Status Batch::scc_phase() {
                 if (h_state_[i].status == 0) next.push_back(i);
             }
             bk.active.swap(next);
+        }
+        if (progress) {
+            size_t left = 0;
+            for (auto& bk : buckets_) left += bk.active.size();
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_scc).count();
+            std::fprintf(stderr, "scc iteration %d: %zu instances still iterating, %.1f ms\n", ++sweep, left, ms);
+            std::fflush(stderr);
         }
     }
     for (int i = 0; i < ninst_; ++i)
