@@ -1002,8 +1002,8 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
                                                                  const double* tfac) {
     extern __shared__ double sm[];
     double* Vb = sm;                                  // staged reflectors, (tile-aligned rows) x kVbS
-    double* part = Vb + (size_t)ld * kVbS;            // per wavefront: T W partials (2 tiles)
-    double* Wp = part + kBTWaves * 2 * 4 * 64;        // -(T W), 16 x 32
+    double* part = Vb + (size_t)ld * kVbS;            // T W partials (2 tiles) of half the wavefronts
+    double* Wp = part + (kBTWaves / 2) * 2 * 4 * 64;  // -(T W), 16 x 32
     double* Tl = Wp + kBT * kBTCols;                  // T of the block, 16 x 16
     const int id = blockIdx.x, per = gridDim.x / kXcds;
     const int logical = (id % kXcds) * per + id / kXcds;
@@ -1073,10 +1073,23 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
             tw0 = mfma16x16x4(ta, aw0[e], tw0);
             tw1 = mfma16x16x4(ta, aw1[e], tw1);
         }
+        // partials summed in two steps through half-size LDS: the upper wavefronts
+        // deposit theirs, the lower ones add their own in place
+        const int slot = wid % (kBTWaves / 2);
+        if (wid >= kBTWaves / 2) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            part[((wid * 2 + 0) * 4 + e) * 64 + lane] = tw0[e];
-            part[((wid * 2 + 1) * 4 + e) * 64 + lane] = tw1[e];
+            for (int e = 0; e < 4; ++e) {
+                part[((slot * 2 + 0) * 4 + e) * 64 + lane] = tw0[e];
+                part[((slot * 2 + 1) * 4 + e) * 64 + lane] = tw1[e];
+            }
+        }
+        __syncthreads();
+        if (wid < kBTWaves / 2) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                part[((slot * 2 + 0) * 4 + e) * 64 + lane] += tw0[e];
+                part[((slot * 2 + 1) * 4 + e) * 64 + lane] += tw1[e];
+            }
         }
         __syncthreads();
         // -(T W)[i][cc]: entry (i, j) of a tile sits in element i / 4 of lane j + 16 (i % 4)
@@ -1085,7 +1098,7 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
             const int off = (ct * 4 + (i >> 2)) * 64 + j + 16 * (i & 3);
             double sum = 0.0;
 #pragma unroll
-            for (int w = 0; w < kBTWaves; ++w) sum += part[w * 2 * 4 * 64 + off];
+            for (int w = 0; w < kBTWaves / 2; ++w) sum += part[w * 2 * 4 * 64 + off];
             Wp[i * kBTCols + cc] = -sum;
         }
         __syncthreads();
@@ -1155,7 +1168,7 @@ hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const 
     if (ld > kEigMaxDim || kcap > ld || ld % 16 != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_block_t, dim3(ld / 16, count), dim3(64), 0, s, act, nvec, ld, A, tau, work);
     auto lds_of = [](int l) {
-        return ((size_t)l * kVbS + kBTWaves * 2 * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
+        return ((size_t)l * kVbS + (kBTWaves / 2) * 2 * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
     };
     static bool attr = false;  // LDS above 64 KiB must be requested explicitly
     if (!attr) {

@@ -637,7 +637,9 @@ Status Batch::scc_phase() {
         bool any = false;
         for (auto& bk : buckets_) any = any || !bk.active.empty();
         if (!any) break;
-        for (auto& bk : buckets_) {
+        // largest matrices first: their chains are the longest, the small buckets fill in
+        for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
+            Bucket& bk = *bit;
             const int cnt = (int)bk.active.size();
             if (!cnt) continue;
             Status st = upload_active(bk, bk.active, 0);

@@ -1,11 +1,12 @@
 """Train the learned SCC start-charge model on the GPU and evaluate it.
 
-    python -m gfn1_amd.train_guess [--train 20000] [--eval 4096] [--epochs 40] [--out PATH]
+    python -m gfn1_amd.train_guess [--train 20000] [--eval 4096] [--schedule 40:1e-3] [--out PATH]
 
 Data: noisy snapshots of the mixed H2O/CH4/NH3 clusters (geometry.noisy_snapshot,
 seeds 10^6 + j, N(0, 0.05 A) displacements), labelled with the converged shell
 charges of the plain FP64 SCC (batch engine).  Training: the library's own HIP
-kernels, FP32 Adam, 1024 atoms per step.  Evaluation: fresh snapshots (indices
+kernels, FP32 Adam, 1024 atoms per step, in stages of (epochs, learning rate)
+(--schedule "100:1e-3,100:3e-4"; each stage restarts the Adam moments).  Evaluation: fresh snapshots (indices
 after the training set) run with the standard and with the learned start at
 the benchmark's SCC tolerance; reports iterations, wall time and the largest
 energy difference.  Prints one JSON line.
@@ -59,9 +60,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=20000)
     ap.add_argument("--eval", type=int, default=4096)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--schedule", default="40:1e-3", help="comma-separated epochs:learning_rate stages")
     ap.add_argument("--batch-atoms", type=int, default=1024)
-    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=DEFAULT_OUT)
     args = ap.parse_args()
@@ -73,11 +73,16 @@ def main():
     train, q = label(train, label_tol)
     log("labelled %d snapshots with the plain SCC in %.1fs" % (len(train), time.time() - t0))
     model = GuessModel(seed=args.seed)
+    stages = [(int(a), float(b)) for a, b in (x.split(":") for x in args.schedule.split(","))]
     t0 = time.time()
-    loss = model.train(train, q, epochs=args.epochs, batch_atoms=args.batch_atoms, learning_rate=args.lr,
-                       seed=args.seed)
+    loss = []
+    for k, (ep, lr) in enumerate(stages):
+        loss += list(model.train(train, q, epochs=ep, batch_atoms=args.batch_atoms, learning_rate=lr,
+                                 seed=args.seed + k))
+        log("stage %d: %d epochs at %.1e, loss %.3e" % (k, ep, lr, loss[-1]))
     t_train = time.time() - t0
-    log("trained %d epochs in %.1fs, loss %.3e -> %.3e" % (args.epochs, t_train, loss[0], loss[-1]))
+    epochs = sum(ep for ep, _ in stages)
+    log("trained %d epochs in %.1fs, loss %.3e -> %.3e" % (epochs, t_train, loss[0], loss[-1]))
     model.save(args.out)
     t0 = time.time()
     fresh = geometry.noisy_snapshots(args.eval, first=args.train)
@@ -85,8 +90,8 @@ def main():
     ev, de = evaluate(fresh, model, dict(energy_tolerance=1e-7))
     ev_tight, de_tight = evaluate(fresh[: min(512, len(fresh))], model, label_tol)
     print(json.dumps({
-        "train_snapshots": len(train), "train_atoms": int(sum(len(z) for z, _ in train)), "epochs": args.epochs,
-        "batch_atoms": args.batch_atoms, "learning_rate": args.lr, "train_seconds": t_train,
+        "train_snapshots": len(train), "train_atoms": int(sum(len(z) for z, _ in train)), "epochs": epochs,
+        "batch_atoms": args.batch_atoms, "schedule": args.schedule, "train_seconds": t_train,
         "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
         "eval_snapshots": len(fresh), "eval_bench_tolerance": ev, "max_abs_dE_bench_tolerance": de,
         "eval_tight_tolerance": ev_tight, "max_abs_dE_tight_tolerance": de_tight, "model": os.path.abspath(args.out),
