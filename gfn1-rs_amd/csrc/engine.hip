@@ -3,11 +3,10 @@
 // Flow of Batch::run (reference: run_electronic electronic.rs:800-2031 and
 // analytic_gradient_from_result gradient.rs:122-242):
 //   setup     CN, S/H0, Cholesky-based orthonormal start basis, geometry terms
-//   SCC       per iteration and per size bucket:
-//             v(q_in) -> F -> A = Cb^T F Cb -> A = V e V^T -> C = Cb V
-//             -> Fermi -> P -> Mulliken q_out -> E, convergence, mixing
-//             The orbitals of one iteration are the start basis of the next,
-//             so A is nearly diagonal after the first iterations.
+//   SCC       per iteration and per size bucket (largest first):
+//             v(q_in) -> A = Cb^T F Cb (reduced congruence, F not formed)
+//             -> A = Q T Q^T -> eigenvalues, Fermi, occupied eigenvectors Z
+//             -> C = Cb Q Z -> P -> Mulliken q_out -> E, convergence, mixing
 //   gradient  W, pair/atom gradient kernels, CN chains.
 #include "engine.hpp"
 
