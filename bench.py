@@ -27,6 +27,25 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import numpy as np  # noqa: E402
 
+# The batch engine runs one HIP stream per matrix-size bucket (14 for this
+# workload); the HIP runtime maps streams onto GPU_MAX_HW_QUEUES hardware queues
+# and streams that share a queue run one after another.  The run uses 16 queues
+# (override with --hw-queues); the variable must be set before the HIP runtime
+# initialises, i.e. before the library or torch touches the GPU.
+HW_QUEUES_DEFAULT = 16
+
+
+def _hw_queues_from_argv():
+    for i, a in enumerate(sys.argv):
+        if a == "--hw-queues" and i + 1 < len(sys.argv):
+            return sys.argv[i + 1]
+        if a.startswith("--hw-queues="):
+            return a.split("=", 1)[1]
+    return str(HW_QUEUES_DEFAULT)
+
+
+os.environ["GPU_MAX_HW_QUEUES"] = _hw_queues_from_argv()
+
 METRIC = "SCC energy+gradient molecules/sec (batched, ML guess+mixed vs plain FP64)"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
@@ -165,6 +184,8 @@ def main():
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precisions run for a single-system workload (fp64 first)")
+    ap.add_argument("--hw-queues", type=int, default=HW_QUEUES_DEFAULT, choices=range(1, 33), metavar="1..32",
+                    help="GPU_MAX_HW_QUEUES for this run (set before the HIP runtime starts)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -240,7 +261,7 @@ def main():
                    "global_batch": len(systems), "ao_min": int(nao.min()), "ao_max": int(nao.max()),
                    "ao_mean": float(nao.mean()),
                    "path": "learned start + mixed precision" if head == "learned" else "plain FP64",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"])},
         "paths": paths,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -14,6 +15,15 @@
 #include "model.hpp"
 
 using namespace gfn1;
+
+namespace {
+// The batch engine runs one HIP stream per size bucket (up to 14), and the HIP
+// runtime maps streams onto GPU_MAX_HW_QUEUES hardware queues (default 4);
+// streams that share a queue serialise.  Ask for 16 queues unless the caller
+// set the variable.  The runtime reads it when it initialises, so this only
+// takes effect if nothing in the process has used HIP before the library loads.
+__attribute__((constructor)) void request_hw_queues() { setenv("GPU_MAX_HW_QUEUES", "16", 0); }
+}  // namespace
 
 struct gfn1_params {
     Model model;

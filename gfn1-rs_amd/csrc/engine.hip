@@ -3,11 +3,14 @@
 // Flow of Batch::run (reference: run_electronic electronic.rs:800-2031 and
 // analytic_gradient_from_result gradient.rs:122-242):
 //   setup     CN, S/H0, Cholesky-based orthonormal start basis, geometry terms
-//   SCC       per iteration and per size bucket (largest first):
+//   SCC       every size bucket iterates on its own stream at its own pace
+//             (launched largest first, no barrier between buckets):
 //             v(q_in) -> A = Cb^T F Cb (reduced congruence, F not formed)
 //             -> A = Q T Q^T -> eigenvalues, Fermi, occupied eigenvectors Z
-//             -> C = Cb Q Z -> P -> Mulliken q_out -> E, convergence, mixing
-//   gradient  W, pair/atom gradient kernels, CN chains.
+//             -> C = Cb Q Z -> P -> Mulliken q_out -> E, convergence, mixing;
+//             a bucket whose instances are all done goes on to W and its pair
+//             gradient while the others still iterate
+//   gradient  per-atom scatter of the bucket sums, atom-wise terms, CN chains.
 #include "engine.hpp"
 
 #include "eigen.hpp"
@@ -20,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace gfn1 {
 
@@ -46,7 +50,6 @@ enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, R_L, R_A0, kRo
 constexpr double kWideSmearing = 1000.0;
 
 enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, A_H0, A_L, A_A0, kActRoles };
-constexpr int kWarmResetPeriod = 64;
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // Each instance's ranges in the per-shell and per-AO arrays start on a 256-byte
@@ -119,6 +122,14 @@ Batch::~Batch() {
         (void)hipFree(bk.beta);
         (void)hipFree(bk.tau);
         (void)hipFree(bk.lu);
+        (void)hipFree(bk.st_dev);
+        if (bk.h_ptrs) (void)hipHostFree(bk.h_ptrs);
+        if (bk.h_act) (void)hipHostFree(bk.h_act);
+        if (bk.st_host) (void)hipHostFree(bk.st_host);
+        if (bk.done) (void)hipEventDestroy(bk.done);
+        (void)hipFree(bk.atoms);
+        (void)hipFree(bk.gpair);
+        (void)hipFree(bk.dedcn);
     }
     for (void* p : allocs_) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -307,12 +318,14 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     std::sort(lds.begin(), lds.end());
     lds.erase(std::unique(lds.begin(), lds.end()), lds.end());
     h_bucket_of_.assign(ninst_, 0);
+    h_bucket_pos_.assign(ninst_, 0);
     buckets_.resize(lds.size());
     std::vector<double*> hmat((size_t)kRoles * ninst_, nullptr);
     for (size_t bi = 0; bi < lds.size(); ++bi) buckets_[bi].ld = lds[bi];
     for (int i = 0; i < ninst_; ++i) {
         int bi = (int)(std::lower_bound(lds.begin(), lds.end(), h_ld_[i]) - lds.begin());
         h_bucket_of_[i] = bi;
+        h_bucket_pos_[i] = (int)buckets_[bi].inst.size();
         buckets_[bi].inst.push_back(i);
     }
     for (auto& bk : buckets_) {
@@ -344,6 +357,21 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         for (int i : bk.inst) kmax = std::max(kmax, (int)std::ceil(0.5 * h_nelec_[i] - 1e-9));
         bk.kcap = opt.electronic_temperature > kWideSmearing ? bk.ld : std::min(bk.ld, kmax + 32);
         HIP_TRY(hipMalloc(&bk.lu, sizeof(double) * 5 * (size_t)bk.ld * bk.kcap * count));
+        HIP_TRY(hipMalloc(&bk.st_dev, sizeof(SccState) * count));
+        HIP_TRY(hipHostMalloc((void**)&bk.h_ptrs, sizeof(double*) * kActRoles * count, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&bk.h_act, sizeof(int) * count, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void**)&bk.st_host, sizeof(SccState) * count, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&bk.done, hipEventDisableTiming));
+        std::vector<int> atoms;
+        for (int i : bk.inst)
+            for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
+        bk.natoms = (int)atoms.size();
+        const size_t na_alloc = std::max<size_t>(atoms.size(), 1);
+        HIP_TRY(hipMalloc(&bk.atoms, sizeof(int) * na_alloc));
+        if (!atoms.empty())
+            HIP_TRY(hipMemcpy(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&bk.gpair, sizeof(double) * 3 * na_alloc));
+        HIP_TRY(hipMalloc(&bk.dedcn, sizeof(double) * na_alloc));
     }
     HIP_TRY(hipMemcpy(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
 
@@ -387,34 +415,35 @@ Status Batch::grow_kcap(Bucket& bk, int kcap) {
     return Status{};
 }
 
+// Pointer tables and active list of the listed instances of a bucket, copied
+// from pinned staging on the bucket's stream.  The caller guarantees that the
+// stream has no pending copy from the staging buffers (the bucket is idle).
 Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity) {
-    // host copy of the pointer tables of the listed instances
-    std::vector<double*> hp((size_t)kActRoles * list.size());
-    std::vector<double*> all((size_t)kRoles * ninst_);
-    // reconstruct the arena pointers of this bucket
     const size_t mat = (size_t)bk.ld * bk.ld;
     const size_t count = bk.inst.size();
-    for (size_t j = 0; j < list.size(); ++j) {
+    const size_t m = list.size();
+    double** hp = bk.h_ptrs;
+    for (size_t j = 0; j < m; ++j) {
         const int i = list[j];
-        const size_t k = std::find(bk.inst.begin(), bk.inst.end(), i) - bk.inst.begin();
+        const size_t k = (size_t)h_bucket_pos_[i];
         auto at = [&](int role) { return bk.arena + ((size_t)role * count + k) * mat; };
         const int par = parity < 0 ? h_final_parity_[i] : parity;
-        hp[(size_t)A_F * list.size() + j] = at(R_F);
-        hp[(size_t)A_CB * list.size() + j] = at(par ? R_CB1 : R_CB0);
-        hp[(size_t)A_C * list.size() + j] = at(par ? R_CB0 : R_CB1);
-        hp[(size_t)A_T * list.size() + j] = at(R_T);
-        hp[(size_t)A_A * list.size() + j] = at(R_A);
-        hp[(size_t)A_P * list.size() + j] = at(R_P);
-        hp[(size_t)A_S * list.size() + j] = at(R_S);
-        hp[(size_t)A_X * list.size() + j] = at(R_W);
-        hp[(size_t)A_C32 * list.size() + j] = (double*)((float*)at(R_A) + (size_t)bk.ld * bk.kcap);
-        hp[(size_t)A_H0 * list.size() + j] = at(R_H0);
-        hp[(size_t)A_L * list.size() + j] = at(R_L);
-        hp[(size_t)A_A0 * list.size() + j] = at(R_A0);
+        hp[(size_t)A_F * m + j] = at(R_F);
+        hp[(size_t)A_CB * m + j] = at(par ? R_CB1 : R_CB0);
+        hp[(size_t)A_C * m + j] = at(par ? R_CB0 : R_CB1);
+        hp[(size_t)A_T * m + j] = at(R_T);
+        hp[(size_t)A_A * m + j] = at(R_A);
+        hp[(size_t)A_P * m + j] = at(R_P);
+        hp[(size_t)A_S * m + j] = at(R_S);
+        hp[(size_t)A_X * m + j] = at(R_W);
+        hp[(size_t)A_C32 * m + j] = (double*)((float*)at(R_A) + (size_t)bk.ld * bk.kcap);
+        hp[(size_t)A_H0 * m + j] = at(R_H0);
+        hp[(size_t)A_L * m + j] = at(R_L);
+        hp[(size_t)A_A0 * m + j] = at(R_A0);
+        bk.h_act[j] = i;
     }
-    HIP_TRY(hipMemcpyAsync(bk.ptrs, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
-    HIP_TRY(hipMemcpyAsync(bk.act, list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice, bk.stream));
-    HIP_TRY(hipStreamSynchronize(bk.stream));  // host staging buffers are stack-local
+    HIP_TRY(hipMemcpyAsync(bk.ptrs, hp, (size_t)kActRoles * m * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
+    HIP_TRY(hipMemcpyAsync(bk.act, bk.h_act, m * sizeof(int), hipMemcpyHostToDevice, bk.stream));
     return Status{};
 }
 
@@ -620,216 +649,46 @@ Status Batch::setup_phase() {
     return Status{};
 }
 
+// The SCC runs each size bucket at its own pace: a bucket's next iteration is
+// launched as soon as its previous one has finished, and the buckets never
+// wait for each other.  Each iteration ends by gathering the states of the
+// bucket's active instances into pinned host memory, behind an event; the host
+// polls the events of the buckets in flight, reads the states of whichever
+// bucket has finished, and launches its next iteration, or, once none of its
+// instances iterates any more, its energy-weighted density and pair gradient.
 Status Batch::scc_phase() {
-    const DeviceTables t = dm_->tables();
     for (auto& bk : buckets_) {
         bk.active.clear();
         for (int i : bk.inst)
             if (h_state_[i].status == 0) bk.active.push_back(i);
+        bk.inflight = false;
+        bk.iter = 0;
     }
-    const double one = 1.0, zero = 0.0;
-    // GFN1_PROGRESS=1: one line per SCC iteration on stderr (long single-system runs)
-    const bool progress = std::getenv("GFN1_PROGRESS") != nullptr;
-    const auto t_scc = std::chrono::steady_clock::now();
-    int sweep = 0;
+    progress_ = std::getenv("GFN1_PROGRESS") != nullptr;
+    t_scc_ = std::chrono::steady_clock::now();
+    // largest matrices first: their chains are the longest, the small buckets fill in
+    for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
+        Status st = bit->active.empty() ? launch_bucket_gradient(*bit) : launch_iteration(*bit);
+        if (!st.ok()) return st;
+    }
     while (true) {
-        bool any = false;
-        for (auto& bk : buckets_) any = any || !bk.active.empty();
-        if (!any) break;
-        // largest matrices first: their chains are the longest, the small buckets fill in
+        bool any = false, progressed = false;
         for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
             Bucket& bk = *bit;
-            const int cnt = (int)bk.active.size();
-            if (!cnt) continue;
-            Status st = upload_active(bk, bk.active, 0);
+            if (!bk.inflight) continue;
+            any = true;
+            const hipError_t q = hipEventQuery(bk.done);
+            if (q == hipErrorNotReady) continue;
+            HIP_TRY(q);
+            bk.inflight = false;
+            progressed = true;
+            Status st = finish_iteration(bk);
             if (!st.ok()) return st;
-            int max_nsh = 0;
-            for (int i : bk.active) max_nsh = std::max(max_nsh, h_nsh_[i]);
-            const int n = bk.ld;
-            // ---- Fock matrix from the input charges
-            hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
-                               t, (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
-            LAUNCH_CHECK();
-            // level shift F += b (S - 1/2 S P_prev S) for instances that carry a previous density
-            std::vector<int> ls;
-            for (int i : bk.active)
-                if (h_state_[i].have_prev_p && h_state_[i].level_shift != 0.0) ls.push_back(i);
-            // F itself is formed only for the result matrices and the level-shift rungs;
-            // otherwise A comes from the reduced congruence below
-            if (opt_.keep_matrices || !ls.empty()) {
-                hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                                   (const double*)d_vin_t_);
-                LAUNCH_CHECK();
-            }
-            if (!ls.empty()) {
-                st = upload_active(bk, ls, 0);
-                if (!st.ok()) return st;
-                const int c = (int)ls.size();
-                double* const* P = bk.ptrs + (size_t)A_P * c;
-                double* const* S = bk.ptrs + (size_t)A_S * c;
-                double* const* T = bk.ptrs + (size_t)A_T * c;
-                double* const* Ab = bk.ptrs + (size_t)A_A * c;
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
-                                             &one, (const double* const*)S, n, (const double* const*)P, n, &zero,
-                                             (double* const*)T, n, c));
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
-                                             &one, (const double* const*)T, n, (const double* const*)S, n, &zero,
-                                             (double* const*)Ab, n, c));
-                std::vector<double> shift(c);
-                for (int k = 0; k < c; ++k) shift[k] = h_state_[ls[k]].level_shift;
-                HIP_TRY(hipMemcpyAsync(bk.eig_e, shift.data(), c * sizeof(double), hipMemcpyHostToDevice,
-                                       bk.stream));
-                hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                                   (const double*)bk.eig_e, (double* const*)Ab);
-                LAUNCH_CHECK();
-                HIP_TRY(hipStreamSynchronize(bk.stream));
-                st = upload_active(bk, bk.active, 0);
-                if (!st.ok()) return st;
-            }
-            double* const* F = bk.ptrs + (size_t)A_F * cnt;
-            double* const* Cb = bk.ptrs + (size_t)A_CB * cnt;  // L^-T (S = L L^T)
-            double* const* Z = bk.ptrs + (size_t)A_C * cnt;    // eigenvectors of the reduced problem
-            double* const* T = bk.ptrs + (size_t)A_T * cnt;
-            double* const* A = bk.ptrs + (size_t)A_A * cnt;
-            // ---- A = Cb^T F Cb (orthonormal basis; lower triangle and diagonal are used)
-            if (!ls.empty()) {
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n,
-                                             &one, (const double* const*)F, n, (const double* const*)Cb, n, &zero,
-                                             (double* const*)T, n, cnt));
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n,
-                                             n, &one, (const double* const*)Cb, n, (const double* const*)T, n,
-                                             &zero, (double* const*)A, n, cnt));
-            } else {
-                const int nt64 = (n + 63) / 64;
-                hipLaunchKernelGGL(k_reduce_fock, dim3(nt64, nt64, cnt), dim3(256), 0, bk.stream, db_,
-                                   (const int*)bk.act, (const double*)d_vin_t_, Cb,
-                                   bk.ptrs + (size_t)A_L * cnt, bk.ptrs + (size_t)A_A0 * cnt, A);
-                LAUNCH_CHECK();
-            }
-            const int kc = bk.kcap;
-            // matrices above kEigMaxDim go to rocSOLVER
-            const bool own_solver = opt_.eigensolver != 1 && n <= kEigMaxDim;
-            if (own_solver) {
-                // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
-                // (returned transposed in Z, kc columns)
-                HIP_TRY(hipEventRecord(bk.ev0, bk.stream));
-                HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, bk.eig,
-                                       bk.eig_e, bk.tau));
-                HIP_TRY(hipEventRecord(bk.ev1, bk.stream));
-                bk.ev_pending = true;
-                bk.ev_bytes = 0.0;
-                for (int i : bk.active) {
-                    const double na = h_nao_[i];
-                    bk.ev_bytes += 8.0 * (na - 1.0) * na * (na + 1.0) / 6.0;
-                }
-                HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n,
-                                           (const double*)bk.eig, (const double*)bk.eig_e, d_eps_,
-                                           (const int64_t*)d_vec0_, d_occ_, d_state_, Z, bk.lu, kc,
-                                           opt_.keep_matrices != 0));
-                HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
-                                             (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
-                // C_k = Cb Z (into T)
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, kc,
-                                             n, &one, (const double* const*)Cb, n, (const double* const*)Z, kc,
-                                             &zero, (double* const*)T, n, cnt));
-            } else {
-                // rocSOLVER runs alone on the device, as in the setup phase
-                HIP_TRY(hipDeviceSynchronize());
-                Status st2 = eigensolve(bk, cnt);
-                if (!st2.ok()) return st2;
-                HIP_TRY(hipStreamSynchronize(bk.stream));
-                hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                                   (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
-                LAUNCH_CHECK();
-                // rocSOLVER returns the eigenvectors in place
-                RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, kc, n,
-                                             &one, (const double* const*)Cb, n, (const double* const*)A, n, &zero,
-                                             (double* const*)T, n, cnt));
-            }
-            // ---- P = C_k diag(f) C_k^T
-            double* const* P = bk.ptrs + (size_t)A_P * cnt;
-            const int nt = (n + 31) / 32;
-            // (FP64 in every precision mode: the SCC energy and charges need it)
-            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               T, A, (const double*)d_eps_, (const double*)d_occ_, kc, 0);
-            LAUNCH_CHECK();
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                                         kc, &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
-                                         (double* const*)P, n, cnt));
-            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_,
-                               (const int*)bk.act, P);
-            LAUNCH_CHECK();
-            // ---- Mulliken charges, energy, convergence and mixing
-            hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_,
-                               d_rowH_);
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
-                               (const double*)d_rowS_, (const double*)d_rowH_, d_qout_, d_state_);
-            LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_,
-                               t, (const int*)bk.act, (const double*)d_qout_, d_vout_g_, d_vout_t_);
-            LAUNCH_CHECK();
-            // Broyden history longer than 32 needs global scratch
-            int mmax = 0;
-            for (int i : bk.active) mmax = std::max(mmax, std::min(h_state_[i].broyden_iter, broyden_memory_));
-            const int beta_ld = std::max(mmax, 1);
-            if (mmax > 32) {
-                size_t need = (size_t)cnt * (beta_ld + 1) * beta_ld;
-                if (need > bk.beta_cap) {
-                    (void)hipFree(bk.beta);
-                    HIP_TRY(hipMalloc(&bk.beta, need * 2 * sizeof(double)));
-                    bk.beta_cap = need * 2;
-                }
-            }
-            hipLaunchKernelGGL(k_scc_update, dim3(cnt), dim3(256), 0, bk.stream, db_, t, (const int*)bk.act, prm_,
-                               (const double*)d_qout_, (const double*)d_vout_g_, d_qin_, (const double*)d_q0_,
-                               d_state_, d_brdf_, d_bru_, d_brom_, d_brql_, d_brdql_, (const int64_t*)d_broff_,
-                               bk.beta, beta_ld);
-            LAUNCH_CHECK();
+            st = bk.active.empty() ? launch_bucket_gradient(bk) : launch_iteration(bk);
+            if (!st.ok()) return st;
         }
-        HIP_TRY(hipDeviceSynchronize());
-        HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
-        for (auto& bk : buckets_) {
-            if (bk.ev_pending) {
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, bk.ev0, bk.ev1));
-                tri_ms_ += ms;
-                tri_bytes_ += bk.ev_bytes;
-                ++tri_launches_;
-                bk.ev_pending = false;
-            }
-            if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
-                std::vector<int> info(bk.active.size());
-                HIP_TRY(hipMemcpy(info.data(), bk.info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
-                for (int v : info)
-                    if (v != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
-            }
-            // the last ladder rung smears at >= 3000 K: keep every eigenvector from then on
-            bool hot = false;
-            for (int i : bk.active) hot = hot || h_state_[i].temperature > kWideSmearing;
-            if (hot && bk.kcap < bk.ld) {
-                Status gs = grow_kcap(bk, bk.ld);
-                if (!gs.ok()) return gs;
-            }
-            std::vector<int> next;
-            for (int i : bk.active) {
-                if (h_state_[i].kneed_full > bk.kcap)
-                    return Status::error(kErrLinearAlgebra,
-                                         "more partially occupied orbitals than the eigenvector capacity");
-                if (h_state_[i].eig_fail)
-                    return Status::error(kErrLinearAlgebra, "eigenvalue cluster too large for the batched solver");
-                if (h_state_[i].status == 0) next.push_back(i);
-            }
-            bk.active.swap(next);
-        }
-        if (progress) {
-            size_t left = 0;
-            for (auto& bk : buckets_) left += bk.active.size();
-            const double ms =
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_scc).count();
-            std::fprintf(stderr, "scc iteration %d: %zu instances still iterating, %.1f ms\n", ++sweep, left, ms);
-            std::fflush(stderr);
-        }
+        if (!any) break;
+        if (!progressed) std::this_thread::yield();
     }
     for (int i = 0; i < ninst_; ++i)
         if (h_state_[i].status == 2 && !h_err_[i]) {
@@ -839,19 +698,222 @@ Status Batch::scc_phase() {
     return Status{};
 }
 
-Status Batch::gradient_phase() {
+// One SCC iteration of the bucket's active instances, ending with the copy of
+// their states to the host and the bucket's `done` event.
+Status Batch::launch_iteration(Bucket& bk) {
     const DeviceTables t = dm_->tables();
     const double one = 1.0, zero = 0.0;
-    // energy-weighted density W = C_k diag(eps f) C_k^T from the orbitals of the
-    // converged iteration (kept in T)
-    for (auto& bk : buckets_) {
-        std::vector<int> done;
-        for (int i : bk.inst)
-            if (h_state_[i].status == 1) done.push_back(i);
-        const int cnt = (int)done.size();
-        if (!cnt) continue;
+    const int cnt = (int)bk.active.size();
+    Status st = upload_active(bk, bk.active, 0);
+    if (!st.ok()) return st;
+    int max_nsh = 0;
+    for (int i : bk.active) max_nsh = std::max(max_nsh, h_nsh_[i]);
+    const int n = bk.ld;
+    // ---- Fock matrix from the input charges
+    hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
+                       (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
+    LAUNCH_CHECK();
+    // level shift F += b (S - 1/2 S P_prev S) for instances that carry a previous density
+    std::vector<int> ls;
+    for (int i : bk.active)
+        if (h_state_[i].have_prev_p && h_state_[i].level_shift != 0.0) ls.push_back(i);
+    // F itself is formed only for the result matrices and the level-shift rungs;
+    // otherwise A comes from the reduced congruence below
+    if (opt_.keep_matrices || !ls.empty()) {
+        hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                           (const double*)d_vin_t_);
+        LAUNCH_CHECK();
+    }
+    if (!ls.empty()) {
+        // the staging buffers still feed the copy of the full active list
+        HIP_TRY(hipStreamSynchronize(bk.stream));
+        st = upload_active(bk, ls, 0);
+        if (!st.ok()) return st;
+        const int c = (int)ls.size();
+        double* const* P = bk.ptrs + (size_t)A_P * c;
+        double* const* S = bk.ptrs + (size_t)A_S * c;
+        double* const* T = bk.ptrs + (size_t)A_T * c;
+        double* const* Ab = bk.ptrs + (size_t)A_A * c;
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)S, n, (const double* const*)P, n, &zero,
+                                     (double* const*)T, n, c));
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)T, n, (const double* const*)S, n, &zero,
+                                     (double* const*)Ab, n, c));
+        std::vector<double> shift(c);
+        for (int k = 0; k < c; ++k) shift[k] = h_state_[ls[k]].level_shift;
+        HIP_TRY(hipMemcpyAsync(bk.eig_e, shift.data(), c * sizeof(double), hipMemcpyHostToDevice, bk.stream));
+        hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                           (const double*)bk.eig_e, (double* const*)Ab);
+        LAUNCH_CHECK();
+        HIP_TRY(hipStreamSynchronize(bk.stream));
+        st = upload_active(bk, bk.active, 0);
+        if (!st.ok()) return st;
+    }
+    double* const* F = bk.ptrs + (size_t)A_F * cnt;
+    double* const* Cb = bk.ptrs + (size_t)A_CB * cnt;  // L^-T (S = L L^T)
+    double* const* Z = bk.ptrs + (size_t)A_C * cnt;    // eigenvectors of the reduced problem
+    double* const* T = bk.ptrs + (size_t)A_T * cnt;
+    double* const* A = bk.ptrs + (size_t)A_A * cnt;
+    // ---- A = Cb^T F Cb (orthonormal basis; lower triangle and diagonal are used)
+    if (!ls.empty()) {
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)F, n, (const double* const*)Cb, n, &zero,
+                                     (double* const*)T, n, cnt));
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)Cb, n, (const double* const*)T, n, &zero,
+                                     (double* const*)A, n, cnt));
+    } else {
+        const int nt64 = (n + 63) / 64;
+        hipLaunchKernelGGL(k_reduce_fock, dim3(nt64, nt64, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                           (const double*)d_vin_t_, Cb, bk.ptrs + (size_t)A_L * cnt, bk.ptrs + (size_t)A_A0 * cnt,
+                           A);
+        LAUNCH_CHECK();
+    }
+    const int kc = bk.kcap;
+    // matrices above kEigMaxDim go to rocSOLVER
+    const bool own_solver = opt_.eigensolver != 1 && n <= kEigMaxDim;
+    if (own_solver) {
+        // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
+        // (returned transposed in Z, kc columns)
+        HIP_TRY(hipEventRecord(bk.ev0, bk.stream));
+        HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, bk.eig, bk.eig_e,
+                               bk.tau));
+        HIP_TRY(hipEventRecord(bk.ev1, bk.stream));
+        bk.ev_pending = true;
+        bk.ev_bytes = 0.0;
+        for (int i : bk.active) {
+            const double na = h_nao_[i];
+            bk.ev_bytes += 8.0 * (na - 1.0) * na * (na + 1.0) / 6.0;
+        }
+        HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, (const double*)bk.eig,
+                                   (const double*)bk.eig_e, d_eps_, (const int64_t*)d_vec0_, d_occ_, d_state_, Z,
+                                   bk.lu, kc, opt_.keep_matrices != 0));
+        HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
+                                     (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
+        // C_k = Cb Z (into T)
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, kc, n, &one,
+                                     (const double* const*)Cb, n, (const double* const*)Z, kc, &zero,
+                                     (double* const*)T, n, cnt));
+    } else {
+        // rocSOLVER runs alone on the device, as in the setup phase
+        HIP_TRY(hipDeviceSynchronize());
+        Status st2 = eigensolve(bk, cnt);
+        if (!st2.ok()) return st2;
+        HIP_TRY(hipStreamSynchronize(bk.stream));
+        hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
+                           (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
+        LAUNCH_CHECK();
+        // rocSOLVER returns the eigenvectors in place
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, kc, n, &one,
+                                     (const double* const*)Cb, n, (const double* const*)A, n, &zero,
+                                     (double* const*)T, n, cnt));
+    }
+    // ---- P = C_k diag(f) C_k^T
+    double* const* P = bk.ptrs + (size_t)A_P * cnt;
+    const int nt = (n + 31) / 32;
+    // (FP64 in every precision mode: the SCC energy and charges need it)
+    hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
+                       (const double*)d_eps_, (const double*)d_occ_, kc, 0);
+    LAUNCH_CHECK();
+    RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc, &one,
+                                 (const double* const*)A, n, (const double* const*)T, n, &zero, (double* const*)P, n,
+                                 cnt));
+    hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, P);
+    LAUNCH_CHECK();
+    // ---- Mulliken charges, energy, convergence and mixing
+    hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_, d_rowH_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
+                       (const double*)d_rowS_, (const double*)d_rowH_, d_qout_, d_state_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
+                       (const int*)bk.act, (const double*)d_qout_, d_vout_g_, d_vout_t_);
+    LAUNCH_CHECK();
+    // Broyden history longer than 32 needs global scratch
+    int mmax = 0;
+    for (int i : bk.active) mmax = std::max(mmax, std::min(h_state_[i].broyden_iter, broyden_memory_));
+    const int beta_ld = std::max(mmax, 1);
+    if (mmax > 32) {
+        size_t need = (size_t)cnt * (beta_ld + 1) * beta_ld;
+        if (need > bk.beta_cap) {
+            (void)hipFree(bk.beta);
+            HIP_TRY(hipMalloc(&bk.beta, need * 2 * sizeof(double)));
+            bk.beta_cap = need * 2;
+        }
+    }
+    hipLaunchKernelGGL(k_scc_update, dim3(cnt), dim3(256), 0, bk.stream, db_, t, (const int*)bk.act, prm_,
+                       (const double*)d_qout_, (const double*)d_vout_g_, d_qin_, (const double*)d_q0_, d_state_,
+                       d_brdf_, d_bru_, d_brom_, d_brql_, d_brdql_, (const int64_t*)d_broff_, bk.beta, beta_ld);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gather_state, dim3((cnt + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, cnt,
+                       (const SccState*)d_state_, bk.st_dev);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMemcpyAsync(bk.st_host, bk.st_dev, cnt * sizeof(SccState), hipMemcpyDeviceToHost, bk.stream));
+    HIP_TRY(hipEventRecord(bk.done, bk.stream));
+    bk.inflight = true;
+    ++bk.iter;
+    return Status{};
+}
+
+// Host side of a finished iteration: states, kernel timing, checks, next active list.
+Status Batch::finish_iteration(Bucket& bk) {
+    for (size_t j = 0; j < bk.active.size(); ++j) h_state_[bk.active[j]] = bk.st_host[j];
+    if (bk.ev_pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, bk.ev0, bk.ev1));
+        tri_ms_ += ms;
+        tri_bytes_ += bk.ev_bytes;
+        ++tri_launches_;
+        bk.ev_pending = false;
+    }
+    if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
+        std::vector<int> info(bk.active.size());
+        HIP_TRY(hipMemcpy(info.data(), bk.info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int v : info)
+            if (v != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
+    }
+    // the last ladder rung smears at >= 3000 K: keep every eigenvector from then on
+    bool hot = false;
+    for (int i : bk.active) hot = hot || h_state_[i].temperature > kWideSmearing;
+    if (hot && bk.kcap < bk.ld) {
+        Status gs = grow_kcap(bk, bk.ld);
+        if (!gs.ok()) return gs;
+    }
+    std::vector<int> next;
+    for (int i : bk.active) {
+        if (h_state_[i].kneed_full > bk.kcap)
+            return Status::error(kErrLinearAlgebra, "more partially occupied orbitals than the eigenvector capacity");
+        if (h_state_[i].eig_fail)
+            return Status::error(kErrLinearAlgebra, "eigenvalue cluster too large for the batched solver");
+        if (h_state_[i].status == 0) next.push_back(i);
+    }
+    bk.active.swap(next);
+    if (progress_) {
+        size_t left = 0;
+        for (auto& b : buckets_) left += b.active.size();
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_scc_).count();
+        std::fprintf(stderr, "scc iteration %d of the %d-AO bucket: %zu instances still iterating, %.1f ms\n", bk.iter,
+                     bk.ld, left, ms);
+        std::fflush(stderr);
+    }
+    return Status{};
+}
+
+// Energy-weighted density W = C_k diag(eps f) C_k^T of the bucket's converged
+// instances, from the orbitals of their last iteration (kept in T), and the
+// pair part of the electronic gradient of the bucket's atoms, on the bucket's
+// stream.  Runs while other buckets are still iterating.
+Status Batch::launch_bucket_gradient(Bucket& bk) {
+    const DeviceTables t = dm_->tables();
+    std::vector<int> done;
+    for (int i : bk.inst)
+        if (h_state_[i].status == 1) done.push_back(i);
+    const int cnt = (int)done.size();
+    if (cnt) {
         Status st = upload_active(bk, done, 0);
         if (!st.ok()) return st;
+        const double one = 1.0, zero = 0.0;
         const int n = bk.ld, kc = bk.kcap;
         double* const* T = bk.ptrs + (size_t)A_T * cnt;
         double* const* A = bk.ptrs + (size_t)A_A * cnt;
@@ -881,24 +943,41 @@ Status Batch::gradient_phase() {
                                W);
             LAUNCH_CHECK();
         }
-        HIP_TRY(hipStreamSynchronize(bk.stream));
     }
+    const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+    if (opt_.compute_gradient && el_on && bk.natoms > 0) {
+        const int cn_on = opt_.enable_cn_hamiltonian;
+        auto pairs = opt_.mixed ? k_grad_pairs_t<false, true> : k_grad_pairs_t<false, false>;
+        hipLaunchKernelGGL(pairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
+                           (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
+                           opt_.integral_cutoff, bk.gpair, bk.dedcn);
+        LAUNCH_CHECK();
+        if (has_d_) {
+            auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true> : k_grad_pairs_t<true, false>;
+            hipLaunchKernelGGL(dpairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
+                               (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
+                               opt_.integral_cutoff, bk.gpair, bk.dedcn);
+            LAUNCH_CHECK();
+        }
+    }
+    return Status{};
+}
+
+// After every bucket has finished: the per-bucket pair sums go to the per-atom
+// arrays, then the atom-wise terms and the CN chain rule.
+Status Batch::gradient_phase() {
+    const DeviceTables t = dm_->tables();
     if (!opt_.compute_gradient || natoms_ == 0) return Status{};
     const int na = natoms_;
     HIP_TRY(hipMemsetAsync(d_gel_, 0, 3 * na * sizeof(double), stream_));
     const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
     const int cn_on = opt_.enable_cn_hamiltonian;
     if (el_on) {
-        auto pairs = opt_.mixed ? k_grad_pairs_t<false, true> : k_grad_pairs_t<false, false>;
-        hipLaunchKernelGGL(pairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
-                           (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
-                           d_dedcn_h_);
-        LAUNCH_CHECK();
-        if (has_d_) {
-            auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true> : k_grad_pairs_t<true, false>;
-            hipLaunchKernelGGL(dpairs, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
-                               (const double*)d_vout_t_, opt_.include_scc, cn_on, opt_.integral_cutoff, d_gel_,
-                               d_dedcn_h_);
+        for (auto& bk : buckets_) {
+            if (bk.natoms == 0) continue;
+            hipLaunchKernelGGL(k_scatter_atoms, dim3((bk.natoms + 127) / 128), dim3(128), 0, stream_,
+                               (const int*)bk.atoms, bk.natoms, (const double*)bk.gpair, (const double*)bk.dedcn,
+                               d_gel_, d_dedcn_h_);
             LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,

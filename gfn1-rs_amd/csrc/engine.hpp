@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <chrono>
 #include <memory>
 #include <string>
 #include <vector>
@@ -131,11 +132,32 @@ class Batch {
         bool ev_pending = false;
         std::vector<int> active;
         int kP = 0;
+        // independent SCC progress: the states of the active instances are
+        // gathered and copied to pinned host memory at the end of every
+        // iteration, and `done` marks their arrival
+        SccState* st_dev = nullptr;
+        SccState* st_host = nullptr;
+        hipEvent_t done = nullptr;
+        bool inflight = false;
+        int iter = 0;  // SCC iterations of the current run
+        // pinned staging of the pointer tables and the active list (the bucket's
+        // stream is idle whenever they are rewritten)
+        double** h_ptrs = nullptr;
+        int* h_act = nullptr;
+        // the bucket's atoms and its own pair-gradient sums (separate allocations,
+        // so buckets that finish at different times never write the same cache line)
+        int* atoms = nullptr;
+        int natoms = 0;
+        double* gpair = nullptr;  // 3 * natoms
+        double* dedcn = nullptr;  // natoms
     };
 
     Status setup_phase();
     Status scc_phase();
     Status gradient_phase();
+    Status launch_iteration(Bucket& bk);
+    Status finish_iteration(Bucket& bk);
+    Status launch_bucket_gradient(Bucket& bk);
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
     Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
@@ -148,6 +170,7 @@ class Batch {
     std::vector<double> h_x_, h_q0_, h_nelec_;
     std::vector<int64_t> h_vec0_, h_broff_;
     std::vector<int> h_bucket_of_, h_final_parity_;
+    std::vector<int> h_bucket_pos_;  // position of each instance in its bucket
     std::vector<int> h_err_, h_err_input_;
     std::vector<std::string> h_errmsg_, h_errmsg_input_;
     std::vector<Bucket> buckets_;
@@ -186,6 +209,8 @@ class Batch {
     std::vector<SccState> h_state_;
     hipStream_t stream_ = nullptr;
     bool ran_ = false;
+    bool progress_ = false;  // GFN1_PROGRESS: one stderr line per bucket iteration
+    std::chrono::steady_clock::time_point t_scc_{};
     bool has_d_ = false;  // any d shell in the batch
     int pad_multiple_ = kPadMultiple;
 

@@ -726,6 +726,12 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     const int inst = act[blockIdx.x];
     SccState& s = st[inst];
     if (s.status != 0) return;
+    // Every thread reads the state fields it needs here, before the first
+    // barrier; thread 0 writes the state only after that barrier.  (A field read
+    // by the other wavefronts after thread 0 has written it would split the
+    // work-group between the branches below.)
+    const double s_e_el = s.e_el, s_ts = s.ts, s_e_last = s.e_last;
+    const int s_have_energy = s.have_energy, s_rung = s.rung, s_iter = s.iter, s_broyden_iter = s.broyden_iter;
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int atom0 = b.atom0[inst], nat = b.atom0[inst + 1] - atom0;
     // E2 = 1/2 q.gamma q, E3 = sum_A Gamma_A q_A^3 / 3
@@ -748,8 +754,8 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     e3 = block_sum(e3, lds);
     ss = block_sum(ss, lds);
     const double rms = nsh ? sqrt(ss / nsh) : 0.0;
-    const double energy = s.e_el + e2 + e3 + s.ts;
-    const double de = s.have_energy ? fabs(energy - s.e_last) : INFINITY;
+    const double energy = s_e_el + e2 + e3 + s_ts;
+    const double de = s_have_energy ? fabs(energy - s_e_last) : INFINITY;
     __syncthreads();
     if (threadIdx.x == 0) {
         s.e2 = e2;
@@ -764,8 +770,8 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
         return;
     }
     // ---- next input charges
-    const LadderRung rung = {prm.rung_broyden[s.rung], prm.rung_mixing[s.rung], prm.rung_level_shift[s.rung],
-                             prm.rung_temperature[s.rung], prm.rung_max_iter[s.rung]};
+    const LadderRung rung = {prm.rung_broyden[s_rung], prm.rung_mixing[s_rung], prm.rung_level_shift[s_rung],
+                             prm.rung_temperature[s_rung], prm.rung_max_iter[s_rung]};
     const double mix = rung.mixing;
     double* qi = qin + s0;
     const double* qo = qout + s0;
@@ -779,7 +785,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
         double* omega = br_omega + (size_t)inst * prm.omega_stride;
         double* qlast = br_qlast + s0;
         double* dqlast = br_dqlast + s0;
-        const int iter = s.broyden_iter + 1;
+        const int iter = s_broyden_iter + 1;
         if (iter == 1) {
             for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
                 const double dq = qo[i] - qi[i];
@@ -882,13 +888,13 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
         s.have_energy = 1;
     }
     // rung exhausted -> next rung or failure (electronic.rs:1413-1446, 1950-1955)
-    if (s.iter >= rung.max_iter) {
+    if (s_iter + 1 >= rung.max_iter) {
         __syncthreads();
-        if (s.rung >= 3) {
+        if (s_rung >= 3) {
             if (threadIdx.x == 0) s.status = 2;
             return;
         }
-        const int nr = s.rung + 1;
+        const int nr = s_rung + 1;
         for (int i = threadIdx.x; i < nsh; i += blockDim.x) qi[i] = q0[s0 + i];
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1468,14 +1474,15 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
     }
 }
 
-// Electronic gradient, pair part (gradient.rs:333-462) for atom A: every pair
-// with B != A inside the integral cutoff, A taken as the bra.  Deterministic
-// per-atom sums (one 64-lane wavefront per atom, no atomics).
+// Electronic gradient, pair part (gradient.rs:333-462) for atom A = atoms[j]:
+// every pair with B != A inside the integral cutoff, A taken as the bra.
+// Deterministic per-atom sums (one 64-lane wavefront per atom, no atomics).
 template <bool DCLASS, bool MIXED>
-__global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn,
-                                                     const double* vsh, int scc_on, int cn_on, double cutoff,
-                                                     double* g_pair, double* dedcn_h) {
-    const int a = blockIdx.x;
+__global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms,
+                                                     const double* cn, const double* vsh, int scc_on, int cn_on,
+                                                     double cutoff, double* g_pair, double* dedcn_h) {
+    const int j = blockIdx.x;
+    const int a = atoms[j];
     const int inst = b.atom_inst[a];
     const int ld = b.ld[inst];
     gptr<const double> P = (gptr<const double>)b.P[inst];
@@ -1521,20 +1528,33 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     const double dc = wave_sum(acc.dcn());
     if (threadIdx.x == 0) {
         // the s/p pass initialises, the d pass accumulates
-        for (int d = 0; d < 3; ++d) g_pair[3 * a + d] = DCLASS ? g_pair[3 * a + d] + g[d] : g[d];
-        dedcn_h[a] = DCLASS ? dedcn_h[a] + dc : dc;
+        for (int d = 0; d < 3; ++d) g_pair[3 * j + d] = DCLASS ? g_pair[3 * j + d] + g[d] : g[d];
+        dedcn_h[j] = DCLASS ? dedcn_h[j] + dc : dc;
     }
+}
+
+__global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const double* dcl, double* g, double* dc) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int a = atoms[j];
+    for (int d = 0; d < 3; ++d) g[3 * a + d] = gl[3 * j + d];
+    dc[a] = dcl[j];
+}
+
+__global__ void k_gather_state(const int* act, int n, const SccState* st, SccState* out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = st[act[j]];
 }
 
 template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const double*, double);
 template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const double*, double);
-template __global__ void k_grad_pairs_t<false, false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
-                                               double, double*, double*);
-template __global__ void k_grad_pairs_t<true, false>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
-                                              double, double*, double*);
-template __global__ void k_grad_pairs_t<false, true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
-                                               double, double*, double*);
-template __global__ void k_grad_pairs_t<true, true>(DeviceBatch, DeviceTables, const double*, const double*, int, int,
-                                              double, double*, double*);
+template __global__ void k_grad_pairs_t<false, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<false, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                const double*, int, int, double, double*, double*);
 
 }  // namespace gfn1

@@ -57,9 +57,15 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
                            int natoms);
 __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal);
 __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
+// Pair part of the electronic gradient for the listed atoms: g_pair[3 j + d] and
+// dedcn_h[j] belong to atom atoms[j].
 template <bool DCLASS, bool MIXED>
-__global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const double* cn, const double* vsh, int scc_on,
-                               int cn_on, double cutoff, double* g_pair, double* dedcn_h);
+__global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn, const double* vsh,
+                               int scc_on, int cn_on, double cutoff, double* g_pair, double* dedcn_h);
+// Per-atom scatter of listed sums: g[3 atoms[j] + d] = gl[3 j + d], dc[atoms[j]] = dcl[j].
+__global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const double* dcl, double* g, double* dc);
+// out[j] = st[act[j]] for j < n.
+__global__ void k_gather_state(const int* act, int n, const SccState* st, SccState* out);
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
                              double* dedcn_h, double* g_el, int natoms);
 

@@ -135,6 +135,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libgfn1_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        # The batch engine runs one HIP stream per size bucket (up to 14 for
+        # matrices of 32..448 AO).  The HIP runtime maps streams onto
+        # GPU_MAX_HW_QUEUES hardware queues (default 4), and streams sharing a
+        # queue serialise; 16 gives every bucket its own queue.  The variable is
+        # read when the HIP runtime initialises, so it only takes effect when
+        # nothing in the process has used the GPU yet; an explicit setting wins.
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         lib = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(lib, name)
