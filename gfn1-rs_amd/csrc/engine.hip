@@ -13,6 +13,7 @@
 //   gradient  per-atom scatter of the bucket sums, atom-wise terms, CN chains.
 #include "engine.hpp"
 
+#include "cholesky.hpp"
 #include "eigen.hpp"
 
 #include <rocsolver/rocsolver.h>
@@ -122,7 +123,6 @@ Batch::~Batch() {
         (void)hipFree(bk.beta);
         (void)hipFree(bk.tau);
         (void)hipFree(bk.lu);
-        (void)hipFree(bk.st_dev);
         if (bk.h_ptrs) (void)hipHostFree(bk.h_ptrs);
         if (bk.h_act) (void)hipHostFree(bk.h_act);
         if (bk.st_host) (void)hipHostFree(bk.st_host);
@@ -357,10 +357,13 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         for (int i : bk.inst) kmax = std::max(kmax, (int)std::ceil(0.5 * h_nelec_[i] - 1e-9));
         bk.kcap = opt.electronic_temperature > kWideSmearing ? bk.ld : std::min(bk.ld, kmax + 32);
         HIP_TRY(hipMalloc(&bk.lu, sizeof(double) * 5 * (size_t)bk.ld * bk.kcap * count));
-        HIP_TRY(hipMalloc(&bk.st_dev, sizeof(SccState) * count));
-        HIP_TRY(hipHostMalloc((void**)&bk.h_ptrs, sizeof(double*) * kActRoles * count, hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc((void**)&bk.h_act, sizeof(int) * count, hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc((void**)&bk.st_host, sizeof(SccState) * count, hipHostMallocDefault));
+        // Per-iteration host traffic goes through coherent pinned buffers that
+        // small kernels read and write directly: an asynchronous copy would go
+        // to a copy engine shared by all streams, where a copy still waiting for
+        // its own stream's kernels holds up the copies of every other bucket.
+        HIP_TRY(hipHostMalloc((void**)&bk.h_ptrs, sizeof(double*) * kActRoles * count, hipHostMallocCoherent));
+        HIP_TRY(hipHostMalloc((void**)&bk.h_act, sizeof(int) * count, hipHostMallocCoherent));
+        HIP_TRY(hipHostMalloc((void**)&bk.st_host, sizeof(SccState) * count, hipHostMallocCoherent));
         HIP_TRY(hipEventCreateWithFlags(&bk.done, hipEventDisableTiming));
         std::vector<int> atoms;
         for (int i : bk.inst)
@@ -415,13 +418,15 @@ Status Batch::grow_kcap(Bucket& bk, int kcap) {
     return Status{};
 }
 
-// Pointer tables and active list of the listed instances of a bucket, copied
-// from pinned staging on the bucket's stream.  The caller guarantees that the
-// stream has no pending copy from the staging buffers (the bucket is idle).
+// Pointer tables and active list of the listed instances of a bucket, staged in
+// pinned host memory and copied by a kernel on the bucket's stream.  The caller
+// guarantees that the stream has no pending read of the staging buffers (the
+// bucket is idle).
 Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity) {
     const size_t mat = (size_t)bk.ld * bk.ld;
     const size_t count = bk.inst.size();
     const size_t m = list.size();
+    if (m == 0) return Status{};
     double** hp = bk.h_ptrs;
     for (size_t j = 0; j < m; ++j) {
         const int i = list[j];
@@ -442,8 +447,13 @@ Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity
         hp[(size_t)A_A0 * m + j] = at(R_A0);
         bk.h_act[j] = i;
     }
-    HIP_TRY(hipMemcpyAsync(bk.ptrs, hp, (size_t)kActRoles * m * sizeof(double*), hipMemcpyHostToDevice, bk.stream));
-    HIP_TRY(hipMemcpyAsync(bk.act, bk.h_act, m * sizeof(int), hipMemcpyHostToDevice, bk.stream));
+    const int wp = (int)(kActRoles * m * sizeof(double*) / sizeof(unsigned int));
+    hipLaunchKernelGGL(k_copy_words, dim3((wp + 255) / 256), dim3(256), 0, bk.stream, (const unsigned int*)hp,
+                       (unsigned int*)bk.ptrs, wp);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_words, dim3(((int)m + 255) / 256), dim3(256), 0, bk.stream,
+                       (const unsigned int*)bk.h_act, (unsigned int*)bk.act, (int)m);
+    LAUNCH_CHECK();
     return Status{};
 }
 
@@ -580,8 +590,50 @@ Status Batch::setup_phase() {
     }
     HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
 
-    // orthonormal start basis Cb = L^-T from S = L L^T (per bucket)
+    // orthonormal start basis Cb = L^-T from S = L L^T and A0 = Cb^T H0 Cb (per bucket)
+    const double one = 1.0, zero = 0.0;
+    auto a0_gemms = [&](Bucket& bk, int count) -> Status {
+        double* const* Cb = bk.ptrs + (size_t)A_CB * count;
+        double* const* H0 = bk.ptrs + (size_t)A_H0 * count;
+        double* const* T = bk.ptrs + (size_t)A_T * count;
+        double* const* A0 = bk.ptrs + (size_t)A_A0 * count;
+        const int n = bk.ld;
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)H0, n, (const double* const*)Cb, n, &zero,
+                                     (double* const*)T, n, count));
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)Cb, n, (const double* const*)T, n, &zero,
+                                     (double* const*)A0, n, count));
+        return Status{};
+    };
+    // Matrices up to kCholMaxDim: the engine's own factorisation kernels, every
+    // bucket on its own stream, all at once.
     for (auto& bk : buckets_) {
+        if (bk.ld > kCholMaxDim) continue;
+        const int count = (int)bk.inst.size();
+        Status st = upload_active(bk, bk.inst, 0);
+        if (!st.ok()) return st;
+        HIP_TRY(launch_cholesky_cb(bk.stream, count, (const int*)bk.act, (const int*)d_nao_, bk.ld,
+                                   bk.ptrs + (size_t)A_S * count, bk.ptrs + (size_t)A_L * count,
+                                   bk.ptrs + (size_t)A_CB * count, opt_.eigen_tolerance, bk.info));
+        st = a0_gemms(bk, count);
+        if (!st.ok()) return st;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto& bk : buckets_) {
+        if (bk.ld > kCholMaxDim) continue;
+        const int count = (int)bk.inst.size();
+        std::vector<int> info(count);
+        HIP_TRY(hipMemcpy(info.data(), bk.info, count * sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < count; ++k)
+            if (info[k] != 0 && !h_err_[bk.inst[k]]) {
+                h_err_[bk.inst[k]] = kErrInvalidInput;
+                h_errmsg_[bk.inst[k]] = "overlap matrix is not positive definite";
+            }
+    }
+    // Larger matrices (single large systems): rocSOLVER, one bucket at a time.
+    for (auto& bk : buckets_) {
+        if (bk.ld <= kCholMaxDim) continue;
         const int count = (int)bk.inst.size();
         Status st = upload_active(bk, bk.inst, 0);
         if (!st.ok()) return st;
@@ -608,27 +660,14 @@ Status Batch::setup_phase() {
         hipLaunchKernelGGL(k_make_cb, dim3(64, count), dim3(256), 0, bk.stream, db_, A, (const int*)bk.act,
                            opt_.eigen_tolerance, d_fail_);
         LAUNCH_CHECK();
-        // A0 = Cb^T H0 Cb
-        {
-            const double one = 1.0, zero = 0.0;
-            double* const* Cb = bk.ptrs + (size_t)A_CB * count;
-            double* const* H0 = bk.ptrs + (size_t)A_H0 * count;
-            double* const* T = bk.ptrs + (size_t)A_T * count;
-            double* const* A0 = bk.ptrs + (size_t)A_A0 * count;
-            const int n = bk.ld;
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                                         (const double* const*)H0, n, (const double* const*)Cb, n, &zero,
-                                         (double* const*)T, n, count));
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n,
-                                         &one, (const double* const*)Cb, n, (const double* const*)T, n, &zero,
-                                         (double* const*)A0, n, count));
-        }
-        // The buckets are set up one after another.  With this bucket's batched
-        // GEMMs still running on its stream while the next bucket's rocSOLVER
-        // potrf/trtri ran on another stream, about 1% of the instances of the
-        // buckets of 256 AO and more converged to wrong energies (differences up
-        // to 1e-2 Eh, different instances on every run, on MI355X / ROCm 7); with
-        // this synchronisation every run is reproducible.
+        st = a0_gemms(bk, count);
+        if (!st.ok()) return st;
+        // With this bucket's batched GEMMs still running on its stream while
+        // another bucket's rocSOLVER potrf/trtri ran on another stream, about 1%
+        // of the instances of the buckets of 256 AO and more converged to wrong
+        // energies (differences up to 1e-2 Eh, different instances on every run,
+        // on MI355X / ROCm 7); the rocSOLVER buckets are therefore set up one
+        // after another.
         HIP_TRY(hipStreamSynchronize(bk.stream));
     }
     HIP_TRY(hipDeviceSynchronize());
@@ -847,9 +886,8 @@ Status Batch::launch_iteration(Bucket& bk) {
                        d_brdf_, d_bru_, d_brom_, d_brql_, d_brdql_, (const int64_t*)d_broff_, bk.beta, beta_ld);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_gather_state, dim3((cnt + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, cnt,
-                       (const SccState*)d_state_, bk.st_dev);
+                       (const SccState*)d_state_, bk.st_host);
     LAUNCH_CHECK();
-    HIP_TRY(hipMemcpyAsync(bk.st_host, bk.st_dev, cnt * sizeof(SccState), hipMemcpyDeviceToHost, bk.stream));
     HIP_TRY(hipEventRecord(bk.done, bk.stream));
     bk.inflight = true;
     ++bk.iter;

@@ -133,9 +133,8 @@ class Batch {
         std::vector<int> active;
         int kP = 0;
         // independent SCC progress: the states of the active instances are
-        // gathered and copied to pinned host memory at the end of every
+        // gathered straight into pinned host memory at the end of every
         // iteration, and `done` marks their arrival
-        SccState* st_dev = nullptr;
         SccState* st_host = nullptr;
         hipEvent_t done = nullptr;
         bool inflight = false;

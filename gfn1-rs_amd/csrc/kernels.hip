@@ -1544,6 +1544,12 @@ __global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const
 __global__ void k_gather_state(const int* act, int n, const SccState* st, SccState* out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) out[j] = st[act[j]];
+    __threadfence_system();
+}
+
+__global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const double*, double);

@@ -64,8 +64,10 @@ __global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms, 
                                int scc_on, int cn_on, double cutoff, double* g_pair, double* dedcn_h);
 // Per-atom scatter of listed sums: g[3 atoms[j] + d] = gl[3 j + d], dc[atoms[j]] = dcl[j].
 __global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const double* dcl, double* g, double* dc);
-// out[j] = st[act[j]] for j < n.
+// out[j] = st[act[j]] for j < n (out may be coherent pinned host memory).
 __global__ void k_gather_state(const int* act, int n, const SccState* st, SccState* out);
+// dst[i] = src[i] for i < n 32-bit words (src may be coherent pinned host memory).
+__global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n);
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
                              double* dedcn_h, double* g_el, int natoms);
 
