@@ -125,6 +125,7 @@ Batch::~Batch() {
         (void)hipFree(bk.lu);
         if (bk.h_ptrs) (void)hipHostFree(bk.h_ptrs);
         if (bk.h_act) (void)hipHostFree(bk.h_act);
+        if (bk.h_aux) (void)hipHostFree(bk.h_aux);
         if (bk.st_host) (void)hipHostFree(bk.st_host);
         if (bk.done) (void)hipEventDestroy(bk.done);
         (void)hipFree(bk.atoms);
@@ -363,6 +364,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         // its own stream's kernels holds up the copies of every other bucket.
         HIP_TRY(hipHostMalloc((void**)&bk.h_ptrs, sizeof(double*) * kActRoles * count, hipHostMallocCoherent));
         HIP_TRY(hipHostMalloc((void**)&bk.h_act, sizeof(int) * count, hipHostMallocCoherent));
+        HIP_TRY(hipHostMalloc((void**)&bk.h_aux, sizeof(double) * count, hipHostMallocCoherent));
         HIP_TRY(hipHostMalloc((void**)&bk.st_host, sizeof(SccState) * count, hipHostMallocCoherent));
         HIP_TRY(hipEventCreateWithFlags(&bk.done, hipEventDisableTiming));
         std::vector<int> atoms;
@@ -779,9 +781,10 @@ Status Batch::launch_iteration(Bucket& bk) {
         RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                                      (const double* const*)T, n, (const double* const*)S, n, &zero,
                                      (double* const*)Ab, n, c));
-        std::vector<double> shift(c);
-        for (int k = 0; k < c; ++k) shift[k] = h_state_[ls[k]].level_shift;
-        HIP_TRY(hipMemcpyAsync(bk.eig_e, shift.data(), c * sizeof(double), hipMemcpyHostToDevice, bk.stream));
+        for (int k = 0; k < c; ++k) bk.h_aux[k] = h_state_[ls[k]].level_shift;
+        hipLaunchKernelGGL(k_copy_words, dim3((2 * c + 255) / 256), dim3(256), 0, bk.stream,
+                           (const unsigned int*)bk.h_aux, (unsigned int*)bk.eig_e, 2 * c);
+        LAUNCH_CHECK();
         hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig_e, (double* const*)Ab);
         LAUNCH_CHECK();
@@ -839,6 +842,10 @@ Status Batch::launch_iteration(Bucket& bk) {
         HIP_TRY(hipDeviceSynchronize());
         Status st2 = eigensolve(bk, cnt);
         if (!st2.ok()) return st2;
+        // rocSOLVER's convergence flags, read by the host once the iteration is done
+        hipLaunchKernelGGL(k_copy_words, dim3((cnt + 255) / 256), dim3(256), 0, bk.stream, (const unsigned int*)bk.info,
+                           (unsigned int*)bk.h_aux, cnt);
+        LAUNCH_CHECK();
         HIP_TRY(hipStreamSynchronize(bk.stream));
         hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
@@ -906,10 +913,9 @@ Status Batch::finish_iteration(Bucket& bk) {
         bk.ev_pending = false;
     }
     if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
-        std::vector<int> info(bk.active.size());
-        HIP_TRY(hipMemcpy(info.data(), bk.info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
-        for (int v : info)
-            if (v != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
+        const int* info = (const int*)bk.h_aux;
+        for (size_t j = 0; j < bk.active.size(); ++j)
+            if (info[j] != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");
     }
     // the last ladder rung smears at >= 3000 K: keep every eigenvector from then on
     bool hot = false;

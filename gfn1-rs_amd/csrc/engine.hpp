@@ -143,6 +143,7 @@ class Batch {
         // stream is idle whenever they are rewritten)
         double** h_ptrs = nullptr;
         int* h_act = nullptr;
+        double* h_aux = nullptr;  // level shifts to the device, rocSOLVER info back (count entries)
         // the bucket's atoms and its own pair-gradient sums (separate allocations,
         // so buckets that finish at different times never write the same cache line)
         int* atoms = nullptr;

@@ -680,37 +680,73 @@ struct LadderRung {
     int max_iter;
 };
 
-__device__ bool solve_gj(double* a, double* bvec, int m, int lda) {
-    // Gauss-Jordan with partial pivoting (electronic.rs:3767-3806); single thread.
+// Gauss-Jordan with partial pivoting (electronic.rs:3767-3806) by the whole
+// work-group: the pivot search is a block arg-max (first index among equal
+// magnitudes, as in the serial loop), each thread owns whole rows in the
+// elimination, and every element sees the same operations in the same order as
+// in the serial algorithm.  a (m x m, row stride lda) and bvec may be in global
+// memory.  Returns the same value in every thread.
+__device__ bool solve_gj_block(double* a, double* bvec, int m, int lda, double* rv, int* ri) {
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
     for (int col = 0; col < m; ++col) {
-        int piv = col;
-        double pa = fabs(a[col * lda + col]);
-        for (int r = col + 1; r < m; ++r)
-            if (fabs(a[r * lda + col]) > pa) {
-                piv = r;
-                pa = fabs(a[r * lda + col]);
+        double best = -1.0;
+        int bi = m;
+        for (int r = col + tid; r < m; r += nt) {
+            const double v = fabs(a[r * lda + col]);
+            if (v > best) {
+                best = v;
+                bi = r;
             }
-        if (pa < 1e-18) return false;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_down(best, off, 64);
+            const int oi = __shfl_down(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            rv[wid] = best;
+            ri[wid] = bi;
+        }
+        __syncthreads();
+        best = rv[0];
+        bi = ri[0];
+        for (int w = 1; w < nw; ++w)
+            if (rv[w] > best || (rv[w] == best && ri[w] < bi)) {
+                best = rv[w];
+                bi = ri[w];
+            }
+        __syncthreads();
+        if (best < 1e-18) return false;
+        const int piv = bi;
         if (piv != col) {
-            for (int j = 0; j < m; ++j) {
-                double tmp = a[piv * lda + j];
+            for (int j = tid; j < m; j += nt) {
+                const double t = a[piv * lda + j];
                 a[piv * lda + j] = a[col * lda + j];
-                a[col * lda + j] = tmp;
+                a[col * lda + j] = t;
             }
-            double tmp = bvec[piv];
-            bvec[piv] = bvec[col];
-            bvec[col] = tmp;
+            if (tid == 0) {
+                const double t = bvec[piv];
+                bvec[piv] = bvec[col];
+                bvec[col] = t;
+            }
+            __syncthreads();
         }
         const double dg = a[col * lda + col];
-        for (int j = col; j < m; ++j) a[col * lda + j] /= dg;
-        bvec[col] /= dg;
-        for (int r = 0; r < m; ++r) {
+        __syncthreads();
+        for (int j = col + tid; j < m; j += nt) a[col * lda + j] /= dg;
+        if (tid == 0) bvec[col] /= dg;
+        __syncthreads();
+        for (int r = tid; r < m; r += nt) {
             if (r == col) continue;
             const double f = a[r * lda + col];
             if (f == 0.0) continue;
             for (int j = col; j < m; ++j) a[r * lda + j] -= f * a[col * lda + j];
             bvec[r] -= f * bvec[col];
         }
+        __syncthreads();
     }
     return true;
 }
@@ -722,7 +758,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     __shared__ double lds[16];
     __shared__ double sh_beta[32 * 33];
     __shared__ double sh_c[32];
-    __shared__ int sh_flag;
+    __shared__ int sh_ri[16];
     const int inst = act[blockIdx.x];
     SccState& s = st[inst];
     if (s.status != 0) return;
@@ -843,9 +879,8 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
                 }
             }
             __syncthreads();
-            if (threadIdx.x == 0) sh_flag = solve_gj(beta, cvec, m, bl) ? 1 : 0;
-            __syncthreads();
-            if (sh_flag) {
+            const bool solved = solve_gj_block(beta, cvec, m, bl, lds, sh_ri);
+            if (solved) {
                 // candidate = q + damp dq - sum_r omega_r c_r u_r ; then qlast/dqlast update
                 int bad = 0;
                 for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
