@@ -133,6 +133,8 @@ Batch::~Batch() {
         (void)hipFree(bk.dedcn);
     }
     for (void* p : allocs_) (void)hipFree(p);
+    if (ev_cn_) (void)hipEventDestroy(ev_cn_);
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -263,6 +265,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     prm_.omega_stride = round_up(prm_.broyden_slots, kRangeAlign);
 
     HIP_TRY(hipStreamCreate(&stream_));
+    HIP_TRY(hipEventCreateWithFlags(&ev_cn_, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
     // ---- device index tables
     auto up_i = [&](int** d, const std::vector<int>& h) -> Status {
         Status st = dalloc(d, h.size());
@@ -488,13 +492,15 @@ Status Batch::run() {
     tri_ms_ = tri_bytes_ = 0.0;
     tri_launches_ = 0;
     auto t0 = std::chrono::steady_clock::now();
+    setup_end_ = t0;
+    // the setup of the buckets overlaps the first SCC iterations of the buckets
+    // that finish it first; the phase boundary is the end of the last setup
     Status st = setup_phase();
     if (!st.ok()) return st;
-    HIP_TRY(hipDeviceSynchronize());
-    auto t1 = std::chrono::steady_clock::now();
     st = scc_phase();
     if (!st.ok()) return st;
     HIP_TRY(hipDeviceSynchronize());
+    auto t1 = setup_end_;
     auto t2 = std::chrono::steady_clock::now();
     st = gradient_phase();
     if (!st.ok()) return st;
@@ -525,60 +531,19 @@ Status Batch::set_guess(const GuessModel* model) {
     return Status{};
 }
 
+// Setup.  The coordination numbers come first (stream_); then every bucket, on
+// its own stream, builds S and H0 over its own atoms, factorises S = L L^T,
+// forms Cb = L^-T and A0 = Cb^T H0 Cb, marks instances whose overlap is not
+// positive definite, and copies those flags to the host behind its `done` event.
+// The geometry terms and the start charges run on stream_ meanwhile; each
+// bucket waits for them only before its first SCC iteration.  Buckets above
+// kCholMaxDim (single large systems) are set up first, one at a time, with
+// rocSOLVER.
 Status Batch::setup_phase() {
     const DeviceTables t = dm_->tables();
     const int na = natoms_;
-    if (na == 0) return Status{};
-    hipLaunchKernelGGL(k_coordination, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
-                       opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_, d_cnd3_, d_gw_,
-                       d_gw_ + (size_t)na * kMaxRef, na);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
-                       opt_.integral_cutoff);
-    LAUNCH_CHECK();
-    if (has_d_) {
-        hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(na), dim3(64), 0, stream_, db_, t, (const double*)d_cn_,
-                           opt_.integral_cutoff);
-        LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_pad_diagonal, dim3(ninst_), dim3(64), 0, stream_, db_);
-    LAUNCH_CHECK();
-    // geometry-only terms
-    hipLaunchKernelGGL(k_geometry_terms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
-                       (const double*)d_gw_, (const double*)d_cnd3_, opt_.enable_dispersion, d_erep_atom_,
-                       d_edisp_atom_, d_grep_, d_gdisp_, d_dedcn_disp_, na);
-    LAUNCH_CHECK();
-    if (opt_.enable_dispersion) {
-        hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
-                           (const double*)d_dedcn_disp_, 25.0, 1e-14, d_gdisp_, na);
-        LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_halogen, dim3((ninst_ + 63) / 64), dim3(64), 0, stream_, db_, t, d_ehal_, d_ghal_);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_erep_atom_,
-                       d_erep_);
-    LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_edisp_atom_,
-                       d_edisp_);
-    LAUNCH_CHECK();
-    if (guess_ || q0_dirty_) {
-        // the caller's start vector, then the learned one where the model applies
-        HIP_TRY(hipMemcpyAsync(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice, stream_));
-        q0_dirty_ = false;
-    }
-    if (guess_) {
-        const float *th, *nm;
-        Status gs = guess_->device_ptrs(&th, &nm);
-        if (!gs.ok()) return gs;
-        HIP_TRY(guess_launch_features(stream_, na, d_atom_z_, d_x_, d_atom_inst_, d_atom0_, d_gfeat_));
-        HIP_TRY(guess_launch_forward(stream_, na, th, nm, d_gfeat_, d_gout_));
-        HIP_TRY(guess_launch_apply(stream_, ninst_, d_guess_usable_, d_atom0_, d_atom_sh0_, d_charge_, d_gout_, d_q0_));
-        q0_dirty_ = true;
-    }
-    HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
-
-    // SCC state
+    for (auto& bk : buckets_) bk.in_setup = false;
+    // SCC state (nothing runs on the device yet)
     h_state_.assign(ninst_, SccState{});
     for (int i = 0; i < ninst_; ++i) {
         SccState& s = h_state_[i];
@@ -591,8 +556,14 @@ Status Batch::setup_phase() {
         s.nelec = h_nelec_[i];
     }
     HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+    if (na == 0) return Status{};
 
-    // orthonormal start basis Cb = L^-T from S = L L^T and A0 = Cb^T H0 Cb (per bucket)
+    hipLaunchKernelGGL(k_coordination, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                       opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_, d_cnd3_, d_gw_,
+                       d_gw_ + (size_t)na * kMaxRef, na);
+    LAUNCH_CHECK();
+    HIP_TRY(hipEventRecord(ev_cn_, stream_));
+
     const double one = 1.0, zero = 0.0;
     auto a0_gemms = [&](Bucket& bk, int count) -> Status {
         double* const* Cb = bk.ptrs + (size_t)A_CB * count;
@@ -608,36 +579,34 @@ Status Batch::setup_phase() {
                                      (double* const*)A0, n, count));
         return Status{};
     };
-    // Matrices up to kCholMaxDim: the engine's own factorisation kernels, every
-    // bucket on its own stream, all at once.
-    for (auto& bk : buckets_) {
-        if (bk.ld > kCholMaxDim) continue;
-        const int count = (int)bk.inst.size();
+    // S, H0 and the padding diagonal of a bucket's instances, on its stream
+    auto integrals = [&](Bucket& bk, int count) -> Status {
+        HIP_TRY(hipStreamWaitEvent(bk.stream, ev_cn_, 0));
         Status st = upload_active(bk, bk.inst, 0);
         if (!st.ok()) return st;
-        HIP_TRY(launch_cholesky_cb(bk.stream, count, (const int*)bk.act, (const int*)d_nao_, bk.ld,
-                                   bk.ptrs + (size_t)A_S * count, bk.ptrs + (size_t)A_L * count,
-                                   bk.ptrs + (size_t)A_CB * count, opt_.eigen_tolerance, bk.info));
-        st = a0_gemms(bk, count);
-        if (!st.ok()) return st;
-    }
-    HIP_TRY(hipDeviceSynchronize());
-    for (auto& bk : buckets_) {
-        if (bk.ld > kCholMaxDim) continue;
-        const int count = (int)bk.inst.size();
-        std::vector<int> info(count);
-        HIP_TRY(hipMemcpy(info.data(), bk.info, count * sizeof(int), hipMemcpyDeviceToHost));
-        for (int k = 0; k < count; ++k)
-            if (info[k] != 0 && !h_err_[bk.inst[k]]) {
-                h_err_[bk.inst[k]] = kErrInvalidInput;
-                h_errmsg_[bk.inst[k]] = "overlap matrix is not positive definite";
+        if (bk.natoms > 0) {
+            hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                               (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+            LAUNCH_CHECK();
+            if (has_d_) {
+                hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                                   (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+                LAUNCH_CHECK();
             }
-    }
-    // Larger matrices (single large systems): rocSOLVER, one bucket at a time.
+        }
+        hipLaunchKernelGGL(k_pad_diagonal, dim3(count), dim3(64), 0, bk.stream, db_, (const int*)bk.act);
+        LAUNCH_CHECK();
+        return Status{};
+    };
+
+    // Larger matrices (single large systems): rocSOLVER, one bucket at a time,
+    // with nothing else on the device.
+    bool large = false;
     for (auto& bk : buckets_) {
         if (bk.ld <= kCholMaxDim) continue;
+        large = true;
         const int count = (int)bk.inst.size();
-        Status st = upload_active(bk, bk.inst, 0);
+        Status st = integrals(bk, count);
         if (!st.ok()) return st;
         const size_t mat = (size_t)bk.ld * bk.ld;
         // A <- S
@@ -672,21 +641,87 @@ Status Batch::setup_phase() {
         // after another.
         HIP_TRY(hipStreamSynchronize(bk.stream));
     }
-    HIP_TRY(hipDeviceSynchronize());
-    std::vector<int> fail(ninst_);
-    HIP_TRY(hipMemcpy(fail.data(), d_fail_, ninst_ * sizeof(int), hipMemcpyDeviceToHost));
-    bool changed = false;
-    for (int i = 0; i < ninst_; ++i)
-        if ((fail[i] || h_err_[i]) && h_state_[i].status == 0) {
-            if (!h_err_[i]) {
-                h_err_[i] = kErrInvalidInput;
-                h_errmsg_[i] = "overlap matrix is not positive definite";
+    if (large) {
+        HIP_TRY(hipDeviceSynchronize());
+        std::vector<int> fail(ninst_);
+        HIP_TRY(hipMemcpy(fail.data(), d_fail_, ninst_ * sizeof(int), hipMemcpyDeviceToHost));
+        bool changed = false;
+        for (int i = 0; i < ninst_; ++i)
+            if ((fail[i] || h_err_[i]) && h_state_[i].status == 0) {
+                if (!h_err_[i]) {
+                    h_err_[i] = kErrInvalidInput;
+                    h_errmsg_[i] = "overlap matrix is not positive definite";
+                }
+                h_state_[i].status = 2;
+                changed = true;
             }
-            h_state_[i].status = 2;
-            changed = true;
+        if (changed)
+            HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+    }
+
+    // Matrices up to kCholMaxDim: every bucket on its own stream, all at once,
+    // largest first.
+    for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
+        Bucket& bk = *bit;
+        if (bk.ld > kCholMaxDim) continue;
+        const int count = (int)bk.inst.size();
+        Status st = integrals(bk, count);
+        if (!st.ok()) return st;
+        HIP_TRY(launch_cholesky_cb(bk.stream, count, (const int*)bk.act, (const int*)d_nao_, bk.ld,
+                                   bk.ptrs + (size_t)A_S * count, bk.ptrs + (size_t)A_L * count,
+                                   bk.ptrs + (size_t)A_CB * count, opt_.eigen_tolerance, bk.info));
+        hipLaunchKernelGGL(k_mark_failed, dim3((count + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, count,
+                           (const int*)bk.info, d_state_);
+        LAUNCH_CHECK();
+        st = a0_gemms(bk, count);
+        if (!st.ok()) return st;
+        hipLaunchKernelGGL(k_copy_words, dim3((count + 255) / 256), dim3(256), 0, bk.stream,
+                           (const unsigned int*)bk.info, (unsigned int*)bk.h_aux, count);
+        LAUNCH_CHECK();
+        bk.in_setup = true;
+    }
+
+    // geometry-only terms and the SCC start charges (stream_)
+    hipLaunchKernelGGL(k_geometry_terms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                       (const double*)d_gw_, (const double*)d_cnd3_, opt_.enable_dispersion, d_erep_atom_,
+                       d_edisp_atom_, d_grep_, d_gdisp_, d_dedcn_disp_, na);
+    LAUNCH_CHECK();
+    if (opt_.enable_dispersion) {
+        hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
+                           (const double*)d_dedcn_disp_, 25.0, 1e-14, d_gdisp_, na);
+        LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_halogen, dim3((ninst_ + 63) / 64), dim3(64), 0, stream_, db_, t, d_ehal_, d_ghal_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_erep_atom_,
+                       d_erep_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_instance_sum, dim3(ninst_), dim3(64), 0, stream_, db_, (const double*)d_edisp_atom_,
+                       d_edisp_);
+    LAUNCH_CHECK();
+    if (guess_ || q0_dirty_) {
+        // the caller's start vector, then the learned one where the model applies
+        HIP_TRY(hipMemcpyAsync(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice, stream_));
+        q0_dirty_ = false;
+    }
+    if (guess_) {
+        const float *th, *nm;
+        Status gs = guess_->device_ptrs(&th, &nm);
+        if (!gs.ok()) return gs;
+        HIP_TRY(guess_launch_features(stream_, na, d_atom_z_, d_x_, d_atom_inst_, d_atom0_, d_gfeat_));
+        HIP_TRY(guess_launch_forward(stream_, na, th, nm, d_gfeat_, d_gout_));
+        HIP_TRY(guess_launch_apply(stream_, ninst_, d_guess_usable_, d_atom0_, d_atom_sh0_, d_charge_, d_gout_, d_q0_));
+        q0_dirty_ = true;
+    }
+    HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    HIP_TRY(hipEventRecord(ev_start_, stream_));
+    for (auto& bk : buckets_) {
+        HIP_TRY(hipStreamWaitEvent(bk.stream, ev_start_, 0));
+        if (bk.in_setup) {
+            HIP_TRY(hipEventRecord(bk.done, bk.stream));
+            bk.inflight = true;
         }
-    if (changed)
-        HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+    }
     return Status{};
 }
 
@@ -700,18 +735,23 @@ Status Batch::setup_phase() {
 Status Batch::scc_phase() {
     for (auto& bk : buckets_) {
         bk.active.clear();
+        bk.iter = 0;
+        if (bk.in_setup) continue;  // its setup is still in flight
         for (int i : bk.inst)
             if (h_state_[i].status == 0) bk.active.push_back(i);
         bk.inflight = false;
-        bk.iter = 0;
     }
     progress_ = std::getenv("GFN1_PROGRESS") != nullptr;
     t_scc_ = std::chrono::steady_clock::now();
     // largest matrices first: their chains are the longest, the small buckets fill in
     for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
+        if (bit->in_setup) continue;
         Status st = bit->active.empty() ? launch_bucket_gradient(*bit) : launch_iteration(*bit);
         if (!st.ok()) return st;
     }
+    int setups = 0;
+    for (auto& bk : buckets_) setups += bk.in_setup ? 1 : 0;
+    if (!setups) setup_end_ = std::chrono::steady_clock::now();
     while (true) {
         bool any = false, progressed = false;
         for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
@@ -723,8 +763,27 @@ Status Batch::scc_phase() {
             HIP_TRY(q);
             bk.inflight = false;
             progressed = true;
-            Status st = finish_iteration(bk);
-            if (!st.ok()) return st;
+            Status st;
+            if (bk.in_setup) {
+                // overlap matrices that are not positive definite (flags copied by the setup)
+                bk.in_setup = false;
+                const int* flag = (const int*)bk.h_aux;
+                for (size_t k = 0; k < bk.inst.size(); ++k) {
+                    const int i = bk.inst[k];
+                    if (flag[k] != 0 && h_state_[i].status == 0) {
+                        h_state_[i].status = 2;
+                        if (!h_err_[i]) {
+                            h_err_[i] = kErrInvalidInput;
+                            h_errmsg_[i] = "overlap matrix is not positive definite";
+                        }
+                    }
+                    if (h_state_[i].status == 0) bk.active.push_back(i);
+                }
+                if (--setups == 0) setup_end_ = std::chrono::steady_clock::now();
+            } else {
+                st = finish_iteration(bk);
+                if (!st.ok()) return st;
+            }
             st = bk.active.empty() ? launch_bucket_gradient(bk) : launch_iteration(bk);
             if (!st.ok()) return st;
         }

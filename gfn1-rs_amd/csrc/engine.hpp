@@ -138,6 +138,7 @@ class Batch {
         SccState* st_host = nullptr;
         hipEvent_t done = nullptr;
         bool inflight = false;
+        bool in_setup = false;  // `done` marks the end of the bucket's setup
         int iter = 0;  // SCC iterations of the current run
         // pinned staging of the pointer tables and the active list (the bucket's
         // stream is idle whenever they are rewritten)
@@ -210,7 +211,8 @@ class Batch {
     hipStream_t stream_ = nullptr;
     bool ran_ = false;
     bool progress_ = false;  // GFN1_PROGRESS: one stderr line per bucket iteration
-    std::chrono::steady_clock::time_point t_scc_{};
+    std::chrono::steady_clock::time_point t_scc_{}, setup_end_{};
+    hipEvent_t ev_cn_ = nullptr, ev_start_ = nullptr;  // coordination numbers; start charges and geometry terms
     bool has_d_ = false;  // any d shell in the batch
     int pad_multiple_ = kPadMultiple;
 

@@ -272,8 +272,8 @@ __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int
 // (S/H0 kernel: see the shell-class kernels below)
 
 // Diagonal of the padded rows/columns: S = 1, H0 = F = kPadEnergy.
-__global__ void k_pad_diagonal(DeviceBatch b) {
-    const int inst = blockIdx.x;
+__global__ void k_pad_diagonal(DeviceBatch b, const int* act) {
+    const int inst = act[blockIdx.x];
     const int n = b.nao[inst], ld = b.ld[inst];
     for (int i = n + threadIdx.x; i < ld; i += blockDim.x) {
         const size_t idx = (size_t)i * ld + i;
@@ -281,6 +281,11 @@ __global__ void k_pad_diagonal(DeviceBatch b) {
         ((gptr<double>)b.H0[inst])[idx] = kPadEnergy;
         ((gptr<double>)b.F[inst])[idx] = kPadEnergy;
     }
+}
+
+__global__ void k_mark_failed(const int* act, int n, const int* flag, SccState* st) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && flag[j] != 0 && st[act[j]].status == 0) st[act[j]].status = 2;
 }
 
 // Cb = (L^-1)^T from the batched triangular inverse of the Cholesky factor
@@ -1464,12 +1469,13 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
 }  // namespace
 
 // S and H0 (integrals.rs:133-242; hamiltonian.rs:91-134).  One 64-lane
-// work-group per atom A; for every shell of A and every angular-momentum class
+// work-group per listed atom A = atoms[blockIdx.x]; for every shell of A and every angular-momentum class
 // the lanes walk the instance's shells of that class on atoms B <= A, so a
 // wavefront always evaluates one (LA, LB) specialisation.
 template <bool DCLASS>
-__global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables t, const double* cn, double cutoff) {
-    const int a = blockIdx.x;
+__global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn,
+                                                     double cutoff) {
+    const int a = atoms[blockIdx.x];
     const int inst = b.atom_inst[a];
     const int ld = b.ld[inst];
     gptr<double> S = (gptr<double>)b.S[inst];
@@ -1587,8 +1593,8 @@ __global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n) 
     if (i < n) dst[i] = src[i];
 }
 
-template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const double*, double);
-template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const double*, double);
+template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const int*, const double*, double);
+template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const int*, const double*, double);
 template __global__ void k_grad_pairs_t<false, false>(DeviceBatch, DeviceTables, const int*, const double*,
                                                 const double*, int, int, double, double*, double*);
 template __global__ void k_grad_pairs_t<true, false>(DeviceBatch, DeviceTables, const int*, const double*,

@@ -25,8 +25,11 @@ struct SccParams {
 __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
                                double* gw, double* dgw, int natoms);
 template <bool DCLASS>
-__global__ void k_overlap_h0_t(DeviceBatch b, DeviceTables t, const double* cn, double cutoff);
-__global__ void k_pad_diagonal(DeviceBatch b);
+__global__ void k_overlap_h0_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn, double cutoff);
+// Padding diagonal of the listed instances (act[blockIdx.x]).
+__global__ void k_pad_diagonal(DeviceBatch b, const int* act);
+// st[act[j]].status = 2 where flag[j] != 0 and the instance is still iterating.
+__global__ void k_mark_failed(const int* act, int n, const int* flag, SccState* st);
 __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, double eig_tol, int* fail);
 __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
                                   double* vt);
