@@ -662,23 +662,9 @@ Status Batch::setup_phase() {
     // Matrices up to kCholMaxDim: every bucket on its own stream, all at once,
     // largest first.
     for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
-        Bucket& bk = *bit;
-        if (bk.ld > kCholMaxDim) continue;
-        const int count = (int)bk.inst.size();
-        Status st = integrals(bk, count);
+        if (bit->ld > kCholMaxDim) continue;
+        Status st = launch_bucket_setup(*bit);
         if (!st.ok()) return st;
-        HIP_TRY(launch_cholesky_cb(bk.stream, count, (const int*)bk.act, (const int*)d_nao_, bk.ld,
-                                   bk.ptrs + (size_t)A_S * count, bk.ptrs + (size_t)A_L * count,
-                                   bk.ptrs + (size_t)A_CB * count, opt_.eigen_tolerance, bk.info));
-        hipLaunchKernelGGL(k_mark_failed, dim3((count + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, count,
-                           (const int*)bk.info, d_state_);
-        LAUNCH_CHECK();
-        st = a0_gemms(bk, count);
-        if (!st.ok()) return st;
-        hipLaunchKernelGGL(k_copy_words, dim3((count + 255) / 256), dim3(256), 0, bk.stream,
-                           (const unsigned int*)bk.info, (unsigned int*)bk.h_aux, count);
-        LAUNCH_CHECK();
-        bk.in_setup = true;
     }
 
     // geometry-only terms and the SCC start charges (stream_)
@@ -722,6 +708,55 @@ Status Batch::setup_phase() {
             bk.inflight = true;
         }
     }
+    return Status{};
+}
+
+// Setup of one bucket of at most kCholMaxDim AO on its stream: S and H0 over
+// its atoms, S = L L^T and Cb = L^-T, failure flags (device state and a host
+// copy), A0 = Cb^T H0 Cb.  Its `done` event is recorded by the caller once the
+// start charges are ready.
+Status Batch::launch_bucket_setup(Bucket& bk) {
+    const DeviceTables t = dm_->tables();
+    const int count = (int)bk.inst.size();
+    const double one = 1.0, zero = 0.0;
+    HIP_TRY(hipStreamWaitEvent(bk.stream, ev_cn_, 0));
+    Status st = upload_active(bk, bk.inst, 0);
+    if (!st.ok()) return st;
+    if (bk.natoms > 0) {
+        hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                           (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+        LAUNCH_CHECK();
+        if (has_d_) {
+            hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                               (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+            LAUNCH_CHECK();
+        }
+    }
+    hipLaunchKernelGGL(k_pad_diagonal, dim3(count), dim3(64), 0, bk.stream, db_, (const int*)bk.act);
+    LAUNCH_CHECK();
+    HIP_TRY(launch_cholesky_cb(bk.stream, count, (const int*)bk.act, (const int*)d_nao_, bk.ld,
+                               bk.ptrs + (size_t)A_S * count, bk.ptrs + (size_t)A_L * count,
+                               bk.ptrs + (size_t)A_CB * count, opt_.eigen_tolerance, bk.info));
+    hipLaunchKernelGGL(k_mark_failed, dim3((count + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, count,
+                       (const int*)bk.info, d_state_);
+    LAUNCH_CHECK();
+    {
+        double* const* Cb = bk.ptrs + (size_t)A_CB * count;
+        double* const* H0 = bk.ptrs + (size_t)A_H0 * count;
+        double* const* T = bk.ptrs + (size_t)A_T * count;
+        double* const* A0 = bk.ptrs + (size_t)A_A0 * count;
+        const int n = bk.ld;
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)H0, n, (const double* const*)Cb, n, &zero,
+                                     (double* const*)T, n, count));
+        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                                     (const double* const*)Cb, n, (const double* const*)T, n, &zero,
+                                     (double* const*)A0, n, count));
+    }
+    hipLaunchKernelGGL(k_copy_words, dim3((count + 255) / 256), dim3(256), 0, bk.stream, (const unsigned int*)bk.info,
+                       (unsigned int*)bk.h_aux, count);
+    LAUNCH_CHECK();
+    bk.in_setup = true;
     return Status{};
 }
 

@@ -159,6 +159,7 @@ class Batch {
     Status launch_iteration(Bucket& bk);
     Status finish_iteration(Bucket& bk);
     Status launch_bucket_gradient(Bucket& bk);
+    Status launch_bucket_setup(Bucket& bk);
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
     Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
