@@ -778,15 +778,15 @@ Status Batch::scc_phase() {
     }
     progress_ = std::getenv("GFN1_PROGRESS") != nullptr;
     t_scc_ = std::chrono::steady_clock::now();
+    int setups = 0;
+    for (auto& bk : buckets_) setups += bk.in_setup ? 1 : 0;
+    if (!setups) setup_end_ = std::chrono::steady_clock::now();
     // largest matrices first: their chains are the longest, the small buckets fill in
     for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
         if (bit->in_setup) continue;
         Status st = bit->active.empty() ? launch_bucket_gradient(*bit) : launch_iteration(*bit);
         if (!st.ok()) return st;
     }
-    int setups = 0;
-    for (auto& bk : buckets_) setups += bk.in_setup ? 1 : 0;
-    if (!setups) setup_end_ = std::chrono::steady_clock::now();
     while (true) {
         bool any = false, progressed = false;
         for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
