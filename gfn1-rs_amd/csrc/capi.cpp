@@ -13,6 +13,7 @@
 #include "../../include/gfn1_amd.h"
 #include "engine.hpp"
 #include "model.hpp"
+#include "optimizer.hpp"
 
 using namespace gfn1;
 
@@ -60,6 +61,11 @@ struct gfn1_gradient_result {
 
 struct gfn1_guess_model {
     GuessModel m;
+};
+
+struct gfn1_optimization {
+    std::vector<OptimizerResult> results;
+    OptimizerStats stats;
 };
 
 struct gfn1_batch {
@@ -455,6 +461,165 @@ void gfn1_batch_timings(const gfn1_batch* b, double out[4]) {
 }
 
 void gfn1_batch_free(gfn1_batch* b) { delete b; }
+
+int gfn1_batch_set_positions(gfn1_batch* b, const double* positions, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch) return fail(err, Status::error(kErrInvalidInput, "invalid batch"));
+    Status st = b->batch->set_positions(positions);
+    if (!st.ok()) return fail(err, st);
+    b->ran = false;  // the results belong to the old positions
+    b->downloaded = false;
+    return GFN1_OK;
+}
+
+void gfn1_optimization_options_default(gfn1_optimization_options* o) {
+    if (!o) return;
+    OptimizerOptions d;
+    o->max_iterations = d.max_iterations;
+    o->gradient_tolerance = d.gradient_tolerance;
+    o->step_tolerance = d.step_tolerance;
+    o->history = d.history;
+    o->initial_step = d.initial_step;
+    o->max_atom_step = d.max_atom_step;
+    o->warm_start = d.warm_start;
+    o->keep_trajectory = d.keep_trajectory;
+}
+
+int gfn1_batch_optimize(gfn1_batch* b, const gfn1_optimization_options* options, gfn1_optimization** out,
+                        gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch || !out) return fail(err, Status::error(kErrInvalidInput, "invalid optimisation arguments"));
+    if (b->learned_guess && !b->guess_set)
+        return fail(err, Status::error(kErrInvalidInput, "learned charge guess selected but no model set"));
+    gfn1_optimization_options def;
+    if (!options) {
+        gfn1_optimization_options_default(&def);
+        options = &def;
+    }
+    OptimizerOptions o;
+    o.max_iterations = options->max_iterations;
+    o.gradient_tolerance = options->gradient_tolerance;
+    o.step_tolerance = options->step_tolerance;
+    o.history = options->history;
+    o.initial_step = options->initial_step;
+    o.max_atom_step = options->max_atom_step;
+    o.warm_start = options->warm_start;
+    o.keep_trajectory = options->keep_trajectory;
+    auto r = std::make_unique<gfn1_optimization>();
+    b->ran = false;
+    b->downloaded = false;
+    Status st = optimize_batch(*b->batch, o, r->results, &r->stats);
+    if (!st.ok()) return fail(err, st);
+    *out = r.release();
+    return GFN1_OK;
+}
+
+int gfn1_optimize_geometry(const gfn1_system* system, const gfn1_params* params,
+                           const gfn1_gradient_options* gradient_options, const gfn1_optimization_options* options,
+                           gfn1_optimization** out, gfn1_error* err) {
+    clear(err);
+    if (!system || !out) return fail(err, Status::error(kErrInvalidInput, "invalid optimisation arguments"));
+    if (system->nat == 0) return fail(err, Status::error(kErrInvalidInput, "cannot optimize an empty system"));
+    gfn1_batch_options bo;
+    gfn1_batch_options_default(&bo);
+    if (gradient_options) bo.gradient = *gradient_options;
+    gfn1_batch* b = nullptr;
+    int rc = gfn1_batch_create(1, system, params, &bo, &b, err);
+    if (rc != GFN1_OK) return rc;
+    std::unique_ptr<gfn1_batch> hold(b);
+    gfn1_optimization* r = nullptr;
+    rc = gfn1_batch_optimize(b, options, &r, err);
+    if (rc != GFN1_OK) return rc;
+    const OptimizerResult& res = r->results[0];
+    if (res.status != kOk) {
+        const int code = res.status;
+        fail(err, Status::error(code, res.message));
+        gfn1_optimization_free(r);
+        return code;
+    }
+    *out = r;
+    return GFN1_OK;
+}
+
+namespace {
+const OptimizerResult* opt_result(const gfn1_optimization* r, int32_t inst) {
+    if (!r || inst < 0 || inst >= (int)r->results.size()) return nullptr;
+    return &r->results[inst];
+}
+}  // namespace
+
+int32_t gfn1_optimization_count(const gfn1_optimization* r) { return r ? (int32_t)r->results.size() : 0; }
+
+int gfn1_optimization_status(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x ? x->status : GFN1_ERR_INVALID_INPUT;
+}
+
+const char* gfn1_optimization_message(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x ? x->message.c_str() : "";
+}
+
+double gfn1_optimization_energy(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x && x->status == kOk ? x->energy : NAN;
+}
+
+double gfn1_optimization_max_gradient(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x && x->status == kOk ? x->max_gradient : NAN;
+}
+
+int32_t gfn1_optimization_iterations(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x ? x->iterations : -1;
+}
+
+int32_t gfn1_optimization_converged(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x && x->converged ? 1 : 0;
+}
+
+int gfn1_optimization_positions(const gfn1_optimization* r, int32_t inst, double* out) {
+    const OptimizerResult* x = opt_result(r, inst);
+    if (!x || !out) return GFN1_ERR_INVALID_INPUT;
+    std::copy(x->positions.begin(), x->positions.end(), out);
+    return x->status;
+}
+
+int gfn1_optimization_gradient(const gfn1_optimization* r, int32_t inst, double* out) {
+    const OptimizerResult* x = opt_result(r, inst);
+    if (!x || !out) return GFN1_ERR_INVALID_INPUT;
+    std::copy(x->gradient.begin(), x->gradient.end(), out);
+    return x->status;
+}
+
+int32_t gfn1_optimization_trajectory_length(const gfn1_optimization* r, int32_t inst) {
+    const OptimizerResult* x = opt_result(r, inst);
+    return x ? (int32_t)x->trajectory.size() : 0;
+}
+
+int gfn1_optimization_trajectory_step(const gfn1_optimization* r, int32_t inst, int32_t step, int32_t* iteration,
+                                      double* energy, double* max_gradient, double* step_norm, double* positions) {
+    const OptimizerResult* x = opt_result(r, inst);
+    if (!x || step < 0 || step >= (int32_t)x->trajectory.size()) return GFN1_ERR_INVALID_INPUT;
+    const OptimizerStep& s = x->trajectory[step];
+    if (iteration) *iteration = s.iteration;
+    if (energy) *energy = s.energy;
+    if (max_gradient) *max_gradient = s.max_gradient;
+    if (step_norm) *step_norm = s.step_norm;
+    if (positions) std::copy(s.positions.begin(), s.positions.end(), positions);
+    return GFN1_OK;
+}
+
+void gfn1_optimization_stats(const gfn1_optimization* r, int32_t* rounds, int64_t* evaluations, double* total_ms) {
+    if (!r) return;
+    if (rounds) *rounds = r->stats.rounds;
+    if (evaluations) *evaluations = r->stats.evaluations;
+    if (total_ms) *total_ms = r->stats.total_ms;
+}
+
+void gfn1_optimization_free(gfn1_optimization* r) { delete r; }
 
 }  // extern "C"
 

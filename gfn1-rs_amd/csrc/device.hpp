@@ -27,7 +27,7 @@ constexpr int kPadMultiple = 32;
 // concurrent streams, and the XCDs' L2 caches are only made coherent at kernel
 // boundaries, so no two instances may share a cache line.
 struct alignas(256) SccState {
-    int32_t status;        // 0 running, 1 converged, 2 failed
+    int32_t status;        // 0 running, 1 converged, 2 failed, 3 not selected for the run
     int32_t rung;          // robust-fallback ladder rung (electronic.rs:1413-1419)
     int32_t iter;          // iterations done on the current rung
     int32_t total_iter;

@@ -482,10 +482,29 @@ Status Batch::eigensolve(Bucket& bk, int count) {
     return Status{};
 }
 
-Status Batch::run() {
+Status Batch::run(const unsigned char* select) {
     // re-runnable: every run starts from the validated inputs
     h_err_ = h_err_input_;
     h_errmsg_ = h_errmsg_input_;
+    // the selected instances of each bucket and their atoms (the atom lists
+    // are rewritten only when the selection changes them)
+    h_sel_.assign(ninst_, 1);
+    if (select)
+        for (int i = 0; i < ninst_; ++i) h_sel_[i] = select[i] ? 1 : 0;
+    for (auto& bk : buckets_) {
+        bk.sel.clear();
+        for (int i : bk.inst)
+            if (h_sel_[i]) bk.sel.push_back(i);
+        const bool all = bk.sel.size() == bk.inst.size();
+        if (all && bk.atoms_all) continue;
+        std::vector<int> atoms;
+        for (int i : bk.sel)
+            for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
+        bk.natoms = (int)atoms.size();
+        if (!atoms.empty())
+            HIP_TRY(hipMemcpy(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
+        bk.atoms_all = all;
+    }
     HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)std::max(natoms_, 1) * sizeof(double)));
     HIP_TRY(hipMemset(d_fail_, 0, (size_t)std::max(ninst_, 1) * sizeof(int)));
     ran_ = true;
@@ -531,6 +550,74 @@ Status Batch::set_guess(const GuessModel* model) {
     return Status{};
 }
 
+Status Batch::set_positions(const double* x) {
+    if (!x && natoms_ > 0) return Status::error(kErrInvalidInput, "null positions");
+    for (size_t k = 0; k < h_x_.size(); ++k)
+        if (!std::isfinite(x[k])) return Status::error(kErrInvalidInput, "non-finite coordinate");
+    HIP_TRY(hipDeviceSynchronize());
+    std::copy(x, x + h_x_.size(), h_x_.begin());
+    if (!h_x_.empty())
+        HIP_TRY(hipMemcpy(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
+    // finite coordinates clear the input errors that non-finite ones raised
+    for (int i = 0; i < ninst_; ++i)
+        if (h_err_input_[i] == kErrInvalidInput && h_errmsg_input_[i] == "non-finite coordinate") {
+            h_err_input_[i] = kOk;
+            h_errmsg_input_[i].clear();
+        }
+    return Status{};
+}
+
+Status Batch::set_start_charges(int inst, const double* q) {
+    if (inst < 0 || inst >= ninst_ || (!q && h_nsh_[inst] > 0))
+        return Status::error(kErrInvalidInput, "invalid start charges");
+    std::copy(q, q + h_nsh_[inst], h_q0_.begin() + h_sh0_[inst]);
+    q0_dirty_ = true;
+    return Status{};
+}
+
+Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double>& energy,
+                                    std::vector<double>& gradient, std::vector<double>* shell_charges) {
+    const int na = natoms_;
+    std::vector<double> gel(3 * na), grep(3 * na), gdisp(3 * na), ghal(3 * na), erep(ninst_), edisp(ninst_),
+        ehal(ninst_);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+    if (na > 0) {
+        HIP_TRY(hipMemcpy(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipMemcpy(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    if (shell_charges) {
+        shell_charges->assign(nshells_, 0.0);
+        if (nshells_ > 0)
+            HIP_TRY(hipMemcpy(shell_charges->data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    status.assign(ninst_, 0);
+    energy.assign(ninst_, 0.0);
+    gradient.assign(3 * (size_t)na, 0.0);
+    const bool el_on = opt_.compute_gradient && (opt_.include_hamiltonian || opt_.include_scc);
+    const bool rep_on = opt_.compute_gradient && opt_.include_repulsion;
+    const bool disp_on = opt_.compute_gradient && opt_.include_dispersion && opt_.enable_dispersion;
+    const bool hal_on = opt_.compute_gradient && opt_.include_halogen;
+    for (int i = 0; i < ninst_; ++i) {
+        const SccState& s = h_state_[i];
+        int st = h_err_[i] ? h_err_[i] : (s.status == 1 ? kOk : kErrSccNotConverged);
+        if (s.status == 3) st = kErrInvalidInput;
+        status[i] = st;
+        if (st != kOk) continue;
+        const double e_disp = opt_.enable_dispersion ? edisp[i] : 0.0;
+        energy[i] = s.e_el + s.e2 + s.e3 + erep[i] + e_disp + ehal[i] + s.ts;
+        for (int k = 3 * h_atom0_[i]; k < 3 * h_atom0_[i + 1]; ++k)
+            gradient[k] = (el_on ? gel[k] : 0.0) + (rep_on ? grep[k] : 0.0) + (disp_on ? gdisp[k] : 0.0) +
+                          (hal_on ? ghal[k] : 0.0);
+    }
+    return Status{};
+}
+
 // Setup.  The coordination numbers come first (stream_); then every bucket, on
 // its own stream, builds S and H0 over its own atoms, factorises S = L L^T,
 // forms Cb = L^-T and A0 = Cb^T H0 Cb, marks instances whose overlap is not
@@ -548,7 +635,7 @@ Status Batch::setup_phase() {
     for (int i = 0; i < ninst_; ++i) {
         SccState& s = h_state_[i];
         std::memset(&s, 0, sizeof(s));
-        s.status = h_err_[i] ? 2 : 0;
+        s.status = !h_sel_[i] ? 3 : (h_err_[i] ? 2 : 0);
         s.mixing = prm_.rung_mixing[0];
         s.level_shift = prm_.rung_level_shift[0];
         s.temperature = prm_.rung_temperature[0];
@@ -582,7 +669,7 @@ Status Batch::setup_phase() {
     // S, H0 and the padding diagonal of a bucket's instances, on its stream
     auto integrals = [&](Bucket& bk, int count) -> Status {
         HIP_TRY(hipStreamWaitEvent(bk.stream, ev_cn_, 0));
-        Status st = upload_active(bk, bk.inst, 0);
+        Status st = upload_active(bk, bk.sel, 0);
         if (!st.ok()) return st;
         if (bk.natoms > 0) {
             hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
@@ -603,30 +690,31 @@ Status Batch::setup_phase() {
     // with nothing else on the device.
     bool large = false;
     for (auto& bk : buckets_) {
-        if (bk.ld <= kCholMaxDim) continue;
+        if (bk.ld <= kCholMaxDim || bk.sel.empty()) continue;
         large = true;
-        const int count = (int)bk.inst.size();
+        const int count = (int)bk.sel.size();
         Status st = integrals(bk, count);
         if (!st.ok()) return st;
         const size_t mat = (size_t)bk.ld * bk.ld;
+        const size_t all = bk.inst.size();  // the arena roles hold every instance of the bucket
         // A <- S
-        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_A * count * mat, bk.arena + (size_t)R_S * count * mat,
-                               count * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
+        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_A * all * mat, bk.arena + (size_t)R_S * all * mat,
+                               all * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
         double* const* A = bk.ptrs + (size_t)A_A * count;
         RB_TRY(rocsolver_dpotrf_batched(bk.handle, rocblas_fill_lower, bk.ld, (double* const*)A, bk.ld, bk.info,
                                         count));
         // keep L (S = L L^T) for the reduced congruence of the SCC iterations
-        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_L * count * mat, bk.arena + (size_t)R_A * count * mat,
-                               count * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
+        HIP_TRY(hipMemcpyAsync(bk.arena + (size_t)R_L * all * mat, bk.arena + (size_t)R_A * all * mat,
+                               all * mat * sizeof(double), hipMemcpyDeviceToDevice, bk.stream));
         RB_TRY(rocsolver_dtrtri_batched(bk.handle, rocblas_fill_lower, rocblas_diagonal_non_unit, bk.ld,
                                         (double* const*)A, bk.ld, bk.info, count));
         std::vector<int> info(count);
         HIP_TRY(hipMemcpyAsync(info.data(), bk.info, count * sizeof(int), hipMemcpyDeviceToHost, bk.stream));
         HIP_TRY(hipStreamSynchronize(bk.stream));
         for (int k = 0; k < count; ++k)
-            if (info[k] != 0 && !h_err_[bk.inst[k]]) {
-                h_err_[bk.inst[k]] = kErrInvalidInput;
-                h_errmsg_[bk.inst[k]] = "overlap matrix is not positive definite";
+            if (info[k] != 0 && !h_err_[bk.sel[k]]) {
+                h_err_[bk.sel[k]] = kErrInvalidInput;
+                h_errmsg_[bk.sel[k]] = "overlap matrix is not positive definite";
             }
         hipLaunchKernelGGL(k_make_cb, dim3(64, count), dim3(256), 0, bk.stream, db_, A, (const int*)bk.act,
                            opt_.eigen_tolerance, d_fail_);
@@ -662,7 +750,7 @@ Status Batch::setup_phase() {
     // Matrices up to kCholMaxDim: every bucket on its own stream, all at once,
     // largest first.
     for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
-        if (bit->ld > kCholMaxDim) continue;
+        if (bit->ld > kCholMaxDim || bit->sel.empty()) continue;
         Status st = launch_bucket_setup(*bit);
         if (!st.ok()) return st;
     }
@@ -717,10 +805,10 @@ Status Batch::setup_phase() {
 // start charges are ready.
 Status Batch::launch_bucket_setup(Bucket& bk) {
     const DeviceTables t = dm_->tables();
-    const int count = (int)bk.inst.size();
+    const int count = (int)bk.sel.size();
     const double one = 1.0, zero = 0.0;
     HIP_TRY(hipStreamWaitEvent(bk.stream, ev_cn_, 0));
-    Status st = upload_active(bk, bk.inst, 0);
+    Status st = upload_active(bk, bk.sel, 0);
     if (!st.ok()) return st;
     if (bk.natoms > 0) {
         hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
@@ -803,8 +891,8 @@ Status Batch::scc_phase() {
                 // overlap matrices that are not positive definite (flags copied by the setup)
                 bk.in_setup = false;
                 const int* flag = (const int*)bk.h_aux;
-                for (size_t k = 0; k < bk.inst.size(); ++k) {
-                    const int i = bk.inst[k];
+                for (size_t k = 0; k < bk.sel.size(); ++k) {
+                    const int i = bk.sel[k];
                     if (flag[k] != 0 && h_state_[i].status == 0) {
                         h_state_[i].status = 2;
                         if (!h_err_[i]) {
@@ -1166,6 +1254,10 @@ Status Batch::download(std::vector<InstanceResult>& out) {
         r.rms = s.rms;
         r.status = h_err_[i] ? h_err_[i] : (s.status == 1 ? kOk : kErrSccNotConverged);
         r.message = h_errmsg_[i];
+        if (s.status == 3) {
+            r.status = kErrInvalidInput;
+            r.message = "instance not selected in the last run";
+        }
         if (r.status != kOk) continue;
         const double e_disp = opt_.enable_dispersion ? edisp[i] : 0.0;
         const double total_internal = s.e_el + s.e2 + s.e3 + erep[i] + e_disp + ehal[i];

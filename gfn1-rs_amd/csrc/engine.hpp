@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <chrono>
 #include <memory>
 #include <string>
@@ -92,8 +93,31 @@ class Batch {
     Batch& operator=(const Batch&) = delete;
 
     Status create(const DeviceModel& dm, const std::vector<SystemInput>& systems, const EngineOptions& opt);
-    Status run();
+    // select: nullptr runs every instance; otherwise only instances with a
+    // nonzero entry are set up, iterated and differentiated (the others keep
+    // status kSkipped in this run's results).
+    Status run(const unsigned char* select = nullptr);
     Status download(std::vector<InstanceResult>& out);
+    // New positions (bohr, atom-major xyz of every atom in instance order) for
+    // the next runs; every coordinate must be finite.
+    Status set_positions(const double* x);
+    // Start shell charges of one instance for the next runs (nsh values).
+    Status set_start_charges(int inst, const double* q);
+    // Total free energy, gradient (3 * natoms, instance order), shell charges
+    // (nullable; nshells of every instance, instance order) and per-instance
+    // status of the last run, without the full result download.
+    Status fetch_energy_gradient(std::vector<int>& status, std::vector<double>& energy, std::vector<double>& gradient,
+                                 std::vector<double>* shell_charges);
+    // Current positions of every atom (bohr, 3 * natoms()).
+    Status positions(double* x) const {
+        std::copy(h_x_.begin(), h_x_.end(), x);
+        return Status{};
+    }
+    int natoms() const { return natoms_; }
+    int atom_offset(int inst) const { return h_atom0_[inst]; }   // inst in [0, size()]
+    int shell_count(int inst) const { return h_nsh_[inst]; }
+    int shell_offset(int inst) const { return h_sh0_[inst]; }    // into fetch_energy_gradient's shell charges
+    const std::string& message(int inst) const { return h_errmsg_[inst]; }
     const double* timings() const { return timings_; }
     int size() const { return ninst_; }
     // Learned start charges for the next runs (nullptr: the caller's or zero start).
@@ -111,6 +135,8 @@ class Batch {
     struct Bucket {
         int ld = 0;
         std::vector<int> inst;            // instance ids
+        std::vector<int> sel;             // instances selected for the current run
+        bool atoms_all = true;            // `atoms` lists every atom of the bucket
         double* arena = nullptr;          // matrices of the bucket
         hipStream_t stream = nullptr;
         rocblas_handle handle = nullptr;
@@ -174,6 +200,7 @@ class Batch {
     std::vector<int> h_bucket_of_, h_final_parity_;
     std::vector<int> h_bucket_pos_;  // position of each instance in its bucket
     std::vector<int> h_err_, h_err_input_;
+    std::vector<unsigned char> h_sel_;    // instances selected for the current run
     std::vector<std::string> h_errmsg_, h_errmsg_input_;
     std::vector<Bucket> buckets_;
     std::vector<void*> allocs_;

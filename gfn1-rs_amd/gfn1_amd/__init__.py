@@ -7,6 +7,7 @@ Python surface of the C ABI in include/gfn1_amd.h:
   Gfn1NativeCalculator          python.rs:115-381
   BatchEngine                   batched energy + gradient of many systems
   GuessModel                    learned SCC start charges (GPU-trained MLP)
+  optimize_geometry(...)        optimizer.rs:78 (L-BFGS; BatchEngine.optimize for many systems)
 """
 from .api import (  # noqa: F401
     ANGSTROM_TO_BOHR,
@@ -18,11 +19,16 @@ from .api import (  # noqa: F401
     ElectronicResult,
     Gfn1NativeCalculator,
     Gfn1Parameters,
+    GeometryOptimizationOptions,
+    GeometryOptimizationResult,
+    GeometryOptimizationStep,
     GuessModel,
+    OptimizationResult,
     HARTREE_TO_EV,
     basis_counts,
     guess_features,
     analytic_gradient,
+    optimize_geometry,
     run_electronic,
 )
 from ._lib import NativeError  # noqa: F401

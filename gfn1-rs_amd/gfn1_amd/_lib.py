@@ -60,6 +60,13 @@ class BatchOptions(ctypes.Structure):
                 ("eigensolver", ctypes.c_int32), ("keep_matrices", ctypes.c_int32)]
 
 
+class OptimizationOptions(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int32), ("gradient_tolerance", ctypes.c_double),
+                ("step_tolerance", ctypes.c_double), ("history", ctypes.c_int32),
+                ("initial_step", ctypes.c_double), ("max_atom_step", ctypes.c_double),
+                ("warm_start", ctypes.c_int32), ("keep_trajectory", ctypes.c_int32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _VP = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -123,6 +130,27 @@ SIGNATURES = [
     ("gfn1_guess_loss_gradient", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP, _DP, _FP,
                                                 ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_set_guess", ctypes.c_int, [_VP, _VP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_batch_set_positions", ctypes.c_int, [_VP, _DP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_optimization_options_default", None, [ctypes.POINTER(OptimizationOptions)]),
+    ("gfn1_batch_optimize", ctypes.c_int, [_VP, ctypes.POINTER(OptimizationOptions), ctypes.POINTER(_VP),
+                                           ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_optimize_geometry", ctypes.c_int, [ctypes.POINTER(System), _VP, ctypes.POINTER(GradientOptions),
+                                              ctypes.POINTER(OptimizationOptions), ctypes.POINTER(_VP),
+                                              ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_optimization_count", ctypes.c_int32, [_VP]),
+    ("gfn1_optimization_status", ctypes.c_int, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_message", ctypes.c_char_p, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_energy", ctypes.c_double, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_max_gradient", ctypes.c_double, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_iterations", ctypes.c_int32, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_converged", ctypes.c_int32, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_positions", ctypes.c_int, [_VP, ctypes.c_int32, _DP]),
+    ("gfn1_optimization_gradient", ctypes.c_int, [_VP, ctypes.c_int32, _DP]),
+    ("gfn1_optimization_trajectory_length", ctypes.c_int32, [_VP, ctypes.c_int32]),
+    ("gfn1_optimization_trajectory_step", ctypes.c_int,
+     [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _DP, _DP, _DP, _DP]),
+    ("gfn1_optimization_stats", None, [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64), _DP]),
+    ("gfn1_optimization_free", None, [_VP]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

@@ -252,6 +252,160 @@ def analytic_gradient(numbers, positions_bohr, params: Optional[Gfn1Parameters] 
 
 
 @dataclass
+class GeometryOptimizationOptions:
+    """GeometryOptimizationOptions (optimizer.rs:18-45) with its defaults
+    (optimizer.rs:10-15).  warm_start and keep_trajectory have no reference
+    counterpart: warm_start=True starts each evaluation's SCC from the charges
+    of the last accepted geometry (the reference starts every evaluation
+    afresh); the trajectory is always kept by the reference."""
+    max_iterations: int = 250
+    gradient_tolerance: float = 1.0e-4
+    step_tolerance: float = 1.0e-7
+    history: int = 12
+    initial_step: float = 1.0
+    max_atom_step: float = 0.30
+    warm_start: bool = False
+    keep_trajectory: bool = True
+
+    def to_c(self):
+        o = _lib.OptimizationOptions()
+        o.max_iterations = int(self.max_iterations)
+        o.gradient_tolerance = float(self.gradient_tolerance)
+        o.step_tolerance = float(self.step_tolerance)
+        o.history = int(self.history)
+        o.initial_step = float(self.initial_step)
+        o.max_atom_step = float(self.max_atom_step)
+        o.warm_start = int(bool(self.warm_start))
+        o.keep_trajectory = int(bool(self.keep_trajectory))
+        return o
+
+
+@dataclass
+class GeometryOptimizationStep:
+    """GeometryOptimizationStep (optimizer.rs:47-55); positions in bohr."""
+    iteration: int
+    energy: float
+    max_gradient: float
+    step_norm: float
+    positions: np.ndarray
+
+
+@dataclass
+class GeometryOptimizationResult:
+    """GeometryOptimizationResult (optimizer.rs:57-67); positions in bohr,
+    gradient in Eh/bohr.  status/message: 0 and "" for a finished
+    optimisation, else the error optimize_geometry returned for the system."""
+    energy: float
+    positions: np.ndarray
+    gradient: np.ndarray
+    forces: np.ndarray
+    iterations: int
+    converged: bool
+    max_gradient: float
+    trajectory: list
+    status: int = 0
+    message: str = ""
+
+
+def _read_optimization(lib, h, i, nat):
+    status = lib.gfn1_optimization_status(h, i)
+    msg = (lib.gfn1_optimization_message(h, i) or b"").decode(errors="replace")
+    pos = np.zeros((nat, 3))
+    grad = np.zeros((nat, 3))
+    traj = []
+    if status == _lib.GFN1_OK:
+        lib.gfn1_optimization_positions(h, i, pos.ctypes.data_as(_lib._DP))
+        lib.gfn1_optimization_gradient(h, i, grad.ctypes.data_as(_lib._DP))
+        for k in range(lib.gfn1_optimization_trajectory_length(h, i)):
+            it, e, mg, sn = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            p = np.zeros((nat, 3))
+            lib.gfn1_optimization_trajectory_step(h, i, k, ctypes.byref(it), ctypes.byref(e), ctypes.byref(mg),
+                                                  ctypes.byref(sn), p.ctypes.data_as(_lib._DP))
+            traj.append(GeometryOptimizationStep(it.value, e.value, mg.value, sn.value, p))
+    return GeometryOptimizationResult(
+        energy=lib.gfn1_optimization_energy(h, i), positions=pos, gradient=grad, forces=-grad,
+        iterations=lib.gfn1_optimization_iterations(h, i), converged=bool(lib.gfn1_optimization_converged(h, i)),
+        max_gradient=lib.gfn1_optimization_max_gradient(h, i), trajectory=traj, status=status, message=msg)
+
+
+def optimize_geometry(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
+                      options: Optional[GeometryOptimizationOptions] = None,
+                      gradient_options: Optional[AnalyticGradientOptions] = None,
+                      charge: float = 0.0) -> GeometryOptimizationResult:
+    """optimize_geometry (optimizer.rs:78-185): L-BFGS relaxation of one
+    non-periodic system; positions in bohr.  Raises NativeError where the
+    reference returns Err."""
+    lib = _lib.load()
+    p = _params(params)
+    oc = (options or GeometryOptimizationOptions()).to_c()
+    gc, keep_q = (gradient_options or AnalyticGradientOptions()).to_c()
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    h = ctypes.c_void_p()
+    err = _lib.Gfn1Error()
+    rc = lib.gfn1_optimize_geometry(ctypes.byref(sysc), p._h, ctypes.byref(gc), ctypes.byref(oc), ctypes.byref(h),
+                                    ctypes.byref(err))
+    _lib.check(rc, err)
+    try:
+        return _read_optimization(lib, h, 0, len(np.asarray(numbers)))
+    finally:
+        lib.gfn1_optimization_free(h)
+
+
+# element symbols Z = 1..86 (the elements GFN1-xTB parametrises)
+_SYMBOLS = (
+    "H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
+    "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb "
+    "Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn").split()
+
+
+def _xyz_frame(numbers, positions_angstrom, comment):
+    """One XYZ frame (python.rs:3035-3045 layout)."""
+    lines = ["%d" % len(numbers), comment]
+    for z, p in zip(numbers, positions_angstrom):
+        sym = _SYMBOLS[int(z) - 1] if 1 <= int(z) <= len(_SYMBOLS) else "X"
+        lines.append("%-2s %18.10f %18.10f %18.10f" % (sym, p[0], p[1], p[2]))
+    return "\n".join(lines) + "\n"
+
+
+@dataclass
+class OptimizationResult:
+    """python.rs PyOptimizationResult (:2954-3078)."""
+    energy_hartree: float
+    energy_ev: float
+    positions_angstrom: np.ndarray
+    gradient_hartree_per_bohr: np.ndarray
+    forces_ev_per_angstrom: np.ndarray
+    iterations: int
+    converged: bool
+    max_gradient: float
+    numbers: np.ndarray
+    trajectory_positions_angstrom: list
+    trajectory_energies_hartree: list
+
+    @classmethod
+    def from_result(cls, numbers, r: GeometryOptimizationResult):
+        b2a = 1.0 / ANGSTROM_TO_BOHR
+        return cls(
+            energy_hartree=r.energy, energy_ev=r.energy * HARTREE_TO_EV, positions_angstrom=r.positions * b2a,
+            gradient_hartree_per_bohr=r.gradient, forces_ev_per_angstrom=r.forces * FORCE_AU_TO_EV_PER_ANGSTROM,
+            iterations=r.iterations, converged=r.converged, max_gradient=r.max_gradient,
+            numbers=np.asarray(numbers, dtype=np.int32),
+            trajectory_positions_angstrom=[s.positions * b2a for s in r.trajectory],
+            trajectory_energies_hartree=[s.energy for s in r.trajectory])
+
+    def to_xyz(self, comment=None):
+        """Final geometry as an XYZ string (Angstrom)."""
+        return _xyz_frame(self.numbers, self.positions_angstrom,
+                          comment if comment is not None else "gfn1 optimized geometry")
+
+    def trajectory_xyz(self):
+        """Every accepted step as a multi-frame XYZ string (Angstrom)."""
+        return "".join(_xyz_frame(self.numbers, f, "iter %d energy %.10f Ha" % (i, e))
+                       for i, (f, e) in enumerate(zip(self.trajectory_positions_angstrom,
+                                                      self.trajectory_energies_hartree)))
+
+
+@dataclass
 class CalculationResult:
     """python.rs CalculationResult (:2845-2935)."""
     energy_hartree: float
@@ -312,6 +466,35 @@ class Gfn1NativeCalculator:
         el = run_electronic(numbers, x, self.params, self.options, charge=self.charge)
         return CalculationResult(el.total_free, None, el.atomic_charges, el.dipole, el.iterations, el.converged,
                                  el.terms)
+
+
+    def optimize(self, numbers, positions, unit="angstrom", max_iterations=250, gradient_tolerance=1.0e-4,
+                 step_tolerance=1.0e-7, history=12, max_atom_step=0.30, trajectory_path=None) -> OptimizationResult:
+        """Calculator optimize (python.rs:486-542), non-periodic systems.  The
+        trajectory file, when asked for, is written once the optimisation has
+        finished (one frame per accepted step, optimizer.rs:187-213 layout)."""
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * _unit_scale(unit)
+        opts = GeometryOptimizationOptions(max_iterations=max_iterations, gradient_tolerance=gradient_tolerance,
+                                           step_tolerance=step_tolerance, history=history,
+                                           max_atom_step=max_atom_step)
+        r = optimize_geometry(numbers, x, self.params, opts, AnalyticGradientOptions(electronic=self.options),
+                              charge=self.charge)
+        if trajectory_path is not None:
+            b2a = 1.0 / ANGSTROM_TO_BOHR
+            with open(trajectory_path, "w") as f:
+                for step in r.trajectory:
+                    f.write(_xyz_frame(numbers, step.positions * b2a, "iter %d energy %.10f Ha max_grad %.3e"
+                                       % (step.iteration, step.energy, step.max_gradient)))
+        return OptimizationResult.from_result(numbers, r)
+
+
+def _unit_scale(unit):
+    unit_l = unit.lower()
+    if unit_l in ("angstrom", "ang", "a"):
+        return ANGSTROM_TO_BOHR
+    if unit_l in ("bohr", "au", "a0"):
+        return 1.0
+    raise ValueError("unit must be 'angstrom' or 'bohr'")
 
 
 def basis_counts(numbers, params: Optional[Gfn1Parameters] = None):
@@ -522,6 +705,32 @@ class BatchEngine:
         out = np.zeros(nsh)
         self._lib.gfn1_batch_shell_charges(self._h, i, out.ctypes.data_as(_lib._DP))
         return out
+
+    def set_positions(self, positions_bohr):
+        """New positions (bohr) for the next runs: one (nat, 3) array per system."""
+        x = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 3)
+                                                 for p in positions_bohr]) if self.n else np.zeros((0, 3)))
+        if x.shape[0] != sum(self.natoms):
+            raise ValueError("positions do not match the batch's atom counts")
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_set_positions(self._h, x.ctypes.data_as(_lib._DP), ctypes.byref(err)), err)
+
+    def optimize(self, options: Optional["GeometryOptimizationOptions"] = None):
+        """optimize_geometry (optimizer.rs:78) of every system at once; one
+        GeometryOptimizationResult per system (status != 0: the error that
+        optimize_geometry would have returned for it)."""
+        oc = (options or GeometryOptimizationOptions()).to_c()
+        h = ctypes.c_void_p()
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_optimize(self._h, ctypes.byref(oc), ctypes.byref(h), ctypes.byref(err)), err)
+        try:
+            res = [_read_optimization(self._lib, h, i, self.natoms[i]) for i in range(self.n)]
+            rounds, evals, ms = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_double()
+            self._lib.gfn1_optimization_stats(h, ctypes.byref(rounds), ctypes.byref(evals), ctypes.byref(ms))
+            self.optimization_stats = {"rounds": rounds.value, "evaluations": evals.value, "total_ms": ms.value}
+            return res
+        finally:
+            self._lib.gfn1_optimization_free(h)
 
     def timings(self):
         t = (ctypes.c_double * 4)()

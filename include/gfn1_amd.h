@@ -264,6 +264,71 @@ int gfn1_guess_loss_gradient(const gfn1_guess_model* model, int32_t n_sys, const
  * before gfn1_batch_run. */
 int gfn1_batch_set_guess(gfn1_batch* batch, const gfn1_guess_model* model, gfn1_error* err);
 
+/* New positions (bohr, atom-major xyz; the atoms of every system in batch
+ * order) for the batch's next runs.  All coordinates must be finite.  No
+ * reference counterpart as a call: the reference builds a new system per
+ * geometry (src/optimizer.rs:316-322). */
+int gfn1_batch_set_positions(gfn1_batch* batch, const double* positions, gfn1_error* err);
+
+/* --- Geometry optimisation ---
+ * optimize_geometry (src/optimizer.rs:78-185): L-BFGS on the total free energy
+ * and the analytic gradient, Armijo backtracking line search, per-atom step
+ * cap.  Non-periodic systems only. */
+typedef struct gfn1_optimization_options {
+    int32_t max_iterations;      /* 250 */
+    double gradient_tolerance;   /* 1e-4, on the largest gradient component */
+    double step_tolerance;       /* 1e-7 bohr */
+    int32_t history;             /* 12 correction pairs */
+    double initial_step;         /* 1.0, first line-search step length */
+    double max_atom_step;        /* 0.30 bohr; <= 0 disables the cap */
+    int32_t warm_start;          /* 0: every evaluation starts its SCC as the
+                                    batch is configured (the reference's
+                                    behaviour); 1: from the converged shell
+                                    charges of the last accepted geometry */
+    int32_t keep_trajectory;     /* 1: record every accepted step */
+} gfn1_optimization_options;
+/* GeometryOptimizationOptions::default (src/optimizer.rs:32-44). */
+void gfn1_optimization_options_default(gfn1_optimization_options* options);
+
+typedef struct gfn1_optimization gfn1_optimization;
+
+/* optimize_geometry for every system of the batch at once, each from its
+ * current positions and with its own L-BFGS state; every round evaluates the
+ * pending geometries of all systems (start points or line-search trials) in
+ * one device pass over just those systems.  The batch's options (precision,
+ * charge guess, gradient terms) apply to every evaluation.  A system that
+ * fails (its start geometry does not evaluate, or a line search finds no
+ * downhill step in 24 halvings) gets the error optimize_geometry would have
+ * returned as its status; the others are unaffected.  Afterwards the batch
+ * holds the last evaluated geometries and no batch results (run it again
+ * for those). */
+int gfn1_batch_optimize(gfn1_batch* batch, const gfn1_optimization_options* options, gfn1_optimization** out,
+                        gfn1_error* err);
+/* optimize_geometry of one system (src/optimizer.rs:78-82): returns the
+ * system's error code when it fails, as the reference returns Err. */
+int gfn1_optimize_geometry(const gfn1_system* system, const gfn1_params* params,
+                           const gfn1_gradient_options* gradient_options, const gfn1_optimization_options* options,
+                           gfn1_optimization** out, gfn1_error* err);
+/* GeometryOptimizationResult fields (src/optimizer.rs:57-67) per system. */
+int32_t gfn1_optimization_count(const gfn1_optimization* r);
+int gfn1_optimization_status(const gfn1_optimization* r, int32_t inst);       /* gfn1_status */
+const char* gfn1_optimization_message(const gfn1_optimization* r, int32_t inst);
+double gfn1_optimization_energy(const gfn1_optimization* r, int32_t inst);     /* total_free */
+double gfn1_optimization_max_gradient(const gfn1_optimization* r, int32_t inst);
+int32_t gfn1_optimization_iterations(const gfn1_optimization* r, int32_t inst);
+int32_t gfn1_optimization_converged(const gfn1_optimization* r, int32_t inst);
+/* Copy nat*3 doubles: final positions (bohr) / gradient (Hartree/bohr). */
+int gfn1_optimization_positions(const gfn1_optimization* r, int32_t inst, double* out);
+int gfn1_optimization_gradient(const gfn1_optimization* r, int32_t inst, double* out);
+/* Trajectory (GeometryOptimizationStep, src/optimizer.rs:47-55): step 0 is the
+ * start geometry; positions (nullable) receives nat*3 doubles. */
+int32_t gfn1_optimization_trajectory_length(const gfn1_optimization* r, int32_t inst);
+int gfn1_optimization_trajectory_step(const gfn1_optimization* r, int32_t inst, int32_t step, int32_t* iteration,
+                                      double* energy, double* max_gradient, double* step_norm, double* positions);
+/* Batch runs, energy + gradient evaluations over all systems, and wall time. */
+void gfn1_optimization_stats(const gfn1_optimization* r, int32_t* rounds, int64_t* evaluations, double* total_ms);
+void gfn1_optimization_free(gfn1_optimization* r);
+
 /* Library version string. */
 const char* gfn1_version(void);
 
