@@ -166,6 +166,63 @@ def run_single(name, precisions):
     print(json.dumps(out))
 
 
+def run_optimize(args):
+    """Batched L-BFGS geometry optimisation (optimize_geometry of every
+    cluster of the batch at once, default options except max_iterations)
+    through the same engine: accepted L-BFGS steps per second over the whole
+    batch, on the plain FP64 path and on the learned-start + mixed-precision
+    path with warm-started SCCs.  The CPU baseline is the oracle's L-BFGS
+    restatement on the first clusters, one core.  Prints one JSON line (not
+    the batched headline metric)."""
+    from gfn1_amd import (AnalyticGradientOptions, BatchEngine, ElectronicOptions, GeometryOptimizationOptions,
+                          GuessModel, geometry)
+    systems = geometry.mixed_batch(args.batch, first_seed=1000)
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-7))
+    gopt = GeometryOptimizationOptions(max_iterations=args.opt_iterations, keep_trajectory=False)
+    model = GuessModel.load(args.model)
+    out = {"metric": "L-BFGS steps/s", "unit": "accepted steps/s", "higher_is_better": True, "n_gpus": 1,
+           "data": "synthetic",
+           "config": {"workload": "batched L-BFGS relaxation of mixed H2O/CH4/NH3 clusters, 2-40 molecules, "
+                                  "0.9 g/cm3, seeds 1000+i, SCC energy tolerance 1e-7, max_iterations %d"
+                                  % args.opt_iterations, "global_batch": len(systems)},
+           "paths": {}}
+    for label, kw, warm in (("plain", {}, False),
+                            ("learned+mixed+warm", {"guess": model, "precision": "mixed"}, True)):
+        eng = BatchEngine(systems, options=opts, **kw)
+        eng.optimize(GeometryOptimizationOptions(max_iterations=1, keep_trajectory=False))  # untimed warm-up
+        eng.set_positions([x for _, x in systems])
+        gopt.warm_start = warm
+        t = time.perf_counter()
+        res = eng.optimize(gopt)
+        dt = time.perf_counter() - t
+        st = eng.optimization_stats
+        ok = [r for r in res if r.status == 0]
+        steps = sum(r.iterations for r in ok)
+        out["paths"][label] = {
+            "steps_per_s": steps / dt, "seconds": dt, "accepted_steps": steps, "evaluations": st["evaluations"],
+            "evaluations_per_s": st["evaluations"] / dt, "rounds": st["rounds"],
+            "converged": sum(1 for r in ok if r.converged), "failed": len(res) - len(ok),
+            "mean_final_max_gradient": float(np.mean([r.max_gradient for r in ok])) if ok else None}
+        print("optimize %s: %.2f s, %d steps, %d evaluations in %d rounds, %d converged" % (
+            label, dt, steps, st["evaluations"], st["rounds"], out["paths"][label]["converged"]),
+            file=sys.stderr, flush=True)
+    head = out["paths"]["learned+mixed+warm"]
+    out["value"] = head["steps_per_s"]
+    if args.opt_cpu_sample > 0:
+        import optimizer_oracle
+        n = min(args.opt_cpu_sample, len(systems))
+        t = time.perf_counter()
+        steps = 0
+        for z, x in systems[:n]:
+            r = optimizer_oracle.optimize_gfn1(z, x, scc_options={"energy_tolerance": 1e-7},
+                                               max_iterations=args.opt_iterations)
+            steps += r["iterations"]
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": steps / dt, "unit": "accepted steps/s", "cores": 1, "kind": "port",
+                               "sample": "first %d clusters, oracle L-BFGS over the FP64 C++ oracle" % n}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,7 +237,10 @@ def main():
     ap.add_argument("--model", default=os.path.join(ROOT, "gfn1-rs_amd", "data", "guess_mlp.bin"))
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB read + 1 GiB write elementwise kernel (HBM counter calibration)")
-    ap.add_argument("--workload", choices=["batch"] + sorted(SINGLE_WORKLOADS), default="batch",
+    ap.add_argument("--opt-iterations", type=int, default=20, help="max_iterations of the optimize workload")
+    ap.add_argument("--opt-cpu-sample", type=int, default=1,
+                    help="clusters optimised by the host oracle for the optimize workload (0 = skip)")
+    ap.add_argument("--workload", choices=["batch", "optimize"] + sorted(SINGLE_WORKLOADS), default="batch",
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precisions run for a single-system workload (fp64 first)")
@@ -189,6 +249,12 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "optimize":
+        if args.gpus != 1:
+            print("bench.py measures one GPU (--gpus %d refused)" % args.gpus, file=sys.stderr)
+            sys.exit(2)
+        run_optimize(args)
         return
     if args.workload != "batch":
         if args.gpus != 1:

@@ -122,6 +122,15 @@ class Batch {
     int size() const { return ninst_; }
     // Learned start charges for the next runs (nullptr: the caller's or zero start).
     Status set_guess(const GuessModel* model);
+    const GuessModel* guess() const { return guess_; }
+    // Start charges of every shell (padded layout), to put them back later.
+    std::vector<double> start_charges() const { return h_q0_; }
+    void restore_start_charges(const std::vector<double>& q0) {
+        if (q0.size() == h_q0_.size() && q0 != h_q0_) {
+            h_q0_ = q0;
+            q0_dirty_ = true;
+        }
+    }
     // Device time of the tridiagonal reduction over the last run(): summed launch
     // durations (HIP events on each bucket's stream), launches, and algorithmic
     // bytes (one pass over the trailing lower triangle per column step).

@@ -170,6 +170,17 @@ Status optimize_batch(Batch& batch, const OptimizerOptions& opt, std::vector<Opt
         set_trial(i);
     };
 
+    // a warm start replaces the batch's start charges and start-charge model;
+    // the caller's are back in place when the optimisation ends
+    struct RestoreStart {
+        Batch& b;
+        const GuessModel* g;
+        std::vector<double> q0;
+        ~RestoreStart() {
+            (void)b.set_guess(g);
+            b.restore_start_charges(q0);
+        }
+    } restore{batch, batch.guess(), batch.start_charges()};
     std::vector<unsigned char> select(n, 0);
     std::vector<int> status;
     std::vector<double> energy, grad, qall;
