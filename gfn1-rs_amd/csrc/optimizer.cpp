@@ -70,7 +70,7 @@ std::vector<double> lbfgs_direction(const std::vector<double>& g, const std::vec
     return q;
 }
 
-// Largest per-atom displacement capped at max_atom_step (src/optimizer.rs:289-305).
+// Largest per-atom displacement capped at max_atom_step (src/optimizer.rs:290-305).
 // fmax ignores a NaN operand, as f64::max does.
 void limit_atom_step(std::vector<double>& d, double max_atom_step) {
     if (max_atom_step <= 0.0) return;
@@ -148,7 +148,7 @@ Status optimize_batch(Batch& batch, const OptimizerOptions& opt, std::vector<Opt
         out[i].status = s.last_err;
         out[i].message = s.last_msg;
     };
-    // start of the next iteration, or the end (src/optimizer.rs:118-130)
+    // start of the next iteration, or the end (src/optimizer.rs:117-130)
     auto next_iteration = [&](int i, bool converged) {
         System& s = sys[i];
         OptimizerResult& r = out[i];
@@ -239,7 +239,7 @@ Status optimize_batch(Batch& batch, const OptimizerOptions& opt, std::vector<Opt
                 for (size_t k = 0; k < yv.size(); ++k) yv[k] = g1[k] - s.g[k];
                 const double step_norm = std::sqrt(dot(sv, sv));
                 const double ys = dot(yv, sv);
-                // history update (src/optimizer.rs:136-148)
+                // history update (src/optimizer.rs:137-151)
                 if (ys > 1.0e-12 && all_finite(sv) && all_finite(yv)) {
                     if (s.s_hist.size() == history_len) {
                         s.s_hist.erase(s.s_hist.begin());
