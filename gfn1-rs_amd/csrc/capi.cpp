@@ -13,6 +13,7 @@
 #include "../../include/gfn1_amd.h"
 #include "engine.hpp"
 #include "model.hpp"
+#include "hessian.hpp"
 #include "optimizer.hpp"
 
 using namespace gfn1;
@@ -620,6 +621,98 @@ void gfn1_optimization_stats(const gfn1_optimization* r, int32_t* rounds, int64_
 }
 
 void gfn1_optimization_free(gfn1_optimization* r) { delete r; }
+
+void gfn1_hessian_options_default(gfn1_hessian_options* o) {
+    if (!o) return;
+    HessianOptions d;
+    o->step = d.step;
+    o->scc_energy_tolerance = d.scc_energy_tolerance;
+    o->scc_charge_tolerance = d.scc_charge_tolerance;
+    o->include_repulsion = d.include_repulsion;
+    o->include_electronic = d.include_electronic;
+    o->include_dispersion = d.include_dispersion;
+    o->include_halogen = d.include_halogen;
+    o->warm_start = d.warm_start;
+    o->max_batch = d.max_batch;
+}
+
+namespace {
+int hessian_common(int32_t n, const gfn1_system* systems, const gfn1_params* params,
+                   const gfn1_electronic_options* eo, const gfn1_hessian_options* options,
+                   std::vector<HessianResult>& res, gfn1_error* err) {
+    clear(err);
+    if (n < 0 || (n > 0 && !systems) || !params)
+        return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
+    gfn1_electronic_options edef;
+    if (!eo) {
+        gfn1_electronic_options_default(&edef);
+        eo = &edef;
+    }
+    gfn1_hessian_options hdef;
+    if (!options) {
+        gfn1_hessian_options_default(&hdef);
+        options = &hdef;
+    }
+    std::vector<SystemInput> sys;
+    for (int i = 0; i < n; ++i) {
+        Status st = validate(&systems[i], eo);
+        if (!st.ok()) return fail(err, st);
+        sys.push_back(to_input(&systems[i], eo));
+    }
+    HessianOptions o;
+    o.step = options->step;
+    o.scc_energy_tolerance = options->scc_energy_tolerance;
+    o.scc_charge_tolerance = options->scc_charge_tolerance;
+    o.include_repulsion = options->include_repulsion;
+    o.include_electronic = options->include_electronic;
+    o.include_dispersion = options->include_dispersion;
+    o.include_halogen = options->include_halogen;
+    o.warm_start = options->warm_start;
+    o.max_batch = options->max_batch;
+    Status st;
+    const DeviceModel* dm = const_cast<gfn1_params*>(params)->device_model(st);
+    if (!dm) return fail(err, st);
+    st = hessian_batch(*dm, sys, to_engine(*eo), o, res, nullptr);
+    if (!st.ok()) return fail(err, st);
+    return GFN1_OK;
+}
+}  // namespace
+
+int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
+                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry, gfn1_error* err) {
+    if (!system || !hessian) {
+        clear(err);
+        return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
+    }
+    std::vector<HessianResult> res;
+    int rc = hessian_common(1, system, params, electronic, options, res, err);
+    if (rc != GFN1_OK) return rc;
+    const HessianResult& r = res[0];
+    if (r.status != kOk) return fail(err, Status::error(r.status, r.message));
+    std::copy(r.hessian.begin(), r.hessian.end(), hessian);
+    if (max_asymmetry) *max_asymmetry = r.max_asymmetry;
+    return GFN1_OK;
+}
+
+int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params* params,
+                       const gfn1_electronic_options* electronic, const gfn1_hessian_options* options,
+                       double* hessians, int32_t* status, gfn1_error* err) {
+    if ((n > 0 && (!hessians || !status))) {
+        clear(err);
+        return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
+    }
+    std::vector<HessianResult> res;
+    int rc = hessian_common(n, systems, params, electronic, options, res, err);
+    if (rc != GFN1_OK) return rc;
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        const size_t len = (size_t)9 * systems[i].nat * systems[i].nat;
+        status[i] = res[i].status;
+        if (res[i].status == kOk) std::copy(res[i].hessian.begin(), res[i].hessian.end(), hessians + off);
+        off += len;
+    }
+    return GFN1_OK;
+}
 
 }  // extern "C"
 

@@ -53,6 +53,11 @@ constexpr double kWideSmearing = 1000.0;
 enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, A_H0, A_L, A_A0, kActRoles };
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
+// hipMemcpy that skips empty ranges (an empty host vector has no buffer, and a
+// null pointer is an invalid argument even for zero bytes)
+hipError_t copy_bytes(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    return bytes ? hipMemcpy(dst, src, bytes, kind) : hipSuccess;
+}
 // Each instance's ranges in the per-shell and per-AO arrays start on a 256-byte
 // boundary: the size buckets update their instances on concurrent streams, and
 // the XCDs' L2 caches are only made coherent at kernel boundaries, so no cache
@@ -271,7 +276,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     auto up_i = [&](int** d, const std::vector<int>& h) -> Status {
         Status st = dalloc(d, h.size());
         if (!st.ok()) return st;
-        HIP_TRY(hipMemcpy(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(copy_bytes(*d, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
         return Status{};
     };
     Status st;
@@ -292,13 +297,13 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = up_i(&d_nsh_, h_nsh_)).ok()) return st;
     if (!(st = dalloc(&d_fail_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_vec0_, ninst_ + 1)).ok()) return st;
-    HIP_TRY(hipMemcpy(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_vec0_, h_vec0_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     if (!(st = dalloc(&d_broff_, ninst_ + 1)).ok()) return st;
-    HIP_TRY(hipMemcpy(d_broff_, h_broff_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_broff_, h_broff_.data(), (ninst_ + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     if (!(st = dalloc(&d_x_, h_x_.size())).ok()) return st;
-    HIP_TRY(hipMemcpy(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!(st = dalloc(&d_q0_, nshells_)).ok()) return st;
-    HIP_TRY(hipMemcpy(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice));
     double** dd[] = {&d_cn_, &d_cnd3_, &d_qin_, &d_qout_, &d_vin_g_, &d_vin_t_, &d_vout_g_, &d_vout_t_,
                      &d_rowS_, &d_rowH_, &d_erep_atom_, &d_edisp_atom_, &d_erep_, &d_edisp_, &d_ehal_,
                      &d_grep_, &d_gdisp_, &d_ghal_, &d_gel_, &d_dedcn_disp_, &d_dedcn_h_, &d_brql_, &d_brdql_};
@@ -378,11 +383,11 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         const size_t na_alloc = std::max<size_t>(atoms.size(), 1);
         HIP_TRY(hipMalloc(&bk.atoms, sizeof(int) * na_alloc));
         if (!atoms.empty())
-            HIP_TRY(hipMemcpy(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(copy_bytes(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(hipMalloc(&bk.gpair, sizeof(double) * 3 * na_alloc));
         HIP_TRY(hipMalloc(&bk.dedcn, sizeof(double) * na_alloc));
     }
-    HIP_TRY(hipMemcpy(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_mat_, hmat.data(), hmat.size() * sizeof(double*), hipMemcpyHostToDevice));
 
     db_.ninst = ninst_;
     db_.atom0 = d_atom0_;
@@ -483,6 +488,9 @@ Status Batch::eigensolve(Bucket& bk, int count) {
 }
 
 Status Batch::run(const unsigned char* select) {
+    // an error that an earlier failed call left in this thread's HIP state must
+    // not be reported by the launch checks of this run
+    (void)hipGetLastError();
     // re-runnable: every run starts from the validated inputs
     h_err_ = h_err_input_;
     h_errmsg_ = h_errmsg_input_;
@@ -502,10 +510,10 @@ Status Batch::run(const unsigned char* select) {
             for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
         bk.natoms = (int)atoms.size();
         if (!atoms.empty())
-            HIP_TRY(hipMemcpy(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(copy_bytes(bk.atoms, atoms.data(), atoms.size() * sizeof(int), hipMemcpyHostToDevice));
         bk.atoms_all = all;
     }
-    HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)std::max(natoms_, 1) * sizeof(double)));
+    if (natoms_ > 0) HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)natoms_ * sizeof(double)));
     HIP_TRY(hipMemset(d_fail_, 0, (size_t)std::max(ninst_, 1) * sizeof(int)));
     ran_ = true;
     tri_ms_ = tri_bytes_ = 0.0;
@@ -543,8 +551,8 @@ Status Batch::set_guess(const GuessModel* model) {
         allocs_.push_back(d_gfeat_);
         HIP_TRY(hipMalloc(&d_gout_, (size_t)natoms_ * kGuessOut * sizeof(float)));
         allocs_.push_back(d_gout_);
-        HIP_TRY(hipMemcpy(d_charge_, h_charge_.data(), ninst_ * sizeof(double), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(d_guess_usable_, h_guess_usable_.data(), ninst_ * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(copy_bytes(d_charge_, h_charge_.data(), ninst_ * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(copy_bytes(d_guess_usable_, h_guess_usable_.data(), ninst_ * sizeof(int), hipMemcpyHostToDevice));
     }
     guess_ = model;
     return Status{};
@@ -557,7 +565,7 @@ Status Batch::set_positions(const double* x) {
     HIP_TRY(hipDeviceSynchronize());
     std::copy(x, x + h_x_.size(), h_x_.begin());
     if (!h_x_.empty())
-        HIP_TRY(hipMemcpy(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(copy_bytes(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
     // finite coordinates clear the input errors that non-finite ones raised
     for (int i = 0; i < ninst_; ++i)
         if (h_err_input_[i] == kErrInvalidInput && h_errmsg_input_[i] == "non-finite coordinate") {
@@ -581,20 +589,20 @@ Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double
     std::vector<double> gel(3 * na), grep(3 * na), gdisp(3 * na), ghal(3 * na), erep(ninst_), edisp(ninst_),
         ehal(ninst_);
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
     if (na > 0) {
-        HIP_TRY(hipMemcpy(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_bytes(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_bytes(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_bytes(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_bytes(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
     }
-    HIP_TRY(hipMemcpy(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
     if (shell_charges) {
         shell_charges->assign(nshells_, 0.0);
         if (nshells_ > 0)
-            HIP_TRY(hipMemcpy(shell_charges->data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_TRY(copy_bytes(shell_charges->data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
     }
     status.assign(ninst_, 0);
     energy.assign(ninst_, 0.0);
@@ -642,7 +650,7 @@ Status Batch::setup_phase() {
         s.kt = std::max(s.temperature, 0.0) * kBoltzmannHartree;
         s.nelec = h_nelec_[i];
     }
-    HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+    HIP_TRY(copy_bytes(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
     if (na == 0) return Status{};
 
     hipLaunchKernelGGL(k_coordination, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
@@ -732,7 +740,7 @@ Status Batch::setup_phase() {
     if (large) {
         HIP_TRY(hipDeviceSynchronize());
         std::vector<int> fail(ninst_);
-        HIP_TRY(hipMemcpy(fail.data(), d_fail_, ninst_ * sizeof(int), hipMemcpyDeviceToHost));
+        HIP_TRY(copy_bytes(fail.data(), d_fail_, ninst_ * sizeof(int), hipMemcpyDeviceToHost));
         bool changed = false;
         for (int i = 0; i < ninst_; ++i)
             if ((fail[i] || h_err_[i]) && h_state_[i].status == 0) {
@@ -744,7 +752,7 @@ Status Batch::setup_phase() {
                 changed = true;
             }
         if (changed)
-            HIP_TRY(hipMemcpy(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
+            HIP_TRY(copy_bytes(d_state_, h_state_.data(), ninst_ * sizeof(SccState), hipMemcpyHostToDevice));
     }
 
     // Matrices up to kCholMaxDim: every bucket on its own stream, all at once,
@@ -1225,21 +1233,21 @@ Status Batch::download(std::vector<InstanceResult>& out) {
     std::vector<double> cn(na), gel(3 * na), grep(3 * na), gdisp(3 * na), ghal(3 * na), qout(nshells_),
         vout(nshells_), erep(ninst_), edisp(ninst_), ehal(ninst_), eps(h_vec0_[ninst_]), occ(h_vec0_[ninst_]);
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(cn.data(), d_cn_, na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(qout.data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(vout.data(), d_vout_t_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(eps.data(), d_eps_, eps.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(occ.data(), d_occ_, occ.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(h_state_.data(), d_state_, ninst_ * sizeof(SccState), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(cn.data(), d_cn_, na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(gel.data(), d_gel_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(grep.data(), d_grep_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(gdisp.data(), d_gdisp_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(ghal.data(), d_ghal_, 3 * na * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(qout.data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(vout.data(), d_vout_t_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(erep.data(), d_erep_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(edisp.data(), d_edisp_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(ehal.data(), d_ehal_, ninst_ * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(eps.data(), d_eps_, eps.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(occ.data(), d_occ_, occ.size() * sizeof(double), hipMemcpyDeviceToHost));
     std::vector<double*> hmat((size_t)kRoles * ninst_);
-    HIP_TRY(hipMemcpy(hmat.data(), d_mat_, hmat.size() * sizeof(double*), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_bytes(hmat.data(), d_mat_, hmat.size() * sizeof(double*), hipMemcpyDeviceToHost));
     for (int i = 0; i < ninst_; ++i) {
         InstanceResult& r = out[i];
         const SccState& s = h_state_[i];
@@ -1304,7 +1312,7 @@ Status Batch::download(std::vector<InstanceResult>& out) {
         if (opt_.keep_matrices) {
             auto grab = [&](int role, std::vector<double>& dst) -> Status {
                 std::vector<double> full((size_t)ld * ld);
-                HIP_TRY(hipMemcpy(full.data(), hmat[(size_t)role * ninst_ + i], full.size() * sizeof(double),
+                HIP_TRY(copy_bytes(full.data(), hmat[(size_t)role * ninst_ + i], full.size() * sizeof(double),
                                   hipMemcpyDeviceToHost));
                 dst.assign((size_t)n * n, 0.0);
                 for (int c = 0; c < n; ++c)

@@ -8,6 +8,9 @@ Python surface of the C ABI in include/gfn1_amd.h:
   BatchEngine                   batched energy + gradient of many systems
   GuessModel                    learned SCC start charges (GPU-trained MLP)
   optimize_geometry(...)        optimizer.rs:78 (L-BFGS; BatchEngine.optimize for many systems)
+  hessian(...), hessian_batch   analytic_hessian(...).hessian, hessian/mod.rs:112 (central differences
+                                of the analytic gradient on the device)
+  vibrational_analysis(...)     vibrational.rs:37
 """
 from .api import (  # noqa: F401
     ANGSTROM_TO_BOHR,
@@ -23,12 +26,18 @@ from .api import (  # noqa: F401
     GeometryOptimizationResult,
     GeometryOptimizationStep,
     GuessModel,
+    HessianOptions,
+    HessianResult,
+    VibrationalModes,
     OptimizationResult,
     HARTREE_TO_EV,
     basis_counts,
     guess_features,
     analytic_gradient,
+    hessian,
+    hessian_batch,
     optimize_geometry,
+    vibrational_analysis,
     run_electronic,
 )
 from ._lib import NativeError  # noqa: F401

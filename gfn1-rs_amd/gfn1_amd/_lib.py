@@ -67,6 +67,13 @@ class OptimizationOptions(ctypes.Structure):
                 ("warm_start", ctypes.c_int32), ("keep_trajectory", ctypes.c_int32)]
 
 
+class HessianOptions(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_double), ("scc_energy_tolerance", ctypes.c_double),
+                ("scc_charge_tolerance", ctypes.c_double), ("include_repulsion", ctypes.c_int32),
+                ("include_electronic", ctypes.c_int32), ("include_dispersion", ctypes.c_int32),
+                ("include_halogen", ctypes.c_int32), ("warm_start", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _VP = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -151,6 +158,12 @@ SIGNATURES = [
      [_VP, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _DP, _DP, _DP, _DP]),
     ("gfn1_optimization_stats", None, [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64), _DP]),
     ("gfn1_optimization_free", None, [_VP]),
+    ("gfn1_hessian_options_default", None, [ctypes.POINTER(HessianOptions)]),
+    ("gfn1_hessian", ctypes.c_int, [ctypes.POINTER(System), _VP, ctypes.POINTER(ElectronicOptions),
+                                    ctypes.POINTER(HessianOptions), _DP, _DP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_hessian_batch", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _VP,
+                                          ctypes.POINTER(ElectronicOptions), ctypes.POINTER(HessianOptions), _DP,
+                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

@@ -351,6 +351,125 @@ def optimize_geometry(numbers, positions_bohr, params: Optional[Gfn1Parameters] 
         lib.gfn1_optimization_free(h)
 
 
+@dataclass
+class HessianOptions:
+    """Options of the Cartesian Hessian (central differences of the analytic
+    gradient, the construction the reference validates analytic_hessian
+    against, tests/hessian.rs:691-760).  include_electronic stands for the
+    reference's four electronic flags (hessian/mod.rs:28-37) together; the
+    SCC tolerances replace those of the electronic options."""
+    step: float = 1.0e-4
+    scc_energy_tolerance: float = 1.0e-11
+    scc_charge_tolerance: float = 1.0e-9
+    include_repulsion: bool = True
+    include_electronic: bool = True
+    include_dispersion: bool = True
+    include_halogen: bool = True
+    warm_start: bool = False
+    max_batch: int = 4096
+
+    def to_c(self):
+        o = _lib.HessianOptions()
+        o.step = float(self.step)
+        o.scc_energy_tolerance = float(self.scc_energy_tolerance)
+        o.scc_charge_tolerance = float(self.scc_charge_tolerance)
+        for name in ("include_repulsion", "include_electronic", "include_dispersion", "include_halogen",
+                     "warm_start"):
+            setattr(o, name, int(bool(getattr(self, name))))
+        o.max_batch = int(self.max_batch)
+        return o
+
+
+@dataclass
+class HessianResult:
+    """hessian: (3 nat, 3 nat) in Eh/bohr^2, index 3*atom + axis, symmetrised
+    (AnalyticHessianResult.hessian, hessian/mod.rs:55-65)."""
+    hessian: np.ndarray
+    max_asymmetry: float
+
+
+def hessian(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
+            electronic: Optional[ElectronicOptions] = None, options: Optional[HessianOptions] = None,
+            charge: float = 0.0) -> HessianResult:
+    """Cartesian Hessian of one system (replaces analytic_hessian(...).hessian,
+    hessian/mod.rs:112-134, at the reference's own 1e-6 Eh/bohr^2 acceptance)."""
+    lib = _lib.load()
+    p = _params(params)
+    eo, keep_q = (electronic or ElectronicOptions()).to_c()
+    ho = (options or HessianOptions()).to_c()
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    ndof = 3 * len(np.asarray(numbers))
+    out = np.zeros((ndof, ndof))
+    asym = ctypes.c_double()
+    err = _lib.Gfn1Error()
+    _lib.check(lib.gfn1_hessian(ctypes.byref(sysc), p._h, ctypes.byref(eo), ctypes.byref(ho),
+                                out.ctypes.data_as(_lib._DP), ctypes.byref(asym), ctypes.byref(err)), err)
+    return HessianResult(out, asym.value)
+
+
+def hessian_batch(systems, params: Optional[Gfn1Parameters] = None, electronic: Optional[ElectronicOptions] = None,
+                  options: Optional[HessianOptions] = None):
+    """Hessians of many systems ((numbers, positions_bohr[, charge]) tuples) in
+    shared device passes: a list of (3 nat, 3 nat) arrays, None where the
+    system failed, and the list of per-system statuses."""
+    lib = _lib.load()
+    p = _params(params)
+    eo, keep_q = (electronic or ElectronicOptions()).to_c()
+    ho = (options or HessianOptions()).to_c()
+    arr, keep = _system_array(systems)
+    sizes = [3 * len(np.asarray(s[0])) for s in systems]
+    flat = np.zeros(sum(k * k for k in sizes))
+    status = np.zeros(len(systems), dtype=np.int32)
+    err = _lib.Gfn1Error()
+    _lib.check(lib.gfn1_hessian_batch(len(systems), arr, p._h, ctypes.byref(eo), ctypes.byref(ho),
+                                      flat.ctypes.data_as(_lib._DP),
+                                      status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(err)), err)
+    out, off = [], 0
+    for k, st in zip(sizes, status):
+        out.append(flat[off:off + k * k].reshape(k, k).copy() if st == 0 else None)
+        off += k * k
+    return out, [int(v) for v in status]
+
+
+# Standard atomic weights (IUPAC conventional values; mass number of the
+# longest-lived isotope for Tc, Pm, Po, At, Rn), Z = 1..86, as the reference
+# tabulates them (data_tables.rs:40-128).
+ATOMIC_MASSES = (
+    1.008, 4.002602, 6.94, 9.0121831, 10.81, 12.011, 14.007, 15.999, 18.998403163, 20.1797,
+    22.98976928, 24.305, 26.9815385, 28.085, 30.973761998, 32.06, 35.45, 39.948, 39.0983, 40.078,
+    44.955908, 47.867, 50.9415, 51.9961, 54.938044, 55.845, 58.933194, 58.6934, 63.546, 65.38,
+    69.723, 72.630, 74.921595, 78.971, 79.904, 83.798, 85.4678, 87.62, 88.90584, 91.224,
+    92.90637, 95.95, 98.0, 101.07, 102.90550, 106.42, 107.8682, 112.414, 114.818, 118.710,
+    121.760, 127.60, 126.90447, 131.293, 132.90545196, 137.327, 138.90547, 140.116, 140.90766, 144.242,
+    145.0, 150.36, 151.964, 157.25, 158.92535, 162.500, 164.93033, 167.259, 168.93422, 173.045,
+    174.9668, 178.49, 180.94788, 183.84, 186.207, 190.23, 192.217, 195.084, 196.966569, 200.592,
+    204.38, 207.2, 208.98040, 209.0, 210.0, 222.0)
+WAVENUMBER_PER_SQRT_AU = 5140.4843  # cm^-1 per sqrt(Eh / (bohr^2 amu)), vibrational.rs:20
+
+
+@dataclass
+class VibrationalModes:
+    """VibrationalModes (vibrational.rs:22-32)."""
+    wavenumbers: np.ndarray  # cm^-1, ascending; negative = imaginary
+    eigenvalues: np.ndarray  # Eh / (bohr^2 amu)
+    modes: np.ndarray        # (3 nat, 3 nat), one Cartesian displacement per row
+
+
+def vibrational_analysis(hessian_matrix, numbers) -> VibrationalModes:
+    """Harmonic analysis of a Cartesian Hessian (vibrational.rs:37-84): mass-
+    weighted, symmetrised, diagonalised; wavenumber = sign(l) sqrt(|l|) *
+    5140.4843 cm^-1.  Host linear algebra (numpy): a 3 nat square problem."""
+    h = np.asarray(hessian_matrix, dtype=np.float64)
+    z = [int(v) for v in numbers]
+    mass = np.array([max(ATOMIC_MASSES[v - 1] if 1 <= v <= len(ATOMIC_MASSES) else 0.0, 1.0e-12)
+                     for v in z for _ in range(3)])
+    mw = h / np.sqrt(np.outer(mass, mass))
+    mw = 0.5 * (mw + mw.T)
+    lam, vec = np.linalg.eigh(mw)
+    w = np.sqrt(np.abs(lam)) * WAVENUMBER_PER_SQRT_AU
+    return VibrationalModes(np.where(lam < 0.0, -w, w), lam, (vec / np.sqrt(mass)[:, None]).T)
+
+
 # element symbols Z = 1..86 (the elements GFN1-xTB parametrises)
 _SYMBOLS = (
     "H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "

@@ -329,6 +329,39 @@ int gfn1_optimization_trajectory_step(const gfn1_optimization* r, int32_t inst, 
 void gfn1_optimization_stats(const gfn1_optimization* r, int32_t* rounds, int64_t* evaluations, double* total_ms);
 void gfn1_optimization_free(gfn1_optimization* r);
 
+/* --- Cartesian Hessian ---
+ * The reference computes analytic_hessian(...).hessian (src/hessian/mod.rs:112-134,
+ * AnalyticHessianResult :55-65) and validates it against central differences of
+ * the analytic gradient at tight SCC tolerances, within 1e-6 Eh/bohr^2
+ * (tests/hessian.rs:691-760).  This is that construction, built on the device:
+ * the 6 nat displaced geometries of every system run as instances of the batch
+ * engine (FP64).  The component breakdown of the analytic result is not
+ * available; the include flags select the differentiated gradient terms. */
+typedef struct gfn1_hessian_options {
+    double step;                  /* 1e-4 bohr */
+    double scc_energy_tolerance;  /* 1e-11, replaces the electronic options' value */
+    double scc_charge_tolerance;  /* 1e-9, likewise */
+    int32_t include_repulsion;    /* 1 */
+    int32_t include_electronic;   /* 1: the reference's include_fixed_scc, include_fixed_pulay,
+                                     include_fixed_cn_h0 and include_electronic together */
+    int32_t include_dispersion;   /* 1 */
+    int32_t include_halogen;      /* 1 */
+    int32_t warm_start;           /* 0; 1: displaced SCCs start from the undisplaced geometry's charges
+                                     (fewer SCC iterations; use SCC tolerances 1e-12 / 1e-10 with it) */
+    int32_t max_batch;            /* 4096 displaced geometries per device pass */
+} gfn1_hessian_options;
+void gfn1_hessian_options_default(gfn1_hessian_options* options);
+/* One system: hessian receives (3 nat)^2 doubles, row-major, index 3*atom + axis,
+ * symmetrised; max_asymmetry (nullable) the largest |H_ij - H_ji| before that. */
+int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
+                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry, gfn1_error* err);
+/* Many systems: hessians receives the (3 nat_i)^2 blocks one after another,
+ * status (n entries) each system's gfn1_status (its block is left untouched
+ * unless GFN1_OK). */
+int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params* params,
+                       const gfn1_electronic_options* electronic, const gfn1_hessian_options* options,
+                       double* hessians, int32_t* status, gfn1_error* err);
+
 /* Library version string. */
 const char* gfn1_version(void);
 

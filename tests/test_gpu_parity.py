@@ -106,6 +106,11 @@ def test_error_paths():
         analytic_gradient([1, 1], np.array([[0, 0, 0], [1.4, 0, 0]]), options=AnalyticGradientOptions(
             electronic=ElectronicOptions(max_scc=2, energy_tolerance=-1.0)))
     assert e.value.kind == "SccNotConverged"
+    # an empty system is invalid input, and the calls after it are unaffected
+    with pytest.raises(NativeError) as e:
+        analytic_gradient([], np.zeros((0, 3)))
+    assert e.value.kind == "InvalidInput"
+    assert np.isfinite(analytic_gradient([1, 1], np.array([[0, 0, 0], [1.4, 0, 0]])).total_energy)
 
 
 MIXED_E_TOL = 1e-6
