@@ -223,6 +223,42 @@ def run_optimize(args):
     print(json.dumps(out))
 
 
+def run_hessian(args):
+    """Cartesian Hessians of the first clusters of the batch (central
+    differences of the analytic gradient, step 1e-4 bohr, SCC tolerances
+    1e-11 / 1e-9): Hessians/s and displaced-geometry evaluations/s.  The CPU
+    baseline is the oracle's finite-difference Hessian of the smallest of
+    those clusters, one core.  Prints one JSON line (not the batched headline
+    metric)."""
+    from gfn1_amd import geometry, hessian, hessian_batch
+    systems = geometry.mixed_batch(args.hess_systems, first_seed=1000)
+    nat = [len(z) for z, _ in systems]
+    small = min(range(len(systems)), key=lambda i: nat[i])
+    hessian(*systems[small])  # untimed warm-up
+    t = time.perf_counter()
+    hs, status = hessian_batch(systems)
+    dt = time.perf_counter() - t
+    evals = sum(1 + 6 * n for n in nat)
+    out = {"metric": "Hessians/s", "unit": "Hessians/s", "higher_is_better": True, "n_gpus": 1, "data": "synthetic",
+           "value": len(systems) / dt, "seconds": dt, "failed": sum(1 for s in status if s != 0),
+           "gradient_evaluations": evals, "evaluations_per_s": evals / dt,
+           "config": {"workload": "Cartesian Hessians of mixed H2O/CH4/NH3 clusters (seeds 1000+i, 2-40 molecules), "
+                                  "central differences of the analytic gradient, step 1e-4 bohr, SCC tolerances "
+                                  "1e-11 / 1e-9, FP64", "systems": len(systems), "atoms_min": min(nat),
+                      "atoms_max": max(nat), "atoms_mean": float(np.mean(nat))}}
+    print("hessian: %d systems, %d evaluations in %.2f s" % (len(systems), evals, dt), file=sys.stderr, flush=True)
+    if args.opt_cpu_sample > 0:
+        import hessian_oracle
+        z, x = systems[small]
+        t = time.perf_counter()
+        hessian_oracle.fd_hessian(z, x)
+        dtc = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 1.0 / dtc, "unit": "Hessians/s", "cores": 1, "kind": "port",
+                               "sample": "smallest cluster (%d atoms), oracle finite-difference Hessian" % nat[small],
+                               "evaluations_per_s": (1 + 6 * nat[small]) / dtc}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +276,9 @@ def main():
     ap.add_argument("--opt-iterations", type=int, default=20, help="max_iterations of the optimize workload")
     ap.add_argument("--opt-cpu-sample", type=int, default=1,
                     help="clusters optimised by the host oracle for the optimize workload (0 = skip)")
-    ap.add_argument("--workload", choices=["batch", "optimize"] + sorted(SINGLE_WORKLOADS), default="batch",
+    ap.add_argument("--hess-systems", type=int, default=32, help="clusters of the hessian workload")
+    ap.add_argument("--workload", choices=["batch", "optimize", "hessian"] + sorted(SINGLE_WORKLOADS),
+                    default="batch",
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precisions run for a single-system workload (fp64 first)")
@@ -250,11 +288,11 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload == "optimize":
+    if args.workload in ("optimize", "hessian"):
         if args.gpus != 1:
             print("bench.py measures one GPU (--gpus %d refused)" % args.gpus, file=sys.stderr)
             sys.exit(2)
-        run_optimize(args)
+        (run_optimize if args.workload == "optimize" else run_hessian)(args)
         return
     if args.workload != "batch":
         if args.gpus != 1:
