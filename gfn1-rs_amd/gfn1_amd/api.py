@@ -607,6 +607,21 @@ class Gfn1NativeCalculator:
         return OptimizationResult.from_result(numbers, r)
 
 
+    def hessian(self, numbers, positions, unit="angstrom"):
+        """Calculator hessian (python.rs:1753-1774): the (3N, 3N) Cartesian
+        Hessian in Eh/bohr^2 as a numpy array (central differences of the
+        analytic gradient on the device)."""
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * _unit_scale(unit)
+        return hessian(numbers, x, self.params, self.options, charge=self.charge).hessian
+
+    def vibrational_frequencies(self, numbers, positions, unit="angstrom"):
+        """Calculator vibrational_frequencies (python.rs:1825-1846): dict with
+        wavenumbers (cm^-1, ascending, imaginary as negative) and modes (one
+        Cartesian displacement per row)."""
+        v = vibrational_analysis(self.hessian(numbers, positions, unit), numbers)
+        return {"wavenumbers": v.wavenumbers, "modes": v.modes}
+
+
 def _unit_scale(unit):
     unit_l = unit.lower()
     if unit_l in ("angstrom", "ang", "a"):

@@ -1,0 +1,120 @@
+"""ASE calculator over the engine (python/gfn1_rs/ase.py, GFN1RSCalculator).
+
+The ASE boundary: positions in Angstrom, energies in eV, forces in eV/Angstrom,
+dipoles in e*Angstrom, Hessians in eV/Angstrom^2, every factor taken from
+``ase.units`` (ase.py:38-57); the engine underneath speaks atomic units.
+Non-periodic systems: properties energy, forces, charges, dipole; the native
+L-BFGS (``optimize_native``), the Hessian and the harmonic frequencies.
+Importing this module needs ASE, as the reference's does.
+"""
+from typing import Any
+
+import numpy as np
+from ase.calculators.calculator import Calculator, all_changes
+from ase.units import Bohr, Hartree, invcm
+
+from .api import Gfn1NativeCalculator
+
+_ENERGY = Hartree
+_LENGTH = Bohr
+_FORCE = Hartree / Bohr
+_HESSIAN = Hartree / Bohr ** 2
+_DIPOLE = Bohr
+_WAVENUMBER = invcm
+
+
+class GFN1AMDCalculator(Calculator):
+    """GFN1RSCalculator (ase.py:115-560) for non-periodic systems."""
+
+    implemented_properties = ["energy", "forces", "charges", "dipole"]
+    default_parameters = {
+        "param_path": None, "charge": 0.0, "multiplicity": None, "max_scc": 250, "energy_tolerance": 1.0e-6,
+        "charge_tolerance": 2.0e-5, "mixing": 0.4, "scc_broyden": True, "scc_broyden_size": 250,
+        "electronic_temperature": 300.0, "enable_dispersion": True, "enable_cn_hamiltonian": True,
+        "level_shift": 0.0,
+    }
+
+    def __init__(self, restart=None, label=None, atoms=None, **kwargs: Any):
+        Calculator.__init__(self, restart=restart, label=label, atoms=atoms, **kwargs)
+        self._native = self._make_native_calculator()
+
+    def set(self, **kwargs: Any):
+        changed = Calculator.set(self, **kwargs)
+        if changed:
+            self._native = self._make_native_calculator()
+        return changed
+
+    def _make_native_calculator(self):
+        p = self.parameters
+        return Gfn1NativeCalculator(
+            param_path=p.param_path, charge=float(p.charge), multiplicity=p.multiplicity, max_scc=int(p.max_scc),
+            energy_tolerance=float(p.energy_tolerance), charge_tolerance=float(p.charge_tolerance),
+            mixing=float(p.mixing), scc_broyden=bool(p.scc_broyden), scc_broyden_size=int(p.scc_broyden_size),
+            electronic_temperature=float(p.electronic_temperature), enable_dispersion=bool(p.enable_dispersion),
+            enable_cn_hamiltonian=bool(p.enable_cn_hamiltonian), level_shift=float(p.level_shift))
+
+    @staticmethod
+    def _molecule(atoms):
+        if atoms is None:
+            raise RuntimeError("no ASE Atoms object supplied or attached")
+        if any(atoms.get_pbc()):
+            raise NotImplementedError("periodic systems are not supported by this engine")
+        return atoms.get_atomic_numbers().tolist(), np.asarray(atoms.get_positions(), dtype=float)
+
+    def calculate(self, atoms=None, properties=("energy",), system_changes=all_changes):
+        """Single point in ASE units (ase.py:358-434), plus the native
+        passthroughs native_energy_terms_hartree, native_dipole_au,
+        native_forces_hartree_per_bohr, native_converged, native_iterations."""
+        Calculator.calculate(self, atoms, properties, system_changes)
+        numbers, positions = self._molecule(self.atoms)
+        need_forces = "forces" in set(properties)
+        r = self._native.calculate(numbers, positions, unit="angstrom", compute_gradient=need_forces)
+        terms = dict(r.energy_terms_hartree())
+        self.results["energy"] = float(r.energy_hartree) * _ENERGY
+        self.results["charges"] = np.asarray(r.charges, dtype=float)
+        dipole_au = np.asarray(r.dipole, dtype=float)
+        self.results["dipole"] = dipole_au * _DIPOLE
+        self.results["native_dipole_au"] = dipole_au
+        self.results["native_energy_terms_hartree"] = terms
+        self.results["native_energy_terms_ev"] = {k: v * _ENERGY for k, v in terms.items()}
+        self.results["native_converged"] = bool(r.converged)
+        self.results["native_iterations"] = int(r.iterations)
+        if r.gradient_hartree_per_bohr is not None:
+            forces_au = -np.asarray(r.gradient_hartree_per_bohr, dtype=float)
+            self.results["native_forces_hartree_per_bohr"] = forces_au
+            self.results["forces"] = forces_au * _FORCE
+
+    def optimize_native(self, atoms=None, *, max_iterations=250, gradient_tolerance=1.0e-4, step_tolerance=1.0e-7,
+                        history=12, max_atom_step=0.30):
+        """The engine's L-BFGS (ase.py:436-489): thresholds in atomic units,
+        atoms updated in Angstrom; returns the native OptimizationResult."""
+        atoms = self.atoms if atoms is None else atoms
+        numbers, positions = self._molecule(atoms)
+        result = self._native.optimize(numbers, positions, unit="angstrom", max_iterations=int(max_iterations),
+                                       gradient_tolerance=float(gradient_tolerance),
+                                       step_tolerance=float(step_tolerance), history=int(history),
+                                       max_atom_step=float(max_atom_step))
+        atoms.set_positions(np.asarray(result.positions_angstrom, dtype=float))
+        self.reset()
+        return result
+
+    def get_dipole_au(self, atoms=None):
+        """Mulliken dipole in e*bohr (ase.py:571-580)."""
+        numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
+        return np.asarray(self._native.calculate(numbers, positions, unit="angstrom").dipole, dtype=float)
+
+    def get_hessian(self, atoms=None):
+        """(3N, 3N) Hessian in eV/Angstrom^2 (ase.py:641-671)."""
+        numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
+        return np.asarray(self._native.hessian(numbers, positions, unit="angstrom"), dtype=float) * _HESSIAN
+
+    def get_vibrational_frequencies(self, atoms=None):
+        """wavenumbers (cm^-1), energies_ev and modes (ase.py:673-687)."""
+        numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
+        out = self._native.vibrational_frequencies(numbers, positions, unit="angstrom")
+        out["energies_ev"] = np.asarray(out["wavenumbers"], dtype=float) * _WAVENUMBER
+        return out
+
+
+# the reference's class name, for code written against it
+GFN1RSCalculator = GFN1AMDCalculator

@@ -1,0 +1,137 @@
+"""The ASE calculator (gfn1_amd.ase) over the engine: unit conversions at the
+ASE boundary (python/gfn1_rs/ase.py:38-57, tests/python/test_ase_units.py) and
+the native optimiser / Hessian passthroughs.
+
+ASE is not installed in this environment.  When it is missing, a minimal
+stand-in for the two ASE modules the calculator imports is put in place: the
+Calculator base (parameters, results, reset) and ase.units with ASE's own
+CODATA 2014 values.  With ASE installed the real package is used.  The
+comparison with ASE proper is therefore unpinned here.
+"""
+import sys
+import types
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ASE_BOHR = 0.52917721067           # ase.units (CODATA 2014)
+ASE_HARTREE = 27.211386024367243
+ASE_INVCM = 1.0 / 8065.544005
+
+
+def _install_stand_in():
+    try:
+        import ase  # noqa: F401
+        return False
+    except ImportError:
+        pass
+    ase = types.ModuleType("ase")
+    calc_pkg = types.ModuleType("ase.calculators")
+    calc_mod = types.ModuleType("ase.calculators.calculator")
+    units = types.ModuleType("ase.units")
+    units.Bohr, units.Hartree, units.invcm = ASE_BOHR, ASE_HARTREE, ASE_INVCM
+
+    class Parameters(dict):
+        def __getattr__(self, k):
+            return self[k]
+
+    class Calculator:
+        default_parameters = {}
+
+        def __init__(self, restart=None, label=None, atoms=None, **kwargs):
+            self.parameters = Parameters(self.default_parameters)
+            self.parameters.update(kwargs)
+            self.atoms = atoms
+            self.results = {}
+
+        def set(self, **kwargs):
+            changed = {k: v for k, v in kwargs.items() if self.parameters.get(k) != v}
+            self.parameters.update(changed)
+            if changed:
+                self.results = {}
+            return changed
+
+        def calculate(self, atoms=None, properties=("energy",), system_changes=None):
+            if atoms is not None:
+                self.atoms = atoms
+
+        def reset(self):
+            self.results = {}
+
+    calc_mod.Calculator = Calculator
+    calc_mod.all_changes = ["positions", "numbers", "cell", "pbc"]
+    sys.modules.update({"ase": ase, "ase.calculators": calc_pkg, "ase.calculators.calculator": calc_mod,
+                        "ase.units": units})
+    return True
+
+
+class _Atoms:
+    """The few Atoms methods the calculator calls."""
+
+    def __init__(self, numbers, positions, pbc=False):
+        self.numbers = np.asarray(numbers)
+        self.positions = np.asarray(positions, dtype=float)
+        self.pbc = np.array([pbc] * 3)
+
+    def get_atomic_numbers(self):
+        return self.numbers
+
+    def get_positions(self):
+        return self.positions.copy()
+
+    def set_positions(self, p):
+        self.positions = np.asarray(p, dtype=float)
+
+    def get_pbc(self):
+        return self.pbc
+
+
+WATER = ([8, 1, 1], [[0.0, 0.0, 0.0], [0.757, 0.586, 0.0], [-0.757, 0.586, 0.0]])
+
+
+@pytest.fixture(scope="module")
+def ase_calc():
+    stand_in = _install_stand_in()
+    from gfn1_amd import ase as gase
+    yield gase, stand_in
+    if stand_in:
+        for k in ("ase", "ase.calculators", "ase.calculators.calculator", "ase.units"):
+            sys.modules.pop(k, None)
+
+
+def test_single_point_units(ase_calc):
+    gase, _ = ase_calc
+    from gfn1_amd import ANGSTROM_TO_BOHR, analytic_gradient
+    from ase.units import Bohr, Hartree
+    calc = gase.GFN1RSCalculator()
+    atoms = _Atoms(*WATER)
+    calc.calculate(atoms, properties=("energy", "forces"))
+    ref = analytic_gradient(WATER[0], np.asarray(WATER[1]) * ANGSTROM_TO_BOHR)
+    assert abs(calc.results["energy"] - ref.total_energy * Hartree) < 1e-9
+    assert np.allclose(calc.results["forces"], -ref.gradient * Hartree / Bohr, rtol=0, atol=1e-9)
+    assert np.allclose(calc.results["native_forces_hartree_per_bohr"], -ref.gradient, rtol=0, atol=1e-12)
+    assert np.allclose(calc.results["dipole"], ref.electronic_result.dipole * Bohr, rtol=0, atol=1e-12)
+    assert np.allclose(calc.results["charges"], ref.electronic_result.atomic_charges, rtol=0, atol=1e-12)
+    assert abs(sum(calc.results["charges"])) < 1e-8
+    with pytest.raises(NotImplementedError):
+        calc.calculate(_Atoms(*WATER, pbc=True))
+
+
+def test_optimize_hessian_frequencies(ase_calc):
+    gase, _ = ase_calc
+    from ase.units import Bohr, Hartree
+    calc = gase.GFN1RSCalculator()
+    atoms = _Atoms(*WATER)
+    e0 = calc._native.calculate(*WATER).energy_hartree
+    res = calc.optimize_native(atoms, gradient_tolerance=1e-6, max_iterations=200)
+    assert res.converged and res.energy_hartree < e0
+    assert np.allclose(atoms.get_positions(), res.positions_angstrom)
+    h = calc.get_hessian(atoms)
+    h_au = calc._native.hessian(atoms.get_atomic_numbers(), atoms.get_positions())
+    assert np.allclose(h, h_au * Hartree / Bohr ** 2, rtol=1e-14, atol=0)
+    out = calc.get_vibrational_frequencies(atoms)
+    w = np.asarray(out["wavenumbers"])
+    assert np.all(np.abs(w[:6]) < 30.0) and 1300.0 < w[6] < 1800.0 and 3000.0 < w[7] < 4000.0
+    assert np.allclose(out["energies_ev"], w * ASE_INVCM)
