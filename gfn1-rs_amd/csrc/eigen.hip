@@ -10,7 +10,8 @@
 //                     dlagtf/dlagts factorisation), one thread per eigenvector;
 //                     groups of nearly equal eigenvalues are re-orthogonalised and
 //                     resolved by a Rayleigh-Ritz step on the group
-//   k_backtransform   Z <- Q Z in compact-WY blocks, one lane per eigenvector
+//   k_backtransform   Z <- Q Z in compact-WY blocks of 16 reflectors on the FP64
+//                     matrix cores, one work-group per (matrix, block of eigenvectors)
 //
 // The SCC needs the occupied subspace (density matrix), the energies of all
 // orbitals (occupations, orbital-energy output) and, at the end, the
@@ -950,7 +951,10 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
 // fixed order.  The chunks of one matrix are mapped onto one XCD (dispatch order
 // is round-robin over the eight XCDs), so its reflectors are shared in that L2.
 typedef double dvec4 __attribute__((ext_vector_type(4)));
-constexpr int kBTCols = 32;                                // eigenvectors per work-group
+// eigenvectors per work-group: 64 (one work-group per CU by registers and LDS)
+// re-reads the reflectors half as often but measured no faster than 32
+constexpr int kBTCols = 32;
+constexpr int kBTCT = kBTCols / 16;                        // their 16-column tiles
 constexpr int kBTWaves = 8;                                // 512 threads
 constexpr int kBTTiles = kEigMaxDim / 16 / kBTWaves;      // row tiles per wavefront
 constexpr int kXcds = 8;
@@ -1021,7 +1025,7 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
     extern __shared__ double sm[];
     double* Vb = sm;                                  // staged reflectors, (tile-aligned rows) x kVbS
     double* part = Vb + (size_t)ld * kVbS;            // T W partials (2 tiles) of half the wavefronts
-    double* Wp = part + (kBTWaves / 2) * 2 * 4 * 64;  // -(T W), 16 x 32
+    double* Wp = part + (kBTWaves / 2) * kBTCT * 4 * 64;  // -(T W), 16 x kBTCols
     double* Tl = Wp + kBT * kBTCols;                  // T of the block, 16 x 16
     const int id = blockIdx.x, per = gridDim.x / kXcds;
     const int logical = (id % kXcds) * per + id / kXcds;
@@ -1035,11 +1039,11 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
     gptr<const double> A = (gptr<const double>)Aptr[l];
     gptr<double> Z = (gptr<double>)Ztptr[l];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, li = lane & 15, lg = lane >> 4;
-    dvec4 z[kBTTiles][2];
+    dvec4 z[kBTTiles][kBTCT];
 #pragma unroll
     for (int t = 0; t < kBTTiles; ++t)
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < kBTCT; ++ct) {
             const int col = c0 + 16 * ct + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1071,7 +1075,9 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
         // this wavefront's share of T W = T V^T Z: W over its row tiles (accumulator
         // layout, element e = reflectors 4e .. 4e + 3), then T W with that W as the B
         // operand of k-step e
-        dvec4 aw0 = {0.0, 0.0, 0.0, 0.0}, aw1 = aw0;
+        dvec4 aw[kBTCT];
+#pragma unroll
+        for (int ct = 0; ct < kBTCT; ++ct) aw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < kBTTiles; ++t) {
             const int row0 = 16 * (wid + kBTWaves * t);
@@ -1079,53 +1085,53 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const double va = Vb[(row0 + 4 * e + lg - base) * kVbS + li];
-                    aw0 = mfma16x16x4(va, z[t][0][e], aw0);
-                    aw1 = mfma16x16x4(va, z[t][1][e], aw1);
+#pragma unroll
+                    for (int ct = 0; ct < kBTCT; ++ct) aw[ct] = mfma16x16x4(va, z[t][ct][e], aw[ct]);
                 }
             }
         }
-        dvec4 tw0 = {0.0, 0.0, 0.0, 0.0}, tw1 = tw0;
+        dvec4 tw[kBTCT];
+#pragma unroll
+        for (int ct = 0; ct < kBTCT; ++ct) tw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const double ta = Tl[li * kBT + 4 * e + lg];
-            tw0 = mfma16x16x4(ta, aw0[e], tw0);
-            tw1 = mfma16x16x4(ta, aw1[e], tw1);
+#pragma unroll
+            for (int ct = 0; ct < kBTCT; ++ct) tw[ct] = mfma16x16x4(ta, aw[ct][e], tw[ct]);
         }
         // partials summed in two steps through half-size LDS: the upper wavefronts
         // deposit theirs, the lower ones add their own in place
         const int slot = wid % (kBTWaves / 2);
         if (wid >= kBTWaves / 2) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                part[((slot * 2 + 0) * 4 + e) * 64 + lane] = tw0[e];
-                part[((slot * 2 + 1) * 4 + e) * 64 + lane] = tw1[e];
-            }
+            for (int ct = 0; ct < kBTCT; ++ct)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) part[((slot * kBTCT + ct) * 4 + e) * 64 + lane] = tw[ct][e];
         }
         __syncthreads();
         if (wid < kBTWaves / 2) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                part[((slot * 2 + 0) * 4 + e) * 64 + lane] += tw0[e];
-                part[((slot * 2 + 1) * 4 + e) * 64 + lane] += tw1[e];
-            }
+            for (int ct = 0; ct < kBTCT; ++ct)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) part[((slot * kBTCT + ct) * 4 + e) * 64 + lane] += tw[ct][e];
         }
         __syncthreads();
         // -(T W)[i][cc]: entry (i, j) of a tile sits in element i / 4 of lane j + 16 (i % 4)
-        {
-            const int i = tid / kBTCols, cc = tid % kBTCols, ct = cc >> 4, j = cc & 15;
+        for (int q = tid; q < kBT * kBTCols; q += 64 * kBTWaves) {
+            const int i = q / kBTCols, cc = q % kBTCols, ct = cc >> 4, j = cc & 15;
             const int off = (ct * 4 + (i >> 2)) * 64 + j + 16 * (i & 3);
             double sum = 0.0;
 #pragma unroll
-            for (int w = 0; w < kBTWaves / 2; ++w) sum += part[w * 2 * 4 * 64 + off];
+            for (int w = 0; w < kBTWaves / 2; ++w) sum += part[w * kBTCT * 4 * 64 + off];
             Wp[i * kBTCols + cc] = -sum;
         }
         __syncthreads();
         // Z -= V (T W): k-step s covers reflectors 4s .. 4s + 3
-        double wb[4][2];
+        double wb[4][kBTCT];
 #pragma unroll
         for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) wb[s2][ct] = Wp[(4 * s2 + lg) * kBTCols + 16 * ct + li];
+            for (int ct = 0; ct < kBTCT; ++ct) wb[s2][ct] = Wp[(4 * s2 + lg) * kBTCols + 16 * ct + li];
 #pragma unroll
         for (int t = 0; t < kBTTiles; ++t) {
             const int row0 = 16 * (wid + kBTWaves * t);
@@ -1133,8 +1139,8 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
 #pragma unroll
                 for (int s2 = 0; s2 < 4; ++s2) {
                     const double va = Vb[(row0 + li - base) * kVbS + 4 * s2 + lg];
-                    z[t][0] = mfma16x16x4(va, wb[s2][0], z[t][0]);
-                    z[t][1] = mfma16x16x4(va, wb[s2][1], z[t][1]);
+#pragma unroll
+                    for (int ct = 0; ct < kBTCT; ++ct) z[t][ct] = mfma16x16x4(va, wb[s2][ct], z[t][ct]);
                 }
             }
         }
@@ -1142,7 +1148,7 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
 #pragma unroll
     for (int t = 0; t < kBTTiles; ++t)
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
+        for (int ct = 0; ct < kBTCT; ++ct) {
             const int col = c0 + 16 * ct + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1186,7 +1192,7 @@ hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const 
     if (ld > kEigMaxDim || kcap > ld || ld % 16 != 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_block_t, dim3(ld / 16, count), dim3(64), 0, s, act, nvec, ld, A, tau, work);
     auto lds_of = [](int l) {
-        return ((size_t)l * kVbS + (kBTWaves / 2) * 2 * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
+        return ((size_t)l * kVbS + (kBTWaves / 2) * kBTCT * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
     };
     static bool attr = false;  // LDS above 64 KiB must be requested explicitly
     if (!attr) {
