@@ -6,6 +6,7 @@ Both sides use tight SCC tolerances, so every energy and gradient agrees to
 counts and convergence flags, final energies within 1e-8 Eh, positions within
 1e-6 bohr.
 """
+import os
 import sys
 
 import numpy as np
@@ -14,7 +15,7 @@ import pytest
 import optimizer_oracle as opt_oracle
 import oracle
 from gfn1_amd import (AnalyticGradientOptions, BatchEngine, ElectronicOptions, GeometryOptimizationOptions,
-                      Gfn1NativeCalculator, NativeError, optimize_geometry)
+                      Gfn1NativeCalculator, GuessModel, NativeError, optimize_geometry)
 from gfn1_amd import geometry
 
 pytestmark = pytest.mark.gpu
@@ -166,3 +167,24 @@ def test_calculator_optimize_writes_trajectory(tmp_path):
     assert r.to_xyz().splitlines()[2].split()[0] == "H"
     assert np.allclose(r.forces_ev_per_angstrom, -np.asarray(r.gradient_hartree_per_bohr) * 27.211386245988
                        * A2B)
+
+
+def test_learned_mixed_warm_path_reaches_the_fp64_minima():
+    """The throughput path (learned start, mixed precision, warm start): its line
+    searches may decide differently at the 1e-6 level, so only the minima are
+    compared, within the mixed path's energy bar."""
+    systems = _systems()
+    names = sorted(n for n in systems if n != "water3")
+    note("case learned + mixed + warm optimisation")
+    model = GuessModel.load(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                         "gfn1-rs_amd", "data", "guess_mlp.bin"))
+    fast = BatchEngine([systems[n] for n in names], options=_tight_options(), guess=model, precision="mixed")
+    got = fast.optimize(GeometryOptimizationOptions(max_iterations=MAX_IT, warm_start=True))
+    ref = BatchEngine([systems[n] for n in names], options=_tight_options()).optimize(
+        GeometryOptimizationOptions(max_iterations=MAX_IT))
+    for n, a, b in zip(names, got, ref):
+        note("%s mixed it=%d E=%.10f fp64 it=%d E=%.10f" % (n, a.iterations, a.energy, b.iterations, b.energy))
+        assert a.status == 0 and a.converged and b.converged
+        assert abs(a.energy - b.energy) <= 1e-5
+        assert a.max_gradient <= 1e-4
+
