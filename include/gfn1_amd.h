@@ -9,7 +9,8 @@
  *
  * Ownership: results own their buffers; free them with the matching *_free.
  * Accessor pointers stay valid until the result is freed.  Thread safety: every
- * call is re-entrant; concurrent calls use separate device streams.
+ * call is re-entrant; concurrent calls use separate device streams.  A batch,
+ * optimisation or result handle is used by one thread at a time.
  */
 #ifndef GFN1_AMD_H
 #define GFN1_AMD_H

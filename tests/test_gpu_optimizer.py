@@ -188,3 +188,24 @@ def test_learned_mixed_warm_path_reaches_the_fp64_minima():
         assert abs(a.energy - b.energy) <= 1e-5
         assert a.max_gradient <= 1e-4
 
+
+def test_large_system_in_a_mixed_batch_matches_it_alone():
+    """A 70-water cluster (560 AO, above the engine's own Cholesky limit, so its
+    setup goes through rocSOLVER) optimised next to a small molecule: the rounds
+    select different systems, and the large system must follow exactly the
+    trajectory it follows alone; the small one matches the oracle."""
+    big = geometry.water_cluster(70, 12.9, 3)
+    zw, xw = _systems()["water"]
+    note("case large system in a mixed batch (%d atoms)" % len(big[0]))
+    opts = GeometryOptimizationOptions(max_iterations=3)
+    both = BatchEngine([big, (zw, xw)], options=_tight_options()).optimize(opts)
+    alone = BatchEngine([big], options=_tight_options()).optimize(opts)[0]
+    a = both[0]
+    note("large: batch it=%d E=%.12f alone it=%d E=%.12f" % (a.iterations, a.energy, alone.iterations, alone.energy))
+    assert a.status == 0 and alone.status == 0
+    assert a.iterations == alone.iterations == 3
+    assert abs(a.energy - alone.energy) <= 1e-10
+    assert np.abs(a.positions - alone.positions).max() <= 1e-9
+    want = opt_oracle.optimize_gfn1(zw, xw, scc_options=TIGHT, max_iterations=3)
+    _compare("water_next_to_large", both[1], want)
+
