@@ -117,6 +117,7 @@ class Batch {
     int atom_offset(int inst) const { return h_atom0_[inst]; }   // inst in [0, size()]
     int shell_count(int inst) const { return h_nsh_[inst]; }
     int shell_offset(int inst) const { return h_sh0_[inst]; }    // into fetch_energy_gradient's shell charges
+    int padded_dimension(int inst) const { return h_ld_[inst]; }  // ld of the instance's matrices
     const std::string& message(int inst) const { return h_errmsg_[inst]; }
     const double* timings() const { return timings_; }
     int size() const { return ninst_; }

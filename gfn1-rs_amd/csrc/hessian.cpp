@@ -14,6 +14,10 @@
 #include <cmath>
 
 namespace gfn1 {
+namespace {
+constexpr double kMatricesPerInstance = 12.0;  // n x n matrices the batch engine keeps per instance
+constexpr double kPassBytes = 64.0e9;           // matrices of one pass of displaced geometries
+}  // namespace
 
 Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& systems, EngineOptions e,
                      const HessianOptions& opt, std::vector<HessianResult>& out, HessianStats* stats) {
@@ -39,6 +43,7 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
 
     // the undisplaced geometries: energy, gradient and start charges
     std::vector<std::vector<double>> q0(n);
+    std::vector<double> inst_bytes(n, 0.0);  // device matrices of one displaced copy
     {
         Batch base;
         Status st = base.create(dm, systems, e);
@@ -53,6 +58,8 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
         st_local.evaluations += n;
         for (int i = 0; i < n; ++i) {
             HessianResult& r = out[i];
+            const double ld = base.padded_dimension(i);
+            inst_bytes[i] = kMatricesPerInstance * ld * ld * sizeof(double);
             if (systems[i].nat == 0) {
                 r.status = kErrInvalidInput;
                 r.message = "cannot build the Hessian of an empty system";
@@ -83,10 +90,19 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 all.push_back({i, c, -1});
             }
         }
-    const size_t chunk = (size_t)std::max(2, opt.max_batch) & ~(size_t)1;  // keep +/- pairs together
+    // a pass holds at most max_batch displaced geometries and about
+    // kPassBytes of matrices, and always whole +/- pairs
+    const size_t chunk = (size_t)std::max(2, opt.max_batch) & ~(size_t)1;
     std::vector<char> failed(n, 0);
-    for (size_t s0 = 0; s0 < all.size(); s0 += chunk) {
-        const size_t m = std::min(chunk, all.size() - s0);
+    for (size_t s0 = 0, m = 0; s0 < all.size(); s0 += m) {
+        double bytes = 0.0;
+        m = 0;
+        while (s0 + m < all.size() && m < chunk) {
+            const double pair = 2.0 * inst_bytes[all[s0 + m].sys];
+            if (m > 0 && bytes + pair > kPassBytes) break;
+            bytes += pair;
+            m += 2;
+        }
         std::vector<std::vector<double>> pos(m);
         std::vector<SystemInput> in(m);
         for (size_t k = 0; k < m; ++k) {
