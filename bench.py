@@ -226,7 +226,8 @@ def run_optimize(args):
 def run_hessian(args):
     """Cartesian Hessians of the first clusters of the batch (central
     differences of the analytic gradient, step 1e-4 bohr, SCC tolerances
-    1e-11 / 1e-9): Hessians/s and displaced-geometry evaluations/s.  The CPU
+    1e-11 / 1e-11, displaced SCCs started from the undisplaced charges):
+    Hessians/s and displaced-geometry evaluations/s.  The CPU
     baseline is the oracle's finite-difference Hessian of the smallest of
     those clusters, one core.  Prints one JSON line (not the batched headline
     metric)."""
@@ -244,7 +245,7 @@ def run_hessian(args):
            "gradient_evaluations": evals, "evaluations_per_s": evals / dt,
            "config": {"workload": "Cartesian Hessians of mixed H2O/CH4/NH3 clusters (seeds 1000+i, 2-40 molecules), "
                                   "central differences of the analytic gradient, step 1e-4 bohr, SCC tolerances "
-                                  "1e-11 / 1e-9, FP64", "systems": len(systems), "atoms_min": min(nat),
+                                  "1e-11 / 1e-11, warm-started displaced SCCs, FP64", "systems": len(systems), "atoms_min": min(nat),
                       "atoms_max": max(nat), "atoms_mean": float(np.mean(nat))}}
     print("hessian: %d systems, %d evaluations in %.2f s" % (len(systems), evals, dt), file=sys.stderr, flush=True)
     if args.opt_cpu_sample > 0:

