@@ -679,7 +679,8 @@ int hessian_common(int32_t n, const gfn1_system* systems, const gfn1_params* par
 }  // namespace
 
 int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
-                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry, gfn1_error* err) {
+                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry,
+                 double* dipole_derivatives, double* dipole, gfn1_error* err) {
     if (!system || !hessian) {
         clear(err);
         return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
@@ -691,12 +692,14 @@ int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn
     if (r.status != kOk) return fail(err, Status::error(r.status, r.message));
     std::copy(r.hessian.begin(), r.hessian.end(), hessian);
     if (max_asymmetry) *max_asymmetry = r.max_asymmetry;
+    if (dipole_derivatives) std::copy(r.dipole_derivatives.begin(), r.dipole_derivatives.end(), dipole_derivatives);
+    if (dipole) std::copy(r.dipole, r.dipole + 3, dipole);
     return GFN1_OK;
 }
 
 int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params* params,
                        const gfn1_electronic_options* electronic, const gfn1_hessian_options* options,
-                       double* hessians, int32_t* status, gfn1_error* err) {
+                       double* hessians, double* dipole_derivatives, int32_t* status, gfn1_error* err) {
     if ((n > 0 && (!hessians || !status))) {
         clear(err);
         return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
@@ -704,12 +707,18 @@ int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params*
     std::vector<HessianResult> res;
     int rc = hessian_common(n, systems, params, electronic, options, res, err);
     if (rc != GFN1_OK) return rc;
-    size_t off = 0;
+    size_t off = 0, doff = 0;
     for (int i = 0; i < n; ++i) {
         const size_t len = (size_t)9 * systems[i].nat * systems[i].nat;
         status[i] = res[i].status;
-        if (res[i].status == kOk) std::copy(res[i].hessian.begin(), res[i].hessian.end(), hessians + off);
+        if (res[i].status == kOk) {
+            std::copy(res[i].hessian.begin(), res[i].hessian.end(), hessians + off);
+            if (dipole_derivatives)
+                std::copy(res[i].dipole_derivatives.begin(), res[i].dipole_derivatives.end(),
+                          dipole_derivatives + doff);
+        }
         off += len;
+        doff += (size_t)9 * systems[i].nat;
     }
     return GFN1_OK;
 }

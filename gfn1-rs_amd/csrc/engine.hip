@@ -626,6 +626,25 @@ Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double
     return Status{};
 }
 
+Status Batch::fetch_dipoles(std::vector<double>& dipoles) {
+    std::vector<double> qout(nshells_);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(copy_bytes(qout.data(), d_qout_, nshells_ * sizeof(double), hipMemcpyDeviceToHost));
+    dipoles.assign(3 * (size_t)ninst_, 0.0);
+    std::vector<double> qa;
+    for (int i = 0; i < ninst_; ++i) {
+        const int a0 = h_atom0_[i], nat = h_atom0_[i + 1] - a0;
+        qa.assign(nat, 0.0);
+        for (int k = h_sh0_[i]; k < h_sh0_[i] + h_nsh_[i]; ++k) qa[h_sh_atom_[k] - a0] += qout[k];
+        for (int d = 0; d < 3; ++d) {
+            double v = 0.0;
+            for (int a = 0; a < nat; ++a) v += h_x_[3 * (a0 + a) + d] * qa[a];
+            dipoles[3 * (size_t)i + d] = v;
+        }
+    }
+    return Status{};
+}
+
 // Setup.  The coordination numbers come first (stream_); then every bucket, on
 // its own stream, builds S and H0 over its own atoms, factorises S = L L^T,
 // forms Cb = L^-T and A0 = Cb^T H0 Cb, marks instances whose overlap is not

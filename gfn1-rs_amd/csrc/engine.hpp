@@ -113,6 +113,9 @@ class Batch {
         std::copy(h_x_.begin(), h_x_.end(), x);
         return Status{};
     }
+    // Mulliken dipole of every instance (e bohr, origin 0, 3 per instance) from
+    // the charges of the last run, as the full download computes it.
+    Status fetch_dipoles(std::vector<double>& dipoles);
     int natoms() const { return natoms_; }
     int atom_offset(int inst) const { return h_atom0_[inst]; }   // inst in [0, size()]
     int shell_count(int inst) const { return h_nsh_[inst]; }

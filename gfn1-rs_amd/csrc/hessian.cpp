@@ -51,8 +51,10 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
         st = base.run();
         if (!st.ok()) return st;
         std::vector<int> status;
-        std::vector<double> energy, grad, q;
+        std::vector<double> energy, grad, q, dip;
         st = base.fetch_energy_gradient(status, energy, grad, &q);
+        if (!st.ok()) return st;
+        st = base.fetch_dipoles(dip);
         if (!st.ok()) return st;
         ++st_local.passes;
         st_local.evaluations += n;
@@ -72,6 +74,7 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
             }
             const int a0 = base.atom_offset(i), nat = systems[i].nat;
             r.energy = energy[i];
+            for (int d = 0; d < 3; ++d) r.dipole[d] = dip[3 * (size_t)i + d];
             r.gradient.assign(grad.begin() + 3 * (size_t)a0, grad.begin() + 3 * (size_t)(a0 + nat));
             q0[i].assign(q.begin() + base.shell_offset(i), q.begin() + base.shell_offset(i) + base.shell_count(i));
         }
@@ -85,6 +88,7 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
     for (int i = 0; i < n; ++i)
         if (out[i].status == kOk) {
             out[i].hessian.assign((size_t)9 * systems[i].nat * systems[i].nat, 0.0);
+            out[i].dipole_derivatives.assign((size_t)9 * systems[i].nat, 0.0);
             for (int c = 0; c < 3 * systems[i].nat; ++c) {
                 all.push_back({i, c, +1});
                 all.push_back({i, c, -1});
@@ -123,8 +127,10 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
         st = b.run();
         if (!st.ok()) return st;
         std::vector<int> status;
-        std::vector<double> energy, grad;
+        std::vector<double> energy, grad, dip;
         st = b.fetch_energy_gradient(status, energy, grad, nullptr);
+        if (!st.ok()) return st;
+        st = b.fetch_dipoles(dip);
         if (!st.ok()) return st;
         ++st_local.passes;
         st_local.evaluations += (long long)m;
@@ -146,12 +152,18 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
             const double* gm = grad.data() + 3 * (size_t)b.atom_offset((int)k + 1);
             for (int row = 0; row < ndof; ++row)
                 r.hessian[(size_t)row * ndof + d.col] = (gp[row] - gm[row]) / (2.0 * opt.step);
+            // dipole derivatives by the same differences (the analytic route is
+            // src/properties.rs:97-145)
+            for (int a = 0; a < 3; ++a)
+                r.dipole_derivatives[3 * (size_t)d.col + a] =
+                    (dip[3 * k + a] - dip[3 * (k + 1) + a]) / (2.0 * opt.step);
         }
     }
     for (int i = 0; i < n; ++i) {
         HessianResult& r = out[i];
         if (r.status != kOk) {
             r.hessian.clear();
+            r.dipole_derivatives.clear();
             continue;
         }
         const int ndof = 3 * systems[i].nat;

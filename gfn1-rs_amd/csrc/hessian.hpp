@@ -31,6 +31,8 @@ struct HessianResult {
     std::vector<double> gradient;  // 3 nat, undisplaced geometry
     std::vector<double> hessian;   // (3 nat)^2, row-major, symmetrised
     double max_asymmetry = 0.0;    // max |H_ij - H_ji| before symmetrisation
+    double dipole[3] = {0.0, 0.0, 0.0};   // Mulliken dipole (e bohr), undisplaced geometry
+    std::vector<double> dipole_derivatives;  // d mu_alpha / d R_c at [3 c + alpha], central differences
 };
 
 struct HessianStats {

@@ -11,6 +11,7 @@ Python surface of the C ABI in include/gfn1_amd.h:
   hessian(...), hessian_batch   analytic_hessian(...).hessian, hessian/mod.rs:112 (central differences
                                 of the analytic gradient on the device)
   vibrational_analysis(...)     vibrational.rs:37
+  ir_spectrum_from(...)         properties.rs:381 (with the dipole derivatives of hessian(...))
 """
 from .api import (  # noqa: F401
     ANGSTROM_TO_BOHR,
@@ -36,6 +37,7 @@ from .api import (  # noqa: F401
     analytic_gradient,
     hessian,
     hessian_batch,
+    ir_spectrum_from,
     optimize_geometry,
     vibrational_analysis,
     run_electronic,

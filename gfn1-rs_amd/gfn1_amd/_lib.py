@@ -160,10 +160,10 @@ SIGNATURES = [
     ("gfn1_optimization_free", None, [_VP]),
     ("gfn1_hessian_options_default", None, [ctypes.POINTER(HessianOptions)]),
     ("gfn1_hessian", ctypes.c_int, [ctypes.POINTER(System), _VP, ctypes.POINTER(ElectronicOptions),
-                                    ctypes.POINTER(HessianOptions), _DP, _DP, ctypes.POINTER(Gfn1Error)]),
+                                    ctypes.POINTER(HessianOptions), _DP, _DP, _DP, _DP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_hessian_batch", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _VP,
                                           ctypes.POINTER(ElectronicOptions), ctypes.POINTER(HessianOptions), _DP,
-                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),
+                                          _DP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

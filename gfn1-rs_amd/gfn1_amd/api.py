@@ -383,9 +383,14 @@ class HessianOptions:
 @dataclass
 class HessianResult:
     """hessian: (3 nat, 3 nat) in Eh/bohr^2, index 3*atom + axis, symmetrised
-    (AnalyticHessianResult.hessian, hessian/mod.rs:55-65)."""
+    (AnalyticHessianResult.hessian, hessian/mod.rs:55-65); from the same
+    displaced evaluations, dipole_derivatives (3 nat, 3): d mu_alpha / d R_c in
+    e (DipoleDerivatives.ddipole_dr, properties.rs:80-88) and the Mulliken
+    dipole (e bohr, origin 0)."""
     hessian: np.ndarray
     max_asymmetry: float
+    dipole_derivatives: np.ndarray = None
+    dipole: np.ndarray = None
 
 
 def hessian(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
@@ -400,11 +405,14 @@ def hessian(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
     sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
     ndof = 3 * len(np.asarray(numbers))
     out = np.zeros((ndof, ndof))
+    ddip = np.zeros((ndof, 3))
+    dip = np.zeros(3)
     asym = ctypes.c_double()
     err = _lib.Gfn1Error()
     _lib.check(lib.gfn1_hessian(ctypes.byref(sysc), p._h, ctypes.byref(eo), ctypes.byref(ho),
-                                out.ctypes.data_as(_lib._DP), ctypes.byref(asym), ctypes.byref(err)), err)
-    return HessianResult(out, asym.value)
+                                out.ctypes.data_as(_lib._DP), ctypes.byref(asym), ddip.ctypes.data_as(_lib._DP),
+                                dip.ctypes.data_as(_lib._DP), ctypes.byref(err)), err)
+    return HessianResult(out, asym.value, ddip, dip)
 
 
 def hessian_batch(systems, params: Optional[Gfn1Parameters] = None, electronic: Optional[ElectronicOptions] = None,
@@ -422,7 +430,7 @@ def hessian_batch(systems, params: Optional[Gfn1Parameters] = None, electronic: 
     status = np.zeros(len(systems), dtype=np.int32)
     err = _lib.Gfn1Error()
     _lib.check(lib.gfn1_hessian_batch(len(systems), arr, p._h, ctypes.byref(eo), ctypes.byref(ho),
-                                      flat.ctypes.data_as(_lib._DP),
+                                      flat.ctypes.data_as(_lib._DP), None,
                                       status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(err)), err)
     out, off = [], 0
     for k, st in zip(sizes, status):
@@ -445,6 +453,24 @@ ATOMIC_MASSES = (
     174.9668, 178.49, 180.94788, 183.84, 186.207, 190.23, 192.217, 195.084, 196.966569, 200.592,
     204.38, 207.2, 208.98040, 209.0, 210.0, 222.0)
 WAVENUMBER_PER_SQRT_AU = 5140.4843  # cm^-1 per sqrt(Eh / (bohr^2 amu)), vibrational.rs:20
+
+
+# Integrated molar absorption (km/mol) per atomic-unit intensity |d mu / d Q|^2
+# (e^2 / u): N_A e^2 / (12 eps0 c^2 u), CODATA 2018 constants; the reference
+# tabulates 974.8801118 (properties.rs:32-35).
+IR_INTENSITY_AU_TO_KM_PER_MOL = (6.02214076e23 * 1.602176634e-19 ** 2
+                                 / (12.0 * 8.8541878128e-12 * 299792458.0 ** 2 * 1.66053906660e-27)) / 1000.0
+
+
+def ir_spectrum_from(hessian_matrix, dipole_derivatives, numbers):
+    """Harmonic IR spectrum (properties.rs:381-420): per mode the wavenumber,
+    d mu / d Q_k = sum_c d mu / d R_c L_ck over the Cartesian mode L, and its
+    squared norm in e^2/u and km/mol."""
+    v = vibrational_analysis(hessian_matrix, numbers)
+    dmu = v.modes @ np.asarray(dipole_derivatives, dtype=np.float64)
+    inten = np.einsum("ka,ka->k", dmu, dmu)
+    return {"wavenumbers": v.wavenumbers, "intensities_km_per_mol": inten * IR_INTENSITY_AU_TO_KM_PER_MOL,
+            "intensities_au": inten, "dipole_gradients": dmu}
 
 
 @dataclass
@@ -620,6 +646,23 @@ class Gfn1NativeCalculator:
         Cartesian displacement per row)."""
         v = vibrational_analysis(self.hessian(numbers, positions, unit), numbers)
         return {"wavenumbers": v.wavenumbers, "modes": v.modes}
+
+    def dipole_derivatives(self, numbers, positions, unit="angstrom", origin=None):
+        """Calculator dipole_derivatives (python.rs:575-600): dipole (e bohr,
+        about origin, bohr) and ddipole_dr ([coord][alpha], e), by central
+        differences on the device."""
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * _unit_scale(unit)
+        r = hessian(numbers, x, self.params, self.options, charge=self.charge)
+        o = np.zeros(3) if origin is None else np.asarray(origin, dtype=np.float64)
+        return {"dipole": r.dipole - self.charge * o, "ddipole_dr": r.dipole_derivatives}
+
+    def ir_spectrum(self, numbers, positions, unit="angstrom", origin=None):
+        """Calculator ir_spectrum (python.rs:600-647): wavenumbers (cm^-1),
+        intensities_km_per_mol, intensities_au, dipole_gradients.  The
+        derivatives do not depend on the origin (the total charge is fixed)."""
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * _unit_scale(unit)
+        r = hessian(numbers, x, self.params, self.options, charge=self.charge)
+        return ir_spectrum_from(r.hessian, r.dipole_derivatives, numbers)
 
 
 def _unit_scale(unit):

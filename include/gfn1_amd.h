@@ -353,15 +353,20 @@ typedef struct gfn1_hessian_options {
 } gfn1_hessian_options;
 void gfn1_hessian_options_default(gfn1_hessian_options* options);
 /* One system: hessian receives (3 nat)^2 doubles, row-major, index 3*atom + axis,
- * symmetrised; max_asymmetry (nullable) the largest |H_ij - H_ji| before that. */
+ * symmetrised; max_asymmetry (nullable) the largest |H_ij - H_ji| before that.
+ * The same displaced evaluations give the Cartesian dipole derivatives
+ * (dipole_derivatives, nullable: 3 nat x 3 doubles, d mu_alpha / d R_c at
+ * [3 c + alpha], e) and the Mulliken dipole (dipole, nullable: 3 doubles, e bohr,
+ * origin 0), the quantities of dipole_derivatives (src/properties.rs:80-145). */
 int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
-                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry, gfn1_error* err);
+                 const gfn1_hessian_options* options, double* hessian, double* max_asymmetry,
+                 double* dipole_derivatives, double* dipole, gfn1_error* err);
 /* Many systems: hessians receives the (3 nat_i)^2 blocks one after another,
- * status (n entries) each system's gfn1_status (its block is left untouched
- * unless GFN1_OK). */
+ * dipole_derivatives (nullable) the 9 nat_i blocks, status (n entries) each
+ * system's gfn1_status (its blocks are left untouched unless GFN1_OK). */
 int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params* params,
                        const gfn1_electronic_options* electronic, const gfn1_hessian_options* options,
-                       double* hessians, int32_t* status, gfn1_error* err);
+                       double* hessians, double* dipole_derivatives, int32_t* status, gfn1_error* err);
 
 /* Library version string. */
 const char* gfn1_version(void);

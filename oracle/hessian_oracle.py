@@ -1,4 +1,5 @@
-"""CPU Hessian by central differences of the oracle's analytic gradient.
+"""CPU Hessian (and dipole derivatives) by central differences of the oracle's
+analytic gradient (and Mulliken dipole).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
 Hessian.  The product never imports it.
@@ -8,7 +9,9 @@ construction the reference validates its analytic Hessian against
 (tests/hessian.rs:691-760), then symmetrised.  Pinned by
 tests/test_hessian_oracle.py against second differences of the energy (a
 separate code path), the translational sum rule, symmetry, and the
-reference's own vibrational-analysis unit tests (vibrational.rs:88-129).
+reference's own vibrational-analysis unit tests (vibrational.rs:88-129); the
+dipole derivatives by the translational (sum of Born charges = total charge)
+and rotational (sum_B dmu/dR_B . (n x R_B) = n x mu) sum rules.
 """
 import numpy as np
 
@@ -21,9 +24,16 @@ def gradient_terms(include_repulsion=True, include_electronic=True, include_disp
                 include_halogen=int(include_halogen))
 
 
-def fd_hessian(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, charge_tolerance=1.0e-11,
-               charge=0.0, warm_start=True, **terms):
+def fd_hessian(numbers, positions_bohr, **kw):
     """(symmetrised Hessian, max asymmetry before symmetrisation)."""
+    h, asym, _, _ = fd_hessian_dipoles(numbers, positions_bohr, **kw)
+    return h, asym
+
+
+def fd_hessian_dipoles(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, charge_tolerance=1.0e-11,
+                       charge=0.0, warm_start=True, **terms):
+    """(Hessian, asymmetry, dipole derivatives [3 nat, 3] = d mu_alpha / d R_c by
+    the same central differences, Mulliken dipole of the undisplaced geometry)."""
     x0 = np.asarray(positions_bohr, dtype=np.float64).reshape(-1)
     ndof = x0.size
     opts = dict(energy_tolerance=energy_tolerance, charge_tolerance=charge_tolerance, charge=charge)
@@ -33,6 +43,7 @@ def fd_hessian(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, c
         raise RuntimeError(base["message"])
     q0 = base["shell_charges"] if warm_start else None
     h = np.zeros((ndof, ndof))
+    ddip = np.zeros((ndof, 3))
     for c in range(ndof):
         xp = x0.copy()
         xp[c] += step
@@ -43,5 +54,6 @@ def fd_hessian(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, c
         if gp["status"] or gm["status"]:
             raise RuntimeError("displaced evaluation failed")
         h[:, c] = (gp["gradient"].reshape(-1) - gm["gradient"].reshape(-1)) / (2.0 * step)
+        ddip[c] = (gp["dipole"] - gm["dipole"]) / (2.0 * step)
     asym = float(np.abs(h - h.T).max()) if ndof else 0.0
-    return 0.5 * (h + h.T), asym
+    return 0.5 * (h + h.T), asym, ddip, base["dipole"]

@@ -11,8 +11,8 @@ import pytest
 
 import hessian_oracle
 import optimizer_oracle
-from gfn1_amd import (ElectronicOptions, GeometryOptimizationOptions, HessianOptions, NativeError,
-                      geometry, hessian, hessian_batch, optimize_geometry, vibrational_analysis)
+from gfn1_amd import (ElectronicOptions, GeometryOptimizationOptions, Gfn1NativeCalculator, HessianOptions,
+                      NativeError, geometry, hessian, hessian_batch, optimize_geometry, vibrational_analysis)
 from gfn1_amd import AnalyticGradientOptions
 
 pytestmark = pytest.mark.gpu
@@ -99,3 +99,46 @@ def test_relaxed_water_frequencies_match_oracle():
     w_ref = vibrational_analysis(hessian_oracle.fd_hessian(z, want["positions"])[0], z).wavenumbers
     note("water wavenumbers device %s oracle %s" % (np.round(w[6:], 3), np.round(w_ref[6:], 3)))
     assert np.abs(w[6:] - w_ref[6:]).max() < 0.01
+
+
+def _hydronium():
+    import math
+    pts = [[0.0, 0.0, 0.0]]
+    for k in range(3):
+        phi = 2.0 * math.pi * k / 3.0
+        pts.append([0.95 * math.cos(phi), 0.95 * math.sin(phi), 0.30])
+    return [8, 1, 1, 1], np.array(pts) * A2B
+
+
+@pytest.mark.parametrize("name", ["water", "methane", "ammonia", "hydronium"])
+def test_dipole_derivatives_match_oracle(name):
+    if name == "hydronium":
+        (z, x), charge = _hydronium(), 1.0
+    else:
+        (z, x), charge = _systems()[name], 0.0
+    note("case dipole derivatives %s" % name)
+    r = hessian(z, x, charge=charge)
+    _, _, ddip, mu = hessian_oracle.fd_hessian_dipoles(z, x, charge=charge)
+    d = float(np.abs(r.dipole_derivatives - ddip).max())
+    note("%s: max|d ddip| %.2e, |d mu| %.2e" % (name, d, float(np.abs(r.dipole - mu).max())))
+    assert d <= 1e-6
+    assert np.abs(r.dipole - mu).max() <= 1e-8
+    born = r.dipole_derivatives.reshape(len(z), 3, 3)
+    assert np.abs(born.sum(axis=0) - charge * np.eye(3)).max() < 1e-6
+
+
+def test_calculator_ir_spectrum():
+    from gfn1_amd.api import ir_spectrum_from
+    z, x = _systems()["water"]
+    note("case calculator ir spectrum")
+    calc = Gfn1NativeCalculator()
+    ir = calc.ir_spectrum(z, x, unit="bohr")
+    assert set(ir) == {"wavenumbers", "intensities_km_per_mol", "intensities_au", "dipole_gradients"}
+    r = hessian(z, x)
+    want = ir_spectrum_from(r.hessian, r.dipole_derivatives, z)
+    assert np.allclose(ir["intensities_km_per_mol"], want["intensities_km_per_mol"], rtol=1e-12, atol=1e-12)
+    dd = calc.dipole_derivatives(z, x, unit="bohr")
+    assert np.array_equal(dd["ddipole_dr"], r.dipole_derivatives)
+    # the three vibrations of water are IR active
+    assert np.all(ir["intensities_km_per_mol"][6:] > 1.0)
+

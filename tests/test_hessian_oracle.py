@@ -108,3 +108,53 @@ def test_relaxed_water_has_six_zero_modes():
     w = vibrational_analysis(h, z).wavenumbers
     assert np.all(np.abs(w[:6]) < 30.0), w[:6]
     assert 1300.0 < w[6] < 1800.0 and 3000.0 < w[7] < w[8] < 4000.0, w[6:]
+
+
+def _hydronium():
+    # H3O+: O at the origin, three H at 0.98 Angstrom in a flattened pyramid
+    import math
+    pts = [[0.0, 0.0, 0.0]]
+    for k in range(3):
+        phi = 2.0 * math.pi * k / 3.0
+        pts.append([0.95 * math.cos(phi), 0.95 * math.sin(phi), 0.30])
+    return [8, 1, 1, 1], np.array(pts) * A2B
+
+
+@pytest.mark.parametrize("case", ["water", "hydronium"])
+def test_dipole_derivative_sum_rules(case):
+    z, x = _water() if case == "water" else _hydronium()
+    charge = 0.0 if case == "water" else 1.0
+    _, _, ddip, mu = hessian_oracle.fd_hessian_dipoles(z, x, charge=charge)
+    nat = len(z)
+    born = ddip.reshape(nat, 3, 3)  # [atom][beta][alpha]
+    # translation by t moves mu by Q t: the Born charges sum to Q * identity
+    assert np.abs(born.sum(axis=0) - charge * np.eye(3)).max() < 1e-6
+    # a small rotation about n moves every atom by n x R and mu by n x mu
+    xr = np.asarray(x).reshape(-1, 3)
+    for n in np.eye(3):
+        dmu = np.einsum("bja,bj->a", born, np.cross(n, xr))
+        assert np.abs(dmu - np.cross(n, mu)).max() < 1e-6
+
+
+def test_ir_intensity_constant_and_point_charge_closed_form():
+    from gfn1_amd.api import IR_INTENSITY_AU_TO_KM_PER_MOL, ir_spectrum_from
+    # the reference tabulates the conversion (properties.rs:35)
+    assert abs(IR_INTENSITY_AU_TO_KM_PER_MOL / 974.8801118 - 1.0) < 1e-9
+    # rigid point charges +q, -q on a spring along x: d mu / d Q = q (L_1 - L_2)
+    q, k = 0.4, 0.5
+    h = np.zeros((6, 6))
+    h[0, 0] = h[3, 3] = k
+    h[0, 3] = h[3, 0] = -k
+    ddip = np.zeros((6, 3))
+    for b, qb in ((0, q), (1, -q)):
+        ddip[3 * b:3 * b + 3] = qb * np.eye(3)
+    ir = ir_spectrum_from(h, ddip, [1, 8])
+    m1, m2 = 1.008, 15.999
+    mu_red = m1 * m2 / (m1 + m2)
+    # the stretch, normalised in mass-weighted coordinates: |dmu/dQ|^2 = q^2 / mu_red
+    assert abs(ir["intensities_au"][-1] - q * q / mu_red) < 1e-12
+    # the modes are an orthonormal basis of the mass-weighted coordinates, so the
+    # intensities of all modes add up to sum_c |d mu / d R_c|^2 / m_c
+    masses = np.repeat([m1, m2], 3)
+    assert abs(ir["intensities_au"].sum() - float((ddip ** 2).sum(axis=1) @ (1.0 / masses))) < 1e-12
+
