@@ -4,7 +4,8 @@ The ASE boundary: positions in Angstrom, energies in eV, forces in eV/Angstrom,
 dipoles in e*Angstrom, Hessians in eV/Angstrom^2, every factor taken from
 ``ase.units`` (ase.py:38-57); the engine underneath speaks atomic units.
 Non-periodic systems: properties energy, forces, charges, dipole; the native
-L-BFGS (``optimize_native``), the Hessian and the harmonic frequencies.
+L-BFGS (``optimize_native``), the Hessian, the harmonic frequencies, the
+dipole derivatives and the IR spectrum.
 Importing this module needs ASE, as the reference's does.
 """
 from typing import Any
@@ -107,6 +108,25 @@ class GFN1AMDCalculator(Calculator):
         """(3N, 3N) Hessian in eV/Angstrom^2 (ase.py:641-671)."""
         numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
         return np.asarray(self._native.hessian(numbers, positions, unit="angstrom"), dtype=float) * _HESSIAN
+
+    def get_dipole_derivatives(self, atoms=None, origin=None):
+        """dipole (e*Angstrom), dipole_au (e*bohr) and ddipole_dr ([coord][alpha],
+        e, the same number in either unit system) (ase.py:603-622); origin in
+        Angstrom."""
+        numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
+        o = None if origin is None else tuple(float(v) / _LENGTH for v in origin)
+        out = self._native.dipole_derivatives(numbers, positions, unit="angstrom", origin=o)
+        dipole_au = np.asarray(out["dipole"], dtype=float)
+        out["dipole_au"] = dipole_au
+        out["dipole"] = dipole_au * _DIPOLE
+        out["ddipole_dr"] = np.asarray(out["ddipole_dr"], dtype=float)
+        return out
+
+    def get_ir_spectrum(self, atoms=None, origin=None):
+        """wavenumbers (cm^-1), intensities_km_per_mol, and the atomic-unit
+        intensities_au and dipole_gradients (ase.py:624-639)."""
+        numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
+        return self._native.ir_spectrum(numbers, positions, unit="angstrom")
 
     def get_vibrational_frequencies(self, atoms=None):
         """wavenumbers (cm^-1), energies_ev and modes (ase.py:673-687)."""

@@ -135,3 +135,8 @@ def test_optimize_hessian_frequencies(ase_calc):
     w = np.asarray(out["wavenumbers"])
     assert np.all(np.abs(w[:6]) < 30.0) and 1300.0 < w[6] < 1800.0 and 3000.0 < w[7] < 4000.0
     assert np.allclose(out["energies_ev"], w * ASE_INVCM)
+    dd = calc.get_dipole_derivatives(atoms)
+    assert np.allclose(dd["dipole"], dd["dipole_au"] * Bohr, rtol=1e-14, atol=0)
+    ir = calc.get_ir_spectrum(atoms)
+    assert np.all(np.asarray(ir["intensities_km_per_mol"])[6:] > 1.0)
+
