@@ -80,6 +80,19 @@ def test_water48_matches_oracle():
     assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
 
 
+def test_water70_matches_oracle():
+    """560 AO: above the engine's own Cholesky and eigensolver limit (512), so
+    the setup and every SCC eigensolve go through rocSOLVER."""
+    note("case water70 (rocSOLVER branch)")
+    z, x = geometry.water_cluster(70, 12.9, 3)
+    ref = oracle.run(z, x, **TIGHT)
+    res = analytic_gradient(z, x, options=AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT)))
+    note("water70: dE %.2e max|dg| %.2e" % (abs(res.total_energy - ref["energy"]),
+                                            float(np.abs(res.gradient - ref["gradient"]).max())))
+    assert abs(res.total_energy - ref["energy"]) <= E_TOL
+    assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
+
+
 def test_fallback_ladder_matches_oracle():
     note("case fallback ladder")
     z, x = mol("water")
