@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -62,6 +63,12 @@ struct gfn1_gradient_result {
 
 struct gfn1_guess_model {
     GuessModel m;
+};
+
+struct gfn1_xyz {
+    std::vector<int32_t> z;
+    std::vector<double> x;
+    gfn1_system sys{};
 };
 
 struct gfn1_optimization {
@@ -621,6 +628,110 @@ void gfn1_optimization_stats(const gfn1_optimization* r, int32_t* rounds, int64_
 }
 
 void gfn1_optimization_free(gfn1_optimization* r) { delete r; }
+
+namespace {
+// Element symbols Z = 1..86 (src/system.rs:255-262).
+const char* const kSymbols[] = {
+    "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl", "Ar",
+    "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr",
+    "Rb", "Sr", "Y",  "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te", "I",  "Xe",
+    "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf",
+    "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn"};
+// symbol_to_z (src/system.rs:154-165): an atomic number 1..86, or a symbol
+// with its first letter upper-cased and the rest lower-cased
+int symbol_to_z(const std::string& t) {
+    if (!t.empty() && t.size() <= 3 && std::all_of(t.begin(), t.end(), [](char c) { return c >= '0' && c <= '9'; })) {
+        const int z = std::atoi(t.c_str());
+        if (z >= 1 && z <= 86) return z;
+    }
+    std::string s = t;
+    for (size_t i = 0; i < s.size(); ++i) s[i] = i == 0 ? (char)std::toupper((unsigned char)s[i]) : (char)std::tolower((unsigned char)s[i]);
+    for (int z = 1; z <= 86; ++z)
+        if (s == kSymbols[z - 1]) return z;
+    return 0;
+}
+
+bool parse_number(const std::string& t, double* v) {
+    if (t.empty()) return false;
+    char* end = nullptr;
+    *v = std::strtod(t.c_str(), &end);
+    return end == t.c_str() + t.size();
+}
+
+std::vector<std::string> split_ws(const std::string& line) {
+    std::vector<std::string> out;
+    size_t i = 0;
+    while (i < line.size()) {
+        while (i < line.size() && std::isspace((unsigned char)line[i])) ++i;
+        const size_t j = i;
+        while (i < line.size() && !std::isspace((unsigned char)line[i])) ++i;
+        if (i > j) out.push_back(line.substr(j, i - j));
+    }
+    return out;
+}
+}  // namespace
+
+int gfn1_system_from_xyz(const char* text, double charge, int32_t coordinates_are_bohr, gfn1_xyz** out,
+                         gfn1_error* err) {
+    clear(err);
+    if (!text || !out) return fail(err, Status::error(kErrInvalidInput, "invalid XYZ arguments"));
+    std::vector<std::string> lines;
+    {
+        std::string cur;
+        for (const char* p = text; *p; ++p) {
+            if (*p == '\n') {
+                if (!cur.empty() && cur.back() == '\r') cur.pop_back();
+                lines.push_back(cur);
+                cur.clear();
+            } else {
+                cur.push_back(*p);
+            }
+        }
+        if (!cur.empty()) lines.push_back(cur);
+    }
+    if (lines.empty()) return fail(err, Status::error(kErrInvalidInput, "empty XYZ"));
+    const std::vector<std::string> head = split_ws(lines[0]);
+    char* end = nullptr;
+    const long long natoms = head.size() == 1 ? std::strtoll(head[0].c_str(), &end, 10) : -1;
+    if (head.size() != 1 || end != head[0].c_str() + head[0].size() || natoms < 0)
+        return fail(err, Status::error(kErrInvalidInput, "invalid XYZ atom count: " + lines[0]));
+    const std::string comment = lines.size() > 1 ? lines[1] : std::string();
+    if (comment.find("Lattice=\"") != std::string::npos)
+        return fail(err, Status::error(kErrInvalidInput, "periodic (Lattice=) XYZ input is not supported"));
+    auto r = std::make_unique<gfn1_xyz>();
+    const double scale = coordinates_are_bohr ? 1.0 : gfn1::kAngstromToBohr;
+    for (long long idx = 0; idx < natoms; ++idx) {
+        const size_t line_no = (size_t)idx + 3;
+        if ((size_t)idx + 2 >= lines.size())
+            return fail(err, Status::error(kErrInvalidInput, "XYZ ended before atom " + std::to_string(idx)));
+        const std::vector<std::string> parts = split_ws(lines[(size_t)idx + 2]);
+        if (parts.size() < 4)
+            return fail(err, Status::error(kErrInvalidInput,
+                                           "XYZ line " + std::to_string(line_no) + " has fewer than 4 fields"));
+        const int z = symbol_to_z(parts[0]);
+        if (!z)
+            return fail(err, Status::error(kErrInvalidInput, "unknown element on line " + std::to_string(line_no) +
+                                                                 ": " + parts[0]));
+        r->z.push_back(z);
+        for (int k = 1; k <= 3; ++k) {
+            double v = 0.0;
+            if (!parse_number(parts[k], &v))
+                return fail(err, Status::error(kErrParse, "line " + std::to_string(line_no) +
+                                                              ": invalid floating point value: " + parts[k]));
+            r->x.push_back(v * scale);
+        }
+    }
+    r->sys.nat = (int32_t)r->z.size();
+    r->sys.numbers = r->z.empty() ? nullptr : r->z.data();
+    r->sys.positions = r->x.empty() ? nullptr : r->x.data();
+    r->sys.charge = charge;
+    *out = r.release();
+    return GFN1_OK;
+}
+
+const gfn1_system* gfn1_xyz_system(const gfn1_xyz* x) { return x ? &x->sys : nullptr; }
+
+void gfn1_xyz_free(gfn1_xyz* x) { delete x; }
 
 void gfn1_hessian_options_default(gfn1_hessian_options* o) {
     if (!o) return;

@@ -2,6 +2,7 @@
 
 Python surface of the C ABI in include/gfn1_amd.h:
   Gfn1Parameters.resolve()      params.rs:204-230
+  system_from_xyz(...)          system.rs:45-99 (PeriodicSystem::from_xyz_str, non-periodic)
   run_electronic(...)           electronic.rs:800
   analytic_gradient(...)        gradient.rs:56
   Gfn1NativeCalculator          python.rs:115-381
@@ -41,5 +42,7 @@ from .api import (  # noqa: F401
     optimize_geometry,
     vibrational_analysis,
     run_electronic,
+    system_from_xyz,
+    system_from_xyz_file,
 )
 from ._lib import NativeError  # noqa: F401

@@ -164,6 +164,10 @@ SIGNATURES = [
     ("gfn1_hessian_batch", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _VP,
                                           ctypes.POINTER(ElectronicOptions), ctypes.POINTER(HessianOptions), _DP,
                                           _DP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_system_from_xyz", ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_int32, ctypes.POINTER(_VP),
+                                            ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_xyz_system", ctypes.POINTER(System), [_VP]),
+    ("gfn1_xyz_free", None, [_VP]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

@@ -175,6 +175,32 @@ class AnalyticGradientResult:
     max_gradient: float
 
 
+def system_from_xyz(text: str, charge: float = 0.0, coordinates_are_bohr: bool = False):
+    """PeriodicSystem::from_xyz_str (system.rs:45-99), non-periodic: returns
+    (numbers, positions in bohr as an (nat, 3) array, charge).  Raises
+    NativeError (InvalidInput, or Parse for a bad coordinate) like the
+    reference's Err."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    err = _lib.Gfn1Error()
+    _lib.check(lib.gfn1_system_from_xyz(text.encode(), float(charge), int(bool(coordinates_are_bohr)),
+                                        ctypes.byref(h), ctypes.byref(err)), err)
+    try:
+        s = lib.gfn1_xyz_system(h).contents
+        nat = s.nat
+        numbers = np.ctypeslib.as_array(s.numbers, shape=(nat,)).copy() if nat else np.zeros(0, dtype=np.int32)
+        pos = np.ctypeslib.as_array(s.positions, shape=(3 * nat,)).reshape(nat, 3).copy() if nat else np.zeros((0, 3))
+        return numbers, pos, s.charge
+    finally:
+        lib.gfn1_xyz_free(h)
+
+
+def system_from_xyz_file(path, charge: float = 0.0, coordinates_are_bohr: bool = False):
+    """PeriodicSystem::from_xyz_file (system.rs:35-43)."""
+    with open(path) as f:
+        return system_from_xyz(f.read(), charge, coordinates_are_bohr)
+
+
 def _params(params):
     return params if params is not None else Gfn1Parameters.resolve()
 

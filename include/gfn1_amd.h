@@ -64,6 +64,20 @@ typedef struct gfn1_system {
     double charge;
 } gfn1_system;
 
+/* PeriodicSystem::from_xyz_str (src/system.rs:45-99) for non-periodic input:
+ * line 1 the atom count, line 2 a comment, then "symbol x y z" per atom (a
+ * symbol in any letter case, or an atomic number 1..86).  Coordinates are
+ * Angstrom unless coordinates_are_bohr.  An extended-XYZ Lattice="..." comment
+ * (a periodic cell) is rejected.  Errors: GFN1_ERR_INVALID_INPUT for the
+ * count, a short or missing line or an unknown element; GFN1_ERR_PARSE for a
+ * coordinate that is not a number (the message names the line). */
+typedef struct gfn1_xyz gfn1_xyz;
+int gfn1_system_from_xyz(const char* text, double charge, int32_t coordinates_are_bohr, gfn1_xyz** out,
+                         gfn1_error* err);
+/* The parsed system (positions in bohr); valid until gfn1_xyz_free. */
+const gfn1_system* gfn1_xyz_system(const gfn1_xyz* xyz);
+void gfn1_xyz_free(gfn1_xyz* xyz);
+
 /* In-scope subset of ElectronicOptions (src/electronic.rs:157-393) with the same
  * defaults (:395-452), plus HamiltonianOptions (src/hamiltonian.rs:16-33). */
 typedef struct gfn1_electronic_options {

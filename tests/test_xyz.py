@@ -1,0 +1,46 @@
+"""XYZ input through the C ABI (gfn1_system_from_xyz), the reader of
+PeriodicSystem::from_xyz_str (src/system.rs:45-99, symbol_to_z :154-165).
+Host-only parsing: runs without a GPU."""
+import numpy as np
+import pytest
+
+from gfn1_amd import NativeError, system_from_xyz, system_from_xyz_file
+
+A2B = 1.8897261246257702  # system.rs:8
+
+
+def test_reference_h2_text():
+    # the stretched-H2 input of tests/optimizer.rs:9-13
+    z, x, q = system_from_xyz("2\nH2 stretched\nH 0.000000 0.000000 0.000000\nH 1.100000 0.000000 0.000000\n", 0.0)
+    assert list(z) == [1, 1] and q == 0.0
+    assert np.array_equal(x, np.array([[0.0, 0.0, 0.0], [1.1 * A2B, 0.0, 0.0]]))
+
+
+def test_units_symbols_and_line_endings(tmp_path):
+    text = "3\r\nwater\r\no 0 0 0\r\nh 0.757 0.586 0\r\n1 -0.757 0.586 0\r\n"
+    z, x, q = system_from_xyz(text, -1.0, coordinates_are_bohr=True)
+    assert list(z) == [8, 1, 1] and q == -1.0
+    assert np.array_equal(x[1], [0.757, 0.586, 0.0])
+    z2, x2, _ = system_from_xyz("2\n\nHE 0 0 0\nCl 1 2 3 extra fields\n")
+    assert list(z2) == [2, 17] and np.allclose(x2[1], np.array([1.0, 2.0, 3.0]) * A2B, rtol=0, atol=0)
+    p = tmp_path / "m.xyz"
+    p.write_text("1\ncomment\nC 0.5 0 0\n")
+    zf, xf, _ = system_from_xyz_file(str(p))
+    assert list(zf) == [6] and xf[0, 0] == 0.5 * A2B
+
+
+@pytest.mark.parametrize("text,kind,fragment", [
+    ("", "InvalidInput", "empty XYZ"),
+    ("two\nc\n", "InvalidInput", "invalid XYZ atom count"),
+    ("2\nc\nH 0 0 0\n", "InvalidInput", "ended before atom 1"),
+    ("1\nc\nH 0 0\n", "InvalidInput", "line 3 has fewer than 4 fields"),
+    ("1\nc\nXx 0 0 0\n", "InvalidInput", "unknown element on line 3: Xx"),
+    ("1\nc\n87 0 0 0\n", "InvalidInput", "unknown element"),
+    ("1\nc\nH 0 zero 0\n", "Parse", "invalid floating point value: zero"),
+    ('1\nLattice="3 0 0 0 3 0 0 0 3" pbc="T T T"\nH 0 0 0\n', "InvalidInput", "periodic"),
+])
+def test_errors(text, kind, fragment):
+    with pytest.raises(NativeError) as e:
+        system_from_xyz(text)
+    assert e.value.kind == kind
+    assert fragment in str(e.value)
