@@ -209,3 +209,15 @@ def test_large_system_in_a_mixed_batch_matches_it_alone():
     want = opt_oracle.optimize_gfn1(zw, xw, scc_options=TIGHT, max_iterations=3)
     _compare("water_next_to_large", both[1], want)
 
+
+def test_batch_of_small_clusters_matches_oracle():
+    """Eight small mixed H2O/CH4/NH3 clusters (2-3 molecules, seeds 1000-1007),
+    15 L-BFGS iterations each, in one batch: every system's trajectory equals the
+    oracle's."""
+    systems = geometry.mixed_batch(8, first_seed=1000, kmin=2, kmax=3)
+    note("case 8 small clusters")
+    res = BatchEngine(systems, options=_tight_options()).optimize(GeometryOptimizationOptions(max_iterations=15))
+    for k, ((z, x), r) in enumerate(zip(systems, res)):
+        want = opt_oracle.optimize_gfn1(z, x, scc_options=TIGHT, max_iterations=15)
+        _compare("cluster%d(%d atoms)" % (k, len(z)), r, want)
+
