@@ -733,6 +733,17 @@ const gfn1_system* gfn1_xyz_system(const gfn1_xyz* x) { return x ? &x->sys : nul
 
 void gfn1_xyz_free(gfn1_xyz* x) { delete x; }
 
+int32_t gfn1_params_source(const gfn1_params* params, char* out, int32_t len) {
+    if (!params) return -1;
+    const std::string d = params->model.source_description();
+    if (out && len > 0) {
+        const size_t k = std::min((size_t)(len - 1), d.size());
+        std::memcpy(out, d.data(), k);
+        out[k] = 0;
+    }
+    return (int32_t)d.size();
+}
+
 void gfn1_hessian_options_default(gfn1_hessian_options* o) {
     if (!o) return;
     HessianOptions d;

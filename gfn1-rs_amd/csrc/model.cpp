@@ -228,6 +228,11 @@ Status Model::load(const char* spec, Model& m) {
             path = strip(env);
             m.source_ = path + " (via GFN1_XTB_PARAM)";
         }
+    } else if (lowercase(s) == "builtin:si" || lowercase(s) == "builtin:gfn1-si" || lowercase(s) == "builtin-si") {
+        // the bundled GFN1(Si)-xTB reparametrization (params.rs:192-196, 210)
+        path = m.data_dir_ + "/param_gfn1-si-xtb.txt";
+        m.source_ = "builtin";
+        m.source_kind_ = 2;
     } else if (lowercase(s) != "builtin" && lowercase(s) != "builtin:gfn1") {
         path = s;
         m.source_ = path;
@@ -235,6 +240,7 @@ Status Model::load(const char* spec, Model& m) {
     if (path.empty()) {
         path = m.data_dir_ + "/param_gfn1-xtb.txt";
         m.source_ = "builtin";
+        m.source_kind_ = 1;
     }
 
     std::ifstream f(path);
@@ -290,6 +296,12 @@ Status Model::load(const char* spec, Model& m) {
             while (vs >> v) vals.push_back(v);
             raw[z].kv[key] = vals;
         } else if (section == 1) {
+            // "key value" (params.rs:743-751); the name labels the parametrization
+            std::istringstream is(t);
+            std::string key, value;
+            is >> key;
+            std::getline(is, value);
+            if (lowercase(key) == "name") m.info_name_ = strip(value);
             continue;
         } else {
             return Status::error(kErrParse, "line " + std::to_string(line_no) + ": data outside a section");
@@ -440,3 +452,20 @@ Status Model::load(const char* spec, Model& m) {
 }
 
 }  // namespace gfn1
+
+namespace gfn1 {
+
+std::string Model::source_description() const {
+    const std::string name = info_name_.empty() ? "unnamed parametrization" : info_name_;
+    // the bundled files are xtb's, commit 2b5cd48 (LGPL-3.0-or-later)
+    const std::string where = source_kind_ ? "builtin, grimme-lab/xtb@2b5cd48, LGPL-3.0-or-later" : source_;
+    std::string refs;
+    if (source_kind_ == 1)
+        refs = "GFN1-xTB DOI 10.1021/acs.jctc.7b00118, Ln f-in-core DOI 10.1021/acs.inorgchem.7b01950";
+    else if (source_kind_ == 2)
+        refs = "GFN1-xTB DOI 10.1021/acs.jctc.7b00118, GFN1(Si)-xTB DOI 10.1021/acs.jcim.1c01170";
+    return refs.empty() ? name + " (" + where + ")" : name + " (" + where + "; refs: " + refs + ")";
+}
+
+}  // namespace gfn1
+

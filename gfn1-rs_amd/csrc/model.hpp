@@ -94,6 +94,9 @@ class Model {
     // split: returns the full factor for the two shells.
     double hscale(int za, int sa, int zb, int sb) const;
     const std::string& source() const { return source_; }
+    // Gfn1Parameters::source_description (params.rs:236-270): name, source and
+    // literature references of the loaded parametrization.
+    std::string source_description() const;
     const std::string& data_dir() const { return data_dir_; }
 
    private:
@@ -103,6 +106,8 @@ class Model {
     std::map<std::pair<int, int>, double> pairpar_;
     Globals glob_{};
     std::string source_, data_dir_;
+    std::string info_name_;
+    int source_kind_ = 0;  // 0 file, 1 bundled GFN1-xTB, 2 bundled GFN1(Si)-xTB
 };
 
 // Location of the bundled data directory (next to the shared library).

@@ -168,6 +168,7 @@ SIGNATURES = [
                                             ctypes.POINTER(Gfn1Error)]),
     ("gfn1_xyz_system", ctypes.POINTER(System), [_VP]),
     ("gfn1_xyz_free", None, [_VP]),
+    ("gfn1_params_source", ctypes.c_int32, [_VP, ctypes.c_char_p, ctypes.c_int32]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

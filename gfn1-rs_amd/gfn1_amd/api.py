@@ -37,6 +37,19 @@ class Gfn1Parameters:
     def builtin(cls):
         return cls.resolve("builtin")
 
+    @classmethod
+    def builtin_si(cls):
+        """The bundled GFN1(Si)-xTB reparametrization (params.rs:192-196)."""
+        return cls.resolve("builtin:si")
+
+    def source_description(self) -> str:
+        """Name, source and references (params.rs:236-270)."""
+        lib = _lib.load()
+        n = lib.gfn1_params_source(self._h, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib.gfn1_params_source(self._h, buf, n + 1)
+        return buf.value.decode()
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
             _lib._lib.gfn1_params_free(self._h)
@@ -618,6 +631,13 @@ class Gfn1NativeCalculator:
             electronic_temperature=electronic_temperature, nprim=nprim, eigen_tolerance=eigen_tolerance,
             enable_dispersion=enable_dispersion, enable_cn_hamiltonian=enable_cn_hamiltonian,
             level_shift=level_shift)
+
+    def param_source(self) -> str:
+        """python.rs:328-330."""
+        return self.params.source_description()
+
+    def __repr__(self):
+        return "Gfn1NativeCalculator(parameters: %s)" % self.params.source_description()
 
     def calculate(self, numbers, positions, unit="angstrom", compute_gradient=False) -> CalculationResult:
         unit_l = unit.lower()

@@ -47,10 +47,15 @@ typedef struct gfn1_error {
 typedef struct gfn1_params gfn1_params;
 
 /* Gfn1Parameters::resolve (src/params.rs:204-230).  spec: NULL or "" ->
- * $GFN1_XTB_PARAM, else the bundled GFN1-xTB file; "builtin" -> bundled file;
- * anything else is a parameter-file path. */
+ * $GFN1_XTB_PARAM, else the bundled GFN1-xTB file; "builtin" / "builtin:gfn1"
+ * -> the bundled GFN1-xTB file; "builtin:si" / "builtin:gfn1-si" / "builtin-si"
+ * -> the bundled GFN1(Si)-xTB file; anything else is a parameter-file path. */
 int gfn1_params_load(const char* spec, gfn1_params** out, gfn1_error* err);
 void gfn1_params_free(gfn1_params* params);
+/* Gfn1Parameters::source_description (src/params.rs:236-270): writes at most
+ * len - 1 characters and a terminating NUL into out (nullable when len is 0);
+ * returns the full length of the description. */
+int32_t gfn1_params_source(const gfn1_params* params, char* out, int32_t len);
 /* Number of shells of element z in the loaded parameters (-1 if absent).  The
  * shell layout of a system is its atoms' shells in input order
  * (src/basis.rs:121-233). */

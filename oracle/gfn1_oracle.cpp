@@ -1076,11 +1076,13 @@ void oracle_default_options(oracle_options* o) {
     o->include_halogen = 1;
 }
 
-void* oracle_create(const char* data_dir) {
+// param_file: NULL for the bundled GFN1-xTB file of data_dir
+void* oracle_create_params(const char* data_dir, const char* param_file) {
     Ctx* c = new Ctx();
     Err err{0, ""};
     std::string d(data_dir);
-    if (!parse_params(d + "/param_gfn1-xtb.txt", c->p, err) || !load_sto(d + "/sto_ng.txt", c->sto, err) ||
+    const std::string pf = param_file ? std::string(param_file) : d + "/param_gfn1-xtb.txt";
+    if (!parse_params(pf, c->p, err) || !load_sto(d + "/sto_ng.txt", c->sto, err) ||
         !load_d3(d + "/d3_reference.bin", *c, err)) {
         std::fprintf(stderr, "oracle_create: %s\n", err.msg.c_str());
         delete c;
@@ -1088,6 +1090,8 @@ void* oracle_create(const char* data_dir) {
     }
     return c;
 }
+
+void* oracle_create(const char* data_dir) { return oracle_create_params(data_dir, nullptr); }
 
 void oracle_destroy(void* c) { delete (Ctx*)c; }
 

@@ -117,9 +117,23 @@ def lib():
     return _lib
 
 
-def ctx():
+_param_ctx = {}
+
+
+def ctx(param_file=None):
+    """The oracle context of the bundled GFN1-xTB parameters, or of another
+    parameter file (one context per file, kept for the process)."""
     lib()
-    return _ctx
+    if param_file is None:
+        return _ctx
+    if param_file not in _param_ctx:
+        _lib.oracle_create_params.restype = ctypes.c_void_p
+        _lib.oracle_create_params.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+        c = _lib.oracle_create_params(DATA_DIR.encode(), param_file.encode())
+        if not c:
+            raise RuntimeError("oracle_create_params failed for %s" % param_file)
+        _param_ctx[param_file] = c
+    return _param_ctx[param_file]
 
 
 def _ptr(a):
@@ -134,21 +148,22 @@ def default_options(**kw):
     return o
 
 
-def basis_size(numbers):
+def basis_size(numbers, param_file=None):
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     nsh, nao = ctypes.c_int(), ctypes.c_int()
-    rc = lib().oracle_basis_size(ctx(), len(z), _ptr(z), ctypes.byref(nsh), ctypes.byref(nao))
+    rc = lib().oracle_basis_size(ctx(param_file), len(z), _ptr(z), ctypes.byref(nsh), ctypes.byref(nao))
     if rc:
         raise ValueError("basis build failed (%d)" % rc)
     return nsh.value, nao.value
 
 
-def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False, **opts):
-    """Energy (+ gradient) of one system; returns a dict of numpy arrays."""
+def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False, param_file=None, **opts):
+    """Energy (+ gradient) of one system; returns a dict of numpy arrays.
+    param_file: another GFN1 parameter file (default: the bundled one)."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions_bohr, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
-    nsh, nao = basis_size(z)
+    nsh, nao = basis_size(z, param_file)
     o = default_options(**opts)
     out = {
         "gradient": np.zeros((nat, 3)), "electronic_gradient": np.zeros((nat, 3)),
@@ -168,7 +183,7 @@ def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False
     if initial_shell_charges is not None:
         q0 = np.ascontiguousarray(initial_shell_charges, dtype=np.float64)
         assert q0.shape == (nsh,)
-    rc = lib().oracle_run(ctx(), nat, _ptr(z), _ptr(x), ctypes.byref(o),
+    rc = lib().oracle_run(ctx(param_file), nat, _ptr(z), _ptr(x), ctypes.byref(o),
                           _ptr(q0) if q0 is not None else None, ctypes.byref(r))
     out["status"] = rc
     out["message"] = r.message.decode()

@@ -93,6 +93,29 @@ def test_water70_matches_oracle():
     assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
 
 
+def test_gfn1_si_parameters_match_oracle():
+    """The bundled GFN1(Si)-xTB set ("builtin:si", params.rs:192-210) on
+    silanol, whose Si element block and O-Si pair scaling differ from GFN1-xTB."""
+    import os
+
+    from gfn1_amd import Gfn1Parameters
+    note("case GFN1(Si)-xTB silanol")
+    z = [14, 8, 1, 1, 1, 1]
+    x = np.array([[0.0, 0.0, 0.0], [1.65, 0.0, 0.0], [2.100693, 0.84763, 0.0], [-0.494034, 1.395109, 0.0],
+                  [-0.494034, -0.697555, 1.2082], [-0.494034, -0.697555, -1.2082]]) * geometry.ANGSTROM_TO_BOHR
+    si_file = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gfn1-rs_amd", "data",
+                           "param_gfn1-si-xtb.txt")
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))
+    res = analytic_gradient(z, x, params=Gfn1Parameters.builtin_si(), options=opts)
+    ref = oracle.run(z, x, param_file=si_file, **TIGHT)
+    note("silanol GFN1(Si): dE %.2e max|dg| %.2e" % (abs(res.total_energy - ref["energy"]),
+                                                    float(np.abs(res.gradient - ref["gradient"]).max())))
+    assert abs(res.total_energy - ref["energy"]) <= E_TOL
+    assert np.abs(res.gradient - ref["gradient"]).max() <= G_TOL
+    base = analytic_gradient(z, x, options=opts)
+    assert abs(base.total_energy - res.total_energy) > 1e-3
+
+
 def test_fallback_ladder_matches_oracle():
     note("case fallback ladder")
     z, x = mol("water")
