@@ -207,6 +207,16 @@ int gfn1_params_load(const char* spec, gfn1_params** out, gfn1_error* err) {
     return GFN1_OK;
 }
 
+int gfn1_params_from_text(const char* text, gfn1_params** out, gfn1_error* err) {
+    clear(err);
+    if (!out || !text) return fail(err, Status::error(kErrInvalidInput, "null argument"));
+    auto p = std::make_unique<gfn1_params>();
+    Status st = Model::load_text(text, p->model);
+    if (!st.ok()) return fail(err, st);
+    *out = p.release();
+    return GFN1_OK;
+}
+
 void gfn1_params_free(gfn1_params* params) { delete params; }
 
 void gfn1_electronic_options_default(gfn1_electronic_options* o) {

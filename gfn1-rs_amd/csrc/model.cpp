@@ -217,12 +217,22 @@ double Model::hscale(int za, int sa, int zb, int sb) const {
     return glob_.kdiff;
 }
 
-Status Model::load(const char* spec, Model& m) {
+Status Model::load(const char* spec, Model& m) { return load_impl(spec, nullptr, m); }
+
+// Gfn1Parameters::from_str (params.rs:272): parameters from text, no file identity.
+Status Model::load_text(const char* text, Model& m) { return load_impl(nullptr, text ? text : "", m); }
+
+Status Model::load_impl(const char* spec, const char* text, Model& m) {
     m = Model();
     m.data_dir_ = default_data_dir();
     std::string path;
+    const std::string inline_text = text ? text : "";
     std::string s = spec ? strip(spec) : "";
-    if (s.empty()) {
+    if (text) {
+        m.source_ = "inline text";
+        m.source_kind_ = 3;
+        path = "(inline text)";
+    } else if (s.empty()) {
         const char* env = std::getenv("GFN1_XTB_PARAM");
         if (env && *env) {
             path = strip(env);
@@ -243,8 +253,15 @@ Status Model::load(const char* spec, Model& m) {
         m.source_kind_ = 1;
     }
 
-    std::ifstream f(path);
-    if (!f) return Status::error(kErrIo, "cannot read parameter file " + path);
+    std::ifstream file;
+    std::istringstream text_in;
+    if (m.source_kind_ == 3) {
+        text_in.str(inline_text);
+    } else {
+        file.open(path);
+        if (!file) return Status::error(kErrIo, "cannot read parameter file " + path);
+    }
+    std::istream& f = m.source_kind_ == 3 ? static_cast<std::istream&>(text_in) : static_cast<std::istream&>(file);
     std::map<std::string, double> glob;
     std::map<int, RawElement> raw;
     int section = 0;  // 1 info, 2 globpar, 3 pairpar, 4 element
@@ -458,7 +475,8 @@ namespace gfn1 {
 std::string Model::source_description() const {
     const std::string name = info_name_.empty() ? "unnamed parametrization" : info_name_;
     // the bundled files are xtb's, commit 2b5cd48 (LGPL-3.0-or-later)
-    const std::string where = source_kind_ ? "builtin, grimme-lab/xtb@2b5cd48, LGPL-3.0-or-later" : source_;
+    const std::string where =
+        (source_kind_ == 1 || source_kind_ == 2) ? "builtin, grimme-lab/xtb@2b5cd48, LGPL-3.0-or-later" : source_;
     std::string refs;
     if (source_kind_ == 1)
         refs = "GFN1-xTB DOI 10.1021/acs.jctc.7b00118, Ln f-in-core DOI 10.1021/acs.inorgchem.7b01950";

@@ -82,6 +82,7 @@ class Model {
     // spec: nullptr/"" -> $GFN1_XTB_PARAM or the bundled file; "builtin" ->
     // bundled; otherwise a path (params.rs:204-230).
     static Status load(const char* spec, Model& out);
+    static Status load_text(const char* text, Model& out);
 
     const ShellType& shell(int z, int s) const { return shells_[z * kMaxShellsPerAtom + s]; }
     const ElementType& element(int z) const { return elements_[z]; }
@@ -107,7 +108,8 @@ class Model {
     Globals glob_{};
     std::string source_, data_dir_;
     std::string info_name_;
-    int source_kind_ = 0;  // 0 file, 1 bundled GFN1-xTB, 2 bundled GFN1(Si)-xTB
+    int source_kind_ = 0;  // 0 file, 1 bundled GFN1-xTB, 2 bundled GFN1(Si)-xTB, 3 inline text
+    static Status load_impl(const char* spec, const char* text, Model& out);
 };
 
 // Location of the bundled data directory (next to the shared library).

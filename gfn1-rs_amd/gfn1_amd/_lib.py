@@ -169,6 +169,7 @@ SIGNATURES = [
     ("gfn1_xyz_system", ctypes.POINTER(System), [_VP]),
     ("gfn1_xyz_free", None, [_VP]),
     ("gfn1_params_source", ctypes.c_int32, [_VP, ctypes.c_char_p, ctypes.c_int32]),
+    ("gfn1_params_from_text", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),
     ("gfn1_version", ctypes.c_char_p, []),
 ]
 

@@ -38,6 +38,15 @@ class Gfn1Parameters:
         return cls.resolve("builtin")
 
     @classmethod
+    def from_str(cls, text: str):
+        """Gfn1Parameters::from_str (params.rs:272): the text of a parameter file."""
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        err = _lib.Gfn1Error()
+        _lib.check(lib.gfn1_params_from_text(text.encode(), ctypes.byref(h), ctypes.byref(err)), err)
+        return cls(h)
+
+    @classmethod
     def builtin_si(cls):
         """The bundled GFN1(Si)-xTB reparametrization (params.rs:192-196)."""
         return cls.resolve("builtin:si")

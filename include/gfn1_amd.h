@@ -51,6 +51,9 @@ typedef struct gfn1_params gfn1_params;
  * -> the bundled GFN1-xTB file; "builtin:si" / "builtin:gfn1-si" / "builtin-si"
  * -> the bundled GFN1(Si)-xTB file; anything else is a parameter-file path. */
 int gfn1_params_load(const char* spec, gfn1_params** out, gfn1_error* err);
+/* Gfn1Parameters::from_str (src/params.rs:272): parameters from the text of a
+ * parameter file (source "inline text"). */
+int gfn1_params_from_text(const char* text, gfn1_params** out, gfn1_error* err);
 void gfn1_params_free(gfn1_params* params);
 /* Gfn1Parameters::source_description (src/params.rs:236-270): writes at most
  * len - 1 characters and a terminating NUL into out (nullable when len is 0);

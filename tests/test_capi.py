@@ -84,3 +84,12 @@ def test_parameter_sources_and_descriptions(tmp_path, monkeypatch):
     assert Gfn1Parameters.resolve("builtin").source_description() == d
     monkeypatch.delenv("GFN1_XTB_PARAM")
     assert "GFN1-xTB (builtin" in repr(Gfn1NativeCalculator())
+    # Gfn1Parameters::from_str: the text of the bundled file, source "inline text"
+    with open(src) as f:
+        inline = Gfn1Parameters.from_str(f.read())
+    assert inline.source_description() == "GFN1-xTB (inline text)"
+    from gfn1_amd import NativeError
+    with pytest.raises(NativeError) as e:
+        Gfn1Parameters.from_str("$globpar\nks not-a-number\n")
+    assert e.value.kind in ("Parse", "MissingGlobal")
+
