@@ -117,13 +117,14 @@ def parity(eng, systems, count):
 
 SINGLE_WORKLOADS = {
     # name: (molecules, cube side in Angstrom, seed)
+    "water48": (48, 11.3, 0),
     "water1000": (1000, 31.0, 1),
     "water2000": (2000, 39.1, 4),
 }
 
 
 def run_single(name, precisions):
-    """One large water cluster (energy + gradient, 300 K, SCC tolerance 1e-7 Eh)
+    """One water cluster (energy + gradient, 300 K, SCC tolerance 1e-7 Eh)
     through the same engine: wall time per energy+gradient, iterations and the
     size-independent checks (translation invariance of the gradient, total
     charge); the mixed path is compared with the FP64 path on the same system.
@@ -163,6 +164,15 @@ def run_single(name, precisions):
             r["max_abs_dg_vs_fp64"] = float(np.abs(g - ref[1]).max())
         out[prec] = r
         print("%s %s: %.2f s, %d iterations" % (name, prec, dt, it), file=sys.stderr, flush=True)
+    if nao <= 1000:
+        # small enough for the host oracle (one core): its time and the parity of the FP64 result
+        import oracle
+        t = time.perf_counter()
+        ref_o = oracle.run(z, x, energy_tolerance=1e-7)
+        out["cpu_baseline"] = {"seconds": time.perf_counter() - t, "cores": 1, "kind": "port",
+                               "sample": "the same system, FP64 C++ oracle"}
+        if "fp64" in out:
+            out["fp64"]["abs_dE_vs_oracle"] = abs(out["fp64"]["energy"] - ref_o["energy"])
     print(json.dumps(out))
 
 
