@@ -490,6 +490,15 @@ int gfn1_batch_set_positions(gfn1_batch* b, const double* positions, gfn1_error*
     return GFN1_OK;
 }
 
+int gfn1_batch_set_start_charges(gfn1_batch* b, int32_t inst, const double* q, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch || inst < 0 || inst >= b->batch->size())
+        return fail(err, Status::error(kErrInvalidInput, "invalid batch instance"));
+    Status st = b->batch->set_start_charges(inst, q);
+    if (!st.ok()) return fail(err, st);
+    return GFN1_OK;
+}
+
 void gfn1_optimization_options_default(gfn1_optimization_options* o) {
     if (!o) return;
     OptimizerOptions d;

@@ -12,6 +12,7 @@ Python surface of the C ABI in include/gfn1_amd.h:
   hessian(...), hessian_batch   analytic_hessian(...).hessian, hessian/mod.rs:112 (central differences
                                 of the analytic gradient on the device)
   vibrational_analysis(...)     vibrational.rs:37
+  md.run_md(...)                batched velocity-Verlet dynamics over BatchEngine
   ir_spectrum_from(...)         properties.rs:381 (with the dipole derivatives of hessian(...))
 """
 from .api import (  # noqa: F401

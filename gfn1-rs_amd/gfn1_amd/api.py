@@ -892,6 +892,7 @@ class BatchEngine:
             arr[i] = sc
             self._keep.append(keep)
             self.natoms.append(len(keep[0]))
+        self.numbers = [np.asarray(k[0], dtype=np.int32) for k in self._keep]
         self._h = ctypes.c_void_p()
         err = _lib.Gfn1Error()
         _lib.check(lib.gfn1_batch_create(len(systems), arr, self.params._h, ctypes.byref(bo), ctypes.byref(self._h),
@@ -946,6 +947,14 @@ class BatchEngine:
             raise ValueError("positions do not match the batch's atom counts")
         err = _lib.Gfn1Error()
         _lib.check(self._lib.gfn1_batch_set_positions(self._h, x.ctypes.data_as(_lib._DP), ctypes.byref(err)), err)
+
+    def set_start_charges(self, i, shell_charges):
+        """Start the next SCCs of system i from these shell charges (the
+        reference's scc_initial_shell_charges hook, electronic.rs:386-392)."""
+        q = np.ascontiguousarray(shell_charges, dtype=np.float64)
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_set_start_charges(self._h, int(i), q.ctypes.data_as(_lib._DP),
+                                                          ctypes.byref(err)), err)
 
     def optimize(self, options: Optional["GeometryOptimizationOptions"] = None):
         """optimize_geometry (optimizer.rs:78) of every system at once; one

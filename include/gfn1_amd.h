@@ -295,6 +295,12 @@ int gfn1_batch_set_guess(gfn1_batch* batch, const gfn1_guess_model* model, gfn1_
  * geometry (src/optimizer.rs:316-322). */
 int gfn1_batch_set_positions(gfn1_batch* batch, const double* positions, gfn1_error* err);
 
+/* Start shell charges (nsh values) of one instance for the batch's next runs,
+ * through the reference's scc_initial_shell_charges hook
+ * (src/electronic.rs:386-392, :1231-1239); for drivers that start each SCC
+ * from the previous geometry's charges.  No reference counterpart as a call. */
+int gfn1_batch_set_start_charges(gfn1_batch* batch, int32_t inst, const double* shell_charges, gfn1_error* err);
+
 /* --- Geometry optimisation ---
  * optimize_geometry (src/optimizer.rs:78-185): L-BFGS on the total free energy
  * and the analytic gradient, Armijo backtracking line search, per-atom step
