@@ -233,6 +233,33 @@ def run_optimize(args):
     print(json.dumps(out))
 
 
+def run_md_workload(args):
+    """Batched NVE molecular dynamics (velocity Verlet, 0.5 fs, 300 K
+    Maxwell-Boltzmann start velocities) of the clusters of the batch, FP64,
+    each SCC warm-started from the previous step's charges: system-steps per
+    second.  Prints one JSON line (not the batched headline metric)."""
+    from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, geometry
+    from gfn1_amd.md import run_md
+    systems = geometry.mixed_batch(args.batch, first_seed=1000)
+    eng = BatchEngine(systems, options=AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-7)))
+    eng.run()  # untimed warm-up
+    t = time.perf_counter()
+    res = run_md(eng, [x for _, x in systems], args.md_steps, timestep_fs=0.5, temperature=300.0, seed=1)
+    dt = time.perf_counter() - t
+    ok = [i for i, s in enumerate(res.status) if s == 0]
+    drifts = np.abs(res.total[:, ok] - res.total[0, ok]).max(axis=0) if ok else None
+    out = {"metric": "MD system-steps/s", "unit": "system-steps/s", "higher_is_better": True, "n_gpus": 1,
+           "data": "synthetic", "value": len(ok) * args.md_steps / dt, "seconds": dt, "steps": args.md_steps,
+           "systems": len(systems), "failed": len(systems) - len(ok),
+           "total_energy_drift": None if drifts is None else {"median": float(np.median(drifts)),
+                                                               "max": float(drifts.max())},
+           "config": {"workload": "NVE velocity Verlet of mixed H2O/CH4/NH3 clusters (seeds 1000+i, 2-40 molecules), "
+                                  "0.5 fs, 300 K start, FP64, SCC energy tolerance 1e-7, warm-started SCCs",
+                      "global_batch": len(systems)}}
+    print("md: %d systems x %d steps in %.2f s" % (len(systems), args.md_steps, dt), file=sys.stderr, flush=True)
+    print(json.dumps(out))
+
+
 def run_hessian(args):
     """Cartesian Hessians of the first clusters of the batch (central
     differences of the analytic gradient, step 1e-4 bohr, SCC tolerances
@@ -288,7 +315,8 @@ def main():
     ap.add_argument("--opt-cpu-sample", type=int, default=1,
                     help="clusters optimised by the host oracle for the optimize workload (0 = skip)")
     ap.add_argument("--hess-systems", type=int, default=32, help="clusters of the hessian workload")
-    ap.add_argument("--workload", choices=["batch", "optimize", "hessian"] + sorted(SINGLE_WORKLOADS),
+    ap.add_argument("--md-steps", type=int, default=10, help="steps of the md workload")
+    ap.add_argument("--workload", choices=["batch", "optimize", "hessian", "md"] + sorted(SINGLE_WORKLOADS),
                     default="batch",
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
@@ -299,11 +327,11 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload in ("optimize", "hessian"):
+    if args.workload in ("optimize", "hessian", "md"):
         if args.gpus != 1:
             print("bench.py measures one GPU (--gpus %d refused)" % args.gpus, file=sys.stderr)
             sys.exit(2)
-        (run_optimize if args.workload == "optimize" else run_hessian)(args)
+        {"optimize": run_optimize, "hessian": run_hessian, "md": run_md_workload}[args.workload](args)
         return
     if args.workload != "batch":
         if args.gpus != 1:

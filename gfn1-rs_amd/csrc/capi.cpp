@@ -490,6 +490,52 @@ int gfn1_batch_set_positions(gfn1_batch* b, const double* positions, gfn1_error*
     return GFN1_OK;
 }
 
+int gfn1_batch_statuses(const gfn1_batch* b, int32_t* out) {
+    if (!b || !b->ready() || !out) return GFN1_ERR_INVALID_INPUT;
+    for (size_t i = 0; i < b->results.size(); ++i) out[i] = b->results[i].status;
+    return GFN1_OK;
+}
+
+int gfn1_batch_energies(const gfn1_batch* b, double* out) {
+    if (!b || !b->ready() || !out) return GFN1_ERR_INVALID_INPUT;
+    for (size_t i = 0; i < b->results.size(); ++i)
+        out[i] = b->results[i].status == kOk ? b->results[i].energies[0] : NAN;
+    return GFN1_OK;
+}
+
+int gfn1_batch_gradients(const gfn1_batch* b, double* out) {
+    if (!b || !b->ready() || !out) return GFN1_ERR_INVALID_INPUT;
+    for (const InstanceResult& r : b->results) {
+        if (r.status == kOk) std::copy(r.gradient.begin(), r.gradient.end(), out);
+        else std::fill(out, out + 3 * (size_t)r.nat, NAN);
+        out += 3 * (size_t)r.nat;
+    }
+    return GFN1_OK;
+}
+
+int gfn1_batch_all_shell_charges(const gfn1_batch* b, double* out) {
+    if (!b || !b->ready() || !out) return GFN1_ERR_INVALID_INPUT;
+    for (size_t i = 0; i < b->results.size(); ++i) {
+        const InstanceResult& r = b->results[i];
+        const size_t nsh = (size_t)b->batch->shell_count((int)i);
+        if (r.status == kOk) std::copy(r.shell_charges.begin(), r.shell_charges.end(), out);
+        else std::fill(out, out + nsh, NAN);
+        out += nsh;
+    }
+    return GFN1_OK;
+}
+
+int gfn1_batch_set_all_start_charges(gfn1_batch* b, const double* q, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch || !q) return fail(err, Status::error(kErrInvalidInput, "invalid start charges"));
+    for (int i = 0; i < b->batch->size(); ++i) {
+        Status st = b->batch->set_start_charges(i, q);
+        if (!st.ok()) return fail(err, st);
+        q += b->batch->shell_count(i);
+    }
+    return GFN1_OK;
+}
+
 int gfn1_batch_set_start_charges(gfn1_batch* b, int32_t inst, const double* q, gfn1_error* err) {
     clear(err);
     if (!b || !b->batch || inst < 0 || inst >= b->batch->size())

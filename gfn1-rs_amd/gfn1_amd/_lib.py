@@ -139,6 +139,11 @@ SIGNATURES = [
     ("gfn1_batch_set_guess", ctypes.c_int, [_VP, _VP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_set_positions", ctypes.c_int, [_VP, _DP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_set_start_charges", ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_batch_statuses", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int32)]),
+    ("gfn1_batch_energies", ctypes.c_int, [_VP, _DP]),
+    ("gfn1_batch_gradients", ctypes.c_int, [_VP, _DP]),
+    ("gfn1_batch_all_shell_charges", ctypes.c_int, [_VP, _DP]),
+    ("gfn1_batch_set_all_start_charges", ctypes.c_int, [_VP, _DP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_optimization_options_default", None, [ctypes.POINTER(OptimizationOptions)]),
     ("gfn1_batch_optimize", ctypes.c_int, [_VP, ctypes.POINTER(OptimizationOptions), ctypes.POINTER(_VP),
                                            ctypes.POINTER(Gfn1Error)]),

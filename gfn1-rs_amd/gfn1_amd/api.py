@@ -948,6 +948,35 @@ class BatchEngine:
         err = _lib.Gfn1Error()
         _lib.check(self._lib.gfn1_batch_set_positions(self._h, x.ctypes.data_as(_lib._DP), ctypes.byref(err)), err)
 
+    def results(self):
+        """All systems at once: statuses (n), energies (n, NaN where failed),
+        gradients (list of (nat, 3)) and shell charges (list of (nsh,))."""
+        lib = self._lib
+        if not hasattr(self, "_nsh"):
+            self._nsh = [basis_counts(z, self.params)[0] if len(z) else 0 for z in self.numbers]
+        st = np.zeros(self.n, dtype=np.int32)
+        e = np.zeros(self.n)
+        g = np.zeros((sum(self.natoms), 3))
+        q = np.zeros(sum(self._nsh))
+        if self.n:
+            for rc in (lib.gfn1_batch_statuses(self._h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                       lib.gfn1_batch_energies(self._h, e.ctypes.data_as(_lib._DP)),
+                       lib.gfn1_batch_gradients(self._h, g.ctypes.data_as(_lib._DP)),
+                       lib.gfn1_batch_all_shell_charges(self._h, q.ctypes.data_as(_lib._DP))):
+                if rc != 0:
+                    raise RuntimeError("batch results are not available (run the batch first)")
+        ga = np.cumsum([0] + self.natoms)
+        qa = np.cumsum([0] + self._nsh)
+        return (st, e, [g[ga[i]:ga[i + 1]] for i in range(self.n)], [q[qa[i]:qa[i + 1]] for i in range(self.n)])
+
+    def set_all_start_charges(self, shell_charges):
+        """Start charges of every system (one (nsh,) array each) for the next runs."""
+        q = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64) for v in shell_charges])
+                                 if self.n else np.zeros(0))
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_set_all_start_charges(self._h, q.ctypes.data_as(_lib._DP),
+                                                              ctypes.byref(err)), err)
+
     def set_start_charges(self, i, shell_charges):
         """Start the next SCCs of system i from these shell charges (the
         reference's scc_initial_shell_charges hook, electronic.rs:386-392)."""

@@ -83,20 +83,23 @@ def run_md(engine: BatchEngine, positions, steps, timestep_fs=0.5, velocities=No
     status = [0] * n
     times, pot, kin, traj = [], [], [], [[] for _ in range(n)]
 
+    q_start = None
+
     def evaluate():
+        nonlocal q_start
         engine.set_positions(x)
         engine.run()
-        e = np.full(n, np.nan)
-        g = [None] * n
+        st, e, g, q = engine.results()
         for i in range(n):
             if status[i] == 0:
-                status[i] = engine.status(i)
+                status[i] = int(st[i])
             if status[i] != 0:
-                continue
-            e[i] = engine.energy(i)
-            g[i] = engine.gradient(i)
-            if warm_start and nsh[i]:
-                engine.set_start_charges(i, engine.shell_charges(i, nsh[i]))
+                e[i] = np.nan
+        if warm_start:
+            # a failed system keeps the charges it last converged to
+            q_start = [q[i] if status[i] == 0 else (q_start[i] if q_start is not None else np.zeros(nsh[i]))
+                       for i in range(n)]
+            engine.set_all_start_charges(q_start)
         return e, g
 
     def record(step, e):

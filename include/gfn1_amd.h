@@ -295,6 +295,16 @@ int gfn1_batch_set_guess(gfn1_batch* batch, const gfn1_guess_model* model, gfn1_
  * geometry (src/optimizer.rs:316-322). */
 int gfn1_batch_set_positions(gfn1_batch* batch, const double* positions, gfn1_error* err);
 
+/* All instances at once, systems in batch order: statuses (n), total free
+ * energies (n; NaN where the status is not GFN1_OK), gradients (3 nat each) and
+ * converged shell charges (nsh each), after gfn1_batch_run. */
+int gfn1_batch_statuses(const gfn1_batch* batch, int32_t* out);
+int gfn1_batch_energies(const gfn1_batch* batch, double* out);
+int gfn1_batch_gradients(const gfn1_batch* batch, double* out);
+int gfn1_batch_all_shell_charges(const gfn1_batch* batch, double* out);
+/* Start shell charges of every instance (nsh each, batch order) for the next runs. */
+int gfn1_batch_set_all_start_charges(gfn1_batch* batch, const double* shell_charges, gfn1_error* err);
+
 /* Start shell charges (nsh values) of one instance for the batch's next runs,
  * through the reference's scc_initial_shell_charges hook
  * (src/electronic.rs:386-392, :1231-1239); for drivers that start each SCC
