@@ -15,6 +15,7 @@
 #include "engine.hpp"
 #include "model.hpp"
 #include "hessian.hpp"
+#include "md.hpp"
 #include "optimizer.hpp"
 
 using namespace gfn1;
@@ -542,6 +543,50 @@ int gfn1_batch_set_start_charges(gfn1_batch* b, int32_t inst, const double* q, g
         return fail(err, Status::error(kErrInvalidInput, "invalid batch instance"));
     Status st = b->batch->set_start_charges(inst, q);
     if (!st.ok()) return fail(err, st);
+    return GFN1_OK;
+}
+
+int gfn1_batch_positions(const gfn1_batch* b, double* out) {
+    if (!b || !b->batch || (!out && b->batch->natoms() > 0)) return GFN1_ERR_INVALID_INPUT;
+    if (b->batch->natoms() > 0) b->batch->positions(out);
+    return GFN1_OK;
+}
+
+void gfn1_md_options_default(gfn1_md_options* o) {
+    if (!o) return;
+    MdOptions d;
+    o->timestep_fs = d.timestep_fs;
+    o->steps = d.steps;
+    o->warm_start = d.warm_start;
+}
+
+int gfn1_batch_md(gfn1_batch* b, const gfn1_md_options* options, double* velocities, double* potential,
+                  double* kinetic, int32_t* status, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch || (!velocities && b->batch->natoms() > 0))
+        return fail(err, Status::error(kErrInvalidInput, "invalid MD arguments"));
+    if (b->learned_guess && !b->guess_set)
+        return fail(err, Status::error(kErrInvalidInput, "learned charge guess selected but no model set"));
+    gfn1_md_options def;
+    if (!options) {
+        gfn1_md_options_default(&def);
+        options = &def;
+    }
+    MdOptions o;
+    o.timestep_fs = options->timestep_fs;
+    o.steps = options->steps;
+    o.warm_start = options->warm_start;
+    const size_t nv = 3 * (size_t)b->batch->natoms();
+    std::vector<double> v(velocities, velocities + nv), pot, kin;
+    std::vector<int> st_out;
+    b->ran = false;
+    b->downloaded = false;
+    Status st = md_batch(*b->batch, o, v, pot, kin, st_out);
+    if (!st.ok()) return fail(err, st);
+    std::copy(v.begin(), v.end(), velocities);
+    if (potential) std::copy(pot.begin(), pot.end(), potential);
+    if (kinetic) std::copy(kin.begin(), kin.end(), kinetic);
+    if (status) std::copy(st_out.begin(), st_out.end(), status);
     return GFN1_OK;
 }
 

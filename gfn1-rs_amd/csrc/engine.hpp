@@ -117,6 +117,7 @@ class Batch {
     // the charges of the last run, as the full download computes it.
     Status fetch_dipoles(std::vector<double>& dipoles);
     int natoms() const { return natoms_; }
+    int atom_number(int atom) const { return h_atom_z_[atom]; }  // Z of an atom (batch-wide index)
     int atom_offset(int inst) const { return h_atom0_[inst]; }   // inst in [0, size()]
     int shell_count(int inst) const { return h_nsh_[inst]; }
     int shell_offset(int inst) const { return h_sh0_[inst]; }    // into fetch_energy_gradient's shell charges

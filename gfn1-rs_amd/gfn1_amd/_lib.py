@@ -74,6 +74,10 @@ class HessianOptions(ctypes.Structure):
                 ("include_halogen", ctypes.c_int32), ("warm_start", ctypes.c_int32), ("max_batch", ctypes.c_int32)]
 
 
+class MdOptions(ctypes.Structure):
+    _fields_ = [("timestep_fs", ctypes.c_double), ("steps", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 _VP = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -140,6 +144,10 @@ SIGNATURES = [
     ("gfn1_batch_set_positions", ctypes.c_int, [_VP, _DP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_set_start_charges", ctypes.c_int, [_VP, ctypes.c_int32, _DP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_statuses", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int32)]),
+    ("gfn1_batch_positions", ctypes.c_int, [_VP, _DP]),
+    ("gfn1_md_options_default", None, [ctypes.POINTER(MdOptions)]),
+    ("gfn1_batch_md", ctypes.c_int, [_VP, ctypes.POINTER(MdOptions), _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_energies", ctypes.c_int, [_VP, _DP]),
     ("gfn1_batch_gradients", ctypes.c_int, [_VP, _DP]),
     ("gfn1_batch_all_shell_charges", ctypes.c_int, [_VP, _DP]),

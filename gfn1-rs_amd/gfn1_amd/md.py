@@ -4,18 +4,19 @@ The reference drives molecular dynamics through ASE (its calculator supplies
 energies and forces to ``ase.md``; scripts/run_mgo_kpoint_md.py).  Here every
 system of a batch advances together: each step is one batched energy +
 gradient pass on the device, and the integration is host work on 3 nat
-vectors.  Each SCC can start from the previous step's converged charges
-through the reference's ``scc_initial_shell_charges`` hook.
+vectors (the C++ driver behind gfn1_batch_md).  Each SCC can start from the
+previous step's converged charges through the reference's
+``scc_initial_shell_charges`` hook.
 
 Units: positions bohr, velocities bohr per atomic time unit, energies Eh,
 time step in femtoseconds.
 """
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional
 
 import numpy as np
 
-from .api import ATOMIC_MASSES, BatchEngine, basis_counts
+from .api import ATOMIC_MASSES, BatchEngine
 
 # CODATA 2018
 AMU_TO_ELECTRON_MASS = 1822.888486209
@@ -44,8 +45,8 @@ def kinetic_energy(numbers, velocities):
 
 @dataclass
 class MdResult:
-    """Per system: final positions and velocities; per recorded step (rows)
-    and system (columns): potential (total free energy), kinetic and total
+    """Per system: final positions and velocities; per step (rows, step 0 =
+    start) and system (columns): potential (total free energy) and kinetic
     energy; status per system (0, or the error that stopped it)."""
     positions: List[np.ndarray]
     velocities: List[np.ndarray]
@@ -53,7 +54,6 @@ class MdResult:
     potential: np.ndarray
     kinetic: np.ndarray
     status: List[int]
-    trajectory: List[List[np.ndarray]] = field(default_factory=list)
 
     @property
     def total(self):
@@ -61,67 +61,42 @@ class MdResult:
 
 
 def run_md(engine: BatchEngine, positions, steps, timestep_fs=0.5, velocities=None, temperature=None, seed=1,
-           warm_start=True, record_every=1, keep_trajectory=False) -> MdResult:
-    """Velocity-Verlet NVE dynamics of every system of the batch.
+           warm_start=True) -> MdResult:
+    """Velocity-Verlet NVE dynamics of every system of the batch, on the
+    device through gfn1_batch_md.
 
     positions: one (nat, 3) array per system (bohr), the start geometry;
     velocities: one (nat, 3) array per system, or None for Maxwell-Boltzmann
     velocities at temperature (K; zero velocities when None too).
     """
+    from . import _lib
+    import ctypes
     n = engine.n
-    x = [np.asarray(p, dtype=np.float64).reshape(-1, 3).copy() for p in positions]
     z = engine.numbers
-    m = [masses_au(zi)[:, None] for zi in z]
     if velocities is not None:
-        v = [np.asarray(vi, dtype=np.float64).reshape(-1, 3).copy() for vi in velocities]
+        v = [np.asarray(vi, dtype=np.float64).reshape(-1, 3) for vi in velocities]
     elif temperature is not None:
         v = [maxwell_boltzmann(zi, temperature, seed + i) for i, zi in enumerate(z)]
     else:
-        v = [np.zeros_like(xi) for xi in x]
-    nsh = [basis_counts(zi)[0] if len(zi) else 0 for zi in z]
-    dt = timestep_fs / FS_PER_ATOMIC_TIME
-    status = [0] * n
-    times, pot, kin, traj = [], [], [], [[] for _ in range(n)]
-
-    q_start = None
-
-    def evaluate():
-        nonlocal q_start
-        engine.set_positions(x)
-        engine.run()
-        st, e, g, q = engine.results()
-        for i in range(n):
-            if status[i] == 0:
-                status[i] = int(st[i])
-            if status[i] != 0:
-                e[i] = np.nan
-        if warm_start:
-            # a failed system keeps the charges it last converged to
-            q_start = [q[i] if status[i] == 0 else (q_start[i] if q_start is not None else np.zeros(nsh[i]))
-                       for i in range(n)]
-            engine.set_all_start_charges(q_start)
-        return e, g
-
-    def record(step, e):
-        times.append(step * timestep_fs)
-        pot.append(e.copy())
-        kin.append([kinetic_energy(z[i], v[i]) if status[i] == 0 else np.nan for i in range(n)])
-        if keep_trajectory:
-            for i in range(n):
-                traj[i].append(x[i].copy())
-
-    e, g = evaluate()
-    record(0, e)
-    for step in range(1, steps + 1):
-        for i in range(n):
-            if status[i] == 0:
-                v[i] -= 0.5 * dt * g[i] / m[i]
-                x[i] += dt * v[i]
-        e, g = evaluate()
-        for i in range(n):
-            if status[i] == 0:
-                v[i] -= 0.5 * dt * g[i] / m[i]
-        if step % max(record_every, 1) == 0 or step == steps:
-            record(step, e)
-    return MdResult(positions=x, velocities=v, times_fs=np.array(times), potential=np.array(pot),
-                    kinetic=np.array(kin), status=status, trajectory=traj if keep_trajectory else [])
+        v = [np.zeros((len(zi), 3)) for zi in z]
+    engine.set_positions(positions)
+    vf = np.ascontiguousarray(np.concatenate(v) if n else np.zeros((0, 3)))
+    pot = np.zeros((steps + 1, n))
+    kin = np.zeros((steps + 1, n))
+    st = np.zeros(n, dtype=np.int32)
+    o = _lib.MdOptions()
+    o.timestep_fs = float(timestep_fs)
+    o.steps = int(steps)
+    o.warm_start = int(bool(warm_start))
+    err = _lib.Gfn1Error()
+    _lib.check(engine._lib.gfn1_batch_md(engine._h, ctypes.byref(o), vf.ctypes.data_as(_lib._DP),
+                                         pot.ctypes.data_as(_lib._DP), kin.ctypes.data_as(_lib._DP),
+                                         st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(err)), err)
+    xf = np.zeros((sum(engine.natoms), 3))
+    if n:
+        engine._lib.gfn1_batch_positions(engine._h, xf.ctypes.data_as(_lib._DP))
+    off = np.cumsum([0] + engine.natoms)
+    return MdResult(positions=[xf[off[i]:off[i + 1]].copy() for i in range(n)],
+                    velocities=[vf[off[i]:off[i + 1]].copy() for i in range(n)],
+                    times_fs=np.arange(steps + 1) * float(timestep_fs), potential=pot, kinetic=kin,
+                    status=[int(s) for s in st])

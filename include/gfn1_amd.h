@@ -311,6 +311,27 @@ int gfn1_batch_set_all_start_charges(gfn1_batch* batch, const double* shell_char
  * from the previous geometry's charges.  No reference counterpart as a call. */
 int gfn1_batch_set_start_charges(gfn1_batch* batch, int32_t inst, const double* shell_charges, gfn1_error* err);
 
+/* Current positions of every atom (bohr, 3 per atom, batch order). */
+int gfn1_batch_positions(const gfn1_batch* batch, double* out);
+
+/* --- Molecular dynamics ---
+ * NVE velocity Verlet for every system of the batch at once.  The reference
+ * runs dynamics through ASE with its calculator (no library call); each step
+ * here is one batched energy + gradient pass over the systems still running. */
+typedef struct gfn1_md_options {
+    double timestep_fs;   /* 0.5 */
+    int32_t steps;        /* 10 */
+    int32_t warm_start;   /* 1: each SCC starts from the previous step's charges */
+} gfn1_md_options;
+void gfn1_md_options_default(gfn1_md_options* options);
+/* From the batch's current positions.  velocities: 3 per atom (bohr per atomic
+ * time unit), updated in place; potential / kinetic (nullable): (steps + 1) x n
+ * energies (Eh), row = step, NaN once a system has stopped; status (nullable):
+ * n gfn1_status values.  Afterwards the batch holds the final positions
+ * (gfn1_batch_positions) and no batch results. */
+int gfn1_batch_md(gfn1_batch* batch, const gfn1_md_options* options, double* velocities, double* potential,
+                  double* kinetic, int32_t* status, gfn1_error* err);
+
 /* --- Geometry optimisation ---
  * optimize_geometry (src/optimizer.rs:78-185): L-BFGS on the total free energy
  * and the analytic gradient, Armijo backtracking line search, per-atom step
