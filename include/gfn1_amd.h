@@ -3,7 +3,8 @@
  *
  * This is the drop-in boundary for the reference's non-periodic SCC path
  * (energy and analytic gradient), and for its callers built on it here: the
- * L-BFGS geometry optimiser and the Cartesian Hessian.  Each entry point names
+ * L-BFGS geometry optimiser, batched molecular dynamics and the Cartesian
+ * Hessian.  Each entry point names
  * the reference interface it replaces (paths relative to the reference
  * repository root).  All coordinates are in bohr, energies in Hartree,
  * gradients in Hartree/bohr.  Matrices are dense n x n FP64; they are symmetric,
