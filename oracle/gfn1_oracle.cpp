@@ -996,6 +996,25 @@ bool eigh(int n, const double* Ain, double* w, double* V) {
 // ------------------------------------------------------------------ options / results
 }  // namespace
 
+// coulomb.rs:82-107 effective_coulomb_matrix (0-d kernel, coulomb.rs:244): the
+// shell hardnesses combine by their harmonic mean (coulomb.rs:239); shells of
+// one atom get that mean itself, other pairs 1 / sqrt(R^2 + eta^-2).
+void gamma_matrix(const Basis& b, const Geo& g, std::vector<double>& gam) {
+    const int nsh = (int)b.sh.size();
+    gam.assign((size_t)nsh * nsh, 0.0);
+    for (int i = 0; i < nsh; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double h = 2.0 / (1.0 / b.sh[i].hardness + 1.0 / b.sh[j].hardness);
+            double v;
+            if (b.sh[i].atom == b.sh[j].atom) v = h;
+            else {
+                double r2 = g.r2(b.sh[i].atom, b.sh[j].atom);
+                v = 1.0 / std::sqrt(r2 + 1.0 / (h * h));
+            }
+            gam[(size_t)i * nsh + j] = gam[(size_t)j * nsh + i] = v;
+        }
+}
+
 extern "C" {
 
 typedef struct {
@@ -1161,6 +1180,70 @@ int oracle_overlap(void* vctx, int nat, const int* z, const double* xyz, double 
                 }
             }
         }
+    return 0;
+}
+
+// Parsed parameter values, for the tests that compare the parser with an
+// independent reading of the parameter file.  key: "present", "nshell", "gam",
+// "gam3", "repa", "repb", "cxb" (element), or "n", "l", "lev", "exp", "poly",
+// "lpar" (shell `shell` of the element, file order).  Returns 0 on success.
+int oracle_element_value(void* vctx, int z, const char* key, int shell, double* out) {
+    const Params& p = ((Ctx*)vctx)->p;
+    if (z < 1 || z > 86) return 1;
+    const ElementParam& e = p.el[z];
+    const std::string k(key);
+    if (k == "present") { *out = e.present ? 1.0 : 0.0; return 0; }
+    if (k == "nshell") { *out = (double)e.shells.size(); return 0; }
+    if (k == "gam") { *out = e.gam; return 0; }
+    if (k == "gam3") { *out = e.gam3; return 0; }
+    if (k == "repa") { *out = e.repa; return 0; }
+    if (k == "repb") { *out = e.repb; return 0; }
+    if (k == "cxb") { *out = e.cxb; return 0; }
+    if (shell < 0 || shell >= (int)e.shells.size()) return 2;
+    const ShellParam& sp = e.shells[shell];
+    if (k == "n") { *out = sp.n; return 0; }
+    if (k == "l") { *out = sp.l; return 0; }
+    if (k == "lev") { *out = sp.level_ev; return 0; }
+    if (k == "exp") { *out = sp.slater; return 0; }
+    if (k == "poly") { *out = sp.poly; return 0; }
+    if (k == "lpar") { *out = sp.lpar; return 0; }
+    return 3;
+}
+// Global parameter (NaN when the file has none) and the pair scaling factor.
+double oracle_global(void* vctx, const char* key) {
+    const Params& p = ((Ctx*)vctx)->p;
+    auto it = p.glob.find(lower(key));
+    return it == p.glob.end() ? NAN : it->second;
+}
+double oracle_pair_scaling(void* vctx, int a, int b) { return ((Ctx*)vctx)->p.pair_scaling(a, b); }
+
+// Shells of a system in basis order: atom, angular momentum, valence flag,
+// AO count, hardness and the CN-free self energy (Eh).  Arrays of nsh entries.
+int oracle_shells(void* vctx, int nat, const int* z, int* atom, int* l, int* valence, int* nao,
+                  double* hardness, double* hdiag) {
+    Basis b;
+    Err err{0, ""};
+    if (!build_basis(*(Ctx*)vctx, nat, z, 0, b, err)) return err.code;
+    for (size_t s = 0; s < b.sh.size(); ++s) {
+        atom[s] = b.sh[s].atom;
+        l[s] = b.sh[s].l;
+        valence[s] = b.sh[s].valence ? 1 : 0;
+        nao[s] = b.sh[s].nao;
+        hardness[s] = b.sh[s].hardness;
+        hdiag[s] = b.sh[s].hdiag;
+    }
+    return 0;
+}
+
+// The shell Coulomb matrix gamma (nsh x nsh) that the SCC uses.
+int oracle_gamma(void* vctx, int nat, const int* z, const double* xyz, double* out) {
+    Basis b;
+    Err err{0, ""};
+    if (!build_basis(*(Ctx*)vctx, nat, z, 0, b, err)) return err.code;
+    Geo g{nat, z, xyz};
+    std::vector<double> gam;
+    gamma_matrix(b, g, gam);
+    std::copy(gam.begin(), gam.end(), out);
     return 0;
 }
 
@@ -1401,18 +1484,8 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
         }
 
     // ---- gamma (coulomb.rs:82-107)
-    std::vector<double> gam((size_t)nsh * nsh);
-    for (int i = 0; i < nsh; ++i)
-        for (int j = 0; j <= i; ++j) {
-            double h = 2.0 / (1.0 / b.sh[i].hardness + 1.0 / b.sh[j].hardness);
-            double v;
-            if (b.sh[i].atom == b.sh[j].atom) v = h;
-            else {
-                double r2 = g.r2(b.sh[i].atom, b.sh[j].atom);
-                v = 1.0 / std::sqrt(r2 + 1.0 / (h * h));
-            }
-            gam[(size_t)i * nsh + j] = gam[(size_t)j * nsh + i] = v;
-        }
+    std::vector<double> gam;
+    gamma_matrix(b, g, gam);
 
     // ---- Lowdin orthogonaliser (linalg.rs:193-217)
     std::vector<double> X((size_t)n * n), Xt((size_t)n * n);

@@ -270,3 +270,53 @@ def run_batch(systems, nthreads=0, **opts):
     lib().oracle_run_batch(ctx(), len(systems), _ptr(offs), _ptr(z), _ptr(x), ctypes.byref(o),
                            _ptr(e), _ptr(g), _ptr(it), _ptr(st), nthreads)
     return e, g, it, st
+
+
+def element_value(z, key, shell=0, param_file=None):
+    """Parsed value of one parameter of element z (see oracle_element_value);
+    None when the element or shell has no such entry."""
+    f = lib().oracle_element_value
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    out = ctypes.c_double()
+    rc = f(ctx(param_file), int(z), key.encode(), int(shell), ctypes.byref(out))
+    return None if rc else out.value
+
+
+def global_value(key, param_file=None):
+    f = lib().oracle_global
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+    return f(ctx(param_file), key.encode())
+
+
+def pair_scaling(a, b, param_file=None):
+    f = lib().oracle_pair_scaling
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    return f(ctx(param_file), int(a), int(b))
+
+
+def shells(numbers, param_file=None):
+    """Per shell (basis order): atom, l, valence, nao, hardness, CN-free self energy."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    nsh, _ = basis_size(z, param_file)
+    ints = [np.zeros(nsh, dtype=np.int32) for _ in range(4)]
+    dbl = [np.zeros(nsh) for _ in range(2)]
+    lib().oracle_shells.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 7
+    rc = lib().oracle_shells(ctx(param_file), len(z), _ptr(z), *[_ptr(a) for a in ints + dbl])
+    if rc:
+        raise ValueError("basis build failed (%d)" % rc)
+    return dict(atom=ints[0], l=ints[1], valence=ints[2].astype(bool), nao=ints[3], hardness=dbl[0], hdiag=dbl[1])
+
+
+def gamma(numbers, positions_bohr, param_file=None):
+    """The shell Coulomb matrix of the SCC (nsh x nsh)."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    nsh, _ = basis_size(z, param_file)
+    out = np.zeros((nsh, nsh))
+    lib().oracle_gamma.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rc = lib().oracle_gamma(ctx(param_file), len(z), _ptr(z), _ptr(x), _ptr(out))
+    if rc:
+        raise ValueError("basis build failed (%d)" % rc)
+    return out
