@@ -20,15 +20,6 @@
 
 using namespace gfn1;
 
-namespace {
-// The batch engine runs one HIP stream per size bucket (up to 14), and the HIP
-// runtime maps streams onto GPU_MAX_HW_QUEUES hardware queues (default 4);
-// streams that share a queue serialise.  Ask for 16 queues unless the caller
-// set the variable.  The runtime reads it when it initialises, so this only
-// takes effect if nothing in the process has used HIP before the library loads.
-__attribute__((constructor)) void request_hw_queues() { setenv("GPU_MAX_HW_QUEUES", "16", 0); }
-}  // namespace
-
 struct gfn1_params {
     Model model;
     std::mutex mu;
@@ -561,7 +552,7 @@ void gfn1_md_options_default(gfn1_md_options* o) {
 }
 
 int gfn1_batch_md(gfn1_batch* b, const gfn1_md_options* options, double* velocities, double* potential,
-                  double* kinetic, int32_t* status, gfn1_error* err) {
+                  double* kinetic, int32_t* status, int32_t* scc_iterations, gfn1_error* err) {
     clear(err);
     if (!b || !b->batch || (!velocities && b->batch->natoms() > 0))
         return fail(err, Status::error(kErrInvalidInput, "invalid MD arguments"));
@@ -578,15 +569,16 @@ int gfn1_batch_md(gfn1_batch* b, const gfn1_md_options* options, double* velocit
     o.warm_start = options->warm_start;
     const size_t nv = 3 * (size_t)b->batch->natoms();
     std::vector<double> v(velocities, velocities + nv), pot, kin;
-    std::vector<int> st_out;
+    std::vector<int> st_out, it_out;
     b->ran = false;
     b->downloaded = false;
-    Status st = md_batch(*b->batch, o, v, pot, kin, st_out);
+    Status st = md_batch(*b->batch, o, v, pot, kin, st_out, it_out);
     if (!st.ok()) return fail(err, st);
     std::copy(v.begin(), v.end(), velocities);
     if (potential) std::copy(pot.begin(), pot.end(), potential);
     if (kinetic) std::copy(kin.begin(), kin.end(), kinetic);
     if (status) std::copy(st_out.begin(), st_out.end(), status);
+    if (scc_iterations) std::copy(it_out.begin(), it_out.end(), scc_iterations);
     return GFN1_OK;
 }
 

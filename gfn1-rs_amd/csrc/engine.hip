@@ -516,6 +516,7 @@ Status Batch::run(const unsigned char* select) {
     if (natoms_ > 0) HIP_TRY(hipMemset(d_ghal_, 0, 3 * (size_t)natoms_ * sizeof(double)));
     HIP_TRY(hipMemset(d_fail_, 0, (size_t)std::max(ninst_, 1) * sizeof(int)));
     ran_ = true;
+    serial_ = std::getenv("GFN1_SERIAL_BUCKETS") != nullptr;
     tri_ms_ = tri_bytes_ = 0.0;
     tri_launches_ = 0;
     auto t0 = std::chrono::steady_clock::now();
@@ -780,6 +781,7 @@ Status Batch::setup_phase() {
         if (bit->ld > kCholMaxDim || bit->sel.empty()) continue;
         Status st = launch_bucket_setup(*bit);
         if (!st.ok()) return st;
+        if (serial_) HIP_TRY(hipDeviceSynchronize());
     }
 
     // geometry-only terms and the SCC start charges (stream_)
@@ -816,6 +818,7 @@ Status Batch::setup_phase() {
     }
     HIP_TRY(hipMemcpyAsync(d_qin_, d_q0_, nshells_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     HIP_TRY(hipEventRecord(ev_start_, stream_));
+    if (serial_) HIP_TRY(hipDeviceSynchronize());
     for (auto& bk : buckets_) {
         HIP_TRY(hipStreamWaitEvent(bk.stream, ev_start_, 0));
         if (bk.in_setup) {
@@ -901,6 +904,7 @@ Status Batch::scc_phase() {
         if (bit->in_setup) continue;
         Status st = bit->active.empty() ? launch_bucket_gradient(*bit) : launch_iteration(*bit);
         if (!st.ok()) return st;
+        if (serial_) HIP_TRY(hipDeviceSynchronize());
     }
     while (true) {
         bool any = false, progressed = false;
@@ -936,6 +940,7 @@ Status Batch::scc_phase() {
             }
             st = bk.active.empty() ? launch_bucket_gradient(bk) : launch_iteration(bk);
             if (!st.ok()) return st;
+            if (serial_) HIP_TRY(hipDeviceSynchronize());
         }
         if (!any) break;
         if (!progressed) std::this_thread::yield();

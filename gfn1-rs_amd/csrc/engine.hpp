@@ -125,6 +125,8 @@ class Batch {
     const std::string& message(int inst) const { return h_errmsg_[inst]; }
     const double* timings() const { return timings_; }
     int size() const { return ninst_; }
+    // SCC iterations of an instance in the last run (valid after fetch_energy_gradient or download)
+    int iterations(int inst) const { return h_state_.empty() ? 0 : h_state_[inst].iter; }
     // Learned start charges for the next runs (nullptr: the caller's or zero start).
     Status set_guess(const GuessModel* model);
     const GuessModel* guess() const { return guess_; }
@@ -253,6 +255,10 @@ class Batch {
     hipStream_t stream_ = nullptr;
     bool ran_ = false;
     bool progress_ = false;  // GFN1_PROGRESS: one stderr line per bucket iteration
+    // GFN1_SERIAL_BUCKETS (profiling only): every bucket's setup, iteration and
+    // gradient runs alone on the device, so kernel durations are those of the
+    // kernel by itself rather than shared with the other buckets' work
+    bool serial_ = false;
     std::chrono::steady_clock::time_point t_scc_{}, setup_end_{};
     hipEvent_t ev_cn_ = nullptr, ev_start_ = nullptr;  // coordination numbers; start charges and geometry terms
     bool has_d_ = false;  // any d shell in the batch

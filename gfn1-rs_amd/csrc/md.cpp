@@ -28,7 +28,7 @@ constexpr double kFsPerAtomicTime = 0.024188843265857;  // CODATA 2018
 }  // namespace
 
 Status md_batch(Batch& batch, const MdOptions& opt, std::vector<double>& v, std::vector<double>& pot,
-                std::vector<double>& kin, std::vector<int>& status) {
+                std::vector<double>& kin, std::vector<int>& status, std::vector<int>& iters) {
     const int n = batch.size(), na = batch.natoms();
     if (opt.steps < 0 || !(opt.timestep_fs > 0.0))
         return Status::error(kErrInvalidInput, "MD needs steps >= 0 and a positive time step");
@@ -46,6 +46,20 @@ Status md_batch(Batch& batch, const MdOptions& opt, std::vector<double>& v, std:
     status.assign(n, kOk);
     pot.assign((size_t)(opt.steps + 1) * n, NAN);
     kin.assign((size_t)(opt.steps + 1) * n, NAN);
+    iters.assign((size_t)(opt.steps + 1) * n, -1);
+    // A warm start replaces the batch's start charges and, after the first
+    // step, its learned start-charge model (whose prediction would otherwise
+    // overwrite the previous step's charges); the caller's are put back when
+    // the dynamics ends, on every exit path.
+    struct RestoreStart {
+        Batch& b;
+        const GuessModel* g;
+        std::vector<double> q0;
+        ~RestoreStart() {
+            (void)b.set_guess(g);
+            b.restore_start_charges(q0);
+        }
+    } restore{batch, batch.guess(), batch.start_charges()};
     std::vector<unsigned char> select(n, 1);
     std::vector<int> st_run;
     auto evaluate = [&](int step) -> Status {
@@ -71,6 +85,7 @@ Status md_batch(Batch& batch, const MdOptions& opt, std::vector<double>& v, std:
                 for (int d = 0; d < 3; ++d) k += 0.5 * m[a] * v[3 * (size_t)a + d] * v[3 * (size_t)a + d];
             pot[(size_t)step * n + i] = e[i];
             kin[(size_t)step * n + i] = k;
+            iters[(size_t)step * n + i] = batch.iterations(i);
         }
         return Status{};
     };
@@ -84,6 +99,10 @@ Status md_batch(Batch& batch, const MdOptions& opt, std::vector<double>& v, std:
     };
     Status st = evaluate(0);
     if (!st.ok()) return st;
+    if (opt.warm_start && batch.guess()) {
+        st = batch.set_guess(nullptr);
+        if (!st.ok()) return st;
+    }
     for (int step = 1; step <= opt.steps; ++step) {
         half_kick();
         for (int i = 0; i < n; ++i) {

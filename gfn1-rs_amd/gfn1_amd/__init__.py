@@ -46,4 +46,4 @@ from .api import (  # noqa: F401
     system_from_xyz,
     system_from_xyz_file,
 )
-from ._lib import NativeError  # noqa: F401
+from ._lib import NativeError, request_hw_queues  # noqa: F401

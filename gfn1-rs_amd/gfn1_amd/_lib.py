@@ -147,7 +147,7 @@ SIGNATURES = [
     ("gfn1_batch_positions", ctypes.c_int, [_VP, _DP]),
     ("gfn1_md_options_default", None, [ctypes.POINTER(MdOptions)]),
     ("gfn1_batch_md", ctypes.c_int, [_VP, ctypes.POINTER(MdOptions), _DP, _DP, _DP, ctypes.POINTER(ctypes.c_int32),
-                                     ctypes.POINTER(Gfn1Error)]),
+                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),
     ("gfn1_batch_energies", ctypes.c_int, [_VP, _DP]),
     ("gfn1_batch_gradients", ctypes.c_int, [_VP, _DP]),
     ("gfn1_batch_all_shell_charges", ctypes.c_int, [_VP, _DP]),
@@ -190,19 +190,26 @@ SIGNATURES = [
 _lib = None
 
 
+def request_hw_queues(count=16):
+    """Ask the HIP runtime for `count` hardware queues (GPU_MAX_HW_QUEUES).
+
+    The batch engine runs one HIP stream per size bucket (up to 14 for
+    matrices of 32..448 AO); the runtime maps streams onto GPU_MAX_HW_QUEUES
+    hardware queues (4 by default), and streams that share a queue run one
+    after another.  The variable is read once, when the HIP runtime starts, so
+    this must be called before anything in the process uses the GPU; a value
+    the caller already exported is left alone.  Returns the value in effect.
+    Opt-in: loading the library changes nothing in the process environment."""
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(int(count)))
+    return int(os.environ["GPU_MAX_HW_QUEUES"])
+
+
 def load():
     """Load the built library; raises if it is missing."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libgfn1_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
-        # The batch engine runs one HIP stream per size bucket (up to 14 for
-        # matrices of 32..448 AO).  The HIP runtime maps streams onto
-        # GPU_MAX_HW_QUEUES hardware queues (default 4), and streams sharing a
-        # queue serialise; 16 gives every bucket its own queue.  The variable is
-        # read when the HIP runtime initialises, so it only takes effect when
-        # nothing in the process has used the GPU yet; an explicit setting wins.
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
         lib = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(lib, name)

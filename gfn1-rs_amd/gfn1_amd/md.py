@@ -47,13 +47,15 @@ def kinetic_energy(numbers, velocities):
 class MdResult:
     """Per system: final positions and velocities; per step (rows, step 0 =
     start) and system (columns): potential (total free energy) and kinetic
-    energy; status per system (0, or the error that stopped it)."""
+    energy; SCC iterations per step and system (-1 once it stopped); status per
+    system (0, or the error that stopped it)."""
     positions: List[np.ndarray]
     velocities: List[np.ndarray]
     times_fs: np.ndarray
     potential: np.ndarray
     kinetic: np.ndarray
     status: List[int]
+    scc_iterations: np.ndarray
 
     @property
     def total(self):
@@ -84,6 +86,7 @@ def run_md(engine: BatchEngine, positions, steps, timestep_fs=0.5, velocities=No
     pot = np.zeros((steps + 1, n))
     kin = np.zeros((steps + 1, n))
     st = np.zeros(n, dtype=np.int32)
+    its = np.zeros((steps + 1, n), dtype=np.int32)
     o = _lib.MdOptions()
     o.timestep_fs = float(timestep_fs)
     o.steps = int(steps)
@@ -91,7 +94,8 @@ def run_md(engine: BatchEngine, positions, steps, timestep_fs=0.5, velocities=No
     err = _lib.Gfn1Error()
     _lib.check(engine._lib.gfn1_batch_md(engine._h, ctypes.byref(o), vf.ctypes.data_as(_lib._DP),
                                          pot.ctypes.data_as(_lib._DP), kin.ctypes.data_as(_lib._DP),
-                                         st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(err)), err)
+                                         st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         its.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(err)), err)
     xf = np.zeros((sum(engine.natoms), 3))
     if n:
         engine._lib.gfn1_batch_positions(engine._h, xf.ctypes.data_as(_lib._DP))
@@ -99,4 +103,4 @@ def run_md(engine: BatchEngine, positions, steps, timestep_fs=0.5, velocities=No
     return MdResult(positions=[xf[off[i]:off[i + 1]].copy() for i in range(n)],
                     velocities=[vf[off[i]:off[i + 1]].copy() for i in range(n)],
                     times_fs=np.arange(steps + 1) * float(timestep_fs), potential=pot, kinetic=kin,
-                    status=[int(s) for s in st])
+                    status=[int(s) for s in st], scc_iterations=its)

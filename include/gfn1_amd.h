@@ -328,10 +328,13 @@ void gfn1_md_options_default(gfn1_md_options* options);
 /* From the batch's current positions.  velocities: 3 per atom (bohr per atomic
  * time unit), updated in place; potential / kinetic (nullable): (steps + 1) x n
  * energies (Eh), row = step, NaN once a system has stopped; status (nullable):
- * n gfn1_status values.  Afterwards the batch holds the final positions
- * (gfn1_batch_positions) and no batch results. */
+ * n gfn1_status values; scc_iterations (nullable): (steps + 1) x n SCC
+ * iteration counts, -1 once a system has stopped.  With warm_start the
+ * batch's learned start-charge model (if any) starts step 0 only; the batch's
+ * model and start charges are restored when the call returns.  Afterwards the
+ * batch holds the final positions (gfn1_batch_positions) and no batch results. */
 int gfn1_batch_md(gfn1_batch* batch, const gfn1_md_options* options, double* velocities, double* potential,
-                  double* kinetic, int32_t* status, gfn1_error* err);
+                  double* kinetic, int32_t* status, int32_t* scc_iterations, gfn1_error* err);
 
 /* --- Geometry optimisation ---
  * optimize_geometry (src/optimizer.rs:78-185): L-BFGS on the total free energy

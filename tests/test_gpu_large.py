@@ -1,0 +1,93 @@
+"""The single large systems on the GPU: 1000 waters (3000 atoms, 8000 AO) and
+2000 waters (6000 atoms, 16000 AO), FP64 and mixed precision.
+
+The CPU oracle needs hours at these sizes, so they are checked through
+size-independent properties of the exact result:
+- translation invariance, sum of the gradient <= 1e-10 Eh/bohr
+  (physical_consistency.rs:286);
+- charge conservation, |sum q| <= 1e-10 e;
+- covariance of the gradient under a rotation of the whole system, with the
+  energy unchanged (physical_consistency.rs:481,514);
+- the mixed-precision path within 1e-6 Eh and 1e-5 Eh/bohr of the FP64 path.
+The FP64 path itself is pinned against the oracle up to 560 AO
+(test_gpu_parity.py), through the same engine branch that runs here.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions, basis_counts, geometry
+
+pytestmark = pytest.mark.gpu
+
+# SCC tolerances tight enough that the gradient sits within 1e-7 of its
+# converged value (a charge error dq moves it by about dq)
+OPTS = AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-9, charge_tolerance=1e-7))
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rotation():
+    a, b = 0.7, 0.4
+    rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, math.cos(b), -math.sin(b)], [0, math.sin(b), math.cos(b)]])
+    return rz @ rx
+
+
+def run_one(z, x, precision):
+    eng = BatchEngine([(z, x)], options=OPTS, precision=precision).run()
+    assert eng.status(0) == 0
+    nsh = basis_counts(z)[0]
+    out = eng.energy(0), eng.gradient(0), eng.shell_charges(0, nsh), eng.iterations(0), eng.timings()
+    del eng
+    return out
+
+
+def check_sum_rules(label, g, q):
+    sg = np.abs(g.sum(axis=0)).max()
+    sq = abs(q.sum())
+    note("%s: |sum g| %.2e  |sum q| %.2e" % (label, sg, sq))
+    assert sg <= 1e-10
+    assert sq <= 1e-10
+
+
+def large_system_checks(nmol, side, seed, rotate_precision):
+    z, x = geometry.water_cluster(nmol, side, seed)
+    nsh, nao = geometry.basis_size(z)
+    note("case water%d: %d atoms, %d AO" % (nmol, len(z), nao))
+    e64, g64, q64, it, t = run_one(z, x, "fp64")
+    note("water%d fp64: E %.10f, %d iterations, %.0f ms" % (nmol, e64, it, t["total_ms"]))
+    check_sum_rules("water%d fp64" % nmol, g64, q64)
+    e32, g32, q32, it, t = run_one(z, x, "mixed")
+    de, dg = abs(e32 - e64), float(np.abs(g32 - g64).max())
+    note("water%d mixed: %d iterations, %.0f ms, |dE| %.2e max|dg| %.2e" % (nmol, it, t["total_ms"], de, dg))
+    check_sum_rules("water%d mixed" % nmol, g32, q32)
+    assert de <= 1e-6 and dg <= 1e-5
+    # rotation about the centroid
+    rot = rotation()
+    c = x.mean(axis=0)
+    xr = (x - c) @ rot.T + c
+    er, gr, qr, it, _ = run_one(z, xr, rotate_precision)
+    e0, g0 = (e64, g64) if rotate_precision == "fp64" else (e32, g32)
+    de, dg = abs(er - e0), float(np.abs(gr - g0 @ rot.T).max())
+    note("water%d rotated (%s): |dE| %.2e, max|g(Rx) - R g(x)| %.2e" % (nmol, rotate_precision, de, dg))
+    assert de <= 1e-8
+    assert dg <= (1e-6 if rotate_precision == "fp64" else 2e-6)
+    check_sum_rules("water%d rotated" % nmol, gr, qr)
+
+
+@pytest.mark.timeout(900)
+def test_water1000_8000_ao():
+    """1000 waters in a 31.0 A cube, seed 1: 3000 atoms, 8000 AO."""
+    large_system_checks(1000, 31.0, 1, "fp64")
+
+
+@pytest.mark.timeout(1200)
+def test_water2000_16000_ao():
+    """2000 waters in a 39.1 A cube, seed 4: 6000 atoms, 16000 AO."""
+    large_system_checks(2000, 39.1, 4, "mixed")

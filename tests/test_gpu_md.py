@@ -57,3 +57,38 @@ def test_energy_conservation_and_reversibility():
     assert drift < 3e-5
     back = run_md(eng, res.positions, 200, timestep_fs=0.2, velocities=[-res.velocities[0]])
     assert np.abs(back.positions[0] - x).max() < 1e-6
+
+
+def test_warm_start_with_a_learned_model_and_restore():
+    """A batch with a learned start-charge model: with warm_start the model
+    starts step 0 only and every later SCC starts from the previous step's
+    charges, so the later steps need fewer iterations than the same dynamics
+    without warm start; afterwards the batch runs exactly as before."""
+    import os
+
+    from gfn1_amd import GuessModel
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    model = GuessModel.load(os.path.join(root, "gfn1-rs_amd", "data", "guess_mlp.bin"))
+    systems = geometry.mixed_batch(4, first_seed=1000, kmax=3)
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))
+    note("case md learned model + warm start, 4 clusters, 6 steps")
+    eng = BatchEngine(systems, options=opts, guess=model)
+    eng.run()
+    before = [eng.iterations(i) for i in range(eng.n)]
+    e_before = eng.energies()
+    x0 = [x for _, x in systems]
+    v0 = [maxwell_boltzmann(z, 300.0, 5 + i) for i, (z, _) in enumerate(systems)]
+    warm = run_md(eng, x0, 6, timestep_fs=0.5, velocities=v0, warm_start=True)
+    cold = run_md(eng, x0, 6, timestep_fs=0.5, velocities=v0, warm_start=False)
+    assert warm.status == [0] * 4 and cold.status == [0] * 4
+    note("warm iterations %s / cold %s" % (warm.scc_iterations.sum(axis=1).tolist(),
+                                          cold.scc_iterations.sum(axis=1).tolist()))
+    # step 0 is the same learned start on both
+    assert (warm.scc_iterations[0] == cold.scc_iterations[0]).all()
+    assert warm.scc_iterations[1:].sum() < cold.scc_iterations[1:].sum()
+    assert np.abs(warm.potential - cold.potential).max() <= 1e-8
+    # the caller's model and start charges are back in place
+    eng.set_positions(x0)
+    eng.run()
+    assert [eng.iterations(i) for i in range(eng.n)] == before
+    assert np.abs(eng.energies() - e_before).max() <= 1e-12

@@ -49,6 +49,62 @@ def test_single_molecule_matches_oracle(name):
     assert np.abs(el.coordination_numbers - ref["coordination_numbers"]).max() <= 1e-13
     assert np.abs(el.orbital_energies - ref["orbital_energies"]).max() <= 1e-8
     assert abs(el.nelec - ref["nelec"]) < 1e-12
+    # the rest of the result contract (electronic.rs:600-654)
+    assert np.abs(el.fock - ref["fock"]).max() <= 1e-8
+    assert np.abs(el.energy_weighted_density - ref["energy_weighted_density"]).max() <= 1e-7
+    assert np.abs(el.shell_scc_potential - ref["shell_scc_potential"]).max() <= 1e-8
+    assert np.abs(el.occupations - ref["occupations"]).max() <= 1e-8
+    assert np.abs(el.atomic_charges - ref["atomic_charges"]).max() <= 1e-8
+    assert np.abs(el.dipole - ref["dipole"]).max() <= 1e-8
+    assert el.electronic_temperature == ref["electronic_temperature"] == 300.0
+    assert el.charge_order == 3
+    assert_fermi_level_matches(el, ref)
+
+
+KT300 = 300.0 * 3.1668115634556e-6
+
+
+def assert_fermi_level_matches(el, ref):
+    """The Fermi level is the f64 fixed point of a bisection on sum f(mu) = nelec
+    (electronic.rs:3853-3921).  Across a HOMO-LUMO gap of many kT that sum is
+    exactly nelec in floating point over a wide range of mu; the bisection
+    stops at the edge of that range, about 35 kT above the HOMO, where the
+    occupation deficit 2 exp(-(mu - e_HOMO)/kT) per HOMO orbital drops below
+    the last bit of the sum.  How the partial sums round places that edge
+    anywhere within a factor of about 8 in the deficit, i.e. ln 8 = 2.1 kT in
+    mu (summing the same eigenvalues in reverse order on the CPU already moves
+    it by 0.5 kT).  Both sides must therefore lie in the gap and agree to
+    3 kT; the occupations, which do not depend on where mu sits inside the
+    flat range, are compared to 1e-8 above."""
+    nocc = int(round(ref["nelec"])) // 2
+    eps = ref["orbital_energies"]
+    assert abs(el.fermi_level - ref["fermi_level"]) <= 3.0 * KT300
+    if 0 < nocc < len(eps) and eps[nocc] - eps[nocc - 1] > 100 * KT300:
+        assert eps[nocc - 1] < el.fermi_level < eps[nocc]
+
+
+def test_fock_and_potential_semantics_at_default_tolerances():
+    """At the default SCC tolerances the input and output charges of the last
+    iteration differ visibly.  The Fock matrix is built from the input charges
+    of the last step and shell_scc_potential from the output charges
+    (electronic.rs:1957-1991); both sides must agree with the oracle on each,
+    far closer than the input/output difference."""
+    note("case default-tolerance result semantics (water)")
+    z, x = mol("water")
+    ref = oracle.run(z, x, want_matrices=True)
+    el = analytic_gradient(z, x).electronic_result
+    assert el.iterations == ref["iterations"]
+    # potential implied by the Fock diagonal (S_mumu = 1): v_in = H0 - F
+    v_in = np.diag(ref["h0"]) - np.diag(ref["fock"])
+    # AO -> shell map of water: O 2s | O 2p (3 AO) | H 1s | H 2s | H 1s | H 2s
+    ao_shell = [0, 1, 1, 1, 2, 3, 4, 5]
+    v_out = ref["shell_scc_potential"][ao_shell]
+    gap = np.abs(v_in - v_out).max()
+    assert gap > 1e-7
+    assert np.abs(el.fock - ref["fock"]).max() <= 1e-3 * gap
+    assert np.abs(el.shell_scc_potential - ref["shell_scc_potential"]).max() <= 1e-3 * gap
+    assert np.abs(el.density - ref["density"]).max() <= 1e-9
+    assert_fermi_level_matches(el, ref)
 
 
 def test_default_tolerances_match_iteration_count():
