@@ -28,6 +28,36 @@
 #include "eigen.hpp"
 
 namespace gfn1 {
+
+// Phase timing (profiling builds only, -DGFN1_PHASE_TIMING): thread 0 of the
+// first kPhaseMats work-groups adds the wall-clock ticks (100 MHz) it spends in
+// each phase of a kernel to gfn1_phase_ticks[kernel][matrix][phase], read back
+// with gfn1_debug_phase_ticks.  Compiled out otherwise.
+#ifdef GFN1_PHASE_TIMING
+constexpr int kPhaseKernels = 2, kPhaseMats = 64, kPhaseSlots = 8;
+__device__ unsigned long long gfn1_phase_ticks[kPhaseKernels][kPhaseMats][kPhaseSlots];
+#define PT_INIT()                          \
+    unsigned long long pt_last = wall_clock64(); \
+    unsigned long long pt_acc[kPhaseSlots] = {};
+#define PT_MARK(slot)                                        \
+    do {                                                     \
+        if (threadIdx.x == 0) {                              \
+            const unsigned long long pt_now = wall_clock64(); \
+            pt_acc[slot] += pt_now - pt_last;                \
+            pt_last = pt_now;                                \
+        }                                                    \
+    } while (0)
+#define PT_STORE(kernel)                                                                   \
+    do {                                                                                   \
+        if (threadIdx.x == 0 && blockIdx.x < kPhaseMats)                                   \
+            for (int s_ = 0; s_ < kPhaseSlots; ++s_) gfn1_phase_ticks[kernel][blockIdx.x][s_] += pt_acc[s_]; \
+    } while (0)
+#else
+#define PT_INIT()
+#define PT_MARK(slot)
+#define PT_STORE(kernel)
+#endif
+
 namespace {
 
 constexpr double kEps = 2.220446049250313e-16;
@@ -144,6 +174,7 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // wavefront-uniform
     const int r = tid;
     double vreg[kNB], wreg[kNB];
+    PT_INIT();
     for (int k0 = 0; k0 < n - 1; k0 += kNB) {
         const int b = min(kNB, n - 1 - k0);
 #pragma unroll
@@ -183,6 +214,7 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                 tau[k] = tk;
             }
             __syncthreads();
+            PT_MARK(0);
             // ---- A_panel v over the lower triangle of the trailing block, in 64 x kTW
             //      tiles (rows c0 + 64t + lane, columns c0 + kTW u .. + kTW - 1, with
             //      u < (64 / kTW)(t + 1)); the row and column contributions of a
@@ -251,6 +283,7 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                 if (lane < 16) ysp[wid * 16 + reduce16_index(lane)] = tot;
             }
             __syncthreads();
+            PT_MARK(1);
             // ---- p = tau (A v - V W^T v - W V^T v); w = p - tau/2 (p.v) v
             double p = 0.0;
             if (r >= c0 && r < n) {
@@ -283,6 +316,7 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                 }
             }
             __syncthreads();
+            PT_MARK(2);
         }
         // ---- rank-2b update of the trailing lower triangle (columns >= cs)
         const int cs = k0 + b;
@@ -334,7 +368,9 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
             }
             __syncthreads();
         }
+        PT_MARK(3);
     }
+    PT_STORE(0);
     if (tid == 0 && n > 0) {
         d[n - 1] = A[(size_t)(n - 1) * ld + (n - 1)];
         e[n - 1] = 0.0;
@@ -548,6 +584,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     int* vflag = tight + ld;         // 1 where eigenvector j is formed by inverse iteration
     int* order = vflag + ld;         // eigenvector indices of the current pass
     const int tid = threadIdx.x, nt = blockDim.x;
+    PT_INIT();
     for (int i = tid; i < n; i += nt) {
         D[i] = dg[i];
         Eo[i] = i < n - 1 ? eg[i] : 0.0;
@@ -650,6 +687,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
             lam[j] = 0.5 * (a + b);
         }
         __syncthreads();
+        PT_MARK(0);
         const int tail = n - m;  // orbitals not formed, all >= 80 kT above mu
         // ---- Fermi occupations (same semantics as k_fermi)
         const double mn = lam[0], mx = tail ? hi : lam[n - 1];
@@ -701,6 +739,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
             b = nb;
         }
         mu = 0.5 * (a + b);
+        PT_MARK(1);
         if (tail && lam[m - 1] - mu < 80.0 * kt) {
             // the smearing reaches beyond the formed eigenvalues: form the rest
             have = m;
@@ -744,6 +783,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         s.kneed = min(kneed, kcap);
     }
     kneed = min(kneed, kcap);
+    PT_MARK(2);
     // ---- groups of nearly equal eigenvalues and inverse-iteration shifts
     // (coincident shifts are separated by pertol as in LAPACK dstein)
     const double tn = fmax(tnorm, 1e-300);
@@ -787,6 +827,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         if (!twisted_vector(D, Eo, n, lam[j], pivmin, kcap, Z + j, rU1 + j, U2 + j, U3 + j)) vflag[j] = 1;
     }
     __syncthreads();
+    PT_MARK(3);
     if (tid == 0) {
         int c = 0;
         for (int j = 0; j < kneed; ++j)
@@ -867,6 +908,8 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         const double inv = 1.0 / sqrt(ss);
         for (int i = 0; i < n; ++i) Z[(size_t)i * kcap + j] *= inv;
     }
+    __syncthreads();
+    PT_MARK(4);
     // zero rows n..ld-1 and columns kneed..kcap-1
     for (size_t q = (size_t)n * kcap + tid; q < fld; q += nt) Z[q] = 0.0;
     const int zc = kcap - kneed;
@@ -936,6 +979,8 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         }
         g0 = g1;
     }
+    PT_MARK(5);
+    PT_STORE(1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,3 +1254,14 @@ hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const 
 }
 
 }  // namespace gfn1
+
+#ifdef GFN1_PHASE_TIMING
+// Profiling builds: copies and clears the phase ticks (kernel, matrix, phase).
+extern "C" int gfn1_debug_phase_ticks(unsigned long long* out) {
+    const size_t bytes = sizeof(gfn1::gfn1_phase_ticks);
+    if (hipMemcpyFromSymbol(out, HIP_SYMBOL(gfn1::gfn1_phase_ticks), bytes, 0, hipMemcpyDeviceToHost) != hipSuccess)
+        return 1;
+    static unsigned long long zero[sizeof(gfn1::gfn1_phase_ticks) / sizeof(unsigned long long)] = {};
+    return hipMemcpyToSymbol(HIP_SYMBOL(gfn1::gfn1_phase_ticks), zero, bytes, 0, hipMemcpyHostToDevice) != hipSuccess;
+}
+#endif

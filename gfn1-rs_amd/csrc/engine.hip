@@ -14,6 +14,7 @@
 #include "engine.hpp"
 
 #include "cholesky.hpp"
+#include "density.hpp"
 #include "eigen.hpp"
 
 #include <rocsolver/rocsolver.h>
@@ -304,11 +305,16 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     HIP_TRY(copy_bytes(d_x_, h_x_.data(), h_x_.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!(st = dalloc(&d_q0_, nshells_)).ok()) return st;
     HIP_TRY(copy_bytes(d_q0_, h_q0_.data(), nshells_ * sizeof(double), hipMemcpyHostToDevice));
+    // Mulliken partial sums: one slice of naos_ per 64-column block of the largest matrix
+    int max_ld = 0;
+    for (int i = 0; i < ninst_; ++i) max_ld = std::max(max_ld, h_ld_[i]);
+    const size_t rowsum_blocks = (size_t)std::max(1, density_row_blocks(max_ld));
     double** dd[] = {&d_cn_, &d_cnd3_, &d_qin_, &d_qout_, &d_vin_g_, &d_vin_t_, &d_vout_g_, &d_vout_t_,
                      &d_rowS_, &d_rowH_, &d_erep_atom_, &d_edisp_atom_, &d_erep_, &d_edisp_, &d_ehal_,
                      &d_grep_, &d_gdisp_, &d_ghal_, &d_gel_, &d_dedcn_disp_, &d_dedcn_h_, &d_brql_, &d_brdql_};
     size_t dn[] = {(size_t)natoms_, (size_t)natoms_, (size_t)nshells_, (size_t)nshells_, (size_t)nshells_,
-                   (size_t)nshells_, (size_t)nshells_, (size_t)nshells_, (size_t)naos_, (size_t)naos_,
+                   (size_t)nshells_, (size_t)nshells_, (size_t)nshells_, rowsum_blocks * (size_t)naos_,
+                   rowsum_blocks * (size_t)naos_,
                    (size_t)natoms_, (size_t)natoms_, (size_t)ninst_, (size_t)ninst_, (size_t)ninst_,
                    3 * (size_t)natoms_, 3 * (size_t)natoms_, 3 * (size_t)natoms_, 3 * (size_t)natoms_,
                    (size_t)natoms_, (size_t)natoms_, (size_t)nshells_, (size_t)nshells_};
@@ -1047,10 +1053,9 @@ Status Batch::launch_iteration(Bucket& bk) {
                                    bk.lu, kc, opt_.keep_matrices != 0));
         HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
                                      (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
-        // C_k = Cb Z (into T)
-        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, kc, n, &one,
-                                     (const double* const*)Cb, n, (const double* const*)Z, kc, &zero,
-                                     (double* const*)T, n, cnt));
+        // C_k = Cb Z (into T); Z is stored transposed, element (m, j) at m * kc + j
+        HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
+                                kc, Cb, Z, kc, 1, T));
     } else {
         // rocSOLVER runs alone on the device, as in the setup phase
         HIP_TRY(hipDeviceSynchronize());
@@ -1064,28 +1069,20 @@ Status Batch::launch_iteration(Bucket& bk) {
         hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
         LAUNCH_CHECK();
-        // rocSOLVER returns the eigenvectors in place
-        RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, kc, n, &one,
-                                     (const double* const*)Cb, n, (const double* const*)A, n, &zero,
-                                     (double* const*)T, n, cnt));
+        // rocSOLVER returns the eigenvectors in place (column-major, element (m, j) at j * n + m)
+        HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
+                                kc, Cb, A, 1, n, T));
     }
-    // ---- P = C_k diag(f) C_k^T
-    double* const* P = bk.ptrs + (size_t)A_P * cnt;
-    const int nt = (n + 31) / 32;
+    // ---- P = C_k diag(f) C_k^T with the Mulliken row sums of S o P and H0 o P
     // (FP64 in every precision mode: the SCC energy and charges need it)
-    hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
-                       (const double*)d_eps_, (const double*)d_occ_, kc, 0);
-    LAUNCH_CHECK();
-    RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc, &one,
-                                 (const double* const*)A, n, (const double* const*)T, n, &zero, (double* const*)P, n,
-                                 cnt));
-    hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, P);
-    LAUNCH_CHECK();
+    double* const* P = bk.ptrs + (size_t)A_P * cnt;
+    HIP_TRY(launch_density(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n, kc,
+                           T, (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_, false, P,
+                           bk.ptrs + (size_t)A_S * cnt, bk.ptrs + (size_t)A_H0 * cnt, (const int*)d_ao0_, d_rowS_,
+                           d_rowH_, naos_));
     // ---- Mulliken charges, energy, convergence and mixing
-    hipLaunchKernelGGL(k_rowsums, dim3(n, cnt), dim3(128), 0, bk.stream, db_, (const int*)bk.act, d_rowS_, d_rowH_);
-    LAUNCH_CHECK();
     hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
-                       (const double*)d_rowS_, (const double*)d_rowH_, d_qout_, d_state_);
+                       (const double*)d_rowS_, (const double*)d_rowH_, naos_, d_qout_, d_state_);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
                        (const int*)bk.act, (const double*)d_qout_, d_vout_g_, d_vout_t_);
@@ -1191,15 +1188,9 @@ Status Batch::launch_bucket_gradient(Bucket& bk) {
                                (const int*)bk.act, X32, W);
             LAUNCH_CHECK();
         } else {
-            hipLaunchKernelGGL(k_scale_columns, dim3(kc, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act, T, A,
-                               (const double*)d_eps_, (const double*)d_occ_, kc, 1);
-            LAUNCH_CHECK();
-            RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_transpose, n, n, kc,
-                                         &one, (const double* const*)A, n, (const double* const*)T, n, &zero,
-                                         (double* const*)W, n, cnt));
-            hipLaunchKernelGGL(k_symmetrize, dim3(nt, nt, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
-                               W);
-            LAUNCH_CHECK();
+            HIP_TRY(launch_density(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_,
+                                   n, kc, T, (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_,
+                                   true, W, nullptr, nullptr, (const int*)d_ao0_, nullptr, nullptr, naos_));
         }
     }
     const bool el_on = opt_.include_hamiltonian || opt_.include_scc;

@@ -545,46 +545,6 @@ __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int ei
     }
 }
 
-// T(:, j) = C(:, j) * w_j for j < k, with w = f (density) or eps*f (energy-weighted).
-__global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
-                                const double* eps_all, const double* occ_all, int k, int weighted) {
-    const int inst = act[blockIdx.y];
-    const int n = b.nao[inst], ld = b.ld[inst];
-    const int col = blockIdx.x;
-    if (col >= k) return;
-    const double f = occ_all[b.vec0[inst] + col];
-    const double w = weighted ? eps_all[b.vec0[inst] + col] * f : f;
-    gptr<const double> c = (gptr<const double>)C[blockIdx.y] + (size_t)col * ld;
-    gptr<double> o = (gptr<double>)T[blockIdx.y] + (size_t)col * ld;
-    for (int row = threadIdx.x; row < ld; row += blockDim.x) o[row] = row < n ? c[row] * w : 0.0;
-}
-
-// M <- (M + M^T) / 2 on the n x n block (linalg.rs:262-270); 32x32 tiles.
-__global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp) {
-    __shared__ double t1[32][33], t2[32][33];
-    const int inst = act[blockIdx.z];
-    const int n = b.nao[inst], ld = b.ld[inst];
-    const int bi = blockIdx.x, bj = blockIdx.y;
-    if (bj > bi) return;
-    if (bi * 32 >= n) return;
-    gptr<double> M = (gptr<double>)Mp[blockIdx.z];
-    const int tx = threadIdx.x & 31, ty0 = threadIdx.x >> 5;  // 256 threads: 8 rows per pass
-    for (int ty = ty0; ty < 32; ty += 8) {
-        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;  // tile (bi, bj): rows bi
-        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;  // tile (bj, bi)
-        t1[ty][tx] = (r1 < n && c1 < n) ? M[(size_t)c1 * ld + r1] : 0.0;
-        t2[ty][tx] = (r2 < n && c2 < n) ? M[(size_t)c2 * ld + r2] : 0.0;
-    }
-    __syncthreads();
-    for (int ty = ty0; ty < 32; ty += 8) {
-        const int r1 = bi * 32 + tx, c1 = bj * 32 + ty;
-        const int r2 = bj * 32 + tx, c2 = bi * 32 + ty;
-        // element (r1, c1) pairs with (c1, r1) = tile2 entry at [tx'=c1-bj*32... ]
-        if (r1 < n && c1 < n) M[(size_t)c1 * ld + r1] = 0.5 * (t1[ty][tx] + t2[tx][ty]);
-        if (bi != bj && r2 < n && c2 < n) M[(size_t)c2 * ld + r2] = 0.5 * (t2[ty][tx] + t1[tx][ty]);
-    }
-}
-
 // Mixed precision operands of M = C_k diag(w) C_k^T: C32 = fp32(C_k) and
 // A32 = fp32(C_k diag(w)), w = f (density) or eps f (energy-weighted density);
 // column col of each at col * ld (rows n..ld-1 zero).
@@ -633,46 +593,30 @@ __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* 
     }
 }
 
-// Column sums of S o P and H0 o P (= row sums, the matrices being symmetric):
-// Mulliken populations per AO and the band energy (electronic.rs:3160-3188).
-__global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH) {
-    __shared__ double lds[16];
-    const int inst = act[blockIdx.y];
-    const int n = b.nao[inst], ld = b.ld[inst];
-    const int col = blockIdx.x;
-    if (col >= n) return;
-    gptr<const double> S = (gptr<const double>)b.S[inst] + (size_t)col * ld;
-    gptr<const double> H = (gptr<const double>)b.H0[inst] + (size_t)col * ld;
-    gptr<const double> P = (gptr<const double>)b.P[inst] + (size_t)col * ld;
-    double ps = 0.0, ph = 0.0;
-    for (int row = threadIdx.x; row < n; row += blockDim.x) {
-        ps += S[row] * P[row];
-        ph += H[row] * P[row];
-    }
-    ps = block_sum(ps, lds);
-    ph = block_sum(ph, lds);
-    if (threadIdx.x == 0) {
-        rowS[b.ao0[inst] + col] = ps;
-        rowH[b.ao0[inst] + col] = ph;
-    }
-}
-
-// Output shell charges q = n0 - sum_{mu in shell} (S P)_mu mu and the band energy.
+// Output shell charges q = n0 - sum_{mu in shell} (S P)_mu mu and the band energy
+// (electronic.rs:3160-3188), from the per-64-column-block partial sums
+// rowS[blk * stride + ao0 + mu] (summed in block order).
 __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
-                       double* qout, SccState* st) {
+                       int stride, double* qout, SccState* st) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.x];
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int a0 = b.ao0[inst], n = b.nao[inst];
+    const int nblk = (n + 63) / 64;
+    auto total = [&](const double* part, int mu) {
+        double v = 0.0;
+        for (int blk = 0; blk < nblk; ++blk) v += part[(size_t)blk * stride + a0 + mu];
+        return v;
+    };
     for (int i = threadIdx.x; i < nsh; i += blockDim.x) {
         const int si = s0 + i;
         const ShellType& T = t.shells[b.sh_type[si]];
         double pop = 0.0;
-        for (int k = 0; k < T.nao; ++k) pop += rowS[a0 + b.sh_ao[si] + k];
+        for (int k = 0; k < T.nao; ++k) pop += total(rowS, b.sh_ao[si] + k);
         qout[si] = T.ref_occ - pop;
     }
     double e = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) e += rowH[a0 + i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) e += total(rowH, i);
     e = block_sum(e, lds);
     if (threadIdx.x == 0) st[inst].e_el = e;
 }

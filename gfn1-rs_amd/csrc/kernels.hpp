@@ -37,18 +37,14 @@ __global__ void k_fock(DeviceBatch b, const int* act, const double* vt);
 __global__ void k_level_shift(DeviceBatch b, const int* act, const double* shift, double* const* SPS);
 __global__ void k_fermi(DeviceBatch b, const int* act, const double* eig, int eig_stride, double* eps_all,
                         double* occ_all, SccState* st);
-__global__ void k_scale_columns(DeviceBatch b, const int* act, double* const* C, double* const* T,
-                                const double* eps_all, const double* occ_all, int k, int weighted);
-__global__ void k_symmetrize(DeviceBatch b, const int* act, double* const* Mp);
 __global__ void k_reduce_fock(DeviceBatch b, const int* act, const double* vt, double* const* Cbp, double* const* Lp,
                               double* const* A0p, double* const* Ap);
 __global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp, double* const* A32p,
                                 double* const* C32p, const double* eps_all, const double* occ_all, int k,
                                 int weighted);
 __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* Xp, double* const* Mp);
-__global__ void k_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH);
 __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
-                       double* qout, SccState* st);
+                       int stride, double* qout, SccState* st);
 __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccParams prm, const double* qout,
                              const double* vg_out, double* qin, const double* q0, SccState* st, double* br_df,
                              double* br_u, double* br_omega, double* br_qlast, double* br_dqlast,

@@ -14,14 +14,59 @@ import numpy as np
 from ase.calculators.calculator import Calculator, all_changes
 from ase.units import Bohr, Hartree, invcm
 
-from .api import Gfn1NativeCalculator
+from .api import FORCE_AU_TO_EV_PER_ANGSTROM, Gfn1NativeCalculator
 
-_ENERGY = Hartree
-_LENGTH = Bohr
-_FORCE = Hartree / Bohr
-_HESSIAN = Hartree / Bohr ** 2
-_DIPOLE = Bohr
-_WAVENUMBER = invcm
+# Every conversion at the ASE boundary is a product of ase.units values, so
+# results agree with whatever constants the installed ASE uses.
+_ENERGY = Hartree                      # eV per Eh
+_LENGTH = Bohr                         # Angstrom per bohr
+_FORCE = Hartree / Bohr                # eV/Angstrom per Eh/bohr
+_HESSIAN = Hartree / Bohr ** 2         # eV/Angstrom^2 per Eh/bohr^2
+_THIRD = Hartree / Bohr ** 3
+_FOURTH = Hartree / Bohr ** 4
+_DIPOLE = Bohr                         # e Angstrom per e bohr
+_POLARIZABILITY = Bohr ** 2 / Hartree
+_WAVENUMBER = invcm                    # eV per cm^-1
+# The native layer's own eV/Angstrom force unit (its CODATA constants, not
+# ASE's); only used to undo it when a result carries forces but no gradient.
+_NATIVE_FORCE_FACTOR = FORCE_AU_TO_EV_PER_ANGSTROM
+
+
+def _forces_hartree_per_bohr(result):
+    """Forces in Eh/bohr from a native result: the negated analytic gradient
+    when there is one, otherwise the native eV/Angstrom forces taken back to
+    atomic units with the native factor; None when the result has neither."""
+    gradient = getattr(result, "gradient_hartree_per_bohr", None)
+    if gradient is not None:
+        return -np.asarray(gradient, dtype=float)
+    native_forces = getattr(result, "forces_ev_per_angstrom", None)
+    if native_forces is None:
+        return None
+    return np.asarray(native_forces, dtype=float) / _NATIVE_FORCE_FACTOR
+
+
+def _origin_bohr(origin_angstrom):
+    """A caller's origin (Angstrom, ASE's length unit) in bohr, or None."""
+    if origin_angstrom is None:
+        return None
+    return np.asarray(origin_angstrom, dtype=float) / _LENGTH
+
+
+def _to_ase_gradient_dict(raw, energy_keys=()):
+    """ASE units for a dictionary of atomic-unit results: for each (source,
+    target) pair in energy_keys, target = source in eV; "gradient" and
+    "forces" become eV/Angstrom and keep their atomic-unit arrays under a
+    "_hartree_per_bohr" suffix.  Other entries pass through unchanged."""
+    out = dict(raw)
+    for source, target in energy_keys:
+        if source in out:
+            out[target] = float(out[source]) * _ENERGY
+    for key in ("gradient", "forces"):
+        if out.get(key) is not None:
+            atomic = np.asarray(out[key], dtype=float)
+            out[key + "_hartree_per_bohr"] = atomic
+            out[key] = atomic * _FORCE
+    return out
 
 
 class GFN1AMDCalculator(Calculator):
@@ -64,26 +109,29 @@ class GFN1AMDCalculator(Calculator):
 
     def calculate(self, atoms=None, properties=("energy",), system_changes=all_changes):
         """Single point in ASE units (ase.py:358-434), plus the native
-        passthroughs native_energy_terms_hartree, native_dipole_au,
+        passthroughs native_energy_terms_hartree / _ev, native_dipole_au,
         native_forces_hartree_per_bohr, native_converged, native_iterations."""
         Calculator.calculate(self, atoms, properties, system_changes)
         numbers, positions = self._molecule(self.atoms)
-        need_forces = "forces" in set(properties)
-        r = self._native.calculate(numbers, positions, unit="angstrom", compute_gradient=need_forces)
-        terms = dict(r.energy_terms_hartree())
-        self.results["energy"] = float(r.energy_hartree) * _ENERGY
-        self.results["charges"] = np.asarray(r.charges, dtype=float)
-        dipole_au = np.asarray(r.dipole, dtype=float)
-        self.results["dipole"] = dipole_au * _DIPOLE
-        self.results["native_dipole_au"] = dipole_au
-        self.results["native_energy_terms_hartree"] = terms
-        self.results["native_energy_terms_ev"] = {k: v * _ENERGY for k, v in terms.items()}
-        self.results["native_converged"] = bool(r.converged)
-        self.results["native_iterations"] = int(r.iterations)
-        if r.gradient_hartree_per_bohr is not None:
-            forces_au = -np.asarray(r.gradient_hartree_per_bohr, dtype=float)
-            self.results["native_forces_hartree_per_bohr"] = forces_au
-            self.results["forces"] = forces_au * _FORCE
+        result = self._native.calculate(numbers, positions, unit="angstrom",
+                                        compute_gradient="forces" in set(properties))
+        terms_au = dict(result.energy_terms_hartree())
+        dipole_au = np.asarray(result.dipole, dtype=float)
+        update = {
+            "energy": float(result.energy_hartree) * _ENERGY,
+            "charges": np.asarray(result.charges, dtype=float),
+            "dipole": dipole_au * _DIPOLE,
+            "native_dipole_au": dipole_au,
+            "native_energy_terms_hartree": terms_au,
+            "native_energy_terms_ev": {name: value * _ENERGY for name, value in terms_au.items()},
+            "native_converged": bool(result.converged),
+            "native_iterations": int(result.iterations),
+        }
+        forces_au = _forces_hartree_per_bohr(result)
+        if forces_au is not None:
+            update["native_forces_hartree_per_bohr"] = forces_au
+            update["forces"] = forces_au * _FORCE
+        self.results.update(update)
 
     def optimize_native(self, atoms=None, *, max_iterations=250, gradient_tolerance=1.0e-4, step_tolerance=1.0e-7,
                         history=12, max_atom_step=0.30):
@@ -114,8 +162,9 @@ class GFN1AMDCalculator(Calculator):
         e, the same number in either unit system) (ase.py:603-622); origin in
         Angstrom."""
         numbers, positions = self._molecule(atoms if atoms is not None else self.atoms)
-        o = None if origin is None else tuple(float(v) / _LENGTH for v in origin)
-        out = self._native.dipole_derivatives(numbers, positions, unit="angstrom", origin=o)
+        o = _origin_bohr(origin)
+        out = self._native.dipole_derivatives(numbers, positions, unit="angstrom",
+                                              origin=None if o is None else tuple(o))
         dipole_au = np.asarray(out["dipole"], dtype=float)
         out["dipole_au"] = dipole_au
         out["dipole"] = dipole_au * _DIPOLE
