@@ -62,9 +62,9 @@ namespace {
 
 constexpr double kEps = 2.220446049250313e-16;
 constexpr double kSafeMin = 2.2250738585072014e-308;
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+constexpr int kThreads = 256;  // k_tridiag_eig
 constexpr int kBT = 16;  // reflectors per compact-WY block in the back-transformation
+constexpr int kLB = 8;   // loads issued together in the dependent eigenvector passes
 
 // Value of v held by the lane that the DPP control CTRL selects (both halves).
 template <int CTRL>
@@ -144,6 +144,7 @@ __device__ __forceinline__ int reduce16_index(int lane) {
 // traffic; the rank-2b update is applied once per panel in 64 x 64 tiles with
 // V and W staged through LDS.  The matrix-vector product reads the lower
 // triangle once, in 64 x 16 tiles shared out over the wavefronts.
+typedef double tvec4 __attribute__((ext_vector_type(4)));
 constexpr int kTriThreads = 512;
 constexpr int kTriWaves = kTriThreads / 64;
 constexpr int kTW = 16;  // columns per symv tile (64 x 16): keeps the kernel within 128 VGPRs, two work-groups per CU
@@ -327,44 +328,67 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                     Vst[(size_t)r * kVS + i] = vreg[i];
                     Wst[(size_t)r * kVS + i] = wreg[i];
                 }
+            } else if (r < ld) {
+#pragma unroll
+                for (int i = 0; i < kNB; ++i) Vst[(size_t)r * kVS + i] = Wst[(size_t)r * kVS + i] = 0.0;
             }
             __syncthreads();
-            const int tb0 = cs / 64, nb = (n + 63) / 64 - tb0;
-            const int ntile = nb * (nb + 1) / 2;
-            for (int q = wid; q < ntile; q += kTriWaves) {
-                int t = 0, u = q;
-                while (u > t) {
-                    u -= t + 1;
-                    ++t;
+            // A -= V W^T + W V^T on the FP64 matrix cores, in 16 x 16 tiles on or
+            // below the diagonal from row / column T0 = cs rounded down to 16.  A
+            // tile sits in the accumulator transposed, element e of lane (li, lg)
+            // being A(R0 + li, C0 + 4 e + lg), so its loads and stores are
+            // column-contiguous; the update is D += Aop Bop with
+            // Aop = -[W_C | V_C] (16 x 2 kNB) and Bop = [V_R ; W_R] (2 kNB x 16).
+            // A wavefront takes strips of up to four tiles in one column block
+            // (sharing Aop) and issues their 16 loads together.
+            static_assert(kNB == 8, "two MFMA k-steps per panel factor");
+            const int T0 = cs & ~15;
+            const int nt16 = (n - T0 + 15) >> 4;
+            const int li = lane & 15, lg = lane >> 4;
+            for (int q = wid;; q += kTriWaves) {
+                int tc = 0, rem = q;
+                while (tc < nt16) {
+                    const int ns = (nt16 - tc + 3) >> 2;
+                    if (rem < ns) break;
+                    rem -= ns;
+                    ++tc;
                 }
-                const int rr = 64 * (tb0 + t) + lane;
-                const bool rv = rr >= cs && rr < n;
-                double vr2[kNB], wr2[kNB];
+                if (tc >= nt16) break;
+                const int C0 = T0 + 16 * tc, Rb = C0 + 64 * rem;
+                const int ntl = min(4, nt16 - tc - 4 * rem);
+                double aop[4];
 #pragma unroll
-                for (int i = 0; i < kNB; ++i) {
-                    vr2[i] = rv ? Vst[(size_t)rr * kVS + i] : 0.0;
-                    wr2[i] = rv ? Wst[(size_t)rr * kVS + i] : 0.0;
-                }
-                const int cb = max(cs, 64 * (tb0 + u)), ce = min(64 * (tb0 + u) + 64, n);
-                for (int c1 = cb; c1 < ce; c1 += 16) {
-                    double a[16];
+                for (int s2 = 0; s2 < 4; ++s2)
+                    aop[s2] = -(s2 < 2 ? Wst[(size_t)(C0 + li) * kVS + 4 * s2 + lg]
+                                       : Vst[(size_t)(C0 + li) * kVS + 4 * (s2 - 2) + lg]);
+                tvec4 acc[4];
 #pragma unroll
-                    for (int q2 = 0; q2 < 16; ++q2) {
-                        const int c = c1 + q2;
-                        a[q2] = (rv && c < ce && c <= rr) ? A[rr + (size_t)c * ld] : 0.0;
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int e2 = 0; e2 < 4; ++e2) {
+                        const int row = Rb + 16 * t + li, col = C0 + 4 * e2 + lg;
+                        const bool ok = t < ntl && row < n && col >= cs && row >= col;
+                        acc[t][e2] = ok ? A[row + (size_t)col * ld] : 0.0;
                     }
 #pragma unroll
-                    for (int q2 = 0; q2 < 16; ++q2) {
-                        const int c = c1 + q2;
-                        if (c < ce) {
-                            double upd = 0.0;
+                for (int t = 0; t < 4; ++t) {
+                    if (t < ntl) {
+                        const int R0 = Rb + 16 * t;
 #pragma unroll
-                            for (int i = 0; i < kNB; ++i)
-                                upd += vr2[i] * Wst[(size_t)c * kVS + i] + wr2[i] * Vst[(size_t)c * kVS + i];
-                            if (rv && c <= rr) A[rr + (size_t)c * ld] = a[q2] - upd;
+                        for (int s2 = 0; s2 < 4; ++s2) {
+                            const double bop = s2 < 2 ? Vst[(size_t)(R0 + li) * kVS + 4 * s2 + lg]
+                                                      : Wst[(size_t)(R0 + li) * kVS + 4 * (s2 - 2) + lg];
+                            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[s2], bop, acc[t], 0, 0, 0);
                         }
                     }
                 }
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int e2 = 0; e2 < 4; ++e2) {
+                        const int row = Rb + 16 * t + li, col = C0 + 4 * e2 + lg;
+                        if (t < ntl && row < n && col >= cs && row >= col) A[row + (size_t)col * ld] = acc[t][e2];
+                    }
             }
             __syncthreads();
         }
@@ -523,36 +547,65 @@ __device__ bool twisted_vector(const double* D, const double* Eo, int n, double 
     double dm = D[n - 1] - x, gbest = dp, zr = 1.0, up2 = 1.0;
     int r = n - 1;
     Z[(size_t)(n - 1) * kcap] = 1.0;
-    for (int i = n - 2; i >= 0; --i) {
-        if (fabs(dm) < pivmin) dm = -pivmin;
-        const size_t o = (size_t)i * kcap;
-        const double u = Eo[i] / dm;
-        Um[o] = u;
-        const double g = Dp[o] - Eo[i] * u;
-        dm = (D[i] - x) - u * Eo[i];
-        double zi;
-        if (fabs(g) < fabs(gbest)) {
-            gbest = g;
-            r = i;
-            zi = 1.0;
-            up2 = 0.0;
-        } else {
-            zi = -Lp[o] * zr;
+    // the passes below read back what the first one stored: kLB loads are issued
+    // together ahead of the dependent steps that use them
+    for (int i0 = n - 2; i0 >= 0; i0 -= kLB) {
+        double dpv[kLB], lpv[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u) {
+            const size_t o = (size_t)max(i0 - u, 0) * kcap;
+            dpv[u] = Dp[o];
+            lpv[u] = Lp[o];
         }
-        Z[o] = zi;
-        up2 += zi * zi;
-        zr = zi;
+#pragma unroll
+        for (int u = 0; u < kLB; ++u) {
+            const int i = i0 - u;
+            if (i < 0) break;
+            if (fabs(dm) < pivmin) dm = -pivmin;
+            const size_t o = (size_t)i * kcap;
+            const double uu = Eo[i] / dm;
+            Um[o] = uu;
+            const double g = dpv[u] - Eo[i] * uu;
+            dm = (D[i] - x) - uu * Eo[i];
+            double zi;
+            if (fabs(g) < fabs(gbest)) {
+                gbest = g;
+                r = i;
+                zi = 1.0;
+                up2 = 0.0;
+            } else {
+                zi = -lpv[u] * zr;
+            }
+            Z[o] = zi;
+            up2 += zi * zi;
+            zr = zi;
+        }
     }
     double zd = 1.0, dn2 = 0.0;
-    for (int i = r + 1; i < n; ++i) {
-        zd = -Um[(size_t)(i - 1) * kcap] * zd;
-        Z[(size_t)i * kcap] = zd;
-        dn2 += zd * zd;
+    for (int i0 = r + 1; i0 < n; i0 += kLB) {
+        double umv[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u) umv[u] = Um[(size_t)(min(i0 + u, n - 1) - 1) * kcap];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u) {
+            const int i = i0 + u;
+            if (i >= n) break;
+            zd = -umv[u] * zd;
+            Z[(size_t)i * kcap] = zd;
+            dn2 += zd * zd;
+        }
     }
     const double nrm2 = up2 + dn2;
     if (!(nrm2 > 0.0 && nrm2 < INFINITY)) return false;
     const double inv = 1.0 / sqrt(nrm2);
-    for (int i = 0; i < n; ++i) Z[(size_t)i * kcap] *= inv;
+    for (int i0 = 0; i0 < n; i0 += kLB) {
+        double zv[kLB];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u) zv[u] = Z[(size_t)min(i0 + u, n - 1) * kcap];
+#pragma unroll
+        for (int u = 0; u < kLB; ++u)
+            if (i0 + u < n) Z[(size_t)(i0 + u) * kcap] = zv[u] * inv;
+    }
     return true;
 }
 
@@ -872,36 +925,60 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
         }
         double scale = 1.0, ss = 1.0;
         for (int it = 0; it < 3; ++it) {
-            // y = L^-1 P z  (row interchanges applied on the fly)
+            // y = L^-1 P z  (row interchanges applied on the fly); the next kLB
+            // entries of z and of the factors are loaded before the steps use them
             double cur = it == 0 ? start_value(0, j) : Z[j] * scale;
-#pragma unroll 4
-            for (int i = 0; i < n - 1; ++i) {
-                const size_t o = (size_t)i * kcap + j;
-                const double nx = it == 0 ? start_value(i + 1, j) : Z[o + kcap] * scale;
-                const double m = Lm[o];
-                double out;
-                if (Pv[o] != 0.0) {
-                    out = nx;
-                    cur = cur - m * nx;
-                } else {
-                    out = cur;
-                    cur = nx - m * cur;
+            for (int i0 = 0; i0 < n - 1; i0 += kLB) {
+                double nxv[kLB], mv[kLB], pv[kLB];
+#pragma unroll
+                for (int u = 0; u < kLB; ++u) {
+                    const int i = min(i0 + u, n - 2);
+                    const size_t o = (size_t)i * kcap + j;
+                    nxv[u] = it == 0 ? start_value(i + 1, j) : Z[o + kcap] * scale;
+                    mv[u] = Lm[o];
+                    pv[u] = Pv[o];
                 }
-                Z[o] = out;
+#pragma unroll
+                for (int u = 0; u < kLB; ++u) {
+                    const int i = i0 + u;
+                    if (i >= n - 1) break;
+                    const double nx = nxv[u], m = mv[u];
+                    double out;
+                    if (pv[u] != 0.0) {
+                        out = nx;
+                        cur = cur - m * nx;
+                    } else {
+                        out = cur;
+                        cur = nx - m * cur;
+                    }
+                    Z[(size_t)i * kcap + j] = out;
+                }
             }
             Z[(size_t)(n - 1) * kcap + j] = cur;
             // z = U^-1 y
             double z1 = 0.0, z2 = 0.0, zmax = 0.0;
             ss = 0.0;
-#pragma unroll 4
-            for (int i = n - 1; i >= 0; --i) {
-                const size_t o = (size_t)i * kcap + j;
-                const double zi = (Z[o] - U2[o] * z1 - U3[o] * z2) * rU1[o];
-                Z[o] = zi;
-                z2 = z1;
-                z1 = zi;
-                zmax = fmax(zmax, fabs(zi));
-                ss += zi * zi;
+            for (int i0 = n - 1; i0 >= 0; i0 -= kLB) {
+                double yv[kLB], u2v[kLB], u3v[kLB], rv[kLB];
+#pragma unroll
+                for (int u = 0; u < kLB; ++u) {
+                    const size_t o = (size_t)max(i0 - u, 0) * kcap + j;
+                    yv[u] = Z[o];
+                    u2v[u] = U2[o];
+                    u3v[u] = U3[o];
+                    rv[u] = rU1[o];
+                }
+#pragma unroll
+                for (int u = 0; u < kLB; ++u) {
+                    const int i = i0 - u;
+                    if (i < 0) break;
+                    const double zi = (yv[u] - u2v[u] * z1 - u3v[u] * z2) * rv[u];
+                    Z[(size_t)i * kcap + j] = zi;
+                    z2 = z1;
+                    z1 = zi;
+                    zmax = fmax(zmax, fabs(zi));
+                    ss += zi * zi;
+                }
             }
             scale = 1.0 / zmax;
         }
