@@ -1087,18 +1087,11 @@ int gfn1_guess_train(gfn1_guess_model* m, int32_t n_sys, const gfn1_system* syst
     return GFN1_OK;
 }
 
-int gfn1_guess_predict(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems,
-                       const gfn1_params* params, double* shell_charges, gfn1_error* err) {
-    clear(err);
-    if (!m || n_sys < 0 || (n_sys > 0 && !systems) || !params || !shell_charges)
-        return fail(err, Status::error(kErrInvalidInput, "invalid prediction arguments"));
-    GuessAtoms a;
-    std::vector<int> nsh;
-    Status st = to_guess_atoms(n_sys, systems, params, a, &nsh);
-    if (!st.ok()) return fail(err, st);
-    std::vector<float> y;
-    st = m->m.predict_atoms(a, y);
-    if (!st.ok()) return fail(err, st);
+namespace {
+// Start vectors from per-atom network outputs: each usable system's shell
+// charges shifted to sum to its charge, zeros for the others.
+void shifted_start_charges(int32_t n_sys, const gfn1_system* systems, const GuessAtoms& a, const std::vector<int>& nsh,
+                           const std::vector<float>& y, double* shell_charges) {
     size_t off = 0;
     for (int i = 0; i < n_sys; ++i) {
         for (int k = 0; k < nsh[i]; ++k) shell_charges[off + k] = 0.0;
@@ -1114,8 +1107,38 @@ int gfn1_guess_predict(const gfn1_guess_model* m, int32_t n_sys, const gfn1_syst
         }
         off += nsh[i];
     }
+}
+
+int guess_predict(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems, const gfn1_params* params,
+                  double* shell_charges, gfn1_error* err, bool noalloc) {
+    clear(err);
+    if (!m || n_sys < 0 || (n_sys > 0 && !systems) || !params || !shell_charges)
+        return fail(err, Status::error(kErrInvalidInput, "invalid prediction arguments"));
+    GuessAtoms a;
+    std::vector<int> nsh;
+    Status st = to_guess_atoms(n_sys, systems, params, a, &nsh);
+    if (!st.ok()) return fail(err, st);
+    std::vector<float> y;
+    st = noalloc ? m->m.predict_atoms_noalloc(a, y) : m->m.predict_atoms(a, y);
+    if (!st.ok()) return fail(err, st);
+    shifted_start_charges(n_sys, systems, a, nsh, y, shell_charges);
     return GFN1_OK;
 }
+}  // namespace
+
+int gfn1_guess_predict(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems,
+                       const gfn1_params* params, double* shell_charges, gfn1_error* err) {
+    return guess_predict(m, n_sys, systems, params, shell_charges, err, false);
+}
+
+int gfn1_guess_predict_noalloc(const gfn1_guess_model* m, int32_t n_sys, const gfn1_system* systems,
+                               const gfn1_params* params, double* shell_charges, gfn1_error* err) {
+    return guess_predict(m, n_sys, systems, params, shell_charges, err, true);
+}
+
+static_assert(GFN1_NOALLOC_MAX_ATOMS == gfn1::kNoAllocMaxAtoms && GFN1_NOALLOC_MAX_SYSTEMS == gfn1::kNoAllocMaxSystems,
+              "no-allocation limits of the header and the library agree");
+size_t gfn1_guess_noalloc_scratch_bytes(void) { return gfn1::guess_noalloc_scratch_bytes(); }
 
 int gfn1_guess_features(int32_t n_sys, const gfn1_system* systems, float* out, gfn1_error* err) {
     clear(err);

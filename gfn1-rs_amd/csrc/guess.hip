@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 namespace gfn1 {
 namespace {
@@ -386,7 +387,19 @@ Status GuessModel::init_random(uint64_t seed) {
     norm_.assign(2 * kGuessFeat, 0.0f);
     for (int i = 0; i < kGuessFeat; ++i) norm_[kGuessFeat + i] = 1.0f;
     dirty_ = true;
-    return Status{};
+    return upload_if_device();
+}
+
+// Parameters go to the current device as soon as the model exists when a
+// device is present, so that later calls (the no-allocation prediction among
+// them) find them resident; without a device they are uploaded on first use.
+Status GuessModel::upload_if_device() const {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return Status{};
+    }
+    return upload();
 }
 
 Status GuessModel::set_params(const float* theta) {
@@ -446,7 +459,7 @@ Status GuessModel::load(const std::string& path) {
     theta_ = th;
     norm_ = nm;
     dirty_ = true;
-    return Status{};
+    return upload_if_device();
 }
 
 Status GuessModel::predict_atoms(const GuessAtoms& atoms, std::vector<float>& out) const {
@@ -585,6 +598,89 @@ Status GuessModel::train(const GuessAtoms& atoms, const std::vector<float>& labe
     loss_per_epoch.assign(opt.epochs, 0.0f);
     for (int ep = 0; ep < opt.epochs; ++ep) loss_per_epoch[ep] = (float)l[ep];
     GHIP(hipMemcpy(theta_.data(), th, kGuessParams * sizeof(float), hipMemcpyDeviceToHost));
+    return Status{};
+}
+
+// ---------------------------------------------------------------------------
+// Prediction without device allocation.  Its device memory is the file-scope
+// array `scratch`: staged atom tables and positions, then the descriptors and
+// network outputs, each region on a 256-byte boundary.  Every kernel writes
+// all of its outputs, so nothing needs clearing between calls.  In host code
+// `scratch` names a host-side stand-in; the device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+namespace {
+constexpr size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaZ = 0;
+constexpr size_t kNaInst = kNaZ + round256(sizeof(int) * kNoAllocMaxAtoms);
+constexpr size_t kNaAtom0 = kNaInst + round256(sizeof(int) * kNoAllocMaxAtoms);
+constexpr size_t kNaX = kNaAtom0 + round256(sizeof(int) * (kNoAllocMaxSystems + 1));
+constexpr size_t kNaFeat = kNaX + round256(sizeof(double) * 3 * kNoAllocMaxAtoms);
+constexpr size_t kNaOut = kNaFeat + round256(sizeof(float) * kGuessFeat * kNoAllocMaxAtoms);
+constexpr size_t kNaBytes = kNaOut + round256(sizeof(float) * kGuessOut * kNoAllocMaxAtoms);
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+std::mutex g_scratch_mu;                  // guards the pointer table and serialises the calls
+std::vector<unsigned char*> g_scratch_ptr;  // device address of `scratch`, per device id
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device).  The caller holds g_scratch_mu.
+Status scratch_address(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return Status::error(kErrDevice, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return Status::error(kErrDevice, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return Status{};
+}
+}  // namespace
+
+size_t guess_noalloc_scratch_bytes() { return kNaBytes; }
+
+Status GuessModel::predict_atoms_noalloc(const GuessAtoms& atoms, std::vector<float>& out) const {
+    const int na = (int)atoms.z.size();
+    const int ni = (int)atoms.inst_atom0.size() - 1;
+    if (na > kNoAllocMaxAtoms || ni > kNoAllocMaxSystems)
+        return Status::error(kErrInvalidInput, "the no-allocation prediction takes at most " +
+                                                   std::to_string(kNoAllocMaxAtoms) + " atoms in " +
+                                                   std::to_string(kNoAllocMaxSystems) + " systems");
+    if (theta_.size() != (size_t)kGuessParams) return Status::error(kErrInvalidInput, "guess model has no parameters");
+    if (!d_theta_ || !d_norm_)
+        return Status::error(kErrInvalidInput, "the model's parameters are not on the device (no allocation allowed)");
+    out.assign(2 * (size_t)na, 0.0f);
+    if (na == 0) return Status{};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dirty_) {
+        // new parameters of an already resident model: copies only
+        GHIP(hipMemcpy(d_theta_, theta_.data(), kGuessParams * sizeof(float), hipMemcpyHostToDevice));
+        GHIP(hipMemcpy(d_norm_, norm_.data(), 2 * kGuessFeat * sizeof(float), hipMemcpyHostToDevice));
+        dirty_ = false;
+    }
+    unsigned char* base = nullptr;
+    Status st = scratch_address(&base);
+    if (!st.ok()) return st;
+    int* z = (int*)(base + kNaZ);
+    int* inst = (int*)(base + kNaInst);
+    int* atom0 = (int*)(base + kNaAtom0);
+    double* x = (double*)(base + kNaX);
+    float* feat = (float*)(base + kNaFeat);
+    float* y = (float*)(base + kNaOut);
+    GHIP(hipMemcpy(z, atoms.z.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    GHIP(hipMemcpy(inst, atoms.inst.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    GHIP(hipMemcpy(atom0, atoms.inst_atom0.data(), (ni + 1) * sizeof(int), hipMemcpyHostToDevice));
+    GHIP(hipMemcpy(x, atoms.x.data(), 3 * (size_t)na * sizeof(double), hipMemcpyHostToDevice));
+    GHIP(guess_launch_features(0, na, z, x, inst, atom0, feat));
+    GHIP(guess_launch_forward(0, na, d_theta_, d_norm_, feat, y));
+    GHIP(hipMemcpy(out.data(), y, out.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < na; ++i)
+        if (!atoms.usable[atoms.inst[i]]) out[2 * i] = out[2 * i + 1] = 0.0f;
     return Status{};
 }
 

@@ -35,6 +35,9 @@ constexpr int kGoB2 = kGoW2 + kGuessHidden * kGuessHidden;
 constexpr int kGoW3 = kGoB2 + kGuessHidden;
 constexpr int kGoB3 = kGoW3 + kGuessOut * kGuessHidden;
 constexpr int kGuessParams = kGoB3 + kGuessOut;  // 8706
+// Limits of the no-allocation prediction (its device scratch is sized for them).
+constexpr int kNoAllocMaxAtoms = 16384;
+constexpr int kNoAllocMaxSystems = 16384;
 
 // Element slot of the descriptor (0..3 for H, C, N, O), -1 otherwise.
 inline int guess_element_slot(int z) { return z == 1 ? 0 : z == 6 ? 1 : z == 7 ? 2 : z == 8 ? 3 : -1; }
@@ -87,14 +90,23 @@ class GuessModel {
                          std::vector<float>& grad) const;
     // Per-atom network outputs (two per atom, FP32) for usable atoms; zeros elsewhere.
     Status predict_atoms(const GuessAtoms& atoms, std::vector<float>& out) const;
+    // The same outputs without allocating device memory: the model must already
+    // be resident (it is, once created or loaded on a machine with a device), the
+    // inputs at most kNoAllocMaxAtoms atoms in kNoAllocMaxSystems systems.  Calls
+    // in one process share one device scratch array and are serialised.
+    Status predict_atoms_noalloc(const GuessAtoms& atoms, std::vector<float>& out) const;
 
    private:
     Status upload() const;
+    Status upload_if_device() const;
     std::vector<float> theta_, norm_;
     mutable float* d_theta_ = nullptr;
     mutable float* d_norm_ = nullptr;
     mutable bool dirty_ = true;
 };
+
+// Bytes of the static device scratch of predict_atoms_noalloc.
+size_t guess_noalloc_scratch_bytes();
 
 // Descriptors on the device for a set of atoms (natoms x kGuessFeat, FP32).
 Status guess_features(const GuessAtoms& atoms, std::vector<float>& out);

@@ -137,6 +137,9 @@ SIGNATURES = [
                                         ctypes.POINTER(GuessTrainOptions), _FP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_guess_predict", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP,
                                           ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_predict_noalloc", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP,
+                                          ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_guess_noalloc_scratch_bytes", ctypes.c_size_t, []),
     ("gfn1_guess_features", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _FP, ctypes.POINTER(Gfn1Error)]),
     ("gfn1_guess_loss_gradient", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(System), _VP, _DP, _DP, _FP,
                                                 ctypes.POINTER(Gfn1Error)]),

@@ -818,17 +818,22 @@ class GuessModel:
         del keep
         return loss[:int(epochs)]
 
-    def predict(self, systems, params: Optional[Gfn1Parameters] = None):
-        """Start vectors, one array of shell charges per system."""
+    def predict(self, systems, params: Optional[Gfn1Parameters] = None, noalloc=False):
+        """Start vectors, one array of shell charges per system.  noalloc: the
+        variant that allocates no device memory (gfn1_guess_predict_noalloc;
+        at most NOALLOC_MAX_ATOMS atoms)."""
         p = _params(params)
         arr, keep = _system_array(systems)
         nsh = [basis_counts(s[0], p)[0] for s in systems]
         out = np.zeros(sum(nsh))
         err = _lib.Gfn1Error()
-        _lib.check(self._lib.gfn1_guess_predict(self._h, len(systems), arr, p._h, out.ctypes.data_as(_lib._DP),
-                                                ctypes.byref(err)), err)
+        fn = self._lib.gfn1_guess_predict_noalloc if noalloc else self._lib.gfn1_guess_predict
+        _lib.check(fn(self._h, len(systems), arr, p._h, out.ctypes.data_as(_lib._DP), ctypes.byref(err)), err)
         del keep
         return np.split(out, np.cumsum(nsh)[:-1])
+
+    NOALLOC_MAX_ATOMS = 16384
+    NOALLOC_MAX_SYSTEMS = 16384
 
     def loss_gradient(self, systems, shell_charges, params: Optional[Gfn1Parameters] = None):
         p = _params(params)

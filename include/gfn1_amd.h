@@ -18,6 +18,7 @@
 #ifndef GFN1_AMD_H
 #define GFN1_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -278,6 +279,21 @@ int gfn1_guess_train(gfn1_guess_model* model, int32_t n_sys, const gfn1_system* 
 /* Start vectors (nsh per system, concatenated). */
 int gfn1_guess_predict(const gfn1_guess_model* model, int32_t n_sys, const gfn1_system* systems,
                        const gfn1_params* params, double* shell_charges, gfn1_error* err);
+/* gfn1_guess_predict without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and results
+ * as gfn1_guess_predict; the library's device memory for the call is a static
+ * array of gfn1_guess_noalloc_scratch_bytes() bytes held by every device the
+ * library runs on until the process ends.  Limits: at most
+ * GFN1_NOALLOC_MAX_ATOMS atoms in GFN1_NOALLOC_MAX_SYSTEMS systems
+ * (GFN1_ERR_INVALID_INPUT beyond them), and the model's parameters must be on
+ * the device already (they are once the model is created or loaded with a
+ * device present).  Calls in one process share the array and are serialised;
+ * inputs and outputs are host arrays. */
+enum { GFN1_NOALLOC_MAX_ATOMS = 16384, GFN1_NOALLOC_MAX_SYSTEMS = 16384 };
+int gfn1_guess_predict_noalloc(const gfn1_guess_model* model, int32_t n_sys, const gfn1_system* systems,
+                               const gfn1_params* params, double* shell_charges, gfn1_error* err);
+size_t gfn1_guess_noalloc_scratch_bytes(void);
 /* Descriptors of every atom (68 floats per atom, systems concatenated). */
 int gfn1_guess_features(int32_t n_sys, const gfn1_system* systems, float* out, gfn1_error* err);
 /* Training loss (mean squared shell-charge error over covered atoms) and its
