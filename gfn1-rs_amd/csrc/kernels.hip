@@ -50,6 +50,23 @@ __device__ double moment1d(int i, int j, double pa, double pb, const double* g) 
     return sum;
 }
 
+// moment1d in the arithmetic type T (FP32 on the mixed-precision gradient path).
+template <typename T>
+__device__ __forceinline__ T moment1d_t(int i, int j, T pa, T pb, const T* g) {
+    T sum = 0;
+    for (int u = 0; u <= i; ++u) {
+        T ca = (T)binom4(i, u);
+        for (int q = 0; q < i - u; ++q) ca *= pa;
+        for (int v = 0; v <= j; ++v) {
+            if ((u + v) & 1) continue;
+            T cb = (T)binom4(j, v);
+            for (int q = 0; q < j - v; ++q) cb *= pb;
+            sum += ca * cb * g[(u + v) >> 1];
+        }
+    }
+    return sum;
+}
+
 // Cartesian content of the real spherical AOs (basis.rs:382-441):
 // s; px py pz; d x2-y2, z2, xy, xz, yz.
 struct CartTerm {
@@ -1333,63 +1350,134 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
     constexpr int order = LA + LB + 3;
     const double csmax = ao_abs_sum(LA, 1) * ao_abs_sum(LB, 1);
     R g0 = 0, g1 = 0, g2 = 0, sp = 0;
-    for (int ip = 0; ip < A.nprim; ++ip) {
-        const double al = A.alpha[ip], cai = A.coeff[ip];
-        for (int jp = 0; jp < B.nprim; ++jp) {
-            const double be = B.alpha[jp];
-            const double pref = cai * B.coeff[jp];
-            const double p = al + be;
-            const double kab = exp(-al * be / p * r2);
-            const double spread = sqrt(1.0 / p);
-            const double gpref = (kPi / p) * sqrt(kPi / p);
-            const double poly = ipow(1.0 + r + spread, order);
-            if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
-            double gm[GM];
-            gm[0] = sqrt(kPi / p);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // Mixed precision: the primitive integrals, their derivatives and the
+        // contraction all in FP32.  The product-centre offsets come from the
+        // bond vector (P - A = be (B - A) / p), not from absolute positions,
+        // so no cancellation is lost in FP32.
+        const float r2f = (float)r2, rf = (float)r;
+        const float bab[3] = {(float)(rb[0] - ra[0]), (float)(rb[1] - ra[1]), (float)(rb[2] - ra[2])};
+        const float csf = (float)csmax;
+        for (int ip = 0; ip < A.nprim; ++ip) {
+            const float al = (float)A.alpha[ip], cai = (float)A.coeff[ip];
+            for (int jp = 0; jp < B.nprim; ++jp) {
+                const float be = (float)B.alpha[jp];
+                const float pref = cai * (float)B.coeff[jp];
+                const float p = al + be, ip_ = 1.0f / p;
+                const float kab = expf(-al * be * ip_ * r2f);
+                const float spread = sqrtf(ip_);
+                const float gpref = ((float)kPi * ip_) * sqrtf((float)kPi * ip_);
+                float poly = 1.0f;
+                for (int q = 0; q < order; ++q) poly *= 1.0f + rf + spread;
+                if (fabsf(pref) * csf * kab * gpref * poly < (float)kPrimScreen) continue;
+                float gm[GM];
+                gm[0] = sqrtf((float)kPi * ip_);
 #pragma unroll
-            for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * (m - 0.5) / p;
-            double mt[3][IM + 1][LB + 1];
+                for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * ((float)m - 0.5f) * ip_;
+                float mt[3][IM + 1][LB + 1];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const double Pc = (al * ra[d] + be * rb[d]) / p;
-                const double pa = Pc - ra[d], pb = Pc - rb[d];
+                for (int d = 0; d < 3; ++d) {
+                    const float pa = be * ip_ * bab[d], pb = -al * ip_ * bab[d];
 #pragma unroll
-                for (int i = 0; i <= IM; ++i)
+                    for (int i = 0; i <= IM; ++i)
 #pragma unroll
-                    for (int j = 0; j <= LB; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, gm);
-            }
+                        for (int j = 0; j <= LB; ++j) mt[d][i][j] = moment1d_t<float>(i, j, pa, pb, gm);
+                }
 #pragma unroll
-            for (int mu = 0; mu < NA; ++mu) {
-                CartTerm ta[3];
-                const int nta = ao_terms(LA, mu, ta);
+                for (int mu = 0; mu < NA; ++mu) {
+                    CartTerm ta[3];
+                    const int nta = ao_terms(LA, mu, ta);
 #pragma unroll
-                for (int nu = 0; nu < NB; ++nu) {
-                    if constexpr (LA == 2 || LB == 2) {
-                        const double cs = ao_abs_sum(LA, mu) * ao_abs_sum(LB, nu);
-                        if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
-                    }
-                    CartTerm tb[3];
-                    const int ntb = ao_terms(LB, nu, tb);
-                    double sv = 0.0, ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;
-                    for (int aq = 0; aq < nta; ++aq)
-                        for (int bq = 0; bq < ntb; ++bq) {
-                            const double cf = pref * ta[aq].c * tb[bq].c * kab;
-                            const int i0 = ta[aq].px, i1 = ta[aq].py, i2 = ta[aq].pz;
-                            const int j0 = tb[bq].px, j1 = tb[bq].py, j2 = tb[bq].pz;
-                            const double m0 = mt[0][i0][j0], m1 = mt[1][i1][j1], m2 = mt[2][i2][j2];
-                            sv += cf * (m0 * m1 * m2);
-                            const double d0 = 2.0 * al * mt[0][i0 + 1][j0] - (i0 > 0 ? i0 * mt[0][i0 > 0 ? i0 - 1 : 0][j0] : 0.0);
-                            const double d1 = 2.0 * al * mt[1][i1 + 1][j1] - (i1 > 0 ? i1 * mt[1][i1 > 0 ? i1 - 1 : 0][j1] : 0.0);
-                            const double d2 = 2.0 * al * mt[2][i2 + 1][j2] - (i2 > 0 ? i2 * mt[2][i2 > 0 ? i2 - 1 : 0][j2] : 0.0);
-                            ga0 += cf * (d0 * (m1 * m2));
-                            ga1 += cf * (d1 * (m2 * m0));
-                            ga2 += cf * (d2 * (m0 * m1));
+                    for (int nu = 0; nu < NB; ++nu) {
+                        if constexpr (LA == 2 || LB == 2) {
+                            const float cs = (float)(ao_abs_sum(LA, mu) * ao_abs_sum(LB, nu));
+                            if (fabsf(pref) * cs * kab * gpref * poly < (float)kPrimScreen) continue;
                         }
-                    const R w = oc[mu * NB + nu];
-                    g0 += (R)ga0 * w;
-                    g1 += (R)ga1 * w;
-                    g2 += (R)ga2 * w;
-                    sp += (R)sv * pw[mu * NB + nu];
+                        CartTerm tb[3];
+                        const int ntb = ao_terms(LB, nu, tb);
+                        float sv = 0.0f, ga0 = 0.0f, ga1 = 0.0f, ga2 = 0.0f;
+                        for (int aq = 0; aq < nta; ++aq)
+                            for (int bq = 0; bq < ntb; ++bq) {
+                                const float cf = pref * (float)(ta[aq].c * tb[bq].c) * kab;
+                                const int i0 = ta[aq].px, i1 = ta[aq].py, i2 = ta[aq].pz;
+                                const int j0 = tb[bq].px, j1 = tb[bq].py, j2 = tb[bq].pz;
+                                const float m0 = mt[0][i0][j0], m1 = mt[1][i1][j1], m2 = mt[2][i2][j2];
+                                sv += cf * (m0 * m1 * m2);
+                                const float d0 = 2.0f * al * mt[0][i0 + 1][j0] - (i0 > 0 ? (float)i0 * mt[0][i0 > 0 ? i0 - 1 : 0][j0] : 0.0f);
+                                const float d1 = 2.0f * al * mt[1][i1 + 1][j1] - (i1 > 0 ? (float)i1 * mt[1][i1 > 0 ? i1 - 1 : 0][j1] : 0.0f);
+                                const float d2 = 2.0f * al * mt[2][i2 + 1][j2] - (i2 > 0 ? (float)i2 * mt[2][i2 > 0 ? i2 - 1 : 0][j2] : 0.0f);
+                                ga0 += cf * (d0 * (m1 * m2));
+                                ga1 += cf * (d1 * (m2 * m0));
+                                ga2 += cf * (d2 * (m0 * m1));
+                            }
+                        const R w = oc[mu * NB + nu];
+                        g0 += ga0 * w;
+                        g1 += ga1 * w;
+                        g2 += ga2 * w;
+                        sp += sv * pw[mu * NB + nu];
+                    }
+                }
+            }
+        }
+    } else {
+        for (int ip = 0; ip < A.nprim; ++ip) {
+            const double al = A.alpha[ip], cai = A.coeff[ip];
+            for (int jp = 0; jp < B.nprim; ++jp) {
+                const double be = B.alpha[jp];
+                const double pref = cai * B.coeff[jp];
+                const double p = al + be;
+                const double kab = exp(-al * be / p * r2);
+                const double spread = sqrt(1.0 / p);
+                const double gpref = (kPi / p) * sqrt(kPi / p);
+                const double poly = ipow(1.0 + r + spread, order);
+                if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
+                double gm[GM];
+                gm[0] = sqrt(kPi / p);
+    #pragma unroll
+                for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * (m - 0.5) / p;
+                double mt[3][IM + 1][LB + 1];
+    #pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double Pc = (al * ra[d] + be * rb[d]) / p;
+                    const double pa = Pc - ra[d], pb = Pc - rb[d];
+    #pragma unroll
+                    for (int i = 0; i <= IM; ++i)
+    #pragma unroll
+                        for (int j = 0; j <= LB; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, gm);
+                }
+    #pragma unroll
+                for (int mu = 0; mu < NA; ++mu) {
+                    CartTerm ta[3];
+                    const int nta = ao_terms(LA, mu, ta);
+    #pragma unroll
+                    for (int nu = 0; nu < NB; ++nu) {
+                        if constexpr (LA == 2 || LB == 2) {
+                            const double cs = ao_abs_sum(LA, mu) * ao_abs_sum(LB, nu);
+                            if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
+                        }
+                        CartTerm tb[3];
+                        const int ntb = ao_terms(LB, nu, tb);
+                        double sv = 0.0, ga0 = 0.0, ga1 = 0.0, ga2 = 0.0;
+                        for (int aq = 0; aq < nta; ++aq)
+                            for (int bq = 0; bq < ntb; ++bq) {
+                                const double cf = pref * ta[aq].c * tb[bq].c * kab;
+                                const int i0 = ta[aq].px, i1 = ta[aq].py, i2 = ta[aq].pz;
+                                const int j0 = tb[bq].px, j1 = tb[bq].py, j2 = tb[bq].pz;
+                                const double m0 = mt[0][i0][j0], m1 = mt[1][i1][j1], m2 = mt[2][i2][j2];
+                                sv += cf * (m0 * m1 * m2);
+                                const double d0 = 2.0 * al * mt[0][i0 + 1][j0] - (i0 > 0 ? i0 * mt[0][i0 > 0 ? i0 - 1 : 0][j0] : 0.0);
+                                const double d1 = 2.0 * al * mt[1][i1 + 1][j1] - (i1 > 0 ? i1 * mt[1][i1 > 0 ? i1 - 1 : 0][j1] : 0.0);
+                                const double d2 = 2.0 * al * mt[2][i2 + 1][j2] - (i2 > 0 ? i2 * mt[2][i2 > 0 ? i2 - 1 : 0][j2] : 0.0);
+                                ga0 += cf * (d0 * (m1 * m2));
+                                ga1 += cf * (d1 * (m2 * m0));
+                                ga2 += cf * (d2 * (m0 * m1));
+                            }
+                        const R w = oc[mu * NB + nu];
+                        g0 += (R)ga0 * w;
+                        g1 += (R)ga1 * w;
+                        g2 += (R)ga2 * w;
+                        sp += (R)sv * pw[mu * NB + nu];
+                    }
                 }
             }
         }
