@@ -152,6 +152,11 @@ SINGLE_WORKLOADS = {
 
 
 def run_single(name, precisions):
+    """Standalone single-system workload: prints one JSON line."""
+    print(json.dumps(single_system(name, precisions)))
+
+
+def single_system(name, precisions, oracle_max_ao=1000):
     """One water cluster (energy + gradient, 300 K, SCC tolerance 1e-7 Eh)
     through the same engine: wall time per energy+gradient, iterations and the
     size-independent checks (translation invariance of the gradient, total
@@ -187,12 +192,13 @@ def run_single(name, precisions):
              "dense_fp64_tflops": dense_flops(nao, it) / dt / 1e12}
         if ref is None:
             ref = (r["energy"], g)
+            ref_g = g
         else:
             r["max_abs_dE_vs_fp64"] = abs(r["energy"] - ref[0])
             r["max_abs_dg_vs_fp64"] = float(np.abs(g - ref[1]).max())
         out[prec] = r
         print("%s %s: %.2f s, %d iterations" % (name, prec, dt, it), file=sys.stderr, flush=True)
-    if nao <= 1000:
+    if nao <= oracle_max_ao:
         # small enough for the host oracle (one core): its time and the parity of the FP64 result
         import oracle
         t = time.perf_counter()
@@ -201,7 +207,9 @@ def run_single(name, precisions):
                                "sample": "the same system, FP64 C++ oracle"}
         if "fp64" in out:
             out["fp64"]["abs_dE_vs_oracle"] = abs(out["fp64"]["energy"] - ref_o["energy"])
-    print(json.dumps(out))
+            out["fp64"]["max_abs_dg_vs_oracle"] = float(np.abs(ref_g - ref_o["gradient"]).max())
+    os.environ.pop("GFN1_PROGRESS", None)
+    return out
 
 
 def run_optimize(args):
@@ -349,6 +357,9 @@ def main():
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
                     help="precisions run for a single-system workload (fp64 first)")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="1: also time the single systems (384, 8000 and 16000 AO) and the learned-start protocol "
+                         "(20000 training / 4096 evaluation snapshots) after the headline, as extra keys")
     ap.add_argument("--hw-queues", type=int, default=HW_QUEUES_DEFAULT, choices=range(1, 33), metavar="1..32",
                     help="GPU_MAX_HW_QUEUES for this run (set before the HIP runtime starts)")
     args = ap.parse_args()
@@ -458,6 +469,15 @@ def main():
                        "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time, headline path"},
         "cpu_baseline": cpu, "phase_ms": engines[head].timings(),
     }
+    if args.extras:
+        del engines
+        out["single_systems"] = {}
+        for name in ("water48", "water1000", "water2000"):
+            print("single system %s" % name, file=sys.stderr, flush=True)
+            out["single_systems"][name] = single_system(name, ["fp64", "mixed"], oracle_max_ao=400)
+        print("learned-start protocol", file=sys.stderr, flush=True)
+        from gfn1_amd import train_guess
+        out["learned_start_protocol"] = train_guess.protocol()
     print(json.dumps(out))
 
 

@@ -56,6 +56,45 @@ def evaluate(systems, model, tol):
     return out, de
 
 
+def protocol(n_train=20000, n_eval=4096, schedule="40:1e-3", batch_atoms=1024, seed=1, out=None, n_tight=512):
+    """The learned-start protocol: label n_train noisy snapshots with the plain
+    SCC, train a fresh model on the GPU, evaluate it on n_eval fresh snapshots
+    at the benchmark tolerance (and the first n_tight at the labelling
+    tolerance).  Saves the model to `out` when given; returns the report."""
+    label_tol = dict(energy_tolerance=1e-10, charge_tolerance=1e-7)
+    t0 = time.time()
+    train = geometry.noisy_snapshots(n_train)
+    log("generated %d training snapshots in %.1fs" % (len(train), time.time() - t0))
+    t0 = time.time()
+    train, q = label(train, label_tol)
+    t_label = time.time() - t0
+    log("labelled %d snapshots with the plain SCC in %.1fs" % (len(train), t_label))
+    model = GuessModel(seed=seed)
+    stages = [(int(a), float(b)) for a, b in (x.split(":") for x in schedule.split(","))]
+    t0 = time.time()
+    loss = []
+    for k, (ep, lr) in enumerate(stages):
+        loss += list(model.train(train, q, epochs=ep, batch_atoms=batch_atoms, learning_rate=lr, seed=seed + k))
+        log("stage %d: %d epochs at %.1e, loss %.3e" % (k, ep, lr, loss[-1]))
+    t_train = time.time() - t0
+    epochs = sum(ep for ep, _ in stages)
+    log("trained %d epochs in %.1fs, loss %.3e -> %.3e" % (epochs, t_train, loss[0], loss[-1]))
+    if out:
+        model.save(out)
+    t0 = time.time()
+    fresh = geometry.noisy_snapshots(n_eval, first=n_train)
+    log("generated %d evaluation snapshots in %.1fs" % (len(fresh), time.time() - t0))
+    ev, de = evaluate(fresh, model, dict(energy_tolerance=1e-7))
+    ev_tight, de_tight = evaluate(fresh[: min(n_tight, len(fresh))], model, label_tol)
+    return {
+        "train_snapshots": len(train), "train_atoms": int(sum(len(z) for z, _ in train)), "epochs": epochs,
+        "batch_atoms": batch_atoms, "schedule": schedule, "label_seconds": t_label, "train_seconds": t_train,
+        "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
+        "eval_snapshots": len(fresh), "eval_bench_tolerance": ev, "max_abs_dE_bench_tolerance": de,
+        "eval_tight_tolerance": ev_tight, "max_abs_dE_tight_tolerance": de_tight,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=20000)
@@ -65,37 +104,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=DEFAULT_OUT)
     args = ap.parse_args()
-    label_tol = dict(energy_tolerance=1e-10, charge_tolerance=1e-7)
-    t0 = time.time()
-    train = geometry.noisy_snapshots(args.train)
-    log("generated %d training snapshots in %.1fs" % (len(train), time.time() - t0))
-    t0 = time.time()
-    train, q = label(train, label_tol)
-    log("labelled %d snapshots with the plain SCC in %.1fs" % (len(train), time.time() - t0))
-    model = GuessModel(seed=args.seed)
-    stages = [(int(a), float(b)) for a, b in (x.split(":") for x in args.schedule.split(","))]
-    t0 = time.time()
-    loss = []
-    for k, (ep, lr) in enumerate(stages):
-        loss += list(model.train(train, q, epochs=ep, batch_atoms=args.batch_atoms, learning_rate=lr,
-                                 seed=args.seed + k))
-        log("stage %d: %d epochs at %.1e, loss %.3e" % (k, ep, lr, loss[-1]))
-    t_train = time.time() - t0
-    epochs = sum(ep for ep, _ in stages)
-    log("trained %d epochs in %.1fs, loss %.3e -> %.3e" % (epochs, t_train, loss[0], loss[-1]))
-    model.save(args.out)
-    t0 = time.time()
-    fresh = geometry.noisy_snapshots(args.eval, first=args.train)
-    log("generated %d evaluation snapshots in %.1fs" % (len(fresh), time.time() - t0))
-    ev, de = evaluate(fresh, model, dict(energy_tolerance=1e-7))
-    ev_tight, de_tight = evaluate(fresh[: min(512, len(fresh))], model, label_tol)
-    print(json.dumps({
-        "train_snapshots": len(train), "train_atoms": int(sum(len(z) for z, _ in train)), "epochs": epochs,
-        "batch_atoms": args.batch_atoms, "schedule": args.schedule, "train_seconds": t_train,
-        "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
-        "eval_snapshots": len(fresh), "eval_bench_tolerance": ev, "max_abs_dE_bench_tolerance": de,
-        "eval_tight_tolerance": ev_tight, "max_abs_dE_tight_tolerance": de_tight, "model": os.path.abspath(args.out),
-    }))
+    rep = protocol(args.train, args.eval, args.schedule, args.batch_atoms, args.seed, out=args.out)
+    rep["model"] = os.path.abspath(args.out)
+    print(json.dumps(rep))
 
 
 if __name__ == "__main__":
