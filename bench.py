@@ -50,6 +50,7 @@ METRIC = "SCC energy+gradient molecules/sec (batched, ML guess+mixed vs plain FP
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tridiag_traffic_batch4096.txt")
+MFMA_FILE = os.path.join(ROOT, "profiles", "mfma_fp64_batch4096.txt")
 
 
 def measured_traffic():
@@ -63,6 +64,25 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def measured_mfma():
+    """FP64 matrix-core rate per MFMA kernel from the committed rocprofv3 --pmc
+    summary (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512 over the kernel's duration, and
+    the SQ_VALU_MFMA_BUSY_CYCLES share), as {kernel: {...}}; {} when absent."""
+    out = {}
+    try:
+        with open(MFMA_FILE) as f:
+            rows = f.read().splitlines()
+    except OSError:
+        return out
+    for line in rows:
+        parts = line.split()
+        if len(parts) == 7 and parts[0].startswith("k_") and parts[5].endswith("%"):
+            out[parts[0]] = {"calls": int(parts[1]), "ms": float(parts[2]), "tflops": float(parts[4]),
+                             "frac_of_peak": float(parts[5].rstrip("%")) / 100.0,
+                             "busy": float(parts[6].rstrip("%")) / 100.0}
+    return out
 
 
 def dense_flops(n, iters):
@@ -459,6 +479,10 @@ def main():
                      "algorithmic_bytes_per_launch": tri_bytes_per_launch,
                      "kernel_ms_per_pass_alone": tri_ms, "serial_pass_ms": 1000.0 * serial_wall,
                      "concurrent_avg_launch_ms": conc_avg_ms,
+                     "limiter": "latency: one work-group per matrix runs n dependent column steps (a symmetric "
+                                "matrix-vector product, a reflector and a rank-2 update each, joined by "
+                                "work-group barriers); launches over fewer than 64 matrices move about "
+                                "0.02 TB/s, full launches 2.5-2.9 TB/s (profiles/k_tridiag_traffic_batch4096.txt)",
                      "note": "algorithmic bytes = 8 (n-1) n (n+1) / 6 per matrix (one pass over the trailing "
                              "lower triangle per column step), summed over the launch; launch durations from HIP "
                              "events on the bucket stream in one extra pass of the headline path with every "
@@ -467,6 +491,8 @@ def main():
         "dense_fp64": {"achieved_tflops": dense_tflops, "peak_tflops": FP64_PEAK_TFLOPS,
                        "frac": dense_tflops / FP64_PEAK_TFLOPS,
                        "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time, headline path"},
+        "mfma_fp64": {"peak_tflops": FP64_PEAK_TFLOPS, "source": "profiles/mfma_fp64_batch4096.txt",
+                      "kernels": measured_mfma()},
         "cpu_baseline": cpu, "phase_ms": engines[head].timings(),
     }
     if args.extras:
