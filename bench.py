@@ -119,7 +119,7 @@ def run_path(eng, label, steps, warmup):
         tri = [tri[0] + k_ms, tri[1] + k_n, tri[2] + k_b]
         print("%s step %d: %.3fs %s" % (label, s, times[-1], eng.timings()), file=sys.stderr, flush=True)
     n = eng.n
-    return {"ms": 1000.0 * float(np.mean(times)), "tri": tri,
+    return {"ms": 1000.0 * float(np.mean(times)), "tri": tri, "launch": eng.launch_stats(),
             "status": np.array([eng.status(i) for i in range(n)]),
             "iters": np.array([eng.iterations(i) for i in range(n)])}
 
@@ -380,6 +380,9 @@ def main():
     ap.add_argument("--extras", type=int, default=1,
                     help="1: also time the single systems (384, 8000 and 16000 AO) and the learned-start protocol "
                          "(20000 training / 4096 evaluation snapshots) after the headline, as extra keys")
+    ap.add_argument("--compare-host-loop", type=int, default=1,
+                    help="1: time one extra pass of the headline path with host-launched SCC iterations "
+                         "(no graph replays), reported under scc_launch")
     ap.add_argument("--hw-queues", type=int, default=HW_QUEUES_DEFAULT, choices=range(1, 33), metavar="1..32",
                     help="GPU_MAX_HW_QUEUES for this run (set before the HIP runtime starts)")
     args = ap.parse_args()
@@ -427,9 +430,21 @@ def main():
     flops = float(sum(dense_flops(n, it) for n, it in zip(nao, r["iters"])))
     dense_tflops = flops / (ms / 1000.0) / 1e12
     # concurrent launches (timed steps): durations include sharing the GPU
+    # (buckets iterating by graph replays record no per-launch events)
     tri_ms_c, tri_n_c, tri_bytes_c = r["tri"]
-    conc_avg_ms = tri_ms_c / max(tri_n_c, 1)
+    conc_avg_ms = tri_ms_c / tri_n_c if tri_n_c else None
     # the same kernel alone on the device (one extra, untimed pass)
+    scc_launch = {"per_pass": r["launch"],
+                  "note": "SCC phase of one headline pass: kernels run, host enqueue calls (one per kernel launch, "
+                          "library call or graph replay), graph replays, host-launched iterations, host reads of a "
+                          "bucket's gathered states"}
+    if args.compare_host_loop:
+        eng_h = BatchEngine(systems, options=opts, eigensolver=args.eigensolver, scc_graphs=False,
+                            **({"guess": GuessModel.load(args.model), "precision": "mixed"} if head == "learned" else {}))
+        rh = run_path(eng_h, "host loop", 1, 1)
+        scc_launch["host_loop"] = {"ms_per_step": rh["ms"], "per_pass": rh["launch"],
+                                   "same_iterations": bool(np.array_equal(rh["iters"], r["iters"]))}
+        del eng_h
     (tri_ms, tri_n, tri_bytes), serial_wall = standalone_kernel_pass(engines[head])
     tri_avg_ms = tri_ms / max(tri_n, 1)
     tri_bytes_per_launch = tri_bytes / max(tri_n, 1)
@@ -491,6 +506,7 @@ def main():
         "dense_fp64": {"achieved_tflops": dense_tflops, "peak_tflops": FP64_PEAK_TFLOPS,
                        "frac": dense_tflops / FP64_PEAK_TFLOPS,
                        "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time, headline path"},
+        "scc_launch": scc_launch,
         "mfma_fp64": {"peak_tflops": FP64_PEAK_TFLOPS, "source": "profiles/mfma_fp64_batch4096.txt",
                       "kernels": measured_mfma()},
         "cpu_baseline": cpu, "phase_ms": engines[head].timings(),

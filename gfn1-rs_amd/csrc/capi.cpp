@@ -357,6 +357,7 @@ void gfn1_batch_options_default(gfn1_batch_options* o) {
     o->guess = GFN1_GUESS_ZERO;
     o->eigensolver = 0;
     o->keep_matrices = 0;
+    o->scc_graphs = 1;
 }
 
 int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_params* params,
@@ -388,6 +389,7 @@ int gfn1_batch_create(int32_t n_inst, const gfn1_system* systems, const gfn1_par
     e.include_halogen = options->gradient.include_halogen;
     e.eigensolver = options->eigensolver;
     e.keep_matrices = options->keep_matrices;
+    e.scc_graphs = options->scc_graphs;
     e.mixed = options->precision == GFN1_PRECISION_MIXED;
     Status st;
     const DeviceModel* dm = const_cast<gfn1_params*>(params)->device_model(st);
@@ -462,6 +464,19 @@ int gfn1_batch_kernel_stats(const gfn1_batch* b, int kernel, double* total_ms, i
     long long nl = 0;
     b->batch->tridiag_stats(total_ms, &nl, bytes);
     *launches = nl;
+    return GFN1_OK;
+}
+
+int gfn1_batch_launch_stats(const gfn1_batch* b, int64_t out[GFN1_LAUNCH_STATS]) {
+    if (!b || !b->batch || !out) return GFN1_ERR_INVALID_INPUT;
+    const LaunchStats& ls = b->batch->launch_stats();
+    out[GFN1_LAUNCH_KERNELS] = ls.kernels;
+    out[GFN1_LAUNCH_ENQUEUES] = ls.enqueues;
+    out[GFN1_LAUNCH_GRAPH_REPLAYS] = ls.graph_replays;
+    out[GFN1_LAUNCH_HOST_ITERATIONS] = ls.host_iterations;
+    out[GFN1_LAUNCH_STATE_READS] = ls.state_reads;
+    out[GFN1_LAUNCH_GRAPHS_CAPTURED] = ls.graphs_captured;
+    out[GFN1_LAUNCH_CAPTURE_US] = (int64_t)(ls.capture_ms * 1000.0 + 0.5);
     return GFN1_OK;
 }
 

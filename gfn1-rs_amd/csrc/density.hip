@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(kThreads) k_orbitals(const int* __restrict__ a
     if (l >= count) return;
     const int I = (logical % per) / ntc, J = (logical % per) % ntc;
     const int inst = act[l];
+    if (inst < 0) return;  // gated off in a captured iteration
     const int n = nvec[inst];
     const int kn = min(st[inst].kneed, kcap);
     const int I0 = I * kT, J0 = J * kT;
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(kThreads) k_density(const int* __restrict__ ac
     }
     const int J = t;
     const int inst = act[l];
+    if (inst < 0) return;  // gated off in a captured iteration
     const int n = nvec[inst];
     const int kn = min(st[inst].kneed, kcap);
     const int I0 = I * kT, J0 = J * kT;

@@ -44,6 +44,26 @@ struct alignas(256) SccState {
     double nelec;
 };
 
+// Smearing temperature (K) from which every eigenvector is kept: the last rung
+// of the robust-fallback ladder smears at >= 3000 K.
+constexpr double kWideSmearing = 1000.0;
+// Longest Broyden history whose normal equations fit the mixing kernel's LDS.
+constexpr int kBroydenLdsHistory = 32;
+
+// Whether an instance's next SCC iteration is the plain one of a captured
+// bucket iteration: still iterating, no level-shift projector to apply, the
+// eigenvector capacity enough for its smearing, a Broyden history that fits in
+// LDS, and no eigensolver failure.  Every other iteration needs the host (level
+// shift GEMMs, a larger eigenvector buffer, global Broyden scratch, an error).
+// The device gate and the host loop evaluate this same function.
+__host__ __device__ inline bool plain_iteration(const SccState& s, int kcap, int ld, int broyden_memory) {
+    if (s.status != 0 || s.eig_fail || s.kneed_full > kcap) return false;
+    if (s.have_prev_p && s.level_shift != 0.0) return false;
+    if (s.temperature > kWideSmearing && kcap < ld) return false;
+    const int m = s.broyden_iter < broyden_memory ? s.broyden_iter : broyden_memory;
+    return m <= kBroydenLdsHistory;
+}
+
 struct DeviceTables {
     const ShellType* shells;     // (kMaxZ+1)*kMaxShellsPerAtom
     const ElementType* elements; // kMaxZ+1

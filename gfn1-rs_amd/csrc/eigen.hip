@@ -156,6 +156,7 @@ __global__ void __launch_bounds__(kTriThreads) k_tridiag(const int* __restrict__
                                                          double* tauvec) {
     extern __shared__ double sm[];
     const int l = blockIdx.x;
+    if (act[l] < 0) return;  // gated off in a captured iteration
     const int n = nvec[act[l]];
     gptr<double> A = (gptr<double>)Aptr[l];
     double* d = dvec + (size_t)l * ld;
@@ -621,6 +622,7 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
     extern __shared__ double sm[];
     const int l = blockIdx.x;
     const int inst = act[l];
+    if (inst < 0) return;
     const int n = nvec[inst];
     const double* dg = dvec + (size_t)l * ld;
     const double* eg = evec + (size_t)l * ld;
@@ -1094,6 +1096,7 @@ __global__ void __launch_bounds__(64) k_block_t(const int* act, const int* nvec,
                                                 const double* tauvec, double* work) {
     __shared__ double Gs[kBT * kBT];
     const int b = blockIdx.x, l = blockIdx.y;
+    if (act[l] < 0) return;
     const int n = nvec[act[l]];
     const int lo = b * kBT;
     if (n < 2 || lo > n - 2) return;
@@ -1154,6 +1157,7 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
     const int l = logical / nch, chunk = logical % nch;
     if (l >= count) return;
     const int inst = act[l];
+    if (inst < 0) return;
     const int n = nvec[inst];
     const int k = st[inst].kneed;
     const int c0 = chunk * kBTCols;

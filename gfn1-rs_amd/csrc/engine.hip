@@ -47,10 +47,6 @@ namespace {
 // R_L: Cholesky factor of S; R_A0: Cb^T H0 Cb (both fixed for the run)
 enum Role { R_S = 0, R_H0, R_F, R_P, R_W, R_CB0, R_CB1, R_T, R_A, R_L, R_A0, kRoles };
 // A_C32: FP32 operand behind the A32 operand in the A matrix (mixed precision)
-// Above this electronic temperature (K) the occupied-plus-smeared orbitals are not
-// bounded by the occupied count, so all eigenvectors are kept.
-constexpr double kWideSmearing = 1000.0;
-
 enum ActRole { A_F = 0, A_CB, A_C, A_T, A_A, A_P, A_S, A_X, A_C32, A_H0, A_L, A_A0, kActRoles };
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -137,6 +133,14 @@ Batch::~Batch() {
         (void)hipFree(bk.atoms);
         (void)hipFree(bk.gpair);
         (void)hipFree(bk.dedcn);
+        for (int slot = 0; slot < 2; ++slot) {
+            if (bk.gexec[slot]) (void)hipGraphExecDestroy(bk.gexec[slot]);
+            if (bk.g_st[slot]) (void)hipHostFree(bk.g_st[slot]);
+            if (bk.g_ev[slot]) (void)hipEventDestroy(bk.g_ev[slot]);
+        }
+        (void)hipFree(bk.g_ptrs);
+        (void)hipFree(bk.g_inst);
+        (void)hipFree(bk.g_act);
     }
     for (void* p : allocs_) (void)hipFree(p);
     if (ev_cn_) (void)hipEventDestroy(ev_cn_);
@@ -382,6 +386,13 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         HIP_TRY(hipHostMalloc((void**)&bk.h_aux, sizeof(double) * count, hipHostMallocCoherent));
         HIP_TRY(hipHostMalloc((void**)&bk.st_host, sizeof(SccState) * count, hipHostMallocCoherent));
         HIP_TRY(hipEventCreateWithFlags(&bk.done, hipEventDisableTiming));
+        HIP_TRY(hipMalloc(&bk.g_ptrs, sizeof(double*) * kActRoles * count));
+        HIP_TRY(hipMalloc(&bk.g_inst, sizeof(int) * count));
+        HIP_TRY(hipMalloc(&bk.g_act, sizeof(int) * count));
+        for (int slot = 0; slot < 2; ++slot) {
+            HIP_TRY(hipHostMalloc((void**)&bk.g_st[slot], sizeof(SccState) * count, hipHostMallocCoherent));
+            HIP_TRY(hipEventCreateWithFlags(&bk.g_ev[slot], hipEventDisableTiming));
+        }
         std::vector<int> atoms;
         for (int i : bk.inst)
             for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
@@ -525,6 +536,7 @@ Status Batch::run(const unsigned char* select) {
     serial_ = std::getenv("GFN1_SERIAL_BUCKETS") != nullptr;
     tri_ms_ = tri_bytes_ = 0.0;
     tri_launches_ = 0;
+    launch_stats_ = LaunchStats{};
     auto t0 = std::chrono::steady_clock::now();
     setup_end_ = t0;
     // the setup of the buckets overlaps the first SCC iterations of the buckets
@@ -895,6 +907,9 @@ Status Batch::scc_phase() {
     for (auto& bk : buckets_) {
         bk.active.clear();
         bk.iter = 0;
+        bk.graph = false;
+        bk.g_leave = false;
+        bk.g_issued = bk.g_done = 0;
         if (bk.in_setup) continue;  // its setup is still in flight
         for (int i : bk.inst)
             if (h_state_[i].status == 0) bk.active.push_back(i);
@@ -905,10 +920,36 @@ Status Batch::scc_phase() {
     int setups = 0;
     for (auto& bk : buckets_) setups += bk.in_setup ? 1 : 0;
     if (!setups) setup_end_ = std::chrono::steady_clock::now();
+    // Next work of a bucket whose stream is idle: replays while every
+    // instance's iteration is plain, the host loop otherwise, the gradient once
+    // nothing iterates any more.
+    auto advance = [&](Bucket& bk) -> Status {
+        if (bk.active.empty()) return launch_bucket_gradient(bk);
+        bool plain = true;
+        for (int i : bk.active) plain = plain && plain_iteration(h_state_[i], bk.kcap, bk.ld, broyden_memory_);
+        if (plain && !bk.g_leave && bk.iter == 0 && graph_bucket(bk)) {
+            Status st = capture_iteration(bk);
+            if (!st.ok()) return st;
+            bk.graph = true;
+            bk.inflight = true;
+            for (int r = 0; r < 2 && st.ok(); ++r) st = issue_replay(bk);
+            return st;
+        }
+        bk.graph = false;
+        // the iteration checks of finish_iteration that precede the next launch
+        bool hot = false;
+        for (int i : bk.active) hot = hot || h_state_[i].temperature > kWideSmearing;
+        if (hot && bk.kcap < bk.ld) {
+            Status gs = grow_kcap(bk, bk.ld);
+            if (!gs.ok()) return gs;
+        }
+        ++launch_stats_.host_iterations;
+        return launch_iteration(bk);
+    };
     // largest matrices first: their chains are the longest, the small buckets fill in
     for (auto bit = buckets_.rbegin(); bit != buckets_.rend(); ++bit) {
         if (bit->in_setup) continue;
-        Status st = bit->active.empty() ? launch_bucket_gradient(*bit) : launch_iteration(*bit);
+        Status st = advance(*bit);
         if (!st.ok()) return st;
         if (serial_) HIP_TRY(hipDeviceSynchronize());
     }
@@ -918,6 +959,34 @@ Status Batch::scc_phase() {
             Bucket& bk = *bit;
             if (!bk.inflight) continue;
             any = true;
+            if (bk.graph) {
+                // read the states of every finished replay; keep two in flight
+                while (bk.g_done < bk.g_issued) {
+                    const hipError_t q = hipEventQuery(bk.g_ev[bk.g_done % 2]);
+                    if (q == hipErrorNotReady) break;
+                    HIP_TRY(q);
+                    progressed = true;
+                    bool to_host = false;
+                    Status st = finish_replay(bk, to_host);
+                    if (!st.ok()) return st;
+                    if (!bk.g_leave && bk.g_issued - bk.g_done < 2) {
+                        st = issue_replay(bk);
+                        if (!st.ok()) return st;
+                    }
+                }
+                if (bk.g_done < bk.g_issued) continue;
+                // no replay in flight: the instances still iterating (if any)
+                // need the host loop
+                bk.inflight = false;
+                bk.active.clear();
+                for (int i : bk.inst)
+                    if (h_sel_[i] && h_state_[i].status == 0) bk.active.push_back(i);
+                bk.g_leave = true;
+                Status st = advance(bk);
+                if (!st.ok()) return st;
+                if (serial_) HIP_TRY(hipDeviceSynchronize());
+                continue;
+            }
             const hipError_t q = hipEventQuery(bk.done);
             if (q == hipErrorNotReady) continue;
             HIP_TRY(q);
@@ -941,10 +1010,11 @@ Status Batch::scc_phase() {
                 }
                 if (--setups == 0) setup_end_ = std::chrono::steady_clock::now();
             } else {
+                ++launch_stats_.state_reads;
                 st = finish_iteration(bk);
                 if (!st.ok()) return st;
             }
-            st = bk.active.empty() ? launch_bucket_gradient(bk) : launch_iteration(bk);
+            st = advance(bk);
             if (!st.ok()) return st;
             if (serial_) HIP_TRY(hipDeviceSynchronize());
         }
@@ -965,12 +1035,15 @@ Status Batch::launch_iteration(Bucket& bk) {
     const DeviceTables t = dm_->tables();
     const double one = 1.0, zero = 0.0;
     const int cnt = (int)bk.active.size();
+    // kernels and library calls this iteration puts on the stream
+    long long nk = 2;  // the two copies of upload_active
     Status st = upload_active(bk, bk.active, 0);
     if (!st.ok()) return st;
     int max_nsh = 0;
     for (int i : bk.active) max_nsh = std::max(max_nsh, h_nsh_[i]);
     const int n = bk.ld;
     // ---- Fock matrix from the input charges
+    ++nk;
     hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
                        (const int*)bk.act, (const double*)d_qin_, d_vin_g_, d_vin_t_);
     LAUNCH_CHECK();
@@ -981,6 +1054,7 @@ Status Batch::launch_iteration(Bucket& bk) {
     // F itself is formed only for the result matrices and the level-shift rungs;
     // otherwise A comes from the reduced congruence below
     if (opt_.keep_matrices || !ls.empty()) {
+        ++nk;
         hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)d_vin_t_);
         LAUNCH_CHECK();
@@ -988,6 +1062,7 @@ Status Batch::launch_iteration(Bucket& bk) {
     if (!ls.empty()) {
         // the staging buffers still feed the copy of the full active list
         HIP_TRY(hipStreamSynchronize(bk.stream));
+        nk += 2;
         st = upload_active(bk, ls, 0);
         if (!st.ok()) return st;
         const int c = (int)ls.size();
@@ -995,20 +1070,25 @@ Status Batch::launch_iteration(Bucket& bk) {
         double* const* S = bk.ptrs + (size_t)A_S * c;
         double* const* T = bk.ptrs + (size_t)A_T * c;
         double* const* Ab = bk.ptrs + (size_t)A_A * c;
+        ++nk;
         RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                                      (const double* const*)S, n, (const double* const*)P, n, &zero,
                                      (double* const*)T, n, c));
+        ++nk;
         RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                                      (const double* const*)T, n, (const double* const*)S, n, &zero,
                                      (double* const*)Ab, n, c));
         for (int k = 0; k < c; ++k) bk.h_aux[k] = h_state_[ls[k]].level_shift;
+        ++nk;
         hipLaunchKernelGGL(k_copy_words, dim3((2 * c + 255) / 256), dim3(256), 0, bk.stream,
                            (const unsigned int*)bk.h_aux, (unsigned int*)bk.eig_e, 2 * c);
         LAUNCH_CHECK();
+        ++nk;
         hipLaunchKernelGGL(k_level_shift, dim3(n, c), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig_e, (double* const*)Ab);
         LAUNCH_CHECK();
         HIP_TRY(hipStreamSynchronize(bk.stream));
+        nk += 2;
         st = upload_active(bk, bk.active, 0);
         if (!st.ok()) return st;
     }
@@ -1019,14 +1099,17 @@ Status Batch::launch_iteration(Bucket& bk) {
     double* const* A = bk.ptrs + (size_t)A_A * cnt;
     // ---- A = Cb^T F Cb (orthonormal basis; lower triangle and diagonal are used)
     if (!ls.empty()) {
+        ++nk;
         RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                                      (const double* const*)F, n, (const double* const*)Cb, n, &zero,
                                      (double* const*)T, n, cnt));
+        ++nk;
         RB_TRY(rocblas_dgemm_batched(bk.handle, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
                                      (const double* const*)Cb, n, (const double* const*)T, n, &zero,
                                      (double* const*)A, n, cnt));
     } else {
         const int nt64 = (n + 63) / 64;
+        ++nk;
         hipLaunchKernelGGL(k_reduce_fock, dim3(nt64, nt64, cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)d_vin_t_, Cb, bk.ptrs + (size_t)A_L * cnt, bk.ptrs + (size_t)A_A0 * cnt,
                            A);
@@ -1039,6 +1122,7 @@ Status Batch::launch_iteration(Bucket& bk) {
         // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
         // (returned transposed in Z, kc columns)
         HIP_TRY(hipEventRecord(bk.ev0, bk.stream));
+        ++nk;
         HIP_TRY(launch_tridiag(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A, bk.eig, bk.eig_e,
                                bk.tau));
         HIP_TRY(hipEventRecord(bk.ev1, bk.stream));
@@ -1048,12 +1132,15 @@ Status Batch::launch_iteration(Bucket& bk) {
             const double na = h_nao_[i];
             bk.ev_bytes += 8.0 * (na - 1.0) * na * (na + 1.0) / 6.0;
         }
+        ++nk;
         HIP_TRY(launch_tridiag_eig(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, (const double*)bk.eig,
                                    (const double*)bk.eig_e, d_eps_, (const int64_t*)d_vec0_, d_occ_, d_state_, Z,
                                    bk.lu, kc, opt_.keep_matrices != 0));
+        nk += 2;
         HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
                                      (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
         // C_k = Cb Z (into T); Z is stored transposed, element (m, j) at m * kc + j
+        ++nk;
         HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
                                 kc, Cb, Z, kc, 1, T));
     } else {
@@ -1061,29 +1148,36 @@ Status Batch::launch_iteration(Bucket& bk) {
         HIP_TRY(hipDeviceSynchronize());
         Status st2 = eigensolve(bk, cnt);
         if (!st2.ok()) return st2;
+        ++nk;  // the rocSOLVER eigensolve counts once
         // rocSOLVER's convergence flags, read by the host once the iteration is done
+        ++nk;
         hipLaunchKernelGGL(k_copy_words, dim3((cnt + 255) / 256), dim3(256), 0, bk.stream, (const unsigned int*)bk.info,
                            (unsigned int*)bk.h_aux, cnt);
         LAUNCH_CHECK();
         HIP_TRY(hipStreamSynchronize(bk.stream));
+        ++nk;
         hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
         LAUNCH_CHECK();
         // rocSOLVER returns the eigenvectors in place (column-major, element (m, j) at j * n + m)
+        ++nk;
         HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
                                 kc, Cb, A, 1, n, T));
     }
     // ---- P = C_k diag(f) C_k^T with the Mulliken row sums of S o P and H0 o P
     // (FP64 in every precision mode: the SCC energy and charges need it)
     double* const* P = bk.ptrs + (size_t)A_P * cnt;
+    ++nk;
     HIP_TRY(launch_density(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n, kc,
                            T, (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_, false, P,
                            bk.ptrs + (size_t)A_S * cnt, bk.ptrs + (size_t)A_H0 * cnt, (const int*)d_ao0_, d_rowS_,
                            d_rowH_, naos_));
     // ---- Mulliken charges, energy, convergence and mixing
+    ++nk;
     hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
                        (const double*)d_rowS_, (const double*)d_rowH_, naos_, d_qout_, d_state_);
     LAUNCH_CHECK();
+    ++nk;
     hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
                        (const int*)bk.act, (const double*)d_qout_, d_vout_g_, d_vout_t_);
     LAUNCH_CHECK();
@@ -1091,7 +1185,7 @@ Status Batch::launch_iteration(Bucket& bk) {
     int mmax = 0;
     for (int i : bk.active) mmax = std::max(mmax, std::min(h_state_[i].broyden_iter, broyden_memory_));
     const int beta_ld = std::max(mmax, 1);
-    if (mmax > 32) {
+    if (mmax > kBroydenLdsHistory) {
         size_t need = (size_t)cnt * (beta_ld + 1) * beta_ld;
         if (need > bk.beta_cap) {
             (void)hipFree(bk.beta);
@@ -1099,16 +1193,202 @@ Status Batch::launch_iteration(Bucket& bk) {
             bk.beta_cap = need * 2;
         }
     }
+    ++nk;
     hipLaunchKernelGGL(k_scc_update, dim3(cnt), dim3(256), 0, bk.stream, db_, t, (const int*)bk.act, prm_,
                        (const double*)d_qout_, (const double*)d_vout_g_, d_qin_, (const double*)d_q0_, d_state_,
                        d_brdf_, d_bru_, d_brom_, d_brql_, d_brdql_, (const int64_t*)d_broff_, bk.beta, beta_ld);
     LAUNCH_CHECK();
+    ++nk;
     hipLaunchKernelGGL(k_gather_state, dim3((cnt + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.act, cnt,
                        (const SccState*)d_state_, bk.st_host);
     LAUNCH_CHECK();
     HIP_TRY(hipEventRecord(bk.done, bk.stream));
+    launch_stats_.kernels += nk;
+    launch_stats_.enqueues += nk;
     bk.inflight = true;
     ++bk.iter;
+    return Status{};
+}
+
+// ---- Captured iterations
+//
+// A bucket solved by the own eigensolver iterates by replays of one captured
+// iteration while its instances' iterations are plain ones (plain_iteration):
+// the replay's kernels cover every position of the bucket, the gate kernel at
+// its head marks the positions that do not iterate with -1, and the kernels
+// skip those.  The host keeps two replays in flight per bucket and reads the
+// states the first one gathered while the second runs, so the bucket's stream
+// never waits for the host.  Each instance's arithmetic is the same as in a
+// host-launched iteration: the gate only decides whether it runs.  Once an
+// iteration that is not plain comes up (level shift, wide smearing beyond the
+// eigenvector capacity, a long Broyden history), the bucket finishes the run
+// in the host loop.
+bool Batch::graph_bucket(const Bucket& bk) const {
+    return opt_.scc_graphs && !serial_ && opt_.eigensolver != 1 && bk.ld <= kEigMaxDim && !bk.inst.empty();
+}
+
+// The kernels of one plain iteration over every position of the bucket, in
+// the order of launch_iteration's own-eigensolver branch, states gathered into
+// pinned slot `slot`.  Enqueued while the bucket's stream is being captured.
+Status Batch::enqueue_plain_iteration(Bucket& bk, int slot) {
+    const DeviceTables t = dm_->tables();
+    const int cnt = (int)bk.inst.size();
+    const int n = bk.ld, kc = bk.kcap;
+    int max_nsh = 0;
+    for (int i : bk.inst) max_nsh = std::max(max_nsh, h_nsh_[i]);
+    auto table = [&](int role) { return (double* const*)(bk.g_ptrs + (size_t)role * cnt); };
+    const int* act = bk.g_act;
+    hipLaunchKernelGGL(k_gate_active, dim3((cnt + 255) / 256), dim3(256), 0, bk.stream, (const int*)bk.g_inst, cnt,
+                       (const SccState*)d_state_, kc, n, broyden_memory_, bk.g_act);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t, act,
+                       (const double*)d_qin_, d_vin_g_, d_vin_t_);
+    LAUNCH_CHECK();
+    if (opt_.keep_matrices) {
+        hipLaunchKernelGGL(k_fock, dim3(n, cnt), dim3(256), 0, bk.stream, db_, act, (const double*)d_vin_t_);
+        LAUNCH_CHECK();
+    }
+    const int nt64 = (n + 63) / 64;
+    hipLaunchKernelGGL(k_reduce_fock, dim3(nt64, nt64, cnt), dim3(256), 0, bk.stream, db_, act,
+                       (const double*)d_vin_t_, table(A_CB), table(A_L), table(A_A0), table(A_A));
+    LAUNCH_CHECK();
+    HIP_TRY(launch_tridiag(bk.stream, cnt, act, (const int*)d_nao_, n, table(A_A), bk.eig, bk.eig_e, bk.tau));
+    HIP_TRY(launch_tridiag_eig(bk.stream, cnt, act, (const int*)d_nao_, n, (const double*)bk.eig,
+                               (const double*)bk.eig_e, d_eps_, (const int64_t*)d_vec0_, d_occ_, d_state_, table(A_C),
+                               bk.lu, kc, opt_.keep_matrices != 0));
+    HIP_TRY(launch_backtransform(bk.stream, cnt, act, (const int*)d_nao_, n, table(A_A), (const double*)bk.tau,
+                                 table(A_C), (const SccState*)d_state_, kc, bk.lu));
+    HIP_TRY(launch_orbitals(bk.stream, cnt, act, (const int*)d_nao_, (const SccState*)d_state_, n, kc, table(A_CB),
+                            table(A_C), kc, 1, table(A_T)));
+    HIP_TRY(launch_density(bk.stream, cnt, act, (const int*)d_nao_, (const SccState*)d_state_, n, kc, table(A_T),
+                           (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_, false, table(A_P),
+                           table(A_S), table(A_H0), (const int*)d_ao0_, d_rowS_, d_rowH_, naos_));
+    hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, act, (const double*)d_rowS_,
+                       (const double*)d_rowH_, naos_, d_qout_, d_state_);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t, act,
+                       (const double*)d_qout_, d_vout_g_, d_vout_t_);
+    LAUNCH_CHECK();
+    // the gate admits Broyden histories that fit in LDS only: no global scratch
+    hipLaunchKernelGGL(k_scc_update, dim3(cnt), dim3(256), 0, bk.stream, db_, t, act, prm_, (const double*)d_qout_,
+                       (const double*)d_vout_g_, d_qin_, (const double*)d_q0_, d_state_, d_brdf_, d_bru_, d_brom_,
+                       d_brql_, d_brdql_, (const int64_t*)d_broff_, bk.beta, 1);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gather_state, dim3((cnt + 63) / 64), dim3(64), 0, bk.stream, (const int*)bk.g_inst, cnt,
+                       (const SccState*)d_state_, bk.g_st[slot]);
+    LAUNCH_CHECK();
+    return Status{};
+}
+
+// Pointer tables of every position (parity 0) and the graphs of both state
+// slots.  Kept across runs; captured again when the eigenvector capacity or
+// its scratch changed.  The bucket's stream is idle.
+Status Batch::capture_iteration(Bucket& bk) {
+    if (bk.gexec[0] && bk.gexec[1] && bk.g_kcap == bk.kcap && bk.g_lu == bk.lu) return Status{};
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int slot = 0; slot < 2; ++slot)
+        if (bk.gexec[slot]) {
+            HIP_TRY(hipGraphExecDestroy(bk.gexec[slot]));
+            bk.gexec[slot] = nullptr;
+        }
+    const size_t mat = (size_t)bk.ld * bk.ld;
+    const size_t count = bk.inst.size();
+    double** hp = bk.h_ptrs;
+    for (size_t k = 0; k < count; ++k) {
+        auto at = [&](int role) { return bk.arena + ((size_t)role * count + k) * mat; };
+        hp[(size_t)A_F * count + k] = at(R_F);
+        hp[(size_t)A_CB * count + k] = at(R_CB0);
+        hp[(size_t)A_C * count + k] = at(R_CB1);
+        hp[(size_t)A_T * count + k] = at(R_T);
+        hp[(size_t)A_A * count + k] = at(R_A);
+        hp[(size_t)A_P * count + k] = at(R_P);
+        hp[(size_t)A_S * count + k] = at(R_S);
+        hp[(size_t)A_X * count + k] = at(R_W);
+        hp[(size_t)A_C32 * count + k] = (double*)((float*)at(R_A) + (size_t)bk.ld * bk.kcap);
+        hp[(size_t)A_H0 * count + k] = at(R_H0);
+        hp[(size_t)A_L * count + k] = at(R_L);
+        hp[(size_t)A_A0 * count + k] = at(R_A0);
+        bk.h_act[k] = bk.inst[k];
+    }
+    const int wp = (int)(kActRoles * count * sizeof(double*) / sizeof(unsigned int));
+    hipLaunchKernelGGL(k_copy_words, dim3((wp + 255) / 256), dim3(256), 0, bk.stream, (const unsigned int*)hp,
+                       (unsigned int*)bk.g_ptrs, wp);
+    LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy_words, dim3(((int)count + 255) / 256), dim3(256), 0, bk.stream,
+                       (const unsigned int*)bk.h_act, (unsigned int*)bk.g_inst, (int)count);
+    LAUNCH_CHECK();
+    // the staging buffers are free again once the copies have run
+    HIP_TRY(hipStreamSynchronize(bk.stream));
+    for (int slot = 0; slot < 2; ++slot) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(bk.stream, hipStreamCaptureModeRelaxed));
+        Status st = enqueue_plain_iteration(bk, slot);
+        const hipError_t end = hipStreamEndCapture(bk.stream, &graph);
+        if (!st.ok()) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        HIP_TRY(end);
+        size_t nodes = 0;
+        hipError_t e = hipGraphGetNodes(graph, nullptr, &nodes);
+        if (e == hipSuccess) e = hipGraphInstantiate(&bk.gexec[slot], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(e);
+        bk.g_nodes = (int)nodes;
+    }
+    bk.g_kcap = bk.kcap;
+    bk.g_lu = bk.lu;
+    ++launch_stats_.graphs_captured;
+    launch_stats_.capture_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return Status{};
+}
+
+Status Batch::issue_replay(Bucket& bk) {
+    const int slot = (int)(bk.g_issued % 2);
+    HIP_TRY(hipGraphLaunch(bk.gexec[slot], bk.stream));
+    HIP_TRY(hipEventRecord(bk.g_ev[slot], bk.stream));
+    ++bk.g_issued;
+    ++launch_stats_.graph_replays;
+    ++launch_stats_.enqueues;
+    launch_stats_.kernels += bk.g_nodes;
+    return Status{};
+}
+
+// States gathered by the oldest replay in flight (its event has completed):
+// the same checks as after a host-launched iteration for the instances that
+// iterated in it.  switch_to_host is set when an instance still iterating needs
+// an iteration that is not plain.
+Status Batch::finish_replay(Bucket& bk, bool& switch_to_host) {
+    const SccState* g = bk.g_st[bk.g_done % 2];
+    ++bk.g_done;
+    ++bk.iter;
+    ++launch_stats_.state_reads;
+    int plain = 0;
+    switch_to_host = false;
+    for (size_t k = 0; k < bk.inst.size(); ++k) {
+        const int i = bk.inst[k];
+        if (!h_sel_[i]) continue;
+        const bool iterated = g[k].total_iter != h_state_[i].total_iter;
+        h_state_[i] = g[k];
+        if (iterated) {
+            if (h_state_[i].kneed_full > bk.kcap)
+                return Status::error(kErrLinearAlgebra, "more partially occupied orbitals than the eigenvector capacity");
+            if (h_state_[i].eig_fail)
+                return Status::error(kErrLinearAlgebra, "eigenvalue cluster too large for the batched solver");
+        }
+        if (h_state_[i].status != 0) continue;
+        if (plain_iteration(h_state_[i], bk.kcap, bk.ld, broyden_memory_)) ++plain;
+        else switch_to_host = true;
+    }
+    if (progress_) {
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_scc_).count();
+        std::fprintf(stderr, "scc replay %d of the %d-AO bucket: %d instances still iterating, %.1f ms\n", bk.iter,
+                     bk.ld, plain, ms);
+        std::fflush(stderr);
+    }
+    if (plain == 0) bk.g_leave = true;
+    if (switch_to_host) bk.g_leave = true;
     return Status{};
 }
 

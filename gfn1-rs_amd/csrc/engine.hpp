@@ -42,6 +42,17 @@ struct SystemInput {
     int q0_len = 0;
 };
 
+// Host-side work of one run's SCC phase.  kernels counts the kernels the SCC
+// iterations ran (a graph replay contributes its node count); enqueues counts
+// the host calls that put them on a stream (one per kernel launch, rocBLAS call
+// or rocSOLVER eigensolve, one per graph replay); state_reads counts the times the host read a
+// bucket's gathered states, one per finished iteration or replay.
+struct LaunchStats {
+    long long kernels = 0, enqueues = 0, graph_replays = 0, host_iterations = 0, state_reads = 0;
+    long long graphs_captured = 0;
+    double capture_ms = 0.0;  // host time spent capturing and instantiating graphs
+};
+
 struct EngineOptions {
     // electronic
     int max_scc = 250;
@@ -64,6 +75,9 @@ struct EngineOptions {
     // execution
     int eigensolver = 0;   // 0 auto, 1 syevd, 2 syevj
     int keep_matrices = 0;
+    // iterate buckets of the own eigensolver by replays of a captured iteration
+    // (0: one host-launched kernel sequence per iteration)
+    int scc_graphs = 1;
     // mixed precision: FP32 MFMA for the energy-weighted density W and FP32
     // gradient contraction with compensated per-atom sums; the eigensolve, the
     // density P, Broyden mixing and energy sums stay FP64
@@ -147,6 +161,9 @@ class Batch {
         *bytes = tri_bytes_;
     }
 
+    // How the last run's SCC phase drove the device.
+    const LaunchStats& launch_stats() const { return launch_stats_; }
+
    private:
     struct Bucket {
         int ld = 0;
@@ -193,6 +210,23 @@ class Batch {
         int natoms = 0;
         double* gpair = nullptr;  // 3 * natoms
         double* dedcn = nullptr;  // natoms
+        // Captured iteration (own eigensolver buckets): one graph per state
+        // slot, the two differing only in the pinned buffer the states are
+        // gathered into.  The pointer tables list every instance of the bucket
+        // at its fixed position; k_gate_active, the graph's first node, marks
+        // the positions whose iteration is not a plain one with -1.
+        hipGraphExec_t gexec[2] = {nullptr, nullptr};
+        int g_nodes = 0;                  // kernels per replay
+        int g_kcap = 0;                   // eigenvector capacity the graphs were captured with
+        double* g_lu = nullptr;           // and the inverse-iteration scratch
+        double** g_ptrs = nullptr;        // kActRoles * count, every position, parity 0
+        int* g_inst = nullptr;            // instance at each position
+        int* g_act = nullptr;             // gated active list of the replay
+        SccState* g_st[2] = {nullptr, nullptr};  // pinned, count entries each
+        hipEvent_t g_ev[2] = {nullptr, nullptr};
+        bool graph = false;               // the current run iterates this bucket by replays
+        bool g_leave = false;             // a non-plain iteration came up: back to the host loop
+        long long g_issued = 0, g_done = 0;  // replays of the current run
     };
 
     Status setup_phase();
@@ -205,6 +239,11 @@ class Batch {
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
     Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
+    bool graph_bucket(const Bucket& bk) const;
+    Status capture_iteration(Bucket& bk);
+    Status enqueue_plain_iteration(Bucket& bk, int slot);
+    Status issue_replay(Bucket& bk);
+    Status finish_replay(Bucket& bk, bool& switch_to_host);
 
     const DeviceModel* dm_ = nullptr;
     EngineOptions opt_;
@@ -232,6 +271,7 @@ class Batch {
     float *d_gfeat_ = nullptr, *d_gout_ = nullptr;
     double tri_ms_ = 0.0, tri_bytes_ = 0.0;
     long long tri_launches_ = 0;
+    LaunchStats launch_stats_{};
 
     // device arrays
     DeviceBatch db_{};

@@ -332,6 +332,7 @@ __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, do
 __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
                                   double* vt) {
     const int inst = act[blockIdx.y];
+    if (inst < 0) return;  // gated off in a captured iteration
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsh) return;
@@ -363,6 +364,7 @@ __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act,
 // F = H0 - 1/2 S (v_mu + v_nu) (electronic.rs:3133-3158); one block per column.
 __global__ void k_fock(DeviceBatch b, const int* act, const double* vt) {
     const int inst = act[blockIdx.y];
+    if (inst < 0) return;
     const int n = b.nao[inst], ld = b.ld[inst];
     const int col = blockIdx.x;
     if (col >= n) return;
@@ -391,6 +393,7 @@ __global__ void __launch_bounds__(256) k_reduce_fock(DeviceBatch b, const int* a
     const int ti = blockIdx.x, tj = blockIdx.y;
     if (tj > ti) return;
     const int inst = act[blockIdx.z];
+    if (inst < 0) return;
     const int n = b.nao[inst], ld = b.ld[inst];
     const int I0 = ti * 64, J0 = tj * 64;
     if (I0 >= ld) return;
@@ -617,6 +620,7 @@ __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const doub
                        int stride, double* qout, SccState* st) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.x];
+    if (inst < 0) return;
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int a0 = b.ao0[inst], n = b.nao[inst];
     const int nblk = (n + 63) / 64;
@@ -726,6 +730,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     __shared__ double sh_c[32];
     __shared__ int sh_ri[16];
     const int inst = act[blockIdx.x];
+    if (inst < 0) return;
     SccState& s = st[inst];
     if (s.status != 0) return;
     // Every thread reads the state fields it needs here, before the first
@@ -825,9 +830,9 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
             // m <= 32 lives in LDS; longer histories use a caller-provided scratch
             // of (beta_ld + 1) * beta_ld doubles per active instance
             double* scratch = beta_scratch + (size_t)blockIdx.x * (beta_ld + 1) * beta_ld;
-            double* beta = m <= 32 ? sh_beta : scratch;
-            const int bl = m <= 32 ? 33 : beta_ld;
-            double* cvec = m <= 32 ? sh_c : scratch + (size_t)beta_ld * beta_ld;
+            double* beta = m <= kBroydenLdsHistory ? sh_beta : scratch;
+            const int bl = m <= kBroydenLdsHistory ? 33 : beta_ld;
+            double* cvec = m <= kBroydenLdsHistory ? sh_c : scratch + (size_t)beta_ld * beta_ld;
             // beta and c by dot products; one (r, c) pair per thread-iteration
             for (int pr = threadIdx.x; pr < m * (m + 1); pr += blockDim.x) {
                 const int r = pr / (m + 1), c = pr % (m + 1);
@@ -1618,6 +1623,17 @@ __global__ void k_gather_state(const int* act, int n, const SccState* st, SccSta
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) out[j] = st[act[j]];
     __threadfence_system();
+}
+
+// Active list of a captured bucket iteration: position j keeps its instance if
+// that instance's next iteration is a plain one, else -1 (the iteration's
+// kernels skip it).
+__global__ void k_gate_active(const int* inst, int count, const SccState* st, int kcap, int ld, int broyden_memory,
+                              int* act) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const int i = inst[j];
+    act[j] = plain_iteration(st[i], kcap, ld, broyden_memory) ? i : -1;
 }
 
 __global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n) {

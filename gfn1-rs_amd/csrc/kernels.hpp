@@ -66,6 +66,8 @@ __global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const
 // out[j] = st[act[j]] for j < n (out may be coherent pinned host memory).
 __global__ void k_gather_state(const int* act, int n, const SccState* st, SccState* out);
 // dst[i] = src[i] for i < n 32-bit words (src may be coherent pinned host memory).
+__global__ void k_gate_active(const int* inst, int count, const SccState* st, int kcap, int ld, int broyden_memory,
+                              int* act);
 __global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n);
 __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int cn_on, int scc_on,
                              double* dedcn_h, double* g_el, int natoms);

@@ -57,7 +57,8 @@ class GradientOptions(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("gradient", GradientOptions), ("precision", ctypes.c_int32), ("guess", ctypes.c_int32),
-                ("eigensolver", ctypes.c_int32), ("keep_matrices", ctypes.c_int32)]
+                ("eigensolver", ctypes.c_int32), ("keep_matrices", ctypes.c_int32),
+                ("scc_graphs", ctypes.c_int32)]
 
 
 class OptimizationOptions(ctypes.Structure):
@@ -124,6 +125,7 @@ SIGNATURES = [
     ("gfn1_batch_timings", None, [_VP, _DP]),
     ("gfn1_batch_kernel_stats", ctypes.c_int,
      [_VP, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_int64), _DP]),
+    ("gfn1_batch_launch_stats", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
     ("gfn1_batch_free", None, [_VP]),
     ("gfn1_params_shell_count", ctypes.c_int, [_VP, ctypes.c_int]),
     ("gfn1_guess_model_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),

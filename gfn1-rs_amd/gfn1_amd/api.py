@@ -871,7 +871,7 @@ class BatchEngine:
 
     def __init__(self, systems, params: Optional[Gfn1Parameters] = None,
                  options: Optional[AnalyticGradientOptions] = None, eigensolver=0, keep_matrices=False,
-                 guess: Optional[GuessModel] = None, precision="fp64"):
+                 guess: Optional[GuessModel] = None, precision="fp64", scc_graphs=True):
         lib = _lib.load()
         self._lib = lib
         self.params = _params(params)
@@ -882,6 +882,9 @@ class BatchEngine:
         bo.gradient = g
         bo.eigensolver = int(eigensolver)
         bo.keep_matrices = int(bool(keep_matrices))
+        # buckets of the own eigensolver iterate by replays of a captured HIP
+        # graph of one SCC iteration (False: host-launched kernel sequences)
+        bo.scc_graphs = int(bool(scc_graphs))
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         # mixed: FP32 MFMA density/energy-weighted-density products and FP32
@@ -1023,6 +1026,18 @@ class BatchEngine:
         if rc != 0:
             raise ValueError("unknown kernel id %r" % (kernel,))
         return ms.value, nl.value, nb.value
+
+    LAUNCH_STATS = ("kernels", "enqueues", "graph_replays", "host_iterations", "state_reads", "graphs_captured",
+                    "capture_us")
+
+    def launch_stats(self):
+        """How the last run's SCC phase drove the device: kernels run, host
+        enqueue calls, graph replays, host-launched iterations, host reads of
+        gathered states, graphs captured and the capture time (microseconds)."""
+        out = (ctypes.c_int64 * len(self.LAUNCH_STATS))()
+        if self._lib.gfn1_batch_launch_stats(self._h, out) != 0:
+            raise RuntimeError("no batch")
+        return dict(zip(self.LAUNCH_STATS, (int(v) for v in out)))
 
     def __del__(self):
         if getattr(self, "_h", None):

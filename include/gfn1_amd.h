@@ -210,6 +210,10 @@ typedef struct gfn1_batch_options {
     int32_t guess;           /* gfn1_guess */
     int32_t eigensolver;     /* 0 auto, 1 rocSOLVER syevd, 2 rocSOLVER syevj */
     int32_t keep_matrices;   /* copy S/H0/F/P/W back to the host per instance */
+    int32_t scc_graphs;      /* 1 (default): buckets of the own eigensolver iterate by
+                                replays of a captured HIP graph of one SCC iteration;
+                                0: one host-launched kernel sequence per iteration.
+                                Results are the same either way. */
 } gfn1_batch_options;
 
 void gfn1_batch_options_default(gfn1_batch_options* options);
@@ -243,6 +247,25 @@ void gfn1_batch_timings(const gfn1_batch* batch, double out[4]);
 enum { GFN1_KERNEL_TRIDIAG = 0 };
 int gfn1_batch_kernel_stats(const gfn1_batch* batch, int kernel, double* total_ms, int64_t* launches,
                             double* bytes);
+/* How the last gfn1_batch_run drove the device in its SCC phase: kernels run
+ * by the SCC iterations (a graph replay counts its nodes), host calls that
+ * enqueued them (a kernel launch, a rocBLAS call, a rocSOLVER eigensolve or a
+ * graph replay each count once), graph replays, host-launched iterations, reads
+ * of a bucket's gathered states by the host (one per finished iteration or
+ * replay), graphs captured (two per bucket, kept for later runs) and the host
+ * time spent capturing them (microseconds).  No reference counterpart
+ * (measurement hook of the batch engine). */
+enum {
+    GFN1_LAUNCH_KERNELS = 0,
+    GFN1_LAUNCH_ENQUEUES,
+    GFN1_LAUNCH_GRAPH_REPLAYS,
+    GFN1_LAUNCH_HOST_ITERATIONS,
+    GFN1_LAUNCH_STATE_READS,
+    GFN1_LAUNCH_GRAPHS_CAPTURED,
+    GFN1_LAUNCH_CAPTURE_US,
+    GFN1_LAUNCH_STATS
+};
+int gfn1_batch_launch_stats(const gfn1_batch* batch, int64_t out[GFN1_LAUNCH_STATS]);
 void gfn1_batch_free(gfn1_batch* batch);
 
 /* --- Learned SCC start charges ---
