@@ -105,10 +105,18 @@ def calibrate_copy():
                       "bytes_written": y.numel() * 8, "launches": 3}))
 
 
-def run_path(eng, label, steps, warmup):
+def run_path(eng, label, steps, warmup, tune=False):
     for w in range(warmup):
         eng.run()
         print("%s warmup %d: %s" % (label, w, eng.timings()), file=sys.stderr, flush=True)
+    tuning = None
+    if tune:
+        # widths of the back-transformation per size bucket, timed on the
+        # warm-up's data (results do not depend on them); one more untimed pass
+        # captures the iteration graphs with the chosen widths
+        tuning = eng.autotune()
+        eng.run()
+        print("%s autotuned: %s" % (label, [(t["ld"], t["cols"]) for t in tuning]), file=sys.stderr, flush=True)
     times = []
     tri = [0.0, 0, 0.0]
     for s in range(steps):
@@ -119,7 +127,7 @@ def run_path(eng, label, steps, warmup):
         tri = [tri[0] + k_ms, tri[1] + k_n, tri[2] + k_b]
         print("%s step %d: %.3fs %s" % (label, s, times[-1], eng.timings()), file=sys.stderr, flush=True)
     n = eng.n
-    return {"ms": 1000.0 * float(np.mean(times)), "tri": tri, "launch": eng.launch_stats(),
+    return {"ms": 1000.0 * float(np.mean(times)), "tri": tri, "launch": eng.launch_stats(), "tuning": tuning,
             "status": np.array([eng.status(i) for i in range(n)]),
             "iters": np.array([eng.iterations(i) for i in range(n)])}
 
@@ -380,6 +388,11 @@ def main():
     ap.add_argument("--extras", type=int, default=1,
                     help="1: also time the single systems (384, 8000 and 16000 AO) and the learned-start protocol "
                          "(20000 training / 4096 evaluation snapshots) after the headline, as extra keys")
+    ap.add_argument("--standalone-pass", type=int, default=1,
+                    help="1: one extra pass with every size bucket alone on the device, for the roofline figures "
+                         "(0 only for profiles of the concurrent pass)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: autotune the back-transformation width per size bucket after the warm-up passes")
     ap.add_argument("--compare-host-loop", type=int, default=1,
                     help="1: time one extra pass of the headline path with host-launched SCC iterations "
                          "(no graph replays), reported under scc_launch")
@@ -417,12 +430,12 @@ def main():
     res, engines = {}, {}
     if args.path in ("both", "plain"):
         engines["plain"] = BatchEngine(systems, options=opts, eigensolver=args.eigensolver)
-        res["plain"] = run_path(engines["plain"], "plain", args.steps, args.warmup)
+        res["plain"] = run_path(engines["plain"], "plain", args.steps, args.warmup, bool(args.autotune))
     if args.path in ("both", "learned"):
         model = GuessModel.load(args.model)
         engines["learned"] = BatchEngine(systems, options=opts, eigensolver=args.eigensolver, guess=model,
                                          precision="mixed")
-        res["learned"] = run_path(engines["learned"], "learned+mixed", args.steps, args.warmup)
+        res["learned"] = run_path(engines["learned"], "learned+mixed", args.steps, args.warmup, bool(args.autotune))
     head = "learned" if "learned" in res else "plain"
     r = res[head]
     ms = r["ms"]
@@ -441,11 +454,14 @@ def main():
     if args.compare_host_loop:
         eng_h = BatchEngine(systems, options=opts, eigensolver=args.eigensolver, scc_graphs=False,
                             **({"guess": GuessModel.load(args.model), "precision": "mixed"} if head == "learned" else {}))
-        rh = run_path(eng_h, "host loop", 1, 1)
+        rh = run_path(eng_h, "host loop", 1, 1, bool(args.autotune))
         scc_launch["host_loop"] = {"ms_per_step": rh["ms"], "per_pass": rh["launch"],
                                    "same_iterations": bool(np.array_equal(rh["iters"], r["iters"]))}
         del eng_h
-    (tri_ms, tri_n, tri_bytes), serial_wall = standalone_kernel_pass(engines[head])
+    if args.standalone_pass:
+        (tri_ms, tri_n, tri_bytes), serial_wall = standalone_kernel_pass(engines[head])
+    else:  # profiling runs of the concurrent pass only: no roofline figures
+        (tri_ms, tri_n, tri_bytes), serial_wall = (0.0, 0, 0.0), 0.0
     tri_avg_ms = tri_ms / max(tri_n, 1)
     tri_bytes_per_launch = tri_bytes / max(tri_n, 1)
     achieved = tri_bytes_per_launch / (tri_avg_ms / 1000.0) / 1e9 if tri_n else 0.0
@@ -507,6 +523,9 @@ def main():
                        "frac": dense_tflops / FP64_PEAK_TFLOPS,
                        "note": "whole-pass dense work 19 n^3 per SCC iteration + 11 n^3 over wall time, headline path"},
         "scc_launch": scc_launch,
+        "autotune": {"kernel": "k_backtransform", "choices": r["tuning"],
+                     "note": "eigenvectors per work-group per size bucket (ld = padded dimension), mean launch ms "
+                             "per width timed on the warm-up pass's reflectors; every width gives identical bits"},
         "mfma_fp64": {"peak_tflops": FP64_PEAK_TFLOPS, "source": "profiles/mfma_fp64_batch4096.txt",
                       "kernels": measured_mfma()},
         "cpu_baseline": cpu, "phase_ms": engines[head].timings(),

@@ -467,6 +467,37 @@ int gfn1_batch_kernel_stats(const gfn1_batch* b, int kernel, double* total_ms, i
     return GFN1_OK;
 }
 
+int gfn1_batch_autotune(gfn1_batch* b, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch) return fail(err, Status::error(kErrInvalidInput, "invalid batch"));
+    Status st = b->batch->autotune();
+    if (!st.ok()) return fail(err, st);
+    return GFN1_OK;
+}
+
+int gfn1_batch_set_backtransform_width(gfn1_batch* b, int32_t cols, gfn1_error* err) {
+    clear(err);
+    if (!b || !b->batch) return fail(err, Status::error(kErrInvalidInput, "invalid batch"));
+    Status st = b->batch->set_backtransform_width(cols);
+    if (!st.ok()) return fail(err, st);
+    return GFN1_OK;
+}
+
+int32_t gfn1_batch_tuning_count(const gfn1_batch* b) {
+    return (b && b->batch) ? (int32_t)b->batch->tuning().size() : 0;
+}
+
+int gfn1_batch_tuning_entry(const gfn1_batch* b, int32_t k, int32_t out_i[3], double out_ms[3]) {
+    if (!b || !b->batch || !out_i || !out_ms || k < 0 || k >= (int32_t)b->batch->tuning().size())
+        return GFN1_ERR_INVALID_INPUT;
+    const auto& r = b->batch->tuning()[k];
+    out_i[0] = r.ld;
+    out_i[1] = r.count;
+    out_i[2] = r.cols;
+    for (int w = 0; w < 3; ++w) out_ms[w] = r.ms[w];
+    return GFN1_OK;
+}
+
 int gfn1_batch_launch_stats(const gfn1_batch* b, int64_t out[GFN1_LAUNCH_STATS]) {
     if (!b || !b->batch || !out) return GFN1_ERR_INVALID_INPUT;
     const LaunchStats& ls = b->batch->launch_stats();

@@ -1075,12 +1075,14 @@ __global__ void __launch_bounds__(kThreads) k_tridiag_eig(const int* act, const 
 // fixed order.  The chunks of one matrix are mapped onto one XCD (dispatch order
 // is round-robin over the eight XCDs), so its reflectors are shared in that L2.
 typedef double dvec4 __attribute__((ext_vector_type(4)));
-// eigenvectors per work-group: 64 (one work-group per CU by registers and LDS)
-// re-reads the reflectors half as often but measured no faster than 32
-constexpr int kBTCols = 32;
-constexpr int kBTCT = kBTCols / 16;                        // their 16-column tiles
+// Eigenvectors per work-group (COLS) and row tiles per wavefront (TILES) are
+// template parameters: TILES = 2 covers matrices up to 256 rows with half the
+// accumulator registers, and the column width trades the number of
+// work-groups against how often each re-stages the reflectors.  Each column's
+// arithmetic (its row tiles per wavefront, the fixed-order sum over the
+// wavefronts) does not depend on either, so every variant gives the same bits
+// and the choice among them is a matter of speed only (launch_backtransform).
 constexpr int kBTWaves = 8;                                // 512 threads
-constexpr int kBTTiles = kEigMaxDim / 16 / kBTWaves;      // row tiles per wavefront
 constexpr int kXcds = 8;
 constexpr int kVbS = kBT + 1;  // LDS row stride of the staged reflectors (odd: conflict-free row writes)
 
@@ -1143,15 +1145,17 @@ __global__ void __launch_bounds__(64) k_block_t(const int* act, const int* nvec,
     }
 }
 
+template <int COLS, int TILES>
 __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act, const int* nvec, int ld,
                                                                  double* const* Aptr, double* const* Ztptr,
                                                                  const SccState* st, int kcap, int count, int nch,
                                                                  const double* tfac) {
+    constexpr int CT = COLS / 16;  // 16-column tiles per work-group
     extern __shared__ double sm[];
     double* Vb = sm;                                  // staged reflectors, (tile-aligned rows) x kVbS
     double* part = Vb + (size_t)ld * kVbS;            // T W partials (2 tiles) of half the wavefronts
-    double* Wp = part + (kBTWaves / 2) * kBTCT * 4 * 64;  // -(T W), 16 x kBTCols
-    double* Tl = Wp + kBT * kBTCols;                  // T of the block, 16 x 16
+    double* Wp = part + (kBTWaves / 2) * CT * 4 * 64;  // -(T W), 16 x COLS
+    double* Tl = Wp + kBT * COLS;                  // T of the block, 16 x 16
     const int id = blockIdx.x, per = gridDim.x / kXcds;
     const int logical = (id % kXcds) * per + id / kXcds;
     const int l = logical / nch, chunk = logical % nch;
@@ -1160,16 +1164,16 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
     if (inst < 0) return;
     const int n = nvec[inst];
     const int k = st[inst].kneed;
-    const int c0 = chunk * kBTCols;
+    const int c0 = chunk * COLS;
     if (c0 >= k || n < 2) return;
     gptr<const double> A = (gptr<const double>)Aptr[l];
     gptr<double> Z = (gptr<double>)Ztptr[l];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, li = lane & 15, lg = lane >> 4;
-    dvec4 z[kBTTiles][kBTCT];
+    dvec4 z[TILES][CT];
 #pragma unroll
-    for (int t = 0; t < kBTTiles; ++t)
+    for (int t = 0; t < TILES; ++t)
 #pragma unroll
-        for (int ct = 0; ct < kBTCT; ++ct) {
+        for (int ct = 0; ct < CT; ++ct) {
             const int col = c0 + 16 * ct + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1201,80 +1205,80 @@ __global__ void __launch_bounds__(64 * kBTWaves) k_backtransform(const int* act,
         // this wavefront's share of T W = T V^T Z: W over its row tiles (accumulator
         // layout, element e = reflectors 4e .. 4e + 3), then T W with that W as the B
         // operand of k-step e
-        dvec4 aw[kBTCT];
+        dvec4 aw[CT];
 #pragma unroll
-        for (int ct = 0; ct < kBTCT; ++ct) aw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < CT; ++ct) aw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < kBTTiles; ++t) {
+        for (int t = 0; t < TILES; ++t) {
             const int row0 = 16 * (wid + kBTWaves * t);
             if (row0 >= base && row0 < n) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const double va = Vb[(row0 + 4 * e + lg - base) * kVbS + li];
 #pragma unroll
-                    for (int ct = 0; ct < kBTCT; ++ct) aw[ct] = mfma16x16x4(va, z[t][ct][e], aw[ct]);
+                    for (int ct = 0; ct < CT; ++ct) aw[ct] = mfma16x16x4(va, z[t][ct][e], aw[ct]);
                 }
             }
         }
-        dvec4 tw[kBTCT];
+        dvec4 tw[CT];
 #pragma unroll
-        for (int ct = 0; ct < kBTCT; ++ct) tw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < CT; ++ct) tw[ct] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const double ta = Tl[li * kBT + 4 * e + lg];
 #pragma unroll
-            for (int ct = 0; ct < kBTCT; ++ct) tw[ct] = mfma16x16x4(ta, aw[ct][e], tw[ct]);
+            for (int ct = 0; ct < CT; ++ct) tw[ct] = mfma16x16x4(ta, aw[ct][e], tw[ct]);
         }
         // partials summed in two steps through half-size LDS: the upper wavefronts
         // deposit theirs, the lower ones add their own in place
         const int slot = wid % (kBTWaves / 2);
         if (wid >= kBTWaves / 2) {
 #pragma unroll
-            for (int ct = 0; ct < kBTCT; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) part[((slot * kBTCT + ct) * 4 + e) * 64 + lane] = tw[ct][e];
+                for (int e = 0; e < 4; ++e) part[((slot * CT + ct) * 4 + e) * 64 + lane] = tw[ct][e];
         }
         __syncthreads();
         if (wid < kBTWaves / 2) {
 #pragma unroll
-            for (int ct = 0; ct < kBTCT; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-                for (int e = 0; e < 4; ++e) part[((slot * kBTCT + ct) * 4 + e) * 64 + lane] += tw[ct][e];
+                for (int e = 0; e < 4; ++e) part[((slot * CT + ct) * 4 + e) * 64 + lane] += tw[ct][e];
         }
         __syncthreads();
         // -(T W)[i][cc]: entry (i, j) of a tile sits in element i / 4 of lane j + 16 (i % 4)
-        for (int q = tid; q < kBT * kBTCols; q += 64 * kBTWaves) {
-            const int i = q / kBTCols, cc = q % kBTCols, ct = cc >> 4, j = cc & 15;
+        for (int q = tid; q < kBT * COLS; q += 64 * kBTWaves) {
+            const int i = q / COLS, cc = q % COLS, ct = cc >> 4, j = cc & 15;
             const int off = (ct * 4 + (i >> 2)) * 64 + j + 16 * (i & 3);
             double sum = 0.0;
 #pragma unroll
-            for (int w = 0; w < kBTWaves / 2; ++w) sum += part[w * kBTCT * 4 * 64 + off];
-            Wp[i * kBTCols + cc] = -sum;
+            for (int w = 0; w < kBTWaves / 2; ++w) sum += part[w * CT * 4 * 64 + off];
+            Wp[i * COLS + cc] = -sum;
         }
         __syncthreads();
         // Z -= V (T W): k-step s covers reflectors 4s .. 4s + 3
-        double wb[4][kBTCT];
+        double wb[4][CT];
 #pragma unroll
         for (int s2 = 0; s2 < 4; ++s2)
 #pragma unroll
-            for (int ct = 0; ct < kBTCT; ++ct) wb[s2][ct] = Wp[(4 * s2 + lg) * kBTCols + 16 * ct + li];
+            for (int ct = 0; ct < CT; ++ct) wb[s2][ct] = Wp[(4 * s2 + lg) * COLS + 16 * ct + li];
 #pragma unroll
-        for (int t = 0; t < kBTTiles; ++t) {
+        for (int t = 0; t < TILES; ++t) {
             const int row0 = 16 * (wid + kBTWaves * t);
             if (row0 >= base && row0 < n) {
 #pragma unroll
                 for (int s2 = 0; s2 < 4; ++s2) {
                     const double va = Vb[(row0 + li - base) * kVbS + 4 * s2 + lg];
 #pragma unroll
-                    for (int ct = 0; ct < kBTCT; ++ct) z[t][ct] = mfma16x16x4(va, wb[s2][ct], z[t][ct]);
+                    for (int ct = 0; ct < CT; ++ct) z[t][ct] = mfma16x16x4(va, wb[s2][ct], z[t][ct]);
                 }
             }
         }
     }
 #pragma unroll
-    for (int t = 0; t < kBTTiles; ++t)
+    for (int t = 0; t < TILES; ++t)
 #pragma unroll
-        for (int ct = 0; ct < kBTCT; ++ct) {
+        for (int ct = 0; ct < CT; ++ct) {
             const int col = c0 + 16 * ct + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1312,26 +1316,52 @@ hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const in
     return hipGetLastError();
 }
 
-hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
-                                double* const* A, const double* tau, double* const* Zt, const SccState* st,
-                                int kcap, double* work) {
-    if (ld > kEigMaxDim || kcap > ld || ld % 16 != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_block_t, dim3(ld / 16, count), dim3(64), 0, s, act, nvec, ld, A, tau, work);
-    auto lds_of = [](int l) {
-        return ((size_t)l * kVbS + (kBTWaves / 2) * kBTCT * 4 * 64 + kBT * kBTCols + kBT * kBT) * sizeof(double);
-    };
+namespace {
+size_t backtransform_lds(int ld, int cols) {
+    return ((size_t)ld * kVbS + (kBTWaves / 2) * (cols / 16) * 4 * 64 + kBT * cols + kBT * kBT) * sizeof(double);
+}
+
+template <int COLS, int TILES>
+hipError_t launch_backtransform_t(hipStream_t s, int count, const int* act, const int* nvec, int ld,
+                                  double* const* A, double* const* Zt, const SccState* st, int kcap, double* work) {
     static bool attr = false;  // LDS above 64 KiB must be requested explicitly
     if (!attr) {
-        hipError_t err = hipFuncSetAttribute((const void*)k_backtransform,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_of(kEigMaxDim));
+        hipError_t err = hipFuncSetAttribute((const void*)k_backtransform<COLS, TILES>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)backtransform_lds(16 * kBTWaves * TILES, COLS));
         if (err != hipSuccess) return err;
         attr = true;
     }
-    const int nch = (kcap + kBTCols - 1) / kBTCols;
+    const int nch = (kcap + COLS - 1) / COLS;
     const int blocks = (count * nch + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_backtransform, dim3(blocks), dim3(64 * kBTWaves), lds_of(ld), s, act, nvec, ld, A, Zt, st,
-                       kcap, count, nch, (const double*)work);
+    hipLaunchKernelGGL((k_backtransform<COLS, TILES>), dim3(blocks), dim3(64 * kBTWaves), backtransform_lds(ld, COLS),
+                       s, act, nvec, ld, A, Zt, st, kcap, count, nch, (const double*)work);
     return hipGetLastError();
+}
+}  // namespace
+
+bool backtransform_width_ok(int ld, int cols) {
+    return cols == 16 || cols == 32 || (cols == 64 && ld <= 16 * kBTWaves * 2);
+}
+
+hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
+                                double* const* A, const double* tau, double* const* Zt, const SccState* st,
+                                int kcap, double* work, int cols) {
+    if (ld > kEigMaxDim || kcap > ld || ld % 16 != 0 || !backtransform_width_ok(ld, cols))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_block_t, dim3(ld / 16, count), dim3(64), 0, s, act, nvec, ld, A, tau, work);
+    static_assert(kEigMaxDim == 16 * kBTWaves * 4, "four row tiles per wavefront cover the largest matrix");
+    const bool small = ld <= 16 * kBTWaves * 2;
+    switch (cols) {
+        case 16:
+            return small ? launch_backtransform_t<16, 2>(s, count, act, nvec, ld, A, Zt, st, kcap, work)
+                         : launch_backtransform_t<16, 4>(s, count, act, nvec, ld, A, Zt, st, kcap, work);
+        case 64:
+            return launch_backtransform_t<64, 2>(s, count, act, nvec, ld, A, Zt, st, kcap, work);
+        default:
+            return small ? launch_backtransform_t<32, 2>(s, count, act, nvec, ld, A, Zt, st, kcap, work)
+                         : launch_backtransform_t<32, 4>(s, count, act, nvec, ld, A, Zt, st, kcap, work);
+    }
 }
 
 }  // namespace gfn1

@@ -16,6 +16,7 @@ namespace gfn1 {
 
 constexpr int kEigMaxDim = 512;     // largest matrix the kernels accept
 constexpr int kEigPanel = 8;        // Householder panel width
+constexpr int kBacktransformDefaultCols = 32;
 constexpr int kEigMaxCluster = 32;  // largest tight eigenvalue cluster resolved by Rayleigh-Ritz
 constexpr int kEigMargin = 16;      // eigenvalues formed above the occupied ones (partial spectrum)
 
@@ -40,8 +41,11 @@ hipError_t launch_tridiag_eig(hipStream_t s, int count, const int* act, const in
 // Zt[l] <- (Q Z)^T with Q the product of the reflectors left by launch_tridiag.
 // work: 16 * ld doubles per active instance (the blocks' T factors); the lu
 // workspace of launch_tridiag_eig is free at this point and large enough.
+// cols: eigenvectors per work-group, 16, 32 or (ld <= 256) 64; every width
+// gives the same result bits.
 hipError_t launch_backtransform(hipStream_t s, int count, const int* act, const int* nvec, int ld,
                                 double* const* A, const double* tau, double* const* Zt, const SccState* st,
-                                int kcap, double* work);
+                                int kcap, double* work, int cols = kBacktransformDefaultCols);
+bool backtransform_width_ok(int ld, int cols);
 
 }  // namespace gfn1

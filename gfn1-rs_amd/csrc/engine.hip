@@ -1138,7 +1138,7 @@ Status Batch::launch_iteration(Bucket& bk) {
                                    bk.lu, kc, opt_.keep_matrices != 0));
         nk += 2;
         HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
-                                     (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu));
+                                     (const double*)bk.tau, Z, (const SccState*)d_state_, kc, bk.lu, bk.bt_cols));
         // C_k = Cb Z (into T); Z is stored transposed, element (m, j) at m * kc + j
         ++nk;
         HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
@@ -1257,7 +1257,7 @@ Status Batch::enqueue_plain_iteration(Bucket& bk, int slot) {
                                (const double*)bk.eig_e, d_eps_, (const int64_t*)d_vec0_, d_occ_, d_state_, table(A_C),
                                bk.lu, kc, opt_.keep_matrices != 0));
     HIP_TRY(launch_backtransform(bk.stream, cnt, act, (const int*)d_nao_, n, table(A_A), (const double*)bk.tau,
-                                 table(A_C), (const SccState*)d_state_, kc, bk.lu));
+                                 table(A_C), (const SccState*)d_state_, kc, bk.lu, bk.bt_cols));
     HIP_TRY(launch_orbitals(bk.stream, cnt, act, (const int*)d_nao_, (const SccState*)d_state_, n, kc, table(A_CB),
                             table(A_C), kc, 1, table(A_T)));
     HIP_TRY(launch_density(bk.stream, cnt, act, (const int*)d_nao_, (const SccState*)d_state_, n, kc, table(A_T),
@@ -1284,7 +1284,8 @@ Status Batch::enqueue_plain_iteration(Bucket& bk, int slot) {
 // slots.  Kept across runs; captured again when the eigenvector capacity or
 // its scratch changed.  The bucket's stream is idle.
 Status Batch::capture_iteration(Bucket& bk) {
-    if (bk.gexec[0] && bk.gexec[1] && bk.g_kcap == bk.kcap && bk.g_lu == bk.lu) return Status{};
+    if (bk.gexec[0] && bk.gexec[1] && bk.g_kcap == bk.kcap && bk.g_lu == bk.lu && bk.g_bt_cols == bk.bt_cols)
+        return Status{};
     const auto t0 = std::chrono::steady_clock::now();
     for (int slot = 0; slot < 2; ++slot)
         if (bk.gexec[slot]) {
@@ -1338,6 +1339,7 @@ Status Batch::capture_iteration(Bucket& bk) {
     }
     bk.g_kcap = bk.kcap;
     bk.g_lu = bk.lu;
+    bk.g_bt_cols = bk.bt_cols;
     ++launch_stats_.graphs_captured;
     launch_stats_.capture_ms +=
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1389,6 +1391,64 @@ Status Batch::finish_replay(Bucket& bk, bool& switch_to_host) {
     }
     if (plain == 0) bk.g_leave = true;
     if (switch_to_host) bk.g_leave = true;
+    return Status{};
+}
+
+// ---- Autotuning of the back-transformation width
+//
+// After a run, every own-eigensolver bucket times launch_backtransform at each
+// admissible width (16, 32, and 64 eigenvectors per work-group up to 256 rows)
+// on the reflectors and T factors its instances left, one untimed launch and
+// three timed ones per width (HIP events on the bucket's stream), and keeps the
+// fastest.  The eigenvector buffers it overwrites are dead between runs (every
+// iteration recomputes them), and all widths give the same result bits, so the
+// choice never changes a result.
+Status Batch::autotune() {
+    if (!ran_) return Status::error(kErrInvalidInput, "autotune times the kernels on a finished run's data");
+    HIP_TRY(hipDeviceSynchronize());
+    tuning_.clear();
+    static const int widths[3] = {16, 32, 64};
+    for (auto& bk : buckets_) {
+        if (opt_.eigensolver == 1 || bk.ld > kEigMaxDim) continue;
+        std::vector<int> list;
+        for (int i : bk.sel)
+            if (h_state_[i].total_iter > 0) list.push_back(i);
+        if (list.empty()) continue;
+        Status st = upload_active(bk, list, 0);
+        if (!st.ok()) return st;
+        const int cnt = (int)list.size(), n = bk.ld;
+        double* const* A = bk.ptrs + (size_t)A_A * cnt;
+        double* const* Z = bk.ptrs + (size_t)A_C * cnt;
+        TuneRecord rec;
+        rec.ld = n;
+        rec.count = cnt;
+        rec.cols = bk.bt_cols;
+        double best = 0.0;
+        for (int w = 0; w < 3; ++w) {
+            if (!backtransform_width_ok(n, widths[w])) continue;
+            HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
+                                         (const double*)bk.tau, Z, (const SccState*)d_state_, bk.kcap, bk.lu,
+                                         widths[w]));
+            HIP_TRY(hipEventRecord(bk.ev0, bk.stream));
+            for (int r = 0; r < 3; ++r)
+                HIP_TRY(launch_backtransform(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, n, A,
+                                             (const double*)bk.tau, Z, (const SccState*)d_state_, bk.kcap, bk.lu,
+                                             widths[w]));
+            HIP_TRY(hipEventRecord(bk.ev1, bk.stream));
+            HIP_TRY(hipEventSynchronize(bk.ev1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, bk.ev0, bk.ev1));
+            rec.ms[w] = ms / 3.0;
+            if (best == 0.0 || rec.ms[w] < best) {
+                best = rec.ms[w];
+                rec.cols = widths[w];
+            }
+        }
+        bk.bt_cols = rec.cols;
+        tuning_.push_back(rec);
+    }
+    // the events of the host loop's timing must not pair with these
+    for (auto& bk : buckets_) bk.ev_pending = false;
     return Status{};
 }
 

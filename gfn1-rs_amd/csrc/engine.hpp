@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "device.hpp"
+#include "eigen.hpp"
 #include "guess.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
@@ -163,6 +164,24 @@ class Batch {
 
     // How the last run's SCC phase drove the device.
     const LaunchStats& launch_stats() const { return launch_stats_; }
+    // Times every back-transformation width on each own-eigensolver bucket's
+    // reflectors from the last run and keeps the fastest for the later runs.
+    // The widths give identical results, so tuning changes speed only.
+    Status autotune();
+    struct TuneRecord {
+        int ld = 0, count = 0, cols = 0;
+        double ms[3] = {-1.0, -1.0, -1.0};  // widths 16, 32, 64; -1 where not a candidate
+    };
+    const std::vector<TuneRecord>& tuning() const { return tuning_; }
+    // Sets the back-transformation width of every bucket where it is admissible
+    // (16, 32; 64 up to 256 rows), in place of the tuned one.
+    Status set_backtransform_width(int cols) {
+        if (cols != 16 && cols != 32 && cols != 64)
+            return Status::error(kErrInvalidInput, "back-transformation width must be 16, 32 or 64");
+        for (auto& bk : buckets_)
+            if (bk.ld <= kEigMaxDim && backtransform_width_ok(bk.ld, cols)) bk.bt_cols = cols;
+        return Status{};
+    }
 
    private:
     struct Bucket {
@@ -215,6 +234,8 @@ class Batch {
         // gathered into.  The pointer tables list every instance of the bucket
         // at its fixed position; k_gate_active, the graph's first node, marks
         // the positions whose iteration is not a plain one with -1.
+        int bt_cols = kBacktransformDefaultCols;  // back-transformation width (autotune)
+        int g_bt_cols = 0;                // the width the graphs were captured with
         hipGraphExec_t gexec[2] = {nullptr, nullptr};
         int g_nodes = 0;                  // kernels per replay
         int g_kcap = 0;                   // eigenvector capacity the graphs were captured with
@@ -272,6 +293,7 @@ class Batch {
     double tri_ms_ = 0.0, tri_bytes_ = 0.0;
     long long tri_launches_ = 0;
     LaunchStats launch_stats_{};
+    std::vector<TuneRecord> tuning_;
 
     // device arrays
     DeviceBatch db_{};

@@ -126,6 +126,10 @@ SIGNATURES = [
     ("gfn1_batch_kernel_stats", ctypes.c_int,
      [_VP, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_int64), _DP]),
     ("gfn1_batch_launch_stats", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int64)]),
+    ("gfn1_batch_autotune", ctypes.c_int, [_VP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_batch_set_backtransform_width", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_batch_tuning_count", ctypes.c_int32, [_VP]),
+    ("gfn1_batch_tuning_entry", ctypes.c_int, [_VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _DP]),
     ("gfn1_batch_free", None, [_VP]),
     ("gfn1_params_shell_count", ctypes.c_int, [_VP, ctypes.c_int]),
     ("gfn1_guess_model_create", ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_VP), ctypes.POINTER(Gfn1Error)]),

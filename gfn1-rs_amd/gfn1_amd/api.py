@@ -920,6 +920,30 @@ class BatchEngine:
         _lib.check(self._lib.gfn1_batch_run(self._h, ctypes.byref(err)), err)
         return self
 
+    def set_backtransform_width(self, cols):
+        """Fix the back-transformation width (16, 32, or 64 for buckets of at
+        most 256 rows) in place of the tuned one."""
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_set_backtransform_width(self._h, int(cols), ctypes.byref(err)), err)
+        return self
+
+    def autotune(self):
+        """Time every back-transformation width on each size bucket's data from
+        the last run and keep the fastest for later runs (results unchanged).
+        Returns one dict per tuned bucket: ld, instances, cols (chosen) and
+        ms (width -> mean launch milliseconds)."""
+        err = _lib.Gfn1Error()
+        _lib.check(self._lib.gfn1_batch_autotune(self._h, ctypes.byref(err)), err)
+        out = []
+        for k in range(self._lib.gfn1_batch_tuning_count(self._h)):
+            iv = (ctypes.c_int32 * 3)()
+            ms = (ctypes.c_double * 3)()
+            if self._lib.gfn1_batch_tuning_entry(self._h, k, iv, ms) != 0:
+                raise RuntimeError("tuning entry %d" % k)
+            out.append({"ld": iv[0], "instances": iv[1], "cols": iv[2],
+                        "ms": {w: m for w, m in zip((16, 32, 64), ms) if m >= 0.0}})
+        return out
+
     def status(self, i):
         return self._lib.gfn1_batch_status(self._h, i)
 

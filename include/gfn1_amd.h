@@ -266,6 +266,19 @@ enum {
     GFN1_LAUNCH_STATS
 };
 int gfn1_batch_launch_stats(const gfn1_batch* batch, int64_t out[GFN1_LAUNCH_STATS]);
+/* Autotuning of the eigenvector back-transformation: after a gfn1_batch_run,
+ * times each admissible width (16, 32, 64 eigenvectors per work-group) on every
+ * size bucket's reflectors from that run and keeps the fastest for the later
+ * runs.  All widths give bit-identical results.  gfn1_batch_tuning_entry
+ * returns, for tuned bucket k < gfn1_batch_tuning_count: out_i = {padded
+ * dimension, instances timed, chosen width}, out_ms = mean launch time per
+ * width 16, 32, 64 (-1 where not admissible).  No reference counterpart. */
+int gfn1_batch_autotune(gfn1_batch* batch, gfn1_error* err);
+/* Fixes the back-transformation width (16, 32 or 64; 64 only for buckets of
+ * at most 256 rows, the others keep theirs) in place of the tuned one. */
+int gfn1_batch_set_backtransform_width(gfn1_batch* batch, int32_t cols, gfn1_error* err);
+int32_t gfn1_batch_tuning_count(const gfn1_batch* batch);
+int gfn1_batch_tuning_entry(const gfn1_batch* batch, int32_t k, int32_t out_i[3], double out_ms[3]);
 void gfn1_batch_free(gfn1_batch* batch);
 
 /* --- Learned SCC start charges ---

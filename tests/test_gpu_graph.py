@@ -103,3 +103,26 @@ def test_mixed_precision_with_the_learned_start():
     _, rg, sg, rh, _ = run_both(systems, precision="mixed", guess=model)
     assert_identical(rg, rh)
     assert sg["graph_replays"] > 0
+
+
+def test_backtransform_widths_agree_bit_for_bit_and_autotune_keeps_results():
+    """Every back-transformation width gives the same bits (each eigenvector's
+    arithmetic does not depend on how many share a work-group), so the
+    autotuner's choice, made from timings, cannot change a result."""
+    systems = geometry.mixed_batch(48, first_seed=7400) + [geometry.water_cluster(48, 11.3, seed=0)]
+    note("case back-transformation widths, %d clusters" % len(systems))
+    eng = BatchEngine(systems)
+    ref = None
+    for cols in (32, 16, 64):
+        eng.set_backtransform_width(cols).run()
+        r = results(eng, systems)
+        if ref is None:
+            ref = r
+        else:
+            assert_identical(ref, r)
+    table = eng.autotune()
+    note("tuning: %s" % table)
+    assert table and all(t["cols"] in t["ms"] for t in table)
+    assert all(set(t["ms"]) == ({16, 32, 64} if t["ld"] <= 256 else {16, 32}) for t in table)
+    eng.run()
+    assert_identical(ref, results(eng, systems))
