@@ -157,7 +157,7 @@ def host_description():
                     break
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count()}
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_available": len(os.sched_getaffinity(0))}
 
 
 def parity(eng, systems, count):
@@ -481,8 +481,8 @@ def main():
     cpu = None
     if args.cpu_sample > 0:
         import oracle
-        # a one-GPU machine gives a job 16 host cores; fewer when the host has fewer
-        nthreads = min(16, os.cpu_count() or 1)
+        # every host core this process may run on (16 on a one-GPU machine)
+        nthreads = len(os.sched_getaffinity(0)) or 1
         sample = systems[: args.cpu_sample]
         t = time.perf_counter()
         oracle.run_batch(sample, nthreads=nthreads, energy_tolerance=1e-7)
