@@ -1117,7 +1117,7 @@ Status Batch::launch_iteration(Bucket& bk) {
     }
     const int kc = bk.kcap;
     // matrices above kEigMaxDim go to rocSOLVER
-    const bool own_solver = opt_.eigensolver != 1 && n <= kEigMaxDim;
+    const bool own_solver = own_eigensolver(bk);
     if (own_solver) {
         // tridiagonal reduction, eigenvalues, occupations, occupied eigenvectors
         // (returned transposed in Z, kc columns)
@@ -1224,7 +1224,7 @@ Status Batch::launch_iteration(Bucket& bk) {
 // eigenvector capacity, a long Broyden history), the bucket finishes the run
 // in the host loop.
 bool Batch::graph_bucket(const Bucket& bk) const {
-    return opt_.scc_graphs && !serial_ && opt_.eigensolver != 1 && bk.ld <= kEigMaxDim && !bk.inst.empty();
+    return opt_.scc_graphs && !serial_ && own_eigensolver(bk) && !bk.inst.empty();
 }
 
 // The kernels of one plain iteration over every position of the bucket, in
@@ -1409,7 +1409,7 @@ Status Batch::autotune() {
     tuning_.clear();
     static const int widths[3] = {16, 32, 64};
     for (auto& bk : buckets_) {
-        if (opt_.eigensolver == 1 || bk.ld > kEigMaxDim) continue;
+        if (!own_eigensolver(bk)) continue;
         std::vector<int> list;
         for (int i : bk.sel)
             if (h_state_[i].total_iter > 0) list.push_back(i);
@@ -1463,7 +1463,7 @@ Status Batch::finish_iteration(Bucket& bk) {
         ++tri_launches_;
         bk.ev_pending = false;
     }
-    if ((opt_.eigensolver == 1 || bk.ld > kEigMaxDim) && !bk.active.empty()) {
+    if (!own_eigensolver(bk) && !bk.active.empty()) {
         const int* info = (const int*)bk.h_aux;
         for (size_t j = 0; j < bk.active.size(); ++j)
             if (info[j] != 0) return Status::error(kErrLinearAlgebra, "eigensolver did not converge");

@@ -74,7 +74,7 @@ struct EngineOptions {
     int include_repulsion = 1, include_dispersion = 1, include_hamiltonian = 1, include_scc = 1,
         include_halogen = 1;
     // execution
-    int eigensolver = 0;   // 0 auto, 1 syevd, 2 syevj
+    int eigensolver = 0;   // 0 auto (own up to 512 AO, syevd above), 1 syevd, 2 syevj, 3 syevdj
     int keep_matrices = 0;
     // iterate buckets of the own eigensolver by replays of a captured iteration
     // (0: one host-launched kernel sequence per iteration)
@@ -260,6 +260,9 @@ class Batch {
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
     Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
+    // The engine's own eigensolver (automatic choice, up to kEigMaxDim AO);
+    // otherwise the rocSOLVER solver the options name.
+    bool own_eigensolver(const Bucket& bk) const { return opt_.eigensolver == 0 && bk.ld <= kEigMaxDim; }
     bool graph_bucket(const Bucket& bk) const;
     Status capture_iteration(Bucket& bk);
     Status enqueue_plain_iteration(Bucket& bk, int slot);

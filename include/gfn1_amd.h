@@ -208,7 +208,8 @@ typedef struct gfn1_batch_options {
     gfn1_gradient_options gradient;
     int32_t precision;       /* gfn1_precision */
     int32_t guess;           /* gfn1_guess */
-    int32_t eigensolver;     /* 0 auto, 1 rocSOLVER syevd, 2 rocSOLVER syevj */
+    int32_t eigensolver;     /* 0 auto (own kernels up to 512 AO, rocSOLVER syevd above),
+                                1 rocSOLVER syevd, 2 rocSOLVER syevj, 3 rocSOLVER syevdj */
     int32_t keep_matrices;   /* copy S/H0/F/P/W back to the host per instance */
     int32_t scc_graphs;      /* 1 (default): buckets of the own eigensolver iterate by
                                 replays of a captured HIP graph of one SCC iteration;

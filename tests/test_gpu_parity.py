@@ -272,3 +272,19 @@ def test_full_batch_reruns_agree_and_largest_match_oracle():
         note("instance %d (%d AO): dE %.2e  dg %.2e" % (i, nao[i], de, dg))
         assert de <= E_TOL
         assert dg <= G_TOL
+
+
+@pytest.mark.parametrize("solver", [1, 2, 3])
+def test_rocsolver_eigensolvers_agree_with_the_own_solver(solver):
+    """eigensolver 1/2/3 (rocSOLVER syevd, syevj, syevdj) in place of the own
+    kernels for a small mixed batch: same energies and gradients within the
+    FP64 bar, same iteration counts."""
+    systems = geometry.mixed_batch(6, first_seed=4300) + [mol("water")]
+    note("case rocSOLVER eigensolver %d, %d systems" % (solver, len(systems)))
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))
+    own = BatchEngine(systems, options=opts).run()
+    other = BatchEngine(systems, options=opts, eigensolver=solver).run()
+    for i in range(len(systems)):
+        assert other.status(i) == 0
+        assert abs(other.energy(i) - own.energy(i)) <= E_TOL
+        assert np.abs(other.gradient(i) - own.gradient(i)).max() <= G_TOL
