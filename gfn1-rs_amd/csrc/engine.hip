@@ -16,6 +16,7 @@
 #include "cholesky.hpp"
 #include "density.hpp"
 #include "eigen.hpp"
+#include "tridiag_large.hpp"
 
 #include <rocsolver/rocsolver.h>
 
@@ -142,6 +143,7 @@ Batch::~Batch() {
         (void)hipFree(bk.g_inst);
         (void)hipFree(bk.g_act);
     }
+    large_work_.release();
     for (void* p : allocs_) (void)hipFree(p);
     if (ev_cn_) (void)hipEventDestroy(ev_cn_);
     if (ev_start_) (void)hipEventDestroy(ev_start_);
@@ -482,6 +484,31 @@ Status Batch::upload_active(Bucket& bk, const std::vector<int>& list, int parity
     hipLaunchKernelGGL(k_copy_words, dim3(((int)m + 255) / 256), dim3(256), 0, bk.stream,
                        (const unsigned int*)bk.h_act, (unsigned int*)bk.act, (int)m);
     LAUNCH_CHECK();
+    return Status{};
+}
+
+// Eigenpairs of the active instances of a bucket above kEigMaxDim: the engine's
+// own Householder reduction (tridiag_large.hip), rocsolver_dstedc for the
+// tridiagonal eigenproblem (eigenvalues ascending into bk.eig, eigenvectors of
+// T into the C slot), and rocsolver_dormtr to take the first kcap of them back
+// to the basis of A.  Same contract as eigensolve(): eigenvalues at bk.eig +
+// l n, convergence flags at bk.info + l; the eigenvectors are in the A_C slot.
+Status Batch::large_eigensolve(Bucket& bk) {
+    const int n = bk.ld, cnt = (int)bk.active.size();
+    const size_t mat = (size_t)n * n, count = bk.inst.size();
+    HIP_TRY(large_work_.allocate(n));
+    for (int l = 0; l < cnt; ++l) {
+        const size_t k = (size_t)h_bucket_pos_[bk.active[l]];
+        double* A = bk.arena + ((size_t)R_A * count + k) * mat;
+        double* C = bk.arena + ((size_t)R_CB1 * count + k) * mat;  // the A_C slot at parity 0
+        double* d = bk.eig + (size_t)l * n;
+        double* e = bk.eig_e + (size_t)l * n;
+        double* tau = bk.tau + (size_t)l * n;
+        HIP_TRY(launch_tridiag_large(bk.stream, n, n, A, d, e, tau, large_work_));
+        RB_TRY(rocsolver_dstedc(bk.handle, rocblas_evect_tridiagonal, n, d, e, C, n, bk.info + l));
+        RB_TRY(rocsolver_dormtr(bk.handle, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, n, bk.kcap,
+                                A, n, tau, C, n));
+    }
     return Status{};
 }
 
@@ -1146,7 +1173,8 @@ Status Batch::launch_iteration(Bucket& bk) {
     } else {
         // rocSOLVER runs alone on the device, as in the setup phase
         HIP_TRY(hipDeviceSynchronize());
-        Status st2 = eigensolve(bk, cnt);
+        const bool own_red = own_reduction(bk);
+        Status st2 = own_red ? large_eigensolve(bk) : eigensolve(bk, cnt);
         if (!st2.ok()) return st2;
         ++nk;  // the rocSOLVER eigensolve counts once
         // rocSOLVER's convergence flags, read by the host once the iteration is done
@@ -1159,10 +1187,11 @@ Status Batch::launch_iteration(Bucket& bk) {
         hipLaunchKernelGGL(k_fermi, dim3(cnt), dim3(256), 0, bk.stream, db_, (const int*)bk.act,
                            (const double*)bk.eig, n, d_eps_, d_occ_, d_state_);
         LAUNCH_CHECK();
-        // rocSOLVER returns the eigenvectors in place (column-major, element (m, j) at j * n + m)
+        // the eigenvectors are column-major (element (m, j) at j * n + m): in place
+        // in A from rocSOLVER's solvers, in the C slot from large_eigensolve
         ++nk;
         HIP_TRY(launch_orbitals(bk.stream, cnt, (const int*)bk.act, (const int*)d_nao_, (const SccState*)d_state_, n,
-                                kc, Cb, A, 1, n, T));
+                                kc, Cb, own_red ? Z : A, 1, n, T));
     }
     // ---- P = C_k diag(f) C_k^T with the Mulliken row sums of S o P and H0 o P
     // (FP64 in every precision mode: the SCC energy and charges need it)
@@ -1172,10 +1201,19 @@ Status Batch::launch_iteration(Bucket& bk) {
                            T, (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_, false, P,
                            bk.ptrs + (size_t)A_S * cnt, bk.ptrs + (size_t)A_H0 * cnt, (const int*)d_ao0_, d_rowS_,
                            d_rowH_, naos_));
-    // ---- Mulliken charges, energy, convergence and mixing
+    // ---- Mulliken charges, energy, convergence and mixing (above 512 AO the
+    // row-sum blocks are first added up AO-parallel)
+    int rs_blocks = 0;
+    if (n > kEigMaxDim) {
+        ++nk;
+        hipLaunchKernelGGL(k_collapse_rowsums, dim3((n + 255) / 256, cnt), dim3(256), 0, bk.stream, db_,
+                           (const int*)bk.act, d_rowS_, d_rowH_, naos_);
+        LAUNCH_CHECK();
+        rs_blocks = 1;
+    }
     ++nk;
     hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, (const int*)bk.act,
-                       (const double*)d_rowS_, (const double*)d_rowH_, naos_, d_qout_, d_state_);
+                       (const double*)d_rowS_, (const double*)d_rowH_, naos_, rs_blocks, d_qout_, d_state_);
     LAUNCH_CHECK();
     ++nk;
     hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t,
@@ -1264,7 +1302,7 @@ Status Batch::enqueue_plain_iteration(Bucket& bk, int slot) {
                            (const double*)d_eps_, (const double*)d_occ_, (const int64_t*)d_vec0_, false, table(A_P),
                            table(A_S), table(A_H0), (const int*)d_ao0_, d_rowS_, d_rowH_, naos_));
     hipLaunchKernelGGL(k_qout, dim3(cnt), dim3(128), 0, bk.stream, db_, t, act, (const double*)d_rowS_,
-                       (const double*)d_rowH_, naos_, d_qout_, d_state_);
+                       (const double*)d_rowH_, naos_, 0, d_qout_, d_state_);
     LAUNCH_CHECK();
     hipLaunchKernelGGL(k_shell_potential, dim3((max_nsh + 127) / 128, cnt), dim3(128), 0, bk.stream, db_, t, act,
                        (const double*)d_qout_, d_vout_g_, d_vout_t_);

@@ -13,6 +13,7 @@
 #include "device.hpp"
 #include "eigen.hpp"
 #include "guess.hpp"
+#include "tridiag_large.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
 
@@ -263,6 +264,13 @@ class Batch {
     // The engine's own eigensolver (automatic choice, up to kEigMaxDim AO);
     // otherwise the rocSOLVER solver the options name.
     bool own_eigensolver(const Bucket& bk) const { return opt_.eigensolver == 0 && bk.ld <= kEigMaxDim; }
+    // Above kEigMaxDim with the automatic choice: the engine's own tridiagonal
+    // reduction, rocSOLVER for the tridiagonal eigenproblem and the
+    // back-transformation of the kept eigenvectors.
+    bool own_reduction(const Bucket& bk) const {
+        return opt_.eigensolver == 0 && bk.ld > kEigMaxDim && bk.ld <= kLargeTridiagMaxDim;
+    }
+    Status large_eigensolve(Bucket& bk);
     bool graph_bucket(const Bucket& bk) const;
     Status capture_iteration(Bucket& bk);
     Status enqueue_plain_iteration(Bucket& bk, int slot);
@@ -296,6 +304,7 @@ class Batch {
     double tri_ms_ = 0.0, tri_bytes_ = 0.0;
     long long tri_launches_ = 0;
     LaunchStats launch_stats_{};
+    LargeTridiagWork large_work_;  // scratch of the reduction above kEigMaxDim
     std::vector<TuneRecord> tuning_;
 
     // device arrays

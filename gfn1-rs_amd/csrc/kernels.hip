@@ -613,17 +613,37 @@ __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* 
     }
 }
 
+// Large matrices: the per-64-column-block row sums of k_density added up in
+// block order (eight interleaved partial sums, fixed combination) into block 0,
+// one thread per AO, so that k_qout (blocks = 1) does not walk every block.
+__global__ void k_collapse_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH, int stride) {
+    const int inst = act[blockIdx.y];
+    if (inst < 0) return;
+    const int a0 = b.ao0[inst], n = b.nao[inst];
+    const int mu = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mu >= n) return;
+    const int nblk = (n + 63) / 64;
+    for (double* part : {rowS, rowH}) {
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int blk = 0;
+        for (; blk + 8 <= nblk; blk += 8)
+            for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(blk + u) * stride + a0 + mu];
+        for (int u = 0; blk < nblk; ++blk, ++u) acc[u] += part[(size_t)blk * stride + a0 + mu];
+        part[a0 + mu] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    }
+}
+
 // Output shell charges q = n0 - sum_{mu in shell} (S P)_mu mu and the band energy
 // (electronic.rs:3160-3188), from the per-64-column-block partial sums
 // rowS[blk * stride + ao0 + mu] (summed in block order).
 __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
-                       int stride, double* qout, SccState* st) {
+                       int stride, int blocks, double* qout, SccState* st) {
     __shared__ double lds[16];
     const int inst = act[blockIdx.x];
     if (inst < 0) return;
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int a0 = b.ao0[inst], n = b.nao[inst];
-    const int nblk = (n + 63) / 64;
+    const int nblk = blocks > 0 ? blocks : (n + 63) / 64;
     auto total = [&](const double* part, int mu) {
         double v = 0.0;
         for (int blk = 0; blk < nblk; ++blk) v += part[(size_t)blk * stride + a0 + mu];

@@ -44,7 +44,8 @@ __global__ void k_operands_fp32(DeviceBatch b, const int* act, double* const* Cp
                                 int weighted);
 __global__ void k_symmetrize_fp32(DeviceBatch b, const int* act, double* const* Xp, double* const* Mp);
 __global__ void k_qout(DeviceBatch b, DeviceTables t, const int* act, const double* rowS, const double* rowH,
-                       int stride, double* qout, SccState* st);
+                       int stride, int blocks, double* qout, SccState* st);
+__global__ void k_collapse_rowsums(DeviceBatch b, const int* act, double* rowS, double* rowH, int stride);
 __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccParams prm, const double* qout,
                              const double* vg_out, double* qin, const double* q0, SccState* st, double* br_df,
                              double* br_u, double* br_omega, double* br_qlast, double* br_dqlast,

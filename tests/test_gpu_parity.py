@@ -137,9 +137,10 @@ def test_water48_matches_oracle():
 
 
 def test_water70_matches_oracle():
-    """560 AO: above the engine's own Cholesky and eigensolver limit (512), so
-    the setup and every SCC eigensolve go through rocSOLVER."""
-    note("case water70 (rocSOLVER branch)")
+    """560 AO: above the batched kernels' limit (512), so the setup goes
+    through rocSOLVER and every SCC eigensolve through the large-matrix branch
+    (own Householder reduction, rocSOLVER tridiagonal solver)."""
+    note("case water70 (large-matrix branch)")
     z, x = geometry.water_cluster(70, 12.9, 3)
     ref = oracle.run(z, x, **TIGHT)
     res = analytic_gradient(z, x, options=AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT)))
@@ -288,3 +289,24 @@ def test_rocsolver_eigensolvers_agree_with_the_own_solver(solver):
         assert other.status(i) == 0
         assert abs(other.energy(i) - own.energy(i)) <= E_TOL
         assert np.abs(other.gradient(i) - own.gradient(i)).max() <= G_TOL
+
+
+@pytest.mark.parametrize("nmol,side,seed", [(70, 13.0, 2), (150, 16.6, 3)])
+def test_own_reduction_above_512_ao_matches_rocsolver_syevd(nmol, side, seed):
+    """Above 512 AO the automatic eigensolver is the engine's own Householder
+    reduction with rocSOLVER's tridiagonal solver and back-transformation;
+    rocSOLVER dsyevd (eigensolver 1) solves the same problem by another route.
+    560 and 1200 AO: same iterations, energies within 1e-8 Eh, gradients
+    within 1e-7 Eh/bohr."""
+    z, x = geometry.water_cluster(nmol, side, seed)
+    nao = geometry.basis_size(z)[1]
+    note("case own reduction vs syevd, %d waters, %d AO" % (nmol, nao))
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**TIGHT))
+    own = BatchEngine([(z, x)], options=opts).run()
+    ref = BatchEngine([(z, x)], options=opts, eigensolver=1).run()
+    assert own.status(0) == 0 and ref.status(0) == 0
+    de = abs(own.energy(0) - ref.energy(0))
+    dg = float(np.abs(own.gradient(0) - ref.gradient(0)).max())
+    note("dE %.2e  dg %.2e  iterations %d / %d" % (de, dg, own.iterations(0), ref.iterations(0)))
+    assert own.iterations(0) == ref.iterations(0)
+    assert de <= E_TOL and dg <= G_TOL
