@@ -329,27 +329,63 @@ __global__ void k_make_cb(DeviceBatch b, double* const* Linv, const int* act, do
 // ---------------------------------------------------------------------------
 // Shell potential v = gamma q (+ on-site third order) (coulomb.rs:82-107, :140-193).
 // vg gets gamma q only; vt the full potential.
+// Instances up to this many shells stage their shells' positions, inverse
+// hardnesses and charges in LDS for k_shell_potential (24 KB).
+constexpr int kShellLds = 512;
+
 __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act, const double* q, double* vg,
                                   double* vt) {
+    __shared__ double sx[kShellLds], sy[kShellLds], sz[kShellLds], sih[kShellLds], sq[kShellLds];
+    __shared__ int sa[kShellLds];
     const int inst = act[blockIdx.y];
     if (inst < 0) return;  // gated off in a captured iteration
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
+    const double* x = b.atom_x;
+    const bool staged = nsh <= kShellLds;  // uniform over the work-group
+    if (staged) {
+        for (int j = threadIdx.x; j < nsh; j += blockDim.x) {
+            const int sj = s0 + j, aj = b.sh_atom[sj];
+            sa[j] = aj;
+            sx[j] = x[3 * aj];
+            sy[j] = x[3 * aj + 1];
+            sz[j] = x[3 * aj + 2];
+            sih[j] = 1.0 / t.shells[b.sh_type[sj]].hardness;
+            sq[j] = q[sj];
+        }
+        __syncthreads();
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nsh) return;
     const int si = s0 + i;
     const int ai = b.sh_atom[si];
     const double hi = t.shells[b.sh_type[si]].hardness;
-    const double* x = b.atom_x;
     double v = 0.0;
-    for (int j = 0; j < nsh; ++j) {
-        const int sj = s0 + j;
-        const int aj = b.sh_atom[sj];
-        const double hj = t.shells[b.sh_type[sj]].hardness;
-        const double h = 2.0 / (1.0 / hi + 1.0 / hj);
-        double gam;
-        if (ai == aj) gam = h;
-        else gam = 1.0 / sqrt(dist2(x, ai, aj) + 1.0 / (h * h));
-        v += gam * q[sj];
+    if (staged) {
+        // the same operations as the global-memory loop below, operands from LDS
+        const double ihi = 1.0 / hi;
+        const double xi = x[3 * ai], yi = x[3 * ai + 1], zi = x[3 * ai + 2];
+        for (int j = 0; j < nsh; ++j) {
+            const double h = 2.0 / (ihi + sih[j]);
+            double gam;
+            if (ai == sa[j]) {
+                gam = h;
+            } else {
+                const double dx = sx[j] - xi, dy = sy[j] - yi, dz = sz[j] - zi;
+                gam = 1.0 / sqrt(dx * dx + dy * dy + dz * dz + 1.0 / (h * h));
+            }
+            v += gam * sq[j];
+        }
+    } else {
+        for (int j = 0; j < nsh; ++j) {
+            const int sj = s0 + j;
+            const int aj = b.sh_atom[sj];
+            const double hj = t.shells[b.sh_type[sj]].hardness;
+            const double h = 2.0 / (1.0 / hi + 1.0 / hj);
+            double gam;
+            if (ai == aj) gam = h;
+            else gam = 1.0 / sqrt(dist2(x, ai, aj) + 1.0 / (h * h));
+            v += gam * q[sj];
+        }
     }
     vg[si] = v;
     // third order: Gamma_A q_A^2 on every shell of atom A
