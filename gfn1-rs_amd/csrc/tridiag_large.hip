@@ -2,24 +2,28 @@
 // device: LAPACK dsytrd with uplo = 'L' (dlatrd panels of kLargeTridiagPanel
 // reflectors, the trailing matrix updated once per panel on the matrix cores).
 //
-// Per column i of a panel (position j), three kernels:
-//   k_lt_column  one work-group: finishes W(:, j-1) = p + a2 v, forms column i
-//                from the partial update k_lt_reduce left, and generates the
-//                reflector (dlarfg) into A, V(:, j), d, e, tau;
-//   k_lt_symv    one work-group per 64 x 64 tile of the trailing lower triangle:
-//                y = A v, the row part and the column part of each tile written
-//                to their own slot of `part` (every slot written exactly once, so
-//                the sum is deterministic), plus the dot products V^T v, W^T v;
-//   k_lt_reduce  one work-group per 256 rows: p = tau (y - V W^T v - W V^T v)
-//                into W(:, j), partial sums of p.v, and the next column after all
-//                of the panel's corrections but the one that needs p.v.
-// The symmetric product reads the trailing lower triangle once per column
-// (8 (n-i)^2 / 2 bytes), across every CU.
+// Column i of a panel (position j) takes two kernels.  With a_r the column
+// after the panel's corrections, u_r = a_r for r >= i+2 (u_{i+1} = 0) and the
+// reflector v = scale u + e_{i+1}, the symmetric product is linear in u:
+// A v = scale (A u) + A(:, i+1), so it runs before the reflector is known.
+//   k_lt_symv    y_u = A u over the trailing lower triangle (strips of 64 x 64
+//                tiles, every partial in its own slot of `part`), the dot
+//                products V^T u, W^T u and the norm |u|^2 per 256-row chunk;
+//   k_lt_reduce  the reflector (dlarfg) from alpha = a_{i+1} and |u|^2, then per
+//                row y = scale y_u + A(r, i+1), p = tau (y - V W^T v - W V^T v)
+//                into W(:, j), v into A and V(:, j), partial sums of p.v, and
+//                the next column after every panel correction but the one that
+//                needs p.v.
+// W(:, j) holds p until the next column's kernels add a2 v (a2 = -tau (p.v)/2)
+// on the fly and write the finished values back (row i+1 excepted: only the
+// trailing update reads it, and adds the term itself).  The symmetric product
+// reads the trailing lower triangle once per column, across every CU.
 #include "tridiag_large.hpp"
-#include "wave_reduce.hpp"
 
 #include <algorithm>
 #include <cmath>
+
+#include "wave_reduce.hpp"
 
 namespace gfn1 {
 namespace {
@@ -27,10 +31,12 @@ namespace {
 constexpr int kNB = kLargeTridiagPanel;
 static_assert(kNB == 32, "the dot products run as two groups of 16 panel columns");
 constexpr int kTile = 64;
-constexpr int kChunk = 256;  // rows per work-group of k_lt_reduce and of the dot products
-constexpr int kColThreads = 1024;
-constexpr int kRowsPerThread = kLargeTridiagMaxDim / kColThreads;
+constexpr int kChunk = 256;        // rows per work-group of k_lt_reduce and of the dot products
+constexpr int kDot = 2 * kNB + 8;  // per chunk: V^T u (kNB), W^T u (kNB), |u|^2, padding
 typedef double tvec4 __attribute__((ext_vector_type(4)));
+
+// scalars of the current column, written by k_lt_symv for k_lt_reduce
+enum { kScD = 0, kScAlpha, kScA2, kScS, kScalars };
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -51,75 +57,22 @@ __device__ double group_sum(double v, double* red) {
     return t;
 }
 
-__global__ void __launch_bounds__(kColThreads)
-    k_lt_column(int n, int ld, int i, int j, int fresh, int finalize_only, double* __restrict__ A,
-                double* __restrict__ V, double* __restrict__ W, int ldv, const double* __restrict__ next,
-                const double* __restrict__ pv, int npv, double* __restrict__ d, double* __restrict__ e,
-                double* __restrict__ tau) {
-    __shared__ double red[kColThreads / 64];
-    __shared__ double sh[2];
-    const int tid = threadIdx.x;
-    double s = 0.0;
-    if (!fresh) {
-        // W(:, j-1) = p + a2 v with a2 = -tau_{i-1} (p . v) / 2 (rows >= i)
-        // npv <= 64: every wavefront forms the same sum (same lanes, same order)
-        const int l = tid & 63;
-        const double pvs = wave_sum(l < npv ? pv[l] : 0.0);
-        const double a2 = -0.5 * tau[i - 1] * pvs;
-        const double* vp = V + (size_t)(j - 1) * ldv;
-        double* wp = W + (size_t)(j - 1) * ldv;
-        // the part of column i's update that needs a2: v_r (p_i + 2 a2 v_i)
-        s = wp[i] + 2.0 * a2 * vp[i];
-        __syncthreads();  // every thread has read wp[i] before it is rewritten
-        for (int r = i + tid; r < n; r += kColThreads) wp[r] += a2 * vp[r];
-    }
-    if (finalize_only) return;
-    double a[kRowsPerThread];
-    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
-#pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-        const int r = i + tid + k * kColThreads;
-        a[k] = 0.0;
-        if (r < n) a[k] = fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * s;
-    }
-    if (tid == 0) sh[0] = a[0];
-    if (tid == 1) sh[1] = a[0];
-    double xn = 0.0;
-#pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-        const int r = i + tid + k * kColThreads;
-        if (r >= i + 2 && r < n) xn += a[k] * a[k];
-    }
-    const double xnorm2 = group_sum<kColThreads>(xn, red);
-    const double dd = sh[0], alpha = sh[1];
-    double beta = alpha, tk = 0.0, scale = 0.0;
-    if (xnorm2 > 0.0) {
-        const double nr = sqrt(alpha * alpha + xnorm2);
-        beta = alpha >= 0.0 ? -nr : nr;
-        tk = (beta - alpha) / beta;
-        scale = 1.0 / (alpha - beta);
-    }
-    double* vj = V + (size_t)j * ldv;
-#pragma unroll
-    for (int k = 0; k < kRowsPerThread; ++k) {
-        const int r = i + tid + k * kColThreads;
-        if (r >= n) continue;
-        if (r == i) {
-            A[r + (size_t)i * ld] = dd;
-        } else if (r == i + 1) {
-            A[r + (size_t)i * ld] = beta;
-            vj[r] = 1.0;
-        } else {
-            const double v = a[k] * scale;
-            A[r + (size_t)i * ld] = v;
-            vj[r] = v;
-        }
-    }
-    if (tid == 0) {
-        d[i] = dd;
-        e[i] = beta;
-        tau[i] = tk;
-    }
+// a2 of the previous panel column (from its p.v partial sums, npv <= 64) and
+// s = p_i + 2 a2 v_i, the scalar of column i's last correction.  Every
+// wavefront forms the same sums in the same order.
+struct PrevColumn {
+    double a2 = 0.0, s = 0.0;
+};
+__device__ __forceinline__ PrevColumn prev_column(int i, int j, const double* __restrict__ V,
+                                                  const double* __restrict__ W, int ldv,
+                                                  const double* __restrict__ pv, int npv,
+                                                  const double* __restrict__ tau) {
+    PrevColumn pc;
+    const int l = threadIdx.x & 63;
+    const double pvs = wave_sum(l < npv ? pv[l] : 0.0);
+    pc.a2 = -0.5 * tau[i - 1] * pvs;
+    pc.s = W[(size_t)(j - 1) * ldv + i] + 2.0 * pc.a2 * V[(size_t)(j - 1) * ldv + i];
+    return pc;
 }
 
 // Strips of the trailing lower triangle (rows and columns >= c0): tile row RB
@@ -132,13 +85,14 @@ __host__ __device__ __forceinline__ int strips_before(int rho, int strip) {
 }
 // Strip length of a column with nt tile rows: long strips while there are
 // tiles enough to give every CU several work-groups, shorter ones below.
+constexpr int kMaxStrip = 8;
 int strip_length(int nt) {
-    for (int strip = 8; strip > 1; strip >>= 1)
+    for (int strip = kMaxStrip; strip > 1; strip >>= 1)
         if (strips_before(nt, strip) >= 2048) return strip;
     return 1;
 }
 
-// y = A v over the trailing lower triangle.  A work-group takes one strip:
+// y_u = A u over the trailing lower triangle.  A work-group takes one strip:
 // lane l holds row R0 + l, wavefront w columns 16 w .. 16 w + 15 of each
 // tile, and the next tile's 16 loads are in flight while the current one is
 // used.  The row part (row r times the columns of the strip) is summed over
@@ -148,41 +102,67 @@ int strip_length(int nt) {
 // block, zero in the others, plus the diagonal tile's column part in block(r)
 // itself); part[b][r] for b > block(r) is the column part of tile (b,
 // block(r)).  Every slot is written exactly once per column, so the sum over
-// b is deterministic.  Work-groups from nstrips on form the dot products V^T
-// v, W^T v over 256-row chunks.
+// b is deterministic.  Work-groups from nstrips on form V^T u, W^T u and |u|^2
+// over 256-row chunks; the first of them also records the column's scalars.
 __global__ void __launch_bounds__(256)
-    k_lt_symv(int n, int ld, int c0, int j, const double* __restrict__ A, const double* __restrict__ V,
-              const double* __restrict__ W, int ldv, double* __restrict__ part, int ldp, double* __restrict__ dots,
-              int nstrips, int nt, int strip) {
+    k_lt_symv(int n, int ld, int c0, int j, int fresh, const double* __restrict__ A, const double* __restrict__ V,
+              const double* __restrict__ W, int ldv, const double* __restrict__ next,
+              const double* __restrict__ pv, int npv, const double* __restrict__ tau, double* __restrict__ part,
+              int ldp, double* __restrict__ dots, double* __restrict__ scal, double* __restrict__ a2s, int nstrips,
+              int nt, int strip) {
     __shared__ double prs[4][kTile];
     __shared__ double pdc[kTile];
+    __shared__ double uc[kMaxStrip * kTile];
     __shared__ double wdots[4][2 * kNB];
+    __shared__ double wn[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const double* v = V + (size_t)j * ldv;
+    const int i = c0 - 1;
+    PrevColumn pc;
+    if (!fresh) pc = prev_column(i, j, V, W, ldv, pv, npv, tau);
+    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
+    // column i after the panel's corrections (rows >= i)
+    auto col = [&](int r) -> double { return fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * pc.s; };
+    auto u_of = [&](int r) -> double { return (r > c0 && r < n) ? col(r) : 0.0; };
     const int t = blockIdx.x;
     if (t >= nstrips) {
-        // V^T v and W^T v of a 256-row chunk: 16 products per lane summed over
-        // the wavefront by one transposed reduction, the four wavefronts in LDS
         const int ch = t - nstrips;
+        if (ch == 0 && tid == 0) {
+            scal[kScD] = col(i);
+            scal[kScAlpha] = col(c0);
+            scal[kScA2] = pc.a2;
+            scal[kScS] = pc.s;
+            if (!fresh) a2s[j - 1] = pc.a2;
+        }
         const int r = c0 + ch * kChunk + tid;
-        const double vv = r < n ? v[r] : 0.0;
+        const double uu = u_of(r);
+        // W(:, j-1) still holds p: a2 v is added on the fly
 #pragma unroll
         for (int g = 0; g < 2 * kNB / 16; ++g) {
             const int jj0 = (g & 1) * 16;
             if (jj0 >= j) continue;  // uniform: depends on j only
-            const double* M = g < 2 ? V : W;
             double cp[16];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) cp[q] = (r < n && jj0 + q < j) ? M[(size_t)(jj0 + q) * ldv + r] * vv : 0.0;
+            for (int q = 0; q < 16; ++q) {
+                const int jj = jj0 + q;
+                double m = 0.0;
+                if (r < n && jj < j) {
+                    m = g < 2 ? V[(size_t)jj * ldv + r] : W[(size_t)jj * ldv + r];
+                    if (g >= 2 && jj == j - 1) m += pc.a2 * vp[r];
+                }
+                cp[q] = m * uu;
+            }
             const double tot = wave::reduce16(cp, lane);
             if (lane < 16) wdots[w][g * 16 + wave::reduce16_index(lane)] = tot;
         }
+        const double nn = wave_sum(uu * uu);
+        if (lane == 0) wn[w] = nn;
         __syncthreads();
         if (tid < 2 * kNB) {
             const int jj = tid & (kNB - 1);
             if (jj < j)
-                dots[(size_t)ch * 2 * kNB + tid] = ((wdots[0][tid] + wdots[1][tid]) + wdots[2][tid]) + wdots[3][tid];
+                dots[(size_t)ch * kDot + tid] = ((wdots[0][tid] + wdots[1][tid]) + wdots[2][tid]) + wdots[3][tid];
         }
+        if (tid == 0) dots[(size_t)ch * kDot + 2 * kNB] = ((wn[0] + wn[1]) + wn[2]) + wn[3];
         return;
     }
     int lo = 0, hi = nt - 1;  // the tile row of strip t: largest rho with strips_before(rho, strip) <= t
@@ -196,7 +176,10 @@ __global__ void __launch_bounds__(256)
     const int CBe = min(CB0 + strip - 1, RB);
     const int R0 = RB * kTile, r = R0 + lane;
     const bool rok = r < n && r >= c0;
-    const double vr = rok ? v[r] : 0.0;
+    const double ur = u_of(r);
+    // u over the strip's columns, once per work-group
+    for (int k = tid; k < (CBe - CB0 + 1) * kTile; k += 256) uc[k] = u_of(CB0 * kTile + k);
+    __syncthreads();
     auto load = [&](double (&x)[16], int cb) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -211,12 +194,12 @@ __global__ void __launch_bounds__(256)
     for (int cb = CB0; cb <= CBe; ++cb) {
         if (cb < CBe) load(xn, cb + 1);
         const int C0 = cb * kTile + 16 * w;
+        const double* ucb = uc + (cb - CB0) * kTile + 16 * w;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int c = C0 + q;
-            const double vc = (c < n && c >= c0) ? v[c] : 0.0;
-            rs += x[q] * vc;
-            x[q] = (cb != RB || r > c) ? x[q] * vr : 0.0;  // the column product, in place
+            rs += x[q] * ucb[q];
+            x[q] = (cb != RB || r > c) ? x[q] * ur : 0.0;  // the column product, in place
         }
         const double tot = wave::reduce16(x, lane);
         const int cq = C0 + wave::reduce16_index(lane);
@@ -240,81 +223,125 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void __launch_bounds__(kChunk)
-    k_lt_reduce(int n, int ld, int c0, int j, int nblk, const double* __restrict__ part, int ldp,
-                const double* __restrict__ A, const double* __restrict__ V, double* __restrict__ W, int ldv,
-                const double* __restrict__ dots, int ndot, const double* __restrict__ tau, double* __restrict__ pv,
-                double* __restrict__ next, int last_in_panel) {
+    k_lt_reduce(int n, int ld, int c0, int j, int fresh, int last_in_panel, int nblk, const double* __restrict__ part,
+                int ldp, double* __restrict__ A, double* __restrict__ V, double* __restrict__ W, int ldv,
+                double* __restrict__ next, const double* __restrict__ dots, int ndot,
+                const double* __restrict__ scal, double* __restrict__ pv_out, double* __restrict__ d,
+                double* __restrict__ e, double* __restrict__ tau) {
     __shared__ double dv[kNB], dw[kNB], wi[kNB], vi[kNB];
-    __shared__ double s4[4][2 * kNB];
+    __shared__ double s4[4][kDot];
     __shared__ double red[kChunk / 64];
     const int tid = threadIdx.x;
+    const int i = c0 - 1;
+    const double a2 = scal[kScA2], s = scal[kScS];
     {
-        // the chunk sums of V^T v, W^T v: four interleaved partial sums per value
-        const int idx = tid & (2 * kNB - 1), q = tid >> 6;
-        double acc = 0.0;
+        // chunk sums of V^T u, W^T u, |u|^2: four interleaved partial sums per value
+        const int idx = tid & 63, q = tid >> 6;
+        double acc = 0.0, accn = 0.0;
         if ((idx & (kNB - 1)) < j)
-            for (int ch = q; ch < ndot; ch += 4) acc += dots[(size_t)ch * 2 * kNB + idx];
+            for (int ch = q; ch < ndot; ch += 4) acc += dots[(size_t)ch * kDot + idx];
+        if (idx == 0)
+            for (int ch = q; ch < ndot; ch += 4) accn += dots[(size_t)ch * kDot + 2 * kNB];
         s4[q][idx] = acc;
+        if (idx == 0) s4[q][2 * kNB] = accn;
     }
+    __syncthreads();
+    // the reflector (dlarfg), the same in every work-group
+    const double nrm2 = ((s4[0][2 * kNB] + s4[1][2 * kNB]) + s4[2][2 * kNB]) + s4[3][2 * kNB];
+    const double dd = scal[kScD], alpha = scal[kScAlpha];
+    double beta = alpha, tk = 0.0, scale = 0.0;
+    if (nrm2 > 0.0) {
+        const double nr = sqrt(alpha * alpha + nrm2);
+        beta = alpha >= 0.0 ? -nr : nr;
+        tk = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+    }
+    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
+    // W(r, jj) with the previous column's a2 v added
+    auto wf = [&](int r, int jj) -> double {
+        double x = W[(size_t)jj * ldv + r];
+        if (!fresh && jj == j - 1) x += a2 * vp[r];
+        return x;
+    };
     if (tid < j) {
-        wi[tid] = W[(size_t)tid * ldv + c0];
+        const double vu = ((s4[0][tid] + s4[1][tid]) + s4[2][tid]) + s4[3][tid];
+        const double wu = ((s4[0][kNB + tid] + s4[1][kNB + tid]) + s4[2][kNB + tid]) + s4[3][kNB + tid];
+        wi[tid] = wf(c0, tid);
         vi[tid] = V[(size_t)tid * ldv + c0];
+        dv[tid] = scale * vu + vi[tid];  // V^T v
+        dw[tid] = scale * wu + wi[tid];  // W^T v
     }
     __syncthreads();
-    if (tid < 2 * kNB && (tid & (kNB - 1)) < j) {
-        const double tot = ((s4[0][tid] + s4[1][tid]) + s4[2][tid]) + s4[3][tid];
-        if (tid < kNB) dv[tid] = tot;  // V^T v
-        else dw[tid - kNB] = tot;      // W^T v
+    if (blockIdx.x == 0 && tid == 0) {
+        d[i] = dd;
+        e[i] = beta;
+        tau[i] = tk;
+        A[i + (size_t)i * ld] = dd;
     }
-    __syncthreads();
-    const int b0 = c0 / kTile;
-    const double tk = tau[c0 - 1];
-    const double* v = V + (size_t)j * ldv;
-    const double vc0 = v[c0];
     const int r = c0 + blockIdx.x * kChunk + tid;
     double pvl = 0.0;
     if (r < n) {
+        const double u = r > c0 ? (fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * s) : 0.0;
+        const double v = r == c0 ? 1.0 : scale * u;
         // eight independent sums keep eight loads in flight; fixed combination order
+        const int b0 = c0 / kTile;
         double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int b = b0;
         for (; b + 8 <= nblk; b += 8) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(b + u) * ldp + r];
+            for (int uq = 0; uq < 8; ++uq) acc[uq] += part[(size_t)(b + uq) * ldp + r];
         }
-        for (int u = 0; b < nblk; ++b, ++u) acc[u] += part[(size_t)b * ldp + r];
-        const double y = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        for (int uq = 0; b < nblk; ++b, ++uq) acc[uq] += part[(size_t)b * ldp + r];
+        const double yu = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        const double acol = A[r + (size_t)c0 * ld];  // column i+1 of the trailing block
+        const double y = scale * yu + acol;
         double corr = 0.0, upd = 0.0;
 #pragma unroll 4
         for (int jj = 0; jj < j; ++jj) {
-            const double vr = V[(size_t)jj * ldv + r], wr = W[(size_t)jj * ldv + r];
+            const double vr = V[(size_t)jj * ldv + r], wr = wf(r, jj);
             corr += vr * dw[jj] + wr * dv[jj];
             upd += vr * wi[jj] + wr * vi[jj];
         }
         const double p = tk * (y - corr);
+        if (!fresh && r != c0) W[(size_t)(j - 1) * ldv + r] = wf(r, j - 1);  // finished
         W[(size_t)j * ldv + r] = p;
-        pvl = p * v[r];
-        if (!last_in_panel) next[r] = A[r + (size_t)c0 * ld] - upd - p * vc0;
+        V[(size_t)j * ldv + r] = v;
+        A[r + (size_t)i * ld] = r == c0 ? beta : v;
+        pvl = p * v;
+        if (!last_in_panel) next[r] = acol - upd - p;  // v_{i+1} = 1
     }
     const double sum = group_sum<kChunk>(pvl, red);
-    if (tid == 0) pv[blockIdx.x] = sum;
+    if (tid == 0) pv_out[blockIdx.x] = sum;
 }
 
 // A(c:, c:) -= V W^T + W V^T on the lower triangle of the trailing block (m =
-// n - c rows; A, V, W offset to row and column c), on the FP64 matrix cores.
-// One work-group per 64 x 64 tile on or below the diagonal; wavefront w takes
-// rows 16 w .. 16 w + 15 of the tile and its four 16 x 16 column blocks.  The
-// block sits in the accumulator transposed (element e of lane (li, lg) is
-// A(R0 + li, C0 + 4 e + lg)), so its loads and stores are column-contiguous;
-// D' += Aop Bop over K = 2 kNB with Aop = -[W_C | V_C] and Bop = [V_R | W_R]^T.
+// n - c rows; A, V, W offset to row and column c), on the FP64 matrix cores,
+// with the W terms still pending added on the fly: a2 v on the panel's last
+// column (a2 from its p.v partials and tau_c = tau of that column) and, in row
+// c, a2s[jb-2] v on the one before.  One work-group per 64 x 64 tile on or
+// below the diagonal; wavefront w takes rows 16 w .. 16 w + 15 of the tile and
+// its four 16 x 16 column blocks.  The block sits in the accumulator transposed
+// (element e of lane (li, lg) is A(R0 + li, C0 + 4 e + lg)), so its loads and
+// stores are column-contiguous; D' += Aop Bop over K = 2 kNB with
+// Aop = -[W_C | V_C] and Bop = [V_R | W_R]^T.
 __global__ void __launch_bounds__(256)
     k_lt_syr2k(int m, int jb, double* __restrict__ A, int ld, const double* __restrict__ V,
-               const double* __restrict__ W, int ldv) {
+               const double* __restrict__ W, int ldv, const double* __restrict__ pv, int npv,
+               const double* __restrict__ tau_c, const double* __restrict__ a2s) {
     const int t = blockIdx.x;
     int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((I + 1) * (I + 2) / 2 <= t) ++I;
     while (I * (I + 1) / 2 > t) --I;
     const int J = t - I * (I + 1) / 2;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const double a2_last = -0.5 * tau_c[0] * wave_sum(lane < npv ? pv[lane] : 0.0);
+    const double a2_prev = jb >= 2 ? a2s[jb - 2] : 0.0;
+    auto wv = [&](int kc, int row) -> double {
+        double x = W[(size_t)kc * ldv + row];
+        if (kc == jb - 1) x += a2_last * V[(size_t)kc * ldv + row];
+        else if (kc == jb - 2 && row == 0) x += a2_prev * V[(size_t)kc * ldv + row];
+        return x;
+    };
     const int R0 = I * kTile + 16 * w;
     if (R0 >= m) return;  // no barrier below: wavefronts are independent
     const int rB = R0 + li;
@@ -333,8 +360,8 @@ __global__ void __launch_bounds__(256)
         for (int k0 = 0; k0 < 2 * kNB; k0 += 4) {
             const int kk = k0 + lg, kc = kk & (kNB - 1);
             const bool upper = kk >= kNB, ok = kc < jb;
-            const double a = (ok && cA < m) ? -(upper ? V : W)[(size_t)kc * ldv + cA] : 0.0;
-            const double b = (ok && rB < m) ? (upper ? W : V)[(size_t)kc * ldv + rB] : 0.0;
+            const double a = (ok && cA < m) ? -(upper ? V[(size_t)kc * ldv + cA] : wv(kc, cA)) : 0.0;
+            const double b = (ok && rB < m) ? (upper ? wv(kc, rB) : V[(size_t)kc * ldv + rB]) : 0.0;
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
 #pragma unroll
@@ -361,15 +388,16 @@ hipError_t LargeTridiagWork::allocate(int rows) {
     if ((err = hipMalloc(&V, sizeof(double) * (size_t)rows * kNB)) != hipSuccess) return err;
     if ((err = hipMalloc(&W, sizeof(double) * (size_t)rows * kNB)) != hipSuccess) return err;
     if ((err = hipMalloc(&part, sizeof(double) * nblk * rows)) != hipSuccess) return err;
-    if ((err = hipMalloc(&dots, sizeof(double) * nch * 2 * kNB)) != hipSuccess) return err;
-    if ((err = hipMalloc(&pv, sizeof(double) * nch)) != hipSuccess) return err;
+    if ((err = hipMalloc(&dots, sizeof(double) * nch * kDot)) != hipSuccess) return err;
+    if ((err = hipMalloc(&pv, sizeof(double) * 2 * nch)) != hipSuccess) return err;
     if ((err = hipMalloc(&next, sizeof(double) * rows)) != hipSuccess) return err;
+    if ((err = hipMalloc(&scal, sizeof(double) * (kScalars + kNB))) != hipSuccess) return err;
     n = rows;
     return hipSuccess;
 }
 
 void LargeTridiagWork::release() {
-    for (double** p : {&V, &W, &part, &dots, &pv, &next}) {
+    for (double** p : {&V, &W, &part, &dots, &pv, &next, &scal}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
@@ -381,32 +409,38 @@ hipError_t launch_tridiag_large(hipStream_t s, int n, int ld, double* A, double*
     if (n < 2 || n > kLargeTridiagMaxDim || ld < n || w.n < n) return hipErrorInvalidValue;
     const int ldv = w.n, ldp = w.n;
     const int nblk = (n + kTile - 1) / kTile;
+    const int nch_max = (w.n + kChunk - 1) / kChunk;
+    double* scal = w.scal;
+    double* a2s = w.scal + kScalars;
     int npv = 0;
+    const double* pv_prev = w.pv;
     for (int i0 = 0; i0 < n - 1; i0 += kNB) {
         const int jb = std::min(kNB, n - 1 - i0);
         for (int j = 0; j < jb; ++j) {
             const int i = i0 + j, c0 = i + 1;
-            hipLaunchKernelGGL(k_lt_column, dim3(1), dim3(kColThreads), 0, s, n, ld, i, j, j == 0 ? 1 : 0, 0, A,
-                               w.V, w.W, ldv, (const double*)w.next, (const double*)w.pv, npv, d, e, tau);
+            // alternate p.v buffers: the previous column's partials stay readable
+            double* pv_out = w.pv + (size_t)(i & 1) * nch_max;
             const int b0 = c0 / kTile, nt = nblk - b0;
             const int strip = strip_length(nt);
             const int nstrips = strips_before(nt, strip);
             const int nch = (n - c0 + kChunk - 1) / kChunk;
-            const int ndot = j > 0 ? nch : 0;
-            hipLaunchKernelGGL(k_lt_symv, dim3(nstrips + ndot), dim3(256), 0, s, n, ld, c0, j, (const double*)A,
-                               (const double*)w.V, (const double*)w.W, ldv, w.part, ldp, w.dots, nstrips, nt, strip);
-            hipLaunchKernelGGL(k_lt_reduce, dim3(nch), dim3(kChunk), 0, s, n, ld, c0, j, nblk,
-                               (const double*)w.part, ldp, (const double*)A, (const double*)w.V, w.W, ldv,
-                               (const double*)w.dots, ndot, (const double*)tau, w.pv, w.next, j == jb - 1 ? 1 : 0);
+            const int fresh = j == 0 ? 1 : 0;
+            hipLaunchKernelGGL(k_lt_symv, dim3(nstrips + nch), dim3(256), 0, s, n, ld, c0, j, fresh,
+                               (const double*)A, (const double*)w.V, (const double*)w.W, ldv,
+                               (const double*)w.next, pv_prev, npv, (const double*)tau, w.part, ldp, w.dots, scal,
+                               a2s, nstrips, nt, strip);
+            hipLaunchKernelGGL(k_lt_reduce, dim3(nch), dim3(kChunk), 0, s, n, ld, c0, j, fresh,
+                               j == jb - 1 ? 1 : 0, nblk, (const double*)w.part, ldp, A, w.V, w.W, ldv, w.next,
+                               (const double*)w.dots, nch, (const double*)scal, pv_out, d, e, tau);
+            pv_prev = pv_out;
             npv = nch;
         }
-        // the panel's last W column, then A(c:, c:) -= V W^T + W V^T
+        // A(c:, c:) -= V W^T + W V^T
         const int c = i0 + jb;
-        hipLaunchKernelGGL(k_lt_column, dim3(1), dim3(kColThreads), 0, s, n, ld, c, jb, 0, 1, A, w.V, w.W, ldv,
-                           (const double*)w.next, (const double*)w.pv, npv, d, e, tau);
         const int m = n - c, mt = (m + kTile - 1) / kTile;
         hipLaunchKernelGGL(k_lt_syr2k, dim3(mt * (mt + 1) / 2), dim3(256), 0, s, m, jb, A + c + (size_t)c * ld, ld,
-                           (const double*)(w.V + c), (const double*)(w.W + c), ldv);
+                           (const double*)(w.V + c), (const double*)(w.W + c), ldv, pv_prev, npv,
+                           (const double*)(tau + c - 1), (const double*)a2s);
     }
     hipLaunchKernelGGL(k_lt_last, dim3(1), dim3(1), 0, s, n, ld, (const double*)A, d, e, tau);
     return hipGetLastError();

@@ -16,9 +16,10 @@ struct LargeTridiagWork {
     double* V = nullptr;     // n x kLargeTridiagPanel reflectors of the current panel
     double* W = nullptr;     // n x kLargeTridiagPanel
     double* part = nullptr;  // ceil(n / 64) x n partial sums of the matrix-vector product
-    double* dots = nullptr;  // per 256-row chunk: V^T v and W^T v (2 x kLargeTridiagPanel)
-    double* pv = nullptr;    // per 256-row chunk: p . v
+    double* dots = nullptr;  // per 256-row chunk: V^T u, W^T u and |u|^2
+    double* pv = nullptr;    // per 256-row chunk: p . v, two columns' worth
     double* next = nullptr;  // the next column after the panel's corrections but the last (n)
+    double* scal = nullptr;  // the current column's scalars and the panel's a2 values
     hipError_t allocate(int rows);
     void release();
 };
