@@ -31,7 +31,9 @@ namespace {
 constexpr int kNB = kLargeTridiagPanel;
 static_assert(kNB == 32, "the dot products run as two groups of 16 panel columns");
 constexpr int kTile = 64;
-constexpr int kChunk = 256;        // rows per work-group of k_lt_reduce and of the dot products
+constexpr int kChunk = 256;        // rows per work-group of the dot products
+constexpr int kRedRows = 64;       // rows per work-group of k_lt_reduce (four threads per row)
+constexpr int kMaxPartials = 256;  // p.v partial sums per column: kLargeTridiagMaxDim / kRedRows
 constexpr int kDot = 2 * kNB + 8;  // per chunk: V^T u (kNB), W^T u (kNB), |u|^2, padding
 typedef double tvec4 __attribute__((ext_vector_type(4)));
 
@@ -57,7 +59,18 @@ __device__ double group_sum(double v, double* red) {
     return t;
 }
 
-// a2 of the previous panel column (from its p.v partial sums, npv <= 64) and
+// Sum of npv <= kMaxPartials partial sums: four per lane in order, then the
+// wavefront; every wavefront forms the same sum in the same order.
+__device__ __forceinline__ double partial_sum(const double* __restrict__ pv, int npv) {
+    const int l = threadIdx.x & 63;
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxPartials / 64; ++k)
+        if (l + 64 * k < npv) x += pv[l + 64 * k];
+    return wave_sum(x);
+}
+
+// a2 of the previous panel column (from its p.v partial sums) and
 // s = p_i + 2 a2 v_i, the scalar of column i's last correction.  Every
 // wavefront forms the same sums in the same order.
 struct PrevColumn {
@@ -68,9 +81,7 @@ __device__ __forceinline__ PrevColumn prev_column(int i, int j, const double* __
                                                   const double* __restrict__ pv, int npv,
                                                   const double* __restrict__ tau) {
     PrevColumn pc;
-    const int l = threadIdx.x & 63;
-    const double pvs = wave_sum(l < npv ? pv[l] : 0.0);
-    pc.a2 = -0.5 * tau[i - 1] * pvs;
+    pc.a2 = -0.5 * tau[i - 1] * partial_sum(pv, npv);
     pc.s = W[(size_t)(j - 1) * ldv + i] + 2.0 * pc.a2 * V[(size_t)(j - 1) * ldv + i];
     return pc;
 }
@@ -222,7 +233,7 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-__global__ void __launch_bounds__(kChunk)
+__global__ void __launch_bounds__(256)
     k_lt_reduce(int n, int ld, int c0, int j, int fresh, int last_in_panel, int nblk, const double* __restrict__ part,
                 int ldp, double* __restrict__ A, double* __restrict__ V, double* __restrict__ W, int ldv,
                 double* __restrict__ next, const double* __restrict__ dots, int ndot,
@@ -230,7 +241,8 @@ __global__ void __launch_bounds__(kChunk)
                 double* __restrict__ e, double* __restrict__ tau) {
     __shared__ double dv[kNB], dw[kNB], wi[kNB], vi[kNB];
     __shared__ double s4[4][kDot];
-    __shared__ double red[kChunk / 64];
+    __shared__ double yq[4][kRedRows], cq[4][kRedRows], uq[4][kRedRows];
+    __shared__ double red[4];
     const int tid = threadIdx.x;
     const int i = c0 - 1;
     const double a2 = scal[kScA2], s = scal[kScS];
@@ -278,31 +290,40 @@ __global__ void __launch_bounds__(kChunk)
         tau[i] = tk;
         A[i + (size_t)i * ld] = dd;
     }
-    const int r = c0 + blockIdx.x * kChunk + tid;
-    double pvl = 0.0;
+    // four threads per row: quarter q adds the slots b = b0 + q, b0 + q + 4, ...
+    // and the panel columns jj = q, q + 4, ...; the quarters are combined in order
+    const int q = tid >> 6;
+    const int r = c0 + blockIdx.x * kRedRows + (tid & 63);
     if (r < n) {
-        const double u = r > c0 ? (fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * s) : 0.0;
-        const double v = r == c0 ? 1.0 : scale * u;
-        // eight independent sums keep eight loads in flight; fixed combination order
         const int b0 = c0 / kTile;
-        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        int b = b0;
-        for (; b + 8 <= nblk; b += 8) {
-#pragma unroll
-            for (int uq = 0; uq < 8; ++uq) acc[uq] += part[(size_t)(b + uq) * ldp + r];
+        double y0 = 0.0, y1 = 0.0;
+        int b = b0 + q;
+        for (; b + 4 < nblk; b += 8) {
+            y0 += part[(size_t)b * ldp + r];
+            y1 += part[(size_t)(b + 4) * ldp + r];
         }
-        for (int uq = 0; b < nblk; ++b, ++uq) acc[uq] += part[(size_t)b * ldp + r];
-        const double yu = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        const double acol = A[r + (size_t)c0 * ld];  // column i+1 of the trailing block
-        const double y = scale * yu + acol;
+        if (b < nblk) y0 += part[(size_t)b * ldp + r];
         double corr = 0.0, upd = 0.0;
-#pragma unroll 4
-        for (int jj = 0; jj < j; ++jj) {
+        for (int jj = q; jj < j; jj += 4) {
             const double vr = V[(size_t)jj * ldv + r], wr = wf(r, jj);
             corr += vr * dw[jj] + wr * dv[jj];
             upd += vr * wi[jj] + wr * vi[jj];
         }
-        const double p = tk * (y - corr);
+        yq[q][tid & 63] = y0 + y1;
+        cq[q][tid & 63] = corr;
+        uq[q][tid & 63] = upd;
+    }
+    __syncthreads();
+    double pvl = 0.0;
+    if (q == 0 && r < n) {
+        const int l = tid;
+        const double yu = ((yq[0][l] + yq[1][l]) + yq[2][l]) + yq[3][l];
+        const double corr = ((cq[0][l] + cq[1][l]) + cq[2][l]) + cq[3][l];
+        const double upd = ((uq[0][l] + uq[1][l]) + uq[2][l]) + uq[3][l];
+        const double u = r > c0 ? (fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * s) : 0.0;
+        const double v = r == c0 ? 1.0 : scale * u;
+        const double acol = A[r + (size_t)c0 * ld];  // column i+1 of the trailing block
+        const double p = tk * (scale * yu + acol - corr);
         if (!fresh && r != c0) W[(size_t)(j - 1) * ldv + r] = wf(r, j - 1);  // finished
         W[(size_t)j * ldv + r] = p;
         V[(size_t)j * ldv + r] = v;
@@ -310,7 +331,7 @@ __global__ void __launch_bounds__(kChunk)
         pvl = p * v;
         if (!last_in_panel) next[r] = acol - upd - p;  // v_{i+1} = 1
     }
-    const double sum = group_sum<kChunk>(pvl, red);
+    const double sum = group_sum<256>(pvl, red);
     if (tid == 0) pv_out[blockIdx.x] = sum;
 }
 
@@ -334,7 +355,7 @@ __global__ void __launch_bounds__(256)
     while (I * (I + 1) / 2 > t) --I;
     const int J = t - I * (I + 1) / 2;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
-    const double a2_last = -0.5 * tau_c[0] * wave_sum(lane < npv ? pv[lane] : 0.0);
+    const double a2_last = -0.5 * tau_c[0] * partial_sum(pv, npv);
     const double a2_prev = jb >= 2 ? a2s[jb - 2] : 0.0;
     auto wv = [&](int kc, int row) -> double {
         double x = W[(size_t)kc * ldv + row];
@@ -389,7 +410,8 @@ hipError_t LargeTridiagWork::allocate(int rows) {
     if ((err = hipMalloc(&W, sizeof(double) * (size_t)rows * kNB)) != hipSuccess) return err;
     if ((err = hipMalloc(&part, sizeof(double) * nblk * rows)) != hipSuccess) return err;
     if ((err = hipMalloc(&dots, sizeof(double) * nch * kDot)) != hipSuccess) return err;
-    if ((err = hipMalloc(&pv, sizeof(double) * 2 * nch)) != hipSuccess) return err;
+    if ((err = hipMalloc(&pv, sizeof(double) * 2 * (((size_t)rows + kRedRows - 1) / kRedRows))) != hipSuccess)
+        return err;
     if ((err = hipMalloc(&next, sizeof(double) * rows)) != hipSuccess) return err;
     if ((err = hipMalloc(&scal, sizeof(double) * (kScalars + kNB))) != hipSuccess) return err;
     n = rows;
@@ -409,7 +431,7 @@ hipError_t launch_tridiag_large(hipStream_t s, int n, int ld, double* A, double*
     if (n < 2 || n > kLargeTridiagMaxDim || ld < n || w.n < n) return hipErrorInvalidValue;
     const int ldv = w.n, ldp = w.n;
     const int nblk = (n + kTile - 1) / kTile;
-    const int nch_max = (w.n + kChunk - 1) / kChunk;
+    const int nred_max = (w.n + kRedRows - 1) / kRedRows;
     double* scal = w.scal;
     double* a2s = w.scal + kScalars;
     int npv = 0;
@@ -419,7 +441,7 @@ hipError_t launch_tridiag_large(hipStream_t s, int n, int ld, double* A, double*
         for (int j = 0; j < jb; ++j) {
             const int i = i0 + j, c0 = i + 1;
             // alternate p.v buffers: the previous column's partials stay readable
-            double* pv_out = w.pv + (size_t)(i & 1) * nch_max;
+            double* pv_out = w.pv + (size_t)(i & 1) * nred_max;
             const int b0 = c0 / kTile, nt = nblk - b0;
             const int strip = strip_length(nt);
             const int nstrips = strips_before(nt, strip);
@@ -429,11 +451,12 @@ hipError_t launch_tridiag_large(hipStream_t s, int n, int ld, double* A, double*
                                (const double*)A, (const double*)w.V, (const double*)w.W, ldv,
                                (const double*)w.next, pv_prev, npv, (const double*)tau, w.part, ldp, w.dots, scal,
                                a2s, nstrips, nt, strip);
-            hipLaunchKernelGGL(k_lt_reduce, dim3(nch), dim3(kChunk), 0, s, n, ld, c0, j, fresh,
+            const int nred = (n - c0 + kRedRows - 1) / kRedRows;
+            hipLaunchKernelGGL(k_lt_reduce, dim3(nred), dim3(256), 0, s, n, ld, c0, j, fresh,
                                j == jb - 1 ? 1 : 0, nblk, (const double*)w.part, ldp, A, w.V, w.W, ldv, w.next,
                                (const double*)w.dots, nch, (const double*)scal, pv_out, d, e, tau);
             pv_prev = pv_out;
-            npv = nch;
+            npv = nred;
         }
         // A(c:, c:) -= V W^T + W V^T
         const int c = i0 + jb;
