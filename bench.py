@@ -129,7 +129,7 @@ def run_path(eng, label, steps, warmup, tune=False):
     n = eng.n
     return {"ms": 1000.0 * float(np.mean(times)), "tri": tri, "launch": eng.launch_stats(), "tuning": tuning,
             "status": np.array([eng.status(i) for i in range(n)]),
-            "iters": np.array([eng.iterations(i) for i in range(n)])}
+            "iters": np.array([eng.iterations(i) for i in range(n)]), "phase_ms": eng.timings()}
 
 
 def standalone_kernel_pass(eng):
@@ -470,7 +470,7 @@ def main():
     for name, rr in res.items():
         paths[name] = {"molecules_per_s": len(systems) / (rr["ms"] / 1000.0), "ms_per_step": rr["ms"],
                        "mean_scc_iterations": float(rr["iters"].mean()),
-                       "converged": int((rr["status"] == 0).sum()),
+                       "converged": int((rr["status"] == 0).sum()), "phase_ms": rr["phase_ms"],
                        "parity_vs_oracle": parity(engines[name], systems, args.check) if args.check > 0 else None}
     if "plain" in res and "learned" in res:
         ep, el = engines["plain"], engines["learned"]
