@@ -16,7 +16,7 @@ LIB = os.path.join(LIB_DIR, "libgfn1_amd.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("GFN1_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["model.cpp", "capi.cpp", "engine.hip", "kernels.hip", "eigen.hip", "guess.hip", "cholesky.hip", "density.hip", "tridiag_large.hip", "optimizer.cpp", "hessian.cpp", "md.cpp"]
+SOURCES = ["model.cpp", "capi.cpp", "engine.hip", "kernels.hip", "eigen.hip", "guess.hip", "cholesky.hip", "density.hip", "tridiag_large.hip", "periodic.hip", "optimizer.cpp", "hessian.cpp", "md.cpp"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

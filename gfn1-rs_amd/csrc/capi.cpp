@@ -60,6 +60,7 @@ struct gfn1_guess_model {
 struct gfn1_xyz {
     std::vector<int32_t> z;
     std::vector<double> x;
+    std::vector<double> lattice;  // 9 values when the input has a Lattice= field
     gfn1_system sys{};
 };
 
@@ -128,6 +129,9 @@ Status validate(const gfn1_system* s, const gfn1_electronic_options* o) {
     if (o && o->nprim != 0)
         return Status::error(kErrInvalidInput, "only the standard GFN1 primitive-count rule (nprim = 0) is supported");
     if (o && o->max_scc < 1) return Status::error(kErrInvalidInput, "max_scc must be at least 1");
+    if (s->lattice)
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(s->lattice[k])) return Status::error(kErrInvalidInput, "non-finite lattice vector");
     return Status{};
 }
 
@@ -137,6 +141,7 @@ SystemInput to_input(const gfn1_system* s, const gfn1_electronic_options* o) {
     in.z = s->numbers;
     in.x = s->positions;
     in.charge = (o && o->has_charge) ? o->charge : s->charge;
+    in.cell = s->lattice;
     if (o && o->scc_initial_shell_charges) {
         in.q0 = o->scc_initial_shell_charges;
         in.q0_len = o->scc_initial_shell_charges_len;
@@ -844,10 +849,50 @@ int gfn1_system_from_xyz(const char* text, double charge, int32_t coordinates_ar
     if (head.size() != 1 || end != head[0].c_str() + head[0].size() || natoms < 0)
         return fail(err, Status::error(kErrInvalidInput, "invalid XYZ atom count: " + lines[0]));
     const std::string comment = lines.size() > 1 ? lines[1] : std::string();
-    if (comment.find("Lattice=\"") != std::string::npos)
-        return fail(err, Status::error(kErrInvalidInput, "periodic (Lattice=) XYZ input is not supported"));
     auto r = std::make_unique<gfn1_xyz>();
     const double scale = coordinates_are_bohr ? 1.0 : gfn1::kAngstromToBohr;
+    // extended-XYZ cell (src/system.rs:191-254)
+    const size_t lp = comment.find("Lattice=\"");
+    if (lp != std::string::npos) {
+        const size_t b0 = lp + 9, e0 = comment.find('"', b0);
+        if (e0 == std::string::npos)
+            return fail(err, Status::error(kErrInvalidInput, "unterminated Lattice=\"...\" field in XYZ comment"));
+        const std::vector<std::string> vals = split_ws(comment.substr(b0, e0 - b0));
+        for (const std::string& v : vals) {
+            double d = 0.0;
+            if (!parse_number(v, &d))
+                return fail(err, Status::error(kErrInvalidInput, "invalid number in XYZ Lattice field"));
+            r->lattice.push_back(d * scale);
+        }
+        if (r->lattice.size() != 9)
+            return fail(err, Status::error(kErrInvalidInput, "XYZ Lattice field needs 9 numbers, got " +
+                                                                 std::to_string(r->lattice.size())));
+        size_t pp = comment.find("pbc=\"");
+        if (pp == std::string::npos) pp = comment.find("PBC=\"");
+        if (pp != std::string::npos) {
+            const size_t b1 = pp + 5, e1 = comment.find('"', b1);
+            if (e1 == std::string::npos)
+                return fail(err, Status::error(kErrInvalidInput, "unterminated pbc=\"...\" field in XYZ comment"));
+            const std::vector<std::string> toks = split_ws(comment.substr(b1, e1 - b1));
+            if (toks.size() != 3)
+                return fail(err, Status::error(kErrInvalidInput, "XYZ pbc field needs 3 boolean tokens, got " +
+                                                                     std::to_string(toks.size())));
+            for (const std::string& tk : toks) {
+                std::string l = tk;
+                for (char& ch : l) ch = (char)std::tolower((unsigned char)ch);
+                const bool yes = l == "t" || l == "true" || l == "1" || l == ".true." || l == "yes" || l == "y";
+                const bool no = l == "f" || l == "false" || l == "0" || l == ".false." || l == "no" || l == "n";
+                if (!yes && !no) return fail(err, Status::error(kErrInvalidInput, "invalid XYZ pbc token: " + tk));
+                if (no)
+                    return fail(err, Status::error(kErrInvalidInput,
+                                                   "only fully periodic cells (pbc=\"T T T\") are supported"));
+            }
+        }
+        CellRec probe;
+        std::vector<double> gprobe;
+        if (!make_cell_record(r->lattice.data(), probe, gprobe))
+            return fail(err, Status::error(kErrSingularCell, "singular cell"));
+    }
     for (long long idx = 0; idx < natoms; ++idx) {
         const size_t line_no = (size_t)idx + 3;
         if ((size_t)idx + 2 >= lines.size())
@@ -873,6 +918,7 @@ int gfn1_system_from_xyz(const char* text, double charge, int32_t coordinates_ar
     r->sys.numbers = r->z.empty() ? nullptr : r->z.data();
     r->sys.positions = r->x.empty() ? nullptr : r->x.data();
     r->sys.charge = charge;
+    r->sys.lattice = r->lattice.empty() ? nullptr : r->lattice.data();
     *out = r.release();
     return GFN1_OK;
 }

@@ -72,6 +72,31 @@ struct DeviceTables {
     Globals glob;
 };
 
+// Periodic cell of one instance (the Gamma-point path, pbc/mod.rs:83-91).
+// Molecular instances have on = 0 and are never read beyond that flag.
+struct CellRec {
+    double a[3][3];    // cell vectors, one per row (bohr)
+    double h[3];       // plane heights: the distance between lattice planes normal to a x b etc.
+    double vol;        // |det|
+    double alpha;      // Ewald splitting, sqrt(pi) / V^(1/3) (pbc/ewald.rs:48-60)
+    double ew_self;    // -2 alpha / sqrt(pi), the 1/R self term of an atom
+    double ew_bg;      // -pi / (alpha^2 V), the neutralising background per atom pair
+    double r3_k0;      // -1/2 * 4 pi [ln k + (ln pi - psi(3/2)) / 2] / V, k = alpha / sqrt(pi), per eta^-2
+    double r3_self;    // -1/2 * -(4 pi / 3) k^3, per eta^-2, same-atom pairs
+    int64_t g0;        // first reciprocal vector of the instance in the G table
+    int64_t gam0;      // offset of the instance's shell Coulomb matrix (nsh x nsh)
+    int32_t ng;        // reciprocal vectors with 0 < |G| <= 2 alpha tau
+    int32_t on;        // periodic instance
+};
+// G table entry: G (bohr^-1), w1 = 4 pi / V exp(-G^2 / 4 alpha^2) / G^2 (Ewald
+// 1/R), w3 = -1/2 * 2 pi / V * E1(G^2 / 4 alpha^2) (generalised Ewald R^-3).
+constexpr int kGStride = 5;
+// Cutoffs of the periodic path: AO image sums (pbc/mod.rs:170), the Ewald
+// real-space remainder (:162), the Ewald convergence factor tau (pbc/ewald.rs:39).
+constexpr double kPbcAoCutoff = 30.0;
+constexpr double kEwaldShortCutoff = 10.0;
+constexpr double kEwaldTau = 5.5;
+
 // Flat (instance-major) index tables of a batch.
 struct DeviceBatch {
     int32_t ninst;
@@ -102,6 +127,14 @@ struct DeviceBatch {
     double* const* CB1;
     double* const* T;
     double* const* A;
+    // periodic instances (nullptr when the batch has none)
+    const CellRec* cell;     // ninst
+    const double* gtab;      // kGStride per reciprocal vector
+    const double* gam;       // shell Coulomb matrices of the periodic instances
 };
+
+__host__ __device__ inline bool periodic_instance(const DeviceBatch& b, int inst) {
+    return b.cell != nullptr && b.cell[inst].on != 0;
+}
 
 }  // namespace gfn1

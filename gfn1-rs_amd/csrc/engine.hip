@@ -168,9 +168,14 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     h_nelec_.assign(ninst_, 0.0);
     h_guess_usable_.assign(ninst_, 1);
     h_charge_.assign(ninst_, 0.0);
+    h_pbc_.assign(ninst_, 0);
+    std::vector<CellRec> cells(ninst_);
+    std::vector<double> gtab;
+    int64_t gam_size = 0;
     for (int i = 0; i < ninst_; ++i) {
         const SystemInput& s = systems[i];
         h_charge_[i] = s.charge;
+        std::memset(&cells[i], 0, sizeof(CellRec));
         int nsh = 0, nao = 0;
         double nref = 0.0;
         for (int a = 0; a < s.nat; ++a) {
@@ -217,6 +222,23 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         } else if (!std::isfinite(nelec) || nelec < -1e-8 || nelec > 2.0 * nao + 1e-8) {
             h_err_[i] = kErrInvalidInput;
             h_errmsg_[i] = "electron count out of range";
+        }
+        if (s.cell) {
+            // Lattice::new (lattice.rs:73-87): a singular cell is an input error
+            if (!make_cell_record(s.cell, cells[i], gtab)) {
+                if (!h_err_[i]) {
+                    h_err_[i] = kErrSingularCell;
+                    h_errmsg_[i] = "singular cell";
+                }
+                std::memset(&cells[i], 0, sizeof(CellRec));
+            } else {
+                h_pbc_[i] = 1;
+                cells[i].gam0 = gam_size;
+                gam_size += round_up(nsh * nsh, kRangeAlign);
+                h_pinst_.push_back(i);
+                max_pnsh_ = std::max(max_pnsh_, nsh);
+                for (int a = 0; a < s.nat; ++a) h_patoms_.push_back(h_atom0_[i] + a);
+            }
         }
         // initial shell charges (electronic.rs:1235-1239): used when the length matches
         for (int k = 0; k < nsh; ++k)
@@ -334,6 +356,15 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     if (!(st = dalloc(&d_brom_, (size_t)ninst_ * prm_.omega_stride)).ok()) return st;
     if (!(st = dalloc(&d_state_, ninst_)).ok()) return st;
     if (!(st = dalloc(&d_mat_, (size_t)kRoles * ninst_)).ok()) return st;
+    if (!h_pinst_.empty()) {
+        if (!(st = dalloc(&d_cell_, ninst_)).ok()) return st;
+        HIP_TRY(copy_bytes(d_cell_, cells.data(), ninst_ * sizeof(CellRec), hipMemcpyHostToDevice));
+        if (!(st = dalloc(&d_gtab_, std::max<size_t>(gtab.size(), 1))).ok()) return st;
+        HIP_TRY(copy_bytes(d_gtab_, gtab.data(), gtab.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (!(st = dalloc(&d_gam_, std::max<int64_t>(gam_size, 1))).ok()) return st;
+        if (!(st = up_i(&d_pinst_, h_pinst_)).ok()) return st;
+        if (!(st = up_i(&d_patoms_, h_patoms_)).ok()) return st;
+    }
 
     // ---- size buckets and matrix arenas
     std::vector<int> lds = h_ld_;
@@ -396,8 +427,10 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
             HIP_TRY(hipEventCreateWithFlags(&bk.g_ev[slot], hipEventDisableTiming));
         }
         std::vector<int> atoms;
-        for (int i : bk.inst)
+        for (int i : bk.inst) {
             for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
+            if (h_pbc_[i]) bk.has_pbc = true;
+        }
         bk.natoms = (int)atoms.size();
         const size_t na_alloc = std::max<size_t>(atoms.size(), 1);
         HIP_TRY(hipMalloc(&bk.atoms, sizeof(int) * na_alloc));
@@ -435,6 +468,9 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     db_.CB1 = d_mat_ + (size_t)R_CB1 * ninst_;
     db_.T = d_mat_ + (size_t)R_T * ninst_;
     db_.A = d_mat_ + (size_t)R_A * ninst_;
+    db_.cell = d_cell_;
+    db_.gtab = d_gtab_;
+    db_.gam = d_gam_;
     h_final_parity_.assign(ninst_, 0);
     return Status{};
 }
@@ -665,9 +701,11 @@ Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double
         if (st != kOk) continue;
         const double e_disp = opt_.enable_dispersion ? edisp[i] : 0.0;
         energy[i] = s.e_el + s.e2 + s.e3 + erep[i] + e_disp + ehal[i] + s.ts;
+        const bool all = periodic(i) && opt_.compute_gradient;  // pbc/gradient.rs:75-163: every term
         for (int k = 3 * h_atom0_[i]; k < 3 * h_atom0_[i + 1]; ++k)
-            gradient[k] = (el_on ? gel[k] : 0.0) + (rep_on ? grep[k] : 0.0) + (disp_on ? gdisp[k] : 0.0) +
-                          (hal_on ? ghal[k] : 0.0);
+            gradient[k] = (el_on || all ? gel[k] : 0.0) + (rep_on || all ? grep[k] : 0.0) +
+                          (disp_on || (all && opt_.enable_dispersion) ? gdisp[k] : 0.0) +
+                          (hal_on || all ? ghal[k] : 0.0);
     }
     return Status{};
 }
@@ -710,7 +748,8 @@ Status Batch::setup_phase() {
         std::memset(&s, 0, sizeof(s));
         s.status = !h_sel_[i] ? 3 : (h_err_[i] ? 2 : 0);
         s.mixing = prm_.rung_mixing[0];
-        s.level_shift = prm_.rung_level_shift[0];
+        // the periodic SCC applies no level shift (pbc/scf.rs:358-529)
+        s.level_shift = periodic(i) ? 0.0 : prm_.rung_level_shift[0];
         s.temperature = prm_.rung_temperature[0];
         s.kt = std::max(s.temperature, 0.0) * kBoltzmannHartree;
         s.nelec = h_nelec_[i];
@@ -723,6 +762,9 @@ Status Batch::setup_phase() {
                        d_gw_ + (size_t)na * kMaxRef, na);
     LAUNCH_CHECK();
     HIP_TRY(hipEventRecord(ev_cn_, stream_));
+    // Ewald shell Coulomb matrices of the periodic instances (needed from the
+    // first SCC iteration on; the buckets wait for ev_start_ below)
+    HIP_TRY(launch_ewald_gamma(stream_, db_, t, d_pinst_, (int)h_pinst_.size(), max_pnsh_));
 
     const double one = 1.0, zero = 0.0;
     auto a0_gemms = [&](Bucket& bk, int count) -> Status {
@@ -745,14 +787,8 @@ Status Batch::setup_phase() {
         Status st = upload_active(bk, bk.sel, 0);
         if (!st.ok()) return st;
         if (bk.natoms > 0) {
-            hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
-                               (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
-            LAUNCH_CHECK();
-            if (has_d_) {
-                hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
-                                   (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
-                LAUNCH_CHECK();
-            }
+            Status st2 = launch_integrals(bk);
+            if (!st2.ok()) return st2;
         }
         hipLaunchKernelGGL(k_pad_diagonal, dim3(count), dim3(64), 0, bk.stream, db_, (const int*)bk.act);
         LAUNCH_CHECK();
@@ -874,6 +910,31 @@ Status Batch::setup_phase() {
     return Status{};
 }
 
+// S and H0 of the bucket's instances on its stream: the molecular kernels, and
+// the periodic (image-summed) ones when the bucket holds a periodic instance.
+Status Batch::launch_integrals(Bucket& bk) {
+    const DeviceTables t = dm_->tables();
+    hipLaunchKernelGGL((k_overlap_h0_t<false, false>), dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                       (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+    LAUNCH_CHECK();
+    if (has_d_) {
+        hipLaunchKernelGGL((k_overlap_h0_t<true, false>), dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                           (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
+        LAUNCH_CHECK();
+    }
+    if (bk.has_pbc) {
+        hipLaunchKernelGGL((k_overlap_h0_t<false, true>), dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                           (const int*)bk.atoms, (const double*)d_cn_, kPbcAoCutoff);
+        LAUNCH_CHECK();
+        if (has_d_) {
+            hipLaunchKernelGGL((k_overlap_h0_t<true, true>), dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
+                               (const int*)bk.atoms, (const double*)d_cn_, kPbcAoCutoff);
+            LAUNCH_CHECK();
+        }
+    }
+    return Status{};
+}
+
 // Setup of one bucket of at most kCholMaxDim AO on its stream: S and H0 over
 // its atoms, S = L L^T and Cb = L^-T, failure flags (device state and a host
 // copy), A0 = Cb^T H0 Cb.  Its `done` event is recorded by the caller once the
@@ -886,14 +947,8 @@ Status Batch::launch_bucket_setup(Bucket& bk) {
     Status st = upload_active(bk, bk.sel, 0);
     if (!st.ok()) return st;
     if (bk.natoms > 0) {
-        hipLaunchKernelGGL(k_overlap_h0_t<false>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
-                           (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
-        LAUNCH_CHECK();
-        if (has_d_) {
-            hipLaunchKernelGGL(k_overlap_h0_t<true>, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t,
-                               (const int*)bk.atoms, (const double*)d_cn_, opt_.integral_cutoff);
-            LAUNCH_CHECK();
-        }
+        st = launch_integrals(bk);
+        if (!st.ok()) return st;
     }
     hipLaunchKernelGGL(k_pad_diagonal, dim3(count), dim3(64), 0, bk.stream, db_, (const int*)bk.act);
     LAUNCH_CHECK();
@@ -1571,20 +1626,36 @@ Status Batch::launch_bucket_gradient(Bucket& bk) {
                                    true, W, nullptr, nullptr, (const int*)d_ao0_, nullptr, nullptr, naos_));
         }
     }
-    const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+    // the periodic gradient ignores the include flags and always has every
+    // term (pbc/gradient.rs:75-163)
+    const bool el_on = opt_.include_hamiltonian || opt_.include_scc || bk.has_pbc;
     if (opt_.compute_gradient && el_on && bk.natoms > 0) {
         const int cn_on = opt_.enable_cn_hamiltonian;
-        auto pairs = opt_.mixed ? k_grad_pairs_t<false, true> : k_grad_pairs_t<false, false>;
+        auto pairs = opt_.mixed ? k_grad_pairs_t<false, true, false> : k_grad_pairs_t<false, false, false>;
         hipLaunchKernelGGL(pairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
                            (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
                            opt_.integral_cutoff, bk.gpair, bk.dedcn);
         LAUNCH_CHECK();
         if (has_d_) {
-            auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true> : k_grad_pairs_t<true, false>;
+            auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true, false> : k_grad_pairs_t<true, false, false>;
             hipLaunchKernelGGL(dpairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
                                (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
                                opt_.integral_cutoff, bk.gpair, bk.dedcn);
             LAUNCH_CHECK();
+        }
+        if (bk.has_pbc) {
+            auto ppairs = opt_.mixed ? k_grad_pairs_t<false, true, true> : k_grad_pairs_t<false, false, true>;
+            hipLaunchKernelGGL(ppairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
+                               (const double*)d_cn_, (const double*)d_vout_t_, 1, cn_on, kPbcAoCutoff, bk.gpair,
+                               bk.dedcn);
+            LAUNCH_CHECK();
+            if (has_d_) {
+                auto pdpairs = opt_.mixed ? k_grad_pairs_t<true, true, true> : k_grad_pairs_t<true, false, true>;
+                hipLaunchKernelGGL(pdpairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
+                                   (const double*)d_cn_, (const double*)d_vout_t_, 1, cn_on, kPbcAoCutoff,
+                                   bk.gpair, bk.dedcn);
+                LAUNCH_CHECK();
+            }
         }
     }
     return Status{};
@@ -1597,7 +1668,7 @@ Status Batch::gradient_phase() {
     if (!opt_.compute_gradient || natoms_ == 0) return Status{};
     const int na = natoms_;
     HIP_TRY(hipMemsetAsync(d_gel_, 0, 3 * na * sizeof(double), stream_));
-    const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+    const bool el_on = opt_.include_hamiltonian || opt_.include_scc || !h_pinst_.empty();
     const int cn_on = opt_.enable_cn_hamiltonian;
     if (el_on) {
         for (auto& bk : buckets_) {
@@ -1610,6 +1681,9 @@ Status Batch::gradient_phase() {
         hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                            (const double*)d_qout_, cn_on, opt_.include_scc, d_dedcn_h_, d_gel_, na);
         LAUNCH_CHECK();
+        // the Ewald Coulomb gradient of the periodic instances
+        HIP_TRY(launch_ewald_grad(stream_, db_, t, d_patoms_, (int)h_patoms_.size(), (const double*)d_qout_,
+                                  d_gel_));
         if (cn_on) {
             hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                                (const double*)d_dedcn_h_, opt_.coordination_cutoff, 1e-12, d_gel_, na);
@@ -1694,11 +1768,14 @@ Status Batch::download(std::vector<InstanceResult>& out) {
             return v;
         };
         if (opt_.compute_gradient) {
-            const bool el_on = opt_.include_hamiltonian || opt_.include_scc;
+            // the periodic gradient has every term whatever the include flags
+            // (pbc/gradient.rs:75-163, gradient.rs:74-120)
+            const bool all = periodic(i);
+            const bool el_on = opt_.include_hamiltonian || opt_.include_scc || all;
             r.g_el = slice3(gel, el_on);
-            r.g_rep = slice3(grep, opt_.include_repulsion);
-            r.g_disp = slice3(gdisp, opt_.include_dispersion && opt_.enable_dispersion);
-            r.g_hal = slice3(ghal, opt_.include_halogen);
+            r.g_rep = slice3(grep, opt_.include_repulsion || all);
+            r.g_disp = slice3(gdisp, (opt_.include_dispersion || all) && opt_.enable_dispersion);
+            r.g_hal = slice3(ghal, opt_.include_halogen || all);
             r.gradient.assign(3 * nat, 0.0);
             for (int k = 0; k < 3 * nat; ++k) r.gradient[k] = r.g_el[k] + r.g_rep[k] + r.g_disp[k] + r.g_hal[k];
         }

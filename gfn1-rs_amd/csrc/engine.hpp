@@ -16,6 +16,7 @@
 #include "tridiag_large.hpp"
 #include "kernels.hpp"
 #include "model.hpp"
+#include "periodic.hpp"
 
 namespace gfn1 {
 
@@ -42,6 +43,10 @@ struct SystemInput {
     double charge = 0.0;
     const double* q0 = nullptr;  // optional initial shell charges
     int q0_len = 0;
+    // periodic cell: three cell vectors (bohr), one after the other; nullptr
+    // for a molecule.  A periodic instance takes the Gamma-point path
+    // (pbc/mod.rs:83-91).
+    const double* cell = nullptr;
 };
 
 // Host-side work of one run's SCC phase.  kernels counts the kernels the SCC
@@ -228,6 +233,7 @@ class Batch {
         // so buckets that finish at different times never write the same cache line)
         int* atoms = nullptr;
         int natoms = 0;
+        bool has_pbc = false;             // a periodic instance is in the bucket
         double* gpair = nullptr;  // 3 * natoms
         double* dedcn = nullptr;  // natoms
         // Captured iteration (own eigensolver buckets): one graph per state
@@ -258,6 +264,7 @@ class Batch {
     Status finish_iteration(Bucket& bk);
     Status launch_bucket_gradient(Bucket& bk);
     Status launch_bucket_setup(Bucket& bk);
+    Status launch_integrals(Bucket& bk);
     Status upload_active(Bucket& bk, const std::vector<int>& list, int parity);
     Status grow_kcap(Bucket& bk, int kcap);
     Status eigensolve(Bucket& bk, int count);
@@ -306,6 +313,14 @@ class Batch {
     LaunchStats launch_stats_{};
     LargeTridiagWork large_work_;  // scratch of the reduction above kEigMaxDim
     std::vector<TuneRecord> tuning_;
+    // periodic instances: cell records, reciprocal vectors, Ewald shell matrices
+    std::vector<unsigned char> h_pbc_;
+    std::vector<int> h_pinst_, h_patoms_;
+    int max_pnsh_ = 0;
+    CellRec* d_cell_ = nullptr;
+    double *d_gtab_ = nullptr, *d_gam_ = nullptr;
+    int *d_pinst_ = nullptr, *d_patoms_ = nullptr;
+    bool periodic(int inst) const { return !h_pbc_.empty() && h_pbc_[inst]; }
 
     // device arrays
     DeviceBatch db_{};

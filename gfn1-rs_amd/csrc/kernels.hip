@@ -214,6 +214,16 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Lattice images of a periodic instance within a cutoff: every translation
+// T = n0 a + n1 b + n2 c with |n_k| <= ceil(cutoff / h_k) + 1 (lattice.rs:249-261);
+// the callers test the distances themselves.
+__device__ __forceinline__ void image_ranges(const CellRec& c, double cut, int r[3]) {
+    for (int k = 0; k < 3; ++k) r[k] = (int)ceil(cut / c.h[k]) + 1;
+}
+__device__ __forceinline__ void image_vector(const CellRec& c, int n0, int n1, int n2, double t[3]) {
+    for (int k = 0; k < 3; ++k) t[k] = n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k];
+}
+
 // Deterministic block sum (blockDim.x a multiple of 64, <= 1024); result on every thread.
 __device__ double block_sum(double v, double* lds) {
     v = wave_sum(v);
@@ -242,17 +252,45 @@ __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int
     const double rca = t.elements[b.atom_z[a]].covalent_radius;
     double c = 0.0, cd = 0.0;
     const double cc2 = cn_cut * cn_cut, cd2 = 25.0 * 25.0;
-    for (int o = a0; o < a1; ++o) {
-        if (o == a) continue;
-        const double r2 = dist2(x, a, o);
-        if (r2 <= kPairEps2) continue;
-        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
-        if (rc <= 1e-12) continue;
-        const double r = sqrt(r2);
-        double v, dv;
-        if (r2 <= cc2 || r2 <= cd2) count_fn(r, rc, v, dv);
-        if (r2 <= cc2) c += v;
-        if (r2 <= cd2) cd += v;
+    if (periodic_instance(b, inst)) {
+        // lattice-summed CN: every (atom, image) other than A itself in the
+        // origin cell, so a pair with its own images counts from both sides
+        // (coordination.rs:64-89 over pairlist.rs:244-317)
+        const CellRec& cr = b.cell[inst];
+        int rg[3];
+        image_ranges(cr, fmax(cn_cut, 25.0), rg);
+        for (int o = a0; o < a1; ++o) {
+            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+            if (rc <= 1e-12) continue;
+            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                        if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
+                        double tv[3];
+                        image_vector(cr, n0, n1, n2, tv);
+                        const double dx = x[3 * o] + tv[0] - x[3 * a], dy = x[3 * o + 1] + tv[1] - x[3 * a + 1],
+                                     dz = x[3 * o + 2] + tv[2] - x[3 * a + 2];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= kPairEps2 || (r2 > cc2 && r2 > cd2)) continue;
+                        double v, dv;
+                        count_fn(sqrt(r2), rc, v, dv);
+                        if (r2 <= cc2) c += v;
+                        if (r2 <= cd2) cd += v;
+                    }
+        }
+    } else {
+        for (int o = a0; o < a1; ++o) {
+            if (o == a) continue;
+            const double r2 = dist2(x, a, o);
+            if (r2 <= kPairEps2) continue;
+            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+            if (rc <= 1e-12) continue;
+            const double r = sqrt(r2);
+            double v, dv;
+            if (r2 <= cc2 || r2 <= cd2) count_fn(r, rc, v, dv);
+            if (r2 <= cc2) c += v;
+            if (r2 <= cd2) cd += v;
+        }
     }
     cn[a] = cn_on ? c : 0.0;
     cn_d3[a] = cd;
@@ -341,7 +379,8 @@ __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act,
     if (inst < 0) return;  // gated off in a captured iteration
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const double* x = b.atom_x;
-    const bool staged = nsh <= kShellLds;  // uniform over the work-group
+    const bool pbc = periodic_instance(b, inst);
+    const bool staged = !pbc && nsh <= kShellLds;  // uniform over the work-group
     if (staged) {
         for (int j = threadIdx.x; j < nsh; j += blockDim.x) {
             const int sj = s0 + j, aj = b.sh_atom[sj];
@@ -360,7 +399,11 @@ __global__ void k_shell_potential(DeviceBatch b, DeviceTables t, const int* act,
     const int ai = b.sh_atom[si];
     const double hi = t.shells[b.sh_type[si]].hardness;
     double v = 0.0;
-    if (staged) {
+    if (pbc) {
+        // the Ewald-summed matrix of the periodic instance (pbc/ewald.rs:253-320)
+        const double* g = b.gam + b.cell[inst].gam0 + (size_t)i * nsh;
+        for (int j = 0; j < nsh; ++j) v += g[j] * q[s0 + j];
+    } else if (staged) {
         // the same operations as the global-memory loop below, operands from LDS
         const double ihi = 1.0 / hi;
         const double xi = x[3 * ai], yi = x[3 * ai + 1], zi = x[3 * ai + 2];
@@ -833,8 +876,12 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
         return;
     }
     // ---- next input charges
-    const LadderRung rung = {prm.rung_broyden[s_rung], prm.rung_mixing[s_rung], prm.rung_level_shift[s_rung],
-                             prm.rung_temperature[s_rung], prm.rung_max_iter[s_rung]};
+    // a periodic instance runs the caller's scheme once, without level shift
+    // and without the fallback ladder (pbc/scf.rs:358-529)
+    const bool pbc = periodic_instance(b, inst);
+    const LadderRung rung = {prm.rung_broyden[s_rung], prm.rung_mixing[s_rung],
+                             pbc ? 0.0 : prm.rung_level_shift[s_rung], prm.rung_temperature[s_rung],
+                             prm.rung_max_iter[s_rung]};
     const double mix = rung.mixing;
     double* qi = qin + s0;
     const double* qo = qout + s0;
@@ -952,7 +999,7 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
     // rung exhausted -> next rung or failure (electronic.rs:1413-1446, 1950-1955)
     if (s_iter + 1 >= rung.max_iter) {
         __syncthreads();
-        if (s_rung >= 3) {
+        if (s_rung >= 3 || pbc) {
             if (threadIdx.x == 0) s.status = 2;
             return;
         }
@@ -993,7 +1040,75 @@ __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw
     double er = 0.0, ed_sum = 0.0, gr[3] = {0, 0, 0}, gd[3] = {0, 0, 0}, dcn = 0.0;
     const double* gwa = gw + (size_t)a * kMaxRef;
     const double* dga = gw + (size_t)natoms * kMaxRef + (size_t)a * kMaxRef;  // dgw follows gw
-    for (int o = a0; o < a1; ++o) {
+    if (periodic_instance(b, inst)) {
+        // Image sums (repulsion.rs:39-59, dispersion.rs:1258-1295 over
+        // pairlist.rs:244-317).  Every (atom, image) pair is seen from both
+        // atoms, so each adds half its energy; a pair of an atom with its own
+        // image moves nothing and adds no force, but its D3 term enters dE/dCN.
+        const CellRec& cr = b.cell[inst];
+        int rg[3];
+        image_ranges(cr, disp_on ? 50.0 : 25.0, rg);
+        for (int o = a0; o < a1; ++o) {
+            const int zb = b.atom_z[o];
+            const ElementType& EB = t.elements[zb];
+            double c6 = 0.0, dc6 = 0.0;
+            if (disp_on) {
+                const double* gwb = gw + (size_t)o * kMaxRef;
+                int hi, lo, ia_is_hi;
+                if (za > zb) { hi = za; lo = zb; ia_is_hi = 1; }
+                else { hi = zb; lo = za; ia_is_hi = 0; }
+                const size_t pair = (size_t)lo + (size_t)hi * (hi - 1) / 2 - 1;
+                const double* tab = t.d3_c6 + pair * kMaxRef * kMaxRef;
+                for (int ra = 0; ra < EA.d3_nref; ++ra)
+                    for (int rb = 0; rb < EB.d3_nref; ++rb) {
+                        double ref;
+                        if (za == zb) ref = tab[ra * kMaxRef + rb];
+                        else ref = ia_is_hi ? tab[rb * kMaxRef + ra] : tab[ra * kMaxRef + rb];
+                        c6 += gwa[ra] * gwb[rb] * ref;
+                        dc6 += dga[ra] * gwb[rb] * ref;
+                    }
+            }
+            const double alpha = sqrt(EA.repa * EB.repa);
+            const double zeff = EA.repb * EB.repb;
+            const double kx = (za <= 2 && zb <= 2) ? g.kexplight : g.kexp;
+            const double r4r2 = 3.0 * EA.d3_r4r2 * EB.d3_r4r2;
+            const double r0 = g.a1 * sqrt(r4r2) + g.a2;
+            const double r0_2 = r0 * r0, r0_6 = r0_2 * r0_2 * r0_2, r0_8 = r0_6 * r0_2;
+            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                        if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
+                        double tv[3];
+                        image_vector(cr, n0, n1, n2, tv);
+                        const double dr[3] = {x[3 * o] + tv[0] - x[3 * a], x[3 * o + 1] + tv[1] - x[3 * a + 1],
+                                              x[3 * o + 2] + tv[2] - x[3 * a + 2]};
+                        const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                        if (r2 <= kPairEps2) continue;
+                        const double r = sqrt(r2);
+                        if (r2 <= 25.0 * 25.0) {
+                            const double rk = pow(r, kx);
+                            const double e = zeff * exp(-alpha * rk) / r;
+                            const double pref = (alpha * rk * kx + 1.0) * e / (r * r);
+                            er += 0.5 * e;
+                            if (o != a)
+                                for (int d = 0; d < 3; ++d) gr[d] += dr[d] * pref;
+                        }
+                        if (disp_on && r2 <= 50.0 * 50.0 && r2 > 1e-14) {
+                            const double r4 = r2 * r2;
+                            const double t6 = 1.0 / (r4 * r2 + r0_6), t8 = 1.0 / (r4 * r4 + r0_8);
+                            const double d6 = -6.0 * r4 * t6 * t6, d8 = -8.0 * r4 * r2 * t8 * t8;
+                            const double ed = g.s6 * t6 + g.s8 * r4r2 * t8;
+                            const double gdv = g.s6 * d6 + g.s8 * r4r2 * d8;
+                            ed_sum -= 0.5 * c6 * ed;
+                            if (o != a)
+                                for (int d = 0; d < 3; ++d) gd[d] += (-dr[d]) * (-c6 * gdv);
+                            dcn -= dc6 * ed;
+                        }
+                    }
+        }
+    }
+    const int o_end = periodic_instance(b, inst) ? a0 : a1;  // molecular pairs
+    for (int o = a0; o < o_end; ++o) {
         if (o == a) continue;
         const double r2 = dist2(x, a, o);
         if (r2 <= kPairEps2) continue;
@@ -1064,6 +1179,37 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
     const double rca = t.elements[b.atom_z[a]].covalent_radius;
     const double c2 = cut * cut;
     double acc[3] = {0, 0, 0};
+    if (periodic_instance(b, inst)) {
+        // every image of every other atom; an atom's own images are skipped
+        // (gradient.rs:477-479 over the periodic pair list)
+        const CellRec& cr = b.cell[inst];
+        int rg[3];
+        image_ranges(cr, cut, rg);
+        for (int o = a0; o < a1; ++o) {
+            if (o == a) continue;
+            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+            if (rc <= 1e-12) continue;
+            const double dedcn_ao = dedcn[a] + dedcn[o];
+            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                        double tv[3];
+                        image_vector(cr, n0, n1, n2, tv);
+                        const double rv[3] = {x[3 * a] - x[3 * o] - tv[0], x[3 * a + 1] - x[3 * o + 1] - tv[1],
+                                              x[3 * a + 2] - x[3 * o + 2] - tv[2]};
+                        const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                        if (r2 <= kPairEps2 || r2 > c2) continue;
+                        const double r = sqrt(r2);
+                        if (r <= reps) continue;
+                        double v, dv;
+                        count_fn(r, rc, v, dv);
+                        const double pref = dedcn_ao * dv / r;
+                        for (int d = 0; d < 3; ++d) acc[d] += rv[d] * pref;
+                    }
+        }
+        for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
+        return;
+    }
     for (int o = a0; o < a1; ++o) {
         if (o == a) continue;
         const double r2 = dist2(x, a, o);
@@ -1080,10 +1226,105 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
     for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
 }
 
+// Halogen bond of a periodic instance (halogen.rs:64-160 over the triples of
+// :716-740): the donor's neighbours are every (atom, image) within 20 bohr but
+// the donor itself in the origin cell; the nearest one that is not an image of
+// the donor is its bonded neighbour; acceptors may be images.  Forces go to the
+// atoms the images belong to.
+__device__ void halogen_periodic(const DeviceBatch& b, const DeviceTables& t, int inst, double* e_hal,
+                                 double* g_hal) {
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const double damp = t.glob.xbdamp, rads = t.glob.xbrad;
+    const double c2 = 20.0 * 20.0, EPS = 1e-18, ALP = 6.0, LJ2 = 6.0;
+    const CellRec& cr = b.cell[inst];
+    int rg[3];
+    image_ranges(cr, 20.0, rg);
+    double e = 0.0;
+    for (int X = a0; X < a1; ++X) {
+        const int zx = b.atom_z[X];
+        if (!(zx == 17 || zx == 35 || zx == 53 || zx == 85)) continue;
+        int K = -1;
+        double best = INFINITY, tk[3] = {0, 0, 0};
+        for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+            for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                    double tv[3];
+                    image_vector(cr, n0, n1, n2, tv);
+                    for (int j = a0; j < a1; ++j) {
+                        if (j == X) continue;
+                        double d[3];
+                        for (int q = 0; q < 3; ++q) d[q] = x[3 * j + q] + tv[q] - x[3 * X + q];
+                        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                        if (r2 <= kPairEps2 || r2 > c2) continue;
+                        if (r2 < best) {
+                            best = r2;
+                            K = j;
+                            for (int q = 0; q < 3; ++q) tk[q] = tv[q];
+                        }
+                    }
+                }
+        if (K < 0) continue;
+        const double cc = t.elements[zx].xb_strength;
+        if (cc == 0.0) continue;
+        for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+            for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                    double tv[3];
+                    image_vector(cr, n0, n1, n2, tv);
+                    for (int j = a0; j < a1; ++j) {
+                        const int zj = b.atom_z[j];
+                        if (!(zj == 7 || zj == 8 || zj == 15 || zj == 16)) continue;
+                        double dxj[3], dxk[3], dkj[3];
+                        for (int d = 0; d < 3; ++d) {
+                            dxj[d] = x[3 * j + d] + tv[d] - x[3 * X + d];
+                            dxk[d] = x[3 * K + d] + tk[d] - x[3 * X + d];
+                            dkj[d] = x[3 * j + d] + tv[d] - x[3 * K + d] - tk[d];
+                        }
+                        const double d2jx = dxj[0] * dxj[0] + dxj[1] * dxj[1] + dxj[2] * dxj[2];
+                        if (d2jx <= kPairEps2 || d2jx > c2) continue;
+                        const double d2kx = dxk[0] * dxk[0] + dxk[1] * dxk[1] + dxk[2] * dxk[2];
+                        const double d2jk = dkj[0] * dkj[0] + dkj[1] * dkj[1] + dkj[2] * dkj[2];
+                        if (d2jx <= EPS || d2kx <= EPS) continue;
+                        const double r0 = rads * (t.elements[zx].atomic_radius + t.elements[zj].atomic_radius);
+                        const double rjx = sqrt(d2jx) + EPS, rkx = sqrt(d2kx) + EPS;
+                        const double xy = sqrt(d2kx * d2jx);
+                        if (xy <= EPS) continue;
+                        const double term = (d2kx + d2jx - d2jk) / xy;
+                        const double ab = 0.5 - 0.25 * term;
+                        const double aterm = pow(ab, ALP);
+                        const double t13 = r0 / rjx;
+                        const double t14lj = pow(t13, 12.0);
+                        e += aterm * cc * (t14lj - damp * pow(t13, LJ2)) / (1.0 + t14lj);
+                        const double t14 = pow(r0 / rjx, LJ2);
+                        const double num = t14 * t14 - damp * t14, den = 1.0 + t14 * t14;
+                        const double tlj = num / den;
+                        double dt = 2.0 * LJ2 * num * t14 * t14 / (rjx * den * den);
+                        dt += LJ2 * t14 * (damp - 2.0 * t14) / (rjx * den);
+                        dt *= aterm * cc / rjx;
+                        const double pf = -0.25 * ALP * pow(ab, ALP - 1.0) * cc * tlj;
+                        const double djx = (2.0 / rkx - term / rjx) * pf / rjx;
+                        const double dkx = (2.0 / rjx - term / rkx) * pf / rkx;
+                        const double djk = 2.0 * pf / xy;
+                        for (int d = 0; d < 3; ++d) {
+                            g_hal[3 * j + d] += dxj[d] * dt + dxj[d] * djx - dkj[d] * djk;
+                            g_hal[3 * X + d] -= dxj[d] * dt + dxj[d] * djx + dxk[d] * dkx;
+                            g_hal[3 * K + d] += dxk[d] * dkx + dkj[d] * djk;
+                        }
+                    }
+                }
+    }
+    e_hal[inst] = e;
+}
+
 // Halogen-bond correction (halogen.rs:64-160, :716-784); one thread per instance.
 __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal) {
     const int inst = blockIdx.x * blockDim.x + threadIdx.x;
     if (inst >= b.ninst) return;
+    if (periodic_instance(b, inst)) {
+        halogen_periodic(b, t, inst, e_hal, g_hal);
+        return;
+    }
     const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
     const double* x = b.atom_x;
     const double damp = t.glob.xbdamp, rads = t.glob.xbrad;
@@ -1184,7 +1425,8 @@ __global__ void k_grad_atoms(DeviceBatch b, DeviceTables t, const double* q, int
         }
         dedcn_h[a] = d;
     }
-    if (!scc_on) return;
+    // periodic instances: the Ewald gradient kernel (periodic.hip) instead
+    if (!scc_on || periodic_instance(b, inst)) return;
     const int s0 = b.sh0[inst], s1 = s0 + b.nsh[inst];
     const double* x = b.atom_x;
     double acc[3] = {0, 0, 0};
@@ -1346,6 +1588,66 @@ __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTabl
         }
 }
 
+// S(Gamma) and H0(Gamma) for one (shell of A, shell of B) pair of a periodic
+// instance (pbc/bloch.rs:58-236): the blocks of every image of B within 30
+// bohr summed, the on-site formula for B = A in the origin cell, the off-site
+// one at the image distance otherwise; writes both triangles.
+template <int LA, int LB>
+__device__ __forceinline__ void s_h0_pair_pbc(const DeviceBatch& b, const DeviceTables& t, const double* cn,
+                                              const CellRec& cr, const int* rg, int a, int sa, int sb, int ld,
+                                              gptr<double> S, gptr<double> H) {
+    const int ob = b.sh_atom[sb];
+    const ShellType& A = t.shells[b.sh_type[sa]];
+    const ShellType& B = t.shells[b.sh_type[sb]];
+    const double* x = b.atom_x;
+    const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+    double bs[NA * NB], bh[NA * NB];
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) bs[k] = bh[k] = 0.0;
+    const int za = b.atom_z[a], zb = b.atom_z[ob];
+    const double sea = A.hdiag - A.kcn * cn[a];
+    const double seb = B.hdiag - B.kcn * cn[ob];
+    const double hsc = hscale(t, za, A, zb, B);
+    const double radsum = t.elements[za].atomic_radius + t.elements[zb].atomic_radius;
+    const double c2 = kPbcAoCutoff * kPbcAoCutoff;
+    for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
+                double tv[3];
+                image_vector(cr, n0, n1, n2, tv);
+                const double rb[3] = {x[3 * ob] + tv[0], x[3 * ob + 1] + tv[1], x[3 * ob + 2] + tv[2]};
+                const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (!same && (r2 <= kPairEps2 || r2 > c2)) continue;
+                double blk[NA * NB];
+                shell_block<LA, LB, false>(A, B, ra, rb, blk, nullptr);
+                double h = 0.5 * (sea + seb);
+                if (!same) {
+                    const double rr = sqrt(sqrt(r2) / radsum);
+                    h *= hsc * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
+                }
+#pragma unroll
+                for (int k = 0; k < NA * NB; ++k) {
+                    bs[k] += blk[k];
+                    bh[k] += blk[k] * h;
+                }
+            }
+    const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
+#pragma unroll
+    for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const size_t e1 = (size_t)(mu0 + i) * ld + (nu0 + j);
+            const size_t e2 = (size_t)(nu0 + j) * ld + (mu0 + i);
+            S[e1] = bs[i * NB + j];
+            S[e2] = bs[i * NB + j];
+            H[e1] = bh[i * NB + j];
+            H[e2] = bh[i * NB + j];
+        }
+}
+
 // Electronic gradient contribution of one (shell of A, shell of B) pair to atom A
 // (gradient.rs:358-455): Pulay/shell-potential term, polynomial term, dE/dCN_A.
 // The AO-pair weights oc = P (2 h - v_mu - v_nu) - 2 W and P are applied inside
@@ -1371,16 +1673,22 @@ struct GradAcc {
     __device__ __forceinline__ double dcn() const { return (double)dc - (double)cdc; }
 };
 
-template <int LA, int LB, typename R>
+// PBC: B sits at its image x_B + tv (P(T) = P at the Gamma point); for an
+// image of A itself (self) only dE/dCN_A is accumulated: the pair's distance
+// does not move with A.
+template <int LA, int LB, typename R, bool PBC = false>
 __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn,
                                           const double* vsh, int scc_on, int cn_on, int a, int sa, int sb, int ld,
-                                          gptr<const double> P, gptr<const double> W, GradAcc<R>& acc) {
+                                          gptr<const double> P, gptr<const double> W, GradAcc<R>& acc,
+                                          const double* tv = nullptr, bool self = false) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
     const double* x = b.atom_x;
     const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
-    const double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
+    double rb[3] = {x[3 * ob], x[3 * ob + 1], x[3 * ob + 2]};
+    if constexpr (PBC)
+        for (int k = 0; k < 3; ++k) rb[k] += tv[k];
     constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
     const int za = b.atom_z[a], zb = b.atom_z[ob];
     const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
@@ -1553,10 +1861,37 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
         dlp2 = dz * (dp / pl);
     }
     const double hp2 = 2.0 * hij * (double)sp;
-    GradAcc<R>::add(acc.g[0], acc.cg[0], (R)((double)g0 + dlp0 * hp2));
-    GradAcc<R>::add(acc.g[1], acc.cg[1], (R)((double)g1 + dlp1 * hp2));
-    GradAcc<R>::add(acc.g[2], acc.cg[2], (R)((double)g2 + dlp2 * hp2));
+    if (!PBC || !self) {
+        GradAcc<R>::add(acc.g[0], acc.cg[0], (R)((double)g0 + dlp0 * hp2));
+        GradAcc<R>::add(acc.g[1], acc.cg[1], (R)((double)g1 + dlp1 * hp2));
+        GradAcc<R>::add(acc.g[2], acc.cg[2], (R)((double)g2 + dlp2 * hp2));
+    }
     if (cn_on) GradAcc<R>::add(acc.dc, acc.cdc, (R)(dse_a * hs * (double)sp));
+}
+
+// Every image of shell sb's atom within the AO cutoff (30 bohr), as the bra
+// side of atom a (pbc/gradient.rs:285-384 at the Gamma point).
+template <int LA, int LB, typename R>
+__device__ __forceinline__ void grad_pair_images(const DeviceBatch& b, const DeviceTables& t, const double* cn,
+                                                 const double* vsh, int scc_on, int cn_on, int a, int sa, int sb,
+                                                 int ld, gptr<const double> P, gptr<const double> W,
+                                                 const CellRec& cr, const int* rg, GradAcc<R>& acc) {
+    const int ob = b.sh_atom[sb];
+    const double* x = b.atom_x;
+    const double c2 = kPbcAoCutoff * kPbcAoCutoff;
+    for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
+                if (ob == a && origin) continue;
+                double tv[3];
+                image_vector(cr, n0, n1, n2, tv);
+                const double dx = x[3 * a] - x[3 * ob] - tv[0], dy = x[3 * a + 1] - x[3 * ob + 1] - tv[1],
+                             dz = x[3 * a + 2] - x[3 * ob + 2] - tv[2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= 1e-12 || r2 > c2) continue;
+                grad_pair<LA, LB, R, true>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc, tv, ob == a);
+            }
 }
 
 }  // namespace
@@ -1565,12 +1900,16 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
 // work-group per listed atom A = atoms[blockIdx.x]; for every shell of A and every angular-momentum class
 // the lanes walk the instance's shells of that class on atoms B <= A, so a
 // wavefront always evaluates one (LA, LB) specialisation.
-template <bool DCLASS>
+template <bool DCLASS, bool PBC>
 __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn,
                                                      double cutoff) {
     const int a = atoms[blockIdx.x];
     const int inst = b.atom_inst[a];
+    // molecular and periodic instances go to separate instantiations
+    if (periodic_instance(b, inst) != PBC) return;
     const int ld = b.ld[inst];
+    int rg[3] = {0, 0, 0};
+    if constexpr (PBC) image_ranges(b.cell[inst], kPbcAoCutoff, rg);
     gptr<double> S = (gptr<double>)b.S[inst];
     gptr<double> H = (gptr<double>)b.H0[inst];
     const double c2 = cutoff * cutoff;
@@ -1587,7 +1926,24 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
                 const int sb = b.sh_by_l[idx];
                 const int ob = b.sh_atom[sb];
                 if (ob > a || (ob == a && sb > sa)) continue;
-                if constexpr (!DCLASS) {
+                if constexpr (PBC && !DCLASS) {
+                    const CellRec& cr = b.cell[inst];
+                    switch (la * 3 + lb) {
+                        case 0: s_h0_pair_pbc<0, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        case 1: s_h0_pair_pbc<0, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        case 3: s_h0_pair_pbc<1, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        default: s_h0_pair_pbc<1, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                    }
+                } else if constexpr (PBC) {
+                    const CellRec& cr = b.cell[inst];
+                    switch (la * 3 + lb) {
+                        case 2: s_h0_pair_pbc<0, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        case 5: s_h0_pair_pbc<1, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        case 6: s_h0_pair_pbc<2, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        case 7: s_h0_pair_pbc<2, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                        default: s_h0_pair_pbc<2, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
+                    }
+                } else if constexpr (!DCLASS) {
                     switch (la * 3 + lb) {
                         case 0: s_h0_pair<0, 0>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
                         case 1: s_h0_pair<0, 1>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
@@ -1611,14 +1967,23 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
 // Electronic gradient, pair part (gradient.rs:333-462) for atom A = atoms[j]:
 // every pair with B != A inside the integral cutoff, A taken as the bra.
 // Deterministic per-atom sums (one 64-lane wavefront per atom, no atomics).
-template <bool DCLASS, bool MIXED>
+// PBC: the instantiation for periodic instances (the others return at once);
+// every image of every atom within 30 bohr, the shell potential always on
+// (pbc/gradient.rs:250-384).
+template <bool DCLASS, bool MIXED, bool PBC>
 __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms,
                                                      const double* cn, const double* vsh, int scc_on, int cn_on,
                                                      double cutoff, double* g_pair, double* dedcn_h) {
     const int j = blockIdx.x;
     const int a = atoms[j];
     const int inst = b.atom_inst[a];
+    if (periodic_instance(b, inst) != PBC) return;
     const int ld = b.ld[inst];
+    int rg[3] = {0, 0, 0};
+    if constexpr (PBC) {
+        image_ranges(b.cell[inst], kPbcAoCutoff, rg);
+        scc_on = 1;
+    }
     gptr<const double> P = (gptr<const double>)b.P[inst];
     gptr<const double> W = (gptr<const double>)b.W[inst];
     const double c2 = cutoff * cutoff;
@@ -1634,6 +1999,26 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
             if ((la == 2 || lb == 2) != DCLASS) continue;
             for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
                 const int sb = b.sh_by_l[idx];
+                if constexpr (PBC) {
+                    const CellRec& cr = b.cell[inst];
+                    if constexpr (!DCLASS) {
+                        switch (la * 3 + lb) {
+                            case 0: grad_pair_images<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 1: grad_pair_images<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 3: grad_pair_images<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            default: grad_pair_images<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                        }
+                    } else {
+                        switch (la * 3 + lb) {
+                            case 2: grad_pair_images<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 5: grad_pair_images<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 6: grad_pair_images<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 7: grad_pair_images<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            default: grad_pair_images<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                        }
+                    }
+                    continue;
+                }
                 const int ob = b.sh_atom[sb];
                 if (ob == a) continue;
                 const double r2 = dist2(b.atom_x, a, ob);
@@ -1697,15 +2082,25 @@ __global__ void k_copy_words(const unsigned int* src, unsigned int* dst, int n) 
     if (i < n) dst[i] = src[i];
 }
 
-template __global__ void k_overlap_h0_t<false>(DeviceBatch, DeviceTables, const int*, const double*, double);
-template __global__ void k_overlap_h0_t<true>(DeviceBatch, DeviceTables, const int*, const double*, double);
-template __global__ void k_grad_pairs_t<false, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                const double*, int, int, double, double*, double*);
-template __global__ void k_grad_pairs_t<true, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                const double*, int, int, double, double*, double*);
-template __global__ void k_grad_pairs_t<false, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                const double*, int, int, double, double*, double*);
-template __global__ void k_grad_pairs_t<true, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                const double*, int, int, double, double*, double*);
+template __global__ void k_overlap_h0_t<false, false>(DeviceBatch, DeviceTables, const int*, const double*, double);
+template __global__ void k_overlap_h0_t<true, false>(DeviceBatch, DeviceTables, const int*, const double*, double);
+template __global__ void k_overlap_h0_t<false, true>(DeviceBatch, DeviceTables, const int*, const double*, double);
+template __global__ void k_overlap_h0_t<true, true>(DeviceBatch, DeviceTables, const int*, const double*, double);
+template __global__ void k_grad_pairs_t<false, false, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<false, false, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<false, true, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<false, true, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, false, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, false, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, true, false>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
+template __global__ void k_grad_pairs_t<true, true, true>(DeviceBatch, DeviceTables, const int*, const double*,
+                                                         const double*, int, int, double, double*, double*);
 
 }  // namespace gfn1

@@ -24,7 +24,9 @@ struct SccParams {
 
 __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
                                double* gw, double* dgw, int natoms);
-template <bool DCLASS>
+// S and H0 of the listed atoms' instances; PBC selects the periodic instances
+// (the Gamma-point image sums), the other instantiation the molecular ones.
+template <bool DCLASS, bool PBC>
 __global__ void k_overlap_h0_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn, double cutoff);
 // Padding diagonal of the listed instances (act[blockIdx.x]).
 __global__ void k_pad_diagonal(DeviceBatch b, const int* act);
@@ -59,7 +61,7 @@ __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* 
 __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
 // Pair part of the electronic gradient for the listed atoms: g_pair[3 j + d] and
 // dedcn_h[j] belong to atom atoms[j].
-template <bool DCLASS, bool MIXED>
+template <bool DCLASS, bool MIXED, bool PBC>
 __global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn, const double* vsh,
                                int scc_on, int cn_on, double cutoff, double* g_pair, double* dedcn_h);
 // Per-atom scatter of listed sums: g[3 atoms[j] + d] = gl[3 j + d], dc[atoms[j]] = dcl[j].

@@ -1,0 +1,376 @@
+// Periodic Gamma-point path: the Ewald-summed shell Coulomb matrix and its
+// nuclear gradient (reference: src/pbc/ewald.rs, src/pbc/gradient.rs:548-779),
+// and the host-side cell records (src/lattice.rs).
+//
+// The Klopman-Ohno interaction gamma(R) = 1/sqrt(R^2 + eta^-2) is split as
+//   gamma = [1/R - eta^-2 / (2 R^3)] + [gamma - 1/R + eta^-2 / (2 R^3)]:
+// the 1/R part by a standard Ewald sum over atom centres, the R^-3 part by the
+// generalised Ewald sum (real space with erfc, reciprocal space with the
+// exponential integral E1, the k = 0 logarithm and the self term), and the
+// remainder, which decays as R^-5, directly in real space within 10 bohr.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "periodic.hpp"
+
+namespace gfn1 {
+
+namespace {
+
+constexpr double kPiP = 3.14159265358979323846;
+constexpr double kSqrtPi = 1.7724538509055160273;
+constexpr double kR3Coeff = -0.5;  // binomial coefficient of the eta^-2 R^-3 term (pbc/ewald.rs:38)
+
+__device__ __forceinline__ double sum3(double x, double y, double z) { return x * x + y * y + z * z; }
+
+// Real-space part of the eta^-2 R^-3 Ewald term and its radial derivative
+// (pbc/ewald.rs:121-140), without the eta^-2 factor.
+__device__ __forceinline__ void r3_real(double r, double alpha, double& v, double& dv) {
+    const double ar = alpha * r, ex = exp(-ar * ar);
+    const double q = erfc(ar) + (2.0 * ar / kSqrtPi) * ex;
+    const double dq = -(4.0 * alpha * alpha * alpha * r * r / kSqrtPi) * ex;
+    const double r3 = r * r * r;
+    v = q / r3;
+    dv = dq / r3 - 3.0 * q / (r3 * r);
+}
+
+// Short-range remainder gamma - 1/R + eta^-2 / (2 R^3) and its radial
+// derivative (pbc/ewald.rs:108-119).
+__device__ __forceinline__ void ko_short(double r, double ie2, double& v, double& dv) {
+    const double den2 = r * r + ie2, den = sqrt(den2);
+    const double r2 = r * r, r3 = r2 * r;
+    v = 1.0 / den - 1.0 / r + 0.5 * ie2 / r3;
+    dv = -r / (den2 * den) + 1.0 / r2 - 1.5 * ie2 / (r2 * r2);
+}
+
+__device__ __forceinline__ void image_ranges(const CellRec& c, double cut, int r[3]) {
+    for (int k = 0; k < 3; ++k) r[k] = (int)ceil(cut / c.h[k]) + 1;
+}
+
+__device__ double block_sum_p(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += lds[w];
+    __syncthreads();
+    return s;
+}
+
+}  // namespace
+
+// Gamma_ij of every periodic instance (pbc/ewald.rs:200-320): one work-group per
+// (row i, instance pinst[blockIdx.y]); its threads take the columns j <= i and
+// write Gamma_ij and Gamma_ji (the matrix is exactly symmetric).
+__global__ void __launch_bounds__(256) k_ewald_gamma(DeviceBatch b, DeviceTables t, const int* pinst) {
+    const int inst = pinst[blockIdx.y];
+    const int nsh = b.nsh[inst];
+    const int i = blockIdx.x;
+    if (i >= nsh) return;
+    const CellRec& c = b.cell[inst];
+    const double alpha = c.alpha;
+    const double rcut = kEwaldTau / alpha;
+    const double realcut = fmax(rcut, kEwaldShortCutoff);
+    int rg[3];
+    image_ranges(c, realcut, rg);
+    const int s0 = b.sh0[inst];
+    const int si = s0 + i, ai = b.sh_atom[si];
+    const double* x = b.atom_x;
+    const double hi = t.shells[b.sh_type[si]].hardness;
+    const double* gt = b.gtab + c.g0 * kGStride;
+    double* gam = const_cast<double*>(b.gam) + c.gam0;
+    for (int j = threadIdx.x; j <= i; j += blockDim.x) {
+        const int sj = s0 + j, aj = b.sh_atom[sj];
+        const double hj = t.shells[b.sh_type[sj]].hardness;
+        const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+        const double ie2 = 1.0 / (eta * eta);
+        const bool same = ai == aj;
+        const double rab[3] = {x[3 * ai] - x[3 * aj], x[3 * ai + 1] - x[3 * aj + 1], x[3 * ai + 2] - x[3 * aj + 2]};
+        // real space: the 1/R Ewald part (erfc), the R^-3 part and the remainder
+        double real1 = 0.0, real3 = 0.0, sr = 0.0;
+        for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+            for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+                for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                    const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
+                    double d[3];
+                    for (int k = 0; k < 3; ++k) d[k] = rab[k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
+                    const double r = sqrt(sum3(d[0], d[1], d[2]));
+                    if (r <= 1e-12) continue;
+                    if (r <= rcut) {
+                        real1 += erfc(alpha * r) / r;
+                        if (!(same && origin)) {
+                            double v, dv;
+                            r3_real(r, alpha, v, dv);
+                            real3 += v;
+                        }
+                    }
+                    if (r <= kEwaldShortCutoff && !(same && origin)) {
+                        double v, dv;
+                        ko_short(r, ie2, v, dv);
+                        sr += v;
+                    }
+                }
+        // reciprocal space
+        double rec1 = 0.0, rec3 = 0.0;
+        for (int g = 0; g < c.ng; ++g) {
+            const double* e = gt + (size_t)g * kGStride;
+            const double cs = cos(e[0] * rab[0] + e[1] * rab[1] + e[2] * rab[2]);
+            rec1 += e[3] * cs;
+            rec3 += e[4] * cs;
+        }
+        double v = real1 + rec1 + c.ew_bg;
+        if (same) v += c.ew_self + eta;
+        v += kR3Coeff * ie2 * real3 + ie2 * rec3 + c.r3_k0 * ie2;
+        if (same) v += c.r3_self * ie2;
+        v += sr;
+        gam[(size_t)i * nsh + j] = v;
+        gam[(size_t)j * nsh + i] = v;
+    }
+}
+
+// Gradient of 1/2 q^T Gamma q at the converged shell charges for every atom of
+// a periodic instance (pbc/gradient.rs:548-603, :679-779), added to g_el.  One
+// 256-thread work-group per listed atom; fixed thread assignment and a fixed
+// reduction order keep the result deterministic.
+__global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables t, const int* patoms,
+                                                    const double* q, double* g_el) {
+    __shared__ double lds[8];
+    const int a = patoms[blockIdx.x];
+    const int inst = b.atom_inst[a];
+    const CellRec& c = b.cell[inst];
+    const double alpha = c.alpha;
+    const double rcut = kEwaldTau / alpha, realcut = fmax(rcut, kEwaldShortCutoff);
+    const double* x = b.atom_x;
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
+    const int fa = b.atom_sh0[a], nsa = t.elements[b.atom_z[a]].nshell;
+    auto qatom = [&](int o) {
+        const int f = b.atom_sh0[o], ns = t.elements[b.atom_z[o]].nshell;
+        double s = 0.0;
+        for (int k = 0; k < ns; ++k) s += q[f + k];
+        return s;
+    };
+    const double qa = qatom(a);
+    int rg[3];
+    image_ranges(c, realcut, rg);
+    const int nimg = (2 * rg[0] + 1) * (2 * rg[1] + 1) * (2 * rg[2] + 1);
+    double acc[3] = {0.0, 0.0, 0.0};
+    // (i) real-space 1/R Ewald force with atomic charges, and (iv) the real-space
+    // R^-3 and remainder terms per shell pair; work item = (atom or shell, image)
+    const int nat = a1 - a0;
+    const double tas = 2.0 * alpha / kSqrtPi;
+    for (int w = threadIdx.x; w < (nat + nsh) * nimg; w += blockDim.x) {
+        const int e = w / nimg, im = w % nimg;
+        const int n0 = im / ((2 * rg[1] + 1) * (2 * rg[2] + 1)) - rg[0];
+        const int n1 = (im / (2 * rg[2] + 1)) % (2 * rg[1] + 1) - rg[1];
+        const int n2 = im % (2 * rg[2] + 1) - rg[2];
+        const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
+        double tv[3];
+        for (int k = 0; k < 3; ++k) tv[k] = n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k];
+        if (e < nat) {
+            const int o = a0 + e;
+            const double qq = qa * qatom(o);
+            if (qq == 0.0) continue;
+            double d[3];
+            for (int k = 0; k < 3; ++k) d[k] = x[3 * a + k] - x[3 * o + k] - tv[k];
+            const double r = sqrt(sum3(d[0], d[1], d[2]));
+            if (r <= 1e-12 || r > rcut) continue;
+            const double dg = -erfc(alpha * r) / (r * r) - tas * exp(-alpha * alpha * r * r) / r;
+            for (int k = 0; k < 3; ++k) acc[k] += d[k] * (qq * dg / r);
+        } else {
+            const int sj = s0 + (e - nat), aj = b.sh_atom[sj];
+            if (aj == a && origin) continue;
+            double d[3];
+            for (int k = 0; k < 3; ++k) d[k] = x[3 * a + k] - x[3 * aj + k] - tv[k];
+            const double r = sqrt(sum3(d[0], d[1], d[2]));
+            if (r <= 1e-12 || r > realcut) continue;
+            const double hj = t.shells[b.sh_type[sj]].hardness;
+            double v3 = 0.0, d3 = 0.0;
+            if (r <= rcut) r3_real(r, alpha, v3, d3);
+            for (int k2 = 0; k2 < nsa; ++k2) {
+                const int si = fa + k2;
+                const double qq = q[si] * q[sj];
+                if (qq == 0.0) continue;
+                const double hi = t.shells[b.sh_type[si]].hardness;
+                const double eta = 2.0 / (1.0 / hi + 1.0 / hj), ie2 = 1.0 / (eta * eta);
+                double rad = 0.0;
+                if (r <= rcut) rad += kR3Coeff * ie2 * d3;
+                if (r <= kEwaldShortCutoff) {
+                    double v, dv;
+                    ko_short(r, ie2, v, dv);
+                    rad += dv;
+                }
+                for (int k = 0; k < 3; ++k) acc[k] += d[k] * (qq * rad / r);
+            }
+        }
+    }
+    // (ii) reciprocal 1/R force and (iii) reciprocal R^-3 force, per G vector
+    const double* gt = b.gtab + c.g0 * kGStride;
+    for (int g = threadIdx.x; g < c.ng; g += blockDim.x) {
+        const double* e = gt + (size_t)g * kGStride;
+        double sc = 0.0, ss = 0.0;
+        for (int o = a0; o < a1; ++o) {
+            const double ph = e[0] * x[3 * o] + e[1] * x[3 * o + 1] + e[2] * x[3 * o + 2];
+            const double qo = qatom(o);
+            sc += qo * cos(ph);
+            ss += qo * sin(ph);
+        }
+        const double pha = e[0] * x[3 * a] + e[1] * x[3 * a + 1] + e[2] * x[3 * a + 2];
+        const double f1 = sin(pha) * sc - cos(pha) * ss;
+        double s = -e[3] * qa * f1;
+        // R^-3: coefficient -w3 per G (pbc/gradient.rs:686-706)
+        for (int k2 = 0; k2 < nsa; ++k2) {
+            const int si = fa + k2;
+            if (q[si] == 0.0) continue;
+            const double hi = t.shells[b.sh_type[si]].hardness;
+            double f = 0.0;
+            for (int j = 0; j < nsh; ++j) {
+                const int sj = s0 + j;
+                if (q[sj] == 0.0) continue;
+                const int aj = b.sh_atom[sj];
+                const double hj = t.shells[b.sh_type[sj]].hardness;
+                const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+                const double phj = e[0] * x[3 * aj] + e[1] * x[3 * aj + 1] + e[2] * x[3 * aj + 2];
+                f += q[sj] * sin(pha - phj) / (eta * eta);
+            }
+            s += -e[4] * q[si] * f;
+        }
+        for (int k = 0; k < 3; ++k) acc[k] += e[k] * s;
+    }
+    for (int k = 0; k < 3; ++k) acc[k] = block_sum_p(acc[k], lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) g_el[3 * a + k] += acc[k];
+}
+
+// ---------------------------------------------------------------- host side
+
+// Exponential integral E1(x), x > 0: its power series
+//   E1 = -gamma - ln x + sum_{k>=1} (-1)^(k+1) x^k / (k k!)
+// up to x = 1, above that the continued fraction
+//   E1 = e^-x / (x + 1 - 1^2 / (x + 3 - 2^2 / (x + 5 - ...)))
+// evaluated by the modified Lentz method (Abramowitz & Stegun 5.1.11, 5.1.22).
+double exponential_integral_e1(double x) {
+    if (x <= 1.0) {
+        double term = 1.0, sum = 0.0;
+        for (int k = 1; k <= 80; ++k) {
+            term *= -x / k;
+            const double add = term / k;
+            sum += add;
+            if (std::fabs(add) < 1e-17 * std::fabs(sum)) break;
+        }
+        return -0.57721566490153286061 - std::log(x) - sum;
+    }
+    const double tiny = 1e-300;
+    double bq = x + 1.0, cq = 1.0 / tiny, dq = 1.0 / bq, h = dq;
+    for (int i = 1; i <= 400; ++i) {
+        const double an = -(double)i * i;
+        bq += 2.0;
+        dq = an * dq + bq;
+        if (std::fabs(dq) < tiny) dq = tiny;
+        cq = bq + an / cq;
+        if (std::fabs(cq) < tiny) cq = tiny;
+        dq = 1.0 / dq;
+        const double del = cq * dq;
+        h *= del;
+        if (std::fabs(del - 1.0) < 1e-16) break;
+    }
+    return h * std::exp(-x);
+}
+
+namespace {
+void cross(const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+double norm(const double* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+}  // namespace
+
+bool make_cell_record(const double* cell, CellRec& c, std::vector<double>& gtab) {
+    std::memset(&c, 0, sizeof(c));
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            c.a[i][k] = cell[3 * i + k];
+            if (!std::isfinite(c.a[i][k])) return false;
+        }
+    double bc[3], ca[3], ab[3];
+    cross(c.a[1], c.a[2], bc);
+    cross(c.a[2], c.a[0], ca);
+    cross(c.a[0], c.a[1], ab);
+    const double det = c.a[0][0] * bc[0] + c.a[0][1] * bc[1] + c.a[0][2] * bc[2];
+    // Lattice::new (lattice.rs:73-87): |det| below 1e-14 is a singular cell
+    if (!(std::fabs(det) >= 1e-14)) return false;
+    c.vol = std::fabs(det);
+    c.h[0] = c.vol / std::max(norm(bc), 1e-12);
+    c.h[1] = c.vol / std::max(norm(ca), 1e-12);
+    c.h[2] = c.vol / std::max(norm(ab), 1e-12);
+    // Ewald constants (pbc/ewald.rs:48-60, :200-216, :253-281)
+    const double alpha = kSqrtPi / std::cbrt(std::max(c.vol, 1e-6));
+    c.alpha = alpha;
+    c.ew_self = -2.0 * alpha / kSqrtPi;
+    c.ew_bg = -kPiP / (alpha * alpha * c.vol);
+    const double kk = alpha / kSqrtPi;
+    const double digamma_3_2 = 0.03648997397857652;  // psi(3/2) = 2 - gamma - 2 ln 2
+    c.r3_k0 = kR3Coeff * 4.0 * kPiP * (std::log(kk) + 0.5 * (std::log(kPiP) - digamma_3_2)) / c.vol;
+    c.r3_self = kR3Coeff * (-(4.0 * kPiP / 3.0) * kk * kk * kk);
+    // reciprocal vectors 0 < |G| <= 2 alpha tau (lattice.rs:312-352): index
+    // ranges from the plane heights of the reciprocal basis
+    double rec[3][3];
+    for (int k = 0; k < 3; ++k) {
+        rec[0][k] = 2.0 * kPiP * bc[k] / det;
+        rec[1][k] = 2.0 * kPiP * ca[k] / det;
+        rec[2][k] = 2.0 * kPiP * ab[k] / det;
+    }
+    double rbc[3], rca[3], rab[3];
+    cross(rec[1], rec[2], rbc);
+    cross(rec[2], rec[0], rca);
+    cross(rec[0], rec[1], rab);
+    const double rvol = std::fabs(rec[0][0] * rbc[0] + rec[0][1] * rbc[1] + rec[0][2] * rbc[2]);
+    const double rh[3] = {rvol / std::max(norm(rbc), 1e-12), rvol / std::max(norm(rca), 1e-12),
+                          rvol / std::max(norm(rab), 1e-12)};
+    const double gcut = 2.0 * alpha * kEwaldTau, gc2 = gcut * gcut;
+    int rg[3];
+    for (int k = 0; k < 3; ++k) rg[k] = (int)std::ceil(gcut / std::max(rh[k], 1e-12)) + 1;
+    const double inv4a2 = 1.0 / (4.0 * alpha * alpha);
+    c.g0 = (int64_t)(gtab.size() / kGStride);
+    int ng = 0;
+    for (int h = -rg[0]; h <= rg[0]; ++h)
+        for (int k = -rg[1]; k <= rg[1]; ++k)
+            for (int l = -rg[2]; l <= rg[2]; ++l) {
+                if (h == 0 && k == 0 && l == 0) continue;
+                double g[3];
+                for (int d = 0; d < 3; ++d) g[d] = h * rec[0][d] + k * rec[1][d] + l * rec[2][d];
+                const double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                if (g2 > gc2 + 64.0 * 2.220446049250313e-16 * std::max(gc2, 1.0)) continue;
+                gtab.push_back(g[0]);
+                gtab.push_back(g[1]);
+                gtab.push_back(g[2]);
+                gtab.push_back(4.0 * kPiP / c.vol * std::exp(-g2 * inv4a2) / g2);
+                gtab.push_back(kR3Coeff * 2.0 * kPiP / c.vol * exponential_integral_e1(g2 * inv4a2));
+                ++ng;
+            }
+    c.ng = ng;
+    c.on = 1;
+    return true;
+}
+
+hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
+                              int npinst, int max_nsh) {
+    if (npinst == 0 || max_nsh == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ewald_gamma, dim3(max_nsh, npinst), dim3(256), 0, s, b, t, pinst);
+    return hipGetLastError();
+}
+
+hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* patoms,
+                             int npatoms, const double* q, double* g_el) {
+    if (npatoms == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ewald_grad, dim3(npatoms), dim3(256), 0, s, b, t, patoms, q, g_el);
+    return hipGetLastError();
+}
+
+}  // namespace gfn1

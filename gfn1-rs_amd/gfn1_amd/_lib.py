@@ -33,7 +33,7 @@ class Gfn1Error(ctypes.Structure):
 
 class System(ctypes.Structure):
     _fields_ = [("nat", ctypes.c_int32), ("numbers", ctypes.POINTER(ctypes.c_int32)),
-                ("positions", _DP), ("charge", ctypes.c_double)]
+                ("positions", _DP), ("charge", ctypes.c_double), ("lattice", _DP)]
 
 
 class ElectronicOptions(ctypes.Structure):
@@ -242,10 +242,16 @@ def check(code, err):
         raise NativeError(code, err.message.decode(errors="replace"), err.iterations, err.rms)
 
 
-def make_system(numbers, positions_bohr, charge=0.0):
+def make_system(numbers, positions_bohr, charge=0.0, lattice=None):
+    """lattice: None for a molecule, or the three cell vectors (rows, bohr) of a
+    fully periodic cell."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions_bohr, dtype=np.float64).reshape(-1, 3)
     if len(z) != len(x):
         raise ValueError("numbers and positions differ in length")
-    s = System(len(z), z.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), x.ctypes.data_as(_DP), float(charge))
-    return s, (z, x)  # keep the arrays alive with the struct
+    cell = None
+    if lattice is not None:
+        cell = np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+    s = System(len(z), z.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), x.ctypes.data_as(_DP), float(charge),
+               cell.ctypes.data_as(_DP) if cell is not None else None)
+    return s, (z, x, cell)  # keep the arrays alive with the struct

@@ -197,10 +197,12 @@ class AnalyticGradientResult:
     max_gradient: float
 
 
-def system_from_xyz(text: str, charge: float = 0.0, coordinates_are_bohr: bool = False):
-    """PeriodicSystem::from_xyz_str (system.rs:45-99), non-periodic: returns
-    (numbers, positions in bohr as an (nat, 3) array, charge).  Raises
-    NativeError (InvalidInput, or Parse for a bad coordinate) like the
+def system_from_xyz(text: str, charge: float = 0.0, coordinates_are_bohr: bool = False, with_lattice=False):
+    """PeriodicSystem::from_xyz_str (system.rs:45-99): returns (numbers,
+    positions in bohr as an (nat, 3) array, charge), and with with_lattice the
+    cell vectors as a fourth item (a (3, 3) array in bohr from an extended-XYZ
+    Lattice="..." comment, None without one).  Raises NativeError
+    (InvalidInput, Parse for a bad coordinate, SingularCell) like the
     reference's Err."""
     lib = _lib.load()
     h = ctypes.c_void_p()
@@ -212,15 +214,18 @@ def system_from_xyz(text: str, charge: float = 0.0, coordinates_are_bohr: bool =
         nat = s.nat
         numbers = np.ctypeslib.as_array(s.numbers, shape=(nat,)).copy() if nat else np.zeros(0, dtype=np.int32)
         pos = np.ctypeslib.as_array(s.positions, shape=(3 * nat,)).reshape(nat, 3).copy() if nat else np.zeros((0, 3))
-        return numbers, pos, s.charge
+        if not with_lattice:
+            return numbers, pos, s.charge
+        cell = np.ctypeslib.as_array(s.lattice, shape=(9,)).reshape(3, 3).copy() if s.lattice else None
+        return numbers, pos, s.charge, cell
     finally:
         lib.gfn1_xyz_free(h)
 
 
-def system_from_xyz_file(path, charge: float = 0.0, coordinates_are_bohr: bool = False):
+def system_from_xyz_file(path, charge: float = 0.0, coordinates_are_bohr: bool = False, with_lattice=False):
     """PeriodicSystem::from_xyz_file (system.rs:35-43)."""
     with open(path) as f:
-        return system_from_xyz(f.read(), charge, coordinates_are_bohr)
+        return system_from_xyz(f.read(), charge, coordinates_are_bohr, with_lattice)
 
 
 def _params(params):
@@ -253,12 +258,15 @@ def _read_electronic(lib, h):
 
 
 def run_electronic(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
-                   options: Optional[ElectronicOptions] = None, charge: float = 0.0) -> ElectronicResult:
-    """run_electronic (electronic.rs:800): positions in bohr."""
+                   options: Optional[ElectronicOptions] = None, charge: float = 0.0,
+                   lattice=None) -> ElectronicResult:
+    """run_electronic (electronic.rs:800): positions in bohr.  lattice: the
+    three cell vectors (rows, bohr) of a fully periodic cell, which takes the
+    Gamma-point path (electronic.rs:829-832)."""
     lib = _lib.load()
     p = _params(params)
     opts = options or ElectronicOptions()
-    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge, lattice)
     oc, keep_q = opts.to_c()
     h = ctypes.c_void_p()
     err = _lib.Gfn1Error()
@@ -272,12 +280,14 @@ def run_electronic(numbers, positions_bohr, params: Optional[Gfn1Parameters] = N
 
 def analytic_gradient(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
                       options: Optional[AnalyticGradientOptions] = None,
-                      charge: float = 0.0) -> AnalyticGradientResult:
-    """analytic_gradient (gradient.rs:56): gradient in Eh/bohr."""
+                      charge: float = 0.0, lattice=None) -> AnalyticGradientResult:
+    """analytic_gradient (gradient.rs:56): gradient in Eh/bohr.  With a lattice
+    (rows, bohr) the periodic Gamma-point gradient (gradient.rs:61-66), which
+    has every term whatever the include flags."""
     lib = _lib.load()
     p = _params(params)
     opts = options or AnalyticGradientOptions()
-    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge, lattice)
     oc, keep_q = opts.to_c()
     h = ctypes.c_void_p()
     err = _lib.Gfn1Error()
@@ -743,12 +753,12 @@ def basis_counts(numbers, params: Optional[Gfn1Parameters] = None):
 
 
 def _system_array(systems):
-    """(numbers, positions_bohr[, charge]) tuples -> ctypes System array + keep-alive list."""
+    """(numbers, positions_bohr[, charge[, lattice]]) tuples -> ctypes System array + keep-alive list."""
     arr = (_lib.System * len(systems))()
     keep = []
     for i, s in enumerate(systems):
         charge = s[2] if len(s) > 2 else 0.0
-        sc, k = _lib.make_system(s[0], s[1], charge)
+        sc, k = _lib.make_system(s[0], s[1], charge, s[3] if len(s) > 3 else None)
         arr[i] = sc
         keep.append(k)
     return arr, keep
@@ -894,9 +904,11 @@ class BatchEngine:
         self._keep = []
         self.natoms = []
         for i, s in enumerate(systems):
+            # (numbers, positions_bohr[, charge[, lattice]]): a lattice (rows,
+            # bohr) makes the instance periodic (Gamma point)
             numbers, pos = s[0], s[1]
             charge = s[2] if len(s) > 2 else 0.0
-            sc, keep = _lib.make_system(numbers, pos, charge)
+            sc, keep = _lib.make_system(numbers, pos, charge, s[3] if len(s) > 3 else None)
             arr[i] = sc
             self._keep.append(keep)
             self.natoms.append(len(keep[0]))

@@ -66,21 +66,34 @@ int32_t gfn1_params_source(const gfn1_params* params, char* out, int32_t len);
  * (src/basis.rs:121-233). */
 int gfn1_params_shell_count(const gfn1_params* params, int z);
 
-/* PeriodicSystem without a lattice (src/system.rs:17-21). */
+/* PeriodicSystem (src/system.rs:17-21).  lattice: NULL for a molecule, or the
+ * three cell vectors a, b, c of a fully periodic cell in bohr, one after the
+ * other (9 doubles).  A system with a lattice takes the periodic Gamma-point
+ * path, as run_electronic and analytic_gradient dispatch it
+ * (src/electronic.rs:829-832, src/gradient.rs:61-66): image sums of S and H0
+ * within 30 bohr, the Ewald-summed shell Coulomb matrix, lattice-summed CN,
+ * repulsion, D3 and halogen terms, no level shift and no fallback ladder in
+ * the SCC, and every gradient term whatever the include flags.  A cell with
+ * |det| < 1e-14 fails with GFN1_ERR_SINGULAR_CELL. */
 typedef struct gfn1_system {
     int32_t nat;
     const int32_t* numbers;   /* atomic numbers, nat */
     const double* positions;  /* bohr, 3*nat, atom-major xyz */
     double charge;
+    const double* lattice;    /* NULL, or 9 doubles: a, b, c (bohr) */
 } gfn1_system;
 
-/* PeriodicSystem::from_xyz_str (src/system.rs:45-99) for non-periodic input:
- * line 1 the atom count, line 2 a comment, then "symbol x y z" per atom (a
- * symbol in any letter case, or an atomic number 1..86).  Coordinates are
- * Angstrom unless coordinates_are_bohr.  An extended-XYZ Lattice="..." comment
- * (a periodic cell) is rejected.  Errors: GFN1_ERR_INVALID_INPUT for the
- * count, a short or missing line or an unknown element; GFN1_ERR_PARSE for a
- * coordinate that is not a number (the message names the line). */
+/* PeriodicSystem::from_xyz_str (src/system.rs:45-99): line 1 the atom count,
+ * line 2 a comment, then "symbol x y z" per atom (a symbol in any letter case,
+ * or an atomic number 1..86).  Coordinates are Angstrom unless
+ * coordinates_are_bohr.  An extended-XYZ comment Lattice="ax ay az bx by bz cx
+ * cy cz" gives the system a lattice (same units as the coordinates;
+ * src/system.rs:191-219); a pbc="T T T" field, when present, must mark all
+ * three directions periodic (only fully periodic cells are supported).  Errors:
+ * GFN1_ERR_INVALID_INPUT for the count, a short or missing line, an unknown
+ * element or a malformed Lattice/pbc field; GFN1_ERR_PARSE for a coordinate
+ * that is not a number (the message names the line); GFN1_ERR_SINGULAR_CELL
+ * for cell vectors with |det| < 1e-14. */
 typedef struct gfn1_xyz gfn1_xyz;
 int gfn1_system_from_xyz(const char* text, double charge, int32_t coordinates_are_bohr, gfn1_xyz** out,
                          gfn1_error* err);

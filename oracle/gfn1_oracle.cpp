@@ -1015,6 +1015,796 @@ void gamma_matrix(const Basis& b, const Geo& g, std::vector<double>& gam) {
         }
 }
 
+// ------------------------------------------------------------------ periodic (Gamma point)
+// The reference's periodic path (src/pbc/, src/lattice.rs, src/pairlist.rs)
+// restated for a fully periodic cell at k = 0.  Every lattice sum below runs
+// over explicit image translations T = n0 a + n1 b + n2 c.
+
+// lattice.rs:73-91 Lattice::new (|det| < 1e-14 is SingularCell), :242-261
+// image_ranges (ceil(cutoff / plane height) + 1 per axis), :263-296
+// image_offsets (sorted by |T|^2, then by the integer triple), :298-376
+// reciprocal vectors (2 pi) and their index ranges.
+struct Lattice {
+    double a[3][3];    // rows: cell vectors (bohr)
+    double rec[3][3];  // rows: reciprocal vectors, a_i . rec_j = 2 pi delta_ij
+    double vol;
+};
+struct Image {
+    int n[3];
+    double t[3];
+};
+
+V3 cross3(const double* u, const double* v) {
+    return {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+}
+double norm3(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+bool make_lattice(const double* cell, Lattice& L) {
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) L.a[i][k] = cell[3 * i + k];
+    V3 bc = cross3(L.a[1], L.a[2]), ca = cross3(L.a[2], L.a[0]), ab = cross3(L.a[0], L.a[1]);
+    double det = L.a[0][0] * bc.x + L.a[0][1] * bc.y + L.a[0][2] * bc.z;
+    if (!(std::fabs(det) >= 1e-14)) return false;
+    L.vol = std::fabs(det);
+    V3 r[3] = {bc, ca, ab};
+    for (int i = 0; i < 3; ++i) {
+        L.rec[i][0] = 2.0 * PI * r[i].x / det;
+        L.rec[i][1] = 2.0 * PI * r[i].y / det;
+        L.rec[i][2] = 2.0 * PI * r[i].z / det;
+    }
+    return true;
+}
+
+// plane heights of a basis (lattice.rs:378-389)
+void plane_heights(const double (*v)[3], double h[3]) {
+    V3 bc = cross3(v[1], v[2]), ca = cross3(v[2], v[0]), ab = cross3(v[0], v[1]);
+    double vol = std::fabs(v[0][0] * bc.x + v[0][1] * bc.y + v[0][2] * bc.z);
+    h[0] = vol / std::max(norm3(bc), 1e-12);
+    h[1] = vol / std::max(norm3(ca), 1e-12);
+    h[2] = vol / std::max(norm3(ab), 1e-12);
+}
+
+std::vector<Image> images(const Lattice& L, double cutoff) {
+    double h[3];
+    plane_heights(L.a, h);
+    int rg[3];
+    for (int k = 0; k < 3; ++k) rg[k] = cutoff <= 0.0 ? 0 : (int)std::ceil(cutoff / std::max(h[k], 1e-12)) + 1;
+    std::vector<Image> out;
+    for (int i = -rg[0]; i <= rg[0]; ++i)
+        for (int j = -rg[1]; j <= rg[1]; ++j)
+            for (int k = -rg[2]; k <= rg[2]; ++k) {
+                Image im{{i, j, k}, {0, 0, 0}};
+                for (int c = 0; c < 3; ++c) im.t[c] = i * L.a[0][c] + j * L.a[1][c] + k * L.a[2][c];
+                out.push_back(im);
+            }
+    auto n2 = [](const Image& m) { return m.t[0] * m.t[0] + m.t[1] * m.t[1] + m.t[2] * m.t[2]; };
+    std::stable_sort(out.begin(), out.end(), [&](const Image& p, const Image& q) {
+        double a = n2(p), b = n2(q);
+        if (a != b) return a < b;
+        return std::lexicographical_compare(p.n, p.n + 3, q.n, q.n + 3);
+    });
+    return out;
+}
+
+// lattice.rs:312-352 reciprocal_vectors_within(cutoff, include_zero = false)
+std::vector<Image> recip_within(const Lattice& L, double cutoff) {
+    double h[3];
+    plane_heights(L.rec, h);
+    int rg[3];
+    for (int k = 0; k < 3; ++k) rg[k] = cutoff <= 0.0 ? 0 : (int)std::ceil(cutoff / std::max(h[k], 1e-12)) + 1;
+    double c2 = cutoff * cutoff, tol = 64.0 * 2.220446049250313e-16 * std::max(c2, 1.0);
+    std::vector<Image> out;
+    for (int i = -rg[0]; i <= rg[0]; ++i)
+        for (int j = -rg[1]; j <= rg[1]; ++j)
+            for (int k = -rg[2]; k <= rg[2]; ++k) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                Image g{{i, j, k}, {0, 0, 0}};
+                for (int c = 0; c < 3; ++c) g.t[c] = i * L.rec[0][c] + j * L.rec[1][c] + k * L.rec[2][c];
+                if (g.t[0] * g.t[0] + g.t[1] * g.t[1] + g.t[2] * g.t[2] <= c2 + tol) out.push_back(g);
+            }
+    return out;
+}
+
+bool is_origin(const Image& m) { return m.n[0] == 0 && m.n[1] == 0 && m.n[2] == 0; }
+// pairlist.rs:574-584: the first nonzero index of the offset is positive
+bool canonical_positive(const Image& m) {
+    for (int k = 0; k < 3; ++k) {
+        if (m.n[k] > 0) return true;
+        if (m.n[k] < 0) return false;
+    }
+    return false;
+}
+
+// pairlist.rs:244-317 for_each_unique_short_range_pair: the origin cell with
+// j > i, and every canonical-positive image with every (i, j), i == j included.
+// f(i, j, dr = x_j + T - x_i, r2).
+template <typename F>
+void unique_pairs(const Geo& g, const Lattice& L, double cutoff, F f) {
+    double c2 = cutoff * cutoff;
+    for (const Image& im : images(L, cutoff)) {
+        bool orig = is_origin(im);
+        if (!orig && !canonical_positive(im)) continue;
+        for (int i = 0; i < g.nat; ++i)
+            for (int j = orig ? i + 1 : 0; j < g.nat; ++j) {
+                double dr[3];
+                for (int c = 0; c < 3; ++c) dr[c] = g.x[3 * j + c] + im.t[c] - g.x[3 * i + c];
+                double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                if (r2 <= PAIR_EPS2 || r2 > c2) continue;
+                f(i, j, dr, r2);
+            }
+    }
+}
+
+// coordination.rs:51-94 with lattice images: a self-image pair counts twice.
+void coordination_pbc(const Geo& g, const Lattice& L, double cutoff, std::vector<double>& cn,
+                      std::vector<CnPair>* pairs) {
+    cn.assign(g.nat, 0.0);
+    if (pairs) pairs->clear();
+    const double kcn = 16.0;
+    unique_pairs(g, L, cutoff, [&](int i, int j, const double* dr, double r2) {
+        double r = std::sqrt(r2);
+        double rc = (COV_RADII_AA[g.z[i]] + COV_RADII_AA[g.z[j]]) * (4.0 / 3.0) * AA2AU;
+        if (rc <= 1e-12) return;
+        double raw = -kcn * (rc / r - 1.0);
+        double arg = std::min(80.0, std::max(-80.0, raw));
+        double e = std::exp(arg);
+        double val = 1.0 / (1.0 + e);
+        double dv = (raw < -80.0 || raw > 80.0) ? 0.0 : -kcn * rc * e / (r * r * (1.0 + e) * (1.0 + e));
+        double m = (i == j) ? 2.0 : 1.0;
+        if (i == j) cn[i] += 2.0 * val;
+        else { cn[i] += val; cn[j] += val; }
+        if (pairs) {
+            CnPair cp;
+            cp.i = i; cp.j = j;
+            for (int c = 0; c < 3; ++c) cp.rij[c] = -dr[c];
+            cp.dcn_dr = m * dv;
+            pairs->push_back(cp);
+        }
+    });
+}
+
+// CN chain over periodic pairs (gradient.rs:475-489, pbc/gradient.rs:387-404):
+// self-image pairs are skipped.
+void cn_chain_pbc(const std::vector<CnPair>& pairs, const std::vector<double>& dedcn, double* grad, double eps) {
+    for (auto& cp : pairs) {
+        if (cp.i == cp.j) continue;
+        double r = std::sqrt(cp.rij[0] * cp.rij[0] + cp.rij[1] * cp.rij[1] + cp.rij[2] * cp.rij[2]);
+        if (r <= eps) continue;
+        double pref = (dedcn[cp.i] + dedcn[cp.j]) * cp.dcn_dr / r;
+        for (int k = 0; k < 3; ++k) {
+            grad[3 * cp.i + k] += cp.rij[k] * pref;
+            grad[3 * cp.j + k] -= cp.rij[k] * pref;
+        }
+    }
+}
+
+// repulsion.rs:39-59 over image pairs (a self-image pair has energy, no force).
+double repulsion_pbc(const Ctx& ctx, const Geo& g, const Lattice& L, double* grad) {
+    double e = 0.0;
+    double kexp = ctx.p.global("kexp", 1.5);
+    double kexpl = ctx.p.global("kexplight", kexp);
+    unique_pairs(g, L, 25.0, [&](int i, int j, const double* dr, double r2) {
+        double r = std::sqrt(r2);
+        const ElementParam &A = ctx.p.el[g.z[i]], &B = ctx.p.el[g.z[j]];
+        double alpha = std::sqrt(A.repa * B.repa);
+        double zeff = A.repb * B.repb;
+        double k = (g.z[i] <= 2 && g.z[j] <= 2) ? kexpl : kexp;
+        double rk = std::pow(r, k);
+        double en = zeff * std::exp(-alpha * rk) / r;
+        double pref = (alpha * rk * k + 1.0) * en / r2;
+        e += en;
+        if (grad && i != j)
+            for (int c = 0; c < 3; ++c) {
+                grad[3 * i + c] += dr[c] * pref;
+                grad[3 * j + c] -= dr[c] * pref;
+            }
+    });
+    return e;
+}
+
+// dispersion.rs:1232-1343 D3(BJ) two-body over image pairs (s9 = 0), with the
+// lattice-summed CN (25 bohr) of dispersion.rs:1245-1251.
+double dispersion_pbc(const Ctx& ctx, const Geo& g, const Lattice& L, double* grad) {
+    const int nat = g.nat, mr = ctx.d3_max_ref;
+    double s6 = ctx.p.global("s6", 1.0), s8 = ctx.p.global("s8", 0), a1 = ctx.p.global("a1", 0),
+           a2 = ctx.p.global("a2", 0);
+    std::vector<double> cn;
+    std::vector<CnPair> cnp;
+    coordination_pbc(g, L, 25.0, cn, &cnp);
+    const double WF = 4.0, wf2 = 2.0 * WF;
+    std::vector<double> gw((size_t)nat * mr, 0.0), dgw((size_t)nat * mr, 0.0);
+    for (int i = 0; i < nat; ++i) {
+        int zi = g.z[i], nref = ctx.d3_nref[zi - 1];
+        double mx = -INFINITY;
+        std::vector<double> logit(nref), cr(nref), raw(nref);
+        for (int r = 0; r < nref; ++r) {
+            cr[r] = ctx.d3_refcn[(size_t)(zi - 1) * mr + r];
+            logit[r] = -WF * (cn[i] - cr[r]) * (cn[i] - cr[r]);
+            mx = std::max(mx, logit[r]);
+        }
+        double norm = 0, dnorm = 0;
+        for (int r = 0; r < nref; ++r) {
+            raw[r] = std::exp(logit[r] - mx);
+            norm += raw[r];
+            dnorm += wf2 * (cr[r] - cn[i]) * raw[r];
+        }
+        double ni = 1.0 / norm;
+        for (int r = 0; r < nref; ++r) {
+            double expd = wf2 * (cr[r] - cn[i]) * raw[r];
+            double w = raw[r] * ni;
+            double dw = expd * ni - raw[r] * dnorm * ni * ni;
+            gw[(size_t)i * mr + r] = std::isfinite(w) ? w : 0.0;
+            dgw[(size_t)i * mr + r] = std::isfinite(dw) ? dw : 0.0;
+        }
+    }
+    std::vector<double> c6((size_t)nat * nat), dc6((size_t)nat * nat);
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j) {
+            int zi = g.z[i], zj = g.z[j];
+            double cij = 0, dci = 0, dcj = 0;
+            for (int ri = 0; ri < ctx.d3_nref[zi - 1]; ++ri)
+                for (int rj = 0; rj < ctx.d3_nref[zj - 1]; ++rj) {
+                    double ref = d3_c6ref(ctx, ri, rj, zi, zj);
+                    cij += gw[(size_t)i * mr + ri] * gw[(size_t)j * mr + rj] * ref;
+                    dci += dgw[(size_t)i * mr + ri] * gw[(size_t)j * mr + rj] * ref;
+                    dcj += gw[(size_t)i * mr + ri] * dgw[(size_t)j * mr + rj] * ref;
+                }
+            c6[(size_t)i * nat + j] = c6[(size_t)j * nat + i] = cij;
+            if (i == j) dc6[(size_t)i * nat + i] = dci + dcj;
+            else { dc6[(size_t)i * nat + j] = dci; dc6[(size_t)j * nat + i] = dcj; }
+        }
+    double e = 0.0;
+    std::vector<double> dedcn(nat, 0.0);
+    unique_pairs(g, L, 50.0, [&](int i, int j, const double* dr, double r2) {
+        if (r2 <= 1e-14) return;
+        int zi = g.z[i], zj = g.z[j];
+        double r4r2ij = 3.0 * ctx.d3_r4r2[zi - 1] * ctx.d3_r4r2[zj - 1];
+        double r0 = a1 * std::sqrt(r4r2ij) + a2;
+        double r0_2 = r0 * r0, r0_6 = r0_2 * r0_2 * r0_2, r0_8 = r0_6 * r0_2;
+        double r4 = r2 * r2;
+        double t6 = 1.0 / (r4 * r2 + r0_6), t8 = 1.0 / (r4 * r4 + r0_8);
+        double d6 = -6.0 * r4 * t6 * t6, d8 = -8.0 * r4 * r2 * t8 * t8;
+        double ed = s6 * t6 + s8 * r4r2ij * t8;
+        double gd = s6 * d6 + s8 * r4r2ij * d8;
+        double cij = c6[(size_t)i * nat + j];
+        e -= cij * ed;
+        if (grad && i != j)
+            for (int c = 0; c < 3; ++c) {
+                double dgv = -dr[c] * (-cij * gd);
+                grad[3 * i + c] += dgv;
+                grad[3 * j + c] -= dgv;
+            }
+        if (i == j) dedcn[i] -= dc6[(size_t)i * nat + i] * ed;
+        else {
+            dedcn[i] -= dc6[(size_t)i * nat + j] * ed;
+            dedcn[j] -= dc6[(size_t)j * nat + i] * ed;
+        }
+    });
+    if (grad) cn_chain_pbc(cnp, dedcn, grad, 1e-14);
+    return e;
+}
+
+// halogen.rs:64-160 with the periodic triples of :716-740: the donor's
+// neighbours are every (atom, image) within 20 bohr except the donor itself
+// in the origin cell; the nearest one that is not an image of the donor is
+// the bonded neighbour; acceptors may be images.
+double halogen_pbc(const Ctx& ctx, const Geo& g, const Lattice& L, double* grad) {
+    const double ALP = 6.0, LJ2 = 6.0, EPS = 1e-18, cut = 20.0;
+    double damp = ctx.p.global("xbdamp", 0.44), rads = ctx.p.global("xbrad", 1.3);
+    double e = 0.0;
+    std::vector<Image> ims = images(L, cut);
+    struct Nb { int j; double t[3]; double r2; };
+    for (int x = 0; x < g.nat; ++x) {
+        int zx = g.z[x];
+        if (!(zx == 17 || zx == 35 || zx == 53 || zx == 85)) continue;
+        std::vector<Nb> nbs;
+        for (const Image& im : ims)
+            for (int j = 0; j < g.nat; ++j) {
+                if (is_origin(im) && j == x) continue;
+                double d[3];
+                for (int c = 0; c < 3; ++c) d[c] = g.x[3 * j + c] + im.t[c] - g.x[3 * x + c];
+                double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (r2 <= PAIR_EPS2 || r2 > cut * cut) continue;
+                nbs.push_back({j, {im.t[0], im.t[1], im.t[2]}, r2});
+            }
+        int kbest = -1;
+        for (int q = 0; q < (int)nbs.size(); ++q)
+            if (nbs[q].j != x && (kbest < 0 || nbs[q].r2 < nbs[kbest].r2)) kbest = q;
+        if (kbest < 0) continue;
+        double cc = ctx.p.el[zx].cxb * 0.1;
+        if (cc == 0.0) continue;
+        const Nb& K = nbs[kbest];
+        int k = K.j;
+        for (const Nb& J : nbs) {
+            int j = J.j, zj = g.z[j];
+            if (!(zj == 7 || zj == 8 || zj == 15 || zj == 16)) continue;
+            double r0 = rads * (ATOMIC_RADII_AA[zx] + ATOMIC_RADII_AA[zj]) * AA2AU;
+            double dxj[3], dxk[3], dkj[3];
+            for (int c = 0; c < 3; ++c) {
+                dxj[c] = g.x[3 * j + c] + J.t[c] - g.x[3 * x + c];
+                dxk[c] = g.x[3 * k + c] + K.t[c] - g.x[3 * x + c];
+                dkj[c] = g.x[3 * j + c] + J.t[c] - g.x[3 * k + c] - K.t[c];
+            }
+            double d2jx = dxj[0] * dxj[0] + dxj[1] * dxj[1] + dxj[2] * dxj[2];
+            double d2kx = dxk[0] * dxk[0] + dxk[1] * dxk[1] + dxk[2] * dxk[2];
+            double d2jk = dkj[0] * dkj[0] + dkj[1] * dkj[1] + dkj[2] * dkj[2];
+            if (d2jx <= EPS || d2kx <= EPS) continue;
+            double rjx = std::sqrt(d2jx) + EPS, rkx = std::sqrt(d2kx) + EPS;
+            double xy = std::sqrt(d2kx * d2jx);
+            if (xy <= EPS) continue;
+            double term = (d2kx + d2jx - d2jk) / xy;
+            double ab = 0.5 - 0.25 * term;
+            double aterm = std::pow(ab, ALP);
+            double t13 = r0 / rjx;
+            double t14lj = std::pow(t13, 12.0);
+            e += aterm * cc * (t14lj - damp * std::pow(t13, LJ2)) / (1.0 + t14lj);
+            if (!grad) continue;
+            double t14 = std::pow(r0 / rjx, LJ2);
+            double num = t14 * t14 - damp * t14, den = 1.0 + t14 * t14;
+            double tlj = num / den;
+            double dt = 2.0 * LJ2 * num * t14 * t14 / (rjx * den * den);
+            dt += LJ2 * t14 * (damp - 2.0 * t14) / (rjx * den);
+            dt *= aterm * cc / rjx;
+            double pf = -0.25 * ALP * std::pow(ab, ALP - 1.0) * cc * tlj;
+            double djx = (2.0 / rkx - term / rjx) * pf / rjx;
+            double dkx = (2.0 / rjx - term / rkx) * pf / rkx;
+            double djk = 2.0 * pf / xy;
+            for (int c = 0; c < 3; ++c) {
+                grad[3 * j + c] += dxj[c] * dt;
+                grad[3 * x + c] -= dxj[c] * dt;
+                grad[3 * j + c] += dxj[c] * djx;
+                grad[3 * x + c] -= dxj[c] * djx;
+                grad[3 * k + c] += dxk[c] * dkx;
+                grad[3 * x + c] -= dxk[c] * dkx;
+                grad[3 * j + c] -= dkj[c] * djk;
+                grad[3 * k + c] += dkj[c] * djk;
+            }
+        }
+    }
+    return e;
+}
+
+// ---- Ewald summation of the Klopman-Ohno interaction (pbc/ewald.rs)
+const double EW_TAU = 5.5;           // ewald.rs:39
+const double EW_R3 = -0.5;           // ewald.rs:38 binomial coefficient of eta^-2 R^-3
+const double SQRTPI = 1.7724538509055160273;
+
+// Exponential integral E1(x), x > 0: the power series
+// E1 = -gamma - ln x - sum_k (-x)^k / (k k!) for x <= 1, the continued fraction
+// E1 = e^-x / (x + 1 - 1/(x + 3 - 4/(x + 5 - ...))) by the modified Lentz
+// method otherwise (Abramowitz & Stegun 5.1.11 and 5.1.22).  Written for this
+// file; the reference evaluates the same function (ewald.rs:142-181).
+double expint_e1(double x) {
+    if (x <= 1.0) {
+        double sum = 0.0, term = 1.0;
+        for (int k = 1; k <= 60; ++k) {
+            term *= -x / k;
+            double add = term / k;
+            sum += add;
+            if (std::fabs(add) < 1e-17 * std::fabs(sum)) break;
+        }
+        return -0.57721566490153286061 - std::log(x) - sum;
+    }
+    const double tiny = 1e-300;
+    double b = x + 1.0, c = 1.0 / tiny, d = 1.0 / b, h = d;
+    for (int i = 1; i <= 300; ++i) {
+        double an = -(double)i * i;
+        b += 2.0;
+        d = an * d + b;
+        if (std::fabs(d) < tiny) d = tiny;
+        c = b + an / c;
+        if (std::fabs(c) < tiny) c = tiny;
+        d = 1.0 / d;
+        double del = c * d;
+        h *= del;
+        if (std::fabs(del - 1.0) < 1e-16) break;
+    }
+    return h * std::exp(-x);
+}
+
+// ewald.rs:48-60: alpha = sqrt(pi) / V^(1/3)
+double ewald_alpha(const Lattice& L) { return SQRTPI / std::cbrt(std::max(L.vol, 1e-6)); }
+
+// ewald.rs:200-243 ewald_atom_matrix: Madelung potential of a unit charge and
+// its images with the neutralising background, 1/R split by erfc.
+void ewald_atom_matrix(const Geo& g, const Lattice& L, double alpha, std::vector<double>& phi) {
+    const int nat = g.nat;
+    double rcut = EW_TAU / alpha, gcut = 2.0 * alpha * EW_TAU;
+    std::vector<Image> ims = images(L, rcut), gs = recip_within(L, gcut);
+    double inv4a2 = 1.0 / (4.0 * alpha * alpha), fpv = 4.0 * PI / L.vol;
+    double self = -2.0 * alpha / SQRTPI, bg = -PI / (alpha * alpha * L.vol);
+    phi.assign((size_t)nat * nat, 0.0);
+    for (int a = 0; a < nat; ++a)
+        for (int b = 0; b <= a; ++b) {
+            double rab[3];
+            for (int c = 0; c < 3; ++c) rab[c] = g.x[3 * a + c] - g.x[3 * b + c];
+            double real = 0.0;
+            for (const Image& im : ims) {
+                double d[3] = {rab[0] - im.t[0], rab[1] - im.t[1], rab[2] - im.t[2]};
+                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (r <= 1e-12 || r > rcut) continue;
+                real += std::erfc(alpha * r) / r;
+            }
+            double rec = 0.0;
+            for (const Image& G : gs) {
+                double g2 = G.t[0] * G.t[0] + G.t[1] * G.t[1] + G.t[2] * G.t[2];
+                rec += std::exp(-g2 * inv4a2) / g2 * std::cos(G.t[0] * rab[0] + G.t[1] * rab[1] + G.t[2] * rab[2]);
+            }
+            double v = real + rec * fpv + bg;
+            if (a == b) v += self;
+            phi[(size_t)a * nat + b] = phi[(size_t)b * nat + a] = v;
+        }
+}
+
+// ewald.rs:96-140: short-range KO remainder and the real-space part of the
+// eta^-2 R^-3 Ewald term, value and radial derivative.
+void ko_short(double r, double eta, double& v, double& dv) {
+    double ie2 = 1.0 / (eta * eta), den2 = r * r + ie2, den = std::sqrt(den2);
+    double r2 = r * r, r3 = r2 * r, r4 = r2 * r2;
+    v = 1.0 / den - 1.0 / r + 0.5 * ie2 / r3;
+    dv = -r / std::pow(den2, 1.5) + 1.0 / r2 - 1.5 * ie2 / r4;
+}
+void r3_real(double r, double eta, double alpha, double& v, double& dv) {
+    double ie2 = 1.0 / (eta * eta), ar = alpha * r, ex = std::exp(-ar * ar);
+    double q = std::erfc(ar) + (2.0 * ar / SQRTPI) * ex;
+    double dq = -(4.0 * alpha * alpha * alpha * r * r / SQRTPI) * ex;
+    double r3 = r * r * r, r4 = r3 * r;
+    v = ie2 * q / r3;
+    dv = ie2 * (dq / r3 - 3.0 * q / r4);
+}
+
+// ewald.rs:253-320 periodic_gamma_matrix: Ewald 1/R between atom centres, the
+// on-site eta, the generalised Ewald eta^-2 R^-3 term (real, reciprocal with
+// E1, the k = 0 logarithm, self) and the directly summed short-range remainder
+// (10 bohr).
+const double EW_SR_CUT = 10.0;  // pbc/mod.rs:162 sr_cutoff
+void ewald_gamma_matrix(const Basis& b, const Geo& g, const Lattice& L, std::vector<double>& gam,
+                        double alpha = 0.0) {
+    const int nsh = (int)b.sh.size();
+    if (alpha <= 0.0) alpha = ewald_alpha(L);
+    std::vector<double> phi;
+    ewald_atom_matrix(g, L, alpha, phi);
+    double r3cut = EW_TAU / alpha;
+    std::vector<Image> sri = images(L, EW_SR_CUT), r3i = images(L, r3cut), gs = recip_within(L, 2.0 * alpha * EW_TAU);
+    double inv4a2 = 1.0 / (4.0 * alpha * alpha);
+    double r3_rec = 2.0 * PI / L.vol;
+    const double DIGAMMA_3_2 = 0.03648997397857652;  // psi(3/2) = 2 - gamma - 2 ln 2
+    double kk = alpha / SQRTPI;
+    double r3_k0 = 4.0 * PI * (std::log(kk) + 0.5 * (std::log(PI) - DIGAMMA_3_2)) / L.vol;
+    double r3_self = -(4.0 * PI / 3.0) * kk * kk * kk;
+    std::vector<double> e1(gs.size());
+    for (size_t q = 0; q < gs.size(); ++q)
+        e1[q] = expint_e1((gs[q].t[0] * gs[q].t[0] + gs[q].t[1] * gs[q].t[1] + gs[q].t[2] * gs[q].t[2]) * inv4a2);
+    gam.assign((size_t)nsh * nsh, 0.0);
+    for (int i = 0; i < nsh; ++i) {
+        int ai = b.sh[i].atom;
+        for (int j = 0; j <= i; ++j) {
+            int aj = b.sh[j].atom;
+            double eta = 2.0 / (1.0 / b.sh[i].hardness + 1.0 / b.sh[j].hardness);
+            double ie2 = 1.0 / (eta * eta);
+            double v = phi[(size_t)ai * g.nat + aj];
+            if (ai == aj) v += eta;
+            double rab[3];
+            for (int c = 0; c < 3; ++c) rab[c] = g.x[3 * ai + c] - g.x[3 * aj + c];
+            for (const Image& im : r3i) {
+                if (ai == aj && is_origin(im)) continue;
+                double d[3] = {rab[0] - im.t[0], rab[1] - im.t[1], rab[2] - im.t[2]};
+                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (r <= 1e-12 || r > r3cut) continue;
+                double rv, rdv;
+                r3_real(r, eta, alpha, rv, rdv);
+                v += EW_R3 * rv;
+            }
+            for (size_t q = 0; q < gs.size(); ++q)
+                v += EW_R3 * r3_rec * ie2 * e1[q] *
+                     std::cos(gs[q].t[0] * rab[0] + gs[q].t[1] * rab[1] + gs[q].t[2] * rab[2]);
+            v += EW_R3 * r3_k0 * ie2;
+            if (ai == aj) v += EW_R3 * r3_self * ie2;
+            for (const Image& im : sri) {
+                if (ai == aj && is_origin(im)) continue;
+                double d[3] = {rab[0] - im.t[0], rab[1] - im.t[1], rab[2] - im.t[2]};
+                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (r <= 1e-12 || r > EW_SR_CUT) continue;
+                double sv, sdv;
+                ko_short(r, eta, sv, sdv);
+                v += sv;
+            }
+            gam[(size_t)i * nsh + j] = gam[(size_t)j * nsh + i] = v;
+        }
+    }
+}
+
+// pbc/gradient.rs:548-603, :679-779: gradient of 1/2 q^T Gamma q at fixed
+// charges (Ewald 1/R with atomic charges, the reciprocal eta^-2 R^-3 part with
+// shell charges, the real-space R^-3 and short-range parts).
+void ewald_gradient(const Basis& b, const Geo& g, const Lattice& L, const std::vector<double>& q,
+                    const std::vector<double>& qat, double* grad) {
+    const int nat = g.nat, nsh = (int)b.sh.size();
+    double alpha = ewald_alpha(L);
+    double rcut = EW_TAU / alpha, gcut = 2.0 * alpha * EW_TAU;
+    std::vector<Image> ims = images(L, rcut), gs = recip_within(L, gcut);
+    double inv4a2 = 1.0 / (4.0 * alpha * alpha), fpv = 4.0 * PI / L.vol, tas = 2.0 * alpha / SQRTPI;
+    for (int a = 0; a < nat; ++a)
+        for (int c = 0; c < nat; ++c) {
+            double qq = qat[a] * qat[c];
+            if (qq == 0.0) continue;
+            for (const Image& im : ims) {
+                double d[3];
+                for (int k = 0; k < 3; ++k) d[k] = g.x[3 * a + k] - g.x[3 * c + k] - im.t[k];
+                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (r <= 1e-12 || r > rcut) continue;
+                double dg = -std::erfc(alpha * r) / (r * r) - tas * std::exp(-alpha * alpha * r * r) / r;
+                for (int k = 0; k < 3; ++k) grad[3 * a + k] += d[k] * (qq * dg / r);
+            }
+        }
+    for (const Image& G : gs) {
+        double g2 = G.t[0] * G.t[0] + G.t[1] * G.t[1] + G.t[2] * G.t[2];
+        double wg = std::exp(-g2 * inv4a2) / g2, sc = 0.0, ss = 0.0;
+        for (int c = 0; c < nat; ++c) {
+            double ph = G.t[0] * g.x[3 * c] + G.t[1] * g.x[3 * c + 1] + G.t[2] * g.x[3 * c + 2];
+            sc += qat[c] * std::cos(ph);
+            ss += qat[c] * std::sin(ph);
+        }
+        for (int c = 0; c < nat; ++c) {
+            double ph = G.t[0] * g.x[3 * c] + G.t[1] * g.x[3 * c + 1] + G.t[2] * g.x[3 * c + 2];
+            double f = std::sin(ph) * sc - std::cos(ph) * ss;
+            for (int k = 0; k < 3; ++k) grad[3 * c + k] -= G.t[k] * (fpv * wg * qat[c] * f);
+        }
+    }
+    // reciprocal eta^-2 R^-3 part, per shell pair
+    double pref0 = -2.0 * EW_R3 * PI / L.vol;
+    std::vector<double> ph(nsh);
+    for (const Image& G : gs) {
+        double g2 = G.t[0] * G.t[0] + G.t[1] * G.t[1] + G.t[2] * G.t[2];
+        double coeff = pref0 * expint_e1(g2 * inv4a2);
+        for (int i = 0; i < nsh; ++i) {
+            int a = b.sh[i].atom;
+            ph[i] = G.t[0] * g.x[3 * a] + G.t[1] * g.x[3 * a + 1] + G.t[2] * g.x[3 * a + 2];
+        }
+        for (int i = 0; i < nsh; ++i) {
+            if (q[i] == 0.0) continue;
+            double f = 0.0;
+            for (int j = 0; j < nsh; ++j) {
+                if (q[j] == 0.0) continue;
+                double eta = 2.0 / (1.0 / b.sh[i].hardness + 1.0 / b.sh[j].hardness);
+                f += q[j] * std::sin(ph[i] - ph[j]) / (eta * eta);
+            }
+            int a = b.sh[i].atom;
+            for (int k = 0; k < 3; ++k) grad[3 * a + k] += G.t[k] * (coeff * q[i] * f);
+        }
+    }
+    // real-space R^-3 and short-range remainder, per shell pair and image
+    double r3cut = rcut, realcut = std::max(r3cut, EW_SR_CUT);
+    std::vector<Image> ri = images(L, realcut);
+    for (int i = 0; i < nsh; ++i) {
+        int ai = b.sh[i].atom;
+        for (int j = 0; j < nsh; ++j) {
+            int aj = b.sh[j].atom;
+            double eta = 2.0 / (1.0 / b.sh[i].hardness + 1.0 / b.sh[j].hardness);
+            double qq = q[i] * q[j];
+            if (qq == 0.0) continue;
+            for (const Image& im : ri) {
+                if (ai == aj && is_origin(im)) continue;
+                double d[3];
+                for (int k = 0; k < 3; ++k) d[k] = g.x[3 * ai + k] - g.x[3 * aj + k] - im.t[k];
+                double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (r <= 1e-12 || r > realcut) continue;
+                double rad = 0.0, v, dv;
+                if (r <= r3cut) { r3_real(r, eta, alpha, v, dv); rad += EW_R3 * dv; }
+                if (r <= EW_SR_CUT) { ko_short(r, eta, v, dv); rad += dv; }
+                for (int k = 0; k < 3; ++k) grad[3 * ai + k] += d[k] * (qq * rad / r);
+            }
+        }
+    }
+}
+
+// pbc/bloch.rs:58-236 BlochBuilder::build and h_s_gamma_real: S(Gamma) and
+// H0(Gamma) as sums over every image (30 bohr AO cutoff, pbc/mod.rs:170), the
+// same-site block at T = 0 taking the on-site formula, every other
+// (atom, image) pair the off-site one at its image distance; atom pairs beyond
+// the overlap screen r^2 e_a e_b > 40 (e_a + e_b) (e = the atom's smallest
+// primitive exponent) are skipped; finally symmetrised.
+const double PBC_AO_CUT = 30.0;
+template <typename HS>
+void bloch_gamma(const Basis& b, const Geo& g, const Lattice& L, const std::vector<double>& se, HS hscale,
+                 std::vector<double>& S, std::vector<double>& H0) {
+    const int n = (int)b.ao.size(), nat = g.nat;
+    std::vector<double> emin(nat, INFINITY);
+    for (auto& a : b.ao)
+        for (auto& p : b.sh[a.shell].prims) emin[a.atom] = std::min(emin[a.atom], p.a);
+    std::vector<std::vector<int>> aos(nat);
+    for (int i = 0; i < n; ++i) aos[b.ao[i].atom].push_back(i);
+    S.assign((size_t)n * n, 0.0);
+    H0.assign((size_t)n * n, 0.0);
+    double c2 = PBC_AO_CUT * PBC_AO_CUT;
+    for (const Image& im : images(L, PBC_AO_CUT)) {
+        bool orig = is_origin(im);
+        for (int a = 0; a < nat; ++a)
+            for (int bb = 0; bb < nat; ++bb) {
+                V3 ra = g.pos(a);
+                V3 rb = {g.x[3 * bb] + im.t[0], g.x[3 * bb + 1] + im.t[1], g.x[3 * bb + 2] + im.t[2]};
+                double dx = ra.x - rb.x, dy = ra.y - rb.y, dz = ra.z - rb.z;
+                double r2 = dx * dx + dy * dy + dz * dz;
+                bool same = orig && a == bb;
+                if (!same) {
+                    if (r2 > c2) continue;
+                    if (r2 * emin[a] * emin[bb] > 40.0 * (emin[a] + emin[bb])) continue;
+                }
+                double r = std::sqrt(r2);
+                for (int mu : aos[a])
+                    for (int nu : aos[bb]) {
+                        const Shell &si = b.sh[b.ao[mu].shell], &sj = b.sh[b.ao[nu].shell];
+                        double s = overlap_pair(b.ao[mu], b.ao[nu], si.prims, sj.prims, ra, rb);
+                        if (s == 0.0) continue;
+                        double hij = 0.5 * (se[b.ao[mu].shell] + se[b.ao[nu].shell]);
+                        if (!same) {
+                            double rr = si.rad + sj.rad > 0.0 ? std::sqrt(r / (si.rad + sj.rad)) : 0.0;
+                            hij *= hscale(si, sj) * (1.0 + si.poly * rr) * (1.0 + sj.poly * rr);
+                        }
+                        S[(size_t)mu * n + nu] += s;
+                        H0[(size_t)mu * n + nu] += s * hij;
+                    }
+            }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            double s = 0.5 * (S[(size_t)i * n + j] + S[(size_t)j * n + i]);
+            double h = 0.5 * (H0[(size_t)i * n + j] + H0[(size_t)j * n + i]);
+            S[(size_t)i * n + j] = S[(size_t)j * n + i] = s;
+            H0[(size_t)i * n + j] = H0[(size_t)j * n + i] = h;
+        }
+}
+
+// hamiltonian.rs:140-166 hscale: valence pairs k_ll' * pair * (1 + enscale
+// 0.01 dEN^2); one valence shell 1/2 (k_ll + kdiff); none kdiff.
+double hscale_fn(const Ctx& ctx, const Shell& si, const Shell& sj) {
+    double kdiff = ctx.p.global("kdiff", 2.85);
+    auto kpair = [&](int li, int lj) {
+        if ((li == 0 && lj == 1) || (li == 1 && lj == 0)) return ctx.p.global("ksp", 2.08);
+        return 0.5 * (ctx.p.k_shell(li) + ctx.p.k_shell(lj));
+    };
+    if (si.valence && sj.valence) {
+        double den = PAULING_EN[si.z] - PAULING_EN[sj.z];
+        den *= den;
+        double ens = ctx.p.global("enscale", -0.7) * 0.01;
+        return ctx.p.pair_scaling(si.z, sj.z) * kpair(si.l, sj.l) * (1.0 + ens * den);
+    }
+    if (si.valence) return 0.5 * (kpair(si.l, si.l) + kdiff);
+    if (sj.valence) return 0.5 * (kpair(sj.l, sj.l) + kdiff);
+    return kdiff;
+}
+
+// S and H0 (integrals.rs:133-242, hamiltonian.rs:91-134) for molecules (pairs
+// within icut), or S(Gamma), H0(Gamma) for a cell (pbc/bloch.rs:58-236);
+// se / dsedcn: the CN-shifted self energies and their CN derivatives.
+void s_h0(const Ctx& ctx, const Basis& b, const Geo& g, const Lattice* L, double icut, const std::vector<double>& cn,
+          std::vector<double>& S, std::vector<double>& H0, std::vector<double>& se, std::vector<double>& dsedcn) {
+    const int n = (int)b.ao.size(), nsh = (int)b.sh.size();
+    auto hscale = [&](const Shell& si, const Shell& sj) { return hscale_fn(ctx, si, sj); };
+    se.assign(nsh, 0.0);
+    dsedcn.assign(nsh, 0.0);
+    for (int s = 0; s < nsh; ++s) {
+        dsedcn[s] = -b.sh[s].kcn;
+        se[s] = b.sh[s].hdiag - b.sh[s].kcn * cn[b.sh[s].atom];
+    }
+    if (L) {
+        bloch_gamma(b, g, *L, se, hscale, S, H0);
+        return;
+    }
+    S.assign((size_t)n * n, 0.0);
+    H0.assign((size_t)n * n, 0.0);
+    double ic2 = icut * icut;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const AO &A = b.ao[i], &B = b.ao[j];
+            if (A.atom != B.atom) {
+                double r2 = g.r2(A.atom, B.atom);
+                if (r2 <= PAIR_EPS2 || r2 > ic2) continue;
+            }
+            S[(size_t)i * n + j] =
+                overlap_pair(A, B, b.sh[A.shell].prims, b.sh[B.shell].prims, g.pos(A.atom), g.pos(B.atom));
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const Shell &si = b.sh[b.ao[i].shell], &sj = b.sh[b.ao[j].shell];
+            double hij;
+            if (si.atom == sj.atom) {
+                hij = 0.5 * (se[b.ao[i].shell] + se[b.ao[j].shell]);
+            } else {
+                double r = std::sqrt(g.r2(si.atom, sj.atom));
+                double rr = std::sqrt(r / (si.rad + sj.rad));
+                hij = 0.5 * (se[b.ao[i].shell] + se[b.ao[j].shell]) * hscale(si, sj) * (1.0 + si.poly * rr) *
+                      (1.0 + sj.poly * rr);
+            }
+            double v = S[(size_t)i * n + j] * hij;
+            H0[(size_t)i * n + j] = v;
+            H0[(size_t)j * n + i] = v;
+        }
+}
+
+// pbc/gradient.rs:250-408 band_and_cn_gradient at k = 0 (P(T) = P, W(T) = W
+// for every image), then the electrostatic part (:548-603).  Pairs: the origin
+// cell with a < b, every canonical-positive image with every (a, b).
+template <typename HS>
+void periodic_gradient(const Basis& b, const Geo& g, const Lattice& L, bool cn_on, double cn_cutoff, HS hscale,
+                       const std::vector<double>& cn, const std::vector<double>& vsh, const std::vector<double>& q,
+                       const std::vector<double>& qat, const std::vector<double>& P, const std::vector<double>& W,
+                       double* grad) {
+    const int n = (int)b.ao.size(), nat = g.nat, nsh = (int)b.sh.size();
+    std::vector<double> sev(nsh), dse(nsh), vao(n), dedcn(nat, 0.0);
+    for (int s = 0; s < nsh; ++s) {
+        dse[s] = cn_on ? -b.sh[s].kcn : 0.0;
+        sev[s] = b.sh[s].hdiag + dse[s] * cn[b.sh[s].atom];
+    }
+    for (int i = 0; i < n; ++i) vao[i] = vsh[b.ao[i].shell];
+    std::vector<double> emin(nat, INFINITY);
+    for (auto& a : b.ao)
+        for (auto& p : b.sh[a.shell].prims) emin[a.atom] = std::min(emin[a.atom], p.a);
+    std::vector<std::vector<int>> aos(nat);
+    for (int i = 0; i < n; ++i) aos[b.ao[i].atom].push_back(i);
+    double c2 = PBC_AO_CUT * PBC_AO_CUT;
+    for (const Image& im : images(L, PBC_AO_CUT)) {
+        bool orig = is_origin(im);
+        if (!orig && !canonical_positive(im)) continue;
+        for (int a = 0; a < nat; ++a)
+            for (int bb = 0; bb < nat; ++bb) {
+                if (orig && a >= bb) continue;
+                V3 ra = g.pos(a);
+                V3 rb = {g.x[3 * bb] + im.t[0], g.x[3 * bb + 1] + im.t[1], g.x[3 * bb + 2] + im.t[2]};
+                double rv[3] = {ra.x - rb.x, ra.y - rb.y, ra.z - rb.z};
+                double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                if (r2 <= 1e-12 || r2 > c2) continue;
+                if (r2 * emin[a] * emin[bb] > 40.0 * (emin[a] + emin[bb])) continue;
+                double ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, dca = 0, dcb = 0;
+                for (int mu : aos[a])
+                    for (int nu : aos[bb]) {
+                        int smu = b.ao[mu].shell, snu = b.ao[nu].shell;
+                        const Shell &SM = b.sh[smu], &SN = b.sh[snu];
+                        double rr = std::sqrt(std::sqrt(r2) / (SM.rad + SN.rad));
+                        double fi = 1.0 + SM.poly * rr, fj = 1.0 + SN.poly * rr;
+                        double hs = hscale(SM, SN) * fi * fj;
+                        double hij = 0.5 * (sev[smu] + sev[snu]) * hs;
+                        double dlp[3] = {0, 0, 0};
+                        if (std::fabs(fi * fj) > 1e-18) {  // pbc/gradient.rs:781-806
+                            double dp = (fi * SN.poly + fj * SM.poly) * 0.5 * rr / r2;
+                            for (int k = 0; k < 3; ++k) dlp[k] = rv[k] * (dp / (fi * fj));
+                        }
+                        double s, da[3], db[3];
+                        overlap_pair_deriv(b.ao[mu], b.ao[nu], SM.prims, SN.prims, ra, rb, s, da, db);
+                        double p = P[(size_t)mu * n + nu], w = W[(size_t)mu * n + nu];
+                        double oc = p * (2.0 * hij - (vao[mu] + vao[nu])) - 2.0 * w;
+                        for (int k = 0; k < 3; ++k) {
+                            ga[k] += da[k] * oc;
+                            gb[k] += db[k] * oc;
+                            double pg = dlp[k] * (2.0 * p * hij * s);
+                            ga[k] += pg;
+                            gb[k] -= pg;
+                        }
+                        if (cn_on) {
+                            dca += dse[smu] * hs * p * s;
+                            dcb += dse[snu] * hs * p * s;
+                        }
+                    }
+                for (int k = 0; k < 3; ++k) {
+                    grad[3 * a + k] += ga[k];
+                    grad[3 * bb + k] += gb[k];
+                }
+                dedcn[a] += dca;
+                dedcn[bb] += dcb;
+            }
+    }
+    if (cn_on) {
+        for (int s = 0; s < nsh; ++s)
+            for (int i = b.sh[s].first_ao; i < b.sh[s].first_ao + b.sh[s].nao; ++i)
+                dedcn[b.sh[s].atom] += dse[s] * P[(size_t)i * n + i];
+        std::vector<double> cn2;
+        std::vector<CnPair> cp;
+        coordination_pbc(g, L, cn_cutoff, cn2, &cp);
+        cn_chain_pbc(cp, dedcn, grad, 1e-12);
+    }
+    ewald_gradient(b, g, L, q, qat, grad);
+}
+
 extern "C" {
 
 typedef struct {
@@ -1409,9 +2199,14 @@ struct Step {
 extern "C" {
 
 // run_electronic (electronic.rs:800-2031) followed by analytic_gradient_from_result
-// (gradient.rs:122-242), non-periodic, stock model, restricted.
-int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracle_options* opt,
-               const double* initial_shell_charges, oracle_result* res) {
+// (gradient.rs:122-242), stock model, restricted.  With cell != NULL (three
+// cell vectors, bohr, row after row) the periodic Gamma-point path that
+// run_electronic / analytic_gradient dispatch a lattice to
+// (electronic.rs:829-832, gradient.rs:61-66): run_pbc_scc (pbc/scf.rs:140-580),
+// pbc_electronic_result (:602-677) and pbc_gradient_from_scc
+// (pbc/gradient.rs:75-163).
+int oracle_run_cell(void* vctx, int nat, const int* z, const double* xyz, const double* cell,
+                    const oracle_options* opt, const double* initial_shell_charges, oracle_result* res) {
     Ctx& ctx = *(Ctx*)vctx;
     Err err{0, ""};
     auto fail = [&](int code, const std::string& m) {
@@ -1423,6 +2218,14 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
     Geo g{nat, z, xyz};
     for (int i = 0; i < 3 * nat; ++i)
         if (!std::isfinite(xyz[i])) return fail(4, "non-finite coordinate");
+    Lattice lat;
+    const Lattice* L = nullptr;
+    if (cell) {
+        for (int i = 0; i < 9; ++i)
+            if (!std::isfinite(cell[i])) return fail(4, "non-finite cell vector");
+        if (!make_lattice(cell, lat)) return fail(7, "singular cell");
+        L = &lat;
+    }
     const int n = (int)b.ao.size(), nsh = (int)b.sh.size();
     if (n == 0) return fail(4, "cannot run GFN1 SCC for an empty basis");
     double nelec = b.nref - opt->charge;
@@ -1432,60 +2235,19 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
     // ---- CN and H0 (hamiltonian.rs:44-134)
     std::vector<double> cn(nat, 0.0);
     std::vector<CnPair> cnpairs;
-    if (opt->enable_cn_hamiltonian) coordination(g, opt->coordination_cutoff, cn, &cnpairs);
-    std::vector<double> S((size_t)n * n, 0.0), H0((size_t)n * n, 0.0);
-    double ic2 = opt->integral_cutoff * opt->integral_cutoff;
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            const AO &A = b.ao[i], &B = b.ao[j];
-            if (A.atom != B.atom) {
-                double r2 = g.r2(A.atom, B.atom);
-                if (r2 <= PAIR_EPS2 || r2 > ic2) continue;
-            }
-            S[(size_t)i * n + j] =
-                overlap_pair(A, B, b.sh[A.shell].prims, b.sh[B.shell].prims, g.pos(A.atom), g.pos(B.atom));
-        }
-    std::vector<double> se(nsh), dsedcn(nsh);
-    for (int s = 0; s < nsh; ++s) {
-        dsedcn[s] = -b.sh[s].kcn;
-        se[s] = b.sh[s].hdiag - b.sh[s].kcn * cn[b.sh[s].atom];
+    if (opt->enable_cn_hamiltonian) {
+        if (L) coordination_pbc(g, *L, opt->coordination_cutoff, cn, &cnpairs);
+        else coordination(g, opt->coordination_cutoff, cn, &cnpairs);
     }
-    auto hscale = [&](const Shell& si, const Shell& sj) {
-        double kdiff = ctx.p.global("kdiff", 2.85);
-        auto kpair = [&](int li, int lj) {
-            if ((li == 0 && lj == 1) || (li == 1 && lj == 0)) return ctx.p.global("ksp", 2.08);
-            return 0.5 * (ctx.p.k_shell(li) + ctx.p.k_shell(lj));
-        };
-        if (si.valence && sj.valence) {
-            double den = PAULING_EN[si.z] - PAULING_EN[sj.z];
-            den *= den;
-            double ens = ctx.p.global("enscale", -0.7) * 0.01;
-            return ctx.p.pair_scaling(si.z, sj.z) * kpair(si.l, sj.l) * (1.0 + ens * den);
-        }
-        if (si.valence) return 0.5 * (kpair(si.l, si.l) + kdiff);
-        if (sj.valence) return 0.5 * (kpair(sj.l, sj.l) + kdiff);
-        return kdiff;
-    };
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= i; ++j) {
-            const Shell &si = b.sh[b.ao[i].shell], &sj = b.sh[b.ao[j].shell];
-            double hij;
-            if (si.atom == sj.atom) {
-                hij = 0.5 * (se[b.ao[i].shell] + se[b.ao[j].shell]);
-            } else {
-                double r = std::sqrt(g.r2(si.atom, sj.atom));
-                double rr = std::sqrt(r / (si.rad + sj.rad));
-                hij = 0.5 * (se[b.ao[i].shell] + se[b.ao[j].shell]) * hscale(si, sj) * (1.0 + si.poly * rr) *
-                      (1.0 + sj.poly * rr);
-            }
-            double v = S[(size_t)i * n + j] * hij;
-            H0[(size_t)i * n + j] = v;
-            H0[(size_t)j * n + i] = v;
-        }
+    std::vector<double> S, H0, se, dsedcn;
+    auto hscale = [&](const Shell& si, const Shell& sj) { return hscale_fn(ctx, si, sj); };
+    s_h0(ctx, b, g, L, opt->integral_cutoff, cn, S, H0, se, dsedcn);
+    double ic2 = opt->integral_cutoff * opt->integral_cutoff;
 
     // ---- gamma (coulomb.rs:82-107)
     std::vector<double> gam;
-    gamma_matrix(b, g, gam);
+    if (L) ewald_gamma_matrix(b, g, *L, gam);  // pbc/ewald.rs:253-320
+    else gamma_matrix(b, g, gam);
 
     // ---- Lowdin orthogonaliser (linalg.rs:193-217)
     std::vector<double> X((size_t)n * n), Xt((size_t)n * n);
@@ -1503,9 +2265,9 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
     }
 
     // ---- geometry-only energies
-    double erep = repulsion(ctx, g, nullptr);
-    double edisp = opt->enable_dispersion ? dispersion(ctx, g, nullptr) : 0.0;
-    double ehal = halogen(ctx, g, nullptr);
+    double erep = L ? repulsion_pbc(ctx, g, *L, nullptr) : repulsion(ctx, g, nullptr);
+    double edisp = !opt->enable_dispersion ? 0.0 : L ? dispersion_pbc(ctx, g, *L, nullptr) : dispersion(ctx, g, nullptr);
+    double ehal = L ? halogen_pbc(ctx, g, *L, nullptr) : halogen(ctx, g, nullptr);
 
     // ---- SCC helpers
     std::vector<int> atom_first_sh(nat, -1);
@@ -1636,7 +2398,10 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
     double final_rms = INFINITY, used_T = opt->electronic_temperature;
     Step last;
     bool have_last = false;
-    for (int rung = 0; rung < 4 && !converged; ++rung) {
+    // the periodic SCC runs the caller's scheme once, with no level shift and
+    // no fallback (pbc/scf.rs:358-529)
+    if (L) ladder[0].ls = 0.0;
+    for (int rung = 0; rung < (L ? 1 : 4) && !converged; ++rung) {
         const Rung& R = ladder[rung];
         used_T = R.T;
         std::vector<double> qmix = q0;
@@ -1726,7 +2491,13 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
 
     // ---- gradient (gradient.rs:122-531)
     std::vector<double> gel(3 * nat, 0.0), grep(3 * nat, 0.0), gdisp(3 * nat, 0.0), ghal(3 * nat, 0.0);
-    if (opt->include_hamiltonian || opt->include_scc) {
+    if (L) {
+        periodic_gradient(b, g, *L, opt->enable_cn_hamiltonian != 0, opt->coordination_cutoff, hscale, cn, vsh, last.q, qat,
+                          last.P, last.W, gel.data());
+        repulsion_pbc(ctx, g, *L, grep.data());
+        if (opt->enable_dispersion) dispersion_pbc(ctx, g, *L, gdisp.data());
+        halogen_pbc(ctx, g, *L, ghal.data());
+    } else if (opt->include_hamiltonian || opt->include_scc) {
         bool cn_on = opt->enable_cn_hamiltonian != 0, scc_on = opt->include_scc != 0;
         std::vector<double> dedcn(nat, 0.0), sev(nsh), dse(nsh);
         for (int s = 0; s < nsh; ++s) {
@@ -1805,7 +2576,7 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
             cn_chain(cp2, dedcn, gel.data(), 1e-12);
         }
     }
-    if (opt->include_scc) {  // gradient.rs:493-516
+    if (opt->include_scc && !L) {  // gradient.rs:493-516
         for (int i = 0; i < nsh; ++i) {
             int ai = b.sh[i].atom;
             for (int j = 0; j < i; ++j) {
@@ -1825,9 +2596,9 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
             }
         }
     }
-    if (opt->include_repulsion) repulsion(ctx, g, grep.data());
-    if (opt->include_dispersion && opt->enable_dispersion) dispersion(ctx, g, gdisp.data());
-    if (opt->include_halogen) halogen(ctx, g, ghal.data());
+    if (opt->include_repulsion && !L) repulsion(ctx, g, grep.data());
+    if (opt->include_dispersion && opt->enable_dispersion && !L) dispersion(ctx, g, gdisp.data());
+    if (opt->include_halogen && !L) halogen(ctx, g, ghal.data());
     std::vector<double> gt(3 * nat);
     for (int k = 0; k < 3 * nat; ++k) gt[k] = gel[k] + grep[k] + gdisp[k] + ghal[k];
     put(res->gradient, gt);
@@ -1835,6 +2606,107 @@ int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracl
     put(res->repulsion_gradient, grep);
     put(res->dispersion_gradient, gdisp);
     put(res->halogen_gradient, ghal);
+    return 0;
+}
+
+int oracle_run(void* vctx, int nat, const int* z, const double* xyz, const oracle_options* opt,
+               const double* initial_shell_charges, oracle_result* res) {
+    return oracle_run_cell(vctx, nat, z, xyz, nullptr, opt, initial_shell_charges, res);
+}
+
+// ---- pieces of the periodic path, for the tests that pin them
+// Ewald matrix of unit point charges (nat x nat) at splitting alpha (<= 0: the default).
+int oracle_ewald_atoms(int nat, const double* xyz, const double* cell, double alpha, double* out) {
+    Lattice L;
+    if (!make_lattice(cell, L)) return 7;
+    std::vector<int> z(nat, 1);
+    Geo g{nat, z.data(), xyz};
+    std::vector<double> phi;
+    ewald_atom_matrix(g, L, alpha > 0.0 ? alpha : ewald_alpha(L), phi);
+    std::copy(phi.begin(), phi.end(), out);
+    return 0;
+}
+
+// Periodic shell Coulomb matrix Gamma (nsh x nsh) at splitting alpha (<= 0: the default).
+int oracle_gamma_cell(void* vctx, int nat, const int* z, const double* xyz, const double* cell, double alpha,
+                      double* out) {
+    Basis b;
+    Err err{0, ""};
+    if (!build_basis(*(Ctx*)vctx, nat, z, 0, b, err)) return err.code;
+    Lattice L;
+    if (!make_lattice(cell, L)) return 7;
+    Geo g{nat, z, xyz};
+    std::vector<double> gam;
+    ewald_gamma_matrix(b, g, L, gam, alpha);
+    std::copy(gam.begin(), gam.end(), out);
+    return 0;
+}
+
+// Gradient of 1/2 q^T Gamma q at fixed shell charges q (3 nat).
+int oracle_gamma_cell_gradient(void* vctx, int nat, const int* z, const double* xyz, const double* cell,
+                               const double* q, double* grad) {
+    Basis b;
+    Err err{0, ""};
+    if (!build_basis(*(Ctx*)vctx, nat, z, 0, b, err)) return err.code;
+    Lattice L;
+    if (!make_lattice(cell, L)) return 7;
+    Geo g{nat, z, xyz};
+    std::vector<double> qs(q, q + b.sh.size()), qat(nat, 0.0);
+    for (size_t s = 0; s < b.sh.size(); ++s) qat[b.sh[s].atom] += qs[s];
+    std::fill(grad, grad + 3 * nat, 0.0);
+    ewald_gradient(b, g, L, qs, qat, grad);
+    return 0;
+}
+
+// Periodic classical terms: which = 0 repulsion, 1 dispersion, 2 halogen; grad nullable.
+double oracle_classical_cell(void* vctx, int nat, const int* z, const double* xyz, const double* cell, int which,
+                             double* grad) {
+    Ctx& ctx = *(Ctx*)vctx;
+    Lattice L;
+    if (!make_lattice(cell, L)) return NAN;
+    Geo g{nat, z, xyz};
+    if (grad) std::fill(grad, grad + 3 * nat, 0.0);
+    if (which == 0) return repulsion_pbc(ctx, g, L, grad);
+    if (which == 1) return dispersion_pbc(ctx, g, L, grad);
+    return halogen_pbc(ctx, g, L, grad);
+}
+
+// S and H0 of a molecule (cell NULL, pairs within 30 bohr) or S(Gamma),
+// H0(Gamma) of a cell, with or without the CN shift of the self energies.
+int oracle_hs(void* vctx, int nat, const int* z, const double* xyz, const double* cell, int cn_on, double* S,
+              double* H0) {
+    Ctx& ctx = *(Ctx*)vctx;
+    Basis b;
+    Err err{0, ""};
+    if (!build_basis(ctx, nat, z, 0, b, err)) return err.code;
+    Geo g{nat, z, xyz};
+    Lattice lat;
+    const Lattice* L = nullptr;
+    if (cell) {
+        if (!make_lattice(cell, lat)) return 7;
+        L = &lat;
+    }
+    std::vector<double> cn(nat, 0.0);
+    if (cn_on) {
+        if (L) coordination_pbc(g, *L, 30.0, cn, nullptr);
+        else coordination(g, 30.0, cn, nullptr);
+    }
+    std::vector<double> s, h, se, ds;
+    s_h0(ctx, b, g, L, 30.0, cn, s, h, se, ds);
+    std::copy(s.begin(), s.end(), S);
+    std::copy(h.begin(), h.end(), H0);
+    return 0;
+}
+
+// Lattice-summed coordination numbers.
+int oracle_coordination_cell(int nat, const int* z, const double* xyz, const double* cell, double cutoff,
+                             double* cn) {
+    Lattice L;
+    if (!make_lattice(cell, L)) return 7;
+    Geo g{nat, z, xyz};
+    std::vector<double> c;
+    coordination_pbc(g, L, cutoff, c, nullptr);
+    std::copy(c.begin(), c.end(), cn);
     return 0;
 }
 

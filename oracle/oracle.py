@@ -100,6 +100,9 @@ def lib():
             f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.oracle_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.POINTER(Options), ctypes.c_void_p, ctypes.POINTER(Result)]
+        _lib.oracle_run_cell.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.POINTER(Options), ctypes.c_void_p,
+                                         ctypes.POINTER(Result)]
         _lib.oracle_basis_size.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib.oracle_overlap.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
@@ -157,9 +160,11 @@ def basis_size(numbers, param_file=None):
     return nsh.value, nao.value
 
 
-def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False, param_file=None, **opts):
+def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False, param_file=None, cell=None,
+        **opts):
     """Energy (+ gradient) of one system; returns a dict of numpy arrays.
-    param_file: another GFN1 parameter file (default: the bundled one)."""
+    param_file: another GFN1 parameter file (default: the bundled one).
+    cell: 3 x 3 cell vectors in bohr (rows) for the periodic Gamma-point path."""
     z = np.ascontiguousarray(numbers, dtype=np.int32)
     x = np.ascontiguousarray(positions_bohr, dtype=np.float64).reshape(-1, 3)
     nat = len(z)
@@ -183,8 +188,9 @@ def run(numbers, positions_bohr, initial_shell_charges=None, want_matrices=False
     if initial_shell_charges is not None:
         q0 = np.ascontiguousarray(initial_shell_charges, dtype=np.float64)
         assert q0.shape == (nsh,)
-    rc = lib().oracle_run(ctx(param_file), nat, _ptr(z), _ptr(x), ctypes.byref(o),
-                          _ptr(q0) if q0 is not None else None, ctypes.byref(r))
+    c = None if cell is None else np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+    rc = lib().oracle_run_cell(ctx(param_file), nat, _ptr(z), _ptr(x), _ptr(c) if c is not None else None,
+                               ctypes.byref(o), _ptr(q0) if q0 is not None else None, ctypes.byref(r))
     out["status"] = rc
     out["message"] = r.message.decode()
     out["energies"] = dict(zip(ENERGY_NAMES, list(r.energies)))
@@ -320,3 +326,95 @@ def gamma(numbers, positions_bohr, param_file=None):
     if rc:
         raise ValueError("basis build failed (%d)" % rc)
     return out
+
+
+# ---- pieces of the periodic Gamma-point path
+
+def _cellarr(cell):
+    return np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+
+
+def ewald_atoms(positions_bohr, cell, alpha=0.0):
+    """Ewald (1/R) potential matrix of unit point charges and their images,
+    with the neutralising background and the self term (nat x nat)."""
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64).reshape(-1, 3)
+    c = _cellarr(cell)
+    out = np.zeros((len(x), len(x)))
+    f = lib().oracle_ewald_atoms
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    if f(len(x), _ptr(x), _ptr(c), alpha, _ptr(out)):
+        raise ValueError("singular cell")
+    return out
+
+
+def gamma_cell(numbers, positions_bohr, cell, alpha=0.0):
+    """Periodic shell Coulomb matrix of the Gamma-point SCC (nsh x nsh)."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    c = _cellarr(cell)
+    nsh, _ = basis_size(z)
+    out = np.zeros((nsh, nsh))
+    f = lib().oracle_gamma_cell
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_double, ctypes.c_void_p]
+    if f(ctx(), len(z), _ptr(z), _ptr(x), _ptr(c), alpha, _ptr(out)):
+        raise ValueError("gamma_cell failed")
+    return out
+
+
+def gamma_cell_gradient(numbers, positions_bohr, cell, q):
+    """Gradient of 1/2 q^T Gamma q at fixed shell charges (nat x 3)."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    c = _cellarr(cell)
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    g = np.zeros((len(z), 3))
+    f = lib().oracle_gamma_cell_gradient
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
+    if f(ctx(), len(z), _ptr(z), _ptr(x), _ptr(c), _ptr(qq), _ptr(g)):
+        raise ValueError("gamma_cell_gradient failed")
+    return g
+
+
+def classical_cell(numbers, positions_bohr, cell, which):
+    """Periodic repulsion / dispersion / halogen energy and gradient."""
+    idx = {"repulsion": 0, "dispersion": 1, "halogen": 2}[which]
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    c = _cellarr(cell)
+    g = np.zeros((len(z), 3))
+    f = lib().oracle_classical_cell
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_void_p]
+    e = f(ctx(), len(z), _ptr(z), _ptr(x), _ptr(c), idx, _ptr(g))
+    return e, g
+
+
+def coordination_cell(numbers, positions_bohr, cell, cutoff=30.0):
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    c = _cellarr(cell)
+    cn = np.zeros(len(z))
+    f = lib().oracle_coordination_cell
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    if f(len(z), _ptr(z), _ptr(x), _ptr(c), cutoff, _ptr(cn)):
+        raise ValueError("singular cell")
+    return cn
+
+
+def hs(numbers, positions_bohr, cell=None, cn_on=True):
+    """S and H0 of a molecule (pairs within 30 bohr), or S(Gamma) and H0(Gamma)
+    of a cell, with or without the CN shift of the self energies."""
+    z = np.ascontiguousarray(numbers, dtype=np.int32)
+    x = np.ascontiguousarray(positions_bohr, dtype=np.float64)
+    _, nao = basis_size(z)
+    S = np.zeros((nao, nao))
+    H = np.zeros((nao, nao))
+    c = None if cell is None else _cellarr(cell)
+    f = lib().oracle_hs
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                  ctypes.c_void_p, ctypes.c_void_p]
+    if f(ctx(), len(z), _ptr(z), _ptr(x), _ptr(c) if c is not None else None, int(bool(cn_on)), _ptr(S), _ptr(H)):
+        raise ValueError("hs failed")
+    return S, H

@@ -1,0 +1,237 @@
+"""Pins of the oracle's periodic Gamma-point restatement (the periodic section of
+oracle/gfn1_oracle.cpp; reference src/pbc/ewald.rs, bloch.rs, scf.rs,
+gradient.rs, src/lattice.rs, src/pairlist.rs).
+
+The reference holds no periodic golden values, so each piece is pinned by what
+the mathematics fixes, chosen so that a plausible slip fails one of them:
+- the Ewald sum: the Madelung constant of rocksalt, independence of the
+  splitting parameter (the reference's own tests, ewald.rs:338-462, neutral and
+  charged), the large-cell limit of the molecular second-order energy
+  (ewald.rs:380-416);
+- every lattice sum: a 2 x 1 x 1 supercell doubles the energy of the cell
+  (a missed or double-counted image, or a wrong self-image weight, breaks it),
+  moving one atom by a lattice vector changes nothing, a rigid rotation of
+  cell and atoms leaves the energy unchanged and rotates the gradient;
+- the Bloch sums S(Gamma), H0(Gamma): block by block equal to the sum over
+  images of the molecular overlap / H0 of the two-atom system (atom A, image
+  of atom B), a separate code path pinned by the molecular goldens;
+- the periodic gradient: central finite differences of the periodic energy,
+  and the translational sum rule;
+- the SCC driver: the level shift is not applied (pbc/scf.rs:358-529).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+A2B = 1.8897261246257702
+MADELUNG_NACL = 1.747564594633182  # rocksalt, per ion pair, nearest-neighbour distance units
+TIGHT = dict(energy_tolerance=1e-11, charge_tolerance=1e-9)
+
+
+def rocksalt(a_bohr):
+    na = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a_bohr
+    cl = na + np.array([.5, 0, 0]) * a_bohr
+    return np.vstack([na, cl]), np.eye(3) * a_bohr
+
+
+def water_box():
+    # four rigid waters in a 7 A cube: images overlap through every face
+    o = np.array([[0.0, 0.0, 0.0], [3.5, 3.4, 0.3], [0.2, 3.6, 3.4], [3.4, 0.1, 3.6]])
+    h1 = np.array([0.757, 0.586, 0.0])
+    h2 = np.array([-0.757, 0.586, 0.0])
+    rots = [np.eye(3), np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1.0]]),
+            np.array([[1, 0, 0], [0, 0, -1], [0, 1, 0.0]]), np.array([[0, 0, 1], [0, 1, 0], [-1, 0, 0.0]])]
+    z, x = [], []
+    for c, r in zip(o, rots):
+        z += [8, 1, 1]
+        x += [c, c + r @ h1, c + r @ h2]
+    return np.array(z), np.array(x) * A2B, np.eye(3) * 7.0 * A2B
+
+
+def halogen_cell():
+    # bromomethane ... ammonia (a halogen bond) in a skewed 9 A cell
+    z = np.array([6, 35, 1, 1, 1, 7, 1, 1, 1])
+    x = np.array([[0, 0, 0], [0, 0, 1.94], [1.03, 0, -0.36], [-0.51, 0.89, -0.36], [-0.51, -0.89, -0.36],
+                  [0, 0, 4.9], [0.94, 0, 5.25], [-0.47, 0.81, 5.25], [-0.47, -0.81, 5.25]]) * A2B
+    cell = np.array([[9.0, 0, 0], [0.5, 8.5, 0], [0.3, 0.2, 8.8]]) * A2B
+    return z, x, cell
+
+
+def test_madelung_constant_of_rocksalt():
+    x, cell = rocksalt(5.64 * A2B)
+    q = np.array([1.0] * 4 + [-1.0] * 4)
+    d = 0.5 * 5.64 * A2B
+    for alpha in (0.0, 0.3, 0.6):
+        e = 0.5 * q @ oracle.ewald_atoms(x, cell, alpha) @ q
+        assert abs(e / 4 + MADELUNG_NACL / d) < 1e-12
+
+
+@pytest.mark.parametrize("charges", [(0.3, -0.3), (0.4, 0.1)])
+def test_ewald_energy_independent_of_splitting(charges):
+    # ewald.rs:338-378 (neutral) and :418-462 (charged: the background term)
+    z = np.array([11, 17])
+    x = np.array([[0, 0, 0], [2.8, 2.8, 2.8]]) * A2B
+    cell = np.eye(3) * 5.6 * A2B
+    nsh, _ = oracle.basis_size(z)
+    q = np.array([charges[i % 2] for i in range(nsh)])
+    es = [0.5 * q @ oracle.gamma_cell(z, x, cell, alpha) @ q for alpha in (0.2, 0.35, 0.5, 0.0)]
+    assert max(es) - min(es) < 1e-10
+
+
+def test_large_cell_reproduces_molecular_second_order():
+    # ewald.rs:380-416: a neutral charge vector in a 40 A cell; only the
+    # fast-decaying dipole-image interaction is left
+    z = np.array([3, 1])
+    x = np.array([[0, 0, 0], [0, 0, 1.6]]) * A2B
+    nsh, _ = oracle.basis_size(z)
+    q = np.array([0.25 if i % 2 == 0 else -0.15 for i in range(nsh)])
+    q -= q.mean()
+    e_mol = 0.5 * q @ oracle.gamma(z, x) @ q
+    e40 = 0.5 * q @ oracle.gamma_cell(z, x, np.eye(3) * 40 * A2B) @ q
+    e80 = 0.5 * q @ oracle.gamma_cell(z, x, np.eye(3) * 80 * A2B) @ q
+    assert abs(e40 - e_mol) < 1e-3
+    # the image correction falls off as L^-3
+    assert abs(e80 - e_mol) < abs(e40 - e_mol) / 4
+
+
+def supercell(z, x, cell):
+    z2 = np.concatenate([z, z])
+    x2 = np.vstack([x, x + cell[0]])
+    c2 = cell.copy()
+    c2[0] *= 2
+    return z2, x2, c2
+
+
+@pytest.mark.parametrize("system", [water_box, halogen_cell])
+def test_supercell_doubles_lattice_sums(system):
+    z, x, cell = system()
+    z2, x2, c2 = supercell(z, x, cell)
+    nat = len(z)
+    for term in ("repulsion", "dispersion", "halogen"):
+        e1, g1 = oracle.classical_cell(z, x, cell, term)
+        e2, g2 = oracle.classical_cell(z2, x2, c2, term)
+        assert abs(e2 - 2 * e1) <= 1e-12 * max(1.0, abs(e1)), term
+        assert np.abs(g2[:nat] - g1).max() < 1e-12 and np.abs(g2[nat:] - g1).max() < 1e-12, term
+    if system is halogen_cell:
+        assert oracle.classical_cell(z, x, cell, "halogen")[0] < -1e-4  # the term is present
+    # second order at fixed (replicated) shell charges, and its gradient
+    nsh, _ = oracle.basis_size(z)
+    rng = np.random.default_rng(5)
+    q = rng.normal(0, 0.2, nsh)
+    q2 = np.concatenate([q, q])
+    e1 = 0.5 * q @ oracle.gamma_cell(z, x, cell) @ q
+    e2 = 0.5 * q2 @ oracle.gamma_cell(z2, x2, c2) @ q2
+    assert abs(e2 - 2 * e1) < 1e-10
+    g1 = oracle.gamma_cell_gradient(z, x, cell, q)
+    g2 = oracle.gamma_cell_gradient(z2, x2, c2, q2)
+    assert np.abs(g2[:nat] - g1).max() < 1e-10 and np.abs(g2[nat:] - g1).max() < 1e-10
+    # the CN of the supercell atoms equals that of their cell copies
+    cn1 = oracle.coordination_cell(z, x, cell)
+    cn2 = oracle.coordination_cell(z2, x2, c2)
+    assert np.abs(cn2[:nat] - cn1).max() < 1e-12 and np.abs(cn2[nat:] - cn1).max() < 1e-12
+
+
+def test_second_order_gradient_matches_finite_differences():
+    z, x, cell = halogen_cell()
+    nsh, _ = oracle.basis_size(z)
+    q = np.random.default_rng(2).normal(0, 0.2, nsh)
+    g = oracle.gamma_cell_gradient(z, x, cell, q)
+    h = 1e-5
+    for a, c in [(0, 0), (1, 2), (5, 1), (8, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[a, c] += h
+        xm[a, c] -= h
+        fd = (0.5 * q @ oracle.gamma_cell(z, xp, cell) @ q - 0.5 * q @ oracle.gamma_cell(z, xm, cell) @ q) / (2 * h)
+        assert abs(fd - g[a, c]) < 1e-8
+
+
+def test_bloch_sums_equal_explicit_image_pairs():
+    # CN shift off, so each H0 block depends on its two atoms only
+    z = np.array([8, 1, 1, 7])
+    x = np.array([[0.0, 0.0, 0.0], [1.4, 1.1, 0.0], [-1.4, 1.1, 0.2], [2.0, -3.0, 3.5]])
+    cell = np.array([[8.5, 0.0, 0.0], [1.0, 9.0, 0.0], [0.5, -0.7, 9.5]])
+    S, H = oracle.hs(z, x, cell, cn_on=False)
+    sh = oracle.shells(z)
+    ao_atom = np.repeat(sh["atom"], sh["nao"])
+    first = {a: np.flatnonzero(ao_atom == a) for a in range(len(z))}
+    Sx, Hx = np.zeros_like(S), np.zeros_like(H)
+    rng = [int(math.ceil(30.0 / 8.0)) + 1] * 3
+    for n0 in range(-rng[0], rng[0] + 1):
+        for n1 in range(-rng[1], rng[1] + 1):
+            for n2 in range(-rng[2], rng[2] + 1):
+                t = n0 * cell[0] + n1 * cell[1] + n2 * cell[2]
+                for a in range(len(z)):
+                    for b in range(len(z)):
+                        ia, ib = first[a], first[b]
+                        if a == b and n0 == n1 == n2 == 0:
+                            s1, h1 = oracle.hs(z[[a]], x[[a]], cn_on=False)
+                            Sx[np.ix_(ia, ib)] += s1
+                            Hx[np.ix_(ia, ib)] += h1
+                            continue
+                        rb = x[b] + t
+                        if np.sum((x[a] - rb) ** 2) > 30.0 ** 2:
+                            continue
+                        s2, h2 = oracle.hs(z[[a, b]], np.vstack([x[a], rb]), cn_on=False)
+                        na = len(ia)
+                        Sx[np.ix_(ia, ib)] += s2[:na, na:]
+                        Hx[np.ix_(ia, ib)] += h2[:na, na:]
+    assert np.abs(Sx - S).max() < 1e-12
+    assert np.abs(Hx - H).max() < 1e-12
+    # a cell so large that no image is within reach reproduces the molecule
+    Sm, Hm = oracle.hs(z, x, cn_on=True)
+    Sb, Hb = oracle.hs(z, x, np.eye(3) * 80.0, cn_on=True)
+    assert np.abs(Sm - Sb).max() < 1e-14 and np.abs(Hm - Hb).max() < 1e-14
+
+
+def rotation():
+    a, b = 0.7, 0.4
+    rz = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    rx = np.array([[1, 0, 0], [0, math.cos(b), -math.sin(b)], [0, math.sin(b), math.cos(b)]])
+    return rz @ rx
+
+
+@pytest.mark.parametrize("system", [water_box, halogen_cell])
+def test_periodic_energy_invariances_and_gradient(system):
+    z, x, cell = system()
+    r = oracle.run(z, x, cell=cell, **TIGHT)
+    assert r["status"] == 0
+    e, g = r["energy"], r["gradient"]
+    assert np.abs(g.sum(axis=0)).max() < 1e-12
+    # an atom moved by a lattice vector is the same crystal
+    xs = x.copy()
+    xs[1] += cell[0] - cell[2]
+    rs = oracle.run(z, xs, cell=cell, **TIGHT)
+    assert abs(rs["energy"] - e) < 1e-9 and np.abs(rs["gradient"] - g).max() < 1e-7
+    # a rigid rotation of cell and atoms
+    R = rotation()
+    rr = oracle.run(z, x @ R.T, cell=cell @ R.T, **TIGHT)
+    assert abs(rr["energy"] - e) < 1e-9 and np.abs(rr["gradient"] - g @ R.T).max() < 1e-7
+    # central differences of the periodic free energy
+    h = 1e-4
+    for a, c in [(0, 0), (1, 2), (len(z) - 1, 1)]:
+        xp, xm = x.copy(), x.copy()
+        xp[a, c] += h
+        xm[a, c] -= h
+        ep = oracle.run(z, xp, cell=cell, energy_tolerance=1e-12, charge_tolerance=1e-10)["energy"]
+        em = oracle.run(z, xm, cell=cell, energy_tolerance=1e-12, charge_tolerance=1e-10)["energy"]
+        assert abs((ep - em) / (2 * h) - g[a, c]) < 2e-8
+    # the terms add up
+    t = r["energies"]
+    parts = t["repulsion"] + t["electronic"] + t["isotropic_scc"] + t["third_order"] + t["dispersion"] + t["halogen"]
+    assert abs(parts - t["total_internal"]) < 1e-12
+
+
+def test_periodic_scc_applies_no_level_shift():
+    z, x, cell = water_box()
+    a = oracle.run(z, x, cell=cell, **TIGHT)
+    b = oracle.run(z, x, cell=cell, level_shift=0.5, **TIGHT)
+    assert a["iterations"] == b["iterations"] and a["energy"] == b["energy"]
+
+
+def test_singular_cell_is_rejected():
+    z, x, _ = water_box()
+    r = oracle.run(z, x, cell=np.array([[5.0, 0, 0], [0, 5.0, 0], [10.0, 0, 0]]))
+    assert r["status"] == 7

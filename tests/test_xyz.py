@@ -37,10 +37,27 @@ def test_units_symbols_and_line_endings(tmp_path):
     ("1\nc\nXx 0 0 0\n", "InvalidInput", "unknown element on line 3: Xx"),
     ("1\nc\n87 0 0 0\n", "InvalidInput", "unknown element"),
     ("1\nc\nH 0 zero 0\n", "Parse", "invalid floating point value: zero"),
-    ('1\nLattice="3 0 0 0 3 0 0 0 3" pbc="T T T"\nH 0 0 0\n', "InvalidInput", "periodic"),
+    ('1\nLattice="3 0 0 0 3 0 0\nH 0 0 0\n', "InvalidInput", "unterminated Lattice"),
+    ('1\nLattice="3 0 0 0 3 0 0 0"\nH 0 0 0\n', "InvalidInput", "needs 9 numbers, got 8"),
+    ('1\nLattice="3 0 0 0 3 0 0 0 x"\nH 0 0 0\n', "InvalidInput", "invalid number in XYZ Lattice"),
+    ('1\nLattice="3 0 0 0 3 0 0 0 3" pbc="T T"\nH 0 0 0\n', "InvalidInput", "needs 3 boolean tokens"),
+    ('1\nLattice="3 0 0 0 3 0 0 0 3" pbc="T T maybe"\nH 0 0 0\n', "InvalidInput", "invalid XYZ pbc token"),
+    ('1\nLattice="3 0 0 0 3 0 0 0 3" pbc="T T F"\nH 0 0 0\n', "InvalidInput", "fully periodic"),
+    ('1\nLattice="3 0 0 0 3 0 6 0 0"\nH 0 0 0\n', "SingularCell", "singular"),
 ])
 def test_errors(text, kind, fragment):
     with pytest.raises(NativeError) as e:
         system_from_xyz(text)
     assert e.value.kind == kind
     assert fragment in str(e.value)
+
+
+def test_extended_xyz_lattice():
+    # system.rs:191-219: Lattice="a b c" in the coordinate unit, pbc optional
+    text = '2\nLattice="5.6 0 0 0 5.6 0 0.1 0.2 5.5" pbc="T T T" Properties=species:S:1:pos:R:3\nNa 0 0 0\nCl 2.8 2.8 2.8\n'
+    z, x, q, cell = system_from_xyz(text, with_lattice=True)
+    assert list(z) == [11, 17]
+    assert np.array_equal(cell, np.array([[5.6, 0, 0], [0, 5.6, 0], [0.1, 0.2, 5.5]]) * A2B)
+    z, x, q, cell = system_from_xyz(text.replace(' pbc="T T T"', ''), coordinates_are_bohr=True, with_lattice=True)
+    assert cell[2, 2] == 5.5
+    assert system_from_xyz("1\nc\nH 0 0 0\n", with_lattice=True)[3] is None
