@@ -117,8 +117,9 @@ def cube_side(kinds, density):
     return (mass / (density * AVOGADRO_SCALE)) ** (1.0 / 3.0)
 
 
-def place(kinds, side, rng, min_dist=2.6, max_tries=2_000_000):
-    """Random sequential placement; returns (numbers, positions in Angstrom)."""
+def place(kinds, side, rng, min_dist=2.6, max_tries=2_000_000, periodic=False):
+    """Random sequential placement; returns (numbers, positions in Angstrom).
+    periodic: distances between centres by the minimum image of the cube."""
     centres = []
     numbers = []
     coords = []
@@ -133,6 +134,8 @@ def place(kinds, side, rng, min_dist=2.6, max_tries=2_000_000):
             c = np.array([rng.uniform() * side, rng.uniform() * side, rng.uniform() * side])
             if centres:
                 d = np.asarray(centres) - c
+                if periodic:
+                    d -= side * np.round(d / side)
                 if np.min(np.einsum("ij,ij->i", d, d)) < md2:
                     continue
             break
@@ -148,6 +151,15 @@ def water_cluster(nmol, side, seed):
     rng = Xoshiro256(seed)
     z, x = place(["water"] * nmol, side, rng)
     return z, x * ANGSTROM_TO_BOHR
+
+
+def water_box(nmol, side, seed):
+    """nmol rigid waters in a periodic cube of the given side (Angstrom): the
+    centres at least 2.6 A apart under the minimum image.  Returns (numbers,
+    positions in bohr, cell vectors in bohr as a 3 x 3 array of rows)."""
+    rng = Xoshiro256(seed)
+    z, x = place(["water"] * nmol, side, rng, periodic=True)
+    return z, x * ANGSTROM_TO_BOHR, np.eye(3) * side * ANGSTROM_TO_BOHR
 
 
 def mixed_cluster(seed, density=0.9, kmin=2, kmax=40):
