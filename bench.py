@@ -240,6 +240,56 @@ def single_system(name, precisions, oracle_max_ao=1000):
     return out
 
 
+def periodic_boxes(args):
+    """The periodic workload: water boxes of 32 molecules in a 9.86 A cube
+    (1.0 g/cm^3), box i seeded with 5000 + i, every box its own periodic cell."""
+    from gfn1_amd import geometry
+    return [geometry.water_box(32, 9.86, 5000 + i) for i in range(args.periodic_boxes)]
+
+
+def periodic_workload(args):
+    """Batched periodic Gamma-point energy + gradient (pbc/gradient.rs:64-73 per
+    box) through the same engine: boxes per second over the batch, FP64, SCC
+    energy tolerance 1e-7; parity of the first boxes against the oracle's
+    periodic restatement, whose single-core time on one box is the CPU
+    baseline.  Returns a dict (an extra key of the default run)."""
+    from gfn1_amd import AnalyticGradientOptions, BatchEngine, ElectronicOptions
+    t0 = time.time()
+    boxes = periodic_boxes(args)
+    print("periodic: %d boxes generated in %.1fs" % (len(boxes), time.time() - t0), file=sys.stderr, flush=True)
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(energy_tolerance=1e-7))
+    eng = BatchEngine([(z, x, 0.0, c) for z, x, c in boxes], options=opts)
+    eng.run()  # warm-up: kernels loaded, graphs captured
+    times = []
+    for s in range(max(args.steps, 1)):
+        t = time.perf_counter()
+        eng.run()
+        times.append(time.perf_counter() - t)
+        print("periodic step %d: %.3fs %s" % (s, times[-1], eng.timings()), file=sys.stderr, flush=True)
+    ms = 1000.0 * float(np.mean(times))
+    it = np.array([eng.iterations(i) for i in range(len(boxes))])
+    st = np.array([eng.status(i) for i in range(len(boxes))])
+    out = {"workload": "%d periodic boxes of 32 rigid waters in a 9.86 A cube (96 atoms, 256 AO, seeds 5000+i), "
+                       "Gamma point, energy + gradient, FP64, 300 K, SCC energy tolerance 1e-7" % len(boxes),
+           "boxes_per_s": len(boxes) / (ms / 1000.0), "ms_per_step": ms, "steps": len(times),
+           "mean_scc_iterations": float(it.mean()), "converged": int((st == 0).sum()), "phase_ms": eng.timings()}
+    if args.check > 0:
+        import oracle
+        de, dg = 0.0, 0.0
+        t = time.perf_counter()
+        for i in range(min(args.check, 2)):
+            z, x, c = boxes[i]
+            ref = oracle.run(z, x, cell=c, energy_tolerance=1e-7)
+            if i == 0:
+                cpu_s = time.perf_counter() - t
+            de = max(de, abs(ref["energy"] - eng.energy(i)))
+            dg = max(dg, float(np.abs(ref["gradient"] - eng.gradient(i)).max()))
+        out["parity_vs_oracle"] = {"boxes": min(args.check, 2), "max_abs_dE": de, "max_abs_dg": dg}
+        out["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "boxes/s", "cores": 1, "kind": "port",
+                               "sample": "box 0, FP64 C++ oracle (periodic restatement), one core"}
+    return out
+
+
 def run_optimize(args):
     """Batched L-BFGS geometry optimisation (optimize_geometry of every
     cluster of the batch at once, default options except max_iterations)
@@ -380,7 +430,8 @@ def main():
                     help="clusters optimised by the host oracle for the optimize workload (0 = skip)")
     ap.add_argument("--hess-systems", type=int, default=32, help="clusters of the hessian workload")
     ap.add_argument("--md-steps", type=int, default=10, help="steps of the md workload")
-    ap.add_argument("--workload", choices=["batch", "optimize", "hessian", "md"] + sorted(SINGLE_WORKLOADS),
+    ap.add_argument("--periodic-boxes", type=int, default=512, help="water boxes of the periodic workload")
+    ap.add_argument("--workload", choices=["batch", "optimize", "hessian", "md", "periodic"] + sorted(SINGLE_WORKLOADS),
                     default="batch",
                     help="batch: the batched headline metric; waterN: one large cluster (separate report)")
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both",
@@ -401,6 +452,12 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "periodic":
+        if args.gpus != 1:
+            print("bench.py measures one GPU (--gpus %d refused)" % args.gpus, file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(periodic_workload(args)))
         return
     if args.workload in ("optimize", "hessian", "md"):
         if args.gpus != 1:
@@ -539,6 +596,8 @@ def main():
         print("learned-start protocol", file=sys.stderr, flush=True)
         from gfn1_amd import train_guess
         out["learned_start_protocol"] = train_guess.protocol()
+        print("periodic workload", file=sys.stderr, flush=True)
+        out["periodic"] = periodic_workload(args)
     print(json.dumps(out))
 
 

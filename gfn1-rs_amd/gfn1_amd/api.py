@@ -7,7 +7,7 @@ Mirrors the reference's run_electronic / analytic_gradient entry points
 """
 import ctypes
 import os
-from dataclasses import dataclass, field, fields
+from dataclasses import dataclass, field, fields, replace
 from typing import Dict, Optional
 
 import numpy as np
@@ -650,6 +650,7 @@ class Gfn1NativeCalculator:
             electronic_temperature=electronic_temperature, nprim=nprim, eigen_tolerance=eigen_tolerance,
             enable_dispersion=enable_dispersion, enable_cn_hamiltonian=enable_cn_hamiltonian,
             level_shift=level_shift)
+        self._pbc_guess = None  # shell charges of the last periodic result
 
     def param_source(self) -> str:
         """python.rs:328-330."""
@@ -676,6 +677,37 @@ class Gfn1NativeCalculator:
         el = run_electronic(numbers, x, self.params, self.options, charge=self.charge)
         return CalculationResult(el.total_free, None, el.atomic_charges, el.dipole, el.iterations, el.converged,
                                  el.terms)
+
+    def calculate_periodic(self, numbers, positions, cell, pbc=(True, True, True), kgrid=None, unit="angstrom",
+                           compute_gradient=False, compute_stress=False) -> CalculationResult:
+        """Periodic single point (python.rs:381-470) at the Gamma point: cell is
+        the three lattice vectors (rows) in the chosen unit.  As in the
+        reference, each periodic calculation starts its SCC from the shell
+        charges of the previous one when the shell count matches (its
+        PbcSccGuess).  Only fully periodic cells and the Gamma point are
+        built; a k-point mesh or the stress tensor raises NotImplementedError."""
+        if tuple(bool(p) for p in pbc) != (True, True, True):
+            raise ValueError("only fully periodic cells (pbc = (True, True, True)) are supported")
+        if kgrid is not None and tuple(kgrid) != (1, 1, 1):
+            raise NotImplementedError("k-point meshes are not built; only the Gamma point (kgrid None or (1, 1, 1))")
+        if compute_stress:
+            raise NotImplementedError("the periodic stress tensor is not built")
+        scale = _unit_scale(unit)
+        x = np.asarray(positions, dtype=np.float64).reshape(-1, 3) * scale
+        lattice = np.asarray(cell, dtype=np.float64).reshape(3, 3) * scale
+        opts = replace(self.options, scc_initial_shell_charges=self._pbc_guess)
+        if compute_gradient:
+            r = analytic_gradient(numbers, x, self.params, AnalyticGradientOptions(electronic=opts),
+                                  charge=self.charge, lattice=lattice)
+            el = r.electronic_result
+            out = CalculationResult(r.total_energy, r.gradient, el.atomic_charges, el.dipole, el.iterations,
+                                    el.converged, el.terms)
+        else:
+            el = run_electronic(numbers, x, self.params, opts, charge=self.charge, lattice=lattice)
+            out = CalculationResult(el.total_free, None, el.atomic_charges, el.dipole, el.iterations, el.converged,
+                                    el.terms)
+        self._pbc_guess = np.array(el.shell_charges)
+        return out
 
 
     def optimize(self, numbers, positions, unit="angstrom", max_iterations=250, gradient_tolerance=1.0e-4,

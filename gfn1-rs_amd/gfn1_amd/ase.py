@@ -3,9 +3,11 @@
 The ASE boundary: positions in Angstrom, energies in eV, forces in eV/Angstrom,
 dipoles in e*Angstrom, Hessians in eV/Angstrom^2, every factor taken from
 ``ase.units`` (ase.py:38-57); the engine underneath speaks atomic units.
-Non-periodic systems: properties energy, forces, charges, dipole; the native
-L-BFGS (``optimize_native``), the Hessian, the harmonic frequencies, the
-dipole derivatives and the IR spectrum.
+Properties energy, forces, charges, dipole for molecules and for fully
+periodic cells (the Gamma-point path; a cell's dipole is the reference-cell
+Mulliken dipole, as in the reference; no stress); for molecules also the
+native L-BFGS (``optimize_native``), the Hessian, the harmonic frequencies,
+the dipole derivatives and the IR spectrum.
 Importing this module needs ASE, as the reference's does.
 """
 from typing import Any
@@ -70,7 +72,8 @@ def _to_ase_gradient_dict(raw, energy_keys=()):
 
 
 class GFN1AMDCalculator(Calculator):
-    """GFN1RSCalculator (ase.py:115-560) for non-periodic systems."""
+    """GFN1RSCalculator (ase.py:115-560): single points of molecules and of
+    fully periodic cells at the Gamma point, the other methods for molecules."""
 
     implemented_properties = ["energy", "forces", "charges", "dipole"]
     default_parameters = {
@@ -104,17 +107,29 @@ class GFN1AMDCalculator(Calculator):
         if atoms is None:
             raise RuntimeError("no ASE Atoms object supplied or attached")
         if any(atoms.get_pbc()):
-            raise NotImplementedError("periodic systems are not supported by this engine")
+            raise NotImplementedError("this method is built for molecules only; periodic cells have single points")
         return atoms.get_atomic_numbers().tolist(), np.asarray(atoms.get_positions(), dtype=float)
 
     def calculate(self, atoms=None, properties=("energy",), system_changes=all_changes):
         """Single point in ASE units (ase.py:358-434), plus the native
         passthroughs native_energy_terms_hartree / _ev, native_dipole_au,
-        native_forces_hartree_per_bohr, native_converged, native_iterations."""
+        native_forces_hartree_per_bohr, native_converged, native_iterations.
+        Atoms periodic in every direction go to the native periodic single
+        point (Gamma point, cell in Angstrom); partly periodic ones raise."""
         Calculator.calculate(self, atoms, properties, system_changes)
-        numbers, positions = self._molecule(self.atoms)
-        result = self._native.calculate(numbers, positions, unit="angstrom",
-                                        compute_gradient="forces" in set(properties))
+        if self.atoms is None:
+            raise RuntimeError("no ASE Atoms object supplied or attached")
+        want_forces = "forces" in set(properties)
+        pbc = [bool(p) for p in self.atoms.get_pbc()]
+        numbers = self.atoms.get_atomic_numbers().tolist()
+        positions = np.asarray(self.atoms.get_positions(), dtype=float)
+        if any(pbc):
+            if not all(pbc):
+                raise NotImplementedError("only cells periodic in all three directions are supported")
+            result = self._native.calculate_periodic(numbers, positions, np.asarray(self.atoms.get_cell(), dtype=float),
+                                                     unit="angstrom", compute_gradient=want_forces)
+        else:
+            result = self._native.calculate(numbers, positions, unit="angstrom", compute_gradient=want_forces)
         terms_au = dict(result.energy_terms_hartree())
         dipole_au = np.asarray(result.dipole, dtype=float)
         update = {

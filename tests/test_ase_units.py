@@ -142,6 +142,10 @@ class _FakeNative:
         self.calls.append((list(numbers), np.asarray(positions), unit, compute_gradient))
         return self.result
 
+    def calculate_periodic(self, numbers, positions, cell, unit="angstrom", compute_gradient=False):
+        self.calls.append((list(numbers), np.asarray(positions), unit, compute_gradient, np.asarray(cell)))
+        return self.result
+
 
 def test_calculate_converts_every_property_with_ase_units(ase_mod):
     """calculate() (ase.py:358-434) against the mocked native layer: energy,
@@ -172,3 +176,37 @@ def test_calculate_converts_every_property_with_ase_units(ase_mod):
     fake.result = _FakeResult(energy_hartree=-5.25)
     calc.calculate(atoms, properties=("energy",))
     assert fake.calls[-1][3] is False and "forces" not in calc.results
+
+
+class _PeriodicAtoms(_Atoms):
+    def __init__(self, numbers, positions, cell, pbc):
+        super().__init__(numbers, positions)
+        self.cell = np.asarray(cell, dtype=float)
+        self.pbc = np.asarray(pbc)
+
+    def get_pbc(self):
+        return self.pbc
+
+    def get_cell(self):
+        return self.cell
+
+
+def test_periodic_atoms_go_to_the_periodic_single_point(ase_mod):
+    """A cell periodic in every direction is handed to calculate_periodic with
+    its cell in Angstrom, and converted like a molecule's result; a partly
+    periodic one is refused."""
+    Bohr, Hartree, _ = units()
+    gradient = np.array([[0.01, 0.0, -0.02], [-0.01, 0.0, 0.02]])
+    fake = _FakeNative(_FakeResult(energy_hartree=-5.25, gradient=gradient))
+    calc = ase_mod.GFN1RSCalculator()
+    calc._native = fake
+    cell = np.diag([5.0, 6.0, 7.0])
+    atoms = _PeriodicAtoms([1, 1], [[0.0, 0.0, 0.0], [0.74, 0.0, 0.0]], cell, [True, True, True])
+    calc.calculate(atoms, properties=("energy", "forces"))
+    numbers, pos, unit, grad, c = fake.calls[-1]
+    assert numbers == [1, 1] and unit == "angstrom" and grad is True
+    np.testing.assert_array_equal(c, cell)
+    assert calc.results["energy"] == -5.25 * Hartree
+    np.testing.assert_allclose(calc.results["forces"], -gradient * Hartree / Bohr, rtol=1e-15)
+    with pytest.raises(NotImplementedError):
+        calc.calculate(_PeriodicAtoms([1, 1], [[0, 0, 0], [0.74, 0, 0]], cell, [True, False, True]))

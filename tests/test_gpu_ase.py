@@ -22,10 +22,11 @@ from ase_stand_in import ASE_BOHR, ASE_HARTREE, ASE_INVCM, install_stand_in as _
 class _Atoms:
     """The few Atoms methods the calculator calls."""
 
-    def __init__(self, numbers, positions, pbc=False):
+    def __init__(self, numbers, positions, pbc=False, cell=None):
         self.numbers = np.asarray(numbers)
         self.positions = np.asarray(positions, dtype=float)
-        self.pbc = np.array([pbc] * 3)
+        self.pbc = np.array(pbc if isinstance(pbc, (list, tuple)) else [pbc] * 3)
+        self.cell = np.zeros((3, 3)) if cell is None else np.asarray(cell, dtype=float)
 
     def get_atomic_numbers(self):
         return self.numbers
@@ -38,6 +39,9 @@ class _Atoms:
 
     def get_pbc(self):
         return self.pbc
+
+    def get_cell(self):
+        return self.cell.copy()
 
 
 WATER = ([8, 1, 1], [[0.0, 0.0, 0.0], [0.757, 0.586, 0.0], [-0.757, 0.586, 0.0]])
@@ -67,8 +71,17 @@ def test_single_point_units(ase_calc):
     assert np.allclose(calc.results["dipole"], ref.electronic_result.dipole * Bohr, rtol=0, atol=1e-12)
     assert np.allclose(calc.results["charges"], ref.electronic_result.atomic_charges, rtol=0, atol=1e-12)
     assert abs(sum(calc.results["charges"])) < 1e-8
+    # a fully periodic cell: the Gamma-point single point, cell in Angstrom
+    cell = np.diag([6.0, 6.5, 7.0])
+    calc.calculate(_Atoms(*WATER, pbc=True, cell=cell), properties=("energy", "forces"))
+    pref = analytic_gradient(WATER[0], np.asarray(WATER[1]) * ANGSTROM_TO_BOHR, lattice=cell * ANGSTROM_TO_BOHR)
+    assert abs(calc.results["energy"] - pref.total_energy * Hartree) < 1e-9
+    assert np.allclose(calc.results["forces"], -pref.gradient * Hartree / Bohr, rtol=0, atol=1e-9)
+    assert abs(pref.total_energy - ref.total_energy) > 1e-6  # the images matter
     with pytest.raises(NotImplementedError):
-        calc.calculate(_Atoms(*WATER, pbc=True))
+        calc.calculate(_Atoms(*WATER, pbc=[True, True, False], cell=cell))
+    with pytest.raises(NotImplementedError):
+        calc.optimize_native(_Atoms(*WATER, pbc=True, cell=cell))
 
 
 def test_optimize_hessian_frequencies(ase_calc):
