@@ -236,7 +236,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 cells[i].gam0 = gam_size;
                 gam_size += round_up(nsh * nsh, kRangeAlign);
                 h_pinst_.push_back(i);
-                max_pnsh_ = std::max(max_pnsh_, nsh);
+                max_pnat_ = std::max(max_pnat_, s.nat);
                 for (int a = 0; a < s.nat; ++a) h_patoms_.push_back(h_atom0_[i] + a);
             }
         }
@@ -764,7 +764,7 @@ Status Batch::setup_phase() {
     HIP_TRY(hipEventRecord(ev_cn_, stream_));
     // Ewald shell Coulomb matrices of the periodic instances (needed from the
     // first SCC iteration on; the buckets wait for ev_start_ below)
-    HIP_TRY(launch_ewald_gamma(stream_, db_, t, d_pinst_, (int)h_pinst_.size(), max_pnsh_));
+    HIP_TRY(launch_ewald_gamma(stream_, db_, t, d_pinst_, (int)h_pinst_.size(), max_pnat_));
 
     const double one = 1.0, zero = 0.0;
     auto a0_gemms = [&](Bucket& bk, int count) -> Status {

@@ -316,7 +316,7 @@ class Batch {
     // periodic instances: cell records, reciprocal vectors, Ewald shell matrices
     std::vector<unsigned char> h_pbc_;
     std::vector<int> h_pinst_, h_patoms_;
-    int max_pnsh_ = 0;
+    int max_pnat_ = 0;
     CellRec* d_cell_ = nullptr;
     double *d_gtab_ = nullptr, *d_gam_ = nullptr;
     int *d_pinst_ = nullptr, *d_patoms_ = nullptr;

@@ -64,58 +64,66 @@ __device__ double block_sum_p(double v, double* lds) {
 
 }  // namespace
 
-// Gamma_ij of every periodic instance (pbc/ewald.rs:200-320): one work-group per
-// (row i, instance pinst[blockIdx.y]); its threads take the columns j <= i and
-// write Gamma_ij and Gamma_ji (the matrix is exactly symmetric).
-__global__ void __launch_bounds__(256) k_ewald_gamma(DeviceBatch b, DeviceTables t, const int* pinst) {
+// Gamma_ij of every periodic instance (pbc/ewald.rs:200-320).  One work-group
+// per (atom a, instance pinst[blockIdx.y]); each thread takes an atom b <= a
+// and walks the lattice images once: the parts that depend on the distance
+// only (erfc / R, the R^-3 real-space factor, the reciprocal sums) are summed
+// per atom pair, the short-range remainder per shell pair (it depends on the
+// pair's eta), and the thread writes Gamma_ij and Gamma_ji for the shells of a
+// and b (the matrix is exactly symmetric).
+__global__ void __launch_bounds__(128) k_ewald_gamma(DeviceBatch b, DeviceTables t, const int* pinst) {
     const int inst = pinst[blockIdx.y];
-    const int nsh = b.nsh[inst];
-    const int i = blockIdx.x;
-    if (i >= nsh) return;
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const int la = blockIdx.x;
+    if (la >= nat) return;
     const CellRec& c = b.cell[inst];
     const double alpha = c.alpha;
     const double rcut = kEwaldTau / alpha;
     const double realcut = fmax(rcut, kEwaldShortCutoff);
     int rg[3];
     image_ranges(c, realcut, rg);
-    const int s0 = b.sh0[inst];
-    const int si = s0 + i, ai = b.sh_atom[si];
+    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const double* x = b.atom_x;
-    const double hi = t.shells[b.sh_type[si]].hardness;
     const double* gt = b.gtab + c.g0 * kGStride;
     double* gam = const_cast<double*>(b.gam) + c.gam0;
-    for (int j = threadIdx.x; j <= i; j += blockDim.x) {
-        const int sj = s0 + j, aj = b.sh_atom[sj];
-        const double hj = t.shells[b.sh_type[sj]].hardness;
-        const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
-        const double ie2 = 1.0 / (eta * eta);
-        const bool same = ai == aj;
-        const double rab[3] = {x[3 * ai] - x[3 * aj], x[3 * ai + 1] - x[3 * aj + 1], x[3 * ai + 2] - x[3 * aj + 2]};
-        // real space: the 1/R Ewald part (erfc), the R^-3 part and the remainder
-        double real1 = 0.0, real3 = 0.0, sr = 0.0;
+    const int A = a0 + la;
+    const int fa = b.atom_sh0[A], na = t.elements[b.atom_z[A]].nshell;
+    for (int lb = threadIdx.x; lb <= la; lb += blockDim.x) {
+        const int B = a0 + lb;
+        const int fb = b.atom_sh0[B], nb = t.elements[b.atom_z[B]].nshell;
+        const bool same = A == B;
+        double ie2[kMaxShellsPerAtom][kMaxShellsPerAtom], eta[kMaxShellsPerAtom][kMaxShellsPerAtom];
+        double sr[kMaxShellsPerAtom][kMaxShellsPerAtom];
+        for (int p = 0; p < na; ++p)
+            for (int q = 0; q < nb; ++q) {
+                const double hi = t.shells[b.sh_type[fa + p]].hardness, hj = t.shells[b.sh_type[fb + q]].hardness;
+                eta[p][q] = 2.0 / (1.0 / hi + 1.0 / hj);
+                ie2[p][q] = 1.0 / (eta[p][q] * eta[p][q]);
+                sr[p][q] = 0.0;
+            }
+        const double rab[3] = {x[3 * A] - x[3 * B], x[3 * A + 1] - x[3 * B + 1], x[3 * A + 2] - x[3 * B + 2]};
+        double real1 = 0.0, real3 = 0.0;
         for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
             for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
                 for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                    const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
                     double d[3];
                     for (int k = 0; k < 3; ++k) d[k] = rab[k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
                     const double r = sqrt(sum3(d[0], d[1], d[2]));
-                    if (r <= 1e-12) continue;
+                    // r <= 1e-12 is the atom itself in the origin cell
+                    if (r <= 1e-12 || r > realcut) continue;
                     if (r <= rcut) {
-                        real1 += erfc(alpha * r) / r;
-                        if (!(same && origin)) {
-                            double v, dv;
-                            r3_real(r, alpha, v, dv);
-                            real3 += v;
-                        }
+                        const double ar = alpha * r, ec = erfc(ar), r3 = r * r * r;
+                        real1 += ec / r;
+                        real3 += (ec + (2.0 * ar / kSqrtPi) * exp(-ar * ar)) / r3;
                     }
-                    if (r <= kEwaldShortCutoff && !(same && origin)) {
-                        double v, dv;
-                        ko_short(r, ie2, v, dv);
-                        sr += v;
-                    }
+                    if (r <= kEwaldShortCutoff)
+                        for (int p = 0; p < na; ++p)
+                            for (int q = 0; q < nb; ++q) {
+                                double v, dv;
+                                ko_short(r, ie2[p][q], v, dv);
+                                sr[p][q] += v;
+                            }
                 }
-        // reciprocal space
         double rec1 = 0.0, rec3 = 0.0;
         for (int g = 0; g < c.ng; ++g) {
             const double* e = gt + (size_t)g * kGStride;
@@ -123,20 +131,28 @@ __global__ void __launch_bounds__(256) k_ewald_gamma(DeviceBatch b, DeviceTables
             rec1 += e[3] * cs;
             rec3 += e[4] * cs;
         }
-        double v = real1 + rec1 + c.ew_bg;
-        if (same) v += c.ew_self + eta;
-        v += kR3Coeff * ie2 * real3 + ie2 * rec3 + c.r3_k0 * ie2;
-        if (same) v += c.r3_self * ie2;
-        v += sr;
-        gam[(size_t)i * nsh + j] = v;
-        gam[(size_t)j * nsh + i] = v;
+        const double phi = real1 + rec1 + c.ew_bg + (same ? c.ew_self : 0.0);
+        for (int p = 0; p < na; ++p)
+            for (int q = 0; q < nb; ++q) {
+                const int i = fa + p - s0, j = fb + q - s0;
+                if (j > i) continue;  // the same atom: lower triangle of its block
+                double v = phi + (same ? eta[p][q] : 0.0);
+                v += kR3Coeff * ie2[p][q] * real3 + ie2[p][q] * rec3 + c.r3_k0 * ie2[p][q];
+                if (same) v += c.r3_self * ie2[p][q];
+                v += sr[p][q];
+                gam[(size_t)i * nsh + j] = v;
+                gam[(size_t)j * nsh + i] = v;
+            }
     }
 }
 
 // Gradient of 1/2 q^T Gamma q at the converged shell charges for every atom of
 // a periodic instance (pbc/gradient.rs:548-603, :679-779), added to g_el.  One
-// 256-thread work-group per listed atom; fixed thread assignment and a fixed
-// reduction order keep the result deterministic.
+// 256-thread work-group per listed atom a; a work item is one (atom, image)
+// pair, whose erfc / R force uses the atomic charges and whose R^-3 and
+// remainder forces the shell pairs of the two atoms; the reciprocal forces go
+// over the G vectors.  Fixed thread assignment and a fixed reduction order keep
+// the result deterministic.
 __global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables t, const int* patoms,
                                                     const double* q, double* g_el) {
     __shared__ double lds[8];
@@ -160,56 +176,51 @@ __global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables 
     image_ranges(c, realcut, rg);
     const int nimg = (2 * rg[0] + 1) * (2 * rg[1] + 1) * (2 * rg[2] + 1);
     double acc[3] = {0.0, 0.0, 0.0};
-    // (i) real-space 1/R Ewald force with atomic charges, and (iv) the real-space
-    // R^-3 and remainder terms per shell pair; work item = (atom or shell, image)
     const int nat = a1 - a0;
     const double tas = 2.0 * alpha / kSqrtPi;
-    for (int w = threadIdx.x; w < (nat + nsh) * nimg; w += blockDim.x) {
-        const int e = w / nimg, im = w % nimg;
+    for (int w = threadIdx.x; w < nat * nimg; w += blockDim.x) {
+        const int o = a0 + w / nimg, im = w % nimg;
         const int n0 = im / ((2 * rg[1] + 1) * (2 * rg[2] + 1)) - rg[0];
         const int n1 = (im / (2 * rg[2] + 1)) % (2 * rg[1] + 1) - rg[1];
         const int n2 = im % (2 * rg[2] + 1) - rg[2];
-        const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
-        double tv[3];
-        for (int k = 0; k < 3; ++k) tv[k] = n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k];
-        if (e < nat) {
-            const int o = a0 + e;
-            const double qq = qa * qatom(o);
-            if (qq == 0.0) continue;
-            double d[3];
-            for (int k = 0; k < 3; ++k) d[k] = x[3 * a + k] - x[3 * o + k] - tv[k];
-            const double r = sqrt(sum3(d[0], d[1], d[2]));
-            if (r <= 1e-12 || r > rcut) continue;
-            const double dg = -erfc(alpha * r) / (r * r) - tas * exp(-alpha * alpha * r * r) / r;
-            for (int k = 0; k < 3; ++k) acc[k] += d[k] * (qq * dg / r);
-        } else {
-            const int sj = s0 + (e - nat), aj = b.sh_atom[sj];
-            if (aj == a && origin) continue;
-            double d[3];
-            for (int k = 0; k < 3; ++k) d[k] = x[3 * a + k] - x[3 * aj + k] - tv[k];
-            const double r = sqrt(sum3(d[0], d[1], d[2]));
-            if (r <= 1e-12 || r > realcut) continue;
-            const double hj = t.shells[b.sh_type[sj]].hardness;
-            double v3 = 0.0, d3 = 0.0;
-            if (r <= rcut) r3_real(r, alpha, v3, d3);
-            for (int k2 = 0; k2 < nsa; ++k2) {
-                const int si = fa + k2;
-                const double qq = q[si] * q[sj];
-                if (qq == 0.0) continue;
-                const double hi = t.shells[b.sh_type[si]].hardness;
-                const double eta = 2.0 / (1.0 / hi + 1.0 / hj), ie2 = 1.0 / (eta * eta);
-                double rad = 0.0;
-                if (r <= rcut) rad += kR3Coeff * ie2 * d3;
-                if (r <= kEwaldShortCutoff) {
-                    double v, dv;
-                    ko_short(r, ie2, v, dv);
-                    rad += dv;
+        double d[3];
+        for (int k = 0; k < 3; ++k)
+            d[k] = x[3 * a + k] - x[3 * o + k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
+        const double r = sqrt(sum3(d[0], d[1], d[2]));
+        // r <= 1e-12 is the atom itself in the origin cell
+        if (r <= 1e-12 || r > realcut) continue;
+        const int fo = b.atom_sh0[o], nso = t.elements[b.atom_z[o]].nshell;
+        double rad = 0.0;
+        if (r <= rcut) {
+            const double ar = alpha * r, ex = exp(-ar * ar), ec = erfc(ar);
+            // 1/R Ewald force with the atomic charges
+            rad += qa * qatom(o) * (-ec / (r * r) - tas * ex / r);
+            // R^-3 real-space force: d/dr [q(r) / r^3] times sum_ij q_i q_j eta_ij^-2
+            const double qq3 = ec + (2.0 * ar / kSqrtPi) * ex;
+            const double dq3 = -(4.0 * alpha * alpha * alpha * r * r / kSqrtPi) * ex;
+            const double d3 = dq3 / (r * r * r) - 3.0 * qq3 / (r * r * r * r);
+            double s3 = 0.0;
+            for (int k2 = 0; k2 < nsa; ++k2)
+                for (int k3 = 0; k3 < nso; ++k3) {
+                    const double hi = t.shells[b.sh_type[fa + k2]].hardness, hj = t.shells[b.sh_type[fo + k3]].hardness;
+                    const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+                    s3 += q[fa + k2] * q[fo + k3] / (eta * eta);
                 }
-                for (int k = 0; k < 3; ++k) acc[k] += d[k] * (qq * rad / r);
-            }
+            rad += kR3Coeff * d3 * s3;
         }
+        if (r <= kEwaldShortCutoff)
+            for (int k2 = 0; k2 < nsa; ++k2)
+                for (int k3 = 0; k3 < nso; ++k3) {
+                    const double hi = t.shells[b.sh_type[fa + k2]].hardness, hj = t.shells[b.sh_type[fo + k3]].hardness;
+                    const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+                    double v, dv;
+                    ko_short(r, 1.0 / (eta * eta), v, dv);
+                    rad += q[fa + k2] * q[fo + k3] * dv;
+                }
+        for (int k = 0; k < 3; ++k) acc[k] += d[k] * (rad / r);
     }
-    // (ii) reciprocal 1/R force and (iii) reciprocal R^-3 force, per G vector
+    // reciprocal 1/R force (atomic charges) and reciprocal R^-3 force (shell
+    // charges, coefficient -w3 per G: pbc/gradient.rs:686-706)
     const double* gt = b.gtab + c.g0 * kGStride;
     for (int g = threadIdx.x; g < c.ng; g += blockDim.x) {
         const double* e = gt + (size_t)g * kGStride;
@@ -223,7 +234,6 @@ __global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables 
         const double pha = e[0] * x[3 * a] + e[1] * x[3 * a + 1] + e[2] * x[3 * a + 2];
         const double f1 = sin(pha) * sc - cos(pha) * ss;
         double s = -e[3] * qa * f1;
-        // R^-3: coefficient -w3 per G (pbc/gradient.rs:686-706)
         for (int k2 = 0; k2 < nsa; ++k2) {
             const int si = fa + k2;
             if (q[si] == 0.0) continue;
@@ -360,9 +370,9 @@ bool make_cell_record(const double* cell, CellRec& c, std::vector<double>& gtab)
 }
 
 hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
-                              int npinst, int max_nsh) {
-    if (npinst == 0 || max_nsh == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ewald_gamma, dim3(max_nsh, npinst), dim3(256), 0, s, b, t, pinst);
+                              int npinst, int max_nat) {
+    if (npinst == 0 || max_nat == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ewald_gamma, dim3(max_nat, npinst), dim3(128), 0, s, b, t, pinst);
     return hipGetLastError();
 }
 

@@ -19,7 +19,7 @@ double exponential_integral_e1(double x);
 
 // Gamma (nsh x nsh) of the listed periodic instances into b.gam.
 hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
-                              int npinst, int max_nsh);
+                              int npinst, int max_nat);
 // g_el[3 a + d] += d(1/2 q^T Gamma q)/dR_a for the listed atoms of periodic instances.
 hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* patoms,
                              int npatoms, const double* q, double* g_el);
