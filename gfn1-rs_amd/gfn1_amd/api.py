@@ -389,15 +389,16 @@ def _read_optimization(lib, h, i, nat):
 def optimize_geometry(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
                       options: Optional[GeometryOptimizationOptions] = None,
                       gradient_options: Optional[AnalyticGradientOptions] = None,
-                      charge: float = 0.0) -> GeometryOptimizationResult:
+                      charge: float = 0.0, lattice=None) -> GeometryOptimizationResult:
     """optimize_geometry (optimizer.rs:78-185): L-BFGS relaxation of one
-    non-periodic system; positions in bohr.  Raises NativeError where the
-    reference returns Err."""
+    system; positions in bohr.  With a lattice (rows, bohr) the atoms relax in
+    the fixed cell over the periodic Gamma-point energy and gradient.  Raises
+    NativeError where the reference returns Err."""
     lib = _lib.load()
     p = _params(params)
     oc = (options or GeometryOptimizationOptions()).to_c()
     gc, keep_q = (gradient_options or AnalyticGradientOptions()).to_c()
-    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge, lattice)
     h = ctypes.c_void_p()
     err = _lib.Gfn1Error()
     rc = lib.gfn1_optimize_geometry(ctypes.byref(sysc), p._h, ctypes.byref(gc), ctypes.byref(oc), ctypes.byref(h),
@@ -453,14 +454,17 @@ class HessianResult:
 
 def hessian(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
             electronic: Optional[ElectronicOptions] = None, options: Optional[HessianOptions] = None,
-            charge: float = 0.0) -> HessianResult:
+            charge: float = 0.0, lattice=None) -> HessianResult:
     """Cartesian Hessian of one system (replaces analytic_hessian(...).hessian,
-    hessian/mod.rs:112-134, at the reference's own 1e-6 Eh/bohr^2 acceptance)."""
+    hessian/mod.rs:112-134, at the reference's own 1e-6 Eh/bohr^2 acceptance).
+    With a lattice (rows, bohr): the fixed-cell Gamma-point Hessian of a
+    periodic system (pbc_gamma_hessian, pbc/hessian.rs), by the same central
+    differences of the periodic gradient."""
     lib = _lib.load()
     p = _params(params)
     eo, keep_q = (electronic or ElectronicOptions()).to_c()
     ho = (options or HessianOptions()).to_c()
-    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge)
+    sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge, lattice)
     ndof = 3 * len(np.asarray(numbers))
     out = np.zeros((ndof, ndof))
     ddip = np.zeros((ndof, 3))

@@ -31,12 +31,13 @@ def fd_hessian(numbers, positions_bohr, **kw):
 
 
 def fd_hessian_dipoles(numbers, positions_bohr, step=1.0e-4, energy_tolerance=1.0e-11, charge_tolerance=1.0e-11,
-                       charge=0.0, warm_start=True, **terms):
+                       charge=0.0, warm_start=True, cell=None, **terms):
     """(Hessian, asymmetry, dipole derivatives [3 nat, 3] = d mu_alpha / d R_c by
-    the same central differences, Mulliken dipole of the undisplaced geometry)."""
+    the same central differences, Mulliken dipole of the undisplaced geometry).
+    cell: the cell vectors (rows, bohr) of a periodic system."""
     x0 = np.asarray(positions_bohr, dtype=np.float64).reshape(-1)
     ndof = x0.size
-    opts = dict(energy_tolerance=energy_tolerance, charge_tolerance=charge_tolerance, charge=charge)
+    opts = dict(energy_tolerance=energy_tolerance, charge_tolerance=charge_tolerance, charge=charge, cell=cell)
     opts.update(gradient_terms(**terms))
     base = _oracle.run(numbers, x0.reshape(-1, 3), **opts)
     if base["status"] != 0:

@@ -14,9 +14,12 @@ import sys
 import numpy as np
 import pytest
 
+import hessian_oracle
+import optimizer_oracle
 import oracle
-from gfn1_amd import (AnalyticGradientOptions, BatchEngine, ElectronicOptions, NativeError, analytic_gradient,
-                      geometry, run_electronic, system_from_xyz)
+from gfn1_amd import (AnalyticGradientOptions, BatchEngine, ElectronicOptions, GeometryOptimizationOptions,
+                      NativeError, analytic_gradient, geometry, hessian, optimize_geometry, run_electronic,
+                      system_from_xyz)
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +161,34 @@ def test_periodic_errors_and_xyz_input():
     res = run_electronic(zz, xx, lattice=cell, options=ElectronicOptions(**TIGHT))
     ref = oracle.run(zz, xx, cell=cell, **TIGHT)
     assert abs(res.total_free - ref["energy"]) <= 1e-8
+
+
+def test_periodic_fixed_cell_optimisation_matches_oracle():
+    # optimize_geometry over the periodic energy and gradient, cell fixed
+    # (optimizer.rs:78-185 through the periodic dispatch of gradient.rs:61-66)
+    note("case periodic optimize")
+    z, x, cell = geometry.water_box(2, 4.6, seed=7)
+    r = optimize_geometry(z, x, options=GeometryOptimizationOptions(max_iterations=12), gradient_options=OPTS,
+                          lattice=cell)
+    want = optimizer_oracle.optimize_gfn1(z, x, scc_options=dict(cell=cell, **TIGHT), max_iterations=12)
+    note("periodic optimize: device it=%d E=%.12f | oracle it=%d E=%.12f" % (r.iterations, r.energy,
+                                                                              want["iterations"], want["energy"]))
+    assert r.status == 0 and r.iterations == want["iterations"] and r.converged == want["converged"]
+    assert abs(r.energy - want["energy"]) <= 1e-8
+    assert np.abs(r.positions - want["positions"]).max() <= 1e-6
+    for a, b in zip(r.trajectory, want["trajectory"]):
+        assert abs(a.energy - b[1]) <= 1e-8
+
+
+def test_periodic_hessian_matches_oracle():
+    note("case periodic hessian")
+    z = np.array([8, 1, 1])
+    x = np.array([[0.0, 0.0, 0.0], [0.78, 0.58, 0.02], [-0.74, 0.6, -0.03]]) * A2B
+    cell = np.array([[5.2, 0.0, 0.0], [0.3, 5.5, 0.0], [0.0, 0.2, 5.8]]) * A2B
+    r = hessian(z, x, lattice=cell)
+    want, asym = hessian_oracle.fd_hessian(z, x, cell=cell)
+    d = float(np.abs(r.hessian - want).max())
+    note("periodic hessian: max|dH| %.2e" % d)
+    assert d <= 1e-6
+    # translational sum rule of the periodic Hessian
+    assert np.abs(r.hessian.reshape(9, 3, 3).sum(axis=1)).max() < 1e-5
