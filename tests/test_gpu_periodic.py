@@ -192,3 +192,14 @@ def test_periodic_hessian_matches_oracle():
     assert d <= 1e-6
     # translational sum rule of the periodic Hessian
     assert np.abs(r.hessian.reshape(9, 3, 3).sum(axis=1)).max() < 1e-5
+
+
+def test_periodic_box_above_512_ao_matches_oracle():
+    # 70 waters, 560 AO: the large-matrix branch (whole-device reduction,
+    # rocSOLVER potrf/trtri, dstedc, dormtr) with a periodic instance
+    note("case periodic water70 (560 AO)")
+    z, x, cell = geometry.water_box(70, 12.8, seed=9)
+    ref = oracle.run(z, x, cell=cell, **TIGHT)
+    assert ref["status"] == 0
+    res = analytic_gradient(z, x, options=OPTS, lattice=cell)
+    check_against_oracle("water70", z, x, cell, res, ref)
