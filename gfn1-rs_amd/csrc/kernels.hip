@@ -1869,29 +1869,17 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
     if (cn_on) GradAcc<R>::add(acc.dc, acc.cdc, (R)(dse_a * hs * (double)sp));
 }
 
-// Every image of shell sb's atom within the AO cutoff (30 bohr), as the bra
-// side of atom a (pbc/gradient.rs:285-384 at the Gamma point).
+// Shell pair (sa, sb) with sb's atom at image (n0, n1, n2), as the bra side of
+// atom a (pbc/gradient.rs:285-384 at the Gamma point); the (LA, LB) class is
+// chosen by the caller's switch.
 template <int LA, int LB, typename R>
-__device__ __forceinline__ void grad_pair_images(const DeviceBatch& b, const DeviceTables& t, const double* cn,
-                                                 const double* vsh, int scc_on, int cn_on, int a, int sa, int sb,
-                                                 int ld, gptr<const double> P, gptr<const double> W,
-                                                 const CellRec& cr, const int* rg, GradAcc<R>& acc) {
-    const int ob = b.sh_atom[sb];
-    const double* x = b.atom_x;
-    const double c2 = kPbcAoCutoff * kPbcAoCutoff;
-    for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                const bool origin = n0 == 0 && n1 == 0 && n2 == 0;
-                if (ob == a && origin) continue;
-                double tv[3];
-                image_vector(cr, n0, n1, n2, tv);
-                const double dx = x[3 * a] - x[3 * ob] - tv[0], dy = x[3 * a + 1] - x[3 * ob + 1] - tv[1],
-                             dz = x[3 * a + 2] - x[3 * ob + 2] - tv[2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= 1e-12 || r2 > c2) continue;
-                grad_pair<LA, LB, R, true>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc, tv, ob == a);
-            }
+__device__ __forceinline__ void grad_pair_image(const DeviceBatch& b, const DeviceTables& t, const double* cn,
+                                                const double* vsh, int scc_on, int cn_on, int a, int sa, int sb,
+                                                int ld, gptr<const double> P, gptr<const double> W,
+                                                const CellRec& cr, int n0, int n1, int n2, GradAcc<R>& acc) {
+    double tv[3];
+    image_vector(cr, n0, n1, n2, tv);
+    grad_pair<LA, LB, R, true>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc, tv, b.sh_atom[sb] == a);
 }
 
 }  // namespace
@@ -1980,6 +1968,7 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     if (periodic_instance(b, inst) != PBC) return;
     const int ld = b.ld[inst];
     int rg[3] = {0, 0, 0};
+    __shared__ int pq[PBC ? 128 : 1];  // queue of candidates inside the cutoff (periodic)
     if constexpr (PBC) {
         image_ranges(b.cell[inst], kPbcAoCutoff, rg);
         scc_on = 1;
@@ -1997,28 +1986,81 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
         const int la = t.shells[b.sh_type[sa]].l;
         for (int lb = 0; lb < 3; ++lb) {
             if ((la == 2 || lb == 2) != DCLASS) continue;
-            for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
-                const int sb = b.sh_by_l[idx];
-                if constexpr (PBC) {
-                    const CellRec& cr = b.cell[inst];
+            if constexpr (PBC) {
+                // Candidates are (shell of class lb, image) pairs, 64 at a time,
+                // one per lane; the ones inside the 30-bohr cutoff are compacted
+                // into an LDS queue (ballot order, so the order is fixed) and
+                // evaluated 64 at a time, one per lane, so the lanes do not idle
+                // on images that are out of reach for their own shell.
+                const CellRec& cr = b.cell[inst];
+                const int d0 = 2 * rg[0] + 1, d1 = 2 * rg[1] + 1, d2 = 2 * rg[2] + 1;
+                const int nimg = d0 * d1 * d2, s_first = lo[lb];
+                const int ncand = (lo[lb + 1] - s_first) * nimg;
+                const int lane = threadIdx.x;
+                const double* x = b.atom_x;
+                const double c2p = kPbcAoCutoff * kPbcAoCutoff;
+                auto decode = [&](int c, int& sb, int& n0, int& n1, int& n2) {
+                    sb = b.sh_by_l[s_first + c / nimg];
+                    const int im = c % nimg;
+                    n0 = im / (d1 * d2) - rg[0];
+                    n1 = (im / d2) % d1 - rg[1];
+                    n2 = im % d2 - rg[2];
+                };
+                auto evaluate = [&](int c) {
+                    int sb, n0, n1, n2;
+                    decode(c, sb, n0, n1, n2);
                     if constexpr (!DCLASS) {
                         switch (la * 3 + lb) {
-                            case 0: grad_pair_images<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            case 1: grad_pair_images<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            case 3: grad_pair_images<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            default: grad_pair_images<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 0: grad_pair_image<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 1: grad_pair_image<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 3: grad_pair_image<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            default: grad_pair_image<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
                         }
                     } else {
                         switch (la * 3 + lb) {
-                            case 2: grad_pair_images<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            case 5: grad_pair_images<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            case 6: grad_pair_images<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            case 7: grad_pair_images<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
-                            default: grad_pair_images<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, rg, acc); break;
+                            case 2: grad_pair_image<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 5: grad_pair_image<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 6: grad_pair_image<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 7: grad_pair_image<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            default: grad_pair_image<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
                         }
                     }
-                    continue;
+                };
+                int cursor = 0;  // queued candidates, uniform over the wavefront
+                for (int c0 = 0; c0 < ncand; c0 += 64) {
+                    const int c = c0 + lane;
+                    bool inside = false;
+                    if (c < ncand) {
+                        int sb, n0, n1, n2;
+                        decode(c, sb, n0, n1, n2);
+                        const int ob = b.sh_atom[sb];
+                        if (!(ob == a && n0 == 0 && n1 == 0 && n2 == 0)) {
+                            double tv[3];
+                            image_vector(cr, n0, n1, n2, tv);
+                            const double dx = x[3 * a] - x[3 * ob] - tv[0], dy = x[3 * a + 1] - x[3 * ob + 1] - tv[1],
+                                         dz = x[3 * a + 2] - x[3 * ob + 2] - tv[2];
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            inside = r2 > 1e-12 && r2 <= c2p;
+                        }
+                    }
+                    const unsigned long long m = __ballot(inside);
+                    if (inside) pq[cursor + __popcll(m & ((1ull << lane) - 1ull))] = c;
+                    cursor += __popcll(m);
+                    __syncthreads();
+                    if (cursor >= 64) {
+                        evaluate(pq[lane]);
+                        __syncthreads();
+                        if (lane < cursor - 64) pq[lane] = pq[64 + lane];
+                        cursor -= 64;
+                        __syncthreads();
+                    }
                 }
+                if (lane < cursor) evaluate(pq[lane]);
+                __syncthreads();
+                continue;
+            }
+            for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
+                const int sb = b.sh_by_l[idx];
                 const int ob = b.sh_atom[sb];
                 if (ob == a) continue;
                 const double r2 = dist2(b.atom_x, a, ob);
