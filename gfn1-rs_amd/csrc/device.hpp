@@ -91,6 +91,11 @@ struct CellRec {
 // G table entry: G (bohr^-1), w1 = 4 pi / V exp(-G^2 / 4 alpha^2) / G^2 (Ewald
 // 1/R), w3 = -1/2 * 2 pi / V * E1(G^2 / 4 alpha^2) (generalised Ewald R^-3).
 constexpr int kGStride = 5;
+// Charge structure factors per reciprocal vector, rewritten before every Ewald
+// gradient: sum_j w_j cos(G.R_j) and sum_j w_j sin(G.R_j) over the shells j of
+// the instance for the weights w = q_j, q_j / h_j and q_j / h_j^2 (h_j the shell
+// hardness), cosine sums first.
+constexpr int kGsfStride = 6;
 // Cutoffs of the periodic path: AO image sums (pbc/mod.rs:170), the Ewald
 // real-space remainder (:162), the Ewald convergence factor tau (pbc/ewald.rs:39).
 constexpr double kPbcAoCutoff = 30.0;
@@ -131,6 +136,7 @@ struct DeviceBatch {
     const CellRec* cell;     // ninst
     const double* gtab;      // kGStride per reciprocal vector
     const double* gam;       // shell Coulomb matrices of the periodic instances
+    double* gsf;             // kGsfStride per reciprocal vector (gradient scratch)
 };
 
 __host__ __device__ inline bool periodic_instance(const DeviceBatch& b, int inst) {

@@ -237,6 +237,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 gam_size += round_up(nsh * nsh, kRangeAlign);
                 h_pinst_.push_back(i);
                 max_pnat_ = std::max(max_pnat_, s.nat);
+                max_png_ = std::max(max_png_, (int)cells[i].ng);
                 for (int a = 0; a < s.nat; ++a) h_patoms_.push_back(h_atom0_[i] + a);
             }
         }
@@ -362,6 +363,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         if (!(st = dalloc(&d_gtab_, std::max<size_t>(gtab.size(), 1))).ok()) return st;
         HIP_TRY(copy_bytes(d_gtab_, gtab.data(), gtab.size() * sizeof(double), hipMemcpyHostToDevice));
         if (!(st = dalloc(&d_gam_, std::max<int64_t>(gam_size, 1))).ok()) return st;
+        if (!(st = dalloc(&d_gsf_, gtab.size() / kGStride * kGsfStride)).ok()) return st;
         if (!(st = up_i(&d_pinst_, h_pinst_)).ok()) return st;
         if (!(st = up_i(&d_patoms_, h_patoms_)).ok()) return st;
     }
@@ -471,6 +473,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     db_.cell = d_cell_;
     db_.gtab = d_gtab_;
     db_.gam = d_gam_;
+    db_.gsf = d_gsf_;
     h_final_parity_.assign(ninst_, 0);
     return Status{};
 }
@@ -1682,8 +1685,8 @@ Status Batch::gradient_phase() {
                            (const double*)d_qout_, cn_on, opt_.include_scc, d_dedcn_h_, d_gel_, na);
         LAUNCH_CHECK();
         // the Ewald Coulomb gradient of the periodic instances
-        HIP_TRY(launch_ewald_grad(stream_, db_, t, d_patoms_, (int)h_patoms_.size(), (const double*)d_qout_,
-                                  d_gel_));
+        HIP_TRY(launch_ewald_grad(stream_, db_, t, d_pinst_, (int)h_pinst_.size(), max_png_, d_patoms_,
+                                  (int)h_patoms_.size(), (const double*)d_qout_, d_gel_));
         if (cn_on) {
             hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                                (const double*)d_dedcn_h_, opt_.coordination_cutoff, 1e-12, d_gel_, na);

@@ -316,9 +316,9 @@ class Batch {
     // periodic instances: cell records, reciprocal vectors, Ewald shell matrices
     std::vector<unsigned char> h_pbc_;
     std::vector<int> h_pinst_, h_patoms_;
-    int max_pnat_ = 0;
+    int max_pnat_ = 0, max_png_ = 0;
     CellRec* d_cell_ = nullptr;
-    double *d_gtab_ = nullptr, *d_gam_ = nullptr;
+    double *d_gtab_ = nullptr, *d_gam_ = nullptr, *d_gsf_ = nullptr;
     int *d_pinst_ = nullptr, *d_patoms_ = nullptr;
     bool periodic(int inst) const { return !h_pbc_.empty() && h_pbc_[inst]; }
 

@@ -50,121 +50,162 @@ __device__ __forceinline__ void image_ranges(const CellRec& c, double cut, int r
     for (int k = 0; k < 3; ++k) r[k] = (int)ceil(cut / c.h[k]) + 1;
 }
 
-__device__ double block_sum_p(double v, double* lds) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    __syncthreads();
-    if (lane == 0) lds[wid] = v;
-    __syncthreads();
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += lds[w];
-    __syncthreads();
-    return s;
-}
-
 }  // namespace
 
-// Gamma_ij of every periodic instance (pbc/ewald.rs:200-320).  One work-group
-// per (atom a, instance pinst[blockIdx.y]); each thread takes an atom b <= a
-// and walks the lattice images once: the parts that depend on the distance
-// only (erfc / R, the R^-3 real-space factor, the reciprocal sums) are summed
-// per atom pair, the short-range remainder per shell pair (it depends on the
-// pair's eta), and the thread writes Gamma_ij and Gamma_ji for the shells of a
-// and b (the matrix is exactly symmetric).
-__global__ void __launch_bounds__(128) k_ewald_gamma(DeviceBatch b, DeviceTables t, const int* pinst) {
+// Decodes p = la (la + 1) / 2 + lb, lb <= la (the lower triangle row by row).
+__device__ __forceinline__ void triangle_pair(int p, int& la, int& lb) {
+    la = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+    while (la * (la + 1) / 2 > p) --la;
+    while ((la + 1) * (la + 2) / 2 <= p) ++la;
+    lb = p - la * (la + 1) / 2;
+}
+
+// Gamma_ij of every periodic instance (pbc/ewald.rs:200-320).  A thread per
+// atom pair (a, b <= a) of an instance, 64 pairs per work-group, the instance
+// in blockIdx.y.  The thread walks the lattice images of its pair once: the
+// parts that depend on the distance only (erfc / R, the R^-3 real-space factor,
+// the reciprocal sums) are summed per atom pair, the short-range remainder per
+// shell pair (it depends on the pair's eta; the at most 3 x 3 sums sit in an
+// LDS column of the thread), and the thread writes Gamma_ij and Gamma_ji for the shells
+// of a and b (the matrix is exactly symmetric).  Images are tested on R^2
+// first; the inclusion tests themselves are the reference's, on R.
+__global__ void __launch_bounds__(64) k_ewald_gamma(DeviceBatch b, DeviceTables t, const int* pinst) {
     const int inst = pinst[blockIdx.y];
     const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
-    const int la = blockIdx.x;
-    if (la >= nat) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat * (nat + 1) / 2) return;
+    int la, lb;
+    triangle_pair(p, la, lb);
     const CellRec& c = b.cell[inst];
     const double alpha = c.alpha;
     const double rcut = kEwaldTau / alpha;
     const double realcut = fmax(rcut, kEwaldShortCutoff);
+    // R^2 above this is certainly outside realcut after the square root
+    const double pre2 = realcut * realcut * (1.0 + 1e-12);
     int rg[3];
     image_ranges(c, realcut, rg);
     const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const double* x = b.atom_x;
     const double* gt = b.gtab + c.g0 * kGStride;
     double* gam = const_cast<double*>(b.gam) + c.gam0;
-    const int A = a0 + la;
+    const int A = a0 + la, B = a0 + lb;
     const int fa = b.atom_sh0[A], na = t.elements[b.atom_z[A]].nshell;
-    for (int lb = threadIdx.x; lb <= la; lb += blockDim.x) {
-        const int B = a0 + lb;
-        const int fb = b.atom_sh0[B], nb = t.elements[b.atom_z[B]].nshell;
-        const bool same = A == B;
-        double ie2[kMaxShellsPerAtom][kMaxShellsPerAtom], eta[kMaxShellsPerAtom][kMaxShellsPerAtom];
-        double sr[kMaxShellsPerAtom][kMaxShellsPerAtom];
-        for (int p = 0; p < na; ++p)
-            for (int q = 0; q < nb; ++q) {
-                const double hi = t.shells[b.sh_type[fa + p]].hardness, hj = t.shells[b.sh_type[fb + q]].hardness;
-                eta[p][q] = 2.0 / (1.0 / hi + 1.0 / hj);
-                ie2[p][q] = 1.0 / (eta[p][q] * eta[p][q]);
-                sr[p][q] = 0.0;
-            }
-        const double rab[3] = {x[3 * A] - x[3 * B], x[3 * A + 1] - x[3 * B + 1], x[3 * A + 2] - x[3 * B + 2]};
-        double real1 = 0.0, real3 = 0.0;
-        for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-            for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-                for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                    double d[3];
-                    for (int k = 0; k < 3; ++k) d[k] = rab[k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
-                    const double r = sqrt(sum3(d[0], d[1], d[2]));
-                    // r <= 1e-12 is the atom itself in the origin cell
-                    if (r <= 1e-12 || r > realcut) continue;
-                    if (r <= rcut) {
-                        const double ar = alpha * r, ec = erfc(ar), r3 = r * r * r;
-                        real1 += ec / r;
-                        real3 += (ec + (2.0 * ar / kSqrtPi) * exp(-ar * ar)) / r3;
-                    }
-                    if (r <= kEwaldShortCutoff)
-                        for (int p = 0; p < na; ++p)
-                            for (int q = 0; q < nb; ++q) {
-                                double v, dv;
-                                ko_short(r, ie2[p][q], v, dv);
-                                sr[p][q] += v;
-                            }
+    const int fb = b.atom_sh0[B], nb = t.elements[b.atom_z[B]].nshell;
+    const bool same = A == B;
+    constexpr int M = kMaxShellsPerAtom;
+    // eta^-2 and the remainder sums of the shell pairs, one column per lane
+    __shared__ double ie2[M * M][64], sr[M * M][64];
+    const int lane = threadIdx.x;
+    const int npq = na * nb;
+    for (int k = 0; k < npq; ++k) {
+        const double hi = t.shells[b.sh_type[fa + k / nb]].hardness, hj = t.shells[b.sh_type[fb + k % nb]].hardness;
+        const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+        ie2[k][lane] = 1.0 / (eta * eta);
+        sr[k][lane] = 0.0;
+    }
+    const double rab[3] = {x[3 * A] - x[3 * B], x[3 * A + 1] - x[3 * B + 1], x[3 * A + 2] - x[3 * B + 2]};
+    double real1 = 0.0, real3 = 0.0;
+    for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                double d[3];
+                for (int k = 0; k < 3; ++k) d[k] = rab[k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
+                const double r2 = sum3(d[0], d[1], d[2]);
+                if (r2 > pre2) continue;
+                const double r = sqrt(r2);
+                // r <= 1e-12 is the atom itself in the origin cell
+                if (r <= 1e-12 || r > realcut) continue;
+                if (r <= rcut) {
+                    const double ar = alpha * r, ec = erfc(ar), r3 = r * r * r;
+                    real1 += ec / r;
+                    real3 += (ec + (2.0 * ar / kSqrtPi) * exp(-ar * ar)) / r3;
                 }
-        double rec1 = 0.0, rec3 = 0.0;
-        for (int g = 0; g < c.ng; ++g) {
-            const double* e = gt + (size_t)g * kGStride;
-            const double cs = cos(e[0] * rab[0] + e[1] * rab[1] + e[2] * rab[2]);
-            rec1 += e[3] * cs;
-            rec3 += e[4] * cs;
-        }
-        const double phi = real1 + rec1 + c.ew_bg + (same ? c.ew_self : 0.0);
-        for (int p = 0; p < na; ++p)
-            for (int q = 0; q < nb; ++q) {
-                const int i = fa + p - s0, j = fb + q - s0;
-                if (j > i) continue;  // the same atom: lower triangle of its block
-                double v = phi + (same ? eta[p][q] : 0.0);
-                v += kR3Coeff * ie2[p][q] * real3 + ie2[p][q] * rec3 + c.r3_k0 * ie2[p][q];
-                if (same) v += c.r3_self * ie2[p][q];
-                v += sr[p][q];
-                gam[(size_t)i * nsh + j] = v;
-                gam[(size_t)j * nsh + i] = v;
+                if (r <= kEwaldShortCutoff)
+                    for (int k = 0; k < npq; ++k) {
+                        double v, dv;
+                        ko_short(r, ie2[k][lane], v, dv);
+                        sr[k][lane] += v;
+                    }
             }
+    double rec1 = 0.0, rec3 = 0.0;
+    for (int g = 0; g < c.ng; ++g) {
+        const double* e = gt + (size_t)g * kGStride;
+        const double cs = cos(e[0] * rab[0] + e[1] * rab[1] + e[2] * rab[2]);
+        rec1 += e[3] * cs;
+        rec3 += e[4] * cs;
+    }
+    const double phi = real1 + rec1 + c.ew_bg + (same ? c.ew_self : 0.0);
+    for (int k = 0; k < npq; ++k) {
+        const int p1 = k / nb, q1 = k % nb;
+        const int i = fa + p1 - s0, j = fb + q1 - s0;
+        if (j > i) continue;  // the same atom: lower triangle of its block
+        const double hi = t.shells[b.sh_type[fa + p1]].hardness, hj = t.shells[b.sh_type[fb + q1]].hardness;
+        const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+        const double e2 = ie2[k][lane];
+        double v = phi + (same ? eta : 0.0);
+        v += kR3Coeff * e2 * real3 + e2 * rec3 + c.r3_k0 * e2;
+        if (same) v += c.r3_self * e2;
+        v += sr[k][lane];
+        gam[(size_t)i * nsh + j] = v;
+        gam[(size_t)j * nsh + i] = v;
+    }
+}
+
+// Charge structure factors of every periodic instance at the converged shell
+// charges (the reciprocal sums of pbc/gradient.rs:679-779 factorised): a thread
+// per (instance, G), the shells of the instance summed in their order.
+__global__ void __launch_bounds__(64) k_ewald_sfac(DeviceBatch b, DeviceTables t, const int* pinst, const double* q) {
+    const int inst = pinst[blockIdx.y];
+    const CellRec& c = b.cell[inst];
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= c.ng) return;
+    const double* e = b.gtab + (c.g0 + g) * kGStride;
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    double cs[3] = {0.0, 0.0, 0.0}, sn[3] = {0.0, 0.0, 0.0};
+    for (int o = a0; o < a1; ++o) {
+        double so, co;
+        sincos(e[0] * x[3 * o] + e[1] * x[3 * o + 1] + e[2] * x[3 * o + 2], &so, &co);
+        const int f = b.atom_sh0[o], ns = t.elements[b.atom_z[o]].nshell;
+        for (int k = 0; k < ns; ++k) {
+            const double qj = q[f + k], ih = 1.0 / t.shells[b.sh_type[f + k]].hardness;
+            const double w[3] = {qj, qj * ih, qj * ih * ih};
+            for (int m = 0; m < 3; ++m) {
+                cs[m] += w[m] * co;
+                sn[m] += w[m] * so;
+            }
+        }
+    }
+    double* out = b.gsf + (c.g0 + g) * kGsfStride;
+    for (int m = 0; m < 3; ++m) {
+        out[m] = cs[m];
+        out[3 + m] = sn[m];
     }
 }
 
 // Gradient of 1/2 q^T Gamma q at the converged shell charges for every atom of
 // a periodic instance (pbc/gradient.rs:548-603, :679-779), added to g_el.  One
-// 256-thread work-group per listed atom a; a work item is one (atom, image)
-// pair, whose erfc / R force uses the atomic charges and whose R^-3 and
-// remainder forces the shell pairs of the two atoms; the reciprocal forces go
-// over the G vectors.  Fixed thread assignment and a fixed reduction order keep
-// the result deterministic.
-__global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables t, const int* patoms,
-                                                    const double* q, double* g_el) {
-    __shared__ double lds[8];
+// 64-lane work-group per listed atom a.  Real space: a work item is one (plane
+// n0 of images, atom o) pair, o fastest over the lanes; it walks the images of
+// its plane, the erfc / R force with the atomic charges, the R^-3 and remainder
+// forces with the shell pairs of a and o (their charge products and eta^-2 in
+// registers).  Reciprocal space: a lane per G over the structure factors of
+// k_ewald_sfac, with 1/eta_ij^2 = (1/h_i + 1/h_j)^2 / 4 expanded so that the
+// shell sum over j factorises.  Fixed lane assignment and a fixed reduction
+// order keep the result deterministic.
+__global__ void __launch_bounds__(64) k_ewald_grad(DeviceBatch b, DeviceTables t, const int* patoms,
+                                                   const double* q, double* g_el) {
     const int a = patoms[blockIdx.x];
+    const int lane = threadIdx.x;
     const int inst = b.atom_inst[a];
     const CellRec& c = b.cell[inst];
     const double alpha = c.alpha;
     const double rcut = kEwaldTau / alpha, realcut = fmax(rcut, kEwaldShortCutoff);
+    const double pre2 = realcut * realcut * (1.0 + 1e-12);
     const double* x = b.atom_x;
     const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
-    const int s0 = b.sh0[inst], nsh = b.nsh[inst];
     const int fa = b.atom_sh0[a], nsa = t.elements[b.atom_z[a]].nshell;
+    constexpr int M = kMaxShellsPerAtom;
     auto qatom = [&](int o) {
         const int f = b.atom_sh0[o], ns = t.elements[b.atom_z[o]].nshell;
         double s = 0.0;
@@ -174,86 +215,77 @@ __global__ void __launch_bounds__(256) k_ewald_grad(DeviceBatch b, DeviceTables 
     const double qa = qatom(a);
     int rg[3];
     image_ranges(c, realcut, rg);
-    const int nimg = (2 * rg[0] + 1) * (2 * rg[1] + 1) * (2 * rg[2] + 1);
+    const int nat = a1 - a0, d0 = 2 * rg[0] + 1;
     double acc[3] = {0.0, 0.0, 0.0};
-    const int nat = a1 - a0;
     const double tas = 2.0 * alpha / kSqrtPi;
-    for (int w = threadIdx.x; w < nat * nimg; w += blockDim.x) {
-        const int o = a0 + w / nimg, im = w % nimg;
-        const int n0 = im / ((2 * rg[1] + 1) * (2 * rg[2] + 1)) - rg[0];
-        const int n1 = (im / (2 * rg[2] + 1)) % (2 * rg[1] + 1) - rg[1];
-        const int n2 = im % (2 * rg[2] + 1) - rg[2];
-        double d[3];
-        for (int k = 0; k < 3; ++k)
-            d[k] = x[3 * a + k] - x[3 * o + k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
-        const double r = sqrt(sum3(d[0], d[1], d[2]));
-        // r <= 1e-12 is the atom itself in the origin cell
-        if (r <= 1e-12 || r > realcut) continue;
+    // charge products and eta^-2 of the shell pairs (a, o), one column per lane
+    __shared__ double qq[M * M][64], ie2[M * M][64];
+    for (int w = lane; w < nat * d0; w += 64) {
+        const int n0 = w / nat - rg[0], o = a0 + w % nat;
         const int fo = b.atom_sh0[o], nso = t.elements[b.atom_z[o]].nshell;
-        double rad = 0.0;
-        if (r <= rcut) {
-            const double ar = alpha * r, ex = exp(-ar * ar), ec = erfc(ar);
-            // 1/R Ewald force with the atomic charges
-            rad += qa * qatom(o) * (-ec / (r * r) - tas * ex / r);
-            // R^-3 real-space force: d/dr [q(r) / r^3] times sum_ij q_i q_j eta_ij^-2
-            const double qq3 = ec + (2.0 * ar / kSqrtPi) * ex;
-            const double dq3 = -(4.0 * alpha * alpha * alpha * r * r / kSqrtPi) * ex;
-            const double d3 = dq3 / (r * r * r) - 3.0 * qq3 / (r * r * r * r);
-            double s3 = 0.0;
-            for (int k2 = 0; k2 < nsa; ++k2)
-                for (int k3 = 0; k3 < nso; ++k3) {
-                    const double hi = t.shells[b.sh_type[fa + k2]].hardness, hj = t.shells[b.sh_type[fo + k3]].hardness;
-                    const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
-                    s3 += q[fa + k2] * q[fo + k3] / (eta * eta);
-                }
-            rad += kR3Coeff * d3 * s3;
+        const double qq1 = qa * qatom(o);
+        const int npq = nsa * nso;
+        double s3 = 0.0;
+        for (int k = 0; k < npq; ++k) {
+            const int k2 = k / nso, k3 = k % nso;
+            const double hi = t.shells[b.sh_type[fa + k2]].hardness, hj = t.shells[b.sh_type[fo + k3]].hardness;
+            const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
+            ie2[k][lane] = 1.0 / (eta * eta);
+            qq[k][lane] = q[fa + k2] * q[fo + k3];
+            s3 += q[fa + k2] * q[fo + k3] / (eta * eta);
         }
-        if (r <= kEwaldShortCutoff)
-            for (int k2 = 0; k2 < nsa; ++k2)
-                for (int k3 = 0; k3 < nso; ++k3) {
-                    const double hi = t.shells[b.sh_type[fa + k2]].hardness, hj = t.shells[b.sh_type[fo + k3]].hardness;
-                    const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
-                    double v, dv;
-                    ko_short(r, 1.0 / (eta * eta), v, dv);
-                    rad += q[fa + k2] * q[fo + k3] * dv;
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                double d[3];
+                for (int k = 0; k < 3; ++k)
+                    d[k] = x[3 * a + k] - x[3 * o + k] - (n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k]);
+                const double r2 = sum3(d[0], d[1], d[2]);
+                if (r2 > pre2) continue;
+                const double r = sqrt(r2);
+                // r <= 1e-12 is the atom itself in the origin cell
+                if (r <= 1e-12 || r > realcut) continue;
+                double rad = 0.0;
+                if (r <= rcut) {
+                    const double ar = alpha * r, ex = exp(-ar * ar), ec = erfc(ar);
+                    // 1/R Ewald force with the atomic charges
+                    rad += qq1 * (-ec / (r * r) - tas * ex / r);
+                    // R^-3 real-space force: d/dr [q(r) / r^3] times sum_ij q_i q_j eta_ij^-2
+                    const double qq3 = ec + (2.0 * ar / kSqrtPi) * ex;
+                    const double dq3 = -(4.0 * alpha * alpha * alpha * r * r / kSqrtPi) * ex;
+                    const double dd3 = dq3 / (r * r * r) - 3.0 * qq3 / (r * r * r * r);
+                    rad += kR3Coeff * dd3 * s3;
                 }
-        for (int k = 0; k < 3; ++k) acc[k] += d[k] * (rad / r);
+                if (r <= kEwaldShortCutoff)
+                    for (int k = 0; k < npq; ++k) {
+                        double v, dv;
+                        ko_short(r, ie2[k][lane], v, dv);
+                        rad += qq[k][lane] * dv;
+                    }
+                for (int k = 0; k < 3; ++k) acc[k] += d[k] * (rad / r);
+            }
     }
     // reciprocal 1/R force (atomic charges) and reciprocal R^-3 force (shell
     // charges, coefficient -w3 per G: pbc/gradient.rs:686-706)
     const double* gt = b.gtab + c.g0 * kGStride;
-    for (int g = threadIdx.x; g < c.ng; g += blockDim.x) {
+    const double* sf = b.gsf + c.g0 * kGsfStride;
+    for (int g = lane; g < c.ng; g += 64) {
         const double* e = gt + (size_t)g * kGStride;
-        double sc = 0.0, ss = 0.0;
-        for (int o = a0; o < a1; ++o) {
-            const double ph = e[0] * x[3 * o] + e[1] * x[3 * o + 1] + e[2] * x[3 * o + 2];
-            const double qo = qatom(o);
-            sc += qo * cos(ph);
-            ss += qo * sin(ph);
-        }
-        const double pha = e[0] * x[3 * a] + e[1] * x[3 * a + 1] + e[2] * x[3 * a + 2];
-        const double f1 = sin(pha) * sc - cos(pha) * ss;
-        double s = -e[3] * qa * f1;
+        const double* f = sf + (size_t)g * kGsfStride;
+        double sa, ca;
+        sincos(e[0] * x[3 * a] + e[1] * x[3 * a + 1] + e[2] * x[3 * a + 2], &sa, &ca);
+        // sum_j w_j sin(phi_a - phi_j) for the three weights
+        const double x1 = sa * f[0] - ca * f[3], x2 = sa * f[1] - ca * f[4], x3 = sa * f[2] - ca * f[5];
+        double s = -e[3] * qa * x1;
         for (int k2 = 0; k2 < nsa; ++k2) {
             const int si = fa + k2;
-            if (q[si] == 0.0) continue;
-            const double hi = t.shells[b.sh_type[si]].hardness;
-            double f = 0.0;
-            for (int j = 0; j < nsh; ++j) {
-                const int sj = s0 + j;
-                if (q[sj] == 0.0) continue;
-                const int aj = b.sh_atom[sj];
-                const double hj = t.shells[b.sh_type[sj]].hardness;
-                const double eta = 2.0 / (1.0 / hi + 1.0 / hj);
-                const double phj = e[0] * x[3 * aj] + e[1] * x[3 * aj + 1] + e[2] * x[3 * aj + 2];
-                f += q[sj] * sin(pha - phj) / (eta * eta);
-            }
-            s += -e[4] * q[si] * f;
+            const double ih = 1.0 / t.shells[b.sh_type[si]].hardness;
+            s += -e[4] * q[si] * 0.25 * (ih * ih * x1 + 2.0 * ih * x2 + x3);
         }
         for (int k = 0; k < 3; ++k) acc[k] += e[k] * s;
     }
-    for (int k = 0; k < 3; ++k) acc[k] = block_sum_p(acc[k], lds);
-    if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
+    if (lane == 0)
         for (int k = 0; k < 3; ++k) g_el[3 * a + k] += acc[k];
 }
 
@@ -372,14 +404,20 @@ bool make_cell_record(const double* cell, CellRec& c, std::vector<double>& gtab)
 hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
                               int npinst, int max_nat) {
     if (npinst == 0 || max_nat == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ewald_gamma, dim3(max_nat, npinst), dim3(128), 0, s, b, t, pinst);
+    const int max_pairs = max_nat * (max_nat + 1) / 2;
+    hipLaunchKernelGGL(k_ewald_gamma, dim3((max_pairs + 63) / 64, npinst), dim3(64), 0, s, b, t, pinst);
     return hipGetLastError();
 }
 
-hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* patoms,
-                             int npatoms, const double* q, double* g_el) {
-    if (npatoms == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ewald_grad, dim3(npatoms), dim3(256), 0, s, b, t, patoms, q, g_el);
+hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
+                             int npinst, int max_ng, const int* patoms, int npatoms, const double* q, double* g_el) {
+    if (npatoms == 0 || npinst == 0) return hipSuccess;
+    if (max_ng > 0) {
+        hipLaunchKernelGGL(k_ewald_sfac, dim3((max_ng + 63) / 64, npinst), dim3(64), 0, s, b, t, pinst, q);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_ewald_grad, dim3(npatoms), dim3(64), 0, s, b, t, patoms, q, g_el);
     return hipGetLastError();
 }
 
