@@ -1588,14 +1588,23 @@ __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTabl
         }
 }
 
-// S(Gamma) and H0(Gamma) for one (shell of A, shell of B) pair of a periodic
-// instance (pbc/bloch.rs:58-236): the blocks of every image of B within 30
-// bohr summed, the on-site formula for B = A in the origin cell, the off-site
-// one at the image distance otherwise; writes both triangles.
+// Images listed per lane at a time in s_h0_pair_pbc.
+constexpr int kImageList = 16;
+
+// S(Gamma) and H0(Gamma) for the (shell sa of A, shell sb of B) pairs of one
+// periodic wavefront, one pair per active lane (pbc/bloch.rs:58-236): the
+// blocks of every image of B within 30 bohr summed, the on-site formula for
+// B = A in the origin cell, the off-site one at the image distance otherwise;
+// writes both triangles.  Every lane first lists the images inside the cutoff
+// for its own pair in its LDS column il[.][lane], kImageList at a time and in
+// the order of the image loop, then the wavefront evaluates the listed blocks
+// in lockstep, so no lane waits on an image that is inside only for another
+// lane's pair.  All 64 lanes call this (inactive ones with sb = sa).
 template <int LA, int LB>
 __device__ __forceinline__ void s_h0_pair_pbc(const DeviceBatch& b, const DeviceTables& t, const double* cn,
-                                              const CellRec& cr, const int* rg, int a, int sa, int sb, int ld,
-                                              gptr<double> S, gptr<double> H) {
+                                              const CellRec& cr, const int* rg, bool active, int a, int sa, int sb,
+                                              int ld, gptr<double> S, gptr<double> H, int (*il)[64]) {
+    const int lane = threadIdx.x;
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
@@ -1611,29 +1620,56 @@ __device__ __forceinline__ void s_h0_pair_pbc(const DeviceBatch& b, const Device
     const double hsc = hscale(t, za, A, zb, B);
     const double radsum = t.elements[za].atomic_radius + t.elements[zb].atomic_radius;
     const double c2 = kPbcAoCutoff * kPbcAoCutoff;
-    for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
-                double tv[3];
-                image_vector(cr, n0, n1, n2, tv);
-                const double rb[3] = {x[3 * ob] + tv[0], x[3 * ob + 1] + tv[1], x[3 * ob + 2] + tv[2]};
-                const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (!same && (r2 <= kPairEps2 || r2 > c2)) continue;
-                double blk[NA * NB];
-                shell_block<LA, LB, false>(A, B, ra, rb, blk, nullptr);
-                double h = 0.5 * (sea + seb);
-                if (!same) {
-                    const double rr = sqrt(sqrt(r2) / radsum);
-                    h *= hsc * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
-                }
-#pragma unroll
-                for (int k = 0; k < NA * NB; ++k) {
-                    bs[k] += blk[k];
-                    bh[k] += blk[k] * h;
+    const int d1 = 2 * rg[1] + 1, d2 = 2 * rg[2] + 1, nimg = (2 * rg[0] + 1) * d1 * d2;
+    int pos = active ? 0 : nimg;
+    int n0 = -rg[0], n1 = -rg[1], n2 = -rg[2];
+    for (;;) {
+        int cnt = 0;
+        while (pos < nimg && cnt < kImageList) {
+            const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
+            double tv[3];
+            image_vector(cr, n0, n1, n2, tv);
+            const double dx = ra[0] - (x[3 * ob] + tv[0]), dy = ra[1] - (x[3 * ob + 1] + tv[1]),
+                         dz = ra[2] - (x[3 * ob + 2] + tv[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (same || (r2 > kPairEps2 && r2 <= c2)) il[cnt++][lane] = pos;
+            ++pos;
+            if (++n2 > rg[2]) {
+                n2 = -rg[2];
+                if (++n1 > rg[1]) {
+                    n1 = -rg[1];
+                    ++n0;
                 }
             }
+        }
+        int most = cnt;
+        for (int off = 32; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off, 64));
+        for (int k = 0; k < most; ++k) {
+            if (k >= cnt) continue;
+            const int ps = il[k][lane];
+            const int m0 = ps / (d1 * d2) - rg[0], m1 = (ps / d2) % d1 - rg[1], m2 = ps % d2 - rg[2];
+            const bool same = ob == a && m0 == 0 && m1 == 0 && m2 == 0;
+            double tv[3];
+            image_vector(cr, m0, m1, m2, tv);
+            const double rb[3] = {x[3 * ob] + tv[0], x[3 * ob + 1] + tv[1], x[3 * ob + 2] + tv[2]};
+            const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double blk[NA * NB];
+            shell_block<LA, LB, false>(A, B, ra, rb, blk, nullptr);
+            double h = 0.5 * (sea + seb);
+            if (!same) {
+                const double rr = sqrt(sqrt(r2) / radsum);
+                h *= hsc * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
+            }
+#pragma unroll
+            for (int k2 = 0; k2 < NA * NB; ++k2) {
+                bs[k2] += blk[k2];
+                bh[k2] += blk[k2] * h;
+            }
+        }
+        if (__ballot(pos < nimg) == 0) break;
+    }
+    if (!active) return;
     const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
 #pragma unroll
     for (int i = 0; i < NA; ++i)
@@ -1898,6 +1934,7 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
     const int ld = b.ld[inst];
     int rg[3] = {0, 0, 0};
     if constexpr (PBC) image_ranges(b.cell[inst], kPbcAoCutoff, rg);
+    __shared__ int il[PBC ? kImageList : 1][64];  // per-lane image lists (periodic)
     gptr<double> S = (gptr<double>)b.S[inst];
     gptr<double> H = (gptr<double>)b.H0[inst];
     const double c2 = cutoff * cutoff;
@@ -1910,28 +1947,44 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
         for (int lb = 0; lb < 3; ++lb) {
             // s/p classes and d classes go to separate instantiations (register budget)
             if ((la == 2 || lb == 2) != DCLASS) continue;
+            if constexpr (PBC) {
+                // rounds of 64 shells of the class, every lane in every call
+                const CellRec& cr = b.cell[inst];
+                for (int base = lo[lb]; base < lo[lb + 1]; base += 64) {
+                    const int idx = base + threadIdx.x;
+                    int sb = sa;
+                    bool act = false;
+                    if (idx < lo[lb + 1]) {
+                        sb = b.sh_by_l[idx];
+                        const int ob = b.sh_atom[sb];
+                        act = !(ob > a || (ob == a && sb > sa));
+                        if (!act) sb = sa;
+                    }
+                    if (__ballot(act) == 0) continue;
+                    if constexpr (!DCLASS) {
+                        switch (la * 3 + lb) {
+                            case 0: s_h0_pair_pbc<0, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            case 1: s_h0_pair_pbc<0, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            case 3: s_h0_pair_pbc<1, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            default: s_h0_pair_pbc<1, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                        }
+                    } else {
+                        switch (la * 3 + lb) {
+                            case 2: s_h0_pair_pbc<0, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            case 5: s_h0_pair_pbc<1, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            case 6: s_h0_pair_pbc<2, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            case 7: s_h0_pair_pbc<2, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                            default: s_h0_pair_pbc<2, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
+                        }
+                    }
+                }
+                continue;
+            }
             for (int idx = lo[lb] + threadIdx.x; idx < lo[lb + 1]; idx += blockDim.x) {
                 const int sb = b.sh_by_l[idx];
                 const int ob = b.sh_atom[sb];
                 if (ob > a || (ob == a && sb > sa)) continue;
-                if constexpr (PBC && !DCLASS) {
-                    const CellRec& cr = b.cell[inst];
-                    switch (la * 3 + lb) {
-                        case 0: s_h0_pair_pbc<0, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        case 1: s_h0_pair_pbc<0, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        case 3: s_h0_pair_pbc<1, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        default: s_h0_pair_pbc<1, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                    }
-                } else if constexpr (PBC) {
-                    const CellRec& cr = b.cell[inst];
-                    switch (la * 3 + lb) {
-                        case 2: s_h0_pair_pbc<0, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        case 5: s_h0_pair_pbc<1, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        case 6: s_h0_pair_pbc<2, 0>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        case 7: s_h0_pair_pbc<2, 1>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                        default: s_h0_pair_pbc<2, 2>(b, t, cn, cr, rg, a, sa, sb, ld, S, H); break;
-                    }
-                } else if constexpr (!DCLASS) {
+                if constexpr (!DCLASS) {
                     switch (la * 3 + lb) {
                         case 0: s_h0_pair<0, 0>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
                         case 1: s_h0_pair<0, 1>(b, t, cn, c2, a, sa, sb, ld, S, H); break;
