@@ -223,6 +223,13 @@ __device__ __forceinline__ void image_ranges(const CellRec& c, double cut, int r
 __device__ __forceinline__ void image_vector(const CellRec& c, int n0, int n1, int n2, double t[3]) {
     for (int k = 0; k < 3; ++k) t[k] = n0 * c.a[0][k] + n1 * c.a[1][k] + n2 * c.a[2][k];
 }
+// Overlap screen of the periodic image sums (basis.rs:373-380, applied in
+// pbc/bloch.rs:140-146 and pbc/gradient.rs:320): an atom pair whose most diffuse
+// primitives overlap below exp(-40) is skipped.
+__device__ __forceinline__ bool overlap_screened(const DeviceTables& t, int za, int zb, double r2) {
+    const double ea = t.elements[za].min_exponent, eb = t.elements[zb].min_exponent;
+    return r2 * ea * eb > 40.0 * (ea + eb);
+}
 
 // Deterministic block sum (blockDim.x a multiple of 64, <= 1024); result on every thread.
 __device__ double block_sum(double v, double* lds) {
@@ -1593,7 +1600,7 @@ constexpr int kImageList = 16;
 
 // S(Gamma) and H0(Gamma) for the (shell sa of A, shell sb of B) pairs of one
 // periodic wavefront, one pair per active lane (pbc/bloch.rs:58-236): the
-// blocks of every image of B within 30 bohr summed, the on-site formula for
+// blocks of every image of B within 30 bohr and outside the overlap screen summed, the on-site formula for
 // B = A in the origin cell, the off-site one at the image distance otherwise;
 // writes both triangles.  Every lane first lists the images inside the cutoff
 // for its own pair in its LDS column il[.][lane], kImageList at a time and in
@@ -1632,7 +1639,7 @@ __device__ __forceinline__ void s_h0_pair_pbc(const DeviceBatch& b, const Device
             const double dx = ra[0] - (x[3 * ob] + tv[0]), dy = ra[1] - (x[3 * ob + 1] + tv[1]),
                          dz = ra[2] - (x[3 * ob + 2] + tv[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
-            if (same || (r2 > kPairEps2 && r2 <= c2)) il[cnt++][lane] = pos;
+            if (same || (r2 > kPairEps2 && r2 <= c2 && !overlap_screened(t, za, zb, r2))) il[cnt++][lane] = pos;
             ++pos;
             if (++n2 > rg[2]) {
                 n2 = -rg[2];
@@ -2041,10 +2048,11 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
             if ((la == 2 || lb == 2) != DCLASS) continue;
             if constexpr (PBC) {
                 // Candidates are (shell of class lb, image) pairs, 64 at a time,
-                // one per lane; the ones inside the 30-bohr cutoff are compacted
-                // into an LDS queue (ballot order, so the order is fixed) and
-                // evaluated 64 at a time, one per lane, so the lanes do not idle
-                // on images that are out of reach for their own shell.
+                // one per lane; the ones inside the 30-bohr cutoff and outside the
+                // overlap screen are compacted into an LDS queue (ballot order, so
+                // the order is fixed) and evaluated 64 at a time, one per lane, so
+                // the lanes do not idle on images that are out of reach for their
+                // own shell.
                 const CellRec& cr = b.cell[inst];
                 const int d0 = 2 * rg[0] + 1, d1 = 2 * rg[1] + 1, d2 = 2 * rg[2] + 1;
                 const int nimg = d0 * d1 * d2, s_first = lo[lb];
@@ -2093,7 +2101,7 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
                             const double dx = x[3 * a] - x[3 * ob] - tv[0], dy = x[3 * a + 1] - x[3 * ob + 1] - tv[1],
                                          dz = x[3 * a + 2] - x[3 * ob + 2] - tv[2];
                             const double r2 = dx * dx + dy * dy + dz * dz;
-                            inside = r2 > 1e-12 && r2 <= c2p;
+                            inside = r2 > 1e-12 && r2 <= c2p && !overlap_screened(t, b.atom_z[a], b.atom_z[ob], r2);
                         }
                     }
                     const unsigned long long m = __ballot(inside);

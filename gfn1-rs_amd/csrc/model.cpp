@@ -464,6 +464,11 @@ Status Model::load_impl(const char* spec, const char* text, Model& m) {
             e.nao += t.nao;
         }
         e.neutral_electrons = occ.s + occ.p + occ.d;
+        e.min_exponent = INFINITY;
+        for (int s = 0; s < e.nshell; ++s) {
+            const ShellType& t = m.shells_[zz * kMaxShellsPerAtom + s];
+            for (int i = 0; i < t.nprim; ++i) e.min_exponent = std::min(e.min_exponent, t.alpha[i]);
+        }
     }
     return Status{};
 }

@@ -71,6 +71,7 @@ struct ElementType {
     double d3_r4r2 = 0.0;          // sqrt(0.5 <r4>/<r2> sqrt(Z))
     double d3_refcn[kMaxRef] = {};
     double neutral_electrons = 0.0;
+    double min_exponent = 0.0;     // smallest primitive exponent of the element's shells (basis.rs:361-371)
 };
 
 struct Globals {
