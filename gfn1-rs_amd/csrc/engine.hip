@@ -764,6 +764,12 @@ Status Batch::setup_phase() {
                        opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_, d_cnd3_, d_gw_,
                        d_gw_ + (size_t)na * kMaxRef, na);
     LAUNCH_CHECK();
+    if (!h_patoms_.empty()) {
+        hipLaunchKernelGGL(k_coordination_pbc, dim3((int)h_patoms_.size()), dim3(64), 0, stream_, db_, t,
+                           (const int*)d_patoms_, opt_.coordination_cutoff, opt_.enable_cn_hamiltonian, d_cn_,
+                           d_cnd3_, d_gw_, d_gw_ + (size_t)na * kMaxRef);
+        LAUNCH_CHECK();
+    }
     HIP_TRY(hipEventRecord(ev_cn_, stream_));
     // Ewald shell Coulomb matrices of the periodic instances (needed from the
     // first SCC iteration on; the buckets wait for ev_start_ below)
@@ -873,10 +879,21 @@ Status Batch::setup_phase() {
                        (const double*)d_gw_, (const double*)d_cnd3_, opt_.enable_dispersion, d_erep_atom_,
                        d_edisp_atom_, d_grep_, d_gdisp_, d_dedcn_disp_, na);
     LAUNCH_CHECK();
+    if (!h_patoms_.empty()) {
+        hipLaunchKernelGGL(k_geometry_terms_pbc, dim3((int)h_patoms_.size()), dim3(64), 0, stream_, db_, t,
+                           (const int*)d_patoms_, (const double*)d_gw_, opt_.enable_dispersion, d_erep_atom_,
+                           d_edisp_atom_, d_grep_, d_gdisp_, d_dedcn_disp_, na);
+        LAUNCH_CHECK();
+    }
     if (opt_.enable_dispersion) {
         hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                            (const double*)d_dedcn_disp_, 25.0, 1e-14, d_gdisp_, na);
         LAUNCH_CHECK();
+        if (!h_patoms_.empty()) {
+            hipLaunchKernelGGL(k_cn_chain_pbc, dim3((int)h_patoms_.size()), dim3(64), 0, stream_, db_, t,
+                               (const int*)d_patoms_, (const double*)d_dedcn_disp_, 25.0, 1e-14, d_gdisp_);
+            LAUNCH_CHECK();
+        }
     }
     hipLaunchKernelGGL(k_halogen, dim3((ninst_ + 63) / 64), dim3(64), 0, stream_, db_, t, d_ehal_, d_ghal_);
     LAUNCH_CHECK();
@@ -1691,6 +1708,12 @@ Status Batch::gradient_phase() {
             hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                                (const double*)d_dedcn_h_, opt_.coordination_cutoff, 1e-12, d_gel_, na);
             LAUNCH_CHECK();
+            if (!h_patoms_.empty()) {
+                hipLaunchKernelGGL(k_cn_chain_pbc, dim3((int)h_patoms_.size()), dim3(64), 0, stream_, db_, t,
+                                   (const int*)d_patoms_, (const double*)d_dedcn_h_, opt_.coordination_cutoff,
+                                   1e-12, d_gel_);
+                LAUNCH_CHECK();
+            }
         }
     }
     HIP_TRY(hipStreamSynchronize(stream_));

@@ -249,60 +249,10 @@ __device__ double block_sum(double v, double* lds) {
 // ---------------------------------------------------------------------------
 // Coordination numbers for H0 (cutoff cn_cut) and for D3 (25 bohr), plus the
 // D3 Gaussian reference weights (coordination.rs:51-94; dispersion.rs:1629-1692).
-__global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
-                               double* gw, double* dgw, int natoms) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natoms) return;
-    const int inst = b.atom_inst[a];
-    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
-    const double* x = b.atom_x;
-    const double rca = t.elements[b.atom_z[a]].covalent_radius;
-    double c = 0.0, cd = 0.0;
-    const double cc2 = cn_cut * cn_cut, cd2 = 25.0 * 25.0;
-    if (periodic_instance(b, inst)) {
-        // lattice-summed CN: every (atom, image) other than A itself in the
-        // origin cell, so a pair with its own images counts from both sides
-        // (coordination.rs:64-89 over pairlist.rs:244-317)
-        const CellRec& cr = b.cell[inst];
-        int rg[3];
-        image_ranges(cr, fmax(cn_cut, 25.0), rg);
-        for (int o = a0; o < a1; ++o) {
-            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
-            if (rc <= 1e-12) continue;
-            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                        if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
-                        double tv[3];
-                        image_vector(cr, n0, n1, n2, tv);
-                        const double dx = x[3 * o] + tv[0] - x[3 * a], dy = x[3 * o + 1] + tv[1] - x[3 * a + 1],
-                                     dz = x[3 * o + 2] + tv[2] - x[3 * a + 2];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 <= kPairEps2 || (r2 > cc2 && r2 > cd2)) continue;
-                        double v, dv;
-                        count_fn(sqrt(r2), rc, v, dv);
-                        if (r2 <= cc2) c += v;
-                        if (r2 <= cd2) cd += v;
-                    }
-        }
-    } else {
-        for (int o = a0; o < a1; ++o) {
-            if (o == a) continue;
-            const double r2 = dist2(x, a, o);
-            if (r2 <= kPairEps2) continue;
-            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
-            if (rc <= 1e-12) continue;
-            const double r = sqrt(r2);
-            double v, dv;
-            if (r2 <= cc2 || r2 <= cd2) count_fn(r, rc, v, dv);
-            if (r2 <= cc2) c += v;
-            if (r2 <= cd2) cd += v;
-        }
-    }
-    cn[a] = cn_on ? c : 0.0;
-    cn_d3[a] = cd;
-    // D3 reference weights
-    const ElementType& e = t.elements[b.atom_z[a]];
+// D3 reference weights of atom a and their CN derivatives from its D3
+// coordination number cd (dispersion.rs:1650-1700), written to gw / dgw.
+__device__ void d3_weights(const DeviceTables& t, int z, int a, double cd, double* gw, double* dgw) {
+    const ElementType& e = t.elements[z];
     const int nref = e.d3_nref;
     double mx = -INFINITY;
     for (int r = 0; r < nref; ++r) {
@@ -329,6 +279,81 @@ __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int
         gw[a * kMaxRef + r] = w;
         dgw[a * kMaxRef + r] = dw;
     }
+}
+
+__global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
+                               double* gw, double* dgw, int natoms) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natoms) return;
+    const int inst = b.atom_inst[a];
+    // periodic atoms: k_coordination_pbc
+    if (periodic_instance(b, inst)) return;
+    const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
+    const double* x = b.atom_x;
+    const double rca = t.elements[b.atom_z[a]].covalent_radius;
+    double c = 0.0, cd = 0.0;
+    const double cc2 = cn_cut * cn_cut, cd2 = 25.0 * 25.0;
+    for (int o = a0; o < a1; ++o) {
+        if (o == a) continue;
+        const double r2 = dist2(x, a, o);
+        if (r2 <= kPairEps2) continue;
+        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+        if (rc <= 1e-12) continue;
+        const double r = sqrt(r2);
+        double v, dv;
+        if (r2 <= cc2 || r2 <= cd2) count_fn(r, rc, v, dv);
+        if (r2 <= cc2) c += v;
+        if (r2 <= cd2) cd += v;
+    }
+    cn[a] = cn_on ? c : 0.0;
+    cn_d3[a] = cd;
+    d3_weights(t, b.atom_z[a], a, cd, gw, dgw);
+}
+
+// Lattice-summed coordination numbers of the listed periodic atoms: every
+// (atom, image) other than A itself in the origin cell, so a pair with its own
+// images counts from both sides (coordination.rs:64-89 over
+// pairlist.rs:244-317).  One 64-lane work-group per atom; a work item is one
+// (plane n0 of images, atom o) pair, o fastest over the lanes, that walks the
+// images of its plane; the lanes' sums are reduced in a fixed order.
+__global__ void __launch_bounds__(64) k_coordination_pbc(DeviceBatch b, DeviceTables t, const int* patoms,
+                                                         double cn_cut, int cn_on, double* cn, double* cn_d3,
+                                                         double* gw, double* dgw) {
+    const int a = patoms[blockIdx.x];
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const double* x = b.atom_x;
+    const double rca = t.elements[b.atom_z[a]].covalent_radius;
+    double c = 0.0, cd = 0.0;
+    const double cc2 = cn_cut * cn_cut, cd2 = 25.0 * 25.0;
+    const CellRec& cr = b.cell[inst];
+    int rg[3];
+    image_ranges(cr, fmax(cn_cut, 25.0), rg);
+    for (int w = threadIdx.x; w < nat * (2 * rg[0] + 1); w += 64) {
+        const int n0 = w / nat - rg[0], o = a0 + w % nat;
+        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+        if (rc <= 1e-12) continue;
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
+                double tv[3];
+                image_vector(cr, n0, n1, n2, tv);
+                const double dx = x[3 * o] + tv[0] - x[3 * a], dy = x[3 * o + 1] + tv[1] - x[3 * a + 1],
+                             dz = x[3 * o + 2] + tv[2] - x[3 * a + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= kPairEps2 || (r2 > cc2 && r2 > cd2)) continue;
+                double v, dv;
+                count_fn(sqrt(r2), rc, v, dv);
+                if (r2 <= cc2) c += v;
+                if (r2 <= cd2) cd += v;
+            }
+    }
+    c = wave_sum(c);
+    cd = wave_sum(cd);
+    if (threadIdx.x != 0) return;
+    cn[a] = cn_on ? c : 0.0;
+    cn_d3[a] = cd;
+    d3_weights(t, b.atom_z[a], a, cd, gw, dgw);
 }
 
 // (S/H0 kernel: see the shell-class kernels below)
@@ -1028,9 +1053,9 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
 }
 
 // ---------------------------------------------------------------------------
-// Geometry-only terms.  One thread per atom A, looping over the instance's
-// other atoms; every pair is evaluated from both sides so the per-atom sums
-// need no atomics and are deterministic.
+// Geometry-only terms of molecular atoms.  One thread per atom A, looping over
+// the instance's other atoms; every pair is evaluated from both sides so the
+// per-atom sums need no atomics and are deterministic.
 //   repulsion (repulsion.rs:39-59, :139-159), 25 bohr
 //   D3(BJ) two-body energy, gradient and dE/dCN (dispersion.rs:1232-1300)
 __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw, const double* cn_d3, int disp_on,
@@ -1039,6 +1064,8 @@ __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natoms) return;
     const int inst = b.atom_inst[a];
+    // periodic atoms: k_geometry_terms_pbc
+    if (periodic_instance(b, inst)) return;
     const int a0 = b.atom0[inst], a1 = b.atom0[inst + 1];
     const double* x = b.atom_x;
     const int za = b.atom_z[a];
@@ -1047,75 +1074,7 @@ __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw
     double er = 0.0, ed_sum = 0.0, gr[3] = {0, 0, 0}, gd[3] = {0, 0, 0}, dcn = 0.0;
     const double* gwa = gw + (size_t)a * kMaxRef;
     const double* dga = gw + (size_t)natoms * kMaxRef + (size_t)a * kMaxRef;  // dgw follows gw
-    if (periodic_instance(b, inst)) {
-        // Image sums (repulsion.rs:39-59, dispersion.rs:1258-1295 over
-        // pairlist.rs:244-317).  Every (atom, image) pair is seen from both
-        // atoms, so each adds half its energy; a pair of an atom with its own
-        // image moves nothing and adds no force, but its D3 term enters dE/dCN.
-        const CellRec& cr = b.cell[inst];
-        int rg[3];
-        image_ranges(cr, disp_on ? 50.0 : 25.0, rg);
-        for (int o = a0; o < a1; ++o) {
-            const int zb = b.atom_z[o];
-            const ElementType& EB = t.elements[zb];
-            double c6 = 0.0, dc6 = 0.0;
-            if (disp_on) {
-                const double* gwb = gw + (size_t)o * kMaxRef;
-                int hi, lo, ia_is_hi;
-                if (za > zb) { hi = za; lo = zb; ia_is_hi = 1; }
-                else { hi = zb; lo = za; ia_is_hi = 0; }
-                const size_t pair = (size_t)lo + (size_t)hi * (hi - 1) / 2 - 1;
-                const double* tab = t.d3_c6 + pair * kMaxRef * kMaxRef;
-                for (int ra = 0; ra < EA.d3_nref; ++ra)
-                    for (int rb = 0; rb < EB.d3_nref; ++rb) {
-                        double ref;
-                        if (za == zb) ref = tab[ra * kMaxRef + rb];
-                        else ref = ia_is_hi ? tab[rb * kMaxRef + ra] : tab[ra * kMaxRef + rb];
-                        c6 += gwa[ra] * gwb[rb] * ref;
-                        dc6 += dga[ra] * gwb[rb] * ref;
-                    }
-            }
-            const double alpha = sqrt(EA.repa * EB.repa);
-            const double zeff = EA.repb * EB.repb;
-            const double kx = (za <= 2 && zb <= 2) ? g.kexplight : g.kexp;
-            const double r4r2 = 3.0 * EA.d3_r4r2 * EB.d3_r4r2;
-            const double r0 = g.a1 * sqrt(r4r2) + g.a2;
-            const double r0_2 = r0 * r0, r0_6 = r0_2 * r0_2 * r0_2, r0_8 = r0_6 * r0_2;
-            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                        if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
-                        double tv[3];
-                        image_vector(cr, n0, n1, n2, tv);
-                        const double dr[3] = {x[3 * o] + tv[0] - x[3 * a], x[3 * o + 1] + tv[1] - x[3 * a + 1],
-                                              x[3 * o + 2] + tv[2] - x[3 * a + 2]};
-                        const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
-                        if (r2 <= kPairEps2) continue;
-                        const double r = sqrt(r2);
-                        if (r2 <= 25.0 * 25.0) {
-                            const double rk = pow(r, kx);
-                            const double e = zeff * exp(-alpha * rk) / r;
-                            const double pref = (alpha * rk * kx + 1.0) * e / (r * r);
-                            er += 0.5 * e;
-                            if (o != a)
-                                for (int d = 0; d < 3; ++d) gr[d] += dr[d] * pref;
-                        }
-                        if (disp_on && r2 <= 50.0 * 50.0 && r2 > 1e-14) {
-                            const double r4 = r2 * r2;
-                            const double t6 = 1.0 / (r4 * r2 + r0_6), t8 = 1.0 / (r4 * r4 + r0_8);
-                            const double d6 = -6.0 * r4 * t6 * t6, d8 = -8.0 * r4 * r2 * t8 * t8;
-                            const double ed = g.s6 * t6 + g.s8 * r4r2 * t8;
-                            const double gdv = g.s6 * d6 + g.s8 * r4r2 * d8;
-                            ed_sum -= 0.5 * c6 * ed;
-                            if (o != a)
-                                for (int d = 0; d < 3; ++d) gd[d] += (-dr[d]) * (-c6 * gdv);
-                            dcn -= dc6 * ed;
-                        }
-                    }
-        }
-    }
-    const int o_end = periodic_instance(b, inst) ? a0 : a1;  // molecular pairs
-    for (int o = a0; o < o_end; ++o) {
+    for (int o = a0; o < a1; ++o) {
         if (o == a) continue;
         const double r2 = dist2(x, a, o);
         if (r2 <= kPairEps2) continue;
@@ -1174,6 +1133,106 @@ __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw
     dedcn_disp[a] = dcn;
 }
 
+// Repulsion and D3 two-body terms of the listed periodic atoms as image sums
+// (repulsion.rs:39-59, dispersion.rs:1258-1295 over pairlist.rs:244-317).
+// Every (atom, image) pair is seen from both atoms, so each adds half its
+// energy; a pair of an atom with its own image moves nothing and adds no force,
+// but its D3 term enters dE/dCN.  One 64-lane work-group per atom; a work item
+// is one (plane n0 of images, atom o) pair, o fastest over the lanes, with the
+// pair's C6 and damping radii computed once for its plane; the lanes' sums are
+// reduced in a fixed order.
+__global__ void __launch_bounds__(64) k_geometry_terms_pbc(DeviceBatch b, DeviceTables t, const int* patoms,
+                                                           const double* gw, int disp_on, double* e_rep_atom,
+                                                           double* e_disp_atom, double* g_rep, double* g_disp,
+                                                           double* dedcn_disp, int natoms) {
+    const int a = patoms[blockIdx.x];
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const double* x = b.atom_x;
+    const int za = b.atom_z[a];
+    const ElementType& EA = t.elements[za];
+    const Globals& g = t.glob;
+    double er = 0.0, ed_sum = 0.0, gr[3] = {0, 0, 0}, gd[3] = {0, 0, 0}, dcn = 0.0;
+    const double* gwa = gw + (size_t)a * kMaxRef;
+    const double* dga = gw + (size_t)natoms * kMaxRef + (size_t)a * kMaxRef;  // dgw follows gw
+    const CellRec& cr = b.cell[inst];
+    int rg[3];
+    image_ranges(cr, disp_on ? 50.0 : 25.0, rg);
+    for (int w = threadIdx.x; w < nat * (2 * rg[0] + 1); w += 64) {
+        const int n0 = w / nat - rg[0], o = a0 + w % nat;
+        const int zb = b.atom_z[o];
+        const ElementType& EB = t.elements[zb];
+        double c6 = 0.0, dc6 = 0.0;
+        if (disp_on) {
+            const double* gwb = gw + (size_t)o * kMaxRef;
+            int hi, lo, ia_is_hi;
+            if (za > zb) { hi = za; lo = zb; ia_is_hi = 1; }
+            else { hi = zb; lo = za; ia_is_hi = 0; }
+            const size_t pair = (size_t)lo + (size_t)hi * (hi - 1) / 2 - 1;
+            const double* tab = t.d3_c6 + pair * kMaxRef * kMaxRef;
+            for (int ra = 0; ra < EA.d3_nref; ++ra)
+                for (int rb = 0; rb < EB.d3_nref; ++rb) {
+                    double ref;
+                    if (za == zb) ref = tab[ra * kMaxRef + rb];
+                    else ref = ia_is_hi ? tab[rb * kMaxRef + ra] : tab[ra * kMaxRef + rb];
+                    c6 += gwa[ra] * gwb[rb] * ref;
+                    dc6 += dga[ra] * gwb[rb] * ref;
+                }
+        }
+        const double alpha = sqrt(EA.repa * EB.repa);
+        const double zeff = EA.repb * EB.repb;
+        const double kx = (za <= 2 && zb <= 2) ? g.kexplight : g.kexp;
+        const double r4r2 = 3.0 * EA.d3_r4r2 * EB.d3_r4r2;
+        const double r0 = g.a1 * sqrt(r4r2) + g.a2;
+        const double r0_2 = r0 * r0, r0_6 = r0_2 * r0_2 * r0_2, r0_8 = r0_6 * r0_2;
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                if (o == a && n0 == 0 && n1 == 0 && n2 == 0) continue;
+                double tv[3];
+                image_vector(cr, n0, n1, n2, tv);
+                const double dr[3] = {x[3 * o] + tv[0] - x[3 * a], x[3 * o + 1] + tv[1] - x[3 * a + 1],
+                                      x[3 * o + 2] + tv[2] - x[3 * a + 2]};
+                const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+                if (r2 <= kPairEps2) continue;
+                if (r2 <= 25.0 * 25.0) {
+                    const double r = sqrt(r2);
+                    const double rk = pow(r, kx);
+                    const double e = zeff * exp(-alpha * rk) / r;
+                    const double pref = (alpha * rk * kx + 1.0) * e / (r * r);
+                    er += 0.5 * e;
+                    if (o != a)
+                        for (int d = 0; d < 3; ++d) gr[d] += dr[d] * pref;
+                }
+                if (disp_on && r2 <= 50.0 * 50.0 && r2 > 1e-14) {
+                    const double r4 = r2 * r2;
+                    const double t6 = 1.0 / (r4 * r2 + r0_6), t8 = 1.0 / (r4 * r4 + r0_8);
+                    const double d6 = -6.0 * r4 * t6 * t6, d8 = -8.0 * r4 * r2 * t8 * t8;
+                    const double ed = g.s6 * t6 + g.s8 * r4r2 * t8;
+                    const double gdv = g.s6 * d6 + g.s8 * r4r2 * d8;
+                    ed_sum -= 0.5 * c6 * ed;
+                    if (o != a)
+                        for (int d = 0; d < 3; ++d) gd[d] += (-dr[d]) * (-c6 * gdv);
+                    dcn -= dc6 * ed;
+                }
+            }
+    }
+    er = wave_sum(er);
+    ed_sum = wave_sum(ed_sum);
+    dcn = wave_sum(dcn);
+    for (int d = 0; d < 3; ++d) {
+        gr[d] = wave_sum(gr[d]);
+        gd[d] = wave_sum(gd[d]);
+    }
+    if (threadIdx.x != 0) return;
+    e_rep_atom[a] = er;
+    e_disp_atom[a] = ed_sum;
+    for (int d = 0; d < 3; ++d) {
+        g_rep[3 * a + d] = gr[d];
+        g_disp[3 * a + d] = gd[d];
+    }
+    dedcn_disp[a] = dcn;
+}
+
 // CN chain rule of a coordination number with cutoff cut:
 // g_A += sum_B (dE/dCN_A + dE/dCN_B) dCN/dr (R_A - R_B)/r (gradient.rs:475-489).
 __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, double cut, double reps, double* g,
@@ -1186,37 +1245,8 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
     const double rca = t.elements[b.atom_z[a]].covalent_radius;
     const double c2 = cut * cut;
     double acc[3] = {0, 0, 0};
-    if (periodic_instance(b, inst)) {
-        // every image of every other atom; an atom's own images are skipped
-        // (gradient.rs:477-479 over the periodic pair list)
-        const CellRec& cr = b.cell[inst];
-        int rg[3];
-        image_ranges(cr, cut, rg);
-        for (int o = a0; o < a1; ++o) {
-            if (o == a) continue;
-            const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
-            if (rc <= 1e-12) continue;
-            const double dedcn_ao = dedcn[a] + dedcn[o];
-            for (int n0 = -rg[0]; n0 <= rg[0]; ++n0)
-                for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
-                    for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
-                        double tv[3];
-                        image_vector(cr, n0, n1, n2, tv);
-                        const double rv[3] = {x[3 * a] - x[3 * o] - tv[0], x[3 * a + 1] - x[3 * o + 1] - tv[1],
-                                              x[3 * a + 2] - x[3 * o + 2] - tv[2]};
-                        const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-                        if (r2 <= kPairEps2 || r2 > c2) continue;
-                        const double r = sqrt(r2);
-                        if (r <= reps) continue;
-                        double v, dv;
-                        count_fn(r, rc, v, dv);
-                        const double pref = dedcn_ao * dv / r;
-                        for (int d = 0; d < 3; ++d) acc[d] += rv[d] * pref;
-                    }
-        }
-        for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
-        return;
-    }
+    // periodic atoms: k_cn_chain_pbc
+    if (periodic_instance(b, inst)) return;
     for (int o = a0; o < a1; ++o) {
         if (o == a) continue;
         const double r2 = dist2(x, a, o);
@@ -1231,6 +1261,49 @@ __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, d
         for (int d = 0; d < 3; ++d) acc[d] += (x[3 * a + d] - x[3 * o + d]) * pref;
     }
     for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
+}
+
+// CN chain rule of the listed periodic atoms: every image of every other atom;
+// an atom's own images are skipped (gradient.rs:477-479 over the periodic pair
+// list).  One 64-lane work-group per atom; a work item is one (plane n0 of
+// images, atom o) pair, o fastest over the lanes; fixed reduction order.
+__global__ void __launch_bounds__(64) k_cn_chain_pbc(DeviceBatch b, DeviceTables t, const int* patoms,
+                                                     const double* dedcn, double cut, double reps, double* g) {
+    const int a = patoms[blockIdx.x];
+    const int inst = b.atom_inst[a];
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const double* x = b.atom_x;
+    const double rca = t.elements[b.atom_z[a]].covalent_radius;
+    const double c2 = cut * cut;
+    double acc[3] = {0, 0, 0};
+    const CellRec& cr = b.cell[inst];
+    int rg[3];
+    image_ranges(cr, cut, rg);
+    for (int w = threadIdx.x; w < nat * (2 * rg[0] + 1); w += 64) {
+        const int n0 = w / nat - rg[0], o = a0 + w % nat;
+        if (o == a) continue;
+        const double rc = rca + t.elements[b.atom_z[o]].covalent_radius;
+        if (rc <= 1e-12) continue;
+        const double dedcn_ao = dedcn[a] + dedcn[o];
+        for (int n1 = -rg[1]; n1 <= rg[1]; ++n1)
+            for (int n2 = -rg[2]; n2 <= rg[2]; ++n2) {
+                double tv[3];
+                image_vector(cr, n0, n1, n2, tv);
+                const double rv[3] = {x[3 * a] - x[3 * o] - tv[0], x[3 * a + 1] - x[3 * o + 1] - tv[1],
+                                      x[3 * a + 2] - x[3 * o + 2] - tv[2]};
+                const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                if (r2 <= kPairEps2 || r2 > c2) continue;
+                const double r = sqrt(r2);
+                if (r <= reps) continue;
+                double v, dv;
+                count_fn(r, rc, v, dv);
+                const double pref = dedcn_ao * dv / r;
+                for (int d = 0; d < 3; ++d) acc[d] += rv[d] * pref;
+            }
+    }
+    for (int d = 0; d < 3; ++d) acc[d] = wave_sum(acc[d]);
+    if (threadIdx.x == 0)
+        for (int d = 0; d < 3; ++d) g[3 * a + d] += acc[d];
 }
 
 // Halogen bond of a periodic instance (halogen.rs:64-160 over the triples of

@@ -22,8 +22,12 @@ struct SccParams {
     int32_t rung_max_iter[4];
 };
 
+// Molecular atoms only; the listed periodic atoms go to the _pbc kernels (one
+// 64-lane work-group per atom, launched after their molecular counterparts).
 __global__ void k_coordination(DeviceBatch b, DeviceTables t, double cn_cut, int cn_on, double* cn, double* cn_d3,
                                double* gw, double* dgw, int natoms);
+__global__ void k_coordination_pbc(DeviceBatch b, DeviceTables t, const int* patoms, double cn_cut, int cn_on,
+                                   double* cn, double* cn_d3, double* gw, double* dgw);
 // S and H0 of the listed atoms' instances; PBC selects the periodic instances
 // (the Gamma-point image sums), the other instantiation the molecular ones.
 template <bool DCLASS, bool PBC>
@@ -55,8 +59,13 @@ __global__ void k_scc_update(DeviceBatch b, DeviceTables t, const int* act, SccP
 __global__ void k_geometry_terms(DeviceBatch b, DeviceTables t, const double* gw, const double* cn_d3, int disp_on,
                                  double* e_rep_atom, double* e_disp_atom, double* g_rep, double* g_disp,
                                  double* dedcn_disp, int natoms);
+__global__ void k_geometry_terms_pbc(DeviceBatch b, DeviceTables t, const int* patoms, const double* gw, int disp_on,
+                                     double* e_rep_atom, double* e_disp_atom, double* g_rep, double* g_disp,
+                                     double* dedcn_disp, int natoms);
 __global__ void k_cn_chain(DeviceBatch b, DeviceTables t, const double* dedcn, double cut, double reps, double* g,
                            int natoms);
+__global__ void k_cn_chain_pbc(DeviceBatch b, DeviceTables t, const int* patoms, const double* dedcn, double cut,
+                               double reps, double* g);
 __global__ void k_halogen(DeviceBatch b, DeviceTables t, double* e_hal, double* g_hal);
 __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* out);
 // Pair part of the electronic gradient for the listed atoms: g_pair[3 j + d] and
