@@ -85,6 +85,7 @@ struct CellRec {
     double r3_self;    // -1/2 * -(4 pi / 3) k^3, per eta^-2, same-atom pairs
     int64_t g0;        // first reciprocal vector of the instance in the G table
     int64_t gam0;      // offset of the instance's shell Coulomb matrix (nsh x nsh)
+    int64_t ket0;      // offset of the instance's ket-share rows of the pair gradient (4 nat x nat)
     int32_t ng;        // reciprocal vectors with 0 < |G| <= 2 alpha tau
     int32_t on;        // periodic instance
 };
@@ -101,6 +102,10 @@ constexpr int kGsfStride = 6;
 constexpr double kPbcAoCutoff = 30.0;
 constexpr double kEwaldShortCutoff = 10.0;
 constexpr double kEwaldTau = 5.5;
+// Largest periodic atom count for which the pair gradient evaluates each
+// symmetry-unique pair once (its ket-share accumulator, 32 bytes per atom, sits
+// in LDS); larger cells evaluate every pair from both sides.
+constexpr int kKetShareMaxAtoms = 512;
 
 // Flat (instance-major) index tables of a batch.
 struct DeviceBatch {

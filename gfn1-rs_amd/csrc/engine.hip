@@ -171,7 +171,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
     h_pbc_.assign(ninst_, 0);
     std::vector<CellRec> cells(ninst_);
     std::vector<double> gtab;
-    int64_t gam_size = 0;
+    int64_t gam_size = 0, ket_size = 0;
     for (int i = 0; i < ninst_; ++i) {
         const SystemInput& s = systems[i];
         h_charge_[i] = s.charge;
@@ -235,6 +235,8 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 h_pbc_[i] = 1;
                 cells[i].gam0 = gam_size;
                 gam_size += round_up(nsh * nsh, kRangeAlign);
+                cells[i].ket0 = ket_size;
+                ket_size += 4 * (int64_t)s.nat * s.nat;
                 h_pinst_.push_back(i);
                 max_pnat_ = std::max(max_pnat_, s.nat);
                 max_png_ = std::max(max_png_, (int)cells[i].ng);
@@ -364,6 +366,7 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         HIP_TRY(copy_bytes(d_gtab_, gtab.data(), gtab.size() * sizeof(double), hipMemcpyHostToDevice));
         if (!(st = dalloc(&d_gam_, std::max<int64_t>(gam_size, 1))).ok()) return st;
         if (!(st = dalloc(&d_gsf_, gtab.size() / kGStride * kGsfStride)).ok()) return st;
+        if (max_pnat_ <= kKetShareMaxAtoms && !(st = dalloc(&d_ket_, ket_size)).ok()) return st;
         if (!(st = up_i(&d_pinst_, h_pinst_)).ok()) return st;
         if (!(st = up_i(&d_patoms_, h_patoms_)).ok()) return st;
     }
@@ -1654,26 +1657,29 @@ Status Batch::launch_bucket_gradient(Bucket& bk) {
         auto pairs = opt_.mixed ? k_grad_pairs_t<false, true, false> : k_grad_pairs_t<false, false, false>;
         hipLaunchKernelGGL(pairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
                            (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
-                           opt_.integral_cutoff, bk.gpair, bk.dedcn);
+                           opt_.integral_cutoff, bk.gpair, bk.dedcn, (double*)nullptr);
         LAUNCH_CHECK();
         if (has_d_) {
             auto dpairs = opt_.mixed ? k_grad_pairs_t<true, true, false> : k_grad_pairs_t<true, false, false>;
             hipLaunchKernelGGL(dpairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
                                (const double*)d_cn_, (const double*)d_vout_t_, opt_.include_scc, cn_on,
-                               opt_.integral_cutoff, bk.gpair, bk.dedcn);
+                               opt_.integral_cutoff, bk.gpair, bk.dedcn, (double*)nullptr);
             LAUNCH_CHECK();
         }
         if (bk.has_pbc) {
+            // symmetry-unique periodic pairs with their ket shares in d_ket_ (LDS
+            // accumulator of 4 doubles per atom of the largest periodic instance)
+            const size_t ket_lds = d_ket_ ? (size_t)4 * max_pnat_ * sizeof(double) : 0;
             auto ppairs = opt_.mixed ? k_grad_pairs_t<false, true, true> : k_grad_pairs_t<false, false, true>;
-            hipLaunchKernelGGL(ppairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
+            hipLaunchKernelGGL(ppairs, dim3(bk.natoms), dim3(64), ket_lds, bk.stream, db_, t, (const int*)bk.atoms,
                                (const double*)d_cn_, (const double*)d_vout_t_, 1, cn_on, kPbcAoCutoff, bk.gpair,
-                               bk.dedcn);
+                               bk.dedcn, d_ket_);
             LAUNCH_CHECK();
             if (has_d_) {
                 auto pdpairs = opt_.mixed ? k_grad_pairs_t<true, true, true> : k_grad_pairs_t<true, false, true>;
-                hipLaunchKernelGGL(pdpairs, dim3(bk.natoms), dim3(64), 0, bk.stream, db_, t, (const int*)bk.atoms,
-                                   (const double*)d_cn_, (const double*)d_vout_t_, 1, cn_on, kPbcAoCutoff,
-                                   bk.gpair, bk.dedcn);
+                hipLaunchKernelGGL(pdpairs, dim3(bk.natoms), dim3(64), ket_lds, bk.stream, db_, t,
+                                   (const int*)bk.atoms, (const double*)d_cn_, (const double*)d_vout_t_, 1, cn_on,
+                                   kPbcAoCutoff, bk.gpair, bk.dedcn, d_ket_);
                 LAUNCH_CHECK();
             }
         }
@@ -1696,6 +1702,12 @@ Status Batch::gradient_phase() {
             hipLaunchKernelGGL(k_scatter_atoms, dim3((bk.natoms + 127) / 128), dim3(128), 0, stream_,
                                (const int*)bk.atoms, bk.natoms, (const double*)bk.gpair, (const double*)bk.dedcn,
                                d_gel_, d_dedcn_h_);
+            LAUNCH_CHECK();
+        }
+        if (d_ket_) {
+            const int np = (int)h_patoms_.size();
+            hipLaunchKernelGGL(k_ket_shares_merge, dim3((np + 127) / 128), dim3(128), 0, stream_, db_,
+                               (const int*)d_patoms_, np, (const double*)d_ket_, d_gel_, d_dedcn_h_);
             LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,

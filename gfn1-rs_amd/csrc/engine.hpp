@@ -319,6 +319,7 @@ class Batch {
     int max_pnat_ = 0, max_png_ = 0;
     CellRec* d_cell_ = nullptr;
     double *d_gtab_ = nullptr, *d_gam_ = nullptr, *d_gsf_ = nullptr;
+    double* d_ket_ = nullptr;  // ket-share rows of the periodic pair gradient (nullptr: both sides evaluated)
     int *d_pinst_ = nullptr, *d_patoms_ = nullptr;
     bool periodic(int inst) const { return !h_pbc_.empty() && h_pbc_[inst]; }
 

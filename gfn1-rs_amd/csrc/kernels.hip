@@ -1796,7 +1796,7 @@ template <int LA, int LB, typename R, bool PBC = false>
 __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTables& t, const double* cn,
                                           const double* vsh, int scc_on, int cn_on, int a, int sa, int sb, int ld,
                                           gptr<const double> P, gptr<const double> W, GradAcc<R>& acc,
-                                          const double* tv = nullptr, bool self = false) {
+                                          const double* tv = nullptr, bool self = false, double* oth = nullptr) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
@@ -1977,12 +1977,23 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
         dlp2 = dz * (dp / pl);
     }
     const double hp2 = 2.0 * hij * (double)sp;
+    const double f0 = (double)g0 + dlp0 * hp2, f1 = (double)g1 + dlp1 * hp2, f2 = (double)g2 + dlp2 * hp2;
     if (!PBC || !self) {
-        GradAcc<R>::add(acc.g[0], acc.cg[0], (R)((double)g0 + dlp0 * hp2));
-        GradAcc<R>::add(acc.g[1], acc.cg[1], (R)((double)g1 + dlp1 * hp2));
-        GradAcc<R>::add(acc.g[2], acc.cg[2], (R)((double)g2 + dlp2 * hp2));
+        GradAcc<R>::add(acc.g[0], acc.cg[0], (R)f0);
+        GradAcc<R>::add(acc.g[1], acc.cg[1], (R)f1);
+        GradAcc<R>::add(acc.g[2], acc.cg[2], (R)f2);
     }
     if (cn_on) GradAcc<R>::add(acc.dc, acc.cdc, (R)(dse_a * hs * (double)sp));
+    if constexpr (PBC) {
+        // the ket atom's share of a symmetry-unique pair (pbc/gradient.rs:345-384):
+        // the opposite force (none for an image of A itself) and its dE/dCN
+        if (oth) {
+            oth[0] = self ? 0.0 : -f0;
+            oth[1] = self ? 0.0 : -f1;
+            oth[2] = self ? 0.0 : -f2;
+            oth[3] = cn_on ? dse_b * hs * (double)sp : 0.0;
+        }
+    }
 }
 
 // Shell pair (sa, sb) with sb's atom at image (n0, n1, n2), as the bra side of
@@ -1992,10 +2003,30 @@ template <int LA, int LB, typename R>
 __device__ __forceinline__ void grad_pair_image(const DeviceBatch& b, const DeviceTables& t, const double* cn,
                                                 const double* vsh, int scc_on, int cn_on, int a, int sa, int sb,
                                                 int ld, gptr<const double> P, gptr<const double> W,
-                                                const CellRec& cr, int n0, int n1, int n2, GradAcc<R>& acc) {
+                                                const CellRec& cr, int n0, int n1, int n2, GradAcc<R>& acc,
+                                                double* oth) {
     double tv[3];
     image_vector(cr, n0, n1, n2, tv);
-    grad_pair<LA, LB, R, true>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc, tv, b.sh_atom[sb] == a);
+    grad_pair<LA, LB, R, true>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, acc, tv, b.sh_atom[sb] == a, oth);
+}
+
+// Adds the lanes' ket-side shares v (3 force components and dE/dCN) to the LDS
+// accumulator accb[4 key + .] of their ket atom key (-1: no share).  The keys
+// of a batch are non-decreasing over the lanes, so equal keys are one run: a
+// segmented scan over the wavefront (fixed order) leaves each run's sum on its
+// last lane, which adds it.  All 64 lanes call this.
+__device__ __forceinline__ void ket_shares_add(int key, double v[4], double* accb) {
+    const int lane = threadIdx.x;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int ku = __shfl_up(key, off, 64);
+        double vu[4];
+        for (int k = 0; k < 4; ++k) vu[k] = __shfl_up(v[k], off, 64);
+        if (lane >= off && ku == key)
+            for (int k = 0; k < 4; ++k) v[k] += vu[k];
+    }
+    const int kn = __shfl_down(key, 1, 64);
+    if (key >= 0 && (lane == 63 || kn != key))
+        for (int k = 0; k < 4; ++k) accb[4 * key + k] += v[k];
 }
 
 }  // namespace
@@ -2089,12 +2120,20 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
 // every pair with B != A inside the integral cutoff, A taken as the bra.
 // Deterministic per-atom sums (one 64-lane wavefront per atom, no atomics).
 // PBC: the instantiation for periodic instances (the others return at once);
-// every image of every atom within 30 bohr, the shell potential always on
-// (pbc/gradient.rs:250-384).
+// images within 30 bohr, the shell potential always on (pbc/gradient.rs:250-384).
+// With ket_rows (the symmetry-unique pairs of the reference: the origin cell
+// with B > A, every image with positive canonical index with every B,
+// pbc/gradient.rs:305-313) each pair is evaluated once, by its bra atom A: A's
+// own share goes to g_pair / dedcn_h, the ket atom's share (opposite force,
+// its dE/dCN) to row A of the instance's ket-share matrix at ket_rows +
+// cell.ket0 (4 doubles per (A, B), summed over A by k_ket_shares_merge), through
+// an LDS accumulator of 4 nat doubles (dynamic shared memory).  Without
+// ket_rows every image of every atom is evaluated from both sides.
 template <bool DCLASS, bool MIXED, bool PBC>
 __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms,
                                                      const double* cn, const double* vsh, int scc_on, int cn_on,
-                                                     double cutoff, double* g_pair, double* dedcn_h) {
+                                                     double cutoff, double* g_pair, double* dedcn_h,
+                                                     double* ket_rows) {
     const int j = blockIdx.x;
     const int a = atoms[j];
     const int inst = b.atom_inst[a];
@@ -2102,9 +2141,16 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     const int ld = b.ld[inst];
     int rg[3] = {0, 0, 0};
     __shared__ int pq[PBC ? 128 : 1];  // queue of candidates inside the cutoff (periodic)
+    extern __shared__ double accb[];   // ket-share accumulator (periodic, 4 per atom)
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const bool halve = PBC && ket_rows != nullptr;
     if constexpr (PBC) {
         image_ranges(b.cell[inst], kPbcAoCutoff, rg);
         scc_on = 1;
+        if (halve) {
+            for (int i = threadIdx.x; i < 4 * nat; i += 64) accb[i] = 0.0;
+            __syncthreads();
+        }
     }
     gptr<const double> P = (gptr<const double>)b.P[inst];
     gptr<const double> W = (gptr<const double>)b.W[inst];
@@ -2140,23 +2186,28 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
                     n1 = (im / d2) % d1 - rg[1];
                     n2 = im % d2 - rg[2];
                 };
+                const int center = (nimg - 1) / 2;  // the origin image
+                int key = -1;                          // ket atom of the lane's evaluation
+                double ov[4] = {0.0, 0.0, 0.0, 0.0};   // its ket-side share
+                double* oth = halve ? ov : nullptr;
                 auto evaluate = [&](int c) {
                     int sb, n0, n1, n2;
                     decode(c, sb, n0, n1, n2);
+                    key = b.sh_atom[sb] - a0;
                     if constexpr (!DCLASS) {
                         switch (la * 3 + lb) {
-                            case 0: grad_pair_image<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            case 1: grad_pair_image<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            case 3: grad_pair_image<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            default: grad_pair_image<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 0: grad_pair_image<0, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            case 1: grad_pair_image<0, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            case 3: grad_pair_image<1, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            default: grad_pair_image<1, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
                         }
                     } else {
                         switch (la * 3 + lb) {
-                            case 2: grad_pair_image<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            case 5: grad_pair_image<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            case 6: grad_pair_image<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            case 7: grad_pair_image<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
-                            default: grad_pair_image<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc); break;
+                            case 2: grad_pair_image<0, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            case 5: grad_pair_image<1, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            case 6: grad_pair_image<2, 0, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            case 7: grad_pair_image<2, 1, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
+                            default: grad_pair_image<2, 2, R>(b, t, cn, vsh, scc_on, cn_on, a, sa, sb, ld, P, W, cr, n0, n1, n2, acc, oth); break;
                         }
                     }
                 };
@@ -2168,7 +2219,10 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
                         int sb, n0, n1, n2;
                         decode(c, sb, n0, n1, n2);
                         const int ob = b.sh_atom[sb];
-                        if (!(ob == a && n0 == 0 && n1 == 0 && n2 == 0)) {
+                        const int im = c % nimg;
+                        // symmetry-unique pairs only when the ket shares are kept
+                        const bool unique = !halve || im > center || (im == center && ob > a);
+                        if (unique && !(ob == a && n0 == 0 && n1 == 0 && n2 == 0)) {
                             double tv[3];
                             image_vector(cr, n0, n1, n2, tv);
                             const double dx = x[3 * a] - x[3 * ob] - tv[0], dy = x[3 * a + 1] - x[3 * ob + 1] - tv[1],
@@ -2183,13 +2237,16 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
                     __syncthreads();
                     if (cursor >= 64) {
                         evaluate(pq[lane]);
+                        if (halve) ket_shares_add(key, ov, accb);
                         __syncthreads();
                         if (lane < cursor - 64) pq[lane] = pq[64 + lane];
                         cursor -= 64;
                         __syncthreads();
                     }
                 }
+                key = -1;
                 if (lane < cursor) evaluate(pq[lane]);
+                if (halve) ket_shares_add(key, ov, accb);
                 __syncthreads();
                 continue;
             }
@@ -2226,6 +2283,28 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
         for (int d = 0; d < 3; ++d) g_pair[3 * j + d] = DCLASS ? g_pair[3 * j + d] + g[d] : g[d];
         dedcn_h[j] = DCLASS ? dedcn_h[j] + dc : dc;
     }
+    if (halve) {
+        double* row = ket_rows + b.cell[inst].ket0 + (size_t)(a - a0) * 4 * nat;
+        for (int i = threadIdx.x; i < 4 * nat; i += 64) row[i] = DCLASS ? row[i] + accb[i] : accb[i];
+    }
+}
+
+// Ket-side shares of the symmetry-unique periodic pairs (k_grad_pairs_t with
+// ket rows) summed over the bra atoms in their order and added to the per-atom
+// electronic gradient and dE/dCN; a thread per listed periodic atom.
+__global__ void k_ket_shares_merge(DeviceBatch b, const int* patoms, int npatoms, const double* ket_rows, double* g,
+                                   double* dedcn) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npatoms) return;
+    const int o = patoms[j];
+    const int inst = b.atom_inst[o];
+    const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
+    const double* col = ket_rows + b.cell[inst].ket0 + (size_t)(o - a0) * 4;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r = 0; r < nat; ++r)
+        for (int k = 0; k < 4; ++k) s[k] += col[(size_t)r * 4 * nat + k];
+    for (int d = 0; d < 3; ++d) g[3 * o + d] += s[d];
+    dedcn[o] += s[3];
 }
 
 __global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const double* dcl, double* g, double* dc) {
@@ -2263,20 +2342,20 @@ template __global__ void k_overlap_h0_t<true, false>(DeviceBatch, DeviceTables, 
 template __global__ void k_overlap_h0_t<false, true>(DeviceBatch, DeviceTables, const int*, const double*, double);
 template __global__ void k_overlap_h0_t<true, true>(DeviceBatch, DeviceTables, const int*, const double*, double);
 template __global__ void k_grad_pairs_t<false, false, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<false, false, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<false, true, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<false, true, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<true, false, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<true, false, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<true, true, false>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 template __global__ void k_grad_pairs_t<true, true, true>(DeviceBatch, DeviceTables, const int*, const double*,
-                                                         const double*, int, int, double, double*, double*);
+                                                         const double*, int, int, double, double*, double*, double*);
 
 }  // namespace gfn1
