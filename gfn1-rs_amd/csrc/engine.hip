@@ -366,7 +366,11 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         HIP_TRY(copy_bytes(d_gtab_, gtab.data(), gtab.size() * sizeof(double), hipMemcpyHostToDevice));
         if (!(st = dalloc(&d_gam_, std::max<int64_t>(gam_size, 1))).ok()) return st;
         if (!(st = dalloc(&d_gsf_, gtab.size() / kGStride * kGsfStride)).ok()) return st;
-        if (max_pnat_ <= kKetShareMaxAtoms && !(st = dalloc(&d_ket_, ket_size)).ok()) return st;
+        // GFN1_KET_SHARE_MAX_ATOMS lowers the limit (0: every pair from both sides)
+        int ket_max = kKetShareMaxAtoms;
+        if (const char* km = std::getenv("GFN1_KET_SHARE_MAX_ATOMS"))
+            ket_max = std::min(kKetShareMaxAtoms, std::max(0, std::atoi(km)));
+        if (max_pnat_ <= ket_max && !(st = dalloc(&d_ket_, ket_size)).ok()) return st;
         if (!(st = up_i(&d_pinst_, h_pinst_)).ok()) return st;
         if (!(st = up_i(&d_patoms_, h_patoms_)).ok()) return st;
     }

@@ -1668,100 +1668,146 @@ __device__ __forceinline__ void s_h0_pair(const DeviceBatch& b, const DeviceTabl
         }
 }
 
-// Images listed per lane at a time in s_h0_pair_pbc.
-constexpr int kImageList = 16;
-
-// S(Gamma) and H0(Gamma) for the (shell sa of A, shell sb of B) pairs of one
-// periodic wavefront, one pair per active lane (pbc/bloch.rs:58-236): the
-// blocks of every image of B within 30 bohr and outside the overlap screen summed, the on-site formula for
-// B = A in the origin cell, the off-site one at the image distance otherwise;
-// writes both triangles.  Every lane first lists the images inside the cutoff
-// for its own pair in its LDS column il[.][lane], kImageList at a time and in
-// the order of the image loop, then the wavefront evaluates the listed blocks
-// in lockstep, so no lane waits on an image that is inside only for another
-// lane's pair.  All 64 lanes call this (inactive ones with sb = sa).
+// One image of the (shell sa of A, shell sb of B) pair of a periodic instance
+// (pbc/bloch.rs:58-236): the overlap block of B at x_B + T and the H0 block,
+// the on-site formula for B = A in the origin cell (same), the off-site one at
+// the image distance otherwise.  bs[i NB + j], bh[i NB + j].
 template <int LA, int LB>
-__device__ __forceinline__ void s_h0_pair_pbc(const DeviceBatch& b, const DeviceTables& t, const double* cn,
-                                              const CellRec& cr, const int* rg, bool active, int a, int sa, int sb,
-                                              int ld, gptr<double> S, gptr<double> H, int (*il)[64]) {
-    const int lane = threadIdx.x;
+__device__ __forceinline__ void s_h0_image(const DeviceBatch& b, const DeviceTables& t, const double* cn,
+                                           const CellRec& cr, int a, int sa, int sb, int n0, int n1, int n2,
+                                           double* bs, double* bh) {
     const int ob = b.sh_atom[sb];
     const ShellType& A = t.shells[b.sh_type[sa]];
     const ShellType& B = t.shells[b.sh_type[sb]];
     const double* x = b.atom_x;
     const double ra[3] = {x[3 * a], x[3 * a + 1], x[3 * a + 2]};
-    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
-    double bs[NA * NB], bh[NA * NB];
-#pragma unroll
-    for (int k = 0; k < NA * NB; ++k) bs[k] = bh[k] = 0.0;
+    double tv[3];
+    image_vector(cr, n0, n1, n2, tv);
+    const double rb[3] = {x[3 * ob] + tv[0], x[3 * ob + 1] + tv[1], x[3 * ob + 2] + tv[2]};
+    const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
+    shell_block<LA, LB, false>(A, B, ra, rb, bs, nullptr);
     const int za = b.atom_z[a], zb = b.atom_z[ob];
-    const double sea = A.hdiag - A.kcn * cn[a];
-    const double seb = B.hdiag - B.kcn * cn[ob];
-    const double hsc = hscale(t, za, A, zb, B);
-    const double radsum = t.elements[za].atomic_radius + t.elements[zb].atomic_radius;
-    const double c2 = kPbcAoCutoff * kPbcAoCutoff;
+    double h = 0.5 * ((A.hdiag - A.kcn * cn[a]) + (B.hdiag - B.kcn * cn[ob]));
+    if (!same) {
+        const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rr = sqrt(sqrt(r2) / (t.elements[za].atomic_radius + t.elements[zb].atomic_radius));
+        h *= hscale(t, za, A, zb, B) * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
+    }
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) bh[k] = bs[k] * h;
+}
+
+// Sums K values over the runs of equal keys of a wavefront (keys non-decreasing
+// over the lanes, -1 for none): a segmented scan in a fixed order, the run's sum
+// ending on its last lane, for which the return value is true.
+template <int K>
+__device__ __forceinline__ bool run_sums(int key, double* v) {
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int ku = __shfl_up(key, off, 64);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double vu = __shfl_up(v[k], off, 64);
+            if (lane >= off && ku == key) v[k] += vu;
+        }
+    }
+    const int kn = __shfl_down(key, 1, 64);
+    return key >= 0 && (lane == 63 || kn != key);
+}
+
+// S(Gamma) and H0(Gamma) of shell sa of atom a against the shells of class LB
+// with index <= sa (sh_by_l[s_first .. s_first + nvalid)), over the images of
+// the 30-bohr cutoff outside the overlap screen (pbc/bloch.rs:58-236), both
+// triangles.  The (shell, image) candidates are tested 64 at a time, the inside
+// ones compacted into the LDS queue pq in candidate order and evaluated one per
+// lane; the lanes' blocks are summed per shell by run_sums and added to the
+// blocks, which are zeroed first.
+template <int LA, int LB>
+__device__ void s_h0_shell_pbc(const DeviceBatch& b, const DeviceTables& t, const double* cn, const CellRec& cr,
+                               const int* rg, int a, int sa, int s_first, int nvalid, int ld, gptr<double> S,
+                               gptr<double> H, int* pq) {
+    constexpr int NA = 2 * LA + 1, NB = 2 * LB + 1;
+    const int lane = threadIdx.x;
+    const int mu0 = b.sh_ao[sa];
+    // zero the blocks (sa, sb) of both triangles
+    for (int q = lane; q < nvalid * NA * NB; q += 64) {
+        const int sb = b.sh_by_l[s_first + q / (NA * NB)];
+        const int i = (q % (NA * NB)) / NB, j = q % NB;
+        const int nu0 = b.sh_ao[sb];
+        S[(size_t)(mu0 + i) * ld + (nu0 + j)] = 0.0;
+        H[(size_t)(mu0 + i) * ld + (nu0 + j)] = 0.0;
+        S[(size_t)(nu0 + j) * ld + (mu0 + i)] = 0.0;
+        H[(size_t)(nu0 + j) * ld + (mu0 + i)] = 0.0;
+    }
+    __syncthreads();
     const int d1 = 2 * rg[1] + 1, d2 = 2 * rg[2] + 1, nimg = (2 * rg[0] + 1) * d1 * d2;
-    int pos = active ? 0 : nimg;
-    int n0 = -rg[0], n1 = -rg[1], n2 = -rg[2];
-    for (;;) {
-        int cnt = 0;
-        while (pos < nimg && cnt < kImageList) {
-            const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
+    const int ncand = nvalid * nimg;
+    const double* x = b.atom_x;
+    const int za = b.atom_z[a];
+    const double c2 = kPbcAoCutoff * kPbcAoCutoff;
+    auto evaluate_and_add = [&](bool have, int c) {
+        double v[2 * NA * NB];
+        int key = -1, sb = 0;
+        if (have) {
+            sb = b.sh_by_l[s_first + c / nimg];
+            const int im = c % nimg;
+            s_h0_image<LA, LB>(b, t, cn, cr, a, sa, sb, im / (d1 * d2) - rg[0], (im / d2) % d1 - rg[1],
+                               im % d2 - rg[2], v, v + NA * NB);
+            key = c / nimg;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 2 * NA * NB; ++k) v[k] = 0.0;
+        }
+        if (run_sums<2 * NA * NB>(key, v)) {
+            const int nu0 = b.sh_ao[sb];
+#pragma unroll
+            for (int i = 0; i < NA; ++i)
+#pragma unroll
+                for (int j = 0; j < NB; ++j) {
+                    const size_t e1 = (size_t)(mu0 + i) * ld + (nu0 + j);
+                    S[e1] += v[i * NB + j];
+                    H[e1] += v[NA * NB + i * NB + j];
+                    if (sb != sa) {  // the diagonal block's mirror is itself
+                        const size_t e2 = (size_t)(nu0 + j) * ld + (mu0 + i);
+                        S[e2] += v[i * NB + j];
+                        H[e2] += v[NA * NB + i * NB + j];
+                    }
+                }
+        }
+    };
+    int cursor = 0;  // queued candidates, uniform over the wavefront
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int c = c0 + lane;
+        bool inside = false;
+        if (c < ncand) {
+            const int sb = b.sh_by_l[s_first + c / nimg];
+            const int ob = b.sh_atom[sb];
+            const int im = c % nimg;
+            const int n0 = im / (d1 * d2) - rg[0], n1 = (im / d2) % d1 - rg[1], n2 = im % d2 - rg[2];
             double tv[3];
             image_vector(cr, n0, n1, n2, tv);
-            const double dx = ra[0] - (x[3 * ob] + tv[0]), dy = ra[1] - (x[3 * ob + 1] + tv[1]),
-                         dz = ra[2] - (x[3 * ob + 2] + tv[2]);
+            const double dx = x[3 * a] - (x[3 * ob] + tv[0]), dy = x[3 * a + 1] - (x[3 * ob + 1] + tv[1]),
+                         dz = x[3 * a + 2] - (x[3 * ob + 2] + tv[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
-            if (same || (r2 > kPairEps2 && r2 <= c2 && !overlap_screened(t, za, zb, r2))) il[cnt++][lane] = pos;
-            ++pos;
-            if (++n2 > rg[2]) {
-                n2 = -rg[2];
-                if (++n1 > rg[1]) {
-                    n1 = -rg[1];
-                    ++n0;
-                }
-            }
+            const bool same = ob == a && n0 == 0 && n1 == 0 && n2 == 0;
+            inside = same || (r2 > kPairEps2 && r2 <= c2 && !overlap_screened(t, za, b.atom_z[ob], r2));
         }
-        int most = cnt;
-        for (int off = 32; off > 0; off >>= 1) most = max(most, __shfl_xor(most, off, 64));
-        for (int k = 0; k < most; ++k) {
-            if (k >= cnt) continue;
-            const int ps = il[k][lane];
-            const int m0 = ps / (d1 * d2) - rg[0], m1 = (ps / d2) % d1 - rg[1], m2 = ps % d2 - rg[2];
-            const bool same = ob == a && m0 == 0 && m1 == 0 && m2 == 0;
-            double tv[3];
-            image_vector(cr, m0, m1, m2, tv);
-            const double rb[3] = {x[3 * ob] + tv[0], x[3 * ob + 1] + tv[1], x[3 * ob + 2] + tv[2]};
-            const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            double blk[NA * NB];
-            shell_block<LA, LB, false>(A, B, ra, rb, blk, nullptr);
-            double h = 0.5 * (sea + seb);
-            if (!same) {
-                const double rr = sqrt(sqrt(r2) / radsum);
-                h *= hsc * ((1.0 + A.poly * rr) * (1.0 + B.poly * rr));
-            }
-#pragma unroll
-            for (int k2 = 0; k2 < NA * NB; ++k2) {
-                bs[k2] += blk[k2];
-                bh[k2] += blk[k2] * h;
-            }
+        const unsigned long long m = __ballot(inside);
+        if (inside) pq[cursor + __popcll(m & ((1ull << lane) - 1ull))] = c;
+        cursor += __popcll(m);
+        __syncthreads();
+        if (cursor >= 64) {
+            evaluate_and_add(true, pq[lane]);
+            __syncthreads();
+            if (lane < cursor - 64) pq[lane] = pq[64 + lane];
+            cursor -= 64;
+            __syncthreads();
         }
-        if (__ballot(pos < nimg) == 0) break;
     }
-    if (!active) return;
-    const int mu0 = b.sh_ao[sa], nu0 = b.sh_ao[sb];
-#pragma unroll
-    for (int i = 0; i < NA; ++i)
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const size_t e1 = (size_t)(mu0 + i) * ld + (nu0 + j);
-            const size_t e2 = (size_t)(nu0 + j) * ld + (mu0 + i);
-            S[e1] = bs[i * NB + j];
-            S[e2] = bs[i * NB + j];
-            H[e1] = bh[i * NB + j];
-            H[e2] = bh[i * NB + j];
-        }
+    evaluate_and_add(lane < cursor, lane < cursor ? pq[lane] : 0);
+    __syncthreads();
 }
 
 // Electronic gradient contribution of one (shell of A, shell of B) pair to atom A
@@ -2045,7 +2091,7 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
     const int ld = b.ld[inst];
     int rg[3] = {0, 0, 0};
     if constexpr (PBC) image_ranges(b.cell[inst], kPbcAoCutoff, rg);
-    __shared__ int il[PBC ? kImageList : 1][64];  // per-lane image lists (periodic)
+    __shared__ int pq[PBC ? 128 : 1];  // queue of candidates inside the cutoff (periodic)
     gptr<double> S = (gptr<double>)b.S[inst];
     gptr<double> H = (gptr<double>)b.H0[inst];
     const double c2 = cutoff * cutoff;
@@ -2059,34 +2105,29 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
             // s/p classes and d classes go to separate instantiations (register budget)
             if ((la == 2 || lb == 2) != DCLASS) continue;
             if constexpr (PBC) {
-                // rounds of 64 shells of the class, every lane in every call
-                const CellRec& cr = b.cell[inst];
+                // the class's shells with index <= sa (a prefix: shells are numbered
+                // atom by atom and each class list is in shell order)
+                int nvalid = 0;
                 for (int base = lo[lb]; base < lo[lb + 1]; base += 64) {
                     const int idx = base + threadIdx.x;
-                    int sb = sa;
-                    bool act = false;
-                    if (idx < lo[lb + 1]) {
-                        sb = b.sh_by_l[idx];
-                        const int ob = b.sh_atom[sb];
-                        act = !(ob > a || (ob == a && sb > sa));
-                        if (!act) sb = sa;
+                    nvalid += __popcll(__ballot(idx < lo[lb + 1] && b.sh_by_l[idx] <= sa));
+                }
+                if (nvalid == 0) continue;
+                const CellRec& cr = b.cell[inst];
+                if constexpr (!DCLASS) {
+                    switch (la * 3 + lb) {
+                        case 0: s_h0_shell_pbc<0, 0>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        case 1: s_h0_shell_pbc<0, 1>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        case 3: s_h0_shell_pbc<1, 0>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        default: s_h0_shell_pbc<1, 1>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
                     }
-                    if (__ballot(act) == 0) continue;
-                    if constexpr (!DCLASS) {
-                        switch (la * 3 + lb) {
-                            case 0: s_h0_pair_pbc<0, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            case 1: s_h0_pair_pbc<0, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            case 3: s_h0_pair_pbc<1, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            default: s_h0_pair_pbc<1, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                        }
-                    } else {
-                        switch (la * 3 + lb) {
-                            case 2: s_h0_pair_pbc<0, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            case 5: s_h0_pair_pbc<1, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            case 6: s_h0_pair_pbc<2, 0>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            case 7: s_h0_pair_pbc<2, 1>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                            default: s_h0_pair_pbc<2, 2>(b, t, cn, cr, rg, act, a, sa, sb, ld, S, H, il); break;
-                        }
+                } else {
+                    switch (la * 3 + lb) {
+                        case 2: s_h0_shell_pbc<0, 2>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        case 5: s_h0_shell_pbc<1, 2>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        case 6: s_h0_shell_pbc<2, 0>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        case 7: s_h0_shell_pbc<2, 1>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
+                        default: s_h0_shell_pbc<2, 2>(b, t, cn, cr, rg, a, sa, lo[lb], nvalid, ld, S, H, pq); break;
                     }
                 }
                 continue;
