@@ -146,6 +146,9 @@ Batch::~Batch() {
     large_work_.release();
     for (void* p : allocs_) (void)hipFree(p);
     if (ev_cn_) (void)hipEventDestroy(ev_cn_);
+    if (ev_ew_q_) (void)hipEventDestroy(ev_ew_q_);
+    if (ev_ew_) (void)hipEventDestroy(ev_ew_);
+    if (ew_stream_) (void)hipStreamDestroy(ew_stream_);
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -239,7 +242,6 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 ket_size += 4 * (int64_t)s.nat * s.nat;
                 h_pinst_.push_back(i);
                 max_pnat_ = std::max(max_pnat_, s.nat);
-                max_png_ = std::max(max_png_, (int)cells[i].ng);
                 for (int a = 0; a < s.nat; ++a) h_patoms_.push_back(h_atom0_[i] + a);
             }
         }
@@ -371,6 +373,10 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
         if (const char* km = std::getenv("GFN1_KET_SHARE_MAX_ATOMS"))
             ket_max = std::min(kKetShareMaxAtoms, std::max(0, std::atoi(km)));
         if (max_pnat_ <= ket_max && !(st = dalloc(&d_ket_, ket_size)).ok()) return st;
+        if (!(st = dalloc(&d_gew_, 3 * (size_t)natoms_)).ok()) return st;
+        HIP_TRY(hipStreamCreate(&ew_stream_));
+        HIP_TRY(hipEventCreateWithFlags(&ev_ew_q_, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ev_ew_, hipEventDisableTiming));
         if (!(st = up_i(&d_pinst_, h_pinst_)).ok()) return st;
         if (!(st = up_i(&d_patoms_, h_patoms_)).ok()) return st;
     }
@@ -435,10 +441,22 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
             HIP_TRY(hipHostMalloc((void**)&bk.g_st[slot], sizeof(SccState) * count, hipHostMallocCoherent));
             HIP_TRY(hipEventCreateWithFlags(&bk.g_ev[slot], hipEventDisableTiming));
         }
-        std::vector<int> atoms;
+        std::vector<int> atoms, pinst, patoms;
         for (int i : bk.inst) {
             for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) atoms.push_back(a);
-            if (h_pbc_[i]) bk.has_pbc = true;
+            if (h_pbc_[i]) {
+                bk.has_pbc = true;
+                pinst.push_back(i);
+                bk.max_ng = std::max(bk.max_ng, (int)cells[i].ng);
+                for (int a = h_atom0_[i]; a < h_atom0_[i + 1]; ++a) patoms.push_back(a);
+            }
+        }
+        if (!pinst.empty()) {
+            Status st;
+            if (!(st = up_i(&bk.pinst, pinst)).ok()) return st;
+            if (!(st = up_i(&bk.patoms, patoms)).ok()) return st;
+            bk.npinst = (int)pinst.size();
+            bk.npatoms = (int)patoms.size();
         }
         bk.natoms = (int)atoms.size();
         const size_t na_alloc = std::max<size_t>(atoms.size(), 1);
@@ -1621,6 +1639,14 @@ Status Batch::finish_iteration(Bucket& bk) {
 // stream.  Runs while other buckets are still iterating.
 Status Batch::launch_bucket_gradient(Bucket& bk) {
     const DeviceTables t = dm_->tables();
+    if (opt_.compute_gradient && bk.npinst > 0) {
+        // the Ewald gradient needs only the converged charges: it runs on its
+        // own stream beside the density and pair-gradient kernels below
+        HIP_TRY(hipEventRecord(ev_ew_q_, bk.stream));
+        HIP_TRY(hipStreamWaitEvent(ew_stream_, ev_ew_q_, 0));
+        HIP_TRY(launch_ewald_grad(ew_stream_, db_, t, bk.pinst, bk.npinst, bk.max_ng, bk.patoms, bk.npatoms,
+                                  (const double*)d_qout_, d_gew_));
+    }
     std::vector<int> done;
     for (int i : bk.inst)
         if (h_state_[i].status == 1) done.push_back(i);
@@ -1708,18 +1734,19 @@ Status Batch::gradient_phase() {
                                d_gel_, d_dedcn_h_);
             LAUNCH_CHECK();
         }
-        if (d_ket_) {
+        if (!h_patoms_.empty()) {
+            // the ket shares of the periodic pair gradient and the Ewald gradient
+            HIP_TRY(hipEventRecord(ev_ew_, ew_stream_));
+            HIP_TRY(hipStreamWaitEvent(stream_, ev_ew_, 0));
             const int np = (int)h_patoms_.size();
-            hipLaunchKernelGGL(k_ket_shares_merge, dim3((np + 127) / 128), dim3(128), 0, stream_, db_,
-                               (const int*)d_patoms_, np, (const double*)d_ket_, d_gel_, d_dedcn_h_);
+            hipLaunchKernelGGL(k_periodic_merge, dim3((np + 127) / 128), dim3(128), 0, stream_, db_,
+                               (const int*)d_patoms_, np, (const double*)d_ket_, (const double*)d_gew_, d_gel_,
+                               d_dedcn_h_);
             LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_grad_atoms, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                            (const double*)d_qout_, cn_on, opt_.include_scc, d_dedcn_h_, d_gel_, na);
         LAUNCH_CHECK();
-        // the Ewald Coulomb gradient of the periodic instances
-        HIP_TRY(launch_ewald_grad(stream_, db_, t, d_pinst_, (int)h_pinst_.size(), max_png_, d_patoms_,
-                                  (int)h_patoms_.size(), (const double*)d_qout_, d_gel_));
         if (cn_on) {
             hipLaunchKernelGGL(k_cn_chain, dim3((na + 127) / 128), dim3(128), 0, stream_, db_, t,
                                (const double*)d_dedcn_h_, opt_.coordination_cutoff, 1e-12, d_gel_, na);

@@ -236,6 +236,11 @@ class Batch {
         bool has_pbc = false;             // a periodic instance is in the bucket
         double* gpair = nullptr;  // 3 * natoms
         double* dedcn = nullptr;  // natoms
+        // the bucket's periodic instances and their atoms (Ewald gradient)
+        int* pinst = nullptr;
+        int npinst = 0, max_ng = 0;
+        int* patoms = nullptr;
+        int npatoms = 0;
         // Captured iteration (own eigensolver buckets): one graph per state
         // slot, the two differing only in the pinned buffer the states are
         // gathered into.  The pointer tables list every instance of the bucket
@@ -316,10 +321,16 @@ class Batch {
     // periodic instances: cell records, reciprocal vectors, Ewald shell matrices
     std::vector<unsigned char> h_pbc_;
     std::vector<int> h_pinst_, h_patoms_;
-    int max_pnat_ = 0, max_png_ = 0;
+    int max_pnat_ = 0;
     CellRec* d_cell_ = nullptr;
     double *d_gtab_ = nullptr, *d_gam_ = nullptr, *d_gsf_ = nullptr;
     double* d_ket_ = nullptr;  // ket-share rows of the periodic pair gradient (nullptr: both sides evaluated)
+    double* d_gew_ = nullptr;  // Ewald gradient of the periodic atoms (3 per atom)
+    // The Ewald gradient of a bucket's periodic instances runs on ew_stream_,
+    // beside the bucket's pair gradient, once its SCC is done (ev_ew_q_); the
+    // gradient phase waits for ev_ew_.
+    hipStream_t ew_stream_ = nullptr;
+    hipEvent_t ev_ew_q_ = nullptr, ev_ew_ = nullptr;
     int *d_pinst_ = nullptr, *d_patoms_ = nullptr;
     bool periodic(int inst) const { return !h_pbc_.empty() && h_pbc_[inst]; }
 

@@ -20,6 +20,7 @@ namespace gfn1 {
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrtPiK = 1.7724538509055160273;  // sqrt(pi)
 constexpr double kPrimScreen = 1.0e-13;   // integrals.rs:17
 constexpr double kPairEps2 = 1.0e-24;     // pairlist.rs:16 (DIST_EPS^2)
 
@@ -96,85 +97,6 @@ __device__ __forceinline__ int ao_terms(int l, int ao, CartTerm* t) {
 __device__ __forceinline__ double ao_abs_sum(int l, int ao) {
     if (l < 2) return 1.0;
     return ao == 1 ? 2.0 : 1.73205080756887729353;
-}
-
-// Overlap block <A mu | B nu> of two shells and, with DERIV, its derivatives
-// with respect to the bra centre (dA) and the ket centre (dB).
-// integrals.rs:290-360 (values) and :481-537, :782-868 (derivatives), with the
-// primitive screening bound of :30-45.  Block index: mu * 5 + nu.
-template <bool DERIV>
-__device__ void shell_pair(const ShellType& A, const ShellType& B, const double* ra, const double* rb, double* S,
-                           double* dA, double* dB) {
-    const int la = A.l, lb = B.l, na = 2 * la + 1, nb = 2 * lb + 1;
-    for (int k = 0; k < 25; ++k) S[k] = 0.0;
-    if (DERIV)
-        for (int k = 0; k < 75; ++k) dA[k] = dB[k] = 0.0;
-    const double dx = ra[0] - rb[0], dy = ra[1] - rb[1], dz = ra[2] - rb[2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double r = sqrt(r2);
-    const int order = la + lb + (DERIV ? 3 : 2);
-    for (int ip = 0; ip < A.nprim; ++ip)
-        for (int jp = 0; jp < B.nprim; ++jp) {
-            const double pref = A.coeff[ip] * B.coeff[jp];
-            const double al = A.alpha[ip], be = B.alpha[jp], p = al + be;
-            const double kab = exp(-al * be / p * r2);
-            const double spread = sqrt(1.0 / p);
-            const double gpref = (kPi / p) * sqrt(kPi / p);
-            const double poly = ipow(1.0 + r + spread, order);
-            // the largest |component| product of this shell pair bounds every AO pair
-            const double csmax = ao_abs_sum(la, 1) * ao_abs_sum(lb, 1);
-            if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
-            double g[5];
-            g[0] = sqrt(kPi / p);
-            for (int m = 1; m < 5; ++m) g[m] = g[m - 1] * (m - 0.5) / p;
-            // 1-D tables m[dim][i][j], i <= la+1, j <= lb+1
-            double mt[3][4][4];
-            const int imax = la + (DERIV ? 1 : 0), jmax = lb + (DERIV ? 1 : 0);
-            for (int d = 0; d < 3; ++d) {
-                const double P = (al * ra[d] + be * rb[d]) / p;
-                const double pa = P - ra[d], pb = P - rb[d];
-                for (int i = 0; i <= imax; ++i)
-                    for (int j = 0; j <= jmax; ++j) mt[d][i][j] = moment1d(i, j, pa, pb, g);
-            }
-            for (int mu = 0; mu < na; ++mu) {
-                CartTerm ta[3];
-                const int nta = ao_terms(la, mu, ta);
-                const double csa = ao_abs_sum(la, mu);
-                for (int nu = 0; nu < nb; ++nu) {
-                    const double cs = csa * ao_abs_sum(lb, nu);
-                    if (fabs(pref) * cs * kab * gpref * poly < kPrimScreen) continue;
-                    CartTerm tb[3];
-                    const int ntb = ao_terms(lb, nu, tb);
-                    double s = 0.0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
-                    for (int a = 0; a < nta; ++a)
-                        for (int b = 0; b < ntb; ++b) {
-                            const double cf = pref * ta[a].c * tb[b].c * kab;
-                            const int ia[3] = {ta[a].px, ta[a].py, ta[a].pz};
-                            const int jb[3] = {tb[b].px, tb[b].py, tb[b].pz};
-                            const double m0 = mt[0][ia[0]][jb[0]], m1 = mt[1][ia[1]][jb[1]], m2 = mt[2][ia[2]][jb[2]];
-                            s += cf * (m0 * m1 * m2);
-                            if (DERIV) {
-                                double mm[3] = {m0, m1, m2};
-                                for (int d = 0; d < 3; ++d) {
-                                    const double o1 = mm[(d + 1) % 3] * mm[(d + 2) % 3];
-                                    const double da_ = 2.0 * al * mt[d][ia[d] + 1][jb[d]] -
-                                                       (ia[d] > 0 ? ia[d] * mt[d][ia[d] - 1][jb[d]] : 0.0);
-                                    const double db_ = 2.0 * be * mt[d][ia[d]][jb[d] + 1] -
-                                                       (jb[d] > 0 ? jb[d] * mt[d][ia[d]][jb[d] - 1] : 0.0);
-                                    ga[d] += cf * (da_ * o1);
-                                    gb[d] += cf * (db_ * o1);
-                                }
-                            }
-                        }
-                    S[mu * 5 + nu] += s;
-                    if (DERIV)
-                        for (int d = 0; d < 3; ++d) {
-                            dA[(mu * 5 + nu) * 3 + d] += ga[d];
-                            dB[(mu * 5 + nu) * 3 + d] += gb[d];
-                        }
-                }
-            }
-        }
 }
 
 __device__ __forceinline__ double kshell(const Globals& g, int l) { return l == 0 ? g.ks : l == 1 ? g.kp : g.kd; }
@@ -1561,20 +1483,22 @@ __device__ __forceinline__ void shell_block(const ShellType& A, const ShellType&
         for (int jp = 0; jp < B.nprim; ++jp) {
             const double be = B.alpha[jp];
             const double pref = cai * B.coeff[jp];
-            const double p = al + be;
-            const double kab = exp(-al * be / p * r2);
-            const double spread = sqrt(1.0 / p);
-            const double gpref = (kPi / p) * sqrt(kPi / p);
+            // one division and one square root per primitive pair: 1/p and sqrt(1/p)
+            const double p = al + be, ip = 1.0 / p;
+            const double kab = exp(-al * be * ip * r2);
+            const double spread = sqrt(ip);
+            const double g0 = kSqrtPiK * spread;  // sqrt(pi / p)
+            const double gpref = kPi * ip * g0;   // (pi / p)^(3/2)
             const double poly = ipow(1.0 + r + spread, order);
             if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
             double g[GM];
-            g[0] = sqrt(kPi / p);
+            g[0] = g0;
 #pragma unroll
-            for (int m = 1; m < GM; ++m) g[m] = g[m - 1] * (m - 0.5) / p;
+            for (int m = 1; m < GM; ++m) g[m] = g[m - 1] * (m - 0.5) * ip;
             double mt[3][IM + 1][LB + 1];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const double P = (al * ra[d] + be * rb[d]) / p;
+                const double P = (al * ra[d] + be * rb[d]) * ip;
                 const double pa = P - ra[d], pb = P - rb[d];
 #pragma unroll
                 for (int i = 0; i <= IM; ++i)
@@ -1956,20 +1880,22 @@ __device__ __forceinline__ void grad_pair(const DeviceBatch& b, const DeviceTabl
             for (int jp = 0; jp < B.nprim; ++jp) {
                 const double be = B.alpha[jp];
                 const double pref = cai * B.coeff[jp];
-                const double p = al + be;
-                const double kab = exp(-al * be / p * r2);
-                const double spread = sqrt(1.0 / p);
-                const double gpref = (kPi / p) * sqrt(kPi / p);
+                // one division and one square root per primitive pair: 1/p and sqrt(1/p)
+                const double p = al + be, ip = 1.0 / p;
+                const double kab = exp(-al * be * ip * r2);
+                const double spread = sqrt(ip);
+                const double sqp = kSqrtPiK * spread;  // sqrt(pi / p)
+                const double gpref = kPi * ip * sqp;   // (pi / p)^(3/2)
                 const double poly = ipow(1.0 + r + spread, order);
                 if (fabs(pref) * csmax * kab * gpref * poly < kPrimScreen) continue;
                 double gm[GM];
-                gm[0] = sqrt(kPi / p);
+                gm[0] = sqp;
     #pragma unroll
-                for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * (m - 0.5) / p;
+                for (int m = 1; m < GM; ++m) gm[m] = gm[m - 1] * (m - 0.5) * ip;
                 double mt[3][IM + 1][LB + 1];
     #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    const double Pc = (al * ra[d] + be * rb[d]) / p;
+                    const double Pc = (al * ra[d] + be * rb[d]) * ip;
                     const double pa = Pc - ra[d], pb = Pc - rb[d];
     #pragma unroll
                     for (int i = 0; i <= IM; ++i)
@@ -2167,7 +2093,7 @@ __global__ void __launch_bounds__(64) k_overlap_h0_t(DeviceBatch b, DeviceTables
 // pbc/gradient.rs:305-313) each pair is evaluated once, by its bra atom A: A's
 // own share goes to g_pair / dedcn_h, the ket atom's share (opposite force,
 // its dE/dCN) to row A of the instance's ket-share matrix at ket_rows +
-// cell.ket0 (4 doubles per (A, B), summed over A by k_ket_shares_merge), through
+// cell.ket0 (4 doubles per (A, B), summed over A by k_periodic_merge), through
 // an LDS accumulator of 4 nat doubles (dynamic shared memory).  Without
 // ket_rows every image of every atom is evaluated from both sides.
 template <bool DCLASS, bool MIXED, bool PBC>
@@ -2330,21 +2256,25 @@ __global__ void __launch_bounds__(64) k_grad_pairs_t(DeviceBatch b, DeviceTables
     }
 }
 
-// Ket-side shares of the symmetry-unique periodic pairs (k_grad_pairs_t with
-// ket rows) summed over the bra atoms in their order and added to the per-atom
-// electronic gradient and dE/dCN; a thread per listed periodic atom.
-__global__ void k_ket_shares_merge(DeviceBatch b, const int* patoms, int npatoms, const double* ket_rows, double* g,
-                                   double* dedcn) {
+// Periodic atoms' gradient parts computed apart from the per-bucket pair sums:
+// the ket-side shares of the symmetry-unique pairs (k_grad_pairs_t with ket
+// rows; none without) summed over the bra atoms in their order, and the Ewald
+// gradient g_ew; added to the per-atom electronic gradient and dE/dCN.  A
+// thread per listed periodic atom.
+__global__ void k_periodic_merge(DeviceBatch b, const int* patoms, int npatoms, const double* ket_rows,
+                                 const double* g_ew, double* g, double* dedcn) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= npatoms) return;
     const int o = patoms[j];
     const int inst = b.atom_inst[o];
     const int a0 = b.atom0[inst], nat = b.atom0[inst + 1] - a0;
-    const double* col = ket_rows + b.cell[inst].ket0 + (size_t)(o - a0) * 4;
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r = 0; r < nat; ++r)
-        for (int k = 0; k < 4; ++k) s[k] += col[(size_t)r * 4 * nat + k];
-    for (int d = 0; d < 3; ++d) g[3 * o + d] += s[d];
+    if (ket_rows) {
+        const double* col = ket_rows + b.cell[inst].ket0 + (size_t)(o - a0) * 4;
+        for (int r = 0; r < nat; ++r)
+            for (int k = 0; k < 4; ++k) s[k] += col[(size_t)r * 4 * nat + k];
+    }
+    for (int d = 0; d < 3; ++d) g[3 * o + d] += s[d] + g_ew[3 * o + d];
     dedcn[o] += s[3];
 }
 

@@ -72,14 +72,14 @@ __global__ void k_instance_sum(DeviceBatch b, const double* per_atom, double* ou
 // dedcn_h[j] belong to atom atoms[j].  Periodic instantiation with ket_rows:
 // each symmetry-unique (atom, atom, image) pair once, the ket atom's share in
 // the instance's ket-share rows (launch with 4 nat doubles of dynamic shared
-// memory, nat the largest periodic atom count), summed by k_ket_shares_merge;
+// memory, nat the largest periodic atom count), summed by k_periodic_merge;
 // ket_rows = nullptr evaluates every pair from both sides.
 template <bool DCLASS, bool MIXED, bool PBC>
 __global__ void k_grad_pairs_t(DeviceBatch b, DeviceTables t, const int* atoms, const double* cn, const double* vsh,
                                int scc_on, int cn_on, double cutoff, double* g_pair, double* dedcn_h,
                                double* ket_rows);
-__global__ void k_ket_shares_merge(DeviceBatch b, const int* patoms, int npatoms, const double* ket_rows, double* g,
-                                   double* dedcn);
+__global__ void k_periodic_merge(DeviceBatch b, const int* patoms, int npatoms, const double* ket_rows,
+                                 const double* g_ew, double* g, double* dedcn);
 // Per-atom scatter of listed sums: g[3 atoms[j] + d] = gl[3 j + d], dc[atoms[j]] = dcl[j].
 __global__ void k_scatter_atoms(const int* atoms, int n, const double* gl, const double* dcl, double* g, double* dc);
 // out[j] = st[act[j]] for j < n (out may be coherent pinned host memory).

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(64) k_ewald_sfac(DeviceBatch b, DeviceTables t
 }
 
 // Gradient of 1/2 q^T Gamma q at the converged shell charges for every atom of
-// a periodic instance (pbc/gradient.rs:548-603, :679-779), added to g_el.  One
+// a periodic instance (pbc/gradient.rs:548-603, :679-779), written to g_ew.  One
 // 64-lane work-group per listed atom a.  Real space: a work item is one (plane
 // n0 of images, atom o) pair, o fastest over the lanes; it walks the images of
 // its plane, the erfc / R force with the atomic charges, the R^-3 and remainder
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(64) k_ewald_sfac(DeviceBatch b, DeviceTables t
 // shell sum over j factorises.  Fixed lane assignment and a fixed reduction
 // order keep the result deterministic.
 __global__ void __launch_bounds__(64) k_ewald_grad(DeviceBatch b, DeviceTables t, const int* patoms,
-                                                   const double* q, double* g_el) {
+                                                   const double* q, double* g_ew) {
     const int a = patoms[blockIdx.x];
     const int lane = threadIdx.x;
     const int inst = b.atom_inst[a];
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(64) k_ewald_grad(DeviceBatch b, DeviceTables t
     for (int k = 0; k < 3; ++k)
         for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
     if (lane == 0)
-        for (int k = 0; k < 3; ++k) g_el[3 * a + k] += acc[k];
+        for (int k = 0; k < 3; ++k) g_ew[3 * a + k] = acc[k];
 }
 
 // ---------------------------------------------------------------- host side
@@ -410,14 +410,14 @@ hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceT
 }
 
 hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
-                             int npinst, int max_ng, const int* patoms, int npatoms, const double* q, double* g_el) {
+                             int npinst, int max_ng, const int* patoms, int npatoms, const double* q, double* g_ew) {
     if (npatoms == 0 || npinst == 0) return hipSuccess;
     if (max_ng > 0) {
         hipLaunchKernelGGL(k_ewald_sfac, dim3((max_ng + 63) / 64, npinst), dim3(64), 0, s, b, t, pinst, q);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_ewald_grad, dim3(npatoms), dim3(64), 0, s, b, t, patoms, q, g_el);
+    hipLaunchKernelGGL(k_ewald_grad, dim3(npatoms), dim3(64), 0, s, b, t, patoms, q, g_ew);
     return hipGetLastError();
 }
 

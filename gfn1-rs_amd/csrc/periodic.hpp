@@ -20,10 +20,10 @@ double exponential_integral_e1(double x);
 // Gamma (nsh x nsh) of the listed periodic instances into b.gam.
 hipError_t launch_ewald_gamma(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
                               int npinst, int max_nat);
-// g_el[3 a + d] += d(1/2 q^T Gamma q)/dR_a for the listed atoms of the listed
+// g_ew[3 a + d] = d(1/2 q^T Gamma q)/dR_a for the listed atoms of the listed
 // periodic instances (max_ng: the largest G count among them); the charge
 // structure factors go to b.gsf first.
 hipError_t launch_ewald_grad(hipStream_t s, const DeviceBatch& b, const DeviceTables& t, const int* pinst,
-                             int npinst, int max_ng, const int* patoms, int npatoms, const double* q, double* g_el);
+                             int npinst, int max_ng, const int* patoms, int npatoms, const double* q, double* g_ew);
 
 }  // namespace gfn1
