@@ -7,7 +7,8 @@ Cases: water boxes (8 and 32 molecules, images through every face), a
 halogen-bonded pair in a skewed cell (the periodic halogen term), hydrogen
 sulfide with water (d shells), periodic and molecular instances in one batch,
 mixed precision, the include flags and the level shift the periodic path
-ignores, and a singular cell.
+ignores, a singular cell, and the pair gradient from both sides (the path of
+cells above the ket-share limit) against the symmetry-unique one.
 """
 import sys
 
@@ -125,6 +126,22 @@ def test_periodic_and_molecular_instances_in_one_batch():
         note("batch %d (%s): |dE| %.2e max|dg| %.2e" % (i, "periodic" if cell is not None else "molecule", de, dg))
         assert eng.status(i) == 0 and de <= 1e-8 and dg <= 1e-7
         assert eng.iterations(i) == ref["iterations"]
+
+
+def test_periodic_pair_gradient_both_sides_matches_unique_pairs(monkeypatch):
+    # cells above the ket-share limit evaluate every (atom, image) pair from both
+    # sides; the limit set to 0 takes that path for a small cell
+    note("case periodic both-sides pair gradient")
+    z, x, cell = sulfide_cell()
+    unique = BatchEngine([(z, x, 0.0, cell)], options=OPTS).run()
+    monkeypatch.setenv("GFN1_KET_SHARE_MAX_ATOMS", "0")
+    both = BatchEngine([(z, x, 0.0, cell)], options=OPTS).run()
+    ref = oracle.run(z, x, cell=cell, **TIGHT)
+    dg = float(np.abs(both.gradient(0) - unique.gradient(0)).max())
+    note("both sides vs unique pairs: max|dg| %.2e" % dg)
+    assert abs(both.energy(0) - unique.energy(0)) <= 1e-12
+    assert dg <= 1e-11
+    assert np.abs(both.gradient(0) - ref["gradient"]).max() <= 1e-7
 
 
 def test_periodic_mixed_precision():
