@@ -1013,6 +1013,26 @@ int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn
     return GFN1_OK;
 }
 
+int gfn1_hessian_terms(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
+                       const gfn1_hessian_options* options, double* hessian, double* max_asymmetry,
+                       double* repulsion, double* dispersion, double* halogen, gfn1_error* err) {
+    if (!system || !hessian) {
+        clear(err);
+        return fail(err, Status::error(kErrInvalidInput, "invalid Hessian arguments"));
+    }
+    std::vector<HessianResult> res;
+    int rc = hessian_common(1, system, params, electronic, options, res, err);
+    if (rc != GFN1_OK) return rc;
+    const HessianResult& r = res[0];
+    if (r.status != kOk) return fail(err, Status::error(r.status, r.message));
+    std::copy(r.hessian.begin(), r.hessian.end(), hessian);
+    if (max_asymmetry) *max_asymmetry = r.max_asymmetry;
+    if (repulsion) std::copy(r.h_repulsion.begin(), r.h_repulsion.end(), repulsion);
+    if (dispersion) std::copy(r.h_dispersion.begin(), r.h_dispersion.end(), dispersion);
+    if (halogen) std::copy(r.h_halogen.begin(), r.h_halogen.end(), halogen);
+    return GFN1_OK;
+}
+
 int gfn1_hessian_batch(int32_t n, const gfn1_system* systems, const gfn1_params* params,
                        const gfn1_electronic_options* electronic, const gfn1_hessian_options* options,
                        double* hessians, double* dipole_derivatives, int32_t* status, gfn1_error* err) {

@@ -694,7 +694,8 @@ Status Batch::set_start_charges(int inst, const double* q) {
 }
 
 Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double>& energy,
-                                    std::vector<double>& gradient, std::vector<double>* shell_charges) {
+                                    std::vector<double>& gradient, std::vector<double>* shell_charges,
+                                    std::vector<double>* term_gradients) {
     const int na = natoms_;
     std::vector<double> gel(3 * na), grep(3 * na), gdisp(3 * na), ghal(3 * na), erep(ninst_), edisp(ninst_),
         ehal(ninst_);
@@ -717,6 +718,7 @@ Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double
     status.assign(ninst_, 0);
     energy.assign(ninst_, 0.0);
     gradient.assign(3 * (size_t)na, 0.0);
+    if (term_gradients) term_gradients->assign(9 * (size_t)na, 0.0);
     const bool el_on = opt_.compute_gradient && (opt_.include_hamiltonian || opt_.include_scc);
     const bool rep_on = opt_.compute_gradient && opt_.include_repulsion;
     const bool disp_on = opt_.compute_gradient && opt_.include_dispersion && opt_.enable_dispersion;
@@ -730,10 +732,17 @@ Status Batch::fetch_energy_gradient(std::vector<int>& status, std::vector<double
         const double e_disp = opt_.enable_dispersion ? edisp[i] : 0.0;
         energy[i] = s.e_el + s.e2 + s.e3 + erep[i] + e_disp + ehal[i] + s.ts;
         const bool all = periodic(i) && opt_.compute_gradient;  // pbc/gradient.rs:75-163: every term
-        for (int k = 3 * h_atom0_[i]; k < 3 * h_atom0_[i + 1]; ++k)
-            gradient[k] = (el_on || all ? gel[k] : 0.0) + (rep_on || all ? grep[k] : 0.0) +
-                          (disp_on || (all && opt_.enable_dispersion) ? gdisp[k] : 0.0) +
-                          (hal_on || all ? ghal[k] : 0.0);
+        for (int k = 3 * h_atom0_[i]; k < 3 * h_atom0_[i + 1]; ++k) {
+            const double r = rep_on || all ? grep[k] : 0.0;
+            const double d = disp_on || (all && opt_.enable_dispersion) ? gdisp[k] : 0.0;
+            const double h = hal_on || all ? ghal[k] : 0.0;
+            gradient[k] = (el_on || all ? gel[k] : 0.0) + r + d + h;
+            if (term_gradients) {
+                (*term_gradients)[k] = r;
+                (*term_gradients)[3 * (size_t)na + k] = d;
+                (*term_gradients)[6 * (size_t)na + k] = h;
+            }
+        }
     }
     return Status{};
 }

@@ -127,8 +127,11 @@ class Batch {
     // Total free energy, gradient (3 * natoms, instance order), shell charges
     // (nullable; nshells of every instance, instance order) and per-instance
     // status of the last run, without the full result download.
+    // term_gradients (nullable): the repulsion, D3 and halogen parts of the
+    // gradient, 3 * natoms each, one block after the other (zero where a term
+    // is not part of the gradient).
     Status fetch_energy_gradient(std::vector<int>& status, std::vector<double>& energy, std::vector<double>& gradient,
-                                 std::vector<double>* shell_charges);
+                                 std::vector<double>* shell_charges, std::vector<double>* term_gradients = nullptr);
     // Current positions of every atom (bohr, 3 * natoms()).
     Status positions(double* x) const {
         std::copy(h_x_.begin(), h_x_.end(), x);

@@ -88,6 +88,9 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
     for (int i = 0; i < n; ++i)
         if (out[i].status == kOk) {
             out[i].hessian.assign((size_t)9 * systems[i].nat * systems[i].nat, 0.0);
+            out[i].h_repulsion.assign(out[i].hessian.size(), 0.0);
+            out[i].h_dispersion.assign(out[i].hessian.size(), 0.0);
+            out[i].h_halogen.assign(out[i].hessian.size(), 0.0);
             out[i].dipole_derivatives.assign((size_t)9 * systems[i].nat, 0.0);
             for (int c = 0; c < 3 * systems[i].nat; ++c) {
                 all.push_back({i, c, +1});
@@ -127,8 +130,8 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
         st = b.run();
         if (!st.ok()) return st;
         std::vector<int> status;
-        std::vector<double> energy, grad, dip;
-        st = b.fetch_energy_gradient(status, energy, grad, nullptr);
+        std::vector<double> energy, grad, dip, terms;
+        st = b.fetch_energy_gradient(status, energy, grad, nullptr, &terms);
         if (!st.ok()) return st;
         st = b.fetch_dipoles(dip);
         if (!st.ok()) return st;
@@ -152,6 +155,16 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
             const double* gm = grad.data() + 3 * (size_t)b.atom_offset((int)k + 1);
             for (int row = 0; row < ndof; ++row)
                 r.hessian[(size_t)row * ndof + d.col] = (gp[row] - gm[row]) / (2.0 * opt.step);
+            // the classical terms' parts from their own gradients (the reference's
+            // repulsion, dispersion and halogen sub-results, src/hessian/mod.rs:55-65)
+            std::vector<double>* parts[3] = {&r.h_repulsion, &r.h_dispersion, &r.h_halogen};
+            const size_t nb = (size_t)b.natoms();
+            for (int t = 0; t < 3; ++t) {
+                const double* tp = terms.data() + 3 * (t * nb + (size_t)b.atom_offset((int)k));
+                const double* tm = terms.data() + 3 * (t * nb + (size_t)b.atom_offset((int)k + 1));
+                for (int row = 0; row < ndof; ++row)
+                    (*parts[t])[(size_t)row * ndof + d.col] = (tp[row] - tm[row]) / (2.0 * opt.step);
+            }
             // dipole derivatives by the same differences (the analytic route is
             // src/properties.rs:97-145)
             for (int a = 0; a < 3; ++a)
@@ -164,19 +177,28 @@ Status hessian_batch(const DeviceModel& dm, const std::vector<SystemInput>& syst
         if (r.status != kOk) {
             r.hessian.clear();
             r.dipole_derivatives.clear();
+            r.h_repulsion.clear();
+            r.h_dispersion.clear();
+            r.h_halogen.clear();
             continue;
         }
         const int ndof = 3 * systems[i].nat;
-        double asym = 0.0;
-        for (int a = 0; a < ndof; ++a)
-            for (int c = 0; c < a; ++c) {
-                double& x = r.hessian[(size_t)a * ndof + c];
-                double& y = r.hessian[(size_t)c * ndof + a];
-                asym = std::max(asym, std::fabs(x - y));
-                const double avg = 0.5 * (x + y);
-                x = y = avg;
-            }
-        r.max_asymmetry = asym;
+        auto symmetrise = [&](std::vector<double>& h) {
+            double asym = 0.0;
+            for (int a = 0; a < ndof; ++a)
+                for (int c = 0; c < a; ++c) {
+                    double& x = h[(size_t)a * ndof + c];
+                    double& y = h[(size_t)c * ndof + a];
+                    asym = std::max(asym, std::fabs(x - y));
+                    const double avg = 0.5 * (x + y);
+                    x = y = avg;
+                }
+            return asym;
+        };
+        r.max_asymmetry = symmetrise(r.hessian);
+        symmetrise(r.h_repulsion);
+        symmetrise(r.h_dispersion);
+        symmetrise(r.h_halogen);
     }
     st_local.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = st_local;

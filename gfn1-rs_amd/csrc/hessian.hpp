@@ -33,6 +33,9 @@ struct HessianResult {
     double max_asymmetry = 0.0;    // max |H_ij - H_ji| before symmetrisation
     double dipole[3] = {0.0, 0.0, 0.0};   // Mulliken dipole (e bohr), undisplaced geometry
     std::vector<double> dipole_derivatives;  // d mu_alpha / d R_c at [3 c + alpha], central differences
+    // the repulsion, D3 and halogen parts of the Hessian ((3 nat)^2 each, same
+    // layout and differences, symmetrised); the electronic part is the rest
+    std::vector<double> h_repulsion, h_dispersion, h_halogen;
 };
 
 struct HessianStats {

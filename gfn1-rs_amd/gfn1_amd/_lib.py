@@ -184,6 +184,9 @@ SIGNATURES = [
     ("gfn1_hessian_options_default", None, [ctypes.POINTER(HessianOptions)]),
     ("gfn1_hessian", ctypes.c_int, [ctypes.POINTER(System), _VP, ctypes.POINTER(ElectronicOptions),
                                     ctypes.POINTER(HessianOptions), _DP, _DP, _DP, _DP, ctypes.POINTER(Gfn1Error)]),
+    ("gfn1_hessian_terms", ctypes.c_int, [ctypes.POINTER(System), _VP, ctypes.POINTER(ElectronicOptions),
+                                          ctypes.POINTER(HessianOptions), _DP, _DP, _DP, _DP, _DP,
+                                          ctypes.POINTER(Gfn1Error)]),
     ("gfn1_hessian_batch", ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(System), _VP,
                                           ctypes.POINTER(ElectronicOptions), ctypes.POINTER(HessianOptions), _DP,
                                           _DP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Gfn1Error)]),

@@ -450,22 +450,41 @@ class HessianResult:
     max_asymmetry: float
     dipole_derivatives: np.ndarray = None
     dipole: np.ndarray = None
+    # with terms=True: the repulsion / dispersion / halogen sub-results'
+    # .hessian (hessian/mod.rs:55-65) and the electronic rest
+    repulsion: np.ndarray = None
+    dispersion: np.ndarray = None
+    halogen: np.ndarray = None
+    electronic: np.ndarray = None
 
 
 def hessian(numbers, positions_bohr, params: Optional[Gfn1Parameters] = None,
             electronic: Optional[ElectronicOptions] = None, options: Optional[HessianOptions] = None,
-            charge: float = 0.0, lattice=None) -> HessianResult:
+            charge: float = 0.0, lattice=None, terms: bool = False) -> HessianResult:
     """Cartesian Hessian of one system (replaces analytic_hessian(...).hessian,
     hessian/mod.rs:112-134, at the reference's own 1e-6 Eh/bohr^2 acceptance).
     With a lattice (rows, bohr): the fixed-cell Gamma-point Hessian of a
     periodic system (pbc_gamma_hessian, pbc/hessian.rs), by the same central
-    differences of the periodic gradient."""
+    differences of the periodic gradient.  terms=True also returns the
+    repulsion, dispersion and halogen parts (differences of those terms'
+    gradients) and the electronic rest (no dipole derivatives then)."""
     lib = _lib.load()
     p = _params(params)
     eo, keep_q = (electronic or ElectronicOptions()).to_c()
     ho = (options or HessianOptions()).to_c()
     sysc, keep_sys = _lib.make_system(numbers, positions_bohr, charge, lattice)
     ndof = 3 * len(np.asarray(numbers))
+    if terms:
+        out = np.zeros((ndof, ndof))
+        parts = [np.zeros((ndof, ndof)) for _ in range(3)]
+        asym = ctypes.c_double()
+        err = _lib.Gfn1Error()
+        _lib.check(lib.gfn1_hessian_terms(ctypes.byref(sysc), p._h, ctypes.byref(eo), ctypes.byref(ho),
+                                          out.ctypes.data_as(_lib._DP), ctypes.byref(asym),
+                                          *[h.ctypes.data_as(_lib._DP) for h in parts], ctypes.byref(err)), err)
+        rep, disp, hal = parts
+        return HessianResult(out, asym.value, repulsion=rep, dispersion=disp, halogen=hal,
+                             electronic=out - rep - disp - hal)
     out = np.zeros((ndof, ndof))
     ddip = np.zeros((ndof, 3))
     dip = np.zeros(3)

@@ -467,8 +467,11 @@ void gfn1_optimization_free(gfn1_optimization* r);
  * the analytic gradient at tight SCC tolerances, within 1e-6 Eh/bohr^2
  * (tests/hessian.rs:691-760).  This is that construction, built on the device:
  * the 6 nat displaced geometries of every system run as instances of the batch
- * engine (FP64).  The component breakdown of the analytic result is not
- * available; the include flags select the differentiated gradient terms. */
+ * engine (FP64).  Of the analytic result's component breakdown, the
+ * repulsion, dispersion and halogen parts come from the same differences of
+ * those terms' own gradients (gfn1_hessian_terms); the electronic parts
+ * (fixed_scc, fixed_pulay, fixed_cn_h0, cpxtb_response) are one remainder.  The
+ * include flags select the differentiated gradient terms. */
 typedef struct gfn1_hessian_options {
     double step;                  /* 1e-4 bohr */
     double scc_energy_tolerance;  /* 1e-11, replaces the electronic options' value */
@@ -492,6 +495,17 @@ void gfn1_hessian_options_default(gfn1_hessian_options* options);
 int gfn1_hessian(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
                  const gfn1_hessian_options* options, double* hessian, double* max_asymmetry,
                  double* dipole_derivatives, double* dipole, gfn1_error* err);
+/* One system, with the classical parts of the breakdown: repulsion, dispersion
+ * and halogen (each nullable, (3 nat)^2 doubles in the layout of hessian,
+ * symmetrised) are the repulsion / dispersion / halogen sub-results of
+ * AnalyticHessianResult (src/hessian/mod.rs:55-65: RepulsionHessianResult,
+ * DispersionHessianResult, HalogenHessianResult .hessian), by central
+ * differences of the terms' analytic gradients; zero for a term whose include
+ * flag is off.  hessian minus the three is the electronic part; hessian and
+ * max_asymmetry (nullable) as in gfn1_hessian. */
+int gfn1_hessian_terms(const gfn1_system* system, const gfn1_params* params, const gfn1_electronic_options* electronic,
+                       const gfn1_hessian_options* options, double* hessian, double* max_asymmetry,
+                       double* repulsion, double* dispersion, double* halogen, gfn1_error* err);
 /* Many systems: hessians receives the (3 nat_i)^2 blocks one after another,
  * dipole_derivatives (nullable) the 9 nat_i blocks, status (n entries) each
  * system's gfn1_status (its blocks are left untouched unless GFN1_OK). */

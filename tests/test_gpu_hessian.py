@@ -2,7 +2,8 @@
 finite-difference Hessian (oracle/hessian_oracle.py), same step and SCC
 tolerances.  Bar: max |dH| <= 1e-6 Eh/bohr^2, the bar at which the reference
 accepts its analytic Hessian against this construction
-(tests/hessian.rs:691-760).
+(tests/hessian.rs:691-760); 1e-9 for the repulsion, dispersion and halogen
+parts of the breakdown, differences of gradients that do not depend on the SCC.
 """
 import sys
 
@@ -73,6 +74,34 @@ def test_term_selection_matches_oracle():
         r = hessian(z, x, options=HessianOptions(**terms))
         want, _ = hessian_oracle.fd_hessian(z, x, **terms)
         assert np.abs(r.hessian - want).max() <= H_TOL, terms
+
+
+def test_hessian_breakdown_matches_oracle_terms():
+    # the repulsion / dispersion / halogen parts of a halogen-bonded complex
+    # (bromomethane and ammonia, all three classical terms nonzero) against the
+    # oracle's Hessians of each term alone; the electronic rest against the
+    # electronic term alone; one displaced set of evaluations on the device
+    z = np.array([6, 35, 1, 1, 1, 7, 1, 1, 1])
+    x = np.array([[0, 0, 0], [0, 0, 1.94], [1.03, 0, -0.36], [-0.51, 0.89, -0.36], [-0.51, -0.89, -0.36],
+                  [0, 0, 4.9], [0.94, 0, 5.25], [-0.47, 0.81, 5.25], [-0.47, -0.81, 5.25]]) * A2B
+    note("case hessian breakdown (CH3Br...NH3)")
+    r = hessian(z, x, terms=True)
+    full, _ = hessian_oracle.fd_hessian(z, x)
+    assert np.abs(r.hessian - full).max() <= H_TOL
+    only = dict(include_repulsion=False, include_electronic=False, include_dispersion=False, include_halogen=False)
+    for part, flag in (("repulsion", "include_repulsion"), ("dispersion", "include_dispersion"),
+                       ("halogen", "include_halogen"), ("electronic", "include_electronic")):
+        want, _ = hessian_oracle.fd_hessian(z, x, **dict(only, **{flag: True}))
+        got = getattr(r, part)
+        d = float(np.abs(got - want).max())
+        note("%s: max|dH| %.2e, largest element %.2e" % (part, d, float(np.abs(want).max())))
+        # the classical parts are differences of SCC-independent gradients
+        assert d <= (H_TOL if part == "electronic" else 1e-9), part
+        assert np.abs(want).max() > 1e-5, part  # each term contributes here
+    # a term switched off contributes nothing to the breakdown
+    r2 = hessian(z, x, options=HessianOptions(include_halogen=False), terms=True)
+    assert np.abs(r2.halogen).max() == 0.0
+    assert np.abs(r2.repulsion - r.repulsion).max() <= 1e-9
 
 
 def test_warm_and_cold_starts_agree():
