@@ -238,8 +238,10 @@ Status Batch::create(const DeviceModel& dm, const std::vector<SystemInput>& syst
                 h_pbc_[i] = 1;
                 cells[i].gam0 = gam_size;
                 gam_size += round_up(nsh * nsh, kRangeAlign);
+                // 256-byte aligned: buckets on different streams write their
+                // instances' rows concurrently
                 cells[i].ket0 = ket_size;
-                ket_size += 4 * (int64_t)s.nat * s.nat;
+                ket_size += (4 * (int64_t)s.nat * s.nat + kRangeAlign - 1) / kRangeAlign * kRangeAlign;
                 h_pinst_.push_back(i);
                 max_pnat_ = std::max(max_pnat_, s.nat);
                 for (int a = 0; a < s.nat; ++a) h_patoms_.push_back(h_atom0_[i] + a);
