@@ -18,6 +18,7 @@ import pytest
 import hessian_oracle
 import optimizer_oracle
 import oracle
+import periodic_cells
 from gfn1_amd import (AnalyticGradientOptions, BatchEngine, ElectronicOptions, GeometryOptimizationOptions,
                       NativeError, analytic_gradient, geometry, hessian, optimize_geometry, run_electronic,
                       system_from_xyz)
@@ -98,6 +99,23 @@ def test_periodic_matches_oracle(name):
     nocc = int(round(ref["nelec"] / 2))
     assert ref["orbital_energies"][nocc - 1] < el.fermi_level < ref["orbital_energies"][nocc]
     assert abs(sum(el.shell_charges)) < 1e-10
+
+
+@pytest.mark.parametrize("name", sorted(periodic_cells.CELLS))
+def test_reference_cells_match_oracle(name):
+    # the periodic cells of the reference's own tests: diamond, rocksalt, LiH and
+    # H2 in large cells, metallic Ni2 and an H2 chain at 3000 K, skewed two-atom
+    # diamond and BN cells (many images per atom pair)
+    note("case periodic reference cell %s" % name)
+    z, x, q, cell = system_from_xyz(periodic_cells.CELLS[name], with_lattice=True)
+    extra = periodic_cells.SMEARED.get(name, {})
+    ref = oracle.run(z, x, cell=cell, **dict(TIGHT, **extra))
+    assert ref["status"] == 0
+    opts = AnalyticGradientOptions(electronic=ElectronicOptions(**dict(TIGHT, **extra)))
+    res = analytic_gradient(z, x, options=opts, lattice=cell)
+    check_against_oracle(name, z, x, cell, res, ref)
+    if name == "diamond":
+        assert np.abs(res.gradient).max() < 1e-4  # tests/pbc_integration.rs:62-80
 
 
 def test_periodic_32_water_box_matches_oracle():

@@ -17,7 +17,11 @@ the mathematics fixes, chosen so that a plausible slip fails one of them:
   of atom B), a separate code path pinned by the molecular goldens;
 - the periodic gradient: central finite differences of the periodic energy,
   and the translational sum rule;
-- the SCC driver: the level shift is not applied (pbc/scf.rs:358-529).
+- the SCC driver: the level shift is not applied (pbc/scf.rs:358-529);
+- the reference's own periodic test cells (tests/periodic_cells.py): every one
+  converges with a neutral cell, and the ideal diamond and rocksalt cells are
+  equilibria by symmetry (the reference asserts a largest gradient component
+  below 1e-4 for diamond, tests/pbc_integration.rs:62-80).
 """
 import math
 
@@ -25,6 +29,8 @@ import numpy as np
 import pytest
 
 import oracle
+import periodic_cells
+from gfn1_amd import system_from_xyz
 
 A2B = 1.8897261246257702
 MADELUNG_NACL = 1.747564594633182  # rocksalt, per ion pair, nearest-neighbour distance units
@@ -235,3 +241,16 @@ def test_singular_cell_is_rejected():
     z, x, _ = water_box()
     r = oracle.run(z, x, cell=np.array([[5.0, 0, 0], [0, 5.0, 0], [10.0, 0, 0]]))
     assert r["status"] == 7
+
+
+@pytest.mark.parametrize("name", sorted(periodic_cells.CELLS))
+def test_reference_cells_converge_neutral(name):
+    z, x, q, cell = system_from_xyz(periodic_cells.CELLS[name], with_lattice=True)
+    r = oracle.run(z, x, cell=cell, **periodic_cells.SMEARED.get(name, {}))
+    assert r["status"] == 0 and r["converged"]
+    assert abs(r["shell_charges"].sum()) < 1e-6
+    assert np.isfinite(r["energy"])
+    assert np.abs(r["gradient"].sum(axis=0)).max() < 1e-10
+    if name in ("diamond", "nacl"):
+        # equilibria by the cells' symmetry
+        assert np.abs(r["gradient"]).max() < 1e-4
