@@ -8,6 +8,8 @@ import pytest
 from scipy import integrate
 
 import oracle
+import reference_molecules
+from gfn1_amd import system_from_xyz
 
 AA = 1.8897261246257702
 TIGHT = dict(energy_tolerance=1e-11, charge_tolerance=1e-9)
@@ -71,6 +73,26 @@ def test_gradient_matches_finite_difference(name):
             em = oracle.run(z, xm, compute_gradient=0, **TIGHT)["energy"]
             fd[a, k] = (ep - em) / (2 * h)
     assert np.abs(fd - r["gradient"]).max() < 2e-7, np.abs(fd - r["gradient"]).max()
+
+
+@pytest.mark.parametrize("name", ["ferrocene", "ni_co4"])
+def test_gradient_of_3d_metal_complexes_matches_finite_difference(name):
+    # the 3d-metal molecules of the reference's own tests and examples: central
+    # differences of the energy for the metal and two ligand atoms
+    z, x, _ = system_from_xyz(reference_molecules.MOLECULES[name])
+    r = oracle.run(z, x, **TIGHT)
+    assert r["status"] == 0, r["message"]
+    h = 1e-4
+    worst = 0.0
+    for a in (0, 1, len(z) - 1):
+        for k in range(3):
+            xp = x.copy(); xp[a, k] += h
+            xm = x.copy(); xm[a, k] -= h
+            ep = oracle.run(z, xp, compute_gradient=0, **TIGHT)["energy"]
+            em = oracle.run(z, xm, compute_gradient=0, **TIGHT)["energy"]
+            worst = max(worst, abs((ep - em) / (2 * h) - r["gradient"][a, k]))
+    assert worst < 2e-7, worst
+    assert np.abs(r["gradient"].sum(axis=0)).max() < 1e-10
 
 
 def test_halogen_term_is_active_and_fd_consistent():
