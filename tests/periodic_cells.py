@@ -28,6 +28,14 @@ CELLS = {
                 "B 0.000000 0.000000 0.000000\nN 0.940000 0.890000 0.920000\n"),
 }
 
+# tests/optimizer.rs:27-80: the diamond cell with atom 0 displaced about 0.15 A,
+# relaxed at fixed cell (max_iterations 20, gradient_tolerance 1e-3)
+DIAMOND_PERTURBED = ("8\nLattice=\"3.567 0 0 0 3.567 0 0 0 3.567\" pbc=\"T T T\"\n"
+                     "C 0.150000 0.080000 0.030000\nC 0.891750 0.891750 0.891750\n"
+                     "C 0.000000 1.783500 1.783500\nC 0.891750 2.675250 2.675250\n"
+                     "C 1.783500 0.000000 1.783500\nC 2.675250 0.891750 2.675250\n"
+                     "C 1.783500 1.783500 0.000000\nC 2.675250 2.675250 0.891750\n")
+
 # the reference's finite-temperature options for the metallic cells (tests/pbc_finite_t.rs:26-35)
 SMEARED = {"ni2": dict(enable_dispersion=0, electronic_temperature=3000.0, max_scc=500),
            "h2chain": dict(enable_dispersion=0, electronic_temperature=3000.0, max_scc=500)}

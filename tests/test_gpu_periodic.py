@@ -215,6 +215,27 @@ def test_periodic_fixed_cell_optimisation_matches_oracle():
         assert abs(a.energy - b[1]) <= 1e-8
 
 
+def test_periodic_diamond_relaxation_matches_oracle():
+    # the reference's periodic optimisation test (tests/optimizer.rs:27-80): a
+    # diamond cell with one displaced atom relaxes at fixed cell, the energy and
+    # the largest gradient drop; here also iterate for iterate against the
+    # oracle's L-BFGS over the oracle's periodic energy
+    note("case periodic diamond relaxation")
+    z, x, _, cell = system_from_xyz(periodic_cells.DIAMOND_PERTURBED, with_lattice=True)
+    r = optimize_geometry(z, x, options=GeometryOptimizationOptions(max_iterations=20, gradient_tolerance=1e-3),
+                          lattice=cell)
+    want = optimizer_oracle.optimize_gfn1(z, x, scc_options=dict(cell=cell), max_iterations=20,
+                                          gradient_tolerance=1e-3)
+    note("diamond relaxation: device it=%d E=%.12f | oracle it=%d E=%.12f" % (r.iterations, r.energy,
+                                                                              want["iterations"], want["energy"]))
+    assert r.status == 0
+    assert r.energy < r.trajectory[0].energy - 1e-6
+    assert r.max_gradient < r.trajectory[0].max_gradient
+    assert r.iterations == want["iterations"] and r.converged == want["converged"]
+    assert abs(r.energy - want["energy"]) <= 1e-8
+    assert np.abs(r.positions - want["positions"]).max() <= 1e-6
+
+
 def test_periodic_hessian_matches_oracle():
     note("case periodic hessian")
     z = np.array([8, 1, 1])
