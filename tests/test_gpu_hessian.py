@@ -14,7 +14,9 @@ import hessian_oracle
 import optimizer_oracle
 from gfn1_amd import (ElectronicOptions, GeometryOptimizationOptions, Gfn1NativeCalculator, HessianOptions,
                       NativeError, geometry, hessian, hessian_batch, optimize_geometry, vibrational_analysis)
-from gfn1_amd import AnalyticGradientOptions
+from gfn1_amd import AnalyticGradientOptions, system_from_xyz
+
+import reference_molecules
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +48,21 @@ def test_hessian_matches_oracle(name):
     assert d <= H_TOL
     assert np.array_equal(r.hessian, r.hessian.T)
     assert r.max_asymmetry < 1e-7
+
+
+@pytest.mark.parametrize("name", sorted(reference_molecules.HESSIAN_MOLECULES))
+def test_reference_hessian_molecules_match_oracle(name):
+    # the other molecules of the reference's Hessian tests (tests/hessian.rs):
+    # HF, chloromethanol and bromoethanol (halogens), Ni(CO)4 (a 3d metal)
+    note("case hessian reference molecule %s" % name)
+    z, x, _ = system_from_xyz(reference_molecules.HESSIAN_MOLECULES[name])
+    r = hessian(z, x)
+    want, asym = hessian_oracle.fd_hessian(z, x)
+    d = float(np.abs(r.hessian - want).max())
+    note("%s: max|dH| %.2e, asymmetry device %.2e oracle %.2e" % (name, d, r.max_asymmetry, asym))
+    assert d <= H_TOL
+    nat = len(z)
+    assert np.abs(r.hessian.reshape(3 * nat, nat, 3).sum(axis=1)).max() < 1e-5
 
 
 def test_batch_matches_single_and_reports_failures():
