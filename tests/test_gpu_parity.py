@@ -44,6 +44,32 @@ def test_reference_molecule_matches_oracle(name):
     check_single_against_oracle(z, x)
 
 
+@pytest.mark.parametrize("name", sorted(reference_molecules.SMEARED_MOLECULES))
+def test_smeared_reference_molecule_matches_oracle(name):
+    # the Fermi-smeared systems of the reference's smearing tests (distorted
+    # Ni(CO)4 at 3000 K, formaldehyde at 10000 K, NiO at 3000 K): every
+    # eigenvector kept, many fractional occupations
+    note("case smeared reference molecule %s" % name)
+    xyz, temp = reference_molecules.SMEARED_MOLECULES[name]
+    z, x, _ = system_from_xyz(xyz)
+    opts = dict(TIGHT, enable_dispersion=False, electronic_temperature=temp, max_scc=500)
+    ref = oracle.run(z, x, **dict(opts, enable_dispersion=0))
+    assert ref["status"] == 0
+    res = analytic_gradient(z, x, options=AnalyticGradientOptions(electronic=ElectronicOptions(**opts)))
+    el = res.electronic_result
+    de = abs(res.total_energy - ref["energy"])
+    dg = float(np.abs(res.gradient - ref["gradient"]).max())
+    note("%s: |dE| %.2e max|dg| %.2e iterations %d / %d" % (name, de, dg, el.iterations, ref["iterations"]))
+    assert el.converged and de <= E_TOL and dg <= G_TOL
+    assert el.iterations == ref["iterations"]
+    assert np.abs(el.occupations - ref["occupations"]).max() <= 1e-8
+    assert abs(el.fermi_level - ref["fermi_level"]) <= 1e-8
+    # the reference's guard that the fixture is meaningfully smeared (smearing.rs:161-190)
+    f = el.occupations
+    frac = f[(f > 1e-8) & (np.abs(f - 2.0) > 1e-8)]
+    assert len(frac) >= 2 and np.minimum(np.abs(frac - 2.0), frac).max() > 1e-4
+
+
 def check_single_against_oracle(z, x):
     ref = oracle.run(z, x, want_matrices=True, **TIGHT)
     assert ref["status"] == 0
