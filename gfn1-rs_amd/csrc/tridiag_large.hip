@@ -97,6 +97,7 @@ __host__ __device__ __forceinline__ int strips_before(int rho, int strip) {
 // Strip length of a column with nt tile rows: long strips while there are
 // tiles enough to give every CU several work-groups, shorter ones below.
 constexpr int kMaxStrip = 8;
+static_assert(kMaxStrip * 64 <= 2 * 256, "k_lt_symv stages a strip's u with two loads per thread");
 int strip_length(int nt) {
     for (int strip = kMaxStrip; strip > 1; strip >>= 1)
         if (strips_before(nt, strip) >= 2048) return strip;
@@ -128,13 +129,62 @@ __global__ void __launch_bounds__(256)
     __shared__ double wn[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int i = c0 - 1;
+    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
+    const int t = blockIdx.x;
+    // A strip's geometry, its first tile's loads and the loads behind u over
+    // its rows and columns go out before the previous column's scalars, whose
+    // sums they do not need.
+    int RB = 0, CB0 = 0, CBe = 0, R0 = 0, r = 0;
+    bool rok = false;
+    double x[16];
+    double un[3] = {0.0, 0.0, 0.0}, uv[3] = {0.0, 0.0, 0.0};  // u sources: columns tid, tid + 256; row r
+    bool uok[3] = {false, false, false};
+    auto load = [&](double (&xx)[16], int cb) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = cb * kTile + 16 * w + q;
+            const bool ok = rok && c < n && c >= c0 && r >= c;
+            xx[q] = ok ? A[r + (size_t)c * ld] : 0.0;
+        }
+    };
+    if (t < nstrips) {
+        int lo = 0, hi = nt - 1;  // the tile row of strip t: largest rho with strips_before(rho, strip) <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (strips_before(mid, strip) <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const int b0 = c0 / kTile, rho = lo;
+        RB = b0 + rho;
+        CB0 = b0 + (t - strips_before(rho, strip)) * strip;
+        CBe = min(CB0 + strip - 1, RB);
+        R0 = RB * kTile;
+        r = R0 + lane;
+        rok = r < n && r >= c0;
+        load(x, CB0);
+        // the strip spans at most kMaxStrip * kTile = 2 * 256 columns
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const int k = tid + 256 * kk;
+            const int c = kk < 2 ? CB0 * kTile + k : r;
+            uok[kk] = (kk == 2 || k < (CBe - CB0 + 1) * kTile) && c > c0 && c < n;
+            if (uok[kk]) {
+                if (fresh) {
+                    un[kk] = A[c + (size_t)i * ld];
+                } else {
+                    un[kk] = next[c];
+                    uv[kk] = vp[c];
+                }
+            }
+        }
+    }
     PrevColumn pc;
     if (!fresh) pc = prev_column(i, j, V, W, ldv, pv, npv, tau);
-    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
     // column i after the panel's corrections (rows >= i)
     auto col = [&](int r) -> double { return fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * pc.s; };
     auto u_of = [&](int r) -> double { return (r > c0 && r < n) ? col(r) : 0.0; };
-    const int t = blockIdx.x;
+    // u from the loaded sources, as u_of forms it
+    auto u_pre = [&](int kk) -> double { return uok[kk] ? (fresh ? un[kk] : un[kk] - uv[kk] * pc.s) : 0.0; };
     if (t >= nstrips) {
         const int ch = t - nstrips;
         if (ch == 0 && tid == 0) {
@@ -176,31 +226,15 @@ __global__ void __launch_bounds__(256)
         if (tid == 0) dots[(size_t)ch * kDot + 2 * kNB] = ((wn[0] + wn[1]) + wn[2]) + wn[3];
         return;
     }
-    int lo = 0, hi = nt - 1;  // the tile row of strip t: largest rho with strips_before(rho, strip) <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (strips_before(mid, strip) <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const int b0 = c0 / kTile, rho = lo;
-    const int RB = b0 + rho, CB0 = b0 + (t - strips_before(rho, strip)) * strip;
-    const int CBe = min(CB0 + strip - 1, RB);
-    const int R0 = RB * kTile, r = R0 + lane;
-    const bool rok = r < n && r >= c0;
-    const double ur = u_of(r);
+    const double ur = u_pre(2);
     // u over the strip's columns, once per work-group
-    for (int k = tid; k < (CBe - CB0 + 1) * kTile; k += 256) uc[k] = u_of(CB0 * kTile + k);
-    __syncthreads();
-    auto load = [&](double (&x)[16], int cb) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = cb * kTile + 16 * w + q;
-            const bool ok = rok && c < n && c >= c0 && r >= c;
-            x[q] = ok ? A[r + (size_t)c * ld] : 0.0;
-        }
-    };
-    double x[16], xn[16];
-    load(x, CB0);
+    for (int kk = 0; kk < 2; ++kk) {
+        const int k = tid + 256 * kk;
+        if (k < (CBe - CB0 + 1) * kTile) uc[k] = u_pre(kk);
+    }
+    __syncthreads();
+    double xn[16];
     double rs = 0.0;
     for (int cb = CB0; cb <= CBe; ++cb) {
         if (cb < CBe) load(xn, cb + 1);
@@ -245,16 +279,53 @@ __global__ void __launch_bounds__(256)
     __shared__ double red[4];
     const int tid = threadIdx.x;
     const int i = c0 - 1;
+    const int q = tid >> 6;
+    const int r = c0 + blockIdx.x * kRedRows + (tid & 63);
+    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
+    // Loads that need nothing computed here are issued first, so that their
+    // round trips overlap the chunk sums: row r's u source and column i+1 of
+    // the trailing block (quarter 0), and row c0 of the panel's V and W.
+    double u_a = 0.0, u_n = 0.0, u_v = 0.0, u_w = 0.0, acol = 0.0;
+    if (q == 0 && r < n) {
+        acol = A[r + (size_t)c0 * ld];
+        if (fresh) {
+            u_a = A[r + (size_t)i * ld];
+        } else {
+            u_n = next[r];
+            u_v = vp[r];
+            u_w = W[(size_t)(j - 1) * ldv + r];
+        }
+    }
+    double w_c0 = 0.0, v_c0 = 0.0, vp_c0 = 0.0;
+    if (tid < j) {
+        w_c0 = W[(size_t)tid * ldv + c0];
+        v_c0 = V[(size_t)tid * ldv + c0];
+        if (!fresh && tid == j - 1) vp_c0 = vp[c0];
+    }
     const double a2 = scal[kScA2], s = scal[kScS];
     {
-        // chunk sums of V^T u, W^T u, |u|^2: four interleaved partial sums per value
-        const int idx = tid & 63, q = tid >> 6;
+        // chunk sums of V^T u, W^T u, |u|^2: four interleaved partial sums per
+        // value, added in chunk order; loads eight chunks at a time (indices
+        // clamped into the ndot >= 1 chunks, the extra values not added)
+        const int idx = tid & 63;
         double acc = 0.0, accn = 0.0;
-        if ((idx & (kNB - 1)) < j)
-            for (int ch = q; ch < ndot; ch += 4) acc += dots[(size_t)ch * kDot + idx];
-        if (idx == 0)
-            for (int ch = q; ch < ndot; ch += 4) accn += dots[(size_t)ch * kDot + 2 * kNB];
-        s4[q][idx] = acc;
+        const bool want = (idx & (kNB - 1)) < j;
+        for (int ch0 = q; ch0 < ndot; ch0 += 32) {
+            double la[8], ln[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const size_t row = (size_t)min(ch0 + 4 * u, ndot - 1) * kDot;
+                la[u] = want ? dots[row + idx] : 0.0;
+                ln[u] = idx == 0 ? dots[row + 2 * kNB] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (ch0 + 4 * u < ndot) {
+                    acc += la[u];
+                    accn += ln[u];
+                }
+        }
+        s4[q][idx] = want ? acc : 0.0;
         if (idx == 0) s4[q][2 * kNB] = accn;
     }
     __syncthreads();
@@ -268,18 +339,15 @@ __global__ void __launch_bounds__(256)
         tk = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
     }
-    const double* vp = fresh ? nullptr : V + (size_t)(j - 1) * ldv;
-    // W(r, jj) with the previous column's a2 v added
-    auto wf = [&](int r, int jj) -> double {
-        double x = W[(size_t)jj * ldv + r];
-        if (!fresh && jj == j - 1) x += a2 * vp[r];
-        return x;
+    // W(., jj) with the previous column's a2 v added, from its loaded values
+    auto wf = [&](double w_raw, double vp_raw, int jj) -> double {
+        return (!fresh && jj == j - 1) ? w_raw + a2 * vp_raw : w_raw;
     };
     if (tid < j) {
         const double vu = ((s4[0][tid] + s4[1][tid]) + s4[2][tid]) + s4[3][tid];
         const double wu = ((s4[0][kNB + tid] + s4[1][kNB + tid]) + s4[2][kNB + tid]) + s4[3][kNB + tid];
-        wi[tid] = wf(c0, tid);
-        vi[tid] = V[(size_t)tid * ldv + c0];
+        wi[tid] = wf(w_c0, vp_c0, tid);
+        vi[tid] = v_c0;
         dv[tid] = scale * vu + vi[tid];  // V^T v
         dw[tid] = scale * wu + wi[tid];  // W^T v
     }
@@ -291,23 +359,41 @@ __global__ void __launch_bounds__(256)
         A[i + (size_t)i * ld] = dd;
     }
     // four threads per row: quarter q adds the slots b = b0 + q, b0 + q + 4, ...
-    // and the panel columns jj = q, q + 4, ...; the quarters are combined in order
-    const int q = tid >> 6;
-    const int r = c0 + blockIdx.x * kRedRows + (tid & 63);
+    // (alternately into two sums, eight loads in flight; slot indices clamped
+    // below nblk, the extra values not added) and the panel columns
+    // jj = q, q + 4, ... (all kNB / 4 loaded, those with jj >= j not added);
+    // the quarters are combined in order
     if (r < n) {
         const int b0 = c0 / kTile;
         double y0 = 0.0, y1 = 0.0;
-        int b = b0 + q;
-        for (; b + 4 < nblk; b += 8) {
-            y0 += part[(size_t)b * ldp + r];
-            y1 += part[(size_t)(b + 4) * ldp + r];
+        for (int b = b0 + q; b < nblk; b += 32) {
+            double ls[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) ls[u] = part[(size_t)min(b + 4 * u, nblk - 1) * ldp + r];
+#pragma unroll
+            for (int u = 0; u < 8; u += 2) {
+                if (b + 4 * u < nblk) y0 += ls[u];
+                if (b + 4 * u + 4 < nblk) y1 += ls[u + 1];
+            }
         }
-        if (b < nblk) y0 += part[(size_t)b * ldp + r];
         double corr = 0.0, upd = 0.0;
-        for (int jj = q; jj < j; jj += 4) {
-            const double vr = V[(size_t)jj * ldv + r], wr = wf(r, jj);
-            corr += vr * dw[jj] + wr * dv[jj];
-            upd += vr * wi[jj] + wr * vi[jj];
+        if (j > 0) {
+            double lv[kNB / 4], lw[kNB / 4];
+#pragma unroll
+            for (int u = 0; u < kNB / 4; ++u) {
+                lv[u] = V[(size_t)(q + 4 * u) * ldv + r];
+                lw[u] = W[(size_t)(q + 4 * u) * ldv + r];
+            }
+            const double vpr = vp[r];
+#pragma unroll
+            for (int u = 0; u < kNB / 4; ++u) {
+                const int jj = q + 4 * u;
+                if (jj < j) {
+                    const double vr = lv[u], wr = wf(lw[u], vpr, jj);
+                    corr += vr * dw[jj] + wr * dv[jj];
+                    upd += vr * wi[jj] + wr * vi[jj];
+                }
+            }
         }
         yq[q][tid & 63] = y0 + y1;
         cq[q][tid & 63] = corr;
@@ -320,11 +406,10 @@ __global__ void __launch_bounds__(256)
         const double yu = ((yq[0][l] + yq[1][l]) + yq[2][l]) + yq[3][l];
         const double corr = ((cq[0][l] + cq[1][l]) + cq[2][l]) + cq[3][l];
         const double upd = ((uq[0][l] + uq[1][l]) + uq[2][l]) + uq[3][l];
-        const double u = r > c0 ? (fresh ? A[r + (size_t)i * ld] : next[r] - vp[r] * s) : 0.0;
+        const double u = r > c0 ? (fresh ? u_a : u_n - u_v * s) : 0.0;
         const double v = r == c0 ? 1.0 : scale * u;
-        const double acol = A[r + (size_t)c0 * ld];  // column i+1 of the trailing block
-        const double p = tk * (scale * yu + acol - corr);
-        if (!fresh && r != c0) W[(size_t)(j - 1) * ldv + r] = wf(r, j - 1);  // finished
+        const double p = tk * (scale * yu + acol - corr);  // acol: column i+1 of the trailing block
+        if (!fresh && r != c0) W[(size_t)(j - 1) * ldv + r] = wf(u_w, u_v, j - 1);  // finished
         W[(size_t)j * ldv + r] = p;
         V[(size_t)j * ldv + r] = v;
         A[r + (size_t)i * ld] = r == c0 ? beta : v;
